@@ -1,0 +1,964 @@
+// fsim_api.hip -- host side of libfsim_hip: handle, grid, initial state, the
+// step orchestration and the C ABI declared in include/fsim.h.
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <chrono>
+#include <string>
+#include <vector>
+
+#include "fsim.h"
+#include "fsim_internal.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, const char* a = "", int b = 0) {
+  char buf[512];
+  snprintf(buf, sizeof(buf), fmt, a, b);
+  g_err = buf;
+  return code;
+}
+
+#define HIP_OK(expr)                                                              \
+  do {                                                                            \
+    hipError_t e_ = (expr);                                                       \
+    if (e_ != hipSuccess) return fail(FS_E_HIP, "HIP error %s at line %d",        \
+                                      hipGetErrorString(e_), __LINE__);          \
+  } while (0)
+
+// x0**6 correctly rounded (gfortran folds the parameter expression of
+// ptcl_props.F90:18 exactly); x0 has a 24-bit significand so x0**4 is exact in
+// binary128 and one rounding follows.
+double pow6_exact(double x) {
+  __float128 q = (__float128)x;
+  __float128 q2 = q * q, q4 = q2 * q2;
+  return (double)(q4 * q2);
+}
+
+double flog32(float v) { return (double)(float)log((double)v); }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// constants: props.F90:24-44 and ptcl_props.F90:5-25 (p and gam at runtime)
+void fs_phys_init(FsPhys* K, double p, double gam) {
+  memset(K, 0, sizeof(*K));
+  K->pi = 4.0 * atan(1.0);
+  K->eps0 = F32(8.85418782e-12);
+  K->mu0 = 4.0 * K->pi * F32(1e-7);
+  K->c0 = 1.0 / sqrt(K->eps0 * K->mu0);
+  K->e = F32(1.60217646e-19);
+  K->kb = F32(1.3806503e-23);
+  K->me = 9.10938188e-31;
+  K->x0 = F32(1e-4);
+  K->x0_2 = K->x0 * K->x0;
+  K->x0_3 = K->x0_2 * K->x0;
+  K->x0_6 = pow6_exact(K->x0);
+  K->ph0 = K->e / (K->eps0 * K->x0);
+  K->t0 = F32(1e-6);
+  K->wp02 = (K->t0 * K->t0) * (K->e * K->e) / (K->eps0 * K->x0_3 * K->me);
+  double tw = K->t0 * F32(14e9);
+  K->w2 = tw * tw;
+  K->Tg = 300.0;
+  K->p = p;
+  K->ninf = p * 101325.0 / 760.0 / K->kb / K->Tg * K->x0_3;
+  K->n_zero = F32(1e8) * K->x0_3;
+  K->n_init = F32(1e11) * K->x0_3;
+  K->mi = 1.67262178e-27 * 39.948;
+  K->mui = 1.45e3 / p * 1e-4 * K->ph0 * K->t0 / K->x0_2;
+  K->vi = sqrt((8.0 * K->kb * K->Tg) / (K->pi * K->mi)) * K->t0 / K->x0;
+  K->Di = K->mui * (K->kb * K->Tg / K->e) / K->ph0;
+  K->Dm = 2.42e18 * K->x0 / K->ninf / 1e2 * K->t0;
+  K->k_r = 2e-7 / 1e6 / K->x0_3 * K->t0;
+  K->k_mp = 6.2e-10 / 1.0e6 / K->x0_3 * K->t0;
+  K->k_2q = 3e-15 / 1e6 / K->x0_3 * K->t0;
+  K->k_3q = 1.1e-31 / 1.0e12 / K->x0_6 * K->t0;
+  K->H_ir = 15.80 / K->ph0;
+  K->H_ex = 11.56 / K->ph0;
+  K->H_si = 4.14 / K->ph0;
+  K->H_sc = -11.56 / K->ph0;
+  K->gam = gam;
+  K->log5119 = flog32(5.119f);
+  K->log5667 = flog32(5.667f);
+  K->log75 = flog32(7.5f);
+  K->ftol = F32(1e-12);
+  K->Cap = 1e-11 * K->ph0 / K->e;
+  K->kTe = K->Tg * K->kb / K->e / K->ph0;
+  K->nmin_t = K->n_zero / K->ph0 / 1e2;
+  K->c53 = 5.0 / 3.0;
+  K->ve_num = 16.0 * K->e * K->ph0;
+  K->ve_den = 3.0 * K->pi * K->me;
+  K->qvi = 2.5e-1 * K->vi;
+  K->hvi = 5e-1 * K->vi;
+  K->k2q_ninf = K->k_2q * K->ninf;
+  K->k3q_ninf2 = K->k_3q * (K->ninf * K->ninf);
+  K->kmp2 = 2.0 * K->k_mp;
+  K->log_kex_lo = log(9.611e-1);
+  K->log_kir_lo = log(1.37);
+  K->log_s_lo = log(5e-1);
+  K->log_s_hi = log(16.0);
+  K->exp_m100 = exp(-100.0);
+}
+
+void fs_phys_table(const FsPhys* K, double* o) {
+  const double v[40] = {K->pi,   K->eps0, K->mu0,  K->e,    K->kb,    K->me,      K->x0,
+                        K->ph0,  K->t0,   K->wp02, K->w2,   K->Tg,    K->p,       K->ninf,
+                        K->n_zero, K->n_init, K->mi, K->mui, K->vi,   K->Di,      K->Dm,
+                        K->k_r,  K->k_mp, K->k_2q, K->k_3q, K->H_ir,  K->H_ex,    K->H_si,
+                        K->H_sc, K->gam,  K->log5119, K->log5667, K->log75, K->ftol, K->Cap,
+                        K->x0_3, K->c0,   K->kTe,  K->nmin_t, K->x0_6};
+  memcpy(o, v, sizeof(v));
+}
+
+// ---------------------------------------------------------------------------
+struct fs_handle {
+  fs_grid_desc g;
+  fs_params p;
+  FsPhys K;
+  int bx, by, P;
+  size_t PL;
+  std::vector<double> x, y, dx, dlx, dy, dly, r;  // reference (1-based shifted to 0)
+  std::vector<int32_t> type_x, type_y;
+  // device
+  hipStream_t st = nullptr;
+  double* d_lev = nullptr;  // 4 buffers x 4 species x PL
+  double* d_k = nullptr;    // k1,k3,k4 x 4 species x PL
+  double* d_kdbg = nullptr; // 5 stages x 4 species x PL (debug)
+  double* d_ph = nullptr;
+  double* d_geo = nullptr;  // dx,dlx,dy,dly,r
+  int8_t* d_types = nullptr;  // ltype, rtype, gl, gr (by each)
+  double* d_dr = nullptr;     // dra, drc
+  double* d_cur = nullptr;
+  double* d_part = nullptr;
+  DevScalars* d_sc = nullptr;
+  DevScalars* h_sc = nullptr;  // pinned mirror
+  double* d_pois = nullptr;    // rhs0, r, cx, cy, dg, x, p0, p1, q
+  double* d_dbg = nullptr;     // b (N) + A5 (5N)
+  double* d_replay = nullptr;
+  int nreplay = 0;
+  int rot = 0;        // level rotation: A = buf[rot], L1 = buf[rot+1], L3 = buf[rot+2]
+  bool alias = true;  // level 1 currently equals level 2 (after accept/reject/init)
+  int nbx = 0, nby = 0, nblk = 0;
+  Geo G;
+  std::vector<double> log;
+  bool timing = false;
+  hipEvent_t ev[8] = {};
+  double tms[4] = {0, 0, 0, 0};
+  std::vector<double> stage_buf;
+
+  double* lev(int buf, int sp) const { return d_lev + ((size_t)buf * 4 + sp) * PL; }
+  int bufA() const { return rot; }
+  int bufL1() const { return (rot + 1) % 3; }
+  int bufL3() const { return (rot + 2) % 3; }
+  int bufB() const { return 3; }
+  double* kk(int which, int sp) const { return d_k + ((size_t)which * 4 + sp) * PL; }
+  double* pois(int which) const { return d_pois + (size_t)which * PL; }
+};
+
+namespace {
+
+// props.F90:96-124 (x) / :134-163 (y); kind 2 = geometric spacing ratio**min(k, n-k)
+void build_axis(int kind, double ratio, int n, double len, int is_y, std::vector<double>& x) {
+  x.assign(n + 2, 0.0);
+  if (kind == 0) {
+    double fl = (double)(float)(n + 1);
+    for (int i = 0; i < n + 2; i++) x[i] = len / fl * (double)i;
+  } else if (kind == 1) {
+    float xt = is_y ? 0.886f / (float)(n + 1) : (2.0f * 1.175f) / (float)(n + 1);
+    double c = is_y ? F32(-0.886) : F32(-1.175);
+    for (int i = 0; i < n + 2; i++) x[i] = tanh(c + (double)xt * (double)i);
+  } else {
+    x[0] = 0.0;
+    for (int k = 0; k <= n; k++) {
+      int m = k < n - k ? k : n - k;
+      x[k + 1] = x[k] + pow(ratio, (double)m);
+    }
+  }
+  double t = x[0];
+  for (auto& v : x) v = v - t;
+  t = x[n + 1];
+  for (auto& v : x) v = v / t;
+  for (auto& v : x) v = v * len;
+}
+
+int upload(fs_handle* h, double* dst, const double* src, size_t n) {
+  HIP_OK(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyHostToDevice, h->st));
+  HIP_OK(hipStreamSynchronize(h->st));
+  return 0;
+}
+
+int sync_scalars(fs_handle* h) {
+  HIP_OK(hipMemcpyAsync(h->h_sc, h->d_sc, sizeof(DevScalars), hipMemcpyDeviceToHost, h->st));
+  HIP_OK(hipStreamSynchronize(h->st));
+  return 0;
+}
+
+int push_scalars(fs_handle* h) {
+  HIP_OK(hipMemcpyAsync(h->d_sc, h->h_sc, sizeof(DevScalars), hipMemcpyHostToDevice, h->st));
+  HIP_OK(hipStreamSynchronize(h->st));
+  return 0;
+}
+
+int copy_level(fs_handle* h, int dst, int src) {
+  for (int s = 0; s < 4; s++)
+    HIP_OK(hipMemcpyAsync(h->lev(dst, s), h->lev(src, s), h->PL * sizeof(double),
+                          hipMemcpyDeviceToDevice, h->st));
+  return 0;
+}
+
+void time_mark(fs_handle* h, int e) {
+  if (h->timing) (void)hipEventRecord(h->ev[e], h->st);
+}
+
+StageArgs stage_args(fs_handle* h, int stage, int rhs_only) {
+  StageArgs a;
+  memset(&a, 0, sizeof(a));
+  // stage 1 reads A writes B; 2 reads B writes A; 3 A->B; 4 B->A; 5 A->L1
+  int rd = (stage == 2 || stage == 4) ? h->bufB() : h->bufA();
+  int wr = stage == 5 ? h->bufL1() : ((stage == 1 || stage == 3) ? h->bufB() : h->bufA());
+  if (rhs_only) rd = h->bufA();
+  for (int s = 0; s < 4; s++) {
+    a.rd[s] = h->lev(rd, s);
+    a.wr[s] = h->lev(wr, s);
+    a.n3[s] = h->lev(h->bufL3(), s);
+    a.k1[s] = h->kk(0, s);
+    a.k3[s] = h->kk(1, s);
+    a.k4[s] = h->kk(2, s);
+    a.kdbg[s] = h->d_kdbg ? h->d_kdbg + ((size_t)(stage - 1) * 4 + s) * h->PL : nullptr;
+  }
+  a.ph = h->d_ph;
+  a.cur = h->d_cur;
+  a.part = h->d_part;
+  a.sc = h->d_sc;
+  a.G = h->G;
+  a.stage = stage;
+  a.rhs_only = rhs_only;
+  a.nbx = h->nbx;
+  a.nby = h->nby;
+  return a;
+}
+
+CircArgs circ_args(fs_handle* h, int stage) {
+  CircArgs c;
+  memset(&c, 0, sizeof(c));
+  c.ph = h->d_ph;
+  c.cur = h->d_cur;
+  c.dra = h->d_dr;
+  c.drc = h->d_dr + h->by;
+  c.gl = h->d_types + 2 * h->by;
+  c.gr = h->d_types + 3 * h->by;
+  c.sc = h->d_sc;
+  c.vset = h->p.vset;
+  c.res = h->p.res_ohm * h->K.e / (h->K.ph0 * h->K.t0);  // main.F90:286
+  c.t_rise = h->p.t_rise;
+  c.t_width = h->p.t_width;
+  c.t_fall = h->p.t_fall;
+  c.rf = h->p.rf;
+  c.circuit = h->p.circuit;
+  c.stage = stage;
+  c.by = h->by;
+  c.P = h->P;
+  c.bx = h->bx;
+  return c;
+}
+
+PcgArgs pcg_args(fs_handle* h, int parity) {
+  PcgArgs a;
+  memset(&a, 0, sizeof(a));
+  a.rhs0 = h->pois(0);
+  a.r = h->pois(1);
+  a.cx = h->pois(2);
+  a.cy = h->pois(3);
+  a.dg = h->pois(4);
+  a.x = h->pois(5);
+  a.pold = parity ? h->pois(7) : h->pois(6);
+  a.pnew = parity ? h->pois(6) : h->pois(7);
+  a.q = h->pois(8);
+  a.ph = h->d_ph;
+  a.part = h->d_part;
+  a.sc = h->d_sc;
+  a.G = h->G;
+  a.nbx = h->nbx;
+  a.nby = h->nby;
+  return a;
+}
+
+int check_status(fs_handle* h) {
+  int st = h->h_sc->status;
+  if (st < 0) {
+    const char* what = st == FS_E_NEGDENS ? "negative density in get_Te"
+                       : st == FS_E_NAN   ? "NaN in a stage slope"
+                       : st == FS_E_DTMIN ? "minimum timestep reached"
+                       : st == FS_E_NANJ  ? "NaN in dFdPh"
+                       : st == FS_E_REPLAY ? "replay table exhausted"
+                                           : "device error";
+    return fail(st, "fatal: %s (status %d)", what, st);
+  }
+  return 0;
+}
+
+int lapl_solve_impl(fs_handle* h, int* lerr, fs_solve_stats* stats) {
+  auto t0 = std::chrono::steady_clock::now();
+  LaplArgs a;
+  memset(&a, 0, sizeof(a));
+  int l1 = h->alias ? h->bufA() : h->bufL1();
+  a.n[0] = h->lev(l1, 0);
+  a.n[1] = h->lev(l1, 1);
+  a.n[2] = h->lev(l1, 2);
+  a.ph = h->d_ph;
+  a.rhs0 = h->pois(0);
+  a.r = h->pois(1);
+  a.cx = h->pois(2);
+  a.cy = h->pois(3);
+  a.dg = h->pois(4);
+  a.x = h->pois(5);
+  a.p0 = h->pois(6);
+  a.p1 = h->pois(7);
+  a.part = h->d_part;
+  a.sc = h->d_sc;
+  a.G = h->G;
+  a.nbx = h->nbx;
+  a.nby = h->nby;
+  if (h->d_dbg) {
+    a.bdbg = h->d_dbg;
+    a.adbg = h->d_dbg + (size_t)h->bx * h->by;
+  }
+  time_mark(h, 4);
+  launch_lapl_assemble(a, h->K, h->st);
+  HIP_OK(hipGetLastError());
+  time_mark(h, 5);
+  const int chunk = h->p.lapl_chunk > 0 ? h->p.lapl_chunk : 32;
+  int it = 0;
+  for (;;) {
+    for (int k = 0; k < chunk; k++, it++) {
+      PcgArgs pa = pcg_args(h, it & 1);
+      launch_pcg_a(pa, h->st);
+      launch_pcg_b(pa, h->st);
+    }
+    HIP_OK(hipGetLastError());
+    if (sync_scalars(h)) return FS_E_HIP;
+    if (h->h_sc->status < 0) return check_status(h);
+    if (h->h_sc->done) break;
+  }
+  PcgArgs pa = pcg_args(h, it & 1);
+  launch_pcg_finish(pa, h->st);
+  time_mark(h, 6);
+  if (sync_scalars(h)) return FS_E_HIP;
+  if (int e = check_status(h)) return e;
+  int ok = h->h_sc->done == 1;
+  *lerr = ok ? 0 : 1;
+  if (h->timing) {
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, h->ev[4], h->ev[5]);
+    h->tms[2] += ms;
+    (void)hipEventElapsedTime(&ms, h->ev[5], h->ev[6]);
+    h->tms[3] += ms;
+  }
+  if (stats) {
+    stats->iters = h->h_sc->iters;
+    stats->lerr = *lerr;
+    stats->resid = h->h_sc->tres;
+    stats->bnorm = sqrt(h->h_sc->bnorm2);
+    stats->seconds =
+        std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  }
+  // log the solve on the last ptcl_step attempt (as the oracle does)
+  size_t n = h->log.size() / FS_REC_LEN;
+  if (n > 0 && h->log[(n - 1) * FS_REC_LEN + 6] < 0) {
+    double* rec = &h->log[(n - 1) * FS_REC_LEN];
+    rec[6] = *lerr;
+    rec[7] = h->h_sc->iters;
+    rec[8] = h->h_sc->tres;
+  }
+  return 0;
+}
+
+int ptcl_step_impl(fs_handle* h, fs_step_info* info) {
+  launch_begin_step(h->d_sc, h->st);
+  int attempts = 0;
+  for (;;) {
+    time_mark(h, 0);
+    for (int stage = 1; stage <= 5; stage++) {
+      StageArgs a = stage_args(h, stage, 0);
+      launch_stage(a, h->K, h->st);
+      CircArgs c = circ_args(h, stage);
+      launch_circ(c, h->K, h->st);
+    }
+    time_mark(h, 1);
+    CtrlArgs ca;
+    ca.part = h->d_part;
+    ca.nblocks = h->nblk;
+    ca.sc = h->d_sc;
+    ca.replay = h->d_replay;
+    launch_ctrl(ca, h->st);
+    HIP_OK(hipGetLastError());
+    time_mark(h, 2);
+    if (sync_scalars(h)) return FS_E_HIP;
+    if (h->timing) {
+      float ms = 0;
+      (void)hipEventElapsedTime(&ms, h->ev[0], h->ev[1]);
+      h->tms[0] += ms;
+      (void)hipEventElapsedTime(&ms, h->ev[1], h->ev[2]);
+      h->tms[1] += ms;
+    }
+    attempts++;
+    DevScalars& s = *h->h_sc;
+    double rec[FS_REC_LEN] = {s.ts,      s.t,     s.dt_stage, s.err_cur, s.dt,
+                              (double)s.accept, -1.0, 0.0, 0.0, s.t_m};
+    h->log.insert(h->log.end(), rec, rec + FS_REC_LEN);
+    if (int e = check_status(h)) return e;
+    h->alias = false;
+    if (s.accept) break;
+    // internal restart, ptcl_lib.F90:76-93: n1 = n2 = n3 (level 1 is rewritten at stage 5)
+    if (copy_level(h, h->bufA(), h->bufL3())) return FS_E_HIP;
+  }
+  if (info) {
+    DevScalars& s = *h->h_sc;
+    info->err_cur = s.err_cur;
+    info->dt_stage = s.dt_stage;
+    info->dt_next = s.dt;
+    info->t_m = s.t_m;
+    info->Vd_pl = s.Vd_pl;
+    info->Ida = s.Ida;
+    info->Idc = s.Idc;
+    info->Vsrc = s.Vsrc;
+    info->attempts = attempts;
+    info->status = s.status;
+  }
+  return 0;
+}
+
+int accept_impl(fs_handle* h) {
+  launch_accept(h->d_sc, h->st);
+  HIP_OK(hipGetLastError());
+  h->rot = (h->rot + 1) % 3;
+  h->alias = true;
+  return 0;
+}
+
+int reject_impl(fs_handle* h) {
+  launch_reject(h->d_sc, h->st);
+  HIP_OK(hipGetLastError());
+  if (copy_level(h, h->bufA(), h->bufL3())) return FS_E_HIP;
+  h->alias = true;
+  return 0;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char* fs_last_error(void) { return g_err.c_str(); }
+
+int fs_constants(double p, double gam, double* out) {
+  FsPhys K;
+  fs_phys_init(&K, p, gam);
+  fs_phys_table(&K, out);
+  return 0;
+}
+
+int fs_create(const fs_grid_desc* g, const fs_params* p, fs_handle** out) {
+  if (!g || !p || !out) return fail(FS_E_ARG, "null argument%s", "");
+  if (g->nx < 3 || g->ny < 2) return fail(FS_E_ARG, "need nx >= 3 and ny >= 2%s", "");
+  if (p->rf < 0 || p->rf > 4) return fail(FS_E_ARG, "unknown waveform rf%s", "");
+  if (p->p_torr <= 0) return fail(FS_E_ARG, "pressure must be positive%s", "");
+  if (p->rf == 4 && (p->t_rise <= 0 || p->t_fall <= 0))
+    return fail(FS_E_ARG, "trapezoid needs t_rise, t_fall > 0%s", "");
+  fs_handle* h = new fs_handle();
+  h->g = *g;
+  h->p = *p;
+  fs_phys_init(&h->K, p->p_torr, p->gam);
+  h->bx = g->nx;
+  h->by = g->ny;
+  h->P = h->bx + 2;
+  h->PL = (size_t)h->P * (h->by + 2);
+  const int bx = h->bx, by = h->by;
+  // grid: props.F90:96-218
+  build_axis(g->grid_x, g->ratio_x, bx, g->l, 0, h->x);
+  build_axis(g->grid_y, g->ratio_y, by, g->w, 1, h->y);
+  h->dx.resize(bx + 1);
+  h->dlx.resize(bx);
+  h->dy.resize(by + 1);
+  h->dly.resize(by);
+  for (int i = 0; i < bx + 1; i++) h->dx[i] = h->x[i + 1] - h->x[i];
+  for (int i = 0; i < bx; i++) h->dlx[i] = 0.5 * (h->dx[i + 1] + h->dx[i]);
+  for (int j = 0; j < by + 1; j++) h->dy[j] = h->y[j + 1] - h->y[j];
+  for (int j = 0; j < by; j++) h->dly[j] = 0.5 * (h->dy[j + 1] + h->dy[j]);
+  h->r = h->y;
+  h->type_x.assign((size_t)bx * by, 0);
+  h->type_y.assign((size_t)bx * by, 0);
+  std::vector<int8_t> types(4 * (size_t)by, 0);
+  std::vector<double> dr(2 * (size_t)by, 0.0);
+  const bool active = p->circuit == 1;
+  for (int j = 0; j < by; j++) {
+    // reference uses y(j) (the node below cell j+1) in the electrode test (props.F90:201)
+    bool el = (h->y[j] <= g->ew) || (h->y[j] >= g->gap);
+    types[j] = el ? -2 : -1;
+    types[by + j] = el ? 2 : 1;
+    for (int i = 0; i < bx; i++) {
+      size_t n = (size_t)j * bx + i;
+      if (j == 0) h->type_y[n] = -1;
+      if (j == by - 1) h->type_y[n] = 1;
+      if (i == 0) h->type_x[n] = types[j];
+      if (i == bx - 1) h->type_x[n] = types[by + j];
+    }
+    // circ_lib.F90:89-151: dr weights; Fortran j = j+2 -> r(j) = r[j+1], dy(j-1) = dy[j]
+    double rr = h->r[j + 1];
+    double w = g->cyl ? h->dy[j] * h->r[j] * 2.0 * h->K.pi : h->dy[j] * g->w;
+    if (!active && rr < g->ew) w = 0.0;
+    dr[j] = el ? w : 0.0;
+    dr[by + j] = el ? w : 0.0;
+    // ghost potential rows, circ_lib.F90:180-198
+    types[2 * by + j] = (rr <= g->ew && el) ? 1 : 0;
+    types[3 * by + j] = (rr <= g->ew && el) ? 1 : 0;
+  }
+  h->nblk = 0;
+  h->nbx = (bx + kTX - 1) / kTX;
+  h->nby = (by + kTY - 1) / kTY;
+  h->nblk = h->nbx * h->nby;
+
+  auto bad = [&](hipError_t e) {
+    if (e == hipSuccess) return false;
+    fail(FS_E_HIP, "HIP allocation failed: %s", hipGetErrorString(e));
+    return true;
+  };
+  size_t PL = h->PL;
+  if (bad(hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking)) ||
+      bad(hipMalloc(&h->d_lev, 16 * PL * sizeof(double))) ||
+      bad(hipMalloc(&h->d_k, 12 * PL * sizeof(double))) ||
+      bad(hipMalloc(&h->d_ph, PL * sizeof(double))) ||
+      bad(hipMalloc(&h->d_geo, (3 * (size_t)bx + 3 * (size_t)by + 6) * sizeof(double))) ||
+      bad(hipMalloc(&h->d_types, 4 * (size_t)by)) ||
+      bad(hipMalloc(&h->d_dr, 2 * (size_t)by * sizeof(double))) ||
+      bad(hipMalloc(&h->d_cur, 2 * (size_t)by * sizeof(double))) ||
+      bad(hipMalloc(&h->d_part, ((size_t)h->nblk + 64) * 8 * sizeof(double))) ||
+      bad(hipMalloc(&h->d_sc, sizeof(DevScalars))) ||
+      bad(hipHostMalloc(&h->h_sc, sizeof(DevScalars), hipHostMallocDefault)) ||
+      bad(hipMalloc(&h->d_pois, 9 * PL * sizeof(double)))) {
+    fs_destroy(h);
+    return FS_E_HIP;
+  }
+  if (p->debug_keep_k) {
+    if (bad(hipMalloc(&h->d_kdbg, 20 * PL * sizeof(double))) ||
+        bad(hipMalloc(&h->d_dbg, 6 * (size_t)bx * by * sizeof(double)))) {
+      fs_destroy(h);
+      return FS_E_HIP;
+    }
+    (void)hipMemsetAsync(h->d_kdbg, 0, 20 * PL * sizeof(double), h->st);
+  }
+  for (int e = 0; e < 8; e++) (void)hipEventCreate(&h->ev[e]);
+  // metrics
+  std::vector<double> geo;
+  geo.insert(geo.end(), h->dx.begin(), h->dx.end());
+  geo.insert(geo.end(), h->dlx.begin(), h->dlx.end());
+  geo.insert(geo.end(), h->dy.begin(), h->dy.end());
+  geo.insert(geo.end(), h->dly.begin(), h->dly.end());
+  geo.insert(geo.end(), h->r.begin(), h->r.end());
+  (void)hipMemcpyAsync(h->d_geo, geo.data(), geo.size() * sizeof(double), hipMemcpyHostToDevice,
+                       h->st);
+  (void)hipMemcpyAsync(h->d_types, types.data(), types.size(), hipMemcpyHostToDevice, h->st);
+  (void)hipMemcpyAsync(h->d_dr, dr.data(), dr.size() * sizeof(double), hipMemcpyHostToDevice,
+                       h->st);
+  h->G.dx = h->d_geo;
+  h->G.dlx = h->G.dx + (bx + 1);
+  h->G.dy = h->G.dlx + bx;
+  h->G.dly = h->G.dy + (by + 1);
+  h->G.r = h->G.dly + by;
+  h->G.ltype = h->d_types;
+  h->G.rtype = h->d_types + by;
+  h->G.bx = bx;
+  h->G.by = by;
+  h->G.P = h->P;
+  h->G.cyl = g->cyl;
+  // zero state; PCG ghosts: r = p = x = 0 and diagonal 1 so that neighbour
+  // recomputation at the boundary yields p = 0
+  (void)hipMemsetAsync(h->d_k, 0, 12 * PL * sizeof(double), h->st);
+  (void)hipMemsetAsync(h->d_ph, 0, PL * sizeof(double), h->st);
+  (void)hipMemsetAsync(h->d_cur, 0, 2 * (size_t)by * sizeof(double), h->st);
+  (void)hipMemsetAsync(h->d_pois, 0, 9 * PL * sizeof(double), h->st);
+  launch_fill(h->pois(4), 1.0, PL, h->st);
+  // circ_init (circ_lib.F90:213-223) and main.F90:65-66
+  DevScalars& s = *h->h_sc;
+  memset(&s, 0, sizeof(s));
+  s.t = 0.0;
+  s.dt = p->dt0;
+  s.err_prev = 1.0;
+  s.t_m = (double)1e9f;
+  s.Vd_pl = s.Vd_mi = s.Vd_or = s.Vsrc = p->vset;
+  s.tol = p->lapl_rtol > 0 ? p->lapl_rtol : 1e-10;
+  s.maxit = p->lapl_maxit > 0 ? p->lapl_maxit : 20000;
+  if (push_scalars(h)) {
+    fs_destroy(h);
+    return FS_E_HIP;
+  }
+  h->stage_buf.resize(PL);
+  if (fs_init_uniform(h) != 0) {
+    fs_destroy(h);
+    return FS_E_HIP;
+  }
+  *out = h;
+  return 0;
+}
+
+int fs_destroy(fs_handle* h) {
+  if (!h) return 0;
+  if (h->st) (void)hipStreamSynchronize(h->st);
+  (void)hipFree(h->d_lev);
+  (void)hipFree(h->d_k);
+  (void)hipFree(h->d_kdbg);
+  (void)hipFree(h->d_ph);
+  (void)hipFree(h->d_geo);
+  (void)hipFree(h->d_types);
+  (void)hipFree(h->d_dr);
+  (void)hipFree(h->d_cur);
+  (void)hipFree(h->d_part);
+  (void)hipFree(h->d_sc);
+  (void)hipHostFree(h->h_sc);
+  (void)hipFree(h->d_pois);
+  (void)hipFree(h->d_dbg);
+  (void)hipFree(h->d_replay);
+  for (int e = 0; e < 8; e++)
+    if (h->ev[e]) (void)hipEventDestroy(h->ev[e]);
+  if (h->st) (void)hipStreamDestroy(h->st);
+  delete h;
+  return 0;
+}
+
+int fs_get_grid(const fs_handle* h, double* x, double* y, double* dx, double* dlx, double* dy,
+                double* dly, double* r, int32_t* tx, int32_t* ty) {
+  if (!h) return fail(FS_E_ARG, "null handle%s");
+  if (x) memcpy(x, h->x.data(), h->x.size() * sizeof(double));
+  if (y) memcpy(y, h->y.data(), h->y.size() * sizeof(double));
+  if (dx) memcpy(dx, h->dx.data(), h->dx.size() * sizeof(double));
+  if (dlx) memcpy(dlx, h->dlx.data(), h->dlx.size() * sizeof(double));
+  if (dy) memcpy(dy, h->dy.data(), h->dy.size() * sizeof(double));
+  if (dly) memcpy(dly, h->dly.data(), h->dly.size() * sizeof(double));
+  if (r) memcpy(r, h->r.data(), h->r.size() * sizeof(double));
+  if (tx) memcpy(tx, h->type_x.data(), h->type_x.size() * sizeof(int32_t));
+  if (ty) memcpy(ty, h->type_y.data(), h->type_y.size() * sizeof(int32_t));
+  return 0;
+}
+
+static double* field_dev(const fs_handle* h, int field, int level, bool for_write) {
+  if (field >= FS_NI && field <= FS_NM) {
+    int buf;
+    if (level == 1) buf = (h->alias && !for_write) ? h->bufA() : h->bufL1();
+    else if (level == 2) buf = h->bufA();
+    else if (level == 3) buf = h->bufL3();
+    else return nullptr;
+    return h->lev(buf, field);
+  }
+  if (field == FS_PH) return h->d_ph;
+  if (field >= FS_KI && field <= FS_KM) {
+    if (!h->d_kdbg || level < 1 || level > 5) return nullptr;
+    return h->d_kdbg + ((size_t)(level - 1) * 4 + (field - FS_KI)) * h->PL;
+  }
+  return nullptr;
+}
+
+int fs_set_field(fs_handle* h, int32_t field, int32_t level, const double* host, int32_t ld) {
+  if (!h || !host) return fail(FS_E_ARG, "null argument%s");
+  if (ld < h->P) return fail(FS_E_ARG, "ld < nx+2%s");
+  double* d = field_dev(h, field, level, true);
+  if (!d) return fail(FS_E_ARG, "bad field/level%s");
+  for (int j = 0; j < h->by + 2; j++)
+    memcpy(&h->stage_buf[(size_t)j * h->P], host + (size_t)j * ld, h->P * sizeof(double));
+  if (field <= FS_NM && level == 1 && h->alias) {
+    // level 1 stops aliasing level 2: give it its own copy first
+    for (int s = 0; s < 4; s++)
+      HIP_OK(hipMemcpyAsync(h->lev(h->bufL1(), s), h->lev(h->bufA(), s), h->PL * sizeof(double),
+                            hipMemcpyDeviceToDevice, h->st));
+    h->alias = false;
+  }
+  return upload(h, d, h->stage_buf.data(), h->PL);
+}
+
+int fs_get_field(const fs_handle* h, int32_t field, int32_t level, double* host, int32_t ld) {
+  if (!h || !host) return fail(FS_E_ARG, "null argument%s");
+  if (ld < h->P) return fail(FS_E_ARG, "ld < nx+2%s");
+  double* d = field_dev(h, field, level, false);
+  if (!d) return fail(FS_E_ARG, "bad field/level%s");
+  fs_handle* hm = const_cast<fs_handle*>(h);
+  HIP_OK(hipMemcpyAsync(hm->stage_buf.data(), d, h->PL * sizeof(double), hipMemcpyDeviceToHost,
+                        h->st));
+  HIP_OK(hipStreamSynchronize(h->st));
+  for (int j = 0; j < h->by + 2; j++)
+    memcpy(host + (size_t)j * ld, &hm->stage_buf[(size_t)j * h->P], h->P * sizeof(double));
+  return 0;
+}
+
+static int set_all_levels(fs_handle* h, int sp, const std::vector<double>& v) {
+  for (int b = 0; b < 4; b++)
+    HIP_OK(hipMemcpyAsync(h->lev(b, sp), v.data(), h->PL * sizeof(double), hipMemcpyHostToDevice,
+                          h->st));
+  HIP_OK(hipStreamSynchronize(h->st));
+  return 0;
+}
+
+int fs_init_uniform(fs_handle* h) {  // ptcl_lib.F90:22-29
+  if (!h) return fail(FS_E_ARG, "null handle%s");
+  const FsPhys& K = h->K;
+  std::vector<double> v(h->PL, K.n_init);
+  if (set_all_levels(h, 0, v) || set_all_levels(h, 1, v) || set_all_levels(h, 3, v))
+    return FS_E_HIP;
+  for (auto& a : v) a = K.n_init * K.Tg * K.kb / K.e / K.ph0;
+  if (set_all_levels(h, 2, v)) return FS_E_HIP;
+  h->alias = true;
+  return 0;
+}
+
+namespace {
+uint64_t splitmix64(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+// uniform in [-1, 1) keyed by (seed, species, i, j), Fortran indices i, j
+double unoise(uint64_t seed, int sp, int i, int j) {
+  uint64_t h = splitmix64(splitmix64(seed) ^ ((uint64_t)sp << 56) ^ ((uint64_t)j << 28) ^
+                          (uint64_t)i);
+  return (double)(h >> 11) * 0x1.0p-53 * 2.0 - 1.0;
+}
+}  // namespace
+
+int fs_init_gaussian(fs_handle* h, uint64_t seed, double peak, double sigma, double noise) {
+  if (!h) return fail(FS_E_ARG, "null handle%s");
+  const FsPhys& K = h->K;
+  double xc = h->g.l * 0.5, yc = h->g.w * 0.5;
+  std::vector<double> f[4];
+  for (auto& a : f) a.resize(h->PL);
+  for (int j = 0; j < h->by + 2; j++)
+    for (int i = 0; i < h->bx + 2; i++) {
+      double ddx = h->x[i] - xc, ddy = h->y[j] - yc;
+      double base = peak * exp(-(ddx * ddx + ddy * ddy) / (2.0 * sigma * sigma));
+      size_t m = (size_t)j * h->P + i;
+      double ni = base * (1.0 + noise * unoise(seed, 0, i + 1, j + 1)) + K.n_zero;
+      double ne = base * (1.0 + noise * unoise(seed, 1, i + 1, j + 1)) + K.n_zero;
+      f[0][m] = ni;
+      f[1][m] = ne;
+      f[2][m] = ne * K.Tg * K.kb / K.e / K.ph0;
+      f[3][m] = base + K.n_zero;
+    }
+  for (int s = 0; s < 4; s++)
+    if (set_all_levels(h, s, f[s])) return FS_E_HIP;
+  h->alias = true;
+  return 0;
+}
+
+int fs_get_scalars(const fs_handle* h, double* o) {
+  if (!h || !o) return fail(FS_E_ARG, "null argument%s");
+  fs_handle* hm = const_cast<fs_handle*>(h);
+  if (sync_scalars(hm)) return FS_E_HIP;
+  const DevScalars& s = *h->h_sc;
+  double v[11] = {s.t,     s.dt,  s.err_prev, s.t_m, s.Vd_pl, s.Vd_mi,
+                  s.Vd_or, s.Ida, s.Idc,      s.Vsrc, s.ts};
+  memcpy(o, v, sizeof(v));
+  return 0;
+}
+
+int fs_set_scalars(fs_handle* h, const double* o) {
+  if (!h || !o) return fail(FS_E_ARG, "null argument%s");
+  if (sync_scalars(h)) return FS_E_HIP;
+  DevScalars& s = *h->h_sc;
+  s.t = o[0];
+  s.dt = o[1];
+  s.err_prev = o[2];
+  s.t_m = o[3];
+  s.Vd_pl = o[4];
+  s.Vd_mi = o[5];
+  s.Vd_or = o[6];
+  s.Ida = o[7];
+  s.Idc = o[8];
+  s.Vsrc = o[9];
+  s.ts = o[10];
+  return push_scalars(h);
+}
+
+int fs_ptcl_step(fs_handle* h, fs_step_info* info) {
+  if (!h) return fail(FS_E_ARG, "null handle%s");
+  return ptcl_step_impl(h, info);
+}
+
+int fs_lapl_solve(fs_handle* h, int32_t* lerr, fs_solve_stats* stats) {
+  if (!h || !lerr) return fail(FS_E_ARG, "null argument%s");
+  int l = 0;
+  int e = lapl_solve_impl(h, &l, stats);
+  *lerr = l;
+  if (e) return e;
+  return l ? FS_NOT_CONVERGED : 0;
+}
+
+int fs_accept(fs_handle* h) {
+  if (!h) return fail(FS_E_ARG, "null handle%s");
+  return accept_impl(h);
+}
+
+int fs_reject(fs_handle* h) {
+  if (!h) return fail(FS_E_ARG, "null handle%s");
+  return reject_impl(h);
+}
+
+int fs_run(fs_handle* h, int32_t n_iters) {
+  if (!h) return fail(FS_E_ARG, "null handle%s");
+  for (int it = 0; it < n_iters; it++) {
+    if (h->h_sc->ts > 1.0) {  // main.F90:75
+      if (int e = ptcl_step_impl(h, nullptr)) return e;
+    } else {
+      launch_begin_step(h->d_sc, h->st);
+    }
+    int lerr = 0;
+    if (int e = lapl_solve_impl(h, &lerr, nullptr)) return e;
+    int e = lerr == 0 ? accept_impl(h) : reject_impl(h);
+    if (e) return e;
+    if (sync_scalars(h)) return FS_E_HIP;
+  }
+  return 0;
+}
+
+int fs_set_replay(fs_handle* h, const double* recs, int32_t n) {
+  if (!h || (n > 0 && !recs)) return fail(FS_E_ARG, "null argument%s");
+  (void)hipFree(h->d_replay);
+  h->d_replay = nullptr;
+  h->nreplay = 0;
+  if (n > 0) {
+    HIP_OK(hipMalloc(&h->d_replay, (size_t)n * FS_REC_LEN * sizeof(double)));
+    HIP_OK(hipMemcpy(h->d_replay, recs, (size_t)n * FS_REC_LEN * sizeof(double),
+                     hipMemcpyHostToDevice));
+    h->nreplay = n;
+  }
+  if (sync_scalars(h)) return FS_E_HIP;
+  h->h_sc->replay_n = n;
+  h->h_sc->replay_idx = 0;
+  return push_scalars(h);
+}
+
+int fs_log_count(const fs_handle* h) { return h ? (int)(h->log.size() / FS_REC_LEN) : 0; }
+
+int fs_get_log(const fs_handle* h, double* out, int32_t max) {
+  if (!h || !out) return fail(FS_E_ARG, "null argument%s");
+  int n = fs_log_count(h);
+  if (n > max) n = max;
+  memcpy(out, h->log.data(), (size_t)n * FS_REC_LEN * sizeof(double));
+  return n;
+}
+
+int fs_clear_log(fs_handle* h) {
+  if (!h) return fail(FS_E_ARG, "null handle%s");
+  h->log.clear();
+  return 0;
+}
+
+int fs_rhs_sweep(fs_handle* h, int32_t stage) {
+  if (!h || stage < 1 || stage > 5) return fail(FS_E_ARG, "bad argument%s");
+  if (!h->d_kdbg) return fail(FS_E_ARG, "fs_rhs_sweep needs debug_keep_k%s");
+  StageArgs a = stage_args(h, stage, 1);
+  launch_stage(a, h->K, h->st);
+  HIP_OK(hipGetLastError());
+  if (sync_scalars(h)) return FS_E_HIP;
+  return check_status(h);
+}
+
+int fs_circ_step(fs_handle* h, int32_t stage) {
+  if (!h || stage < 1 || stage > 5) return fail(FS_E_ARG, "bad argument%s");
+  // currents need the stage-kernel face terms of the current level 2
+  StageArgs a = stage_args(h, stage, 1);
+  for (int s = 0; s < 4; s++) a.kdbg[s] = nullptr;
+  launch_stage(a, h->K, h->st);
+  CircArgs c = circ_args(h, stage);
+  launch_circ(c, h->K, h->st);
+  HIP_OK(hipGetLastError());
+  if (sync_scalars(h)) return FS_E_HIP;
+  return check_status(h);
+}
+
+int fs_lapl_assemble(fs_handle* h, double* b, double* A5) {
+  if (!h || !b || !A5) return fail(FS_E_ARG, "null argument%s");
+  if (!h->d_dbg) return fail(FS_E_ARG, "fs_lapl_assemble needs debug_keep_k%s");
+  LaplArgs a;
+  memset(&a, 0, sizeof(a));
+  int l1 = h->alias ? h->bufA() : h->bufL1();
+  for (int s = 0; s < 3; s++) a.n[s] = h->lev(l1, s);
+  a.ph = h->d_ph;
+  a.rhs0 = h->pois(0);
+  a.r = h->pois(1);
+  a.cx = h->pois(2);
+  a.cy = h->pois(3);
+  a.dg = h->pois(4);
+  a.x = h->pois(5);
+  a.p0 = h->pois(6);
+  a.p1 = h->pois(7);
+  a.bdbg = h->d_dbg;
+  a.adbg = h->d_dbg + (size_t)h->bx * h->by;
+  a.part = h->d_part;
+  a.sc = h->d_sc;
+  a.G = h->G;
+  a.nbx = h->nbx;
+  a.nby = h->nby;
+  launch_lapl_assemble(a, h->K, h->st);
+  HIP_OK(hipGetLastError());
+  size_t N = (size_t)h->bx * h->by;
+  HIP_OK(hipMemcpyAsync(b, h->d_dbg, N * sizeof(double), hipMemcpyDeviceToHost, h->st));
+  HIP_OK(hipMemcpyAsync(A5, h->d_dbg + N, 5 * N * sizeof(double), hipMemcpyDeviceToHost, h->st));
+  HIP_OK(hipStreamSynchronize(h->st));
+  if (sync_scalars(h)) return FS_E_HIP;
+  return check_status(h);
+}
+
+int fs_eval_fits(double p, int32_t n, const double* T, double* out) {
+  if (n <= 0) return 0;
+  FsPhys K;
+  fs_phys_init(&K, p, 0.1);
+  double *dT = nullptr, *dO = nullptr;
+  HIP_OK(hipMalloc(&dT, n * sizeof(double)));
+  HIP_OK(hipMalloc(&dO, 12 * (size_t)n * sizeof(double)));
+  HIP_OK(hipMemcpy(dT, T, n * sizeof(double), hipMemcpyHostToDevice));
+  launch_eval_fits(K, dT, dO, n, nullptr);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipMemcpy(out, dO, 12 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
+  (void)hipFree(dT);
+  (void)hipFree(dO);
+  return 0;
+}
+
+static int eval_flux_common(int32_t n, const double* a, double* out, int dfdph) {
+  if (n <= 0) return 0;
+  FsPhys K;
+  fs_phys_init(&K, 10.0, 0.1);
+  double *dA = nullptr, *dO = nullptr;
+  HIP_OK(hipMalloc(&dA, 7 * (size_t)n * sizeof(double)));
+  HIP_OK(hipMalloc(&dO, (size_t)n * sizeof(double)));
+  HIP_OK(hipMemcpy(dA, a, 7 * (size_t)n * sizeof(double), hipMemcpyHostToDevice));
+  launch_eval_flux(K, dA, dO, n, dfdph, nullptr);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipMemcpy(out, dO, (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
+  (void)hipFree(dA);
+  (void)hipFree(dO);
+  return 0;
+}
+
+int fs_eval_flux(int32_t n, const double* a, double* out) { return eval_flux_common(n, a, out, 0); }
+int fs_eval_dfdph(int32_t n, const double* a, double* out) {
+  return eval_flux_common(n, a, out, 1);
+}
+
+int fs_get_timers(const fs_handle* h, double* out4) {
+  if (!h || !out4) return fail(FS_E_ARG, "null argument%s");
+  memcpy(out4, h->tms, sizeof(h->tms));
+  return 0;
+}
+
+int fs_set_timing(fs_handle* h, int32_t on) {
+  if (!h) return fail(FS_E_ARG, "null handle%s");
+  h->timing = on != 0;
+  for (double& v : h->tms) v = 0;
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,129 @@
+// fsim_internal.h -- device state layout and kernel launchers of libfsim_hip.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "fsim.h"
+#include "fsim_phys.h"
+
+// Scalars that live on the device so that a whole step can run without host
+// round trips.  One instance per handle.
+struct DevScalars {
+  double t, dt, dt_stage, err_prev, err_cur, t_m;
+  double Vd_pl, Vd_mi, Vd_or, Ida, Idc, Vsrc;
+  double kv[6];
+  double err4[4];
+  double ts;
+  // Poisson / PCG
+  double rz, alpha, beta, bnorm2, resid, tol, tres, pad_d;
+  int32_t iters, done, maxit, pad_i;
+  int32_t status, accept, attempts, replay_idx;
+  int32_t replay_n, pad_j[3];
+  uint32_t ticket[8];
+};
+
+// Grid metrics on the device, 0-based over interior cells ci in [0,bx), cj in
+// [0,by).  Face k of the x axis lies between cells k-1 and k (k in [0,bx]).
+struct Geo {
+  const double* dx;   // bx+1  (Fortran dx(k+1))
+  const double* dlx;  // bx
+  const double* dy;   // by+1
+  const double* dly;  // by
+  const double* r;    // by+2  (Fortran r(k+1))
+  const int8_t* ltype;  // by: type_x of the first cell of each row (-2 / -1)
+  const int8_t* rtype;  // by: type_x of the last cell of each row (2 / 1)
+  int32_t bx, by, P, cyl;
+};
+
+// species order everywhere: 0 ni, 1 ne, 2 nt, 3 nm
+struct StageArgs {
+  const double* rd[4];  // level-2 densities read by this stage
+  double* wr[4];        // stage 1-4: next predictor, stage 5: level 1
+  const double* n3[4];  // RK base (level 3)
+  double* k1[4];
+  double* k3[4];
+  double* k4[4];
+  double* kdbg[4];      // all-stage slope copy (debug), may be null
+  const double* ph;
+  double* cur;          // 2*by: Ida face terms, then Idc face terms
+  double* part;         // per-block partials, 8 doubles per block
+  DevScalars* sc;
+  Geo G;
+  int32_t stage, rhs_only, nbx, nby;
+};
+
+struct CircArgs {
+  double* ph;
+  const double* cur;
+  const double* dra;    // by: current weights at the left electrode face
+  const double* drc;    // by: current weights at the right electrode face
+  const int8_t* gl;     // by: ghost potential set on the left
+  const int8_t* gr;     // by: ghost potential set on the right
+  DevScalars* sc;
+  double vset, res, t_rise, t_width, t_fall;
+  int32_t rf, circuit, stage, by, P, bx;
+};
+
+struct CtrlArgs {
+  const double* part;
+  int32_t nblocks;
+  DevScalars* sc;
+  const double* replay;  // FS_REC_LEN per record
+};
+
+struct LaplArgs {
+  const double* n[3];   // level-1 ni, ne, nt
+  const double* ph;
+  double* rhs0;         // V*b (kept for the true residual)
+  double* r;            // CG residual (starts as V*b)
+  double* cx;           // x-face coupling to the right neighbour (symmetrised)
+  double* cy;           // y-face coupling to the upper neighbour
+  double* dg;           // diagonal of -V*A
+  double* x;            // solution, zeroed
+  double* p0;
+  double* p1;
+  double* bdbg;         // debug: reference-form b (nx*ny), may be null
+  double* adbg;         // debug: reference-form A5 (nx*ny*5), may be null
+  double* part;
+  DevScalars* sc;
+  Geo G;
+  int32_t nbx, nby;
+};
+
+struct PcgArgs {
+  const double* cx;
+  const double* cy;
+  const double* dg;
+  double* r;
+  double* x;
+  const double* pold;
+  double* pnew;
+  double* q;
+  const double* rhs0;
+  double* ph;
+  double* part;
+  DevScalars* sc;
+  Geo G;
+  int32_t nbx, nby;
+};
+
+constexpr int kTX = 64;  // tile width (one wave64 per tile row)
+constexpr int kTY = 4;   // tile height
+constexpr int kNT = kTX * kTY;
+
+void launch_stage(const StageArgs& a, const FsPhys& K, hipStream_t s);
+void launch_circ(const CircArgs& a, const FsPhys& K, hipStream_t s);
+void launch_ctrl(const CtrlArgs& a, hipStream_t s);
+void launch_accept(DevScalars* sc, hipStream_t s);
+void launch_reject(DevScalars* sc, hipStream_t s);
+void launch_begin_step(DevScalars* sc, hipStream_t s);
+void launch_lapl_assemble(const LaplArgs& a, const FsPhys& K, hipStream_t s);
+void launch_pcg_a(const PcgArgs& a, hipStream_t s);
+void launch_pcg_b(const PcgArgs& a, hipStream_t s);
+void launch_pcg_finish(const PcgArgs& a, hipStream_t s);
+void launch_fill(double* p, double v, size_t n, hipStream_t s);
+void launch_eval_fits(const FsPhys& K, const double* T, double* out, int n, hipStream_t s);
+void launch_eval_flux(const FsPhys& K, const double* a, double* out, int n, int dfdph,
+                      hipStream_t s);

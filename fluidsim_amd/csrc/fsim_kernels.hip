@@ -1,0 +1,1004 @@
+// fsim_kernels.hip -- gfx950 kernels of the plasma-transport hot path.
+//
+//   k_stage      one RK-Merson stage: ptclEqn (ptcl_lib.F90:99-550) fused with
+//                rk_step (ptcl_props.F90:71-138) and the embedded error sum.
+//                64x4 tiles; densities, potential and per-cell transport
+//                properties of the tile plus a one-cell halo are staged in
+//                LDS, each face flux is computed once and shared by the two
+//                cells it separates.
+//   k_circ       circ_step (circ_lib.F90:19-210): electrode currents, RC
+//                circuit stage, ghost potential.
+//   k_ctrl       the dt controller of ptcl_step (ptcl_lib.F90:59-94).
+//   k_lapl       lapl_solve assembly (laplEqn + laplJ, lapl_lib.F90:123-609)
+//                into the volume-symmetrised operator -V*A (3 planes).
+//   k_pcg_a/b    matrix-free Jacobi-PCG iteration (replaces KSPSolve,
+//                lapl_lib.F90:103) with deterministic last-block reductions.
+//
+// Reductions are deterministic: per-block partials in a fixed tree order,
+// then the last block (atomic ticket) sums the partials in a fixed order.
+#include <hip/hip_runtime.h>
+
+#include "fsim_internal.h"
+
+namespace {
+
+constexpr int kHX = kTX + 2, kHY = kTY + 2, kNH = kHX * kHY;
+constexpr int kNFX = (kTX + 1) * kTY, kNFY = kTX * (kTY + 1);
+constexpr int kNF = kNFX > kNFY ? kNFX : kNFY;
+constexpr int kNW = kNT / 64;
+
+// Blocks are dispatched round-robin over the 8 XCDs; give each XCD a
+// contiguous band of tiles so that halo rows are shared inside one L2.
+__device__ __forceinline__ int xcd_tile(int b, int nb) {
+  int nb8 = (nb / 8) * 8;
+  if (b >= nb8) return b;
+  return (b % 8) * (nb8 / 8) + b / 8;
+}
+
+template <int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV], double* s_red) {
+#pragma unroll
+  for (int k = 0; k < NV; k++) {
+    double a = v[k];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) a += __shfl_down(a, off, 64);
+    v[k] = a;
+  }
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < NV; k++) s_red[k * kNW + w] = v[k];
+  __syncthreads();
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+      double a = s_red[k * kNW];
+      for (int q = 1; q < kNW; q++) a += s_red[k * kNW + q];
+      v[k] = a;
+    }
+  __syncthreads();
+}
+
+__device__ __forceinline__ double block_min(double v, double* s_red) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_down(v, off, 64));
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) s_red[w] = v;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    for (int q = 1; q < kNW; q++) v = fmin(v, s_red[q]);
+  __syncthreads();
+  return v;
+}
+
+// Writes this block's NV partials, then lets the last block to arrive sum all
+// blocks' partials in block order.  Returns true in the last block, where
+// `tot` holds the totals (valid in thread 0).
+template <int NV>
+__device__ bool last_block_sum(double (&v)[NV], double* part, uint32_t* ticket, double (&tot)[NV],
+                               double* s_red) {
+  __shared__ int s_last;
+  block_sum<NV>(v, s_red);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; k++)
+      __hip_atomic_store(&part[blockIdx.x * NV + k], v[k], __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+    __threadfence();
+    uint32_t t = atomicAdd(ticket, 1u);
+    s_last = (t == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!s_last) return false;
+  __threadfence();
+  double acc[NV];
+#pragma unroll
+  for (int k = 0; k < NV; k++) acc[k] = 0.0;
+  // fixed order: thread t owns blocks t, t+NT, ... (sequential), then a tree
+  for (int b = threadIdx.x; b < (int)gridDim.x; b += kNT)
+#pragma unroll
+    for (int k = 0; k < NV; k++)
+      acc[k] += __hip_atomic_load(&part[b * NV + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  block_sum<NV>(acc, s_red);
+#pragma unroll
+  for (int k = 0; k < NV; k++) tot[k] = acc[k];
+  if (threadIdx.x == 0) *ticket = 0u;
+  return true;
+}
+
+// ---------------------------------------------------------------------------
+// Electrode boundary fluxes, ptcl_lib.F90:187-213 (left) and :247-272 (right)
+struct Flux4 {
+  double i, e, t, m;
+};
+
+__device__ __forceinline__ Flux4 electrode_left(const FsPhys& K, double E, double phL, double phR,
+                                                double te, double mue, double ni, double ne,
+                                                double nt, double nm) {
+  Flux4 F;
+  double a = phL > phR ? 1.0 : 0.0;
+  double mut = K.c53 * mue;
+  double ve = sqrt(K.ve_num * te / K.ve_den) * K.t0 / K.x0;
+  F.i = (1.0 - a) * K.mui * E * ni - K.qvi * ni;
+  F.e = -(a * mue * E * ne) - 2.5e-1 * ve * ne - K.gam * F.i;
+  F.t = -(a * mut * E * nt) - 1.0 / 3.0 * ve * nt - K.gam * te * F.i;
+  F.m = -(K.hvi * nm);
+  return F;
+}
+
+__device__ __forceinline__ Flux4 electrode_right(const FsPhys& K, double E, double phL,
+                                                 double phR, double te, double mue, double ni,
+                                                 double ne, double nt, double nm) {
+  Flux4 F;
+  double a = phR > phL ? 1.0 : 0.0;
+  double mut = K.c53 * mue;
+  double ve = sqrt(K.ve_num * te / K.ve_den) * K.t0 / K.x0;
+  F.i = (1.0 - a) * K.mui * E * ni + K.qvi * ni;
+  F.e = -(a * mue * E * ne) + 2.5e-1 * ve * ne - K.gam * F.i;
+  F.t = -(a * mut * E * nt) + 1.0 / 3.0 * ve * nt - K.gam * te * F.i;
+  F.m = K.hvi * nm;
+  return F;
+}
+
+// interior face between cells L and R (ptcl_lib.F90:131-161)
+__device__ __forceinline__ Flux4 interior_face(const FsPhys& K, double E, double h, double mueL,
+                                               double mueR, double deL, double deR, double niL,
+                                               double niR, double neL, double neR, double ntL,
+                                               double ntR, double nmL, double nmR) {
+  Flux4 F;
+  double mue = 5e-1 * (mueL + mueR);
+  double de = 5e-1 * (deL + deR);
+  double mut = K.c53 * mue;
+  double dt = K.c53 * de;
+  F.i = sg_flux(K, E, h, 1.0, K.mui, K.Di, niL, niR);
+  F.e = sg_flux(K, E, h, -1.0, mue, de, neL, neR);
+  F.t = sg_flux(K, E, h, -1.0, mut, dt, ntL, ntR);
+  F.m = -(K.Dm * (nmR - nmL) / h);
+  return F;
+}
+
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kNT) k_stage(StageArgs A, FsPhys K) {
+  __shared__ double s_ni[kNH], s_ne[kNH], s_nt[kNH], s_nm[kNH], s_ph[kNH];
+  __shared__ double s_te[kNH], s_mue[kNH], s_de[kNH], s_xa[kNH];
+  __shared__ double s_f[4][kNF];
+  __shared__ double s_red[8 * kNW];
+  __shared__ int s_bad;
+
+  const Geo& G = A.G;
+  const int bx = G.bx, by = G.by, P = G.P;
+  const int t = xcd_tile(blockIdx.x, gridDim.x);
+  const int i0 = (t % A.nbx) * kTX, j0 = (t / A.nbx) * kTY;
+  const double dt = A.sc->dt;
+  if (threadIdx.x == 0) s_bad = 0;
+  __syncthreads();
+
+  // ---- phase 1: stage densities + potential of tile and halo; cell properties
+  for (int h = threadIdx.x; h < kNH; h += kNT) {
+    int hx = h % kHX, hy = h / kHX;
+    int ci = i0 - 1 + hx, cj = j0 - 1 + hy;
+    double vi = 0, ve = 0, vt = 0, vm = 0, vp = 0, te = 0, mue = 0, de = 0, xa = 0;
+    if (ci <= bx && cj <= by) {
+      size_t m = (size_t)(cj + 1) * P + (ci + 1);
+      vi = A.rd[0][m];
+      ve = A.rd[1][m];
+      vt = A.rd[2][m];
+      vm = A.rd[3][m];
+      vp = A.ph[m];
+      if (ci >= 0 && ci < bx && cj >= 0 && cj < by) {
+        if (vt >= 0.0 && ve >= 0.0) {
+          te = vt / ve;  // get_Te, ptcl_props.F90:167-180
+        } else {
+          te = 1e-8;
+          s_bad = 1;
+        }
+        xa = fit_x(te * K.ph0);
+        mue = mue_of_x(K, xa);
+        de = de_of_x(K, xa);
+      }
+    }
+    s_ni[h] = vi;
+    s_ne[h] = ve;
+    s_nt[h] = vt;
+    s_nm[h] = vm;
+    s_ph[h] = vp;
+    s_te[h] = te;
+    s_mue[h] = mue;
+    s_de[h] = de;
+    s_xa[h] = xa;
+  }
+  __syncthreads();
+
+  // ---- phase 2: x faces (left face of cells i0..i0+TX, rows j0..j0+TY-1)
+  for (int f = threadIdx.x; f < kNFX; f += kNT) {
+    int fx = f % (kTX + 1), fy = f / (kTX + 1);
+    int fi = i0 + fx, cj = j0 + fy;
+    Flux4 F = {0, 0, 0, 0};
+    if (cj < by && fi <= bx) {
+      int hL = (fy + 1) * kHX + fx, hR = hL + 1;
+      double h = G.dx[fi];
+      double E = -(s_ph[hR] - s_ph[hL]) / h;
+      if (fi == 0) {
+        if (G.ltype[cj] == -2)
+          F = electrode_left(K, E, s_ph[hL], s_ph[hR], s_te[hR], s_mue[hR], s_ni[hR], s_ne[hR],
+                             s_nt[hR], s_nm[hR]);
+      } else if (fi == bx) {
+        if (G.rtype[cj] == 2)
+          F = electrode_right(K, E, s_ph[hL], s_ph[hR], s_te[hL], s_mue[hL], s_ni[hL],
+                              s_ne[hL], s_nt[hL], s_nm[hL]);
+      } else {
+        F = interior_face(K, E, h, s_mue[hL], s_mue[hR], s_de[hL], s_de[hR], s_ni[hL], s_ni[hR],
+                          s_ne[hL], s_ne[hR], s_nt[hL], s_nt[hR], s_nm[hL], s_nm[hR]);
+        if (fi == 1) A.cur[cj] = F.i - F.e;            // Ida face (circ_lib.F90:87-118)
+        if (fi == bx - 1) A.cur[by + cj] = F.i - F.e;  // Idc face (circ_lib.F90:120-151)
+      }
+    }
+    s_f[0][f] = F.i;
+    s_f[1][f] = F.e;
+    s_f[2][f] = F.t;
+    s_f[3][f] = F.m;
+  }
+  __syncthreads();
+
+  // ---- phase 3a: x divergence per cell (kept in registers)
+  const int tx = threadIdx.x % kTX, ty = threadIdx.x / kTX;
+  const int ci = i0 + tx, cj = j0 + ty;
+  const bool valid = (ci < bx) && (cj < by);
+  const int hc = (ty + 1) * kHX + tx + 1;
+  double dfx[4] = {0, 0, 0, 0}, feX = 0, exA = 0, mue2 = 0;
+  int typx = 0;
+  if (valid) {
+    typx = ci == 0 ? G.ltype[cj] : (ci == bx - 1 ? G.rtype[cj] : 0);
+    int fL = ty * (kTX + 1) + tx, fR = fL + 1;
+    double dlx = G.dlx[ci];
+#pragma unroll
+    for (int s = 0; s < 4; s++) dfx[s] = (s_f[s][fR] - s_f[s][fL]) / dlx;
+    double exL, exR;
+    if (typx == -1) {
+      exL = 0.0;
+      exR = -(s_ph[hc + 1] - s_ph[hc]) / G.dx[ci + 1];
+    } else if (typx == 1) {
+      exL = -(s_ph[hc] - s_ph[hc - 1]) / G.dx[ci];
+      exR = 0.0;
+    } else {
+      exL = -(s_ph[hc] - s_ph[hc - 1]) / G.dx[ci];
+      exR = -(s_ph[hc + 1] - s_ph[hc]) / G.dx[ci + 1];
+    }
+    feX = (s_f[1][fR] + s_f[1][fL]) / 2.0;
+    exA = (exL + exR) / 2.0;
+    // mue(2) left by the x branch (used by t_m on the top row)
+    if (typx == 2) mue2 = s_mue[hc];
+    else if (typx == 1) mue2 = 0.0;
+    else mue2 = 5e-1 * (s_mue[hc] + s_mue[hc + 1]);
+  }
+  __syncthreads();
+
+  // ---- phase 2b: y faces (bottom face of rows j0..j0+TY, columns i0..i0+TX-1)
+  for (int f = threadIdx.x; f < kNFY; f += kNT) {
+    int fx = f % kTX, fy = f / kTX;
+    int fci = i0 + fx, fj = j0 + fy;
+    Flux4 F = {0, 0, 0, 0};
+    if (fci < bx && fj > 0 && fj < by) {  // faces on y = 0 and y = w carry no flux
+      int hL = fy * kHX + fx + 1, hR = hL + kHX;
+      double h = G.dy[fj];
+      double E = -(s_ph[hR] - s_ph[hL]) / h;
+      F = interior_face(K, E, h, s_mue[hL], s_mue[hR], s_de[hL], s_de[hR], s_ni[hL], s_ni[hR],
+                        s_ne[hL], s_ne[hR], s_nt[hL], s_nt[hR], s_nm[hL], s_nm[hR]);
+    }
+    s_f[0][f] = F.i;
+    s_f[1][f] = F.e;
+    s_f[2][f] = F.t;
+    s_f[3][f] = F.m;
+  }
+  __syncthreads();
+
+  // ---- phase 3b: y divergence, sources, stage slope, RK-Merson update
+  double acc[4] = {0, 0, 0, 0};
+  double tmin = 1e300;
+  if (valid) {
+    int fB = ty * kTX + tx, fT = fB + kTX;
+    double dly = G.dly[cj];
+    double dfy[4];
+    if (G.cyl) {
+      double rj = G.r[cj + 1], rp = G.r[cj + 2], rm = G.r[cj];
+#pragma unroll
+      for (int s = 0; s < 4; s++)
+        dfy[s] = (s_f[s][fT] * (rp + rj) / 2.0 - s_f[s][fB] * (rj + rm) / 2.0) / dly / rj;
+    } else {
+#pragma unroll
+      for (int s = 0; s < 4; s++) dfy[s] = (s_f[s][fT] - s_f[s][fB]) / dly;
+    }
+    int typy = cj == 0 ? -1 : (cj == by - 1 ? 1 : 0);
+    double eyB, eyT;
+    if (typy == -1) {
+      eyB = 0.0;
+      eyT = -(s_ph[hc + kHX] - s_ph[hc]) / G.dy[cj + 1];
+    } else if (typy == 1) {
+      eyB = -(s_ph[hc] - s_ph[hc - kHX]) / G.dy[cj];
+      eyT = 0.0;
+    } else {
+      eyB = -(s_ph[hc] - s_ph[hc - kHX]) / G.dy[cj];
+      eyT = -(s_ph[hc + kHX] - s_ph[hc]) / G.dy[cj + 1];
+    }
+    double feY = (s_f[1][fT] + s_f[1][fB]) / 2.0;
+    double eyA = (eyB + eyT) / 2.0;
+    if (typy <= 0) mue2 = 5e-1 * (s_mue[hc] + s_mue[hc + kHX]);
+
+    // rates and sources, ptcl_lib.F90:483-514 (Em = 0: its +0 term is dropped)
+    double te = s_te[hc];
+    Rates R = cell_rates(K, te * K.ph0, s_xa[hc]);
+    double ne = s_ne[hc], ni = s_ni[hc], nm = s_nm[hc], ninf = K.ninf;
+    double nm2 = nm * nm;
+    double term_sie = R.k_ir * ninf * ne - R.beta * ni * ne + R.k_si * nm * ne + K.k_mp * nm2;
+    double term_sm = R.k_ex * ninf * ne - R.k_si * nm * ne - R.k_sc * nm * ne - K.k_r * nm * ne -
+                     K.kmp2 * nm2 - K.k2q_ninf * nm - K.k3q_ninf2 * nm;
+    double st1 = -(feX * exA + feY * eyA);
+    double st2 = -(ne * R.nu * K.me / K.mi * (te / 1.5 - K.kTe));
+    double st3 = -(K.H_ir * R.k_ir * ninf * ne) - K.H_si * R.k_si * nm * ne -
+                 K.H_ex * R.k_ex * ninf * ne;
+    double kv[4];
+    kv[0] = -dfx[0] - dfy[0] + term_sie;
+    kv[1] = -dfx[1] - dfy[1] + term_sie;
+    kv[2] = -dfx[2] - dfy[2] + st1 + st2 + st3;
+    kv[3] = -dfx[3] - dfy[3] + term_sm;
+    if (isnan(kv[0]) || isnan(kv[1]) || isnan(kv[2]) || isnan(kv[3])) s_bad = 2;
+
+    size_t m = (size_t)(cj + 1) * P + (ci + 1);
+    if (A.kdbg[0]) {
+#pragma unroll
+      for (int s = 0; s < 4; s++) A.kdbg[s][m] = kv[s];
+    }
+    if (!A.rhs_only) {
+      const int stage = A.stage;
+#pragma unroll
+      for (int s = 0; s < 4; s++) {
+        double n3 = A.n3[s][m], kk = kv[s], nw;
+        double nmin = s == 2 ? K.nmin_t : K.n_zero;
+        if (stage == 1) {
+          nw = n3 + kk * dt / 3.0;
+          A.k1[s][m] = kk;
+        } else if (stage == 2) {
+          nw = n3 + dt * (A.k1[s][m] + kk) / 6.0;
+        } else if (stage == 3) {
+          nw = n3 + dt * (A.k1[s][m] + kk * 3.0) / 8.0;
+          A.k3[s][m] = kk;
+        } else if (stage == 4) {
+          nw = n3 + dt * (A.k1[s][m] - A.k3[s][m] * 3.0 + kk * 4.0) / 2.0;
+          A.k4[s][m] = kk;
+        } else {
+          double k1 = A.k1[s][m], k3 = A.k3[s][m], k4 = A.k4[s][m];
+          nw = n3 + dt * (k1 + k4 * 4.0 + kk) / 6.0;
+          double e = fabs(dt * (k1 * 2.0 / 30.0 - k3 * 3.0 / 10.0 + k4 * 4.0 / 15.0 - kk / 30.0));
+          double q = e / (1e-3 + 1e-3 * fabs(n3));
+          acc[s] = q * q;
+        }
+        A.wr[s][m] = nw > nmin ? nw : nmin;
+      }
+      if (stage == 5) tmin = 1.0 / (mue2 * ne + K.mui * ni);
+    }
+  }
+  if (!A.rhs_only && A.stage == 5) {
+    block_sum<4>(acc, s_red);
+    double tm = block_min(tmin, s_red);
+    if (threadIdx.x == 0) {
+      double* p = A.part + (size_t)blockIdx.x * 8;
+      p[0] = acc[0];
+      p[1] = acc[1];
+      p[2] = acc[2];
+      p[3] = acc[3];
+      p[4] = tm;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0 && s_bad)
+    atomicMin(&A.sc->status, s_bad == 1 ? FS_E_NEGDENS : FS_E_NAN);
+}
+
+// ---------------------------------------------------------------------------
+__device__ double vsource(const CircArgs& C, const FsPhys& K, double t) {
+  const double Vmax = C.vset, pi = K.pi;
+  switch (C.rf) {
+    case 1: return Vmax * sin(pi * t);
+    case 2: {
+      if (t < 2.5e-1) {
+        double s = sin(4.0 * pi * t);
+        return Vmax * (s * (s * s));
+      }
+      return 0.0;
+    }
+    case 3: {
+      if (t < 2.3e-1) {
+        double s = sin(4.0 * pi * t);
+        return Vmax * (s * s);
+      }
+      if (t < 3.35e-1) return Vmax * (sin(3.81 * pi * (t + 5.85e-2)) * 7.19e-1 + 2.81e-1);
+      if (t < 8.2e-1) {
+        double s = sin(3.81 * pi * (t + 1.2e-1) / 3.5);
+        return Vmax * ((s * (s * s)) * (-4.38e-1));
+      }
+      return 0.0;
+    }
+    case 4: {
+      double tr = C.t_rise, tw = C.t_width, tf = C.t_fall;
+      if (t < tr) return Vmax * (t / tr);
+      if (t < tr + tw) return Vmax;
+      double te = tr + tw + tf;
+      if (t < te) return Vmax * ((te - t) / tf);
+      return 0.0;
+    }
+    default: return Vmax;
+  }
+}
+
+__global__ void __launch_bounds__(kNT) k_circ(CircArgs C, FsPhys K) {
+  __shared__ double s_red[8 * kNW];
+  __shared__ double s_vd;
+  DevScalars* sc = C.sc;
+  double v[2] = {0.0, 0.0};
+  for (int j = threadIdx.x; j < C.by; j += kNT) {
+    v[0] += C.dra[j] * C.cur[j];
+    v[1] += C.drc[j] * C.cur[C.by + j];
+  }
+  block_sum<2>(v, s_red);
+  if (threadIdx.x == 0) {
+    const int stage = C.stage;
+    double Vsrc = vsource(C, K, sc->t);
+    double Ida = v[0], Idc = v[1];
+    double* kv = sc->kv;
+    kv[stage] = -(Idc - (Vsrc - sc->Vd_mi) / C.res) / K.Cap;
+    double dt = sc->dt;
+    if (stage == 1) sc->Vd_mi = sc->Vd_or + dt * kv[1] / 3.0;
+    else if (stage == 2) sc->Vd_mi = sc->Vd_or + dt * (kv[1] + kv[2]) / 6.0;
+    else if (stage == 3) sc->Vd_mi = sc->Vd_or + dt * (kv[1] + kv[3] * 3.0) / 8.0;
+    else if (stage == 4) sc->Vd_mi = sc->Vd_or + dt * (kv[1] - kv[3] * 3.0 + kv[4] * 4.0) / 2.0;
+    else sc->Vd_pl = sc->Vd_or + dt * (kv[1] + kv[4] * 4.0 + kv[5]) / 6.0;
+    if (C.rf != 1 && C.circuit == 0) {  // circ_lib.F90:176-177
+      sc->Vd_pl = Vsrc;
+      sc->Vd_mi = Vsrc;
+    }
+    sc->Vsrc = Vsrc;
+    sc->Ida = Ida;
+    sc->Idc = Idc;
+    s_vd = stage == 5 ? sc->Vd_pl : sc->Vd_mi;
+  }
+  __syncthreads();
+  const double vd = s_vd;
+  for (int j = threadIdx.x; j < C.by; j += kNT) {  // circ_lib.F90:180-198
+    size_t row = (size_t)(j + 1) * C.P;
+    if (C.gl[j]) C.ph[row] = vd / 2.0;
+    if (C.gr[j]) C.ph[row + C.bx + 1] = vd / (-2.0);
+  }
+}
+
+// dt controller: ptcl_lib.F90:59-94 (plus replayed decisions)
+__global__ void __launch_bounds__(kNT) k_ctrl(CtrlArgs A) {
+  __shared__ double s_red[8 * kNW];
+  double v[4] = {0, 0, 0, 0};
+  double tm = 1e300;
+  for (int b = threadIdx.x; b < A.nblocks; b += kNT) {
+    const double* p = A.part + (size_t)b * 8;
+    v[0] += p[0];
+    v[1] += p[1];
+    v[2] += p[2];
+    v[3] += p[3];
+    tm = fmin(tm, p[4]);
+  }
+  block_sum<4>(v, s_red);
+  tm = block_min(tm, s_red);
+  if (threadIdx.x != 0) return;
+  DevScalars* sc = A.sc;
+  double err_cur = sqrt(v[0] + v[1] + v[3] + v[2]);  // (ni + ne + nm + nt)**0.5
+  double scfac = 8e-1 * pow(err_cur, -7e-1 / 4.0) * pow(sc->err_prev, 4e-1 / 4.0);
+  scfac = fmin(2.5, fmax(3e-1, scfac));
+  sc->dt_stage = sc->dt;
+  double dtn = scfac * sc->dt;
+  int accept = err_cur <= 1.0;
+  if (sc->replay_n > 0) {
+    if (sc->replay_idx < sc->replay_n) {
+      const double* rec = A.replay + (size_t)sc->replay_idx * 10;
+      dtn = rec[4];
+      accept = rec[5] != 0.0;
+      sc->replay_idx++;
+    } else {
+      sc->status = min(sc->status, FS_E_REPLAY);
+    }
+  }
+  sc->dt = dtn;
+  sc->err_cur = err_cur;
+  for (int k = 0; k < 4; k++) sc->err4[k] = v[k];
+  sc->t_m = fmin(sc->t_m, tm);
+  sc->accept = accept;
+  sc->attempts++;
+  if (dtn <= 1.1e-12) sc->status = min(sc->status, FS_E_DTMIN);
+  if (accept) {
+    sc->err_prev = err_cur;
+  } else {
+    sc->Vd_pl = sc->Vd_or;
+    sc->Vd_mi = sc->Vd_or;
+  }
+}
+
+__global__ void k_begin_step(DevScalars* sc) {
+  sc->t_m = (double)1e9f;  // main.F90:74
+  sc->attempts = 0;
+}
+
+__global__ void k_accept(DevScalars* sc) {  // main.F90:83-101
+  sc->ts = sc->ts + 1.0;
+  sc->t = sc->t + sc->dt;
+  sc->Vd_or = sc->Vd_mi;
+  sc->Vd_mi = sc->Vd_pl;
+}
+
+__global__ void k_reject(DevScalars* sc) {  // main.F90:103-121
+  sc->dt = sc->dt / 2.0;
+  sc->Vd_pl = sc->Vd_or;
+  sc->Vd_mi = sc->Vd_or;
+}
+
+// ---------------------------------------------------------------------------
+// Poisson assembly.  Per x/y face: ion and electron SG fluxes (laplEqn) and
+// P = (mu/h)*dFdPh (laplJ) from one shared exponential per species.
+struct FaceJ {
+  double fi, fe, pi, pe;
+};
+
+__device__ __forceinline__ FaceJ lapl_face(const FsPhys& K, double E, double h, double mueL,
+                                           double mueR, double deL, double deR, double niL,
+                                           double niR, double neL, double neR) {
+  FaceJ F;
+  double mue = 5e-1 * (mueL + mueR);
+  double de = 5e-1 * (deL + deR);
+  double dfi, dfe;
+  sg_flux_dfdph(K, E, h, 1.0, K.mui, K.Di, niL, niR, F.fi, dfi);
+  sg_flux_dfdph(K, E, h, -1.0, mue, de, neL, neR, F.fe, dfe);
+  F.pi = K.mui / h * dfi;  // dfxi_mi (lapl_lib.F90:405); dfxi_pl = -pi (:440)
+  F.pe = mue / h * dfe;    // dfxe_pl (lapl_lib.F90:438); dfxe_mi = -pe (:403)
+  return F;
+}
+
+__global__ void __launch_bounds__(kNT) k_lapl(LaplArgs A, FsPhys K) {
+  __shared__ double s_ni[kNH], s_ne[kNH], s_ph[kNH], s_te[kNH], s_mue[kNH], s_de[kNH];
+  __shared__ double s_f[4][kNF];
+  __shared__ double s_red[8 * kNW];
+  __shared__ int s_bad;
+  const Geo& G = A.G;
+  const int bx = G.bx, by = G.by, P = G.P;
+  const int t = xcd_tile(blockIdx.x, gridDim.x);
+  const int i0 = (t % A.nbx) * kTX, j0 = (t / A.nbx) * kTY;
+  const double dt = A.sc->dt;
+  if (threadIdx.x == 0) s_bad = 0;
+  __syncthreads();
+
+  for (int h = threadIdx.x; h < kNH; h += kNT) {
+    int hx = h % kHX, hy = h / kHX;
+    int ci = i0 - 1 + hx, cj = j0 - 1 + hy;
+    double vi = 0, ve = 0, vp = 0, te = 0, mue = 0, de = 0;
+    if (ci <= bx && cj <= by) {
+      size_t m = (size_t)(cj + 1) * P + (ci + 1);
+      vi = A.n[0][m];
+      ve = A.n[1][m];
+      double vt = A.n[2][m];
+      vp = A.ph[m];
+      if (ci >= 0 && ci < bx && cj >= 0 && cj < by) {
+        if (vt >= 0.0 && ve >= 0.0) {
+          te = vt / ve;
+        } else {
+          te = 1e-8;
+          s_bad = 1;
+        }
+        double xa = fit_x(te * K.ph0);
+        mue = mue_of_x(K, xa);
+        de = de_of_x(K, xa);
+      }
+    }
+    s_ni[h] = vi;
+    s_ne[h] = ve;
+    s_ph[h] = vp;
+    s_te[h] = te;
+    s_mue[h] = mue;
+    s_de[h] = de;
+  }
+  __syncthreads();
+
+  // x faces: fluxes (laplEqn) and Jacobian terms (laplJ)
+  for (int f = threadIdx.x; f < kNFX; f += kNT) {
+    int fx = f % (kTX + 1), fy = f / (kTX + 1);
+    int fi = i0 + fx, cj = j0 + fy;
+    FaceJ F = {0, 0, 0, 0};
+    if (cj < by && fi <= bx) {
+      int hL = (fy + 1) * kHX + fx, hR = hL + 1;
+      double h = G.dx[fi];
+      double E = -(s_ph[hR] - s_ph[hL]) / h;
+      if (fi == 0) {
+        if (G.ltype[cj] == -2) {  // lapl_lib.F90:186-203 and :410-421
+          double a = s_ph[hL] > s_ph[hR] ? 1.0 : 0.0;
+          double te = s_te[hR], mue = s_mue[hR], ni = s_ni[hR], ne = s_ne[hR];
+          double ve = sqrt(K.ve_num * te / K.ve_den) * K.t0 / K.x0;
+          F.fi = (1.0 - a) * K.mui * E * ni - K.qvi * ni;
+          F.fe = -(a * mue * E * ne) - 2.5e-1 * ve * ne - K.gam * F.fi;
+          F.pi = (1.0 - a) * K.mui * ni / h;          // dfxi_mi
+          F.pe = -(a * mue * ne / h) - K.gam * F.pi;  // dfxe_mi
+        }
+      } else if (fi == bx) {
+        if (G.rtype[cj] == 2) {  // lapl_lib.F90:224-240 and :445-456
+          double a = s_ph[hR] > s_ph[hL] ? 1.0 : 0.0;
+          double te = s_te[hL], mue = s_mue[hL], ni = s_ni[hL], ne = s_ne[hL];
+          double ve = sqrt(K.ve_num * te / K.ve_den) * K.t0 / K.x0;
+          F.fi = (1.0 - a) * K.mui * E * ni + K.qvi * ni;
+          F.fe = -(a * mue * E * ne) + 2.5e-1 * ve * ne - K.gam * F.fi;
+          F.pi = (a - 1.0) * K.mui * ni / h;       // dfxi_pl
+          F.pe = a * mue * ne / h - K.gam * F.pi;  // dfxe_pl
+        }
+      } else {
+        F = lapl_face(K, E, h, s_mue[hL], s_mue[hR], s_de[hL], s_de[hR], s_ni[hL], s_ni[hR],
+                      s_ne[hL], s_ne[hR]);
+      }
+      if (isnan(F.pi) || isnan(F.pe)) s_bad = 3;
+    }
+    s_f[0][f] = F.fi;
+    s_f[1][f] = F.fe;
+    s_f[2][f] = F.pi;
+    s_f[3][f] = F.pe;
+  }
+  __syncthreads();
+
+  const int tx = threadIdx.x % kTX, ty = threadIdx.x / kTX;
+  const int ci = i0 + tx, cj = j0 + ty;
+  const bool valid = (ci < bx) && (cj < by);
+  const int hc = (ty + 1) * kHX + tx + 1;
+  double dflxi_x = 0, dflxe_x = 0, dfdx = 0, Cx = 0, aW = 0, aE = 0, gR = 0;
+  int typx = 0;
+  if (valid) {
+    typx = ci == 0 ? G.ltype[cj] : (ci == bx - 1 ? G.rtype[cj] : 0);
+    int fL = ty * (kTX + 1) + tx, fR = fL + 1;
+    double dlx = G.dlx[ci], hL = G.dx[ci], hR = G.dx[ci + 1];
+    dflxi_x = (s_f[0][fR] - s_f[0][fL]) / dlx;
+    dflxe_x = (s_f[1][fR] - s_f[1][fL]) / dlx;
+    double exL, exR;
+    if (typx == -1) {
+      exL = 0.0;
+      exR = -(s_ph[hc + 1] - s_ph[hc]) / hR;
+    } else if (typx == 1) {
+      exL = -(s_ph[hc] - s_ph[hc - 1]) / hL;
+      exR = 0.0;
+    } else {
+      exL = -(s_ph[hc] - s_ph[hc - 1]) / hL;
+      exR = -(s_ph[hc + 1] - s_ph[hc]) / hR;
+    }
+    dfdx = -(exR - exL) / dlx;
+    double dfxi_mi, dfxe_mi, dfxi_pl, dfxe_pl;
+    if (typx >= 0) {
+      dfxi_mi = s_f[2][fL];
+      dfxe_mi = -s_f[3][fL];
+      double gL = 1.0 / hL + dt * (dfxi_mi - dfxe_mi);
+      aW = gL / dlx;
+    } else if (typx == -2) {
+      dfxi_mi = s_f[2][fL];
+      dfxe_mi = s_f[3][fL];
+    } else {
+      dfxi_mi = 0.0;
+      dfxe_mi = 0.0;
+    }
+    if (typx <= 0) {
+      dfxi_pl = -s_f[2][fR];
+      dfxe_pl = s_f[3][fR];
+      gR = 1.0 / hR - dt * (dfxi_pl - dfxe_pl);
+      aE = gR / dlx;
+    } else if (typx == 2) {
+      dfxi_pl = s_f[2][fR];
+      dfxe_pl = s_f[3][fR];
+    } else {
+      dfxi_pl = 0.0;
+      dfxe_pl = 0.0;
+    }
+    Cx = dt * (dfxi_pl - dfxi_mi - dfxe_pl + dfxe_mi) / dlx;  // lapl_lib.F90:568-575
+    if (typx == -1 || typx == 1) Cx = Cx - 1.0 / hL / dlx;
+    else Cx = Cx - 1.0 / dlx * (1.0 / hR + 1.0 / hL);
+  }
+  __syncthreads();
+
+  // y faces (those on y = 0 and y = w carry nothing: type_y = -1 / 1)
+  for (int f = threadIdx.x; f < kNFY; f += kNT) {
+    int fx = f % kTX, fy = f / kTX;
+    int fci = i0 + fx, fj = j0 + fy;
+    FaceJ F = {0, 0, 0, 0};
+    if (fci < bx && fj > 0 && fj < by) {
+      int hL = fy * kHX + fx + 1, hR = hL + kHX;
+      double h = G.dy[fj];
+      double E = -(s_ph[hR] - s_ph[hL]) / h;
+      F = lapl_face(K, E, h, s_mue[hL], s_mue[hR], s_de[hL], s_de[hR], s_ni[hL], s_ni[hR],
+                    s_ne[hL], s_ne[hR]);
+      if (isnan(F.pi) || isnan(F.pe)) s_bad = 3;
+    }
+    s_f[0][f] = F.fi;
+    s_f[1][f] = F.fe;
+    s_f[2][f] = F.pi;
+    s_f[3][f] = F.pe;
+  }
+  __syncthreads();
+
+  double acc[2] = {0.0, 0.0};
+  if (valid) {
+    int fB = ty * kTX + tx, fT = fB + kTX;
+    double dly = G.dly[cj], hB = G.dy[cj], hT = G.dy[cj + 1];
+    double rj = G.r[cj + 1], rp = G.r[cj + 2], rm = G.r[cj];
+    int typy = cj == 0 ? -1 : (cj == by - 1 ? 1 : 0);
+    double eyB, eyT;
+    if (typy == -1) {
+      eyB = 0.0;
+      eyT = -(s_ph[hc + kHX] - s_ph[hc]) / hT;
+    } else if (typy == 1) {
+      eyB = -(s_ph[hc] - s_ph[hc - kHX]) / hB;
+      eyT = 0.0;
+    } else {
+      eyB = -(s_ph[hc] - s_ph[hc - kHX]) / hB;
+      eyT = -(s_ph[hc + kHX] - s_ph[hc]) / hT;
+    }
+    double dfdy, dflxi_y, dflxe_y;
+    double fiB = s_f[0][fB], fiT = s_f[0][fT], feB = s_f[1][fB], feT = s_f[1][fT];
+    if (G.cyl) {
+      dfdy = -(eyT * (rp + rj) / 2.0 - eyB * (rj + rm) / 2.0) / dly / rj;
+      dflxi_y = (fiT * (rp + rj) / 2.0 - fiB * (rj + rm) / 2.0) / dly / rj;
+      dflxe_y = (feT * (rp + rj) / 2.0 - feB * (rj + rm) / 2.0) / dly / rj;
+    } else {
+      dfdy = -(eyT - eyB) / dly;
+      dflxi_y = (fiT - fiB) / dly;
+      dflxe_y = (feT - feB) / dly;
+    }
+    // laplEqn residual, lapl_lib.F90:374-375
+    size_t m = (size_t)(cj + 1) * P + (ci + 1);
+    double ni = s_ni[hc], ne = s_ne[hc];
+    double b = dfdx + dfdy + ni - ne - dt * (dflxi_x + dflxi_y - dflxe_x - dflxe_y);
+
+    // laplJ y part and diagonal, lapl_lib.F90:463-607
+    double dfyi_mi = 0, dfye_mi = 0, dfyi_pl = 0, dfye_pl = 0, aS = 0, aN = 0, gT = 0;
+    if (typy >= 0) {
+      dfyi_mi = s_f[2][fB];
+      dfye_mi = -s_f[3][fB];
+      if (G.cyl)
+        aS = 5e-1 * (rj + rm) / hB / dly / rj + dt * (dfyi_mi - dfye_mi) * (rj + rm) / 2.0 / dly / rj;
+      else
+        aS = 1.0 / hB / dly + dt * (dfyi_mi - dfye_mi) / dly;
+    }
+    if (typy <= 0) {
+      dfyi_pl = -s_f[2][fT];
+      dfye_pl = s_f[3][fT];
+      gT = 1.0 / hT - dt * (dfyi_pl - dfye_pl);
+      if (G.cyl)
+        aN = 5e-1 * (rp + rj) / hT / dly / rj - dt * (dfyi_pl - dfye_pl) * (rp + rj) / 2.0 / dly / rj;
+      else
+        aN = 1.0 / hT / dly - dt * (dfyi_pl - dfye_pl) / dly;
+    }
+    double C = Cx;
+    if (G.cyl) {
+      C = C + dt * ((dfyi_pl - dfye_pl) * (rp + rj) / 2.0 - (dfyi_mi - dfye_mi) * (rj + rm) / 2.0) /
+                  dly / rj;
+      if (typy == -1) C = C - 1.0 / hB / dly * (rp + rj) / 2.0 / rj;
+      else if (typy == 1) C = C - 1.0 / hB / dly * (rj + rm) / 2.0 / rj;
+      else C = C - 5e-1 * ((rp + rj) / hT + (rj + rm) / hB) / dly / rj;
+    } else {
+      C = C + dt * (dfyi_pl - dfyi_mi - dfye_pl + dfye_mi) / dly;
+      if (typy == -1 || typy == 1) C = C - 1.0 / hB / dly;
+      else C = C - 1.0 / dly * (1.0 / hT + 1.0 / hB);
+    }
+    // symmetrised storage: row (i,j) times V = dlx*dly[*r]
+    double dlx = G.dlx[ci];
+    double rw = G.cyl ? rj : 1.0;
+    double V = dlx * dly * rw;
+    double rhs = V * b;  // -V*A*delta = V*b
+    double dgv = -V * C;
+    A.rhs0[m] = rhs;
+    A.r[m] = rhs;
+    A.dg[m] = dgv;
+    A.cx[m] = typx <= 0 ? gR * dly * rw : 0.0;
+    A.cy[m] = typy <= 0 ? gT * dlx * (G.cyl ? (rp + rj) / 2.0 : 1.0) : 0.0;
+    A.x[m] = 0.0;
+    A.p0[m] = 0.0;
+    A.p1[m] = 0.0;
+    if (A.bdbg) {
+      size_t n = (size_t)cj * bx + ci;
+      A.bdbg[n] = b;
+      double* a5 = A.adbg + 5 * n;
+      a5[0] = aW;
+      a5[1] = aE;
+      a5[2] = aS;
+      a5[3] = aN;
+      a5[4] = C;
+    }
+    if (isnan(b)) s_bad = 4;
+    acc[0] = b * b;
+    acc[1] = rhs * (rhs / dgv);
+  }
+  if (threadIdx.x == 0 && s_bad == 1) atomicMin(&A.sc->status, FS_E_NEGDENS);
+  if (threadIdx.x == 0 && s_bad == 3) atomicMin(&A.sc->status, FS_E_NANJ);
+  double tot[2];
+  if (last_block_sum<2>(acc, A.part, &A.sc->ticket[0], tot, s_red) && threadIdx.x == 0) {
+    DevScalars* sc = A.sc;
+    sc->bnorm2 = tot[0];
+    sc->rz = tot[1];
+    sc->alpha = 0.0;
+    sc->beta = 0.0;
+    sc->iters = 0;
+    sc->resid = tot[0] > 0 ? 1.0 : 0.0;
+    sc->done = tot[0] > 0 ? 0 : 1;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// PCG on M = -V*A (SPD): kernel A forms p = z + beta*p_old (z = r/d, the
+// neighbours' p recomputed in place of a separate pass), q = M p and p.q;
+// kernel B updates x, r and forms r.z and the reference-form residual.
+__global__ void __launch_bounds__(kNT) k_pcg_a(PcgArgs A) {
+  __shared__ double s_red[8 * kNW];
+  DevScalars* sc = A.sc;
+  if (sc->done) return;
+  const Geo& G = A.G;
+  const int t = xcd_tile(blockIdx.x, gridDim.x);
+  const int ci = (t % A.nbx) * kTX + (int)(threadIdx.x % kTX);
+  const int cj = (t / A.nbx) * kTY + (int)(threadIdx.x / kTX);
+  const double beta = sc->beta;
+  double acc[1] = {0.0};
+  if (ci < G.bx && cj < G.by) {
+    const int P = G.P;
+    size_t m = (size_t)(cj + 1) * P + (ci + 1);
+    const double *r = A.r, *d = A.dg, *po = A.pold;
+    double pc = r[m] / d[m] + beta * po[m];
+    double pw = r[m - 1] / d[m - 1] + beta * po[m - 1];
+    double pe = r[m + 1] / d[m + 1] + beta * po[m + 1];
+    double ps = r[m - P] / d[m - P] + beta * po[m - P];
+    double pn = r[m + P] / d[m + P] + beta * po[m + P];
+    double q = d[m] * pc - A.cx[m - 1] * pw - A.cx[m] * pe - A.cy[m - P] * ps - A.cy[m] * pn;
+    A.pnew[m] = pc;
+    A.q[m] = q;
+    acc[0] = pc * q;
+  }
+  double tot[1];
+  if (last_block_sum<1>(acc, A.part, &sc->ticket[1], tot, s_red) && threadIdx.x == 0)
+    sc->alpha = sc->rz / tot[0];
+}
+
+__global__ void __launch_bounds__(kNT) k_pcg_b(PcgArgs A) {
+  __shared__ double s_red[8 * kNW];
+  DevScalars* sc = A.sc;
+  if (sc->done) return;
+  const Geo& G = A.G;
+  const int t = xcd_tile(blockIdx.x, gridDim.x);
+  const int ci = (t % A.nbx) * kTX + (int)(threadIdx.x % kTX);
+  const int cj = (t / A.nbx) * kTY + (int)(threadIdx.x / kTX);
+  const double alpha = sc->alpha;
+  double acc[2] = {0.0, 0.0};
+  if (ci < G.bx && cj < G.by) {
+    size_t m = (size_t)(cj + 1) * G.P + (ci + 1);
+    double p = A.pnew[m];
+    A.x[m] = A.x[m] + alpha * p;
+    double r = A.r[m] - alpha * A.q[m];
+    A.r[m] = r;
+    double z = r / A.dg[m];
+    double V = G.dlx[ci] * G.dly[cj] * (G.cyl ? G.r[cj + 1] : 1.0);
+    double rr = r / V;
+    acc[0] = r * z;
+    acc[1] = rr * rr;
+  }
+  double tot[2];
+  if (last_block_sum<2>(acc, A.part, &sc->ticket[2], tot, s_red) && threadIdx.x == 0) {
+    sc->beta = tot[0] / sc->rz;
+    sc->rz = tot[0];
+    sc->iters = sc->iters + 1;
+    double res = sqrt(tot[1] / sc->bnorm2);
+    sc->resid = res;
+    if (res <= sc->tol) sc->done = 1;
+    else if (sc->iters >= sc->maxit) sc->done = 2;
+  }
+}
+
+// On convergence: ph += delta (lapl_lib.F90:112-116) and the true residual
+// || rhs0 - M x || in the reference-form norm.
+__global__ void __launch_bounds__(kNT) k_pcg_finish(PcgArgs A) {
+  __shared__ double s_red[8 * kNW];
+  DevScalars* sc = A.sc;
+  const Geo& G = A.G;
+  const int t = xcd_tile(blockIdx.x, gridDim.x);
+  const int ci = (t % A.nbx) * kTX + (int)(threadIdx.x % kTX);
+  const int cj = (t / A.nbx) * kTY + (int)(threadIdx.x / kTX);
+  const bool ok = sc->done == 1;
+  double acc[1] = {0.0};
+  if (ci < G.bx && cj < G.by) {
+    const int P = G.P;
+    size_t m = (size_t)(cj + 1) * P + (ci + 1);
+    const double* x = A.x;
+    double mx = A.dg[m] * x[m] - A.cx[m - 1] * x[m - 1] - A.cx[m] * x[m + 1] -
+                A.cy[m - P] * x[m - P] - A.cy[m] * x[m + P];
+    double V = G.dlx[ci] * G.dly[cj] * (G.cyl ? G.r[cj + 1] : 1.0);
+    double rr = (A.rhs0[m] - mx) / V;
+    acc[0] = rr * rr;
+    if (ok) A.ph[m] = A.ph[m] + x[m];
+  }
+  double tot[1];
+  if (last_block_sum<1>(acc, A.part, &sc->ticket[3], tot, s_red) && threadIdx.x == 0)
+    sc->tres = sc->bnorm2 > 0 ? sqrt(tot[0] / sc->bnorm2) : sqrt(tot[0]);
+}
+
+__global__ void k_fill(double* p, double v, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ void k_eval_fits(FsPhys K, const double* T, double* out, int n) {
+  int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= n) return;
+  double t = T[m], v = t * K.ph0, xa = fit_x(v);
+  Rates R = cell_rates(K, v, xa);
+  double* o = out + 12 * (size_t)m;
+  o[0] = mue_of_x(K, xa);
+  o[1] = mut_of_x(K, xa);
+  o[2] = de_of_x(K, xa);
+  o[3] = dt_of_x(K, xa);
+  o[4] = R.k_ex;
+  o[5] = R.k_ir;
+  o[6] = R.nu;
+  o[7] = R.k_sc;
+  o[8] = R.k_si;
+  o[9] = R.beta;
+  o[10] = 0.0;
+  o[11] = 0.0;
+}
+
+__global__ void k_eval_flux(FsPhys K, const double* a, double* out, int n, int dfdph) {
+  int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= n) return;
+  const double* v = a + 7 * (size_t)m;
+  double f, df;
+  sg_flux_dfdph(K, v[0], v[1], v[2], v[3], v[4], v[5], v[6], f, df);
+  out[m] = dfdph ? df : sg_flux(K, v[0], v[1], v[2], v[3], v[4], v[5], v[6]);
+}
+
+}  // namespace
+
+static inline int nblocks_of(const Geo& G, int* nbx, int* nby) {
+  *nbx = (G.bx + kTX - 1) / kTX;
+  *nby = (G.by + kTY - 1) / kTY;
+  return *nbx * *nby;
+}
+
+void launch_stage(const StageArgs& a, const FsPhys& K, hipStream_t s) {
+  hipLaunchKernelGGL(k_stage, dim3(a.nbx * a.nby), dim3(kNT), 0, s, a, K);
+}
+void launch_circ(const CircArgs& a, const FsPhys& K, hipStream_t s) {
+  hipLaunchKernelGGL(k_circ, dim3(1), dim3(kNT), 0, s, a, K);
+}
+void launch_ctrl(const CtrlArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(k_ctrl, dim3(1), dim3(kNT), 0, s, a);
+}
+void launch_accept(DevScalars* sc, hipStream_t s) {
+  hipLaunchKernelGGL(k_accept, dim3(1), dim3(1), 0, s, sc);
+}
+void launch_reject(DevScalars* sc, hipStream_t s) {
+  hipLaunchKernelGGL(k_reject, dim3(1), dim3(1), 0, s, sc);
+}
+void launch_begin_step(DevScalars* sc, hipStream_t s) {
+  hipLaunchKernelGGL(k_begin_step, dim3(1), dim3(1), 0, s, sc);
+}
+void launch_lapl_assemble(const LaplArgs& a, const FsPhys& K, hipStream_t s) {
+  hipLaunchKernelGGL(k_lapl, dim3(a.nbx * a.nby), dim3(kNT), 0, s, a, K);
+}
+void launch_pcg_a(const PcgArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(k_pcg_a, dim3(a.nbx * a.nby), dim3(kNT), 0, s, a);
+}
+void launch_pcg_b(const PcgArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(k_pcg_b, dim3(a.nbx * a.nby), dim3(kNT), 0, s, a);
+}
+void launch_pcg_finish(const PcgArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(k_pcg_finish, dim3(a.nbx * a.nby), dim3(kNT), 0, s, a);
+}
+void launch_fill(double* p, double v, size_t n, hipStream_t s) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, v, n);
+}
+void launch_eval_fits(const FsPhys& K, const double* T, double* out, int n, hipStream_t s) {
+  hipLaunchKernelGGL(k_eval_fits, dim3((n + 255) / 256), dim3(256), 0, s, K, T, out, n);
+}
+void launch_eval_flux(const FsPhys& K, const double* a, double* out, int n, int dfdph,
+                      hipStream_t s) {
+  hipLaunchKernelGGL(k_eval_flux, dim3((n + 255) / 256), dim3(256), 0, s, K, a, out, n, dfdph);
+}

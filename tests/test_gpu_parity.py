@@ -1,0 +1,148 @@
+"""GPU parity: libfsim_hip (through its C ABI) against the CPU oracle.
+
+Tolerances: the only intended differences are the device exp/log/pow versus
+the host libm (<= 1-2 ulp each), amplified where the reference itself is
+ill-conditioned (SG cancellation 1 - exp(-arg) for small arg), reduction order
+(error sums, Poisson dot products) and the Poisson solution, which both sides
+compute to near its rounding floor.
+"""
+import numpy as np
+import pytest
+
+from conftest import oracle_cfg, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _pair(O, cfg, seed=1, solver=0):
+    from fluidsim_amd import Discharge
+
+    d = Discharge(cfg)
+    o = O.Oracle(oracle_cfg(cfg, solver=solver))
+    d.init_gaussian(seed=seed)
+    o.init_gaussian(seed=seed)
+    return d, o
+
+
+def test_fits_match_oracle(O):
+    from fluidsim_amd import eval_fits
+
+    k = O.consts(10.0)
+    v = np.concatenate([np.geomspace(0.05, 400.0, 3001),
+                        [0.234, 0.9611, 1.37, 0.5, 16.0, 157.0, 5.119, 5.667, 7.5]])
+    T = v / k["ph0"]
+    for p in (0.5, 2.0, 10.0):
+        a = eval_fits(T, p)
+        b = O.eval_fits(T, p)
+        # a 1-ulp log difference moves s = poly(x) by |ds/dx|*x*eps (up to ~1e4*eps
+        # for the 11th-degree De/Dt fits), hence exp(s) by ~1e-12 relative
+        for col in range(10):
+            r = np.abs(a[:, col] - b[:, col]) / np.abs(b[:, col])
+            assert r.max() < 1e-11, (p, col, r.max())
+
+
+def test_sg_flux_and_dfdph_match_oracle(O):
+    from fluidsim_amd import eval_dfdph, eval_flux
+
+    rng = np.random.default_rng(7)
+    n = 20000
+    E = rng.standard_normal(n) * 10.0 ** rng.uniform(-14, 5, n)
+    h = rng.uniform(0.2, 3.0, n)
+    q = rng.choice([-1.0, 1.0], n)
+    mu = 10.0 ** rng.uniform(-3, 3, n)
+    D = 10.0 ** rng.uniform(-2, 4, n)
+    nl = 10.0 ** rng.uniform(-4, 8, n)
+    nr = 10.0 ** rng.uniform(-4, 8, n)
+    args = np.stack([E, h, q, mu, D, nl, nr], 1)
+    arg = np.abs(q * mu * E * h / D)
+    # SG cancellation: |1 - exp(-arg)| carries the device/host exp ulp as eps/arg
+    tol = 1e-13 + 8e-16 / np.minimum(arg, 1.0)
+    for fn_d, fn_o in ((eval_flux, O.eval_flux), (eval_dfdph, O.eval_dfdph)):
+        a, b = fn_d(args), fn_o(args)
+        # where 1 - exp(-a) rounds to 0 both sides give inf/NaN (the reference
+        # stops there: lapl_lib.F90:632-635); the same inputs must be non-finite
+        fin = np.isfinite(b)
+        assert np.array_equal(np.isfinite(a), fin)
+        scale = np.abs(b) + np.abs(q * mu * E) * (nl + nr) + D * (nl + nr) / h
+        err = np.abs(a - b)[fin] / scale[fin]
+        assert np.all(err <= tol[fin]), float((err / tol[fin]).max())
+
+
+def _warm(O, cfg, steps=4, seed=1):
+    """Oracle run to a nontrivial state, then the same state in both."""
+    d, o = _pair(O, cfg, seed=seed)
+    o.run(steps)
+    for name in ("ni", "ne", "nt", "nm"):
+        for lev in (1, 2, 3):
+            d.set(name, lev, o.get(name, lev))
+    d.set("ph", 1, o.get("ph"))
+    s = o.scalars()
+    d.set_scalars({k: s[k] for k in d.SCALARS})
+    return d, o
+
+
+@pytest.mark.parametrize("shape", [(64, 48), (17, 9)])
+@pytest.mark.parametrize("stage", [1, 3, 5])
+def test_rhs_sweep_matches_oracle(O, shape, stage):
+    from fluidsim_amd import configs
+
+    cfg = configs.base(*shape, debug=1, rtol=1e-13)
+    d, o = _warm(O, cfg)
+    d.rhs_sweep(stage)
+    o.rhs_sweep(stage)
+    for name in ("ki", "ke", "kt", "km"):
+        a = d.get(name, stage)[1:-1, 1:-1]
+        b = o.get(name, stage)[1:-1, 1:-1]
+        assert rel_l2(a, b) < 1e-11, (name, rel_l2(a, b))
+
+
+def test_lapl_assembly_matches_oracle(O):
+    from fluidsim_amd import configs
+
+    cfg = configs.base(64, 48, debug=1, rtol=1e-13)
+    d, o = _warm(O, cfg)
+    b_d, A_d = d.lapl_assemble()
+    st, b_o, A_o = o.lapl_assemble()
+    assert st == 0
+    assert rel_l2(b_d, b_o) < 1e-10, rel_l2(b_d, b_o)
+    for c in range(5):
+        assert rel_l2(A_d[..., c], A_o[..., c]) < 1e-12, (c, rel_l2(A_d[..., c], A_o[..., c]))
+
+
+def _compare(d, o, level=1):
+    out = {}
+    for name in ("ni", "ne", "nt", "nm"):
+        out[name] = rel_l2(d.get(name, level)[1:-1, 1:-1], o.get(name, level)[1:-1, 1:-1])
+    out["ph"] = rel_l2(d.get("ph")[1:-1, 1:-1], o.get("ph")[1:-1, 1:-1])
+    return out
+
+
+def test_one_step_parity(O):
+    """From an identical state, one main-loop pass (5 stages, circuit,
+    controller, Poisson, accept) with the oracle's dt decisions replayed."""
+    from fluidsim_amd import configs
+
+    cfg = configs.config0(rtol=1e-13)
+    d, o = _warm(O, cfg, steps=6)
+    o.clear_log()
+    o.run(1)
+    d.set_replay(o.log())
+    d.run(1)
+    errs = _compare(d, o)
+    print("one-step rel L2:", errs)
+    assert max(errs.values()) <= 1e-10, errs
+
+
+def test_trajectory_config0_replayed(O):
+    """Config-0 (64x64) trajectory with the oracle's dt sequence replayed:
+    relative L2 per field after 200 main-loop passes."""
+    from fluidsim_amd import configs
+
+    cfg = configs.config0(rtol=1e-13)
+    d, o = _pair(O, cfg, seed=0)
+    o.run(200)
+    d.set_replay(o.log())
+    d.run(200)
+    errs = _compare(d, o)
+    print("200-step rel L2:", errs)
+    assert max(errs.values()) <= 1e-8, errs
