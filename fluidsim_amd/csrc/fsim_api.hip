@@ -137,13 +137,14 @@ struct fs_handle {
   double* d_part = nullptr;
   DevScalars* d_sc = nullptr;
   DevScalars* h_sc = nullptr;  // pinned mirror
-  double* d_pois = nullptr;    // rhs0, r, cx, cy, dg, x, p0, p1, q
+  double* d_pois = nullptr;    // rhs0 r0 cx cy dg x p r1 w0 w1 s0 s1 (kPois planes)
   double* d_dbg = nullptr;     // b (N) + A5 (5N)
   double* d_replay = nullptr;
   int nreplay = 0;
   int rot = 0;        // level rotation: A = buf[rot], L1 = buf[rot+1], L3 = buf[rot+2]
   bool alias = true;  // level 1 currently equals level 2 (after accept/reject/init)
-  int nbx = 0, nby = 0, nblk = 0;
+  int nbx = 0, nby = 0, nblk = 0;   // stage / assembly tiles (64x4)
+  int pnby = 0, pnblk = 0;          // PCG tiles (64x8)
   Geo G;
   std::vector<double> log;
   bool timing = false;
@@ -158,7 +159,12 @@ struct fs_handle {
   int bufB() const { return 3; }
   double* kk(int which, int sp) const { return d_k + ((size_t)which * 4 + sp) * PL; }
   double* pois(int which) const { return d_pois + (size_t)which * PL; }
+  double* part(int which) const { return d_part + (size_t)which * 8 * ((size_t)nblk + 64); }
 };
+
+constexpr int kPois = 12;
+enum { PO_RHS0 = 0, PO_R0 = 1, PO_CX = 2, PO_CY = 3, PO_DG = 4, PO_X = 5, PO_P = 6, PO_R1 = 7,
+       PO_W0 = 8, PO_W1 = 9, PO_S0 = 10, PO_S1 = 11 };
 
 namespace {
 
@@ -233,7 +239,7 @@ StageArgs stage_args(fs_handle* h, int stage, int rhs_only) {
   }
   a.ph = h->d_ph;
   a.cur = h->d_cur;
-  a.part = h->d_part;
+  a.part = h->part(0);
   a.sc = h->d_sc;
   a.G = h->G;
   a.stage = stage;
@@ -267,24 +273,31 @@ CircArgs circ_args(fs_handle* h, int stage) {
   return c;
 }
 
-PcgArgs pcg_args(fs_handle* h, int parity) {
+// PCG buffers for iteration `it` (r, w, s double-buffered by parity)
+PcgArgs pcg_args(fs_handle* h, int it) {
   PcgArgs a;
   memset(&a, 0, sizeof(a));
-  a.rhs0 = h->pois(0);
-  a.r = h->pois(1);
-  a.cx = h->pois(2);
-  a.cy = h->pois(3);
-  a.dg = h->pois(4);
-  a.x = h->pois(5);
-  a.pold = parity ? h->pois(7) : h->pois(6);
-  a.pnew = parity ? h->pois(6) : h->pois(7);
-  a.q = h->pois(8);
+  int par = it & 1;
+  a.cx = h->pois(PO_CX);
+  a.cy = h->pois(PO_CY);
+  a.dg = h->pois(PO_DG);
+  a.r_old = h->pois(par ? PO_R1 : PO_R0);
+  a.r_new = h->pois(par ? PO_R0 : PO_R1);
+  a.w_old = h->pois(par ? PO_W1 : PO_W0);
+  a.w_new = h->pois(par ? PO_W0 : PO_W1);
+  a.s_old = h->pois(par ? PO_S1 : PO_S0);
+  a.s_new = h->pois(par ? PO_S0 : PO_S1);
+  a.p = h->pois(PO_P);
+  a.x = h->pois(PO_X);
+  a.rhs0 = h->pois(PO_RHS0);
   a.ph = h->d_ph;
-  a.part = h->d_part;
+  a.part_in = h->part(par);
+  a.part_out = h->part(par ^ 1);
   a.sc = h->d_sc;
   a.G = h->G;
   a.nbx = h->nbx;
-  a.nby = h->nby;
+  a.nby = h->pnby;
+  a.it = it;
   return a;
 }
 
@@ -311,15 +324,12 @@ int lapl_solve_impl(fs_handle* h, int* lerr, fs_solve_stats* stats) {
   a.n[1] = h->lev(l1, 1);
   a.n[2] = h->lev(l1, 2);
   a.ph = h->d_ph;
-  a.rhs0 = h->pois(0);
-  a.r = h->pois(1);
-  a.cx = h->pois(2);
-  a.cy = h->pois(3);
-  a.dg = h->pois(4);
-  a.x = h->pois(5);
-  a.p0 = h->pois(6);
-  a.p1 = h->pois(7);
-  a.part = h->d_part;
+  a.rhs0 = h->pois(PO_RHS0);
+  a.r = h->pois(PO_R0);
+  a.cx = h->pois(PO_CX);
+  a.cy = h->pois(PO_CY);
+  a.dg = h->pois(PO_DG);
+  a.part = h->part(2);
   a.sc = h->d_sc;
   a.G = h->G;
   a.nbx = h->nbx;
@@ -332,20 +342,22 @@ int lapl_solve_impl(fs_handle* h, int* lerr, fs_solve_stats* stats) {
   launch_lapl_assemble(a, h->K, h->st);
   HIP_OK(hipGetLastError());
   time_mark(h, 5);
+  PcgArgs ia = pcg_args(h, 0);
+  ia.part_out = h->part(0);  // what iteration 0 reads
+  ia.w_new = h->pois(PO_W0);  // w_0 is iteration 0's w_old
+  launch_pcg_init(ia, h->st);
   const int chunk = h->p.lapl_chunk > 0 ? h->p.lapl_chunk : 32;
   int it = 0;
   for (;;) {
-    for (int k = 0; k < chunk; k++, it++) {
-      PcgArgs pa = pcg_args(h, it & 1);
-      launch_pcg_a(pa, h->st);
-      launch_pcg_b(pa, h->st);
-    }
+    for (int k = 0; k < chunk; k++, it++) launch_pcg_it(pcg_args(h, it), h->st);
     HIP_OK(hipGetLastError());
     if (sync_scalars(h)) return FS_E_HIP;
     if (h->h_sc->status < 0) return check_status(h);
     if (h->h_sc->done) break;
   }
-  PcgArgs pa = pcg_args(h, it & 1);
+  PcgArgs pa = pcg_args(h, 0);
+  pa.part_out = h->part(2);
+  pa.part_in = h->part(2);
   launch_pcg_finish(pa, h->st);
   time_mark(h, 6);
   if (sync_scalars(h)) return FS_E_HIP;
@@ -391,7 +403,7 @@ int ptcl_step_impl(fs_handle* h, fs_step_info* info) {
     }
     time_mark(h, 1);
     CtrlArgs ca;
-    ca.part = h->d_part;
+    ca.part = h->part(0);
     ca.nblocks = h->nblk;
     ca.sc = h->d_sc;
     ca.replay = h->d_replay;
@@ -522,6 +534,8 @@ int fs_create(const fs_grid_desc* g, const fs_params* p, fs_handle** out) {
   h->nbx = (bx + kTX - 1) / kTX;
   h->nby = (by + kTY - 1) / kTY;
   h->nblk = h->nbx * h->nby;
+  h->pnby = (by + kPcgTY - 1) / kPcgTY;
+  h->pnblk = h->nbx * h->pnby;
 
   auto bad = [&](hipError_t e) {
     if (e == hipSuccess) return false;
@@ -537,10 +551,10 @@ int fs_create(const fs_grid_desc* g, const fs_params* p, fs_handle** out) {
       bad(hipMalloc(&h->d_types, 4 * (size_t)by)) ||
       bad(hipMalloc(&h->d_dr, 2 * (size_t)by * sizeof(double))) ||
       bad(hipMalloc(&h->d_cur, 2 * (size_t)by * sizeof(double))) ||
-      bad(hipMalloc(&h->d_part, ((size_t)h->nblk + 64) * 8 * sizeof(double))) ||
+      bad(hipMalloc(&h->d_part, 3 * ((size_t)h->nblk + 64) * 8 * sizeof(double))) ||
       bad(hipMalloc(&h->d_sc, sizeof(DevScalars))) ||
       bad(hipHostMalloc(&h->h_sc, sizeof(DevScalars), hipHostMallocDefault)) ||
-      bad(hipMalloc(&h->d_pois, 9 * PL * sizeof(double)))) {
+      bad(hipMalloc(&h->d_pois, kPois * PL * sizeof(double)))) {
     fs_destroy(h);
     return FS_E_HIP;
   }
@@ -581,8 +595,8 @@ int fs_create(const fs_grid_desc* g, const fs_params* p, fs_handle** out) {
   (void)hipMemsetAsync(h->d_k, 0, 12 * PL * sizeof(double), h->st);
   (void)hipMemsetAsync(h->d_ph, 0, PL * sizeof(double), h->st);
   (void)hipMemsetAsync(h->d_cur, 0, 2 * (size_t)by * sizeof(double), h->st);
-  (void)hipMemsetAsync(h->d_pois, 0, 9 * PL * sizeof(double), h->st);
-  launch_fill(h->pois(4), 1.0, PL, h->st);
+  (void)hipMemsetAsync(h->d_pois, 0, kPois * PL * sizeof(double), h->st);
+  launch_fill(h->pois(PO_DG), 1.0, PL, h->st);
   // circ_init (circ_lib.F90:213-223) and main.F90:65-66
   DevScalars& s = *h->h_sc;
   memset(&s, 0, sizeof(s));
@@ -886,17 +900,14 @@ int fs_lapl_assemble(fs_handle* h, double* b, double* A5) {
   int l1 = h->alias ? h->bufA() : h->bufL1();
   for (int s = 0; s < 3; s++) a.n[s] = h->lev(l1, s);
   a.ph = h->d_ph;
-  a.rhs0 = h->pois(0);
-  a.r = h->pois(1);
-  a.cx = h->pois(2);
-  a.cy = h->pois(3);
-  a.dg = h->pois(4);
-  a.x = h->pois(5);
-  a.p0 = h->pois(6);
-  a.p1 = h->pois(7);
+  a.rhs0 = h->pois(PO_RHS0);
+  a.r = h->pois(PO_R0);
+  a.cx = h->pois(PO_CX);
+  a.cy = h->pois(PO_CY);
+  a.dg = h->pois(PO_DG);
   a.bdbg = h->d_dbg;
   a.adbg = h->d_dbg + (size_t)h->bx * h->by;
-  a.part = h->d_part;
+  a.part = h->part(2);
   a.sc = h->d_sc;
   a.G = h->G;
   a.nbx = h->nbx;
@@ -946,6 +957,81 @@ static int eval_flux_common(int32_t n, const double* a, double* out, int dfdph) 
 int fs_eval_flux(int32_t n, const double* a, double* out) { return eval_flux_common(n, a, out, 0); }
 int fs_eval_dfdph(int32_t n, const double* a, double* out) {
   return eval_flux_common(n, a, out, 1);
+}
+
+int fs_time_kernel(fs_handle* h, int32_t which, int32_t n, double* avg_ms) {
+  if (!h || !avg_ms || n <= 0) return fail(FS_E_ARG, "bad argument%s");
+  if (sync_scalars(h)) return FS_E_HIP;
+  h->h_sc->done = 0;  // let the PCG kernels run
+  h->h_sc->maxit = 1 << 30;
+  h->h_sc->tol = 0.0;
+  if (push_scalars(h)) return FS_E_HIP;
+  auto launch = [&](int k) {
+    if (which >= 1 && which <= 5) {
+      StageArgs a = stage_args(h, which, 0);
+      launch_stage(a, h->K, h->st);
+    } else if (which == 10) {
+      LaplArgs a;
+      memset(&a, 0, sizeof(a));
+      for (int s = 0; s < 3; s++) a.n[s] = h->lev(h->bufL1(), s);
+      a.ph = h->d_ph;
+      a.rhs0 = h->pois(PO_RHS0);
+      a.r = h->pois(PO_R0);
+      a.cx = h->pois(PO_CX);
+      a.cy = h->pois(PO_CY);
+      a.dg = h->pois(PO_DG);
+      a.part = h->part(2);
+      a.sc = h->d_sc;
+      a.G = h->G;
+      a.nbx = h->nbx;
+      a.nby = h->nby;
+      launch_lapl_assemble(a, h->K, h->st);
+    } else if (which == 11) {
+      launch_pcg_it(pcg_args(h, 2 + (k & 1)), h->st);
+    } else {
+      PcgArgs ia = pcg_args(h, 0);
+      ia.part_out = h->part(0);
+      ia.w_new = h->pois(PO_W0);
+      launch_pcg_init(ia, h->st);
+    }
+  };
+  if (!(which >= 1 && which <= 5) && which != 10 && which != 11 && which != 12)
+    return fail(FS_E_ARG, "unknown kernel id%s");
+  launch(0);  // warm
+  HIP_OK(hipEventRecord(h->ev[6], h->st));
+  for (int k = 0; k < n; k++) launch(k);
+  HIP_OK(hipEventRecord(h->ev[7], h->st));
+  HIP_OK(hipEventSynchronize(h->ev[7]));
+  HIP_OK(hipGetLastError());
+  float ms = 0;
+  HIP_OK(hipEventElapsedTime(&ms, h->ev[6], h->ev[7]));
+  *avg_ms = (double)ms / n;
+  return 0;
+}
+
+int fs_copy_bench(int64_t bytes, int32_t reps, double* gbps) {
+  if (bytes < 16 || reps <= 0 || !gbps) return fail(FS_E_ARG, "bad argument%s");
+  size_t n2 = (size_t)bytes / 16;
+  double *a = nullptr, *b = nullptr;
+  hipEvent_t e0, e1;
+  HIP_OK(hipMalloc(&a, n2 * 16));
+  HIP_OK(hipMalloc(&b, n2 * 16));
+  HIP_OK(hipMemset(a, 0, n2 * 16));
+  HIP_OK(hipEventCreate(&e0));
+  HIP_OK(hipEventCreate(&e1));
+  launch_copy(a, b, n2, nullptr);
+  HIP_OK(hipEventRecord(e0, nullptr));
+  for (int r = 0; r < reps; r++) launch_copy(a, b, n2, nullptr);
+  HIP_OK(hipEventRecord(e1, nullptr));
+  HIP_OK(hipEventSynchronize(e1));
+  float ms = 0;
+  HIP_OK(hipEventElapsedTime(&ms, e0, e1));
+  *gbps = 2.0 * (double)(n2 * 16) * reps / (ms * 1e-3) / 1e9;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(a);
+  (void)hipFree(b);
+  return 0;
 }
 
 int fs_get_timers(const fs_handle* h, double* out4) {

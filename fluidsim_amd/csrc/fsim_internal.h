@@ -17,7 +17,7 @@ struct DevScalars {
   double err4[4];
   double ts;
   // Poisson / PCG
-  double rz, alpha, beta, bnorm2, resid, tol, tres, pad_d;
+  double cg_gam[2], cg_alpha[2], bnorm2, resid, tol, tres;
   int32_t iters, done, maxit, pad_i;
   int32_t status, accept, attempts, replay_idx;
   int32_t replay_n, pad_j[3];
@@ -81,9 +81,6 @@ struct LaplArgs {
   double* cx;           // x-face coupling to the right neighbour (symmetrised)
   double* cy;           // y-face coupling to the upper neighbour
   double* dg;           // diagonal of -V*A
-  double* x;            // solution, zeroed
-  double* p0;
-  double* p1;
   double* bdbg;         // debug: reference-form b (nx*ny), may be null
   double* adbg;         // debug: reference-form A5 (nx*ny*5), may be null
   double* part;
@@ -96,22 +93,28 @@ struct PcgArgs {
   const double* cx;
   const double* cy;
   const double* dg;
-  double* r;
+  const double* r_old;
+  double* r_new;
+  const double* w_old;
+  double* w_new;
+  double* s_old;        // s_{i-1} (read) -- zeroed by the init kernel
+  double* s_new;
+  double* p;
   double* x;
-  const double* pold;
-  double* pnew;
-  double* q;
   const double* rhs0;
   double* ph;
-  double* part;
+  const double* part_in;   // previous kernel's partials (3 per block)
+  double* part_out;
   DevScalars* sc;
   Geo G;
-  int32_t nbx, nby;
+  int32_t nbx, nby, it, pad_;
 };
 
 constexpr int kTX = 64;  // tile width (one wave64 per tile row)
 constexpr int kTY = 4;   // tile height
 constexpr int kNT = kTX * kTY;
+constexpr int kPcgTY = 8;    // PCG tile height: each thread owns kPcgRows cells
+constexpr int kPcgRows = kPcgTY / kTY;
 
 void launch_stage(const StageArgs& a, const FsPhys& K, hipStream_t s);
 void launch_circ(const CircArgs& a, const FsPhys& K, hipStream_t s);
@@ -120,10 +123,11 @@ void launch_accept(DevScalars* sc, hipStream_t s);
 void launch_reject(DevScalars* sc, hipStream_t s);
 void launch_begin_step(DevScalars* sc, hipStream_t s);
 void launch_lapl_assemble(const LaplArgs& a, const FsPhys& K, hipStream_t s);
-void launch_pcg_a(const PcgArgs& a, hipStream_t s);
-void launch_pcg_b(const PcgArgs& a, hipStream_t s);
+void launch_pcg_init(const PcgArgs& a, hipStream_t s);
+void launch_pcg_it(const PcgArgs& a, hipStream_t s);
 void launch_pcg_finish(const PcgArgs& a, hipStream_t s);
 void launch_fill(double* p, double v, size_t n, hipStream_t s);
+void launch_copy(const double* a, double* b, size_t n2, hipStream_t s);
 void launch_eval_fits(const FsPhys& K, const double* T, double* out, int n, hipStream_t s);
 void launch_eval_flux(const FsPhys& K, const double* a, double* out, int n, int dfdph,
                       hipStream_t s);

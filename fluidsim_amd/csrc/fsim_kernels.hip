@@ -718,7 +718,6 @@ __global__ void __launch_bounds__(kNT) k_lapl(LaplArgs A, FsPhys K) {
   }
   __syncthreads();
 
-  double acc[2] = {0.0, 0.0};
   if (valid) {
     int fB = ty * kTX + tx, fT = fB + kTX;
     double dly = G.dly[cj], hB = G.dy[cj], hT = G.dy[cj + 1];
@@ -793,9 +792,6 @@ __global__ void __launch_bounds__(kNT) k_lapl(LaplArgs A, FsPhys K) {
     A.dg[m] = dgv;
     A.cx[m] = typx <= 0 ? gR * dly * rw : 0.0;
     A.cy[m] = typy <= 0 ? gT * dlx * (G.cyl ? (rp + rj) / 2.0 : 1.0) : 0.0;
-    A.x[m] = 0.0;
-    A.p0[m] = 0.0;
-    A.p1[m] = 0.0;
     if (A.bdbg) {
       size_t n = (size_t)cj * bx + ci;
       A.bdbg[n] = b;
@@ -807,116 +803,194 @@ __global__ void __launch_bounds__(kNT) k_lapl(LaplArgs A, FsPhys K) {
       a5[4] = C;
     }
     if (isnan(b)) s_bad = 4;
-    acc[0] = b * b;
-    acc[1] = rhs * (rhs / dgv);
   }
   if (threadIdx.x == 0 && s_bad == 1) atomicMin(&A.sc->status, FS_E_NEGDENS);
   if (threadIdx.x == 0 && s_bad == 3) atomicMin(&A.sc->status, FS_E_NANJ);
-  double tot[2];
-  if (last_block_sum<2>(acc, A.part, &A.sc->ticket[0], tot, s_red) && threadIdx.x == 0) {
-    DevScalars* sc = A.sc;
-    sc->bnorm2 = tot[0];
-    sc->rz = tot[1];
-    sc->alpha = 0.0;
-    sc->beta = 0.0;
-    sc->iters = 0;
-    sc->resid = tot[0] > 0 ? 1.0 : 0.0;
-    sc->done = tot[0] > 0 ? 0 : 1;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    A.sc->done = 0;
+    A.sc->iters = 0;
   }
 }
 
 // ---------------------------------------------------------------------------
-// PCG on M = -V*A (SPD): kernel A forms p = z + beta*p_old (z = r/d, the
-// neighbours' p recomputed in place of a separate pass), q = M p and p.q;
-// kernel B updates x, r and forms r.z and the reference-form residual.
-__global__ void __launch_bounds__(kNT) k_pcg_a(PcgArgs A) {
-  __shared__ double s_red[8 * kNW];
-  DevScalars* sc = A.sc;
-  if (sc->done) return;
-  const Geo& G = A.G;
-  const int t = xcd_tile(blockIdx.x, gridDim.x);
-  const int ci = (t % A.nbx) * kTX + (int)(threadIdx.x % kTX);
-  const int cj = (t / A.nbx) * kTY + (int)(threadIdx.x / kTX);
-  const double beta = sc->beta;
-  double acc[1] = {0.0};
-  if (ci < G.bx && cj < G.by) {
-    const int P = G.P;
-    size_t m = (size_t)(cj + 1) * P + (ci + 1);
-    const double *r = A.r, *d = A.dg, *po = A.pold;
-    double pc = r[m] / d[m] + beta * po[m];
-    double pw = r[m - 1] / d[m - 1] + beta * po[m - 1];
-    double pe = r[m + 1] / d[m + 1] + beta * po[m + 1];
-    double ps = r[m - P] / d[m - P] + beta * po[m - P];
-    double pn = r[m + P] / d[m + P] + beta * po[m + P];
-    double q = d[m] * pc - A.cx[m - 1] * pw - A.cx[m] * pe - A.cy[m - P] * ps - A.cy[m] * pn;
-    A.pnew[m] = pc;
-    A.q[m] = q;
-    acc[0] = pc * q;
+// PCG on M = -V*A (SPD), Chronopoulos-Gear form: one kernel per iteration and
+// one fused reduction (gamma = r.u, delta = w.u, rho = |r/V|^2).  No kernel
+// communicates across its own blocks: every block reduces the previous
+// kernel's per-block partials itself (fixed order, so all blocks obtain the
+// same scalars), which needs no device-scope fence inside a kernel.
+//
+//   p_i = u_i + b_i p_{i-1}      s_i = w_i + b_i s_{i-1}      (s = M p)
+//   x_{i+1} = x_i + a_i p_i      r_{i+1} = r_i - a_i s_i
+//   u_{i+1} = r_{i+1} / d        w_{i+1} = M u_{i+1}
+// u at the four neighbours is recomputed from r_i, w_i, s_{i-1} of the
+// neighbour, so w_{i+1} needs no second pass.  r, w, s are double-buffered.
+
+// Sum the previous kernel's 3 partials per block, identically in every block.
+__device__ __forceinline__ void reduce_partials3(const double* part, int nb, double* s_red,
+                                                 double (&tot)[3]) {
+  double v[3] = {0.0, 0.0, 0.0};
+  for (int b = threadIdx.x; b < nb; b += kNT) {
+    v[0] += part[3 * b];
+    v[1] += part[3 * b + 1];
+    v[2] += part[3 * b + 2];
   }
-  double tot[1];
-  if (last_block_sum<1>(acc, A.part, &sc->ticket[1], tot, s_red) && threadIdx.x == 0)
-    sc->alpha = sc->rz / tot[0];
+  block_sum<3>(v, s_red);
+  __shared__ double s_tot[3];
+  if (threadIdx.x == 0) {
+    s_tot[0] = v[0];
+    s_tot[1] = v[1];
+    s_tot[2] = v[2];
+  }
+  __syncthreads();
+  tot[0] = s_tot[0];
+  tot[1] = s_tot[1];
+  tot[2] = s_tot[2];
 }
 
-__global__ void __launch_bounds__(kNT) k_pcg_b(PcgArgs A) {
+__device__ __forceinline__ void write_partials3(double (&v)[3], double* part, double* s_red) {
+  block_sum<3>(v, s_red);
+  if (threadIdx.x == 0) {
+    part[3 * blockIdx.x] = v[0];
+    part[3 * blockIdx.x + 1] = v[1];
+    part[3 * blockIdx.x + 2] = v[2];
+  }
+}
+
+// w0 = M u0 with u0 = r0/d, and the partials of gamma0, delta0, rho0.
+__global__ void __launch_bounds__(kNT) k_pcg_init(PcgArgs A) {
+  __shared__ double s_red[8 * kNW];
+  const Geo& G = A.G;
+  const int P = G.P;
+  const int t = xcd_tile(blockIdx.x, gridDim.x);
+  double acc[3] = {0.0, 0.0, 0.0};
+  for (int k = 0; k < kPcgRows; k++) {
+    const int ci = (t % A.nbx) * kTX + (int)(threadIdx.x % kTX);
+    const int cj = (t / A.nbx) * kPcgTY + (int)(threadIdx.x / kTX) + k * (kPcgTY / kPcgRows);
+    if (ci < G.bx && cj < G.by) {
+      size_t m = (size_t)(cj + 1) * P + (ci + 1);
+      const double *r = A.r_old, *d = A.dg;
+      double uc = r[m] / d[m];
+      double w = d[m] * uc - A.cx[m - 1] * (r[m - 1] / d[m - 1]) - A.cx[m] * (r[m + 1] / d[m + 1]) -
+                 A.cy[m - P] * (r[m - P] / d[m - P]) - A.cy[m] * (r[m + P] / d[m + P]);
+      A.w_new[m] = w;
+      A.p[m] = 0.0;
+      A.s_old[m] = 0.0;
+      A.x[m] = 0.0;
+      double V = G.dlx[ci] * G.dly[cj] * (G.cyl ? G.r[cj + 1] : 1.0);
+      double rv = r[m] / V;
+      acc[0] += r[m] * uc;
+      acc[1] += w * uc;
+      acc[2] += rv * rv;
+    }
+  }
+  write_partials3(acc, A.part_out, s_red);
+}
+
+__global__ void __launch_bounds__(kNT) k_pcg_it(PcgArgs A) {
   __shared__ double s_red[8 * kNW];
   DevScalars* sc = A.sc;
   if (sc->done) return;
-  const Geo& G = A.G;
-  const int t = xcd_tile(blockIdx.x, gridDim.x);
-  const int ci = (t % A.nbx) * kTX + (int)(threadIdx.x % kTX);
-  const int cj = (t / A.nbx) * kTY + (int)(threadIdx.x / kTX);
-  const double alpha = sc->alpha;
-  double acc[2] = {0.0, 0.0};
-  if (ci < G.bx && cj < G.by) {
-    size_t m = (size_t)(cj + 1) * G.P + (ci + 1);
-    double p = A.pnew[m];
-    A.x[m] = A.x[m] + alpha * p;
-    double r = A.r[m] - alpha * A.q[m];
-    A.r[m] = r;
-    double z = r / A.dg[m];
-    double V = G.dlx[ci] * G.dly[cj] * (G.cyl ? G.r[cj + 1] : 1.0);
-    double rr = r / V;
-    acc[0] = r * z;
-    acc[1] = rr * rr;
+  const int it = A.it;
+  double tot[3];
+  reduce_partials3(A.part_in, gridDim.x, s_red, tot);
+  const double gam = tot[0], del = tot[1], rho = tot[2];
+  const double bnorm2 = it == 0 ? rho : sc->bnorm2;
+  const double res = bnorm2 > 0 ? sqrt(rho / bnorm2) : 0.0;
+  double alpha, beta;
+  if (it == 0) {
+    beta = 0.0;
+    alpha = gam / del;
+  } else {
+    beta = gam / sc->cg_gam[(it + 1) & 1];
+    alpha = gam / (del - beta * gam / sc->cg_alpha[(it + 1) & 1]);
   }
-  double tot[2];
-  if (last_block_sum<2>(acc, A.part, &sc->ticket[2], tot, s_red) && threadIdx.x == 0) {
-    sc->beta = tot[0] / sc->rz;
-    sc->rz = tot[0];
-    sc->iters = sc->iters + 1;
-    double res = sqrt(tot[1] / sc->bnorm2);
+  const bool conv = (it > 0 && res <= sc->tol) || bnorm2 == 0.0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (it == 0) sc->bnorm2 = rho;
+    sc->cg_gam[it & 1] = gam;
+    sc->cg_alpha[it & 1] = alpha;
+    sc->iters = it;
     sc->resid = res;
-    if (res <= sc->tol) sc->done = 1;
-    else if (sc->iters >= sc->maxit) sc->done = 2;
+    if (conv) sc->done = 1;
+    else if (it >= sc->maxit) sc->done = 2;
   }
+  if (conv || it >= sc->maxit) return;
+
+  const Geo& G = A.G;
+  const int P = G.P;
+  const int t = xcd_tile(blockIdx.x, gridDim.x);
+  const double *r = A.r_old, *w = A.w_old, *so = A.s_old, *d = A.dg;
+  double acc[3] = {0.0, 0.0, 0.0};
+  for (int k = 0; k < kPcgRows; k++) {
+    const int ci = (t % A.nbx) * kTX + (int)(threadIdx.x % kTX);
+    const int cj = (t / A.nbx) * kPcgTY + (int)(threadIdx.x / kTX) + k * (kPcgTY / kPcgRows);
+    if (ci < G.bx && cj < G.by) {
+      size_t m = (size_t)(cj + 1) * P + (ci + 1);
+      // u_{i+1} of this cell and its neighbours (zero outside: r = s = w = 0, d = 1)
+      auto unext = [&](size_t q) { return (r[q] - alpha * (w[q] + beta * so[q])) / d[q]; };
+      double sc_ = w[m] + beta * so[m];
+      double pc = r[m] / d[m] + beta * A.p[m];
+      double rn = r[m] - alpha * sc_;
+      double un = rn / d[m];
+      double wn = d[m] * un - A.cx[m - 1] * unext(m - 1) - A.cx[m] * unext(m + 1) -
+                  A.cy[m - P] * unext(m - P) - A.cy[m] * unext(m + P);
+      A.p[m] = pc;
+      A.x[m] = A.x[m] + alpha * pc;
+      A.s_new[m] = sc_;
+      A.r_new[m] = rn;
+      A.w_new[m] = wn;
+      double V = G.dlx[ci] * G.dly[cj] * (G.cyl ? G.r[cj + 1] : 1.0);
+      double rv = rn / V;
+      acc[0] += rn * un;
+      acc[1] += wn * un;
+      acc[2] += rv * rv;
+    }
+  }
+  write_partials3(acc, A.part_out, s_red);
 }
 
-// On convergence: ph += delta (lapl_lib.F90:112-116) and the true residual
-// || rhs0 - M x || in the reference-form norm.
+// After the solve: ph += delta (lapl_lib.F90:112-116) when converged, and the
+// partials of the true residual |(rhs0 - M x)/V|^2.
 __global__ void __launch_bounds__(kNT) k_pcg_finish(PcgArgs A) {
   __shared__ double s_red[8 * kNW];
   DevScalars* sc = A.sc;
   const Geo& G = A.G;
+  const int P = G.P;
   const int t = xcd_tile(blockIdx.x, gridDim.x);
-  const int ci = (t % A.nbx) * kTX + (int)(threadIdx.x % kTX);
-  const int cj = (t / A.nbx) * kTY + (int)(threadIdx.x / kTX);
   const bool ok = sc->done == 1;
-  double acc[1] = {0.0};
-  if (ci < G.bx && cj < G.by) {
-    const int P = G.P;
-    size_t m = (size_t)(cj + 1) * P + (ci + 1);
-    const double* x = A.x;
-    double mx = A.dg[m] * x[m] - A.cx[m - 1] * x[m - 1] - A.cx[m] * x[m + 1] -
-                A.cy[m - P] * x[m - P] - A.cy[m] * x[m + P];
-    double V = G.dlx[ci] * G.dly[cj] * (G.cyl ? G.r[cj + 1] : 1.0);
-    double rr = (A.rhs0[m] - mx) / V;
-    acc[0] = rr * rr;
-    if (ok) A.ph[m] = A.ph[m] + x[m];
+  double acc[3] = {0.0, 0.0, 0.0};
+  for (int k = 0; k < kPcgRows; k++) {
+    const int ci = (t % A.nbx) * kTX + (int)(threadIdx.x % kTX);
+    const int cj = (t / A.nbx) * kPcgTY + (int)(threadIdx.x / kTX) + k * (kPcgTY / kPcgRows);
+    if (ci < G.bx && cj < G.by) {
+      size_t m = (size_t)(cj + 1) * P + (ci + 1);
+      const double* x = A.x;
+      double mx = A.dg[m] * x[m] - A.cx[m - 1] * x[m - 1] - A.cx[m] * x[m + 1] -
+                  A.cy[m - P] * x[m - P] - A.cy[m] * x[m + P];
+      double V = G.dlx[ci] * G.dly[cj] * (G.cyl ? G.r[cj + 1] : 1.0);
+      double rr = (A.rhs0[m] - mx) / V;
+      acc[0] += rr * rr;
+      if (ok) A.ph[m] = A.ph[m] + x[m];
+    }
   }
-  double tot[1];
-  if (last_block_sum<1>(acc, A.part, &sc->ticket[3], tot, s_red) && threadIdx.x == 0)
+  write_partials3(acc, A.part_out, s_red);
+}
+
+// Single block: true residual from the finish partials.
+__global__ void __launch_bounds__(kNT) k_pcg_tres(PcgArgs A, int nb) {
+  __shared__ double s_red[8 * kNW];
+  double tot[3];
+  reduce_partials3(A.part_in, nb, s_red, tot);
+  if (threadIdx.x == 0) {
+    DevScalars* sc = A.sc;
     sc->tres = sc->bnorm2 > 0 ? sqrt(tot[0] / sc->bnorm2) : sqrt(tot[0]);
+  }
+}
+
+__global__ void __launch_bounds__(256) k_copy(const double2* __restrict__ a,
+                                             double2* __restrict__ b, size_t n) {
+  size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) b[i] = a[i];
 }
 
 __global__ void k_fill(double* p, double v, size_t n) {
@@ -982,14 +1056,19 @@ void launch_begin_step(DevScalars* sc, hipStream_t s) {
 void launch_lapl_assemble(const LaplArgs& a, const FsPhys& K, hipStream_t s) {
   hipLaunchKernelGGL(k_lapl, dim3(a.nbx * a.nby), dim3(kNT), 0, s, a, K);
 }
-void launch_pcg_a(const PcgArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL(k_pcg_a, dim3(a.nbx * a.nby), dim3(kNT), 0, s, a);
+void launch_pcg_init(const PcgArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(k_pcg_init, dim3(a.nbx * a.nby), dim3(kNT), 0, s, a);
 }
-void launch_pcg_b(const PcgArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL(k_pcg_b, dim3(a.nbx * a.nby), dim3(kNT), 0, s, a);
+void launch_pcg_it(const PcgArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(k_pcg_it, dim3(a.nbx * a.nby), dim3(kNT), 0, s, a);
 }
 void launch_pcg_finish(const PcgArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(k_pcg_finish, dim3(a.nbx * a.nby), dim3(kNT), 0, s, a);
+  hipLaunchKernelGGL(k_pcg_tres, dim3(1), dim3(kNT), 0, s, a, a.nbx * a.nby);
+}
+void launch_copy(const double* a, double* b, size_t n2, hipStream_t s) {
+  hipLaunchKernelGGL(k_copy, dim3(256 * 16), dim3(256), 0, s, (const double2*)a, (double2*)b,
+                     n2);
 }
 void launch_fill(double* p, double v, size_t n, hipStream_t s) {
   if (n == 0) return;

@@ -166,10 +166,22 @@ class Discharge:
     def set_timing(self, on: bool):
         _L.check(_L.load().fs_set_timing(self.h, int(on)))
 
+    def time_kernel(self, which: int, n: int = 50) -> float:
+        out = np.zeros(1)
+        _L.check(_L.load().fs_time_kernel(self.h, int(which), int(n), _dp(out)))
+        return float(out[0])
+
     def timers(self) -> np.ndarray:
         out = np.zeros(4)
         _L.check(_L.load().fs_get_timers(self.h, _dp(out)))
         return out
+
+
+def copy_bandwidth(nbytes: int = 1 << 30, reps: int = 20) -> float:
+    """Measured device copy bandwidth (GB/s, read + write bytes)."""
+    out = np.zeros(1)
+    _L.check(_L.load().fs_copy_bench(int(nbytes), int(reps), _dp(out)))
+    return float(out[0])
 
 
 def constants(p_torr=10.0, gam=0.1) -> np.ndarray:

@@ -181,6 +181,19 @@ int fs_eval_dfdph(int32_t n, const double* args7, double* out);
 int fs_get_timers(const fs_handle* h, double* out4);
 int fs_set_timing(fs_handle* h, int32_t on);
 
+/* Average duration (ms) of n back-to-back launches of one hot-path kernel on
+ * the handle's stream, measured with HIP events on the current state.  The
+ * state is overwritten, so call it after the timed run.  which: 1..5 = the
+ * fused stage kernel of that RK stage, 10 = Poisson assembly, 11/12 = the two
+ * PCG iteration kernels. */
+int fs_time_kernel(fs_handle* h, int32_t which, int32_t n, double* avg_ms);
+
+/* Device copy bandwidth: `reps` launches of a plain copy kernel moving
+ * `bytes` (read) + `bytes` (write); *gbps = 2*bytes*reps / time.  The measured
+ * HBM reference next to the kernel rooflines, and the calibration kernel for
+ * the FETCH_SIZE / WRITE_SIZE counters. */
+int fs_copy_bench(int64_t bytes, int32_t reps, double* gbps);
+
 const char* fs_last_error(void);
 
 #ifdef __cplusplus
