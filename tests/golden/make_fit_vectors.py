@@ -229,15 +229,68 @@ def evaluate(ref: str, T: np.ndarray) -> dict:
     return out
 
 
+def ptcl_params(ref: str, env: dict) -> dict:
+    """Parameter constants of ptcl_props.F90 (module header), evaluated like props."""
+    text = open(os.path.join(ref, "ptcl_props.F90")).read()
+    head = text[: text.lower().index("contains")]
+    out = dict(env)
+    for st in statements(head):
+        m = re.match(r"real\(8\),\s*parameter\s*::\s*(.*)", st, re.I)
+        if not m:
+            continue
+        for part in re.split(r",\s*(?=[A-Za-z_]\w*\s*=)", m.group(1)):
+            name, expr = part.split("=", 1)
+            out[name.strip().lower()] = float(Expr(tokenize(expr), out, FENV).parse())
+    return out
+
+
+def python_restatements(ref: str, x: np.ndarray) -> dict:
+    """The reference's own Python restatements with double literals:
+    coeftest.py getDe/getDt (its x0/ninf*t0 scale is commented out) and
+    wvgRes_data.py nu (the argument of np.exp only), evaluated at the same x."""
+    out = {}
+    for fname, funcs in (("coeftest.py", ("getDt", "getDe")), ("wvgRes_data.py", ("nu",))):
+        text = open(os.path.join(ref, fname)).read()
+        for fn in funcs:
+            m = re.search(rf"def {fn}\(\w+\):(.*?)return", text, re.S)
+            body = m.group(1)
+            coef = {k: float(v) for k, v in re.findall(r"^\s*(\w)\s*=\s*([-+\d.eE]+)\s*$",
+                                                         body, re.M)}
+            rest = text[m.end():]
+            i0 = rest.index("np.exp(") + len("np.exp(")
+            depth, i = 1, i0
+            while depth:
+                depth += {"(": 1, ")": -1}.get(rest[i], 0)
+                i += 1
+            arg = rest[i0:i - 1].replace("\n", " ")
+            var = "y" if re.search(r"\by\b", arg) else "x"
+            env = dict(coef)
+            env[var] = x
+            toks = [(k, v) if k != "num" else ("num", v + "d0" if "e" not in v.lower()
+                                                 else v.lower().replace("e", "d"))
+                    for k, v in tokenize(arg)]
+            out[f"{fname}:{fn}"] = np.exp(Expr(toks, env, FENV).parse())
+    return out
+
+
 def main():
     ref = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
     env = props_env(ref)
+    penv = ptcl_params(ref, env)
     v = np.concatenate([np.geomspace(0.05, 400.0, 401),
                         [0.234, 0.9611, 1.37, 0.5, 16.0, 157.0, 5.119, 5.667, 7.5]])
     T = v / env["ph0"]
     res = evaluate(ref, T)
+    xa = np.log(np.maximum(0.234, np.minimum(157.0, T * env["ph0"])))
+    py = python_restatements(ref, xa)
+    consts = {f"c_{k}": np.array(penv[k]) for k in
+              ["eps0", "mu0", "e", "kb", "me", "x0", "ph0", "t0", "wp02", "w2", "tg", "p",
+               "ninf", "n_zero", "mi", "mui", "vi", "di", "dm", "k_r", "k_mp", "k_2q", "k_3q",
+               "h_ir", "h_ex", "h_si", "h_sc", "gam"]}
     dst = os.path.join(os.path.dirname(os.path.abspath(__file__)), "fit_vectors.npz")
-    np.savez(dst, T=T, p_torr=np.array(env["p"]), **{k: res[k] for k in FUNCS})
+    np.savez(dst, T=T, p_torr=np.array(env["p"]), **{k: res[k] for k in FUNCS},
+             py_De=py["coeftest.py:getDe"], py_Dt=py["coeftest.py:getDt"],
+             py_nu=py["wvgRes_data.py:nu"], **consts)
     print(f"wrote {dst}: {len(T)} points, p = {env['p']} Torr, ph0 = {env['ph0']!r}")
 
 

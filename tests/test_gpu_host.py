@@ -1,0 +1,160 @@
+"""GPU: grid, initial state, failure paths and model variants through the C ABI.
+
+Grid metrics and the Gaussian initial state are computed on the host by the
+library and must equal the oracle's bitwise.  Failure paths are forced from
+the host (negative density, dt floor, a solve capped below convergence,
+an exhausted replay table), never by making a kernel fault.
+"""
+import numpy as np
+import pytest
+
+from conftest import oracle_cfg, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _cfg(nx=24, ny=16, **kw):
+    from fluidsim_amd import configs
+
+    return configs.base(nx, ny, rtol=1e-13, **kw)
+
+
+@pytest.mark.parametrize("gx,gy,cyl", [(0, 0, 0), (1, 1, 0), (2, 0, 0), (2, 2, 1)])
+def test_grid_matches_oracle_bitwise(O, gx, gy, cyl):
+    from fluidsim_amd import Discharge
+
+    c = _cfg(30, 20, grid_x=gx, cyl=cyl)
+    c["grid_y"], c["ratio_y"] = gy, 1.03
+    d = Discharge(c)
+    o = O.Oracle(oracle_cfg(c))
+    gd, go = d.grid(), o.grid()
+    for k in go:
+        assert np.array_equal(gd[k], go[k]), k
+
+
+def test_gaussian_initial_state_matches_oracle_bitwise(O):
+    from fluidsim_amd import Discharge
+
+    c = _cfg(20, 14)
+    d = Discharge(c)
+    o = O.Oracle(oracle_cfg(c))
+    d.init_gaussian(seed=11, noise=0.05)
+    o.init_gaussian(seed=11, noise=0.05)
+    for name in ("ni", "ne", "nt", "nm"):
+        for lev in (1, 2, 3):
+            assert np.array_equal(d.get(name, lev), o.get(name, lev)), (name, lev)
+    d.ptcl_init()
+    o.init_uniform()
+    assert np.array_equal(d.get("nt", 2), o.get("nt", 2))
+
+
+def test_negative_density_is_fatal(O):
+    from fluidsim_amd import Discharge, lib
+
+    d = Discharge(_cfg())
+    d.init_gaussian(seed=1)
+    d.run(3)
+    nt = d.get("nt", 2)
+    nt[4, 5] = -1.0
+    d.set("nt", 2, nt)
+    with pytest.raises(lib.FsError) as e:
+        d.ptcl_step()
+    assert e.value.code == -2
+
+
+def test_dt_floor_is_fatal(O):
+    from fluidsim_amd import Discharge, lib
+
+    d = Discharge(_cfg())
+    d.init_gaussian(seed=1)
+    d.run(2)
+    d.set_scalars({"dt": 4e-13})
+    with pytest.raises(lib.FsError) as e:
+        d.ptcl_step()
+    assert e.value.code == -4
+
+
+def test_unconverged_solve_takes_the_reject_path(O):
+    from fluidsim_amd import Discharge
+
+    c = _cfg()
+    c["lapl_rtol"], c["lapl_maxit"] = 1e-30, 3
+    d = Discharge(c)
+    d.init_gaussian(seed=1)
+    s0 = d.scalars()
+    ph0, n3 = d.get("ph"), d.get("ne", 3)
+    lerr, st = d.step()
+    s1 = d.scalars()
+    assert lerr == 1 and st.iters >= 3
+    assert s1["dt"] == s0["dt"] / 2 and s1["ts"] == 0
+    assert np.array_equal(d.get("ph"), ph0)
+    assert np.array_equal(d.get("ne", 1), n3) and np.array_equal(d.get("ne", 2), n3)
+
+
+def test_exhausted_replay_is_fatal(O):
+    from fluidsim_amd import Discharge, lib
+
+    c = _cfg()
+    o = O.Oracle(oracle_cfg(c))
+    d = Discharge(c)
+    o.init_gaussian(seed=2)
+    d.init_gaussian(seed=2)
+    o.run(3)
+    d.set_replay(o.log()[:1])
+    with pytest.raises(lib.FsError) as e:
+        d.run(6)
+    assert e.value.code == -7
+
+
+def _replayed_parity(O, c, n, seed=0, fields=("ni", "ne", "nt", "nm", "ph"), init=None):
+    from fluidsim_amd import Discharge
+
+    d = Discharge(c)
+    o = O.Oracle(oracle_cfg(c))
+    d.init_gaussian(seed=seed)
+    o.init_gaussian(seed=seed)
+    if init:
+        init(d, o)
+    o.run(n)
+    d.set_replay(o.log())
+    d.run(n)
+    errs = {}
+    for f in fields:
+        a = d.get(f, 1)[1:-1, 1:-1]
+        b = o.get(f, 1)[1:-1, 1:-1]
+        errs[f] = rel_l2(a, b)
+    return d, o, errs
+
+
+def test_cylindrical_geometry_parity(O):
+    c = _cfg(24, 20, cyl=1, grid_x=0, ratio_x=1.0)
+    d, o, errs = _replayed_parity(O, c, 12)
+    print("cyl:", errs)
+    assert max(errs.values()) <= 1e-10, errs
+
+
+def test_partial_electrode_parity(O):
+    c = _cfg(24, 20)
+    c["ew"], c["gap"] = 0.4 * c["w"], 0.8 * c["w"]
+    d, o, errs = _replayed_parity(O, c, 12)
+    print("partial electrode:", errs)
+    assert max(errs.values()) <= 1e-10, errs
+    sd, so = d.scalars(), o.scalars()
+    for k in ("Vd_pl", "Ida", "Idc"):
+        assert sd[k] == pytest.approx(so[k], rel=1e-9, abs=1e-12 * abs(so["Vd_pl"])), k
+
+
+@pytest.mark.parametrize("rf,circuit", [(0, 1), (1, 0), (4, 0)])
+def test_circuit_variants_parity(O, rf, circuit):
+    """Active RC ballast (DC), AC source through the reference's RC stage, and
+    the trapezoid pulse: fields and circuit scalars after replayed steps."""
+    c = _cfg(24, 16, rf=rf, circuit=circuit, volts=300.0, res_ohm=1e5,
+             t_rise=1e-6, t_width=1e-5, t_fall=1e-6)
+    d, o, errs = _replayed_parity(O, c, 14)
+    print(f"rf={rf} circuit={circuit}:", errs)
+    assert max(errs.values()) <= 1e-10, errs
+    sd, so = d.scalars(), o.scalars()
+    for k in ("Vd_pl", "Vd_mi", "Vd_or", "Vsrc"):
+        assert sd[k] == pytest.approx(so[k], rel=1e-12), k
+    for k in ("Ida", "Idc"):
+        assert sd[k] == pytest.approx(so[k], rel=1e-8, abs=1e-14), k
