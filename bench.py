@@ -81,6 +81,19 @@ def cpu_baseline(args):
                        f"untimed passes; Jacobi-PCG Poisson to 1e-10, {nth} OpenMP threads")
 
 
+def profiled_traffic(kernel: str, n: int):
+    """HBM bytes per dispatch of `kernel` from the committed FETCH_SIZE /
+    WRITE_SIZE summary of this workload (profiles/), or None."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{n}x{n}_pulse_traffic.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if cells and cells[0] == kernel and len(cells) >= 7:
+            return float(cells[6]) * 1e6
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -142,7 +155,8 @@ def main():
         "kernel_ms": r["kernels"],
         "roofline": {"kernel": "k_pcg_it", "bound": "hbm", "achieved": ach_a,
                      "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": ach_a / PEAK_HBM_GBS,
-                     "traffic": None},
+                     "traffic": profiled_traffic("k_pcg_it", args.n),
+                     "algorithmic_bytes": bytes_pcg},
         "roofline_stage": {"kernel": "k_stage (stage 5)", "bound": "hbm", "achieved": ach_s,
                            "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": ach_s / PEAK_HBM_GBS},
         "hbm_copy_GBs_measured": r["copy_gbs"],

@@ -1,0 +1,87 @@
+! props.F90 (host) -- grid type, constants and g_init over libfsim_hip.
+!
+! Keeps the interface of the reference module `props` (props.F90:16-21 type
+! grid, props.F90:52 g_init) without MPI: one process, one GPU, one handle.
+! The grid metrics are computed by the library (fs_get_grid) so that host
+! and device agree bit for bit.
+module props
+  use iso_c_binding
+  use fsim_capi
+  implicit none
+
+  real(8) :: t_m = 1e9
+
+  type :: grid
+    integer :: nx, ny, bx, by, dof
+    integer, allocatable :: type_x(:,:), type_y(:,:)
+    real(8) :: l, w, ew, gap, dt, t
+    real(8), allocatable :: dx(:), dlx(:), dy(:), dly(:), r(:), x(:), y(:)
+    type(c_ptr) :: h = c_null_ptr      ! the device state of this discharge
+  end type
+
+  ! constants (props.F90:24-44), filled from the library's table
+  real(8) :: pi, eps0, e, kb, me, x0, ph0, t0, Tg, ninf, n_zero, n_init
+
+  ! run parameters that the reference fixes at compile time or reads in read_in
+  real(8) :: p = 10d0, gam = 0.1d0, res_ohm = 1e4, ratio = 1.05d0
+  real(8) :: t_rise = 0.01d0, t_width = 0.1d0, t_fall = 0.01d0
+  real(8) :: lapl_rtol = 1d-10
+  integer :: rf = 0, circuit = 0, grid_x = 0, grid_y = 0, lapl_maxit = 20000
+  logical :: unif = .True., cyl = .False., rwall = .False.
+
+contains
+
+  subroutine load_constants()
+    real(c_double) :: tab(40)
+    call fs_check(fs_constants(p, gam, tab), 'fs_constants')
+    pi = tab(1); eps0 = tab(2); e = tab(4); kb = tab(5); me = tab(6); x0 = tab(7)
+    ph0 = tab(8); t0 = tab(9); Tg = tab(12); ninf = tab(14); n_zero = tab(15)
+    n_init = tab(16)
+  end subroutine
+
+  ! *** Initialize grid and device state (props.F90:52) ***
+  ! px, py and path are accepted for interface compatibility: one rank, and
+  ! output paths are handled by the driver.
+  subroutine g_init(g, nx, ny, px, py, dof, l, w, ew, gap, path, vl, dt0)
+    type(grid), intent(inout), target :: g
+    integer, intent(in) :: nx, ny, px, py, dof
+    real(8), intent(in) :: l, w, ew, gap, vl, dt0
+    character(*), intent(in) :: path
+    type(fs_grid_desc) :: gd
+    type(fs_params) :: pr
+    integer :: rc
+
+    if (px * py /= 1) then
+      write(*,*) 'Error: one process drives one GPU (px = py = 1). Stop.'
+      stop 2
+    end if
+    call load_constants()
+    g%nx = nx; g%ny = ny; g%bx = nx; g%by = ny; g%dof = dof
+    g%l = l; g%w = w; g%ew = ew; g%gap = gap
+
+    gd%nx = nx; gd%ny = ny
+    gd%grid_x = grid_x; gd%grid_y = grid_y
+    if (.not. unif .and. grid_x == 0) gd%grid_x = 1
+    if (.not. unif .and. grid_y == 0) gd%grid_y = 1
+    gd%ratio_x = ratio; gd%ratio_y = 1d0
+    gd%l = l; gd%w = w; gd%ew = ew; gd%gap = gap
+    gd%cyl = merge(1, 0, cyl); gd%pad_ = 0
+
+    pr%p_torr = p; pr%gam = gam; pr%vset = vl; pr%res_ohm = res_ohm
+    pr%rf = rf; pr%circuit = circuit
+    pr%t_rise = t_rise; pr%t_width = t_width; pr%t_fall = t_fall
+    pr%dt0 = dt0; pr%lapl_rtol = lapl_rtol; pr%lapl_maxit = lapl_maxit
+    pr%lapl_chunk = 32; pr%debug_keep_k = 0; pr%pad_ = 0
+
+    rc = fs_create(gd, pr, g%h)
+    call fs_check(rc, 'fs_create')
+
+    allocate(g%x(nx+2), g%y(ny+2), g%dx(nx+1), g%dlx(nx), g%dy(ny+1), g%dly(ny), &
+             g%r(ny+2), g%type_x(nx,ny), g%type_y(nx,ny))
+    rc = fs_get_grid(g%h, c_loc(g%x), c_loc(g%y), c_loc(g%dx), c_loc(g%dlx), &
+                     c_loc(g%dy), c_loc(g%dly), c_loc(g%r), c_loc(g%type_x), &
+                     c_loc(g%type_y))
+    call fs_check(rc, 'fs_get_grid')
+    if (len_trim(path) < 0) write(*,*) path
+  end subroutine
+end module props
