@@ -145,6 +145,10 @@ struct fs_handle {
   bool alias = true;  // level 1 currently equals level 2 (after accept/reject/init)
   int nbx = 0, nby = 0, nblk = 0;   // stage / assembly tiles (64x4)
   int pnby = 0, pnblk = 0;          // PCG tiles (64x8)
+  std::vector<MgLevel> mg;          // multigrid hierarchy; level 0 aliases the PCG planes
+  double* d_mg = nullptr;
+  double* d_mginv = nullptr;        // dense inverse of the coarsest operator (<= 64 cells)
+  hipGraphExec_t mg_graph = nullptr;  // two MG-PCG iterations (parity 1, then 0)
   Geo G;
   std::vector<double> log;
   bool timing = false;
@@ -162,9 +166,13 @@ struct fs_handle {
   double* part(int which) const { return d_part + (size_t)which * 8 * ((size_t)nblk + 64); }
 };
 
-constexpr int kPois = 12;
+constexpr int kPois = 17;
 enum { PO_RHS0 = 0, PO_R0 = 1, PO_CX = 2, PO_CY = 3, PO_DG = 4, PO_X = 5, PO_P = 6, PO_R1 = 7,
-       PO_W0 = 8, PO_W1 = 9, PO_S0 = 10, PO_S1 = 11 };
+       PO_W0 = 8, PO_W1 = 9, PO_S0 = 10, PO_S1 = 11,
+       // multigrid-preconditioned PCG: p (double-buffered), q, u = B r, pre-smoothed x1
+       PO_MP0 = 12, PO_MP1 = 13, PO_MQ = 14, PO_MU = 15, PO_MX1 = 16 };
+constexpr double kMgOmega = 0.8;   // Jacobi damping of the smoother
+constexpr double kMgOver = 1.8;    // scale of the piecewise-constant coarse correction
 
 namespace {
 
@@ -315,6 +323,148 @@ int check_status(fs_handle* h) {
   return 0;
 }
 
+// Aggregation hierarchy: halve both axes until the coarsest level has at most
+// 64 cells (solved with its dense inverse).  Coarse planes: d (ghost 1), cx,
+// cy, r, x1, e (ghost 0).
+int build_mg(fs_handle* h) {
+  MgLevel L0;
+  L0.d = h->pois(PO_DG);
+  L0.cx = h->pois(PO_CX);
+  L0.cy = h->pois(PO_CY);
+  L0.r = h->pois(PO_R0);
+  L0.x1 = h->pois(PO_MX1);
+  L0.e = h->pois(PO_MU);
+  L0.nx = h->bx;
+  L0.ny = h->by;
+  L0.P = h->P;
+  L0.pad_ = 0;
+  h->mg.assign(1, L0);
+  std::vector<int> nxs, nys;
+  int nx = h->bx, ny = h->by;
+  size_t total = 0;
+  HIP_OK(hipMalloc(&h->d_mginv, 64 * 64 * sizeof(double)));
+  while ((size_t)nx * ny > 64) {
+    nx = (nx + 1) / 2;
+    ny = (ny + 1) / 2;
+    nxs.push_back(nx);
+    nys.push_back(ny);
+    total += 6 * (size_t)(nx + 2) * (ny + 2);
+  }
+  // very elongated grids would need more levels than the coarse-cycle kernel
+  // takes: they keep the one-level hierarchy (Jacobi-preconditioned solve)
+  if (total == 0 || nxs.size() + 1 > (size_t)kMgMaxLevels) return 0;
+  HIP_OK(hipMalloc(&h->d_mg, total * sizeof(double)));
+  HIP_OK(hipMemsetAsync(h->d_mg, 0, total * sizeof(double), h->st));
+  double* p = h->d_mg;
+  for (size_t l = 0; l < nxs.size(); l++) {
+    MgLevel L;
+    L.nx = nxs[l];
+    L.ny = nys[l];
+    L.P = L.nx + 2;
+    L.pad_ = 0;
+    size_t pl = (size_t)L.P * (L.ny + 2);
+    L.d = p;
+    L.cx = p + pl;
+    L.cy = p + 2 * pl;
+    L.r = p + 3 * pl;
+    L.x1 = p + 4 * pl;
+    L.e = p + 5 * pl;
+    p += 6 * pl;
+    launch_fill(L.d, 1.0, pl, h->st);
+    h->mg.push_back(L);
+  }
+  return 0;
+}
+
+MgArgs mg_args(fs_handle* h, size_t l) {
+  MgArgs a;
+  memset(&a, 0, sizeof(a));
+  a.F = h->mg[l];
+  if (l + 1 < h->mg.size()) a.C = h->mg[l + 1];
+  a.sc = h->d_sc;
+  a.G = h->G;
+  a.omega = kMgOmega;
+  a.oc = kMgOver;
+  a.level0 = l == 0;
+  return a;
+}
+
+MgPcgArgs mgpcg_args(fs_handle* h, int it) {
+  MgPcgArgs a;
+  memset(&a, 0, sizeof(a));
+  a.cx = h->pois(PO_CX);
+  a.cy = h->pois(PO_CY);
+  a.dg = h->pois(PO_DG);
+  a.u = h->pois(PO_MU);
+  a.p_old = h->pois((it & 1) ? PO_MP1 : PO_MP0);
+  a.p_new = h->pois((it & 1) ? PO_MP0 : PO_MP1);
+  a.q = h->pois(PO_MQ);
+  a.x = h->pois(PO_X);
+  a.r = h->pois(PO_R0);
+  a.sc = h->d_sc;
+  a.G = h->G;
+  a.nbx = h->nbx;
+  a.nby = h->pnby;
+  a.it = it;
+  a.first = it == 0;
+  return a;
+}
+
+// Partial-sum buffers of the MG-PCG chain: 0 = |r/V|^2, 1 = r.u, 2 = p.q
+void launch_vcycle(fs_handle* h, int it, bool init, int nb_rho) {
+  const size_t L = h->mg.size();
+  // levels of <= 64x64 cells (and never level 0) run in one workgroup
+  size_t first = 1;
+  while (first + 1 < L && (size_t)h->mg[first].nx * h->mg[first].ny > 4096) first++;
+  for (size_t l = 0; l < first; l++) {
+    MgArgs a = mg_args(h, l);
+    a.it = it;
+    a.init = init;
+    a.part_in = h->part(0);
+    a.nb_in = nb_rho;
+    launch_mg_down(a, h->st);
+  }
+  if (first + 1 < L)
+    launch_mg_coarse_cycle(h->mg.data(), (int)L, (int)first, h->d_mginv, h->d_sc, kMgOmega,
+                           kMgOver, h->st);
+  else
+    launch_mg_bottom(mg_args(h, L - 1), h->d_mginv, h->st);
+  for (size_t l = first; l-- > 0;) {
+    MgArgs a = mg_args(h, l);
+    a.part_out = h->part(1);
+    launch_mg_up(a, mg_up_blocks(a.F), h->st);
+  }
+}
+
+// One MG-PCG iteration: p/q, x/r update, V-cycle (convergence is tested in
+// the V-cycle's first kernel).  Only the parity of `it` matters after it = 0.
+void launch_mg_iteration(fs_handle* h, int it) {
+  const int nb_pcg = h->nbx * h->pnby;
+  MgPcgArgs a = mgpcg_args(h, it);
+  a.part_in = h->part(1);
+  a.nb_in = mg_up_blocks(h->mg[0]);
+  a.part_out = h->part(2);
+  launch_mgpcg_pq(a, h->st);
+  a.part_in = h->part(2);
+  a.nb_in = nb_pcg;
+  a.part_out = h->part(0);
+  launch_mgpcg_xr(a, h->st);
+  launch_vcycle(h, it, false, nb_pcg);
+}
+
+// Capture two iterations (parity 1 then 0) once per handle; every buffer is
+// fixed for the handle's lifetime, so the graph is replayed for all solves.
+int make_mg_graph(fs_handle* h) {
+  hipGraph_t g = nullptr;
+  HIP_OK(hipStreamBeginCapture(h->st, hipStreamCaptureModeThreadLocal));
+  launch_mg_iteration(h, 1);
+  launch_mg_iteration(h, 2);
+  HIP_OK(hipStreamEndCapture(h->st, &g));
+  HIP_OK(hipGraphInstantiate(&h->mg_graph, g, nullptr, nullptr, 0));
+  HIP_OK(hipGraphDestroy(g));
+  return 0;
+}
+
 int lapl_solve_impl(fs_handle* h, int* lerr, fs_solve_stats* stats) {
   auto t0 = std::chrono::steady_clock::now();
   LaplArgs a;
@@ -342,18 +492,44 @@ int lapl_solve_impl(fs_handle* h, int* lerr, fs_solve_stats* stats) {
   launch_lapl_assemble(a, h->K, h->st);
   HIP_OK(hipGetLastError());
   time_mark(h, 5);
-  PcgArgs ia = pcg_args(h, 0);
-  ia.part_out = h->part(0);  // what iteration 0 reads
-  ia.w_new = h->pois(PO_W0);  // w_0 is iteration 0's w_old
-  launch_pcg_init(ia, h->st);
   const int chunk = h->p.lapl_chunk > 0 ? h->p.lapl_chunk : 32;
   int it = 0;
-  for (;;) {
-    for (int k = 0; k < chunk; k++, it++) launch_pcg_it(pcg_args(h, it), h->st);
-    HIP_OK(hipGetLastError());
-    if (sync_scalars(h)) return FS_E_HIP;
-    if (h->h_sc->status < 0) return check_status(h);
-    if (h->h_sc->done) break;
+  if (h->p.lapl_solver == 1 && h->mg.size() > 1) {
+    // multigrid-preconditioned PCG (grids of <= 64 cells use Jacobi-PCG)
+    for (size_t l = 0; l + 1 < h->mg.size(); l++) launch_mg_coarsen(mg_args(h, l), h->st);
+    launch_mg_factor(mg_args(h, h->mg.size() - 1), h->d_mginv, h->st);
+    MgPcgArgs ia = mgpcg_args(h, 1);  // zero the p that iteration 0 reads as p_old
+    ia.part_out = h->part(0);
+    launch_mgpcg_init(ia, h->st);
+    const int nb_pcg = h->nbx * h->pnby;
+    launch_vcycle(h, 0, true, nb_pcg);
+    launch_mg_iteration(h, 0);
+    it = 1;
+    if (h->p.use_graphs && !h->mg_graph && make_mg_graph(h)) return FS_E_HIP;
+    const int mg_chunk = chunk > 8 ? chunk / 4 : chunk;
+    for (;;) {
+      if (h->p.use_graphs) {
+        for (int k = 0; k < mg_chunk; k += 2, it += 2) HIP_OK(hipGraphLaunch(h->mg_graph, h->st));
+      } else {
+        for (int k = 0; k < mg_chunk; k++, it++) launch_mg_iteration(h, it);
+      }
+      HIP_OK(hipGetLastError());
+      if (sync_scalars(h)) return FS_E_HIP;
+      if (h->h_sc->status < 0) return check_status(h);
+      if (h->h_sc->done) break;
+    }
+  } else {
+    PcgArgs ia = pcg_args(h, 0);
+    ia.part_out = h->part(0);  // what iteration 0 reads
+    ia.w_new = h->pois(PO_W0);  // w_0 is iteration 0's w_old
+    launch_pcg_init(ia, h->st);
+    for (;;) {
+      for (int k = 0; k < chunk; k++, it++) launch_pcg_it(pcg_args(h, it), h->st);
+      HIP_OK(hipGetLastError());
+      if (sync_scalars(h)) return FS_E_HIP;
+      if (h->h_sc->status < 0) return check_status(h);
+      if (h->h_sc->done) break;
+    }
   }
   PcgArgs pa = pcg_args(h, 0);
   pa.part_out = h->part(2);
@@ -597,6 +773,10 @@ int fs_create(const fs_grid_desc* g, const fs_params* p, fs_handle** out) {
   (void)hipMemsetAsync(h->d_cur, 0, 2 * (size_t)by * sizeof(double), h->st);
   (void)hipMemsetAsync(h->d_pois, 0, kPois * PL * sizeof(double), h->st);
   launch_fill(h->pois(PO_DG), 1.0, PL, h->st);
+  if (build_mg(h)) {
+    fs_destroy(h);
+    return FS_E_HIP;
+  }
   // circ_init (circ_lib.F90:213-223) and main.F90:65-66
   DevScalars& s = *h->h_sc;
   memset(&s, 0, sizeof(s));
@@ -637,6 +817,9 @@ int fs_destroy(fs_handle* h) {
   (void)hipFree(h->d_pois);
   (void)hipFree(h->d_dbg);
   (void)hipFree(h->d_replay);
+  (void)hipFree(h->d_mg);
+  (void)hipFree(h->d_mginv);
+  if (h->mg_graph) (void)hipGraphExecDestroy(h->mg_graph);
   for (int e = 0; e < 8; e++)
     if (h->ev[e]) (void)hipEventDestroy(h->ev[e]);
   if (h->st) (void)hipStreamDestroy(h->st);

@@ -21,6 +21,7 @@ struct DevScalars {
   int32_t iters, done, maxit, pad_i;
   int32_t status, accept, attempts, replay_idx;
   int32_t replay_n, pad_j[3];
+  int32_t itc[2], pad_k[2];   // MG-PCG iteration count by parity
   uint32_t ticket[8];
 };
 
@@ -110,6 +111,48 @@ struct PcgArgs {
   int32_t nbx, nby, it, pad_;
 };
 
+// One level of the aggregation multigrid hierarchy: (nx+2)*(ny+2) planes
+// with zero ghosts (d ghost = 1) so stencils need no bounds checks.
+struct MgLevel {
+  double* d;    // diagonal of the (Galerkin) operator
+  double* cx;   // coupling to the right neighbour
+  double* cy;   // coupling to the upper neighbour
+  double* r;    // residual handed down to this level (level 0: the PCG residual)
+  double* x1;   // pre-smoothed iterate
+  double* e;    // correction returned to the finer level (level 0: u = B r)
+  int32_t nx, ny, P, pad_;
+};
+
+struct MgArgs {
+  MgLevel F;             // fine level of this kernel
+  MgLevel C;             // coarse level
+  const double* part_in; // level-0 down: residual partials of the x/r update
+  int32_t nb_in;
+  double* part_out;      // level-0 up: partials of gamma = r.u
+  DevScalars* sc;
+  Geo G;                 // level-0 metrics (volumes for the residual norm)
+  double omega, oc;
+  int32_t it, level0, init, pad_;
+};
+
+struct MgPcgArgs {
+  const double* cx;
+  const double* cy;
+  const double* dg;
+  const double* u;
+  const double* p_old;
+  double* p_new;
+  double* q;
+  double* x;
+  double* r;
+  const double* part_in;
+  int32_t nb_in, it;      // `it` is used for its parity only
+  double* part_out;
+  DevScalars* sc;
+  Geo G;
+  int32_t nbx, nby, first, pad_;
+};
+
 constexpr int kTX = 64;  // tile width (one wave64 per tile row)
 constexpr int kTY = 4;   // tile height
 constexpr int kNT = kTX * kTY;
@@ -126,6 +169,19 @@ void launch_lapl_assemble(const LaplArgs& a, const FsPhys& K, hipStream_t s);
 void launch_pcg_init(const PcgArgs& a, hipStream_t s);
 void launch_pcg_it(const PcgArgs& a, hipStream_t s);
 void launch_pcg_finish(const PcgArgs& a, hipStream_t s);
+void launch_mg_coarsen(const MgArgs& a, hipStream_t s);
+void launch_mg_down(const MgArgs& a, hipStream_t s);
+void launch_mg_up(const MgArgs& a, int nb, hipStream_t s);
+void launch_mg_factor(const MgArgs& a, double* inv, hipStream_t s);
+// most levels a hierarchy may have (the coarse-cycle kernel takes them by value)
+constexpr int kMgMaxLevels = 16;
+void launch_mg_coarse_cycle(const MgLevel* lv, int nlev, int first, const double* inv,
+                            DevScalars* sc, double omega, double oc, hipStream_t s);
+void launch_mg_bottom(const MgArgs& a, const double* inv, hipStream_t s);
+void launch_mgpcg_init(const MgPcgArgs& a, hipStream_t s);
+void launch_mgpcg_pq(const MgPcgArgs& a, hipStream_t s);
+void launch_mgpcg_xr(const MgPcgArgs& a, hipStream_t s);
+int mg_up_blocks(const MgLevel& F);
 void launch_fill(double* p, double v, size_t n, hipStream_t s);
 void launch_copy(const double* a, double* b, size_t n2, hipStream_t s);
 void launch_eval_fits(const FsPhys& K, const double* T, double* out, int n, hipStream_t s);

@@ -17,6 +17,7 @@
 // Reductions are deterministic: per-block partials in a fixed tree order,
 // then the last block (atomic ticket) sums the partials in a fixed order.
 #include <hip/hip_runtime.h>
+#include <string.h>
 
 #include "fsim_internal.h"
 
@@ -987,6 +988,331 @@ __global__ void __launch_bounds__(kNT) k_pcg_tres(PcgArgs A, int nb) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Aggregation multigrid V(1,1)-cycle used as the PCG preconditioner.
+//
+// Level l+1 aggregates 2x2 cells of level l.  With piecewise-constant
+// prolongation P and restriction P^T the Galerkin operator P^T M P keeps the
+// 5-point face storage: a coarse face coupling is the sum of the fine faces
+// that cross it; the coarse diagonal is the sum of the four fine diagonals
+// minus twice the four internal faces.  One cycle, from a zero guess:
+//   x1 = w D^-1 r,  r1 = r - M x1  (restricted: r_c = P^T r1)      [down]
+//   x2 = x1 + oc P e_c,  e = x2 + w D^-1 (r - M x2)                  [up]
+// which is symmetric positive definite for w*lambda_max(D^-1 M) < 2, so plain
+// CG remains valid.  The neighbour values of x1 and x2 that the stencils need
+// are recomputed (x1 = w r/d is pointwise, x2 = x1 + oc*e_c), so each level
+// costs one down and one up kernel.  The coarsest level (<= 8x8 cells) is
+// smoothed to convergence inside one workgroup.
+
+__device__ __forceinline__ size_t mg_at(const MgLevel& L, int i, int j) {
+  return (size_t)(j + 1) * L.P + (i + 1);
+}
+
+// Down step for coarse cell n of C: pre-smooth its (up to) 2x2 fine cells,
+// keep x1, restrict the residual.  The neighbours' x1 = w r/d is recomputed.
+__device__ __forceinline__ void mg_down_cell(const MgLevel& F, const MgLevel& C, int n,
+                                             double w) {
+  int I = n % C.nx, J = n / C.nx;
+  const double *r = F.r, *d = F.d;
+  auto x1 = [&](size_t q) { return w * r[q] / d[q]; };
+  double rc = 0.0;
+  for (int b = 0; b < 2; b++)
+    for (int a = 0; a < 2; a++) {
+      int i = 2 * I + a, j = 2 * J + b;
+      if (i >= F.nx || j >= F.ny) continue;
+      size_t m = mg_at(F, i, j), P = F.P;
+      double xc = x1(m);
+      double mx = d[m] * xc - F.cx[m - 1] * x1(m - 1) - F.cx[m] * x1(m + 1) -
+                  F.cy[m - P] * x1(m - P) - F.cy[m] * x1(m + P);
+      F.x1[m] = xc;
+      rc += r[m] - mx;
+    }
+  C.r[mg_at(C, I, J)] = rc;
+}
+
+// Up step for fine cell n of F: x2 = x1 + oc * e_c(aggregate), one
+// post-smoothing sweep; returns r.e (level 0 sums it into gamma).
+__device__ __forceinline__ double mg_up_cell(const MgLevel& F, const MgLevel& C, int n, double w,
+                                             double oc) {
+  int i = n % F.nx, j = n / F.nx;
+  const double* ec = C.e;
+  size_t m = mg_at(F, i, j), P = F.P;
+  // cells outside the level carry zero couplings: their value only has to be finite
+  auto x2 = [&](size_t q, int ii, int jj) {
+    return F.x1[q] + oc * ec[mg_at(C, ii >> 1, jj >> 1)];
+  };
+  double xc = x2(m, i, j);
+  double mx = F.d[m] * xc - F.cx[m - 1] * x2(m - 1, i - 1, j) - F.cx[m] * x2(m + 1, i + 1, j) -
+              F.cy[m - P] * x2(m - P, i, j - 1) - F.cy[m] * x2(m + P, i, j + 1);
+  double e = xc + w * (F.r[m] - mx) / F.d[m];
+  F.e[m] = e;
+  return F.r[m] * e;
+}
+
+__global__ void __launch_bounds__(256) k_mg_coarsen(MgArgs A) {
+  const MgLevel &F = A.F, &C = A.C;
+  int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= C.nx * C.ny) return;
+  int I = n % C.nx, J = n / C.nx;
+  int i0 = 2 * I, j0 = 2 * J;
+  bool hx = i0 + 1 < F.nx, hy = j0 + 1 < F.ny;
+  double d = F.d[mg_at(F, i0, j0)];
+  double internal = 0.0;
+  if (hx) {
+    d += F.d[mg_at(F, i0 + 1, j0)];
+    internal += F.cx[mg_at(F, i0, j0)];
+  }
+  if (hy) {
+    d += F.d[mg_at(F, i0, j0 + 1)];
+    internal += F.cy[mg_at(F, i0, j0)];
+  }
+  if (hx && hy) {
+    d += F.d[mg_at(F, i0 + 1, j0 + 1)];
+    internal += F.cx[mg_at(F, i0, j0 + 1)] + F.cy[mg_at(F, i0 + 1, j0)];
+  }
+  size_t m = mg_at(C, I, J);
+  C.d[m] = d - 2.0 * internal;
+  // faces leaving the aggregate to the right / upwards (zero at the domain edge)
+  int ir = hx ? i0 + 1 : i0, ju = hy ? j0 + 1 : j0;
+  double cx = 0.0, cy = 0.0;
+  if (I + 1 < C.nx) {
+    cx = F.cx[mg_at(F, ir, j0)];
+    if (hy) cx += F.cx[mg_at(F, ir, j0 + 1)];
+  }
+  if (J + 1 < C.ny) {
+    cy = F.cy[mg_at(F, i0, ju)];
+    if (hx) cy += F.cy[mg_at(F, i0 + 1, ju)];
+  }
+  C.cx[m] = cx;
+  C.cy[m] = cy;
+}
+
+// One thread per coarse cell: pre-smooth and residual of its (up to) 2x2
+// fine cells, restricted sum into the coarse residual.  Level 0 first checks
+// convergence from the x/r-update partials (every block identically).
+__global__ void __launch_bounds__(256) k_mg_down(MgArgs A) {
+  __shared__ double s_red[8 * kNW];
+  DevScalars* sc = A.sc;
+  if (sc->done) return;
+  if (A.level0) {
+    double tot[3];
+    reduce_partials3(A.part_in, A.nb_in, s_red, tot);
+    const double rho = tot[0];
+    if (A.init) {
+      if (blockIdx.x == 0 && threadIdx.x == 0) {
+        sc->bnorm2 = rho;
+        sc->resid = rho > 0 ? 1.0 : 0.0;
+        sc->iters = 0;
+        sc->itc[0] = 0;
+        sc->itc[1] = 0;
+        if (rho == 0.0) sc->done = 1;
+      }
+      if (rho == 0.0) return;
+    } else {
+      // iteration count kept in two slots by parity: this kernel reads the
+      // other slot (written by the previous iteration) and block 0 writes its own
+      const int cur = sc->itc[(A.it + 1) & 1] + 1;
+      const double res = sqrt(rho / sc->bnorm2);
+      const bool conv = res <= sc->tol;
+      const bool stop = conv || cur >= sc->maxit;
+      if (blockIdx.x == 0 && threadIdx.x == 0) {
+        sc->itc[A.it & 1] = cur;
+        sc->iters = cur;
+        sc->resid = res;
+        if (conv) sc->done = 1;
+        else if (stop) sc->done = 2;
+      }
+      if (stop) return;
+    }
+  }
+  int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n < A.C.nx * A.C.ny) mg_down_cell(A.F, A.C, n, A.omega);
+}
+
+// One thread per fine cell: prolongate the coarse correction, post-smooth.
+// Level 0 writes u = B r and the partials of gamma = r.u.
+__global__ void __launch_bounds__(256) k_mg_up(MgArgs A) {
+  __shared__ double s_red[8 * kNW];
+  if (A.sc->done) return;
+  int n = blockIdx.x * blockDim.x + threadIdx.x;
+  double acc[3] = {0.0, 0.0, 0.0};
+  if (n < A.F.nx * A.F.ny) acc[0] = mg_up_cell(A.F, A.C, n, A.omega, A.oc);
+  if (A.level0) write_partials3(acc, A.part_out, s_red);
+}
+
+// Coarsest level (<= 64 cells): its operator is inverted once per solve
+// (Gauss-Jordan in LDS, SPD so no pivoting) and each cycle applies the dense
+// inverse, stored column-major so that lane t reads row t coalesced.
+constexpr int kMgBottomMax = 64;
+__global__ void __launch_bounds__(64) k_mg_factor(MgArgs A, double* inv) {
+  __shared__ double s_a[kMgBottomMax][kMgBottomMax + 1];
+  const MgLevel& L = A.F;
+  const int n = L.nx * L.ny, nx = L.nx, t = threadIdx.x;
+  if (t < n) {
+    for (int k = 0; k < n; k++) s_a[t][k] = 0.0;
+    size_t m = mg_at(L, t % nx, t / nx);
+    s_a[t][t] = L.d[m];
+    if (t % nx > 0) s_a[t][t - 1] = -L.cx[m - 1];
+    if (t % nx + 1 < nx) s_a[t][t + 1] = -L.cx[m];
+    if (t >= nx) s_a[t][t - nx] = -L.cy[m - L.P];
+    if (t + nx < n) s_a[t][t + nx] = -L.cy[m];
+  }
+  __syncthreads();
+  for (int p = 0; p < n; p++) {  // in-place Gauss-Jordan inversion, thread t owns row t
+    __shared__ double s_prow[kMgBottomMax];
+    if (t == p) {
+      double piv = s_a[p][p];
+      s_a[p][p] = 1.0;
+      for (int k = 0; k < n; k++) s_a[p][k] = s_a[p][k] / piv;
+    }
+    __syncthreads();
+    if (t < n) s_prow[t] = s_a[p][t];
+    __syncthreads();
+    if (t < n && t != p) {
+      double f = s_a[t][p];
+      s_a[t][p] = 0.0;
+      for (int k = 0; k < n; k++) s_a[t][k] = s_a[t][k] - f * s_prow[k];
+    }
+    __syncthreads();
+  }
+  if (t < n)
+    for (int k = 0; k < n; k++) inv[(size_t)k * n + t] = s_a[t][k];
+}
+
+__global__ void __launch_bounds__(64) k_mg_bottom(MgArgs A, const double* inv) {
+  __shared__ double s_r[kMgBottomMax];
+  if (A.sc->done) return;
+  const MgLevel& L = A.F;
+  const int n = L.nx * L.ny, nx = L.nx, t = threadIdx.x;
+  size_t m = t < n ? mg_at(L, t % nx, t / nx) : 0;
+  s_r[t] = t < n ? L.r[m] : 0.0;
+  __syncthreads();
+  if (t < n) {
+    double y = 0.0;
+    for (int k = 0; k < n; k++) y += inv[(size_t)k * n + t] * s_r[k];
+    L.e[m] = y;
+  }
+}
+
+// The levels from `first` (<= 64x64 cells) to the coarsest, in one workgroup
+// of 1024 threads: down sweeps, the dense bottom solve, up sweeps, separated
+// by workgroup barriers (no cross-workgroup synchronisation is involved).
+struct MgCycleArgs {
+  MgLevel lv[kMgMaxLevels];
+  int32_t first, nlev;
+  const double* inv;
+  DevScalars* sc;
+  double omega, oc;
+};
+
+__global__ void __launch_bounds__(1024) k_mg_coarse_cycle(MgCycleArgs A) {
+  __shared__ double s_r[kMgBottomMax];
+  if (A.sc->done) return;
+  const int t = threadIdx.x, nt = blockDim.x;
+  for (int l = A.first; l + 1 < A.nlev; l++) {
+    const MgLevel &F = A.lv[l], &C = A.lv[l + 1];
+    for (int n = t; n < C.nx * C.ny; n += nt) mg_down_cell(F, C, n, A.omega);
+    __syncthreads();
+  }
+  {
+    const MgLevel& L = A.lv[A.nlev - 1];
+    const int n = L.nx * L.ny, nx = L.nx;
+    if (t < kMgBottomMax) s_r[t] = t < n ? L.r[mg_at(L, t % nx, t / nx)] : 0.0;
+    __syncthreads();
+    if (t < n) {
+      double y = 0.0;
+      for (int k = 0; k < n; k++) y += A.inv[(size_t)k * n + t] * s_r[k];
+      L.e[mg_at(L, t % nx, t / nx)] = y;
+    }
+    __syncthreads();
+  }
+  for (int l = A.nlev - 2; l >= A.first; l--) {
+    const MgLevel &F = A.lv[l], &C = A.lv[l + 1];
+    for (int n = t; n < F.nx * F.ny; n += nt) mg_up_cell(F, C, n, A.omega, A.oc);
+    __syncthreads();
+  }
+}
+
+// MG-PCG: x = 0, r = rhs (from the assembly), p = 0, partials of |rhs/V|^2.
+__global__ void __launch_bounds__(kNT) k_mgpcg_init(MgPcgArgs A) {
+  __shared__ double s_red[8 * kNW];
+  const Geo& G = A.G;
+  const int t = xcd_tile(blockIdx.x, gridDim.x);
+  double acc[3] = {0.0, 0.0, 0.0};
+  for (int k = 0; k < kPcgRows; k++) {
+    const int ci = (t % A.nbx) * kTX + (int)(threadIdx.x % kTX);
+    const int cj = (t / A.nbx) * kPcgTY + (int)(threadIdx.x / kTX) + k * (kPcgTY / kPcgRows);
+    if (ci < G.bx && cj < G.by) {
+      size_t m = (size_t)(cj + 1) * G.P + (ci + 1);
+      A.x[m] = 0.0;
+      A.p_new[m] = 0.0;
+      double V = G.dlx[ci] * G.dly[cj] * (G.cyl ? G.r[cj + 1] : 1.0);
+      double rv = A.r[m] / V;
+      acc[0] += rv * rv;
+    }
+  }
+  write_partials3(acc, A.part_out, s_red);
+}
+
+// p = u + beta p_old (neighbours recomputed), q = M p, partials of p.q
+__global__ void __launch_bounds__(kNT) k_mgpcg_pq(MgPcgArgs A) {
+  __shared__ double s_red[8 * kNW];
+  DevScalars* sc = A.sc;
+  if (sc->done) return;
+  double tot[3];
+  reduce_partials3(A.part_in, A.nb_in, s_red, tot);
+  const double gam = tot[0];
+  const double beta = A.first ? 0.0 : gam / sc->cg_gam[(A.it + 1) & 1];
+  if (blockIdx.x == 0 && threadIdx.x == 0) sc->cg_gam[A.it & 1] = gam;
+  const Geo& G = A.G;
+  const int P = G.P;
+  const int t = xcd_tile(blockIdx.x, gridDim.x);
+  double acc[3] = {0.0, 0.0, 0.0};
+  const double *u = A.u, *po = A.p_old;
+  for (int k = 0; k < kPcgRows; k++) {
+    const int ci = (t % A.nbx) * kTX + (int)(threadIdx.x % kTX);
+    const int cj = (t / A.nbx) * kPcgTY + (int)(threadIdx.x / kTX) + k * (kPcgTY / kPcgRows);
+    if (ci < G.bx && cj < G.by) {
+      size_t m = (size_t)(cj + 1) * P + (ci + 1);
+      auto pv = [&](size_t q) { return u[q] + beta * po[q]; };
+      double pc = pv(m);
+      double q = A.dg[m] * pc - A.cx[m - 1] * pv(m - 1) - A.cx[m] * pv(m + 1) -
+                 A.cy[m - P] * pv(m - P) - A.cy[m] * pv(m + P);
+      A.p_new[m] = pc;
+      A.q[m] = q;
+      acc[0] += pc * q;
+    }
+  }
+  write_partials3(acc, A.part_out, s_red);
+}
+
+// alpha = gamma / p.q; x += alpha p; r -= alpha q; partials of |r/V|^2
+__global__ void __launch_bounds__(kNT) k_mgpcg_xr(MgPcgArgs A) {
+  __shared__ double s_red[8 * kNW];
+  DevScalars* sc = A.sc;
+  if (sc->done) return;
+  double tot[3];
+  reduce_partials3(A.part_in, A.nb_in, s_red, tot);
+  const double alpha = sc->cg_gam[A.it & 1] / tot[0];
+  const Geo& G = A.G;
+  const int t = xcd_tile(blockIdx.x, gridDim.x);
+  double acc[3] = {0.0, 0.0, 0.0};
+  for (int k = 0; k < kPcgRows; k++) {
+    const int ci = (t % A.nbx) * kTX + (int)(threadIdx.x % kTX);
+    const int cj = (t / A.nbx) * kPcgTY + (int)(threadIdx.x / kTX) + k * (kPcgTY / kPcgRows);
+    if (ci < G.bx && cj < G.by) {
+      size_t m = (size_t)(cj + 1) * G.P + (ci + 1);
+      A.x[m] = A.x[m] + alpha * A.p_new[m];
+      double r = A.r[m] - alpha * A.q[m];
+      A.r[m] = r;
+      double V = G.dlx[ci] * G.dly[cj] * (G.cyl ? G.r[cj + 1] : 1.0);
+      double rv = r / V;
+      acc[0] += rv * rv;
+    }
+  }
+  write_partials3(acc, A.part_out, s_red);
+}
+
 __global__ void __launch_bounds__(256) k_copy(const double2* __restrict__ a,
                                              double2* __restrict__ b, size_t n) {
   size_t stride = (size_t)gridDim.x * blockDim.x;
@@ -1065,6 +1391,45 @@ void launch_pcg_it(const PcgArgs& a, hipStream_t s) {
 void launch_pcg_finish(const PcgArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(k_pcg_finish, dim3(a.nbx * a.nby), dim3(kNT), 0, s, a);
   hipLaunchKernelGGL(k_pcg_tres, dim3(1), dim3(kNT), 0, s, a, a.nbx * a.nby);
+}
+static int cells_blocks(int n) { return (n + 255) / 256; }
+void launch_mg_coarsen(const MgArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(k_mg_coarsen, dim3(cells_blocks(a.C.nx * a.C.ny)), dim3(256), 0, s, a);
+}
+void launch_mg_down(const MgArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(k_mg_down, dim3(cells_blocks(a.C.nx * a.C.ny)), dim3(256), 0, s, a);
+}
+int mg_up_blocks(const MgLevel& F) { return cells_blocks(F.nx * F.ny); }
+void launch_mg_up(const MgArgs& a, int nb, hipStream_t s) {
+  hipLaunchKernelGGL(k_mg_up, dim3(nb), dim3(256), 0, s, a);
+}
+void launch_mg_factor(const MgArgs& a, double* inv, hipStream_t s) {
+  hipLaunchKernelGGL(k_mg_factor, dim3(1), dim3(64), 0, s, a, inv);
+}
+void launch_mg_bottom(const MgArgs& a, const double* inv, hipStream_t s) {
+  hipLaunchKernelGGL(k_mg_bottom, dim3(1), dim3(64), 0, s, a, inv);
+}
+void launch_mg_coarse_cycle(const MgLevel* lv, int nlev, int first, const double* inv,
+                            DevScalars* sc, double omega, double oc, hipStream_t s) {
+  MgCycleArgs a;
+  memset(&a, 0, sizeof(a));
+  for (int l = 0; l < nlev && l < kMgMaxLevels; l++) a.lv[l] = lv[l];
+  a.first = first;
+  a.nlev = nlev;
+  a.inv = inv;
+  a.sc = sc;
+  a.omega = omega;
+  a.oc = oc;
+  hipLaunchKernelGGL(k_mg_coarse_cycle, dim3(1), dim3(1024), 0, s, a);
+}
+void launch_mgpcg_init(const MgPcgArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(k_mgpcg_init, dim3(a.nbx * a.nby), dim3(kNT), 0, s, a);
+}
+void launch_mgpcg_pq(const MgPcgArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(k_mgpcg_pq, dim3(a.nbx * a.nby), dim3(kNT), 0, s, a);
+}
+void launch_mgpcg_xr(const MgPcgArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(k_mgpcg_xr, dim3(a.nbx * a.nby), dim3(kNT), 0, s, a);
 }
 void launch_copy(const double* a, double* b, size_t n2, hipStream_t s) {
   hipLaunchKernelGGL(k_copy, dim3(256 * 16), dim3(256), 0, s, (const double2*)a, (double2*)b,

@@ -144,6 +144,7 @@ contains
         case ('-seed'); read(val,*) seed
         case ('-steps'); read(val,*) max_steps
         case ('-rtol'); read(val,*) lapl_rtol
+        case ('-solver'); read(val,*) lapl_solver
         case ('-out'); path = val
         case ('-verbose'); read(val,*) verbose
         case ('-trise'); read(val,*) t_rise

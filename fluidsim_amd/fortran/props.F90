@@ -26,7 +26,7 @@ module props
   real(8) :: p = 10d0, gam = 0.1d0, res_ohm = 1e4, ratio = 1.05d0
   real(8) :: t_rise = 0.01d0, t_width = 0.1d0, t_fall = 0.01d0
   real(8) :: lapl_rtol = 1d-10
-  integer :: rf = 0, circuit = 0, grid_x = 0, grid_y = 0, lapl_maxit = 20000
+  integer :: rf = 0, circuit = 0, grid_x = 0, grid_y = 0, lapl_maxit = 20000, lapl_solver = 1
   logical :: unif = .True., cyl = .False., rwall = .False.
 
 contains
@@ -71,7 +71,8 @@ contains
     pr%rf = rf; pr%circuit = circuit
     pr%t_rise = t_rise; pr%t_width = t_width; pr%t_fall = t_fall
     pr%dt0 = dt0; pr%lapl_rtol = lapl_rtol; pr%lapl_maxit = lapl_maxit
-    pr%lapl_chunk = 32; pr%debug_keep_k = 0; pr%pad_ = 0
+    pr%lapl_chunk = 32; pr%debug_keep_k = 0; pr%lapl_solver = lapl_solver
+    pr%use_graphs = 1; pr%pad_ = 0
 
     rc = fs_create(gd, pr, g%h)
     call fs_check(rc, 'fs_create')

@@ -74,6 +74,8 @@ typedef struct fs_params {
   int32_t lapl_maxit;
   int32_t lapl_chunk;     /* PCG iterations launched between convergence polls */
   int32_t debug_keep_k;   /* keep all five stage slopes for fs_get_field(FS_K*) */
+  int32_t lapl_solver;    /* 0 Jacobi-PCG, 1 multigrid-preconditioned PCG */
+  int32_t use_graphs;     /* replay the PCG iterations from a captured HIP graph */
   int32_t pad_;
 } fs_params;
 
@@ -132,8 +134,9 @@ int fs_set_scalars(fs_handle* h, const double* in11);
 int fs_ptcl_step(fs_handle* h, fs_step_info* info);
 
 /* lapl_solve(g, ne1, ni1, nt1, sig, lerr) (lapl_lib.F90:57-121): assemble the
- * semi-implicit Newton system, solve it with matrix-free Jacobi-PCG on the
- * volume-symmetrised operator, and add the correction to ph.  *lerr = 1 when
+ * semi-implicit Newton system, solve it with matrix-free PCG on the
+ * volume-symmetrised operator (Jacobi or aggregation-multigrid V-cycle
+ * preconditioner, lapl_solver), and add the correction to ph.  *lerr = 1 when
  * the solve did not reach lapl_rtol within lapl_maxit (ph unchanged). */
 int fs_lapl_solve(fs_handle* h, int32_t* lerr, fs_solve_stats* stats);
 

@@ -135,12 +135,14 @@ def test_one_step_parity(O):
     assert max(errs.values()) <= 1e-10, errs
 
 
-def test_trajectory_config0_replayed(O):
+@pytest.mark.parametrize("solver", [0, 1])
+def test_trajectory_config0_replayed(O, solver):
     """Config-0 (64x64) trajectory with the oracle's dt sequence replayed:
-    relative L2 per field after 200 main-loop passes."""
+    relative L2 per field after 200 main-loop passes (Jacobi-PCG and
+    multigrid-preconditioned PCG)."""
     from fluidsim_amd import configs
 
-    cfg = configs.config0(rtol=1e-13)
+    cfg = configs.config0(rtol=1e-13, solver=solver)
     d, o = _pair(O, cfg, seed=0)
     o.run(200)
     d.set_replay(o.log())
