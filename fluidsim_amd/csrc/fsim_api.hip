@@ -148,6 +148,7 @@ struct fs_handle {
   std::vector<MgLevel> mg;          // multigrid hierarchy; level 0 aliases the PCG planes
   double* d_mg = nullptr;
   double* d_mginv = nullptr;        // dense inverse of the coarsest operator (<= 64 cells)
+  size_t mg_first = 0;              // first level of the one-workgroup coarse kernel
   hipGraphExec_t mg_graph = nullptr;  // two MG-PCG iterations (parity 1, then 0)
   Geo G;
   std::vector<double> log;
@@ -171,7 +172,6 @@ enum { PO_RHS0 = 0, PO_R0 = 1, PO_CX = 2, PO_CY = 3, PO_DG = 4, PO_X = 5, PO_P =
        PO_W0 = 8, PO_W1 = 9, PO_S0 = 10, PO_S1 = 11,
        // multigrid-preconditioned PCG: p (double-buffered), q, u = B r, pre-smoothed x1
        PO_MP0 = 12, PO_MP1 = 13, PO_MQ = 14, PO_MU = 15, PO_MX1 = 16 };
-constexpr double kMgOmega = 0.8;   // Jacobi damping of the smoother
 constexpr double kMgOver = 1.8;    // scale of the piecewise-constant coarse correction
 
 namespace {
@@ -326,6 +326,8 @@ int check_status(fs_handle* h) {
 // Aggregation hierarchy: halve both axes until the coarsest level has at most
 // 64 cells (solved with its dense inverse).  Coarse planes: d (ghost 1), cx,
 // cy, r, x1, e (ghost 0).
+size_t mg_lds_first(const fs_handle* h);
+
 int build_mg(fs_handle* h) {
   MgLevel L0;
   L0.d = h->pois(PO_DG);
@@ -373,6 +375,7 @@ int build_mg(fs_handle* h) {
     launch_fill(L.d, 1.0, pl, h->st);
     h->mg.push_back(L);
   }
+  h->mg_first = mg_lds_first(h);
   return 0;
 }
 
@@ -383,7 +386,6 @@ MgArgs mg_args(fs_handle* h, size_t l) {
   if (l + 1 < h->mg.size()) a.C = h->mg[l + 1];
   a.sc = h->d_sc;
   a.G = h->G;
-  a.omega = kMgOmega;
   a.oc = kMgOver;
   a.level0 = l == 0;
   return a;
@@ -410,13 +412,25 @@ MgPcgArgs mgpcg_args(fs_handle* h, int it) {
   return a;
 }
 
+// The first level run by the one-workgroup kernel: at most kMgLdsCells cells,
+// at most kMgLdsLevels levels from there down, padded iterates within kMgLdsX
+// doubles; never level 0 (its kernels carry the PCG reductions).
+size_t mg_lds_first(const fs_handle* h) {
+  const size_t L = h->mg.size();
+  for (size_t f = 1; f + 1 < L; f++) {
+    size_t x = 0;
+    for (size_t q = f; q < L; q++) x += (size_t)(h->mg[q].nx + 2) * (h->mg[q].ny + 2);
+    if ((size_t)h->mg[f].nx * h->mg[f].ny <= (size_t)kMgLdsCells && L - f <= (size_t)kMgLdsLevels &&
+        x <= (size_t)kMgLdsX)
+      return f;
+  }
+  return L - 1;
+}
+
 // Partial-sum buffers of the MG-PCG chain: 0 = |r/V|^2, 1 = r.u, 2 = p.q
 void launch_vcycle(fs_handle* h, int it, bool init, int nb_rho) {
-  const size_t L = h->mg.size();
-  // levels of <= 64x64 cells (and never level 0) run in one workgroup
-  size_t first = 1;
-  while (first + 1 < L && (size_t)h->mg[first].nx * h->mg[first].ny > 4096) first++;
-  for (size_t l = 0; l < first; l++) {
+  const size_t L = h->mg.size(), f = h->mg_first;
+  for (size_t l = 0; l < f; l++) {
     MgArgs a = mg_args(h, l);
     a.it = it;
     a.init = init;
@@ -424,15 +438,23 @@ void launch_vcycle(fs_handle* h, int it, bool init, int nb_rho) {
     a.nb_in = nb_rho;
     launch_mg_down(a, h->st);
   }
-  if (first + 1 < L)
-    launch_mg_coarse_cycle(h->mg.data(), (int)L, (int)first, h->d_mginv, h->d_sc, kMgOmega,
-                           kMgOver, h->st);
-  else
-    launch_mg_bottom(mg_args(h, L - 1), h->d_mginv, h->st);
-  for (size_t l = first; l-- > 0;) {
+  MgCycleArgs c;
+  memset(&c, 0, sizeof(c));
+  c.nq = (int)(L - f);
+  int off = 0;
+  for (int q = 0; q < c.nq; q++) {
+    c.lv[q] = h->mg[f + q];
+    c.off[q] = off;
+    off += (c.lv[q].nx + 2) * (c.lv[q].ny + 2);
+  }
+  c.inv = h->d_mginv;
+  c.sc = h->d_sc;
+  c.oc = kMgOver;
+  launch_mg_coarse_cycle(c, h->st);
+  for (size_t l = f; l-- > 0;) {
     MgArgs a = mg_args(h, l);
     a.part_out = h->part(1);
-    launch_mg_up(a, mg_up_blocks(a.F), h->st);
+    launch_mg_up(a, h->st);
   }
 }
 
@@ -442,7 +464,7 @@ void launch_mg_iteration(fs_handle* h, int it) {
   const int nb_pcg = h->nbx * h->pnby;
   MgPcgArgs a = mgpcg_args(h, it);
   a.part_in = h->part(1);
-  a.nb_in = mg_up_blocks(h->mg[0]);
+  a.nb_in = mg_tiles(h->mg[0]);
   a.part_out = h->part(2);
   launch_mgpcg_pq(a, h->st);
   a.part_in = h->part(2);

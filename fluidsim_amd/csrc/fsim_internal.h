@@ -131,8 +131,24 @@ struct MgArgs {
   double* part_out;      // level-0 up: partials of gamma = r.u
   DevScalars* sc;
   Geo G;                 // level-0 metrics (volumes for the residual norm)
-  double omega, oc;
+  double oc;
   int32_t it, level0, init, pad_;
+};
+
+// most levels a hierarchy may have
+constexpr int kMgMaxLevels = 16;
+constexpr int kMgBottomMax = 64;   // cells of the coarsest level (dense inverse)
+constexpr int kMgLdsLevels = 4;    // levels run by the one-workgroup kernel
+constexpr int kMgLdsCells = 1024;  // cells of the finest of those
+constexpr int kMgLdsX = 4096;      // LDS doubles for their padded iterates
+
+struct MgCycleArgs {
+  MgLevel lv[kMgLdsLevels];  // levels f .. L-1, the last one the coarsest
+  int32_t nq;                // how many of them
+  int32_t off[kMgLdsLevels]; // LDS offset of each level's (nx+2)(ny+2) iterate
+  const double* inv;         // dense inverse of the coarsest operator
+  DevScalars* sc;
+  double oc;
 };
 
 struct MgPcgArgs {
@@ -170,18 +186,14 @@ void launch_pcg_init(const PcgArgs& a, hipStream_t s);
 void launch_pcg_it(const PcgArgs& a, hipStream_t s);
 void launch_pcg_finish(const PcgArgs& a, hipStream_t s);
 void launch_mg_coarsen(const MgArgs& a, hipStream_t s);
+int mg_tiles(const MgLevel& F);  // tiles (= blocks, = partials) of a down/up launch
 void launch_mg_down(const MgArgs& a, hipStream_t s);
-void launch_mg_up(const MgArgs& a, int nb, hipStream_t s);
+void launch_mg_up(const MgArgs& a, hipStream_t s);
 void launch_mg_factor(const MgArgs& a, double* inv, hipStream_t s);
-// most levels a hierarchy may have (the coarse-cycle kernel takes them by value)
-constexpr int kMgMaxLevels = 16;
-void launch_mg_coarse_cycle(const MgLevel* lv, int nlev, int first, const double* inv,
-                            DevScalars* sc, double omega, double oc, hipStream_t s);
-void launch_mg_bottom(const MgArgs& a, const double* inv, hipStream_t s);
+void launch_mg_coarse_cycle(const MgCycleArgs& a, hipStream_t s);
 void launch_mgpcg_init(const MgPcgArgs& a, hipStream_t s);
 void launch_mgpcg_pq(const MgPcgArgs& a, hipStream_t s);
 void launch_mgpcg_xr(const MgPcgArgs& a, hipStream_t s);
-int mg_up_blocks(const MgLevel& F);
 void launch_fill(double* p, double v, size_t n, hipStream_t s);
 void launch_copy(const double* a, double* b, size_t n2, hipStream_t s);
 void launch_eval_fits(const FsPhys& K, const double* T, double* out, int n, hipStream_t s);

@@ -989,64 +989,26 @@ __global__ void __launch_bounds__(kNT) k_pcg_tres(PcgArgs A, int nb) {
 }
 
 // ---------------------------------------------------------------------------
-// Aggregation multigrid V(1,1)-cycle used as the PCG preconditioner.
+// Aggregation multigrid V(2,2)-cycle used as the PCG preconditioner.
 //
 // Level l+1 aggregates 2x2 cells of level l.  With piecewise-constant
 // prolongation P and restriction P^T the Galerkin operator P^T M P keeps the
 // 5-point face storage: a coarse face coupling is the sum of the fine faces
 // that cross it; the coarse diagonal is the sum of the four fine diagonals
-// minus twice the four internal faces.  One cycle, from a zero guess:
-//   x1 = w D^-1 r,  r1 = r - M x1  (restricted: r_c = P^T r1)      [down]
-//   x2 = x1 + oc P e_c,  e = x2 + w D^-1 (r - M x2)                  [up]
-// which is symmetric positive definite for w*lambda_max(D^-1 M) < 2, so plain
-// CG remains valid.  The neighbour values of x1 and x2 that the stencils need
-// are recomputed (x1 = w r/d is pointwise, x2 = x1 + oc*e_c), so each level
-// costs one down and one up kernel.  The coarsest level (<= 8x8 cells) is
-// smoothed to convergence inside one workgroup.
+// minus twice the four internal faces.  One cycle on a level, from a zero
+// guess, with S = two red-black Gauss-Seidel sweeps (red, black, red, black)
+// and S^T the same sweeps in reverse colour order:
+//   x1 = S r,  r_c = P^T (r - M x1)                                  [down]
+//   x2 = x1 + oc P e_c,  e = x2 + S^T (r - M x2)                     [up]
+// The post-smoother is the adjoint of the pre-smoother, so the cycle is a
+// symmetric positive definite operator and plain CG remains valid.  Levels
+// above 1024 cells are swept in tiles staged in LDS with a halo of four cells
+// (one per half-sweep), one down and one up kernel per level; the remaining
+// levels run inside one workgroup with their iterates in LDS, and the
+// coarsest (<= 64 cells) is solved with a dense inverse.
 
 __device__ __forceinline__ size_t mg_at(const MgLevel& L, int i, int j) {
   return (size_t)(j + 1) * L.P + (i + 1);
-}
-
-// Down step for coarse cell n of C: pre-smooth its (up to) 2x2 fine cells,
-// keep x1, restrict the residual.  The neighbours' x1 = w r/d is recomputed.
-__device__ __forceinline__ void mg_down_cell(const MgLevel& F, const MgLevel& C, int n,
-                                             double w) {
-  int I = n % C.nx, J = n / C.nx;
-  const double *r = F.r, *d = F.d;
-  auto x1 = [&](size_t q) { return w * r[q] / d[q]; };
-  double rc = 0.0;
-  for (int b = 0; b < 2; b++)
-    for (int a = 0; a < 2; a++) {
-      int i = 2 * I + a, j = 2 * J + b;
-      if (i >= F.nx || j >= F.ny) continue;
-      size_t m = mg_at(F, i, j), P = F.P;
-      double xc = x1(m);
-      double mx = d[m] * xc - F.cx[m - 1] * x1(m - 1) - F.cx[m] * x1(m + 1) -
-                  F.cy[m - P] * x1(m - P) - F.cy[m] * x1(m + P);
-      F.x1[m] = xc;
-      rc += r[m] - mx;
-    }
-  C.r[mg_at(C, I, J)] = rc;
-}
-
-// Up step for fine cell n of F: x2 = x1 + oc * e_c(aggregate), one
-// post-smoothing sweep; returns r.e (level 0 sums it into gamma).
-__device__ __forceinline__ double mg_up_cell(const MgLevel& F, const MgLevel& C, int n, double w,
-                                             double oc) {
-  int i = n % F.nx, j = n / F.nx;
-  const double* ec = C.e;
-  size_t m = mg_at(F, i, j), P = F.P;
-  // cells outside the level carry zero couplings: their value only has to be finite
-  auto x2 = [&](size_t q, int ii, int jj) {
-    return F.x1[q] + oc * ec[mg_at(C, ii >> 1, jj >> 1)];
-  };
-  double xc = x2(m, i, j);
-  double mx = F.d[m] * xc - F.cx[m - 1] * x2(m - 1, i - 1, j) - F.cx[m] * x2(m + 1, i + 1, j) -
-              F.cy[m - P] * x2(m - P, i, j - 1) - F.cy[m] * x2(m + P, i, j + 1);
-  double e = xc + w * (F.r[m] - mx) / F.d[m];
-  F.e[m] = e;
-  return F.r[m] * e;
 }
 
 __global__ void __launch_bounds__(256) k_mg_coarsen(MgArgs A) {
@@ -1087,10 +1049,37 @@ __global__ void __launch_bounds__(256) k_mg_coarsen(MgArgs A) {
   C.cy[m] = cy;
 }
 
-// One thread per coarse cell: pre-smooth and residual of its (up to) 2x2
-// fine cells, restricted sum into the coarse residual.  Level 0 first checks
-// convergence from the x/r-update partials (every block identically).
-__global__ void __launch_bounds__(256) k_mg_down(MgArgs A) {
+constexpr int kGsTX = 32, kGsTY = 16, kGsH = 4;  // tile and halo (four half-sweeps)
+constexpr int kGsW = kGsTX + 2 * kGsH, kGsR = kGsTY + 2 * kGsH;
+static_assert((kGsTX / 2) * kGsTY == kNT, "one thread per horizontal cell pair");
+
+// One Gauss-Seidel half-sweep over the cells of colour c ((i + j) & 1 == c)
+// within h <= kGsH - 1 cells of the tile at (i0, j0).  Cells of the other
+// colour are read from LDS; cells outside the level stay zero.
+__device__ __forceinline__ void gs_half(const MgLevel& F, double (*sx)[kGsW], int i0, int j0,
+                                        int c, int h) {
+  const int w2 = (kGsTX + 2 * h) / 2, rows = kGsTY + 2 * h, a0 = kGsH - h;
+  for (int k = threadIdx.x; k < rows * w2; k += kNT) {
+    const int b = a0 + k / w2;
+    const int j = j0 - kGsH + b;
+    const int a = a0 + 2 * (k % w2) + ((c + i0 - kGsH + a0 + j) & 1);
+    const int i = i0 - kGsH + a;
+    if (i < 0 || i >= F.nx || j < 0 || j >= F.ny) continue;
+    const size_t m = mg_at(F, i, j), P = F.P;
+    sx[b][a] = (F.r[m] + F.cx[m - 1] * sx[b][a - 1] + F.cx[m] * sx[b][a + 1] +
+                F.cy[m - P] * sx[b - 1][a] + F.cy[m] * sx[b + 1][a]) /
+               F.d[m];
+  }
+}
+
+__device__ __forceinline__ int gs_tiles_x(const MgLevel& F) { return (F.nx + kGsTX - 1) / kGsTX; }
+
+// Down: pre-smooth M x = r on the tile, keep x1, restrict the residual into
+// the coarse level.  Level 0 first checks convergence from the x/r-update
+// partials (every block identically).
+__global__ void __launch_bounds__(kNT) k_mg_down(MgArgs A) {
+  __shared__ double s_x[kGsR][kGsW];
+  __shared__ double s_res[kGsTY][kGsTX / 2];
   __shared__ double s_red[8 * kNW];
   DevScalars* sc = A.sc;
   if (sc->done) return;
@@ -1125,31 +1114,105 @@ __global__ void __launch_bounds__(256) k_mg_down(MgArgs A) {
       if (stop) return;
     }
   }
-  int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n < A.C.nx * A.C.ny) mg_down_cell(A.F, A.C, n, A.omega);
+  const MgLevel &F = A.F, &C = A.C;
+  const int ntx = gs_tiles_x(F);
+  const int t = xcd_tile(blockIdx.x, gridDim.x);
+  const int i0 = (t % ntx) * kGsTX, j0 = (t / ntx) * kGsTY;
+  // first (red) half-sweep from zero is pointwise: stage the whole halo
+  for (int k = threadIdx.x; k < kGsR * kGsW; k += kNT) {
+    const int b = k / kGsW, a = k % kGsW;
+    const int i = i0 - kGsH + a, j = j0 - kGsH + b;
+    double v = 0.0;
+    if (i >= 0 && i < F.nx && j >= 0 && j < F.ny && ((i + j) & 1) == 0) {
+      const size_t m = mg_at(F, i, j);
+      v = F.r[m] / F.d[m];
+    }
+    s_x[b][a] = v;
+  }
+  __syncthreads();
+  gs_half(F, s_x, i0, j0, 1, 3);
+  __syncthreads();
+  gs_half(F, s_x, i0, j0, 0, 2);
+  __syncthreads();
+  gs_half(F, s_x, i0, j0, 1, 1);
+  __syncthreads();
+  // residual of one horizontal cell pair (half an aggregate) per thread
+  const int cI = threadIdx.x % (kGsTX / 2), fj = threadIdx.x / (kGsTX / 2);
+  double rs = 0.0;
+  for (int q = 0; q < 2; q++) {
+    const int a = kGsH + 2 * cI + q, b = kGsH + fj;
+    const int i = i0 + 2 * cI + q, j = j0 + fj;
+    if (i < F.nx && j < F.ny) {
+      const size_t m = mg_at(F, i, j), P = F.P;
+      const double xc = s_x[b][a];
+      const double mx = F.d[m] * xc - F.cx[m - 1] * s_x[b][a - 1] - F.cx[m] * s_x[b][a + 1] -
+                        F.cy[m - P] * s_x[b - 1][a] - F.cy[m] * s_x[b + 1][a];
+      F.x1[m] = xc;
+      rs += F.r[m] - mx;
+    }
+  }
+  s_res[fj][cI] = rs;
+  __syncthreads();
+  if ((fj & 1) == 0) {
+    const int I = i0 / 2 + cI, J = (j0 + fj) / 2;
+    if (I < C.nx && J < C.ny) C.r[mg_at(C, I, J)] = s_res[fj][cI] + s_res[fj + 1][cI];
+  }
 }
 
-// One thread per fine cell: prolongate the coarse correction, post-smooth.
+// Up: x2 = x1 + oc * e_c(aggregate) on the staged halo, post-smooth, write e.
 // Level 0 writes u = B r and the partials of gamma = r.u.
-__global__ void __launch_bounds__(256) k_mg_up(MgArgs A) {
+__global__ void __launch_bounds__(kNT) k_mg_up(MgArgs A) {
+  __shared__ double s_x[kGsR][kGsW];
   __shared__ double s_red[8 * kNW];
   if (A.sc->done) return;
-  int n = blockIdx.x * blockDim.x + threadIdx.x;
+  const MgLevel &F = A.F, &C = A.C;
+  const int ntx = gs_tiles_x(F);
+  const int t = xcd_tile(blockIdx.x, gridDim.x);
+  const int i0 = (t % ntx) * kGsTX, j0 = (t / ntx) * kGsTY;
+  for (int k = threadIdx.x; k < kGsR * kGsW; k += kNT) {
+    const int b = k / kGsW, a = k % kGsW;
+    const int i = i0 - kGsH + a, j = j0 - kGsH + b;
+    double v = 0.0;
+    if (i >= 0 && i < F.nx && j >= 0 && j < F.ny)
+      v = F.x1[mg_at(F, i, j)] + A.oc * C.e[mg_at(C, i >> 1, j >> 1)];
+    s_x[b][a] = v;
+  }
+  __syncthreads();
+  gs_half(F, s_x, i0, j0, 1, 3);
+  __syncthreads();
+  gs_half(F, s_x, i0, j0, 0, 2);
+  __syncthreads();
+  gs_half(F, s_x, i0, j0, 1, 1);
+  __syncthreads();
+  gs_half(F, s_x, i0, j0, 0, 0);
+  __syncthreads();
   double acc[3] = {0.0, 0.0, 0.0};
-  if (n < A.F.nx * A.F.ny) acc[0] = mg_up_cell(A.F, A.C, n, A.omega, A.oc);
+  for (int k = threadIdx.x; k < kGsTX * kGsTY; k += kNT) {
+    const int i = i0 + k % kGsTX, j = j0 + k / kGsTX;
+    if (i < F.nx && j < F.ny) {
+      const size_t m = mg_at(F, i, j);
+      const double e = s_x[kGsH + k / kGsTX][kGsH + k % kGsTX];
+      F.e[m] = e;
+      acc[0] += F.r[m] * e;
+    }
+  }
   if (A.level0) write_partials3(acc, A.part_out, s_red);
 }
 
 // Coarsest level (<= 64 cells): its operator is inverted once per solve
-// (Gauss-Jordan in LDS, SPD so no pivoting) and each cycle applies the dense
-// inverse, stored column-major so that lane t reads row t coalesced.
-constexpr int kMgBottomMax = 64;
-__global__ void __launch_bounds__(64) k_mg_factor(MgArgs A, double* inv) {
+// (Gauss-Jordan in LDS, SPD so no pivoting; 64 rows x 16 column groups of
+// threads) and each cycle applies the dense inverse, stored column-major so
+// that lane t reads row t coalesced.
+__global__ void __launch_bounds__(1024) k_mg_factor(MgArgs A, double* inv) {
   __shared__ double s_a[kMgBottomMax][kMgBottomMax + 1];
+  __shared__ double s_prow[kMgBottomMax];
+  constexpr int kGroups = 1024 / kMgBottomMax;
   const MgLevel& L = A.F;
   const int n = L.nx * L.ny, nx = L.nx, t = threadIdx.x;
+  const int row = t % kMgBottomMax, cg = t / kMgBottomMax;
+  for (int k = cg; k < kMgBottomMax; k += kGroups) s_a[row][k] = 0.0;
+  __syncthreads();
   if (t < n) {
-    for (int k = 0; k < n; k++) s_a[t][k] = 0.0;
     size_t m = mg_at(L, t % nx, t / nx);
     s_a[t][t] = L.d[m];
     if (t % nx > 0) s_a[t][t - 1] = -L.cx[m - 1];
@@ -1158,78 +1221,120 @@ __global__ void __launch_bounds__(64) k_mg_factor(MgArgs A, double* inv) {
     if (t + nx < n) s_a[t][t + nx] = -L.cy[m];
   }
   __syncthreads();
-  for (int p = 0; p < n; p++) {  // in-place Gauss-Jordan inversion, thread t owns row t
-    __shared__ double s_prow[kMgBottomMax];
-    if (t == p) {
-      double piv = s_a[p][p];
-      s_a[p][p] = 1.0;
-      for (int k = 0; k < n; k++) s_a[p][k] = s_a[p][k] / piv;
-    }
+  for (int p = 0; p < n; p++) {  // in-place Gauss-Jordan inversion
+    const double piv = s_a[p][p];
+    const double f = s_a[row][p];
+    if (t < n) s_prow[t] = (t == p ? 1.0 : s_a[p][t]) / piv;
     __syncthreads();
-    if (t < n) s_prow[t] = s_a[p][t];
-    __syncthreads();
-    if (t < n && t != p) {
-      double f = s_a[t][p];
-      s_a[t][p] = 0.0;
-      for (int k = 0; k < n; k++) s_a[t][k] = s_a[t][k] - f * s_prow[k];
-    }
+    if (row < n)
+      for (int k = cg; k < n; k += kGroups)
+        s_a[row][k] = row == p ? s_prow[k] : (k == p ? 0.0 : s_a[row][k]) - f * s_prow[k];
     __syncthreads();
   }
-  if (t < n)
-    for (int k = 0; k < n; k++) inv[(size_t)k * n + t] = s_a[t][k];
+  for (int e = t; e < n * n; e += 1024) inv[e] = s_a[e % n][e / n];
 }
 
-__global__ void __launch_bounds__(64) k_mg_bottom(MgArgs A, const double* inv) {
-  __shared__ double s_r[kMgBottomMax];
-  if (A.sc->done) return;
-  const MgLevel& L = A.F;
-  const int n = L.nx * L.ny, nx = L.nx, t = threadIdx.x;
-  size_t m = t < n ? mg_at(L, t % nx, t / nx) : 0;
-  s_r[t] = t < n ? L.r[m] : 0.0;
-  __syncthreads();
-  if (t < n) {
-    double y = 0.0;
-    for (int k = 0; k < n; k++) y += inv[(size_t)k * n + t] * s_r[k];
-    L.e[m] = y;
-  }
-}
-
-// The levels from `first` (<= 64x64 cells) to the coarsest, in one workgroup
-// of 1024 threads: down sweeps, the dense bottom solve, up sweeps, separated
-// by workgroup barriers (no cross-workgroup synchronisation is involved).
-struct MgCycleArgs {
-  MgLevel lv[kMgMaxLevels];
-  int32_t first, nlev;
-  const double* inv;
-  DevScalars* sc;
-  double omega, oc;
-};
-
+// Levels f .. L-1 (<= 1024 cells each, at most kMgLdsLevels) in one workgroup
+// of 1024 threads: thread t owns cell t of every level, keeps that cell's
+// operator row and right-hand side in registers, and the iterates live in LDS
+// with a zero ghost ring.  Same cycle as the tile kernels; level f's
+// correction is written back for the tile levels above.
 __global__ void __launch_bounds__(1024) k_mg_coarse_cycle(MgCycleArgs A) {
-  __shared__ double s_r[kMgBottomMax];
+  __shared__ double s_x[kMgLdsX];
+  __shared__ double s_r[1024];
   if (A.sc->done) return;
-  const int t = threadIdx.x, nt = blockDim.x;
-  for (int l = A.first; l + 1 < A.nlev; l++) {
-    const MgLevel &F = A.lv[l], &C = A.lv[l + 1];
-    for (int n = t; n < C.nx * C.ny; n += nt) mg_down_cell(F, C, n, A.omega);
+  const int t = threadIdx.x;
+  double cd[kMgLdsLevels], cl[kMgLdsLevels], cr[kMgLdsLevels], cb[kMgLdsLevels],
+      ct[kMgLdsLevels], rr[kMgLdsLevels];
+  for (int k = t; k < kMgLdsX; k += 1024) s_x[k] = 0.0;
+#pragma unroll
+  for (int q = 0; q < kMgLdsLevels; q++) {
+    cd[q] = 1.0;
+    cl[q] = cr[q] = cb[q] = ct[q] = rr[q] = 0.0;
+    if (q < A.nq) {
+      const MgLevel& L = A.lv[q];
+      if (t < L.nx * L.ny) {
+        const size_t m = mg_at(L, t % L.nx, t / L.nx), P = L.P;
+        cd[q] = L.d[m];
+        cl[q] = L.cx[m - 1];
+        cr[q] = L.cx[m];
+        cb[q] = L.cy[m - P];
+        ct[q] = L.cy[m];
+        if (q == 0) rr[q] = L.r[m];
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < kMgLdsLevels; q++) {
+    if (q >= A.nq) break;
+    const MgLevel& L = A.lv[q];
+    const int n = L.nx * L.ny, W = L.nx + 2;
+    const int i = t % L.nx, j = t / L.nx, col = (i + j) & 1;
+    const bool act = t < n;
+    double* x = s_x + A.off[q];
+    const int k = (j + 1) * W + (i + 1);
+    if (q + 1 < A.nq) {
+      if (act && col == 0) x[k] = rr[q] / cd[q];
+      __syncthreads();
+      for (int s = 0; s < 3; s++) {  // black, red, black
+        if (act && col == ((s & 1) ^ 1))
+          x[k] = (rr[q] + cl[q] * x[k - 1] + cr[q] * x[k + 1] + cb[q] * x[k - W] +
+                  ct[q] * x[k + W]) /
+                 cd[q];
+        __syncthreads();
+      }
+      if (act)
+        s_r[t] = rr[q] - (cd[q] * x[k] - cl[q] * x[k - 1] - cr[q] * x[k + 1] -
+                          cb[q] * x[k - W] - ct[q] * x[k + W]);
+      __syncthreads();
+      const MgLevel& C = A.lv[q + 1];
+      if (q + 1 < kMgLdsLevels && t < C.nx * C.ny) {
+        const int I = t % C.nx, J = t / C.nx;
+        double v = 0.0;
+        for (int b = 0; b < 2; b++)
+          for (int a = 0; a < 2; a++)
+            if (2 * I + a < L.nx && 2 * J + b < L.ny) v += s_r[(2 * J + b) * L.nx + 2 * I + a];
+        rr[q + 1 < kMgLdsLevels ? q + 1 : q] = v;
+      }
+      __syncthreads();
+    } else {  // coarsest level
+      if (act) s_r[t] = rr[q];
+      __syncthreads();
+      if (act) {
+        double y = 0.0;
+        for (int c = 0; c < n; c++) y += A.inv[(size_t)c * n + t] * s_r[c];
+        x[k] = y;
+      }
+      __syncthreads();
+    }
+  }
+#pragma unroll
+  for (int q = kMgLdsLevels - 2; q >= 0; q--) {
+    if (q + 1 >= A.nq) continue;
+    const MgLevel &L = A.lv[q], &C = A.lv[q + 1];
+    const int n = L.nx * L.ny, W = L.nx + 2;
+    const int i = t % L.nx, j = t / L.nx, col = (i + j) & 1;
+    const bool act = t < n;
+    double* x = s_x + A.off[q];
+    const double* xc = s_x + A.off[q + 1];
+    const int k = (j + 1) * W + (i + 1);
+    if (act) x[k] += A.oc * xc[((j >> 1) + 1) * (C.nx + 2) + (i >> 1) + 1];
     __syncthreads();
+    for (int s = 0; s < 4; s++) {  // black, red, black, red
+      if (act && col == ((s & 1) ^ 1))
+        x[k] = (rr[q] + cl[q] * x[k - 1] + cr[q] * x[k + 1] + cb[q] * x[k - W] +
+                ct[q] * x[k + W]) /
+               cd[q];
+      __syncthreads();
+    }
   }
   {
-    const MgLevel& L = A.lv[A.nlev - 1];
-    const int n = L.nx * L.ny, nx = L.nx;
-    if (t < kMgBottomMax) s_r[t] = t < n ? L.r[mg_at(L, t % nx, t / nx)] : 0.0;
-    __syncthreads();
-    if (t < n) {
-      double y = 0.0;
-      for (int k = 0; k < n; k++) y += A.inv[(size_t)k * n + t] * s_r[k];
-      L.e[mg_at(L, t % nx, t / nx)] = y;
+    const MgLevel& L = A.lv[0];
+    if (t < L.nx * L.ny) {
+      const int i = t % L.nx, j = t / L.nx;
+      L.e[mg_at(L, i, j)] = s_x[A.off[0] + (j + 1) * (L.nx + 2) + i + 1];
     }
-    __syncthreads();
-  }
-  for (int l = A.nlev - 2; l >= A.first; l--) {
-    const MgLevel &F = A.lv[l], &C = A.lv[l + 1];
-    for (int n = t; n < F.nx * F.ny; n += nt) mg_up_cell(F, C, n, A.omega, A.oc);
-    __syncthreads();
   }
 }
 
@@ -1396,30 +1501,19 @@ static int cells_blocks(int n) { return (n + 255) / 256; }
 void launch_mg_coarsen(const MgArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(k_mg_coarsen, dim3(cells_blocks(a.C.nx * a.C.ny)), dim3(256), 0, s, a);
 }
-void launch_mg_down(const MgArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL(k_mg_down, dim3(cells_blocks(a.C.nx * a.C.ny)), dim3(256), 0, s, a);
+int mg_tiles(const MgLevel& F) {
+  return ((F.nx + kGsTX - 1) / kGsTX) * ((F.ny + kGsTY - 1) / kGsTY);
 }
-int mg_up_blocks(const MgLevel& F) { return cells_blocks(F.nx * F.ny); }
-void launch_mg_up(const MgArgs& a, int nb, hipStream_t s) {
-  hipLaunchKernelGGL(k_mg_up, dim3(nb), dim3(256), 0, s, a);
+void launch_mg_down(const MgArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(k_mg_down, dim3(mg_tiles(a.F)), dim3(kNT), 0, s, a);
+}
+void launch_mg_up(const MgArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(k_mg_up, dim3(mg_tiles(a.F)), dim3(kNT), 0, s, a);
 }
 void launch_mg_factor(const MgArgs& a, double* inv, hipStream_t s) {
-  hipLaunchKernelGGL(k_mg_factor, dim3(1), dim3(64), 0, s, a, inv);
+  hipLaunchKernelGGL(k_mg_factor, dim3(1), dim3(1024), 0, s, a, inv);
 }
-void launch_mg_bottom(const MgArgs& a, const double* inv, hipStream_t s) {
-  hipLaunchKernelGGL(k_mg_bottom, dim3(1), dim3(64), 0, s, a, inv);
-}
-void launch_mg_coarse_cycle(const MgLevel* lv, int nlev, int first, const double* inv,
-                            DevScalars* sc, double omega, double oc, hipStream_t s) {
-  MgCycleArgs a;
-  memset(&a, 0, sizeof(a));
-  for (int l = 0; l < nlev && l < kMgMaxLevels; l++) a.lv[l] = lv[l];
-  a.first = first;
-  a.nlev = nlev;
-  a.inv = inv;
-  a.sc = sc;
-  a.omega = omega;
-  a.oc = oc;
+void launch_mg_coarse_cycle(const MgCycleArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(k_mg_coarse_cycle, dim3(1), dim3(1024), 0, s, a);
 }
 void launch_mgpcg_init(const MgPcgArgs& a, hipStream_t s) {
