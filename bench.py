@@ -26,7 +26,10 @@ PEAK_HBM_GBS = 8000.0
 def gpu_run(args):
     from fluidsim_amd import Discharge, configs
 
-    cfg = configs.config1(args.n, rtol=args.rtol, chunk=args.chunk)
+    if args.config == 2:
+        cfg = configs.config2(args.n, rtol=args.rtol, chunk=args.chunk)
+    else:
+        cfg = configs.config1(args.n, rtol=args.rtol, chunk=args.chunk)
     d = Discharge(cfg)
     d.init_gaussian(seed=0)
     # warm-up: the two Poisson-only passes (main.F90:75) and the first steps,
@@ -45,19 +48,49 @@ def gpu_run(args):
     attempts = len(log)
     iters = [float(r[7]) for r in log if r[6] >= 0]
     elapsed = t1 - t0
-    # per-launch kernel durations on the live state (HIP events on the stream)
     N = args.n * args.n
-    k = {
-        "stage5": d.time_kernel(5, 50),
-        "stage3": d.time_kernel(3, 50),
-        "assembly": d.time_kernel(10, 50),
-        "pcg_it": d.time_kernel(11, 400),
-    }
+    poisson = poisson_compare(d, args.solve_reps)
+    # per-launch kernel durations on the live state (HIP events on the stream)
+    k = {f"stage{s}": d.time_kernel(s, 50) for s in (1, 2, 3, 4, 5)}
+    k["assembly"] = d.time_kernel(10, 50)
+    k["mg_down0"] = d.time_kernel(13, 200)
+    k["mg_up0"] = d.time_kernel(14, 200)
+    k["jacobi_pcg_it"] = d.time_kernel(11, 400)
     from fluidsim_amd.discharge import copy_bandwidth
 
     copy_gbs = copy_bandwidth(1 << 30, 20)
     return dict(cfg=cfg, elapsed=elapsed, accepted=accepted, attempts=attempts,
-                iters=iters, timers=timers, kernels=k, N=N, s1=s1, copy_gbs=copy_gbs)
+                iters=iters, timers=timers, kernels=k, N=N, s1=s1, copy_gbs=copy_gbs,
+                poisson=poisson)
+
+
+def poisson_compare(d, reps):
+    """Time-to-1e-10 of the same assembled system (the final state of the run)
+    with each preconditioner: wall seconds of fs_lapl_solve (assembly + solve
+    + the true-residual check), median over `reps`, from the same phi."""
+    import numpy as np
+
+    ph0 = d.get("ph")
+    out, sols = {}, {}
+    for name, solver in (("mg_pcg", 1), ("jacobi_pcg", 0)):
+        d.set_solver(solver)
+        secs, its, res = [], [], []
+        for _ in range(reps):
+            d.set("ph", 1, ph0)
+            lerr, st = d.lapl_solve()
+            secs.append(st.seconds)
+            its.append(st.iters)
+            res.append(st.resid)
+        sols[name] = d.get("ph")
+        out[name] = {"ms_median": 1e3 * float(np.median(secs)), "iters": int(np.median(its)),
+                     "true_rel_residual": float(max(res)), "converged": lerr == 0}
+    d.set("ph", 1, ph0)
+    d.set_solver(1)
+    a, b = sols["mg_pcg"][1:-1, 1:-1] - ph0[1:-1, 1:-1], sols["jacobi_pcg"][1:-1, 1:-1] - ph0[1:-1, 1:-1]
+    out["correction_rel_l2_mg_vs_jacobi"] = float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    out["plain_over_mg_time"] = out["jacobi_pcg"]["ms_median"] / out["mg_pcg"]["ms_median"]
+    out["learned"] = None
+    return out
 
 
 def cpu_baseline(args):
@@ -99,13 +132,20 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--n", type=int, default=0, help="grid side (default 512, 2048 for config 2)")
     ap.add_argument("--rtol", type=float, default=1e-10)
     ap.add_argument("--chunk", type=int, default=32)
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--solve-reps", type=int, default=3)
+    ap.add_argument("--config", type=int, default=1, choices=(1, 2),
+                    help="1: 512^2 pulse (the metric's workload); 2: 2048^2 DC, 5 cm, 1 Torr")
     args = ap.parse_args()
+    if args.n == 0:
+        args.n = 2048 if args.config == 2 else 512
+    if args.config == 2:
+        args.no_cpu = True  # a CPU step at 2048^2 takes about a minute; not sampled
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU: one discharge per device; "
                                    "multi-GPU sharding is a design note only",
@@ -116,17 +156,19 @@ def main():
     steps_per_s = r["accepted"] / r["elapsed"]
     N = r["N"]
     plane = 8.0 * N
-    # algorithmic bytes per launch (one FP64 plane = 8*N bytes):
-    #  PCG iteration kernel reads r, w, s, p, x, d, cx, cy and writes p, x, s,
-    #  r, w: 13 planes;
-    #  fused stage kernel at stage 5 reads 4 densities, ph, n3 x4, k1, k3, k4 x4
-    #  and writes 4 densities: 25 planes
-    bytes_pcg = 13 * plane
-    bytes_stage5 = 25 * plane
-    ka = r["kernels"]["pcg_it"] * 1e-3
-    ks = r["kernels"]["stage5"] * 1e-3
-    ach_a = bytes_pcg / ka / 1e9
-    ach_s = bytes_stage5 / ks / 1e9
+    kern = r["kernels"]
+    # Algorithmic bytes (one FP64 plane = 8*N bytes).  Transport per accepted
+    # step: 133 planes (RHS 9 per stage x 5 + RK updates 22 per species x 4),
+    # carried by the five launches of the fused stage kernel.
+    bytes_step = 133 * plane
+    t_stages = sum(kern[f"stage{s}"] for s in (1, 2, 3, 4, 5)) * 1e-3
+    ach_s = bytes_step / t_stages / 1e9
+    stage_traffic = profiled_traffic("k_stage", args.n)
+    # multigrid finest level: down reads r, d, cx, cy, writes x1 and the
+    # quarter-size coarse residual (5.25 planes); up reads x1, r, d, cx, cy and
+    # the coarse correction, writes e (6.25 planes)
+    ach_d = 5.25 * plane / (kern["mg_down0"] * 1e-3) / 1e9
+    ach_u = 6.25 * plane / (kern["mg_up0"] * 1e-3) / 1e9
     timers = r["timers"]
     tot_ms = r["elapsed"] * 1e3
     iters = r["iters"]
@@ -143,26 +185,33 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (Gaussian + seeded 1% noise initial state)",
-        "config": {"workload": f"argon DC-glow pulse, {args.n}x{args.n} FP64, 2 Torr, 400 V "
-                               f"trapezoid, geometric x grid",
+        "config": {"workload": (f"argon DC-glow pulse, {args.n}x{args.n} FP64, 2 Torr, 400 V "
+                                f"trapezoid, geometric x grid" if args.config == 1 else
+                                f"argon DC glow, {args.n}x{args.n} FP64, 5 cm x 5 cm uniform, "
+                                f"1 Torr, 300 V through 5e4 ohm"),
                    "nx": args.n, "ny": args.n, "poisson_rtol": args.rtol,
-                   "poisson_solver": "matrix-free Jacobi-PCG (Chronopoulos-Gear)"},
+                   "poisson_solver": "CG preconditioned by an aggregation multigrid V(2,2) "
+                                     "cycle with red-black Gauss-Seidel smoothing"},
         "accepted_steps": r["accepted"],
         "ptcl_attempts": r["attempts"],
         "pcg_iters_mean": sum(iters) / max(len(iters), 1),
         "time_split_ms": {"stage_kernels": timers[0], "circuit_controller": timers[1],
-                          "assembly": timers[2], "pcg": timers[3], "wall": tot_ms},
-        "kernel_ms": r["kernels"],
-        "roofline": {"kernel": "k_pcg_it", "bound": "hbm", "achieved": ach_a,
-                     "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": ach_a / PEAK_HBM_GBS,
-                     "traffic": profiled_traffic("k_pcg_it", args.n),
-                     "algorithmic_bytes": bytes_pcg},
-        "roofline_stage": {"kernel": "k_stage (stage 5)", "bound": "hbm", "achieved": ach_s,
-                           "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": ach_s / PEAK_HBM_GBS},
+                          "assembly": timers[2], "poisson_solve": timers[3], "wall": tot_ms},
+        "kernel_ms": kern,
+        "roofline": {"kernel": "k_stage (5 launches per step)", "bound": "hbm",
+                     "achieved": ach_s, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": ach_s / PEAK_HBM_GBS,
+                     "traffic": None if stage_traffic is None else stage_traffic,
+                     "algorithmic_bytes_per_step": bytes_step},
+        "roofline_poisson": {"kernels": "k_mg_down / k_mg_up, finest level", "bound": "hbm",
+                             "achieved_down": ach_d, "achieved_up": ach_u,
+                             "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                             "frac_down": ach_d / PEAK_HBM_GBS, "frac_up": ach_u / PEAK_HBM_GBS},
         "hbm_copy_GBs_measured": r["copy_gbs"],
-        "poisson": {"plain_pcg_ms_per_solve": timers[3] / max(r["accepted"], 1),
-                    "plain_pcg_iters": sum(iters) / max(len(iters), 1), "learned": None},
+        "poisson": r["poisson"],
     }
+    if args.config == 2:
+        line["metric"] = "RK-Merson steps/s at 2048^2 FP64 (large single instance)"
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args)
     else:

@@ -1193,6 +1193,14 @@ int fs_time_kernel(fs_handle* h, int32_t which, int32_t n, double* avg_ms) {
       launch_lapl_assemble(a, h->K, h->st);
     } else if (which == 11) {
       launch_pcg_it(pcg_args(h, 2 + (k & 1)), h->st);
+    } else if (which == 13 || which == 14) {
+      MgArgs a = mg_args(h, 0);
+      a.it = 2 + (k & 1);
+      a.part_in = h->part(0);
+      a.nb_in = h->nbx * h->pnby;
+      a.part_out = h->part(1);
+      if (which == 13) launch_mg_down(a, h->st);
+      else launch_mg_up(a, h->st);
     } else {
       PcgArgs ia = pcg_args(h, 0);
       ia.part_out = h->part(0);
@@ -1200,8 +1208,10 @@ int fs_time_kernel(fs_handle* h, int32_t which, int32_t n, double* avg_ms) {
       launch_pcg_init(ia, h->st);
     }
   };
-  if (!(which >= 1 && which <= 5) && which != 10 && which != 11 && which != 12)
+  if (!(which >= 1 && which <= 5) && !(which >= 10 && which <= 14))
     return fail(FS_E_ARG, "unknown kernel id%s");
+  if ((which == 13 || which == 14) && h->mg.size() < 2)
+    return fail(FS_E_ARG, "grid too small for a multigrid hierarchy%s");
   launch(0);  // warm
   HIP_OK(hipEventRecord(h->ev[6], h->st));
   for (int k = 0; k < n; k++) launch(k);
@@ -1242,6 +1252,13 @@ int fs_copy_bench(int64_t bytes, int32_t reps, double* gbps) {
 int fs_get_timers(const fs_handle* h, double* out4) {
   if (!h || !out4) return fail(FS_E_ARG, "null argument%s");
   memcpy(out4, h->tms, sizeof(h->tms));
+  return 0;
+}
+
+int fs_set_solver(fs_handle* h, int32_t solver) {
+  if (!h) return fail(FS_E_ARG, "null handle%s");
+  if (solver < 0 || solver > 1) return fail(FS_E_ARG, "unknown Poisson solver%s");
+  h->p.lapl_solver = solver;
   return 0;
 }
 

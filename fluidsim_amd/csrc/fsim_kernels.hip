@@ -827,14 +827,19 @@ __global__ void __launch_bounds__(kNT) k_lapl(LaplArgs A, FsPhys K) {
 // neighbour, so w_{i+1} needs no second pass.  r, w, s are double-buffered.
 
 // Sum the previous kernel's 3 partials per block, identically in every block.
-__device__ __forceinline__ void reduce_partials3(const double* part, int nb, double* s_red,
-                                                 double (&tot)[3]) {
-  double v[3] = {0.0, 0.0, 0.0};
+// Partial sums of the previous kernel: this thread's share (loads only, so
+// that a kernel can issue them together with its own loads) ...
+__device__ __forceinline__ void load_partials3(const double* part, int nb, double (&v)[3]) {
+  v[0] = v[1] = v[2] = 0.0;
   for (int b = threadIdx.x; b < nb; b += kNT) {
     v[0] += part[3 * b];
     v[1] += part[3 * b + 1];
     v[2] += part[3 * b + 2];
   }
+}
+
+// ... and the block-wide total, identical in every block.
+__device__ __forceinline__ void finish_partials3(double (&v)[3], double* s_red, double (&tot)[3]) {
   block_sum<3>(v, s_red);
   __shared__ double s_tot[3];
   if (threadIdx.x == 0) {
@@ -846,6 +851,13 @@ __device__ __forceinline__ void reduce_partials3(const double* part, int nb, dou
   tot[0] = s_tot[0];
   tot[1] = s_tot[1];
   tot[2] = s_tot[2];
+}
+
+__device__ __forceinline__ void reduce_partials3(const double* part, int nb, double* s_red,
+                                                 double (&tot)[3]) {
+  double v[3];
+  load_partials3(part, nb, v);
+  finish_partials3(v, s_red, tot);
 }
 
 __device__ __forceinline__ void write_partials3(double (&v)[3], double* part, double* s_red) {
@@ -1053,39 +1065,77 @@ constexpr int kGsTX = 32, kGsTY = 16, kGsH = 4;  // tile and halo (four half-swe
 constexpr int kGsW = kGsTX + 2 * kGsH, kGsR = kGsTY + 2 * kGsH;
 static_assert((kGsTX / 2) * kGsTY == kNT, "one thread per horizontal cell pair");
 
-// One Gauss-Seidel half-sweep over the cells of colour c ((i + j) & 1 == c)
-// within h <= kGsH - 1 cells of the tile at (i0, j0).  Cells of the other
-// colour are read from LDS; cells outside the level stay zero.
-__device__ __forceinline__ void gs_half(const MgLevel& F, double (*sx)[kGsW], int i0, int j0,
-                                        int c, int h) {
-  const int w2 = (kGsTX + 2 * h) / 2, rows = kGsTY + 2 * h, a0 = kGsH - h;
-  for (int k = threadIdx.x; k < rows * w2; k += kNT) {
-    const int b = a0 + k / w2;
-    const int j = j0 - kGsH + b;
-    const int a = a0 + 2 * (k % w2) + ((c + i0 - kGsH + a0 + j) & 1);
-    const int i = i0 - kGsH + a;
-    if (i < 0 || i >= F.nx || j < 0 || j >= F.ny) continue;
-    const size_t m = mg_at(F, i, j), P = F.P;
-    sx[b][a] = (F.r[m] + F.cx[m - 1] * sx[b][a - 1] + F.cx[m] * sx[b][a + 1] +
-                F.cy[m - P] * sx[b - 1][a] + F.cy[m] * sx[b + 1][a]) /
-               F.d[m];
-  }
-}
+// A level's equation on one tile plus a halo of kGsH cells, staged in LDS:
+// right-hand side, diagonal, right/upper couplings and the iterate.  Cells
+// outside the level carry r = 0, d = 1 and no couplings, so they stay zero.
+struct GsTile {
+  double x[kGsR][kGsW];
+  double r[kGsR][kGsW];
+  double d[kGsR][kGsW];
+  double cx[kGsR][kGsW];
+  double cy[kGsR][kGsW];
+};
 
 __device__ __forceinline__ int gs_tiles_x(const MgLevel& F) { return (F.nx + kGsTX - 1) / kGsTX; }
 
+// Stage r, d, cx, cy of the tile at (i0, j0) and its halo.
+__device__ __forceinline__ void gs_stage(const MgLevel& F, GsTile& s, int i0, int j0) {
+  for (int k = threadIdx.x; k < kGsR * kGsW; k += kNT) {
+    const int b = k / kGsW, a = k % kGsW;
+    const int i = i0 - kGsH + a, j = j0 - kGsH + b;
+    double r = 0.0, d = 1.0, cx = 0.0, cy = 0.0;
+    if (i >= 0 && i < F.nx && j >= 0 && j < F.ny) {
+      const size_t m = mg_at(F, i, j);
+      r = F.r[m];
+      d = F.d[m];
+      cx = F.cx[m];
+      cy = F.cy[m];
+    }
+    s.r[b][a] = r;
+    s.d[b][a] = d;
+    s.cx[b][a] = cx;
+    s.cy[b][a] = cy;
+  }
+}
+
+// One Gauss-Seidel half-sweep over the cells of colour c ((i + j) & 1 == c)
+// within h <= kGsH - 1 cells of the tile at (i0, j0), entirely in LDS.
+__device__ __forceinline__ void gs_half(GsTile& s, int i0, int j0, int c, int h) {
+  const int w2 = (kGsTX + 2 * h) / 2, rows = kGsTY + 2 * h, a0 = kGsH - h;
+  for (int k = threadIdx.x; k < rows * w2; k += kNT) {
+    const int b = a0 + k / w2;
+    const int a = a0 + 2 * (k % w2) + ((c + i0 - kGsH + a0 + j0 - kGsH + b) & 1);
+    s.x[b][a] = (s.r[b][a] + s.cx[b][a - 1] * s.x[b][a - 1] + s.cx[b][a] * s.x[b][a + 1] +
+                 s.cy[b - 1][a] * s.x[b - 1][a] + s.cy[b][a] * s.x[b + 1][a]) /
+                s.d[b][a];
+  }
+}
+
+__device__ __forceinline__ double gs_mx(const GsTile& s, int b, int a) {
+  return s.d[b][a] * s.x[b][a] - s.cx[b][a - 1] * s.x[b][a - 1] - s.cx[b][a] * s.x[b][a + 1] -
+         s.cy[b - 1][a] * s.x[b - 1][a] - s.cy[b][a] * s.x[b + 1][a];
+}
+
 // Down: pre-smooth M x = r on the tile, keep x1, restrict the residual into
 // the coarse level.  Level 0 first checks convergence from the x/r-update
-// partials (every block identically).
+// partials (every block identically); their loads are issued together with
+// the tile's.
 __global__ void __launch_bounds__(kNT) k_mg_down(MgArgs A) {
-  __shared__ double s_x[kGsR][kGsW];
+  __shared__ GsTile s;
   __shared__ double s_res[kGsTY][kGsTX / 2];
   __shared__ double s_red[8 * kNW];
   DevScalars* sc = A.sc;
   if (sc->done) return;
+  const MgLevel &F = A.F, &C = A.C;
+  const int ntx = gs_tiles_x(F);
+  const int t = xcd_tile(blockIdx.x, gridDim.x);
+  const int i0 = (t % ntx) * kGsTX, j0 = (t / ntx) * kGsTY;
+  double pv[3];
+  if (A.level0) load_partials3(A.part_in, A.nb_in, pv);
+  gs_stage(F, s, i0, j0);
   if (A.level0) {
     double tot[3];
-    reduce_partials3(A.part_in, A.nb_in, s_red, tot);
+    finish_partials3(pv, s_red, tot);
     const double rho = tot[0];
     if (A.init) {
       if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -1114,27 +1164,18 @@ __global__ void __launch_bounds__(kNT) k_mg_down(MgArgs A) {
       if (stop) return;
     }
   }
-  const MgLevel &F = A.F, &C = A.C;
-  const int ntx = gs_tiles_x(F);
-  const int t = xcd_tile(blockIdx.x, gridDim.x);
-  const int i0 = (t % ntx) * kGsTX, j0 = (t / ntx) * kGsTY;
-  // first (red) half-sweep from zero is pointwise: stage the whole halo
+  __syncthreads();
+  // first (red) half-sweep from zero is pointwise: the whole staged halo
   for (int k = threadIdx.x; k < kGsR * kGsW; k += kNT) {
     const int b = k / kGsW, a = k % kGsW;
-    const int i = i0 - kGsH + a, j = j0 - kGsH + b;
-    double v = 0.0;
-    if (i >= 0 && i < F.nx && j >= 0 && j < F.ny && ((i + j) & 1) == 0) {
-      const size_t m = mg_at(F, i, j);
-      v = F.r[m] / F.d[m];
-    }
-    s_x[b][a] = v;
+    s.x[b][a] = ((i0 + a + j0 + b) & 1) == 0 ? s.r[b][a] / s.d[b][a] : 0.0;
   }
   __syncthreads();
-  gs_half(F, s_x, i0, j0, 1, 3);
+  gs_half(s, i0, j0, 1, 3);
   __syncthreads();
-  gs_half(F, s_x, i0, j0, 0, 2);
+  gs_half(s, i0, j0, 0, 2);
   __syncthreads();
-  gs_half(F, s_x, i0, j0, 1, 1);
+  gs_half(s, i0, j0, 1, 1);
   __syncthreads();
   // residual of one horizontal cell pair (half an aggregate) per thread
   const int cI = threadIdx.x % (kGsTX / 2), fj = threadIdx.x / (kGsTX / 2);
@@ -1143,12 +1184,8 @@ __global__ void __launch_bounds__(kNT) k_mg_down(MgArgs A) {
     const int a = kGsH + 2 * cI + q, b = kGsH + fj;
     const int i = i0 + 2 * cI + q, j = j0 + fj;
     if (i < F.nx && j < F.ny) {
-      const size_t m = mg_at(F, i, j), P = F.P;
-      const double xc = s_x[b][a];
-      const double mx = F.d[m] * xc - F.cx[m - 1] * s_x[b][a - 1] - F.cx[m] * s_x[b][a + 1] -
-                        F.cy[m - P] * s_x[b - 1][a] - F.cy[m] * s_x[b + 1][a];
-      F.x1[m] = xc;
-      rs += F.r[m] - mx;
+      F.x1[mg_at(F, i, j)] = s.x[b][a];
+      rs += s.r[b][a] - gs_mx(s, b, a);
     }
   }
   s_res[fj][cI] = rs;
@@ -1162,76 +1199,95 @@ __global__ void __launch_bounds__(kNT) k_mg_down(MgArgs A) {
 // Up: x2 = x1 + oc * e_c(aggregate) on the staged halo, post-smooth, write e.
 // Level 0 writes u = B r and the partials of gamma = r.u.
 __global__ void __launch_bounds__(kNT) k_mg_up(MgArgs A) {
-  __shared__ double s_x[kGsR][kGsW];
+  __shared__ GsTile s;
   __shared__ double s_red[8 * kNW];
   if (A.sc->done) return;
   const MgLevel &F = A.F, &C = A.C;
   const int ntx = gs_tiles_x(F);
   const int t = xcd_tile(blockIdx.x, gridDim.x);
   const int i0 = (t % ntx) * kGsTX, j0 = (t / ntx) * kGsTY;
+  gs_stage(F, s, i0, j0);
   for (int k = threadIdx.x; k < kGsR * kGsW; k += kNT) {
     const int b = k / kGsW, a = k % kGsW;
     const int i = i0 - kGsH + a, j = j0 - kGsH + b;
     double v = 0.0;
     if (i >= 0 && i < F.nx && j >= 0 && j < F.ny)
       v = F.x1[mg_at(F, i, j)] + A.oc * C.e[mg_at(C, i >> 1, j >> 1)];
-    s_x[b][a] = v;
+    s.x[b][a] = v;
   }
   __syncthreads();
-  gs_half(F, s_x, i0, j0, 1, 3);
+  gs_half(s, i0, j0, 1, 3);
   __syncthreads();
-  gs_half(F, s_x, i0, j0, 0, 2);
+  gs_half(s, i0, j0, 0, 2);
   __syncthreads();
-  gs_half(F, s_x, i0, j0, 1, 1);
+  gs_half(s, i0, j0, 1, 1);
   __syncthreads();
-  gs_half(F, s_x, i0, j0, 0, 0);
+  gs_half(s, i0, j0, 0, 0);
   __syncthreads();
   double acc[3] = {0.0, 0.0, 0.0};
   for (int k = threadIdx.x; k < kGsTX * kGsTY; k += kNT) {
     const int i = i0 + k % kGsTX, j = j0 + k / kGsTX;
     if (i < F.nx && j < F.ny) {
-      const size_t m = mg_at(F, i, j);
-      const double e = s_x[kGsH + k / kGsTX][kGsH + k % kGsTX];
-      F.e[m] = e;
-      acc[0] += F.r[m] * e;
+      const int b = kGsH + k / kGsTX, a = kGsH + k % kGsTX;
+      const double e = s.x[b][a];
+      F.e[mg_at(F, i, j)] = e;
+      acc[0] += s.r[b][a] * e;
     }
   }
   if (A.level0) write_partials3(acc, A.part_out, s_red);
 }
 
-// Coarsest level (<= 64 cells): its operator is inverted once per solve
-// (Gauss-Jordan in LDS, SPD so no pivoting; 64 rows x 16 column groups of
-// threads) and each cycle applies the dense inverse, stored column-major so
-// that lane t reads row t coalesced.
+// Coarsest level (<= 64 cells): its operator is inverted once per solve by
+// Gauss-Jordan elimination (SPD, so no pivoting).  Thread (row, g) keeps the
+// four entries (row, g + 16 c) in registers; each pivot step publishes the
+// pivot row and column through LDS buffers alternated by step parity, so one
+// barrier per step suffices.  The inverse is stored column-major.
 __global__ void __launch_bounds__(1024) k_mg_factor(MgArgs A, double* inv) {
-  __shared__ double s_a[kMgBottomMax][kMgBottomMax + 1];
-  __shared__ double s_prow[kMgBottomMax];
-  constexpr int kGroups = 1024 / kMgBottomMax;
+  constexpr int kG = 1024 / kMgBottomMax, kC = kMgBottomMax / kG;
+  __shared__ double s_prow[2][kMgBottomMax], s_pcol[2][kMgBottomMax];
   const MgLevel& L = A.F;
   const int n = L.nx * L.ny, nx = L.nx, t = threadIdx.x;
-  const int row = t % kMgBottomMax, cg = t / kMgBottomMax;
-  for (int k = cg; k < kMgBottomMax; k += kGroups) s_a[row][k] = 0.0;
-  __syncthreads();
-  if (t < n) {
-    size_t m = mg_at(L, t % nx, t / nx);
-    s_a[t][t] = L.d[m];
-    if (t % nx > 0) s_a[t][t - 1] = -L.cx[m - 1];
-    if (t % nx + 1 < nx) s_a[t][t + 1] = -L.cx[m];
-    if (t >= nx) s_a[t][t - nx] = -L.cy[m - L.P];
-    if (t + nx < n) s_a[t][t + nx] = -L.cy[m];
+  const int row = t % kMgBottomMax, g = t / kMgBottomMax;
+  double a[kC];
+#pragma unroll
+  for (int c = 0; c < kC; c++) {
+    const int k = g + kG * c;
+    double v = 0.0;
+    if (row < n && k < n) {
+      const size_t m = mg_at(L, row % nx, row / nx);
+      if (k == row) v = L.d[m];
+      else if (k == row - 1 && row % nx > 0) v = -L.cx[m - 1];
+      else if (k == row + 1 && row % nx + 1 < nx) v = -L.cx[m];
+      else if (k == row - nx) v = -L.cy[m - L.P];
+      else if (k == row + nx) v = -L.cy[m];
+    }
+    a[c] = v;
   }
-  __syncthreads();
-  for (int p = 0; p < n; p++) {  // in-place Gauss-Jordan inversion
-    const double piv = s_a[p][p];
-    const double f = s_a[row][p];
-    if (t < n) s_prow[t] = (t == p ? 1.0 : s_a[p][t]) / piv;
+  for (int p = 0; p < n; p++) {
+    const int buf = p & 1;
+    if (row == p) {
+#pragma unroll
+      for (int c = 0; c < kC; c++) s_prow[buf][g + kG * c] = a[c];
+    }
+    if (g == p % kG) {
+#pragma unroll
+      for (int c = 0; c < kC; c++)
+        if (c == p / kG) s_pcol[buf][row] = a[c];
+    }
     __syncthreads();
-    if (row < n)
-      for (int k = cg; k < n; k += kGroups)
-        s_a[row][k] = row == p ? s_prow[k] : (k == p ? 0.0 : s_a[row][k]) - f * s_prow[k];
-    __syncthreads();
+    const double piv = s_prow[buf][p], f = s_pcol[buf][row];
+#pragma unroll
+    for (int c = 0; c < kC; c++) {
+      const int k = g + kG * c;
+      const double pk = (k == p ? 1.0 : s_prow[buf][k]) / piv;
+      a[c] = row == p ? pk : (k == p ? 0.0 : a[c]) - f * pk;
+    }
   }
-  for (int e = t; e < n * n; e += 1024) inv[e] = s_a[e % n][e / n];
+#pragma unroll
+  for (int c = 0; c < kC; c++) {
+    const int k = g + kG * c;
+    if (row < n && k < n) inv[(size_t)k * n + row] = a[c];
+  }
 }
 
 // Levels f .. L-1 (<= 1024 cells each, at most kMgLdsLevels) in one workgroup
@@ -1242,10 +1298,19 @@ __global__ void __launch_bounds__(1024) k_mg_factor(MgArgs A, double* inv) {
 __global__ void __launch_bounds__(1024) k_mg_coarse_cycle(MgCycleArgs A) {
   __shared__ double s_x[kMgLdsX];
   __shared__ double s_r[1024];
+  __shared__ double s_bp[1024 / kMgBottomMax][kMgBottomMax];
   if (A.sc->done) return;
   const int t = threadIdx.x;
   double cd[kMgLdsLevels], cl[kMgLdsLevels], cr[kMgLdsLevels], cb[kMgLdsLevels],
       ct[kMgLdsLevels], rr[kMgLdsLevels];
+  // the coarsest level's inverse, entries (row, g + 16 c) of thread (row, g)
+  constexpr int kG = 1024 / kMgBottomMax, kC = kMgBottomMax / kG;
+  const int nb = A.lv[A.nq - 1].nx * A.lv[A.nq - 1].ny;
+  const int brow = t % kMgBottomMax, bg = t / kMgBottomMax;
+  double binv[kC];
+#pragma unroll
+  for (int c = 0; c < kC; c++)
+    binv[c] = (brow < nb && bg + kG * c < nb) ? A.inv[(size_t)(bg + kG * c) * nb + brow] : 0.0;
   for (int k = t; k < kMgLdsX; k += 1024) s_x[k] = 0.0;
 #pragma unroll
   for (int q = 0; q < kMgLdsLevels; q++) {
@@ -1298,13 +1363,19 @@ __global__ void __launch_bounds__(1024) k_mg_coarse_cycle(MgCycleArgs A) {
         rr[q + 1 < kMgLdsLevels ? q + 1 : q] = v;
       }
       __syncthreads();
-    } else {  // coarsest level
+    } else {  // coarsest level: x = inv r, partial sums over 16 column groups
       if (act) s_r[t] = rr[q];
       __syncthreads();
+      double y = 0.0;
+#pragma unroll
+      for (int c = 0; c < kC; c++)
+        if (bg + kG * c < n) y += binv[c] * s_r[bg + kG * c];
+      s_bp[bg][brow] = y;
+      __syncthreads();
       if (act) {
-        double y = 0.0;
-        for (int c = 0; c < n; c++) y += A.inv[(size_t)c * n + t] * s_r[c];
-        x[k] = y;
+        double v = 0.0;
+        for (int g = 0; g < kG; g++) v += s_bp[g][t];
+        x[k] = v;
       }
       __syncthreads();
     }
@@ -1359,61 +1430,100 @@ __global__ void __launch_bounds__(kNT) k_mgpcg_init(MgPcgArgs A) {
   write_partials3(acc, A.part_out, s_red);
 }
 
-// p = u + beta p_old (neighbours recomputed), q = M p, partials of p.q
+// p = u + beta p_old (neighbours recomputed), q = M p, partials of p.q.
+// The cell loads are issued before the reduction that yields beta.
 __global__ void __launch_bounds__(kNT) k_mgpcg_pq(MgPcgArgs A) {
   __shared__ double s_red[8 * kNW];
   DevScalars* sc = A.sc;
   if (sc->done) return;
-  double tot[3];
-  reduce_partials3(A.part_in, A.nb_in, s_red, tot);
-  const double gam = tot[0];
-  const double beta = A.first ? 0.0 : gam / sc->cg_gam[(A.it + 1) & 1];
-  if (blockIdx.x == 0 && threadIdx.x == 0) sc->cg_gam[A.it & 1] = gam;
   const Geo& G = A.G;
   const int P = G.P;
   const int t = xcd_tile(blockIdx.x, gridDim.x);
-  double acc[3] = {0.0, 0.0, 0.0};
   const double *u = A.u, *po = A.p_old;
+  double pv[3];
+  load_partials3(A.part_in, A.nb_in, pv);
+  double uu[kPcgRows][5], pp[kPcgRows][5], cf[kPcgRows][5];
+  size_t mm[kPcgRows];
+  bool in[kPcgRows];
+#pragma unroll
   for (int k = 0; k < kPcgRows; k++) {
     const int ci = (t % A.nbx) * kTX + (int)(threadIdx.x % kTX);
     const int cj = (t / A.nbx) * kPcgTY + (int)(threadIdx.x / kTX) + k * (kPcgTY / kPcgRows);
-    if (ci < G.bx && cj < G.by) {
-      size_t m = (size_t)(cj + 1) * P + (ci + 1);
-      auto pv = [&](size_t q) { return u[q] + beta * po[q]; };
-      double pc = pv(m);
-      double q = A.dg[m] * pc - A.cx[m - 1] * pv(m - 1) - A.cx[m] * pv(m + 1) -
-                 A.cy[m - P] * pv(m - P) - A.cy[m] * pv(m + P);
-      A.p_new[m] = pc;
-      A.q[m] = q;
-      acc[0] += pc * q;
+    in[k] = ci < G.bx && cj < G.by;
+    const size_t m = in[k] ? (size_t)(cj + 1) * P + (ci + 1) : (size_t)P + 1;
+    mm[k] = m;
+    const size_t nb[5] = {m, m - 1, m + 1, m - P, m + P};
+#pragma unroll
+    for (int s = 0; s < 5; s++) {
+      uu[k][s] = u[nb[s]];
+      pp[k][s] = po[nb[s]];
     }
+    cf[k][0] = A.dg[m];
+    cf[k][1] = A.cx[m - 1];
+    cf[k][2] = A.cx[m];
+    cf[k][3] = A.cy[m - P];
+    cf[k][4] = A.cy[m];
+  }
+  double tot[3];
+  finish_partials3(pv, s_red, tot);
+  const double gam = tot[0];
+  const double beta = A.first ? 0.0 : gam / sc->cg_gam[(A.it + 1) & 1];
+  if (blockIdx.x == 0 && threadIdx.x == 0) sc->cg_gam[A.it & 1] = gam;
+  double acc[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int k = 0; k < kPcgRows; k++) {
+    if (!in[k]) continue;
+    double pn[5];
+#pragma unroll
+    for (int s = 0; s < 5; s++) pn[s] = uu[k][s] + beta * pp[k][s];
+    const double q = cf[k][0] * pn[0] - cf[k][1] * pn[1] - cf[k][2] * pn[2] -
+                     cf[k][3] * pn[3] - cf[k][4] * pn[4];
+    A.p_new[mm[k]] = pn[0];
+    A.q[mm[k]] = q;
+    acc[0] += pn[0] * q;
   }
   write_partials3(acc, A.part_out, s_red);
 }
 
-// alpha = gamma / p.q; x += alpha p; r -= alpha q; partials of |r/V|^2
+// alpha = gamma / p.q; x += alpha p; r -= alpha q; partials of |r/V|^2.
+// The cell loads are issued before the reduction that yields alpha.
 __global__ void __launch_bounds__(kNT) k_mgpcg_xr(MgPcgArgs A) {
   __shared__ double s_red[8 * kNW];
   DevScalars* sc = A.sc;
   if (sc->done) return;
-  double tot[3];
-  reduce_partials3(A.part_in, A.nb_in, s_red, tot);
-  const double alpha = sc->cg_gam[A.it & 1] / tot[0];
   const Geo& G = A.G;
   const int t = xcd_tile(blockIdx.x, gridDim.x);
-  double acc[3] = {0.0, 0.0, 0.0};
+  double pv[3];
+  load_partials3(A.part_in, A.nb_in, pv);
+  double xv[kPcgRows], pn[kPcgRows], rv[kPcgRows], qv[kPcgRows], vol[kPcgRows];
+  size_t mm[kPcgRows];
+  bool in[kPcgRows];
+#pragma unroll
   for (int k = 0; k < kPcgRows; k++) {
     const int ci = (t % A.nbx) * kTX + (int)(threadIdx.x % kTX);
     const int cj = (t / A.nbx) * kPcgTY + (int)(threadIdx.x / kTX) + k * (kPcgTY / kPcgRows);
-    if (ci < G.bx && cj < G.by) {
-      size_t m = (size_t)(cj + 1) * G.P + (ci + 1);
-      A.x[m] = A.x[m] + alpha * A.p_new[m];
-      double r = A.r[m] - alpha * A.q[m];
-      A.r[m] = r;
-      double V = G.dlx[ci] * G.dly[cj] * (G.cyl ? G.r[cj + 1] : 1.0);
-      double rv = r / V;
-      acc[0] += rv * rv;
-    }
+    in[k] = ci < G.bx && cj < G.by;
+    const int cic = in[k] ? ci : 0, cjc = in[k] ? cj : 0;
+    const size_t m = (size_t)(cjc + 1) * G.P + (cic + 1);
+    mm[k] = m;
+    xv[k] = A.x[m];
+    pn[k] = A.p_new[m];
+    rv[k] = A.r[m];
+    qv[k] = A.q[m];
+    vol[k] = G.dlx[cic] * G.dly[cjc] * (G.cyl ? G.r[cjc + 1] : 1.0);
+  }
+  double tot[3];
+  finish_partials3(pv, s_red, tot);
+  const double alpha = sc->cg_gam[A.it & 1] / tot[0];
+  double acc[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int k = 0; k < kPcgRows; k++) {
+    if (!in[k]) continue;
+    A.x[mm[k]] = xv[k] + alpha * pn[k];
+    const double r = rv[k] - alpha * qv[k];
+    A.r[mm[k]] = r;
+    const double rn = r / vol[k];
+    acc[0] += rn * rn;
   }
   write_partials3(acc, A.part_out, s_red);
 }

@@ -163,6 +163,10 @@ class Discharge:
         _L.check(_L.load().fs_lapl_assemble(self.h, _dp(b), _dp(A5)))
         return b.reshape(self.ny, self.nx), A5.reshape(self.ny, self.nx, 5)
 
+    def set_solver(self, solver: int):
+        """Poisson preconditioner of later solves: 0 Jacobi, 1 multigrid."""
+        _L.check(_L.load().fs_set_solver(self.h, int(solver)))
+
     def set_timing(self, on: bool):
         _L.check(_L.load().fs_set_timing(self.h, int(on)))
 

@@ -25,7 +25,7 @@ EXPORTS = [
     "fs_ptcl_step", "fs_lapl_solve", "fs_accept", "fs_reject", "fs_run", "fs_set_replay",
     "fs_log_count", "fs_get_log", "fs_clear_log", "fs_rhs_sweep", "fs_circ_step",
     "fs_lapl_assemble", "fs_constants", "fs_eval_fits", "fs_eval_flux", "fs_eval_dfdph",
-    "fs_get_timers", "fs_set_timing", "fs_time_kernel", "fs_copy_bench", "fs_last_error",
+    "fs_get_timers", "fs_set_timing", "fs_set_solver", "fs_time_kernel", "fs_copy_bench", "fs_last_error",
 ]
 
 
@@ -105,6 +105,7 @@ def load():
         "fs_eval_dfdph": [C.c_int32, D, D],
         "fs_get_timers": [P, D],
         "fs_set_timing": [P, C.c_int32],
+        "fs_set_solver": [P, C.c_int32],
         "fs_time_kernel": [P, C.c_int32, C.c_int32, D],
         "fs_copy_bench": [C.c_int64, C.c_int32, D],
         "fs_last_error": [],

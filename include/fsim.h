@@ -179,6 +179,12 @@ int fs_eval_fits(double p_torr, int32_t n, const double* T, double* out);
 int fs_eval_flux(int32_t n, const double* args7, double* out);
 int fs_eval_dfdph(int32_t n, const double* args7, double* out);
 
+/* Select the Poisson preconditioner of later fs_lapl_solve calls (0 Jacobi,
+ * 1 multigrid), as fs_params.lapl_solver does at creation.  The reference
+ * takes its Krylov method and preconditioner from the runtime options database
+ * (KSPSetFromOptions, lapl_lib.F90:50-54); this is the equivalent switch. */
+int fs_set_solver(fs_handle* h, int32_t solver);
+
 /* Device timing of the last fs_run/fs_ptcl_step/fs_lapl_solve phases in ms:
  * [stage kernels, circuit+controller, assembly, pcg]. */
 int fs_get_timers(const fs_handle* h, double* out4);
@@ -187,8 +193,9 @@ int fs_set_timing(fs_handle* h, int32_t on);
 /* Average duration (ms) of n back-to-back launches of one hot-path kernel on
  * the handle's stream, measured with HIP events on the current state.  The
  * state is overwritten, so call it after the timed run.  which: 1..5 = the
- * fused stage kernel of that RK stage, 10 = Poisson assembly, 11/12 = the two
- * PCG iteration kernels. */
+ * fused stage kernel of that RK stage, 10 = Poisson assembly, 11/12 = the
+ * Jacobi-PCG iteration and initialisation kernels, 13/14 = the multigrid down
+ * and up kernels of the finest level. */
 int fs_time_kernel(fs_handle* h, int32_t which, int32_t n, double* avg_ms);
 
 /* Device copy bandwidth: `reps` launches of a plain copy kernel moving
