@@ -461,10 +461,10 @@ void launch_vcycle(fs_handle* h, int it, bool init, int nb_rho) {
 // One MG-PCG iteration: p/q, x/r update, V-cycle (convergence is tested in
 // the V-cycle's first kernel).  Only the parity of `it` matters after it = 0.
 void launch_mg_iteration(fs_handle* h, int it) {
-  const int nb_pcg = h->nbx * h->pnby;
+  const int nb_pcg = capped_grid(h->nbx * h->pnby);
   MgPcgArgs a = mgpcg_args(h, it);
   a.part_in = h->part(1);
-  a.nb_in = mg_tiles(h->mg[0]);
+  a.nb_in = mg_grid(h->mg[0]);
   a.part_out = h->part(2);
   launch_mgpcg_pq(a, h->st);
   a.part_in = h->part(2);
@@ -523,7 +523,7 @@ int lapl_solve_impl(fs_handle* h, int* lerr, fs_solve_stats* stats) {
     MgPcgArgs ia = mgpcg_args(h, 1);  // zero the p that iteration 0 reads as p_old
     ia.part_out = h->part(0);
     launch_mgpcg_init(ia, h->st);
-    const int nb_pcg = h->nbx * h->pnby;
+    const int nb_pcg = capped_grid(h->nbx * h->pnby);
     launch_vcycle(h, 0, true, nb_pcg);
     launch_mg_iteration(h, 0);
     it = 1;
@@ -1197,7 +1197,7 @@ int fs_time_kernel(fs_handle* h, int32_t which, int32_t n, double* avg_ms) {
       MgArgs a = mg_args(h, 0);
       a.it = 2 + (k & 1);
       a.part_in = h->part(0);
-      a.nb_in = h->nbx * h->pnby;
+      a.nb_in = capped_grid(h->nbx * h->pnby);
       a.part_out = h->part(1);
       if (which == 13) launch_mg_down(a, h->st);
       else launch_mg_up(a, h->st);

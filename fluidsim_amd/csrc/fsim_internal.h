@@ -135,6 +135,12 @@ struct MgArgs {
   int32_t it, level0, init, pad_;
 };
 
+// Kernels whose per-block partial sums every block of the next kernel reads
+// run at most kMaxPartials blocks (each covering a contiguous range of tiles),
+// so that the cross-kernel reduction stays O(kMaxPartials) per block.
+constexpr int kMaxPartials = 1024;
+inline int capped_grid(int ntiles) { return ntiles < kMaxPartials ? ntiles : kMaxPartials; }
+
 // most levels a hierarchy may have
 constexpr int kMgMaxLevels = 16;
 constexpr int kMgBottomMax = 64;   // cells of the coarsest level (dense inverse)
@@ -186,7 +192,7 @@ void launch_pcg_init(const PcgArgs& a, hipStream_t s);
 void launch_pcg_it(const PcgArgs& a, hipStream_t s);
 void launch_pcg_finish(const PcgArgs& a, hipStream_t s);
 void launch_mg_coarsen(const MgArgs& a, hipStream_t s);
-int mg_tiles(const MgLevel& F);  // tiles (= blocks, = partials) of a down/up launch
+int mg_grid(const MgLevel& F);  // blocks (= partials) of a down/up launch
 void launch_mg_down(const MgArgs& a, hipStream_t s);
 void launch_mg_up(const MgArgs& a, hipStream_t s);
 void launch_mg_factor(const MgArgs& a, double* inv, hipStream_t s);

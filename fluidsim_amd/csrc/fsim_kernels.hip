@@ -36,6 +36,18 @@ __device__ __forceinline__ int xcd_tile(int b, int nb) {
   return (b % 8) * (nb8 / 8) + b / 8;
 }
 
+// Contiguous range [t0, t1) of the ntiles tiles for this block when a capped
+// grid covers them (XCD-aware block order; trailing blocks may be empty).
+struct TileRange {
+  int t0, t1;
+};
+__device__ __forceinline__ TileRange tile_range(int ntiles) {
+  const int b = xcd_tile(blockIdx.x, gridDim.x);
+  const int per = (ntiles + (int)gridDim.x - 1) / (int)gridDim.x;
+  const int t0 = b * per;
+  return {t0, t0 + per < ntiles ? t0 + per : ntiles};
+}
+
 template <int NV>
 __device__ __forceinline__ void block_sum(double (&v)[NV], double* s_red) {
 #pragma unroll
@@ -874,26 +886,28 @@ __global__ void __launch_bounds__(kNT) k_pcg_init(PcgArgs A) {
   __shared__ double s_red[8 * kNW];
   const Geo& G = A.G;
   const int P = G.P;
-  const int t = xcd_tile(blockIdx.x, gridDim.x);
+  const TileRange tr = tile_range(A.nbx * A.nby);
   double acc[3] = {0.0, 0.0, 0.0};
-  for (int k = 0; k < kPcgRows; k++) {
-    const int ci = (t % A.nbx) * kTX + (int)(threadIdx.x % kTX);
-    const int cj = (t / A.nbx) * kPcgTY + (int)(threadIdx.x / kTX) + k * (kPcgTY / kPcgRows);
-    if (ci < G.bx && cj < G.by) {
-      size_t m = (size_t)(cj + 1) * P + (ci + 1);
-      const double *r = A.r_old, *d = A.dg;
-      double uc = r[m] / d[m];
-      double w = d[m] * uc - A.cx[m - 1] * (r[m - 1] / d[m - 1]) - A.cx[m] * (r[m + 1] / d[m + 1]) -
-                 A.cy[m - P] * (r[m - P] / d[m - P]) - A.cy[m] * (r[m + P] / d[m + P]);
-      A.w_new[m] = w;
-      A.p[m] = 0.0;
-      A.s_old[m] = 0.0;
-      A.x[m] = 0.0;
-      double V = G.dlx[ci] * G.dly[cj] * (G.cyl ? G.r[cj + 1] : 1.0);
-      double rv = r[m] / V;
-      acc[0] += r[m] * uc;
-      acc[1] += w * uc;
-      acc[2] += rv * rv;
+  for (int t = tr.t0; t < tr.t1; t++) {
+    for (int k = 0; k < kPcgRows; k++) {
+      const int ci = (t % A.nbx) * kTX + (int)(threadIdx.x % kTX);
+      const int cj = (t / A.nbx) * kPcgTY + (int)(threadIdx.x / kTX) + k * (kPcgTY / kPcgRows);
+      if (ci < G.bx && cj < G.by) {
+        size_t m = (size_t)(cj + 1) * P + (ci + 1);
+        const double *r = A.r_old, *d = A.dg;
+        double uc = r[m] / d[m];
+        double w = d[m] * uc - A.cx[m - 1] * (r[m - 1] / d[m - 1]) - A.cx[m] * (r[m + 1] / d[m + 1]) -
+                   A.cy[m - P] * (r[m - P] / d[m - P]) - A.cy[m] * (r[m + P] / d[m + P]);
+        A.w_new[m] = w;
+        A.p[m] = 0.0;
+        A.s_old[m] = 0.0;
+        A.x[m] = 0.0;
+        double V = G.dlx[ci] * G.dly[cj] * (G.cyl ? G.r[cj + 1] : 1.0);
+        double rv = r[m] / V;
+        acc[0] += r[m] * uc;
+        acc[1] += w * uc;
+        acc[2] += rv * rv;
+      }
     }
   }
   write_partials3(acc, A.part_out, s_red);
@@ -931,32 +945,34 @@ __global__ void __launch_bounds__(kNT) k_pcg_it(PcgArgs A) {
 
   const Geo& G = A.G;
   const int P = G.P;
-  const int t = xcd_tile(blockIdx.x, gridDim.x);
+  const TileRange tr = tile_range(A.nbx * A.nby);
   const double *r = A.r_old, *w = A.w_old, *so = A.s_old, *d = A.dg;
   double acc[3] = {0.0, 0.0, 0.0};
-  for (int k = 0; k < kPcgRows; k++) {
-    const int ci = (t % A.nbx) * kTX + (int)(threadIdx.x % kTX);
-    const int cj = (t / A.nbx) * kPcgTY + (int)(threadIdx.x / kTX) + k * (kPcgTY / kPcgRows);
-    if (ci < G.bx && cj < G.by) {
-      size_t m = (size_t)(cj + 1) * P + (ci + 1);
-      // u_{i+1} of this cell and its neighbours (zero outside: r = s = w = 0, d = 1)
-      auto unext = [&](size_t q) { return (r[q] - alpha * (w[q] + beta * so[q])) / d[q]; };
-      double sc_ = w[m] + beta * so[m];
-      double pc = r[m] / d[m] + beta * A.p[m];
-      double rn = r[m] - alpha * sc_;
-      double un = rn / d[m];
-      double wn = d[m] * un - A.cx[m - 1] * unext(m - 1) - A.cx[m] * unext(m + 1) -
-                  A.cy[m - P] * unext(m - P) - A.cy[m] * unext(m + P);
-      A.p[m] = pc;
-      A.x[m] = A.x[m] + alpha * pc;
-      A.s_new[m] = sc_;
-      A.r_new[m] = rn;
-      A.w_new[m] = wn;
-      double V = G.dlx[ci] * G.dly[cj] * (G.cyl ? G.r[cj + 1] : 1.0);
-      double rv = rn / V;
-      acc[0] += rn * un;
-      acc[1] += wn * un;
-      acc[2] += rv * rv;
+  for (int t = tr.t0; t < tr.t1; t++) {
+    for (int k = 0; k < kPcgRows; k++) {
+      const int ci = (t % A.nbx) * kTX + (int)(threadIdx.x % kTX);
+      const int cj = (t / A.nbx) * kPcgTY + (int)(threadIdx.x / kTX) + k * (kPcgTY / kPcgRows);
+      if (ci < G.bx && cj < G.by) {
+        size_t m = (size_t)(cj + 1) * P + (ci + 1);
+        // u_{i+1} of this cell and its neighbours (zero outside: r = s = w = 0, d = 1)
+        auto unext = [&](size_t q) { return (r[q] - alpha * (w[q] + beta * so[q])) / d[q]; };
+        double sc_ = w[m] + beta * so[m];
+        double pc = r[m] / d[m] + beta * A.p[m];
+        double rn = r[m] - alpha * sc_;
+        double un = rn / d[m];
+        double wn = d[m] * un - A.cx[m - 1] * unext(m - 1) - A.cx[m] * unext(m + 1) -
+                    A.cy[m - P] * unext(m - P) - A.cy[m] * unext(m + P);
+        A.p[m] = pc;
+        A.x[m] = A.x[m] + alpha * pc;
+        A.s_new[m] = sc_;
+        A.r_new[m] = rn;
+        A.w_new[m] = wn;
+        double V = G.dlx[ci] * G.dly[cj] * (G.cyl ? G.r[cj + 1] : 1.0);
+        double rv = rn / V;
+        acc[0] += rn * un;
+        acc[1] += wn * un;
+        acc[2] += rv * rv;
+      }
     }
   }
   write_partials3(acc, A.part_out, s_red);
@@ -969,21 +985,23 @@ __global__ void __launch_bounds__(kNT) k_pcg_finish(PcgArgs A) {
   DevScalars* sc = A.sc;
   const Geo& G = A.G;
   const int P = G.P;
-  const int t = xcd_tile(blockIdx.x, gridDim.x);
+  const TileRange tr = tile_range(A.nbx * A.nby);
   const bool ok = sc->done == 1;
   double acc[3] = {0.0, 0.0, 0.0};
-  for (int k = 0; k < kPcgRows; k++) {
-    const int ci = (t % A.nbx) * kTX + (int)(threadIdx.x % kTX);
-    const int cj = (t / A.nbx) * kPcgTY + (int)(threadIdx.x / kTX) + k * (kPcgTY / kPcgRows);
-    if (ci < G.bx && cj < G.by) {
-      size_t m = (size_t)(cj + 1) * P + (ci + 1);
-      const double* x = A.x;
-      double mx = A.dg[m] * x[m] - A.cx[m - 1] * x[m - 1] - A.cx[m] * x[m + 1] -
-                  A.cy[m - P] * x[m - P] - A.cy[m] * x[m + P];
-      double V = G.dlx[ci] * G.dly[cj] * (G.cyl ? G.r[cj + 1] : 1.0);
-      double rr = (A.rhs0[m] - mx) / V;
-      acc[0] += rr * rr;
-      if (ok) A.ph[m] = A.ph[m] + x[m];
+  for (int t = tr.t0; t < tr.t1; t++) {
+    for (int k = 0; k < kPcgRows; k++) {
+      const int ci = (t % A.nbx) * kTX + (int)(threadIdx.x % kTX);
+      const int cj = (t / A.nbx) * kPcgTY + (int)(threadIdx.x / kTX) + k * (kPcgTY / kPcgRows);
+      if (ci < G.bx && cj < G.by) {
+        size_t m = (size_t)(cj + 1) * P + (ci + 1);
+        const double* x = A.x;
+        double mx = A.dg[m] * x[m] - A.cx[m - 1] * x[m - 1] - A.cx[m] * x[m + 1] -
+                    A.cy[m - P] * x[m - P] - A.cy[m] * x[m + P];
+        double V = G.dlx[ci] * G.dly[cj] * (G.cyl ? G.r[cj + 1] : 1.0);
+        double rr = (A.rhs0[m] - mx) / V;
+        acc[0] += rr * rr;
+        if (ok) A.ph[m] = A.ph[m] + x[m];
+      }
     }
   }
   write_partials3(acc, A.part_out, s_red);
@@ -1116,10 +1134,14 @@ __device__ __forceinline__ double gs_mx(const GsTile& s, int b, int a) {
          s.cy[b - 1][a] * s.x[b - 1][a] - s.cy[b][a] * s.x[b + 1][a];
 }
 
-// Down: pre-smooth M x = r on the tile, keep x1, restrict the residual into
+__device__ __forceinline__ int gs_ntiles(const MgLevel& F) {
+  return gs_tiles_x(F) * ((F.ny + kGsTY - 1) / kGsTY);
+}
+
+// Down: pre-smooth M x = r on each tile, keep x1, restrict the residual into
 // the coarse level.  Level 0 first checks convergence from the x/r-update
 // partials (every block identically); their loads are issued together with
-// the tile's.
+// the first tile's.
 __global__ void __launch_bounds__(kNT) k_mg_down(MgArgs A) {
   __shared__ GsTile s;
   __shared__ double s_res[kGsTY][kGsTX / 2];
@@ -1128,11 +1150,11 @@ __global__ void __launch_bounds__(kNT) k_mg_down(MgArgs A) {
   if (sc->done) return;
   const MgLevel &F = A.F, &C = A.C;
   const int ntx = gs_tiles_x(F);
-  const int t = xcd_tile(blockIdx.x, gridDim.x);
-  const int i0 = (t % ntx) * kGsTX, j0 = (t / ntx) * kGsTY;
+  const TileRange tr = tile_range(gs_ntiles(F));
+  if (tr.t0 >= tr.t1) return;  // block 0 always has tiles (it writes the scalars)
   double pv[3];
   if (A.level0) load_partials3(A.part_in, A.nb_in, pv);
-  gs_stage(F, s, i0, j0);
+  gs_stage(F, s, (tr.t0 % ntx) * kGsTX, (tr.t0 / ntx) * kGsTY);
   if (A.level0) {
     double tot[3];
     finish_partials3(pv, s_red, tot);
@@ -1164,35 +1186,42 @@ __global__ void __launch_bounds__(kNT) k_mg_down(MgArgs A) {
       if (stop) return;
     }
   }
-  __syncthreads();
-  // first (red) half-sweep from zero is pointwise: the whole staged halo
-  for (int k = threadIdx.x; k < kGsR * kGsW; k += kNT) {
-    const int b = k / kGsW, a = k % kGsW;
-    s.x[b][a] = ((i0 + a + j0 + b) & 1) == 0 ? s.r[b][a] / s.d[b][a] : 0.0;
-  }
-  __syncthreads();
-  gs_half(s, i0, j0, 1, 3);
-  __syncthreads();
-  gs_half(s, i0, j0, 0, 2);
-  __syncthreads();
-  gs_half(s, i0, j0, 1, 1);
-  __syncthreads();
-  // residual of one horizontal cell pair (half an aggregate) per thread
-  const int cI = threadIdx.x % (kGsTX / 2), fj = threadIdx.x / (kGsTX / 2);
-  double rs = 0.0;
-  for (int q = 0; q < 2; q++) {
-    const int a = kGsH + 2 * cI + q, b = kGsH + fj;
-    const int i = i0 + 2 * cI + q, j = j0 + fj;
-    if (i < F.nx && j < F.ny) {
-      F.x1[mg_at(F, i, j)] = s.x[b][a];
-      rs += s.r[b][a] - gs_mx(s, b, a);
+  for (int t = tr.t0; t < tr.t1; t++) {
+    const int i0 = (t % ntx) * kGsTX, j0 = (t / ntx) * kGsTY;
+    if (t > tr.t0) {
+      __syncthreads();
+      gs_stage(F, s, i0, j0);
     }
-  }
-  s_res[fj][cI] = rs;
-  __syncthreads();
-  if ((fj & 1) == 0) {
-    const int I = i0 / 2 + cI, J = (j0 + fj) / 2;
-    if (I < C.nx && J < C.ny) C.r[mg_at(C, I, J)] = s_res[fj][cI] + s_res[fj + 1][cI];
+    __syncthreads();
+    // first (red) half-sweep from zero is pointwise: the whole staged halo
+    for (int k = threadIdx.x; k < kGsR * kGsW; k += kNT) {
+      const int b = k / kGsW, a = k % kGsW;
+      s.x[b][a] = ((i0 + a + j0 + b) & 1) == 0 ? s.r[b][a] / s.d[b][a] : 0.0;
+    }
+    __syncthreads();
+    gs_half(s, i0, j0, 1, 3);
+    __syncthreads();
+    gs_half(s, i0, j0, 0, 2);
+    __syncthreads();
+    gs_half(s, i0, j0, 1, 1);
+    __syncthreads();
+    // residual of one horizontal cell pair (half an aggregate) per thread
+    const int cI = threadIdx.x % (kGsTX / 2), fj = threadIdx.x / (kGsTX / 2);
+    double rs = 0.0;
+    for (int q = 0; q < 2; q++) {
+      const int a = kGsH + 2 * cI + q, b = kGsH + fj;
+      const int i = i0 + 2 * cI + q, j = j0 + fj;
+      if (i < F.nx && j < F.ny) {
+        F.x1[mg_at(F, i, j)] = s.x[b][a];
+        rs += s.r[b][a] - gs_mx(s, b, a);
+      }
+    }
+    s_res[fj][cI] = rs;
+    __syncthreads();
+    if ((fj & 1) == 0) {
+      const int I = i0 / 2 + cI, J = (j0 + fj) / 2;
+      if (I < C.nx && J < C.ny) C.r[mg_at(C, I, J)] = s_res[fj][cI] + s_res[fj + 1][cI];
+    }
   }
 }
 
@@ -1204,34 +1233,37 @@ __global__ void __launch_bounds__(kNT) k_mg_up(MgArgs A) {
   if (A.sc->done) return;
   const MgLevel &F = A.F, &C = A.C;
   const int ntx = gs_tiles_x(F);
-  const int t = xcd_tile(blockIdx.x, gridDim.x);
-  const int i0 = (t % ntx) * kGsTX, j0 = (t / ntx) * kGsTY;
-  gs_stage(F, s, i0, j0);
-  for (int k = threadIdx.x; k < kGsR * kGsW; k += kNT) {
-    const int b = k / kGsW, a = k % kGsW;
-    const int i = i0 - kGsH + a, j = j0 - kGsH + b;
-    double v = 0.0;
-    if (i >= 0 && i < F.nx && j >= 0 && j < F.ny)
-      v = F.x1[mg_at(F, i, j)] + A.oc * C.e[mg_at(C, i >> 1, j >> 1)];
-    s.x[b][a] = v;
-  }
-  __syncthreads();
-  gs_half(s, i0, j0, 1, 3);
-  __syncthreads();
-  gs_half(s, i0, j0, 0, 2);
-  __syncthreads();
-  gs_half(s, i0, j0, 1, 1);
-  __syncthreads();
-  gs_half(s, i0, j0, 0, 0);
-  __syncthreads();
+  const TileRange tr = tile_range(gs_ntiles(F));
   double acc[3] = {0.0, 0.0, 0.0};
-  for (int k = threadIdx.x; k < kGsTX * kGsTY; k += kNT) {
-    const int i = i0 + k % kGsTX, j = j0 + k / kGsTX;
-    if (i < F.nx && j < F.ny) {
-      const int b = kGsH + k / kGsTX, a = kGsH + k % kGsTX;
-      const double e = s.x[b][a];
-      F.e[mg_at(F, i, j)] = e;
-      acc[0] += s.r[b][a] * e;
+  for (int t = tr.t0; t < tr.t1; t++) {
+    const int i0 = (t % ntx) * kGsTX, j0 = (t / ntx) * kGsTY;
+    if (t > tr.t0) __syncthreads();
+    gs_stage(F, s, i0, j0);
+    for (int k = threadIdx.x; k < kGsR * kGsW; k += kNT) {
+      const int b = k / kGsW, a = k % kGsW;
+      const int i = i0 - kGsH + a, j = j0 - kGsH + b;
+      double v = 0.0;
+      if (i >= 0 && i < F.nx && j >= 0 && j < F.ny)
+        v = F.x1[mg_at(F, i, j)] + A.oc * C.e[mg_at(C, i >> 1, j >> 1)];
+      s.x[b][a] = v;
+    }
+    __syncthreads();
+    gs_half(s, i0, j0, 1, 3);
+    __syncthreads();
+    gs_half(s, i0, j0, 0, 2);
+    __syncthreads();
+    gs_half(s, i0, j0, 1, 1);
+    __syncthreads();
+    gs_half(s, i0, j0, 0, 0);
+    __syncthreads();
+    for (int k = threadIdx.x; k < kGsTX * kGsTY; k += kNT) {
+      const int i = i0 + k % kGsTX, j = j0 + k / kGsTX;
+      if (i < F.nx && j < F.ny) {
+        const int b = kGsH + k / kGsTX, a = kGsH + k % kGsTX;
+        const double e = s.x[b][a];
+        F.e[mg_at(F, i, j)] = e;
+        acc[0] += s.r[b][a] * e;
+      }
     }
   }
   if (A.level0) write_partials3(acc, A.part_out, s_red);
@@ -1413,117 +1445,141 @@ __global__ void __launch_bounds__(1024) k_mg_coarse_cycle(MgCycleArgs A) {
 __global__ void __launch_bounds__(kNT) k_mgpcg_init(MgPcgArgs A) {
   __shared__ double s_red[8 * kNW];
   const Geo& G = A.G;
-  const int t = xcd_tile(blockIdx.x, gridDim.x);
+  const TileRange tr = tile_range(A.nbx * A.nby);
   double acc[3] = {0.0, 0.0, 0.0};
-  for (int k = 0; k < kPcgRows; k++) {
-    const int ci = (t % A.nbx) * kTX + (int)(threadIdx.x % kTX);
-    const int cj = (t / A.nbx) * kPcgTY + (int)(threadIdx.x / kTX) + k * (kPcgTY / kPcgRows);
-    if (ci < G.bx && cj < G.by) {
-      size_t m = (size_t)(cj + 1) * G.P + (ci + 1);
-      A.x[m] = 0.0;
-      A.p_new[m] = 0.0;
-      double V = G.dlx[ci] * G.dly[cj] * (G.cyl ? G.r[cj + 1] : 1.0);
-      double rv = A.r[m] / V;
-      acc[0] += rv * rv;
+  for (int t = tr.t0; t < tr.t1; t++) {
+    for (int k = 0; k < kPcgRows; k++) {
+      const int ci = (t % A.nbx) * kTX + (int)(threadIdx.x % kTX);
+      const int cj = (t / A.nbx) * kPcgTY + (int)(threadIdx.x / kTX) + k * (kPcgTY / kPcgRows);
+      if (ci < G.bx && cj < G.by) {
+        size_t m = (size_t)(cj + 1) * G.P + (ci + 1);
+        A.x[m] = 0.0;
+        A.p_new[m] = 0.0;
+        double V = G.dlx[ci] * G.dly[cj] * (G.cyl ? G.r[cj + 1] : 1.0);
+        double rv = A.r[m] / V;
+        acc[0] += rv * rv;
+      }
     }
   }
   write_partials3(acc, A.part_out, s_red);
 }
 
-// p = u + beta p_old (neighbours recomputed), q = M p, partials of p.q.
-// The cell loads are issued before the reduction that yields beta.
-__global__ void __launch_bounds__(kNT) k_mgpcg_pq(MgPcgArgs A) {
-  __shared__ double s_red[8 * kNW];
-  DevScalars* sc = A.sc;
-  if (sc->done) return;
+// Cells of one 64x8 PCG tile held by a thread (kPcgRows of them).
+struct PqCells {
+  double u[kPcgRows][5], po[kPcgRows][5], cf[kPcgRows][5];
+  size_t m[kPcgRows];
+  bool in[kPcgRows];
+};
+
+__device__ __forceinline__ void pq_load(const MgPcgArgs& A, int t, PqCells& c) {
   const Geo& G = A.G;
   const int P = G.P;
-  const int t = xcd_tile(blockIdx.x, gridDim.x);
-  const double *u = A.u, *po = A.p_old;
-  double pv[3];
-  load_partials3(A.part_in, A.nb_in, pv);
-  double uu[kPcgRows][5], pp[kPcgRows][5], cf[kPcgRows][5];
-  size_t mm[kPcgRows];
-  bool in[kPcgRows];
 #pragma unroll
   for (int k = 0; k < kPcgRows; k++) {
     const int ci = (t % A.nbx) * kTX + (int)(threadIdx.x % kTX);
     const int cj = (t / A.nbx) * kPcgTY + (int)(threadIdx.x / kTX) + k * (kPcgTY / kPcgRows);
-    in[k] = ci < G.bx && cj < G.by;
-    const size_t m = in[k] ? (size_t)(cj + 1) * P + (ci + 1) : (size_t)P + 1;
-    mm[k] = m;
+    c.in[k] = ci < G.bx && cj < G.by;
+    const size_t m = c.in[k] ? (size_t)(cj + 1) * P + (ci + 1) : (size_t)P + 1;
+    c.m[k] = m;
     const size_t nb[5] = {m, m - 1, m + 1, m - P, m + P};
 #pragma unroll
-    for (int s = 0; s < 5; s++) {
-      uu[k][s] = u[nb[s]];
-      pp[k][s] = po[nb[s]];
+    for (int q = 0; q < 5; q++) {
+      c.u[k][q] = A.u[nb[q]];
+      c.po[k][q] = A.p_old[nb[q]];
     }
-    cf[k][0] = A.dg[m];
-    cf[k][1] = A.cx[m - 1];
-    cf[k][2] = A.cx[m];
-    cf[k][3] = A.cy[m - P];
-    cf[k][4] = A.cy[m];
+    c.cf[k][0] = A.dg[m];
+    c.cf[k][1] = A.cx[m - 1];
+    c.cf[k][2] = A.cx[m];
+    c.cf[k][3] = A.cy[m - P];
+    c.cf[k][4] = A.cy[m];
   }
+}
+
+// p = u + beta p_old (neighbours recomputed), q = M p, partials of p.q.
+// The first tile's loads are issued before the reduction that yields beta.
+__global__ void __launch_bounds__(kNT) k_mgpcg_pq(MgPcgArgs A) {
+  __shared__ double s_red[8 * kNW];
+  DevScalars* sc = A.sc;
+  if (sc->done) return;
+  const TileRange tr = tile_range(A.nbx * A.nby);
+  double pv[3];
+  load_partials3(A.part_in, A.nb_in, pv);
+  PqCells c;
+  if (tr.t0 < tr.t1) pq_load(A, tr.t0, c);
   double tot[3];
   finish_partials3(pv, s_red, tot);
   const double gam = tot[0];
   const double beta = A.first ? 0.0 : gam / sc->cg_gam[(A.it + 1) & 1];
   if (blockIdx.x == 0 && threadIdx.x == 0) sc->cg_gam[A.it & 1] = gam;
   double acc[3] = {0.0, 0.0, 0.0};
+  for (int t = tr.t0; t < tr.t1; t++) {
+    if (t > tr.t0) pq_load(A, t, c);
 #pragma unroll
-  for (int k = 0; k < kPcgRows; k++) {
-    if (!in[k]) continue;
-    double pn[5];
+    for (int k = 0; k < kPcgRows; k++) {
+      if (!c.in[k]) continue;
+      double pn[5];
 #pragma unroll
-    for (int s = 0; s < 5; s++) pn[s] = uu[k][s] + beta * pp[k][s];
-    const double q = cf[k][0] * pn[0] - cf[k][1] * pn[1] - cf[k][2] * pn[2] -
-                     cf[k][3] * pn[3] - cf[k][4] * pn[4];
-    A.p_new[mm[k]] = pn[0];
-    A.q[mm[k]] = q;
-    acc[0] += pn[0] * q;
+      for (int q = 0; q < 5; q++) pn[q] = c.u[k][q] + beta * c.po[k][q];
+      const double q = c.cf[k][0] * pn[0] - c.cf[k][1] * pn[1] - c.cf[k][2] * pn[2] -
+                       c.cf[k][3] * pn[3] - c.cf[k][4] * pn[4];
+      A.p_new[c.m[k]] = pn[0];
+      A.q[c.m[k]] = q;
+      acc[0] += pn[0] * q;
+    }
   }
   write_partials3(acc, A.part_out, s_red);
 }
 
-// alpha = gamma / p.q; x += alpha p; r -= alpha q; partials of |r/V|^2.
-// The cell loads are issued before the reduction that yields alpha.
-__global__ void __launch_bounds__(kNT) k_mgpcg_xr(MgPcgArgs A) {
-  __shared__ double s_red[8 * kNW];
-  DevScalars* sc = A.sc;
-  if (sc->done) return;
-  const Geo& G = A.G;
-  const int t = xcd_tile(blockIdx.x, gridDim.x);
-  double pv[3];
-  load_partials3(A.part_in, A.nb_in, pv);
-  double xv[kPcgRows], pn[kPcgRows], rv[kPcgRows], qv[kPcgRows], vol[kPcgRows];
-  size_t mm[kPcgRows];
+struct XrCells {
+  double x[kPcgRows], p[kPcgRows], r[kPcgRows], q[kPcgRows], vol[kPcgRows];
+  size_t m[kPcgRows];
   bool in[kPcgRows];
+};
+
+__device__ __forceinline__ void xr_load(const MgPcgArgs& A, int t, XrCells& c) {
+  const Geo& G = A.G;
 #pragma unroll
   for (int k = 0; k < kPcgRows; k++) {
     const int ci = (t % A.nbx) * kTX + (int)(threadIdx.x % kTX);
     const int cj = (t / A.nbx) * kPcgTY + (int)(threadIdx.x / kTX) + k * (kPcgTY / kPcgRows);
-    in[k] = ci < G.bx && cj < G.by;
-    const int cic = in[k] ? ci : 0, cjc = in[k] ? cj : 0;
+    c.in[k] = ci < G.bx && cj < G.by;
+    const int cic = c.in[k] ? ci : 0, cjc = c.in[k] ? cj : 0;
     const size_t m = (size_t)(cjc + 1) * G.P + (cic + 1);
-    mm[k] = m;
-    xv[k] = A.x[m];
-    pn[k] = A.p_new[m];
-    rv[k] = A.r[m];
-    qv[k] = A.q[m];
-    vol[k] = G.dlx[cic] * G.dly[cjc] * (G.cyl ? G.r[cjc + 1] : 1.0);
+    c.m[k] = m;
+    c.x[k] = A.x[m];
+    c.p[k] = A.p_new[m];
+    c.r[k] = A.r[m];
+    c.q[k] = A.q[m];
+    c.vol[k] = G.dlx[cic] * G.dly[cjc] * (G.cyl ? G.r[cjc + 1] : 1.0);
   }
+}
+
+// alpha = gamma / p.q; x += alpha p; r -= alpha q; partials of |r/V|^2.
+// The first tile's loads are issued before the reduction that yields alpha.
+__global__ void __launch_bounds__(kNT) k_mgpcg_xr(MgPcgArgs A) {
+  __shared__ double s_red[8 * kNW];
+  DevScalars* sc = A.sc;
+  if (sc->done) return;
+  const TileRange tr = tile_range(A.nbx * A.nby);
+  double pv[3];
+  load_partials3(A.part_in, A.nb_in, pv);
+  XrCells c;
+  if (tr.t0 < tr.t1) xr_load(A, tr.t0, c);
   double tot[3];
   finish_partials3(pv, s_red, tot);
   const double alpha = sc->cg_gam[A.it & 1] / tot[0];
   double acc[3] = {0.0, 0.0, 0.0};
+  for (int t = tr.t0; t < tr.t1; t++) {
+    if (t > tr.t0) xr_load(A, t, c);
 #pragma unroll
-  for (int k = 0; k < kPcgRows; k++) {
-    if (!in[k]) continue;
-    A.x[mm[k]] = xv[k] + alpha * pn[k];
-    const double r = rv[k] - alpha * qv[k];
-    A.r[mm[k]] = r;
-    const double rn = r / vol[k];
-    acc[0] += rn * rn;
+    for (int k = 0; k < kPcgRows; k++) {
+      if (!c.in[k]) continue;
+      A.x[c.m[k]] = c.x[k] + alpha * c.p[k];
+      const double r = c.r[k] - alpha * c.q[k];
+      A.r[c.m[k]] = r;
+      const double rn = r / c.vol[k];
+      acc[0] += rn * rn;
+    }
   }
   write_partials3(acc, A.part_out, s_red);
 }
@@ -1598,27 +1654,27 @@ void launch_lapl_assemble(const LaplArgs& a, const FsPhys& K, hipStream_t s) {
   hipLaunchKernelGGL(k_lapl, dim3(a.nbx * a.nby), dim3(kNT), 0, s, a, K);
 }
 void launch_pcg_init(const PcgArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL(k_pcg_init, dim3(a.nbx * a.nby), dim3(kNT), 0, s, a);
+  hipLaunchKernelGGL(k_pcg_init, dim3(capped_grid(a.nbx * a.nby)), dim3(kNT), 0, s, a);
 }
 void launch_pcg_it(const PcgArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL(k_pcg_it, dim3(a.nbx * a.nby), dim3(kNT), 0, s, a);
+  hipLaunchKernelGGL(k_pcg_it, dim3(capped_grid(a.nbx * a.nby)), dim3(kNT), 0, s, a);
 }
 void launch_pcg_finish(const PcgArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL(k_pcg_finish, dim3(a.nbx * a.nby), dim3(kNT), 0, s, a);
-  hipLaunchKernelGGL(k_pcg_tres, dim3(1), dim3(kNT), 0, s, a, a.nbx * a.nby);
+  hipLaunchKernelGGL(k_pcg_finish, dim3(capped_grid(a.nbx * a.nby)), dim3(kNT), 0, s, a);
+  hipLaunchKernelGGL(k_pcg_tres, dim3(1), dim3(kNT), 0, s, a, capped_grid(a.nbx * a.nby));
 }
 static int cells_blocks(int n) { return (n + 255) / 256; }
 void launch_mg_coarsen(const MgArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(k_mg_coarsen, dim3(cells_blocks(a.C.nx * a.C.ny)), dim3(256), 0, s, a);
 }
-int mg_tiles(const MgLevel& F) {
-  return ((F.nx + kGsTX - 1) / kGsTX) * ((F.ny + kGsTY - 1) / kGsTY);
+int mg_grid(const MgLevel& F) {
+  return capped_grid(((F.nx + kGsTX - 1) / kGsTX) * ((F.ny + kGsTY - 1) / kGsTY));
 }
 void launch_mg_down(const MgArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL(k_mg_down, dim3(mg_tiles(a.F)), dim3(kNT), 0, s, a);
+  hipLaunchKernelGGL(k_mg_down, dim3(mg_grid(a.F)), dim3(kNT), 0, s, a);
 }
 void launch_mg_up(const MgArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL(k_mg_up, dim3(mg_tiles(a.F)), dim3(kNT), 0, s, a);
+  hipLaunchKernelGGL(k_mg_up, dim3(mg_grid(a.F)), dim3(kNT), 0, s, a);
 }
 void launch_mg_factor(const MgArgs& a, double* inv, hipStream_t s) {
   hipLaunchKernelGGL(k_mg_factor, dim3(1), dim3(1024), 0, s, a, inv);
@@ -1627,13 +1683,13 @@ void launch_mg_coarse_cycle(const MgCycleArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(k_mg_coarse_cycle, dim3(1), dim3(1024), 0, s, a);
 }
 void launch_mgpcg_init(const MgPcgArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL(k_mgpcg_init, dim3(a.nbx * a.nby), dim3(kNT), 0, s, a);
+  hipLaunchKernelGGL(k_mgpcg_init, dim3(capped_grid(a.nbx * a.nby)), dim3(kNT), 0, s, a);
 }
 void launch_mgpcg_pq(const MgPcgArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL(k_mgpcg_pq, dim3(a.nbx * a.nby), dim3(kNT), 0, s, a);
+  hipLaunchKernelGGL(k_mgpcg_pq, dim3(capped_grid(a.nbx * a.nby)), dim3(kNT), 0, s, a);
 }
 void launch_mgpcg_xr(const MgPcgArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL(k_mgpcg_xr, dim3(a.nbx * a.nby), dim3(kNT), 0, s, a);
+  hipLaunchKernelGGL(k_mgpcg_xr, dim3(capped_grid(a.nbx * a.nby)), dim3(kNT), 0, s, a);
 }
 void launch_copy(const double* a, double* b, size_t n2, hipStream_t s) {
   hipLaunchKernelGGL(k_copy, dim3(256 * 16), dim3(256), 0, s, (const double2*)a, (double2*)b,
