@@ -26,7 +26,7 @@ FC = os.environ.get("FC", "/opt/rocm/bin/amdflang")
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-ffp-contract=off", "-fPIC",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-I" + os.path.join(ROOT, "include"), "-I" + CSRC]
-HIP_SOURCES = ["fsim_api.hip", "fsim_kernels.hip"]
+HIP_SOURCES = ["fsim_api.hip", "fsim_kernels.hip", "fsim_learn.hip"]
 
 
 def build_lib(verbose: bool = False) -> str:

@@ -12,6 +12,7 @@
 
 #include "fsim.h"
 #include "fsim_internal.h"
+#include "fsim_learn.h"
 
 namespace {
 
@@ -149,6 +150,7 @@ struct fs_handle {
   double* d_mg = nullptr;
   double* d_mginv = nullptr;        // dense inverse of the coarsest operator (<= 64 cells)
   size_t mg_first = 0;              // first level of the one-workgroup coarse kernel
+  fs_learn* net = nullptr;          // learned coarse solver (lapl_solver = 2)
   hipGraphExec_t mg_graph = nullptr;  // two MG-PCG iterations (parity 1, then 0)
   Geo G;
   std::vector<double> log;
@@ -428,7 +430,9 @@ size_t mg_lds_first(const fs_handle* h) {
 }
 
 // Partial-sum buffers of the MG-PCG chain: 0 = |r/V|^2, 1 = r.u, 2 = p.q
-void launch_vcycle(fs_handle* h, int it, bool init, int nb_rho) {
+// With `harvest`, the coarse residual after the finest down kernel is
+// appended to its training set (eager use only: the stream is synchronised).
+int launch_vcycle(fs_handle* h, int it, bool init, int nb_rho, fsl::Net* harvest = nullptr) {
   const size_t L = h->mg.size(), f = h->mg_first;
   for (size_t l = 0; l < f; l++) {
     MgArgs a = mg_args(h, l);
@@ -437,6 +441,10 @@ void launch_vcycle(fs_handle* h, int it, bool init, int nb_rho) {
     a.part_in = h->part(0);
     a.nb_in = nb_rho;
     launch_mg_down(a, h->st);
+    if (l == 0 && harvest) {
+      if (sync_scalars(h)) return FS_E_HIP;
+      if (!h->h_sc->done && fsl::net_store_sample(harvest, h->mg[1], h->st) < 0) return FS_E_HIP;
+    }
   }
   MgCycleArgs c;
   memset(&c, 0, sizeof(c));
@@ -456,11 +464,12 @@ void launch_vcycle(fs_handle* h, int it, bool init, int nb_rho) {
     a.part_out = h->part(1);
     launch_mg_up(a, h->st);
   }
+  return 0;
 }
 
 // One MG-PCG iteration: p/q, x/r update, V-cycle (convergence is tested in
 // the V-cycle's first kernel).  Only the parity of `it` matters after it = 0.
-void launch_mg_iteration(fs_handle* h, int it) {
+int launch_mg_iteration(fs_handle* h, int it, fsl::Net* harvest = nullptr) {
   const int nb_pcg = capped_grid(h->nbx * h->pnby);
   MgPcgArgs a = mgpcg_args(h, it);
   a.part_in = h->part(1);
@@ -471,7 +480,44 @@ void launch_mg_iteration(fs_handle* h, int it) {
   a.nb_in = nb_pcg;
   a.part_out = h->part(0);
   launch_mgpcg_xr(a, h->st);
-  launch_vcycle(h, it, false, nb_pcg);
+  return launch_vcycle(h, it, false, nb_pcg, harvest);
+}
+
+// Learned two-grid preconditioner: finest down kernel (pre-smoothing,
+// restriction, and the scale partials in buffer 2, free at this point of the
+// chain), the network as coarse solver, finest up kernel (prolongation without
+// over-correction, post-smoothing, partials of r.z and z.q).
+int launch_learned_cycle(fs_handle* h, int it, bool init, int nb_rho) {
+  MgArgs a = mg_args(h, 0);
+  a.it = it;
+  a.init = init;
+  a.part_in = h->part(0);
+  a.nb_in = nb_rho;
+  a.part_scale = h->part(2);
+  launch_mg_down(a, h->st);
+  if (fsl::net_apply(&h->net->net, h->mg[1], h->part(2), mg_grid(h->mg[0]), h->d_sc, h->st))
+    return FS_E_HIP;
+  MgArgs u = mg_args(h, 0);
+  u.part_out = h->part(1);
+  u.oc = 1.0;
+  u.q = init ? nullptr : h->pois(PO_MQ);
+  launch_mg_up(u, h->st);
+  return 0;
+}
+
+int launch_learned_iteration(fs_handle* h, int it) {
+  const int nb_pcg = capped_grid(h->nbx * h->pnby);
+  MgPcgArgs a = mgpcg_args(h, it);
+  a.flex = 1;
+  a.part_in = h->part(1);
+  a.nb_in = mg_grid(h->mg[0]);
+  a.part_out = h->part(2);
+  launch_mgpcg_pq(a, h->st);
+  a.part_in = h->part(2);
+  a.nb_in = nb_pcg;
+  a.part_out = h->part(0);
+  launch_mgpcg_xr(a, h->st);
+  return launch_learned_cycle(h, it, false, nb_pcg);
 }
 
 // Capture two iterations (parity 1 then 0) once per handle; every buffer is
@@ -479,16 +525,14 @@ void launch_mg_iteration(fs_handle* h, int it) {
 int make_mg_graph(fs_handle* h) {
   hipGraph_t g = nullptr;
   HIP_OK(hipStreamBeginCapture(h->st, hipStreamCaptureModeThreadLocal));
-  launch_mg_iteration(h, 1);
-  launch_mg_iteration(h, 2);
+  if (launch_mg_iteration(h, 1) || launch_mg_iteration(h, 2)) return FS_E_HIP;
   HIP_OK(hipStreamEndCapture(h->st, &g));
   HIP_OK(hipGraphInstantiate(&h->mg_graph, g, nullptr, nullptr, 0));
   HIP_OK(hipGraphDestroy(g));
   return 0;
 }
 
-int lapl_solve_impl(fs_handle* h, int* lerr, fs_solve_stats* stats) {
-  auto t0 = std::chrono::steady_clock::now();
+void launch_assembly(fs_handle* h) {
   LaplArgs a;
   memset(&a, 0, sizeof(a));
   int l1 = h->alias ? h->bufA() : h->bufL1();
@@ -510,13 +554,37 @@ int lapl_solve_impl(fs_handle* h, int* lerr, fs_solve_stats* stats) {
     a.bdbg = h->d_dbg;
     a.adbg = h->d_dbg + (size_t)h->bx * h->by;
   }
-  time_mark(h, 4);
   launch_lapl_assemble(a, h->K, h->st);
+}
+
+int lapl_solve_impl(fs_handle* h, int* lerr, fs_solve_stats* stats) {
+  auto t0 = std::chrono::steady_clock::now();
+  time_mark(h, 4);
+  launch_assembly(h);
   HIP_OK(hipGetLastError());
   time_mark(h, 5);
   const int chunk = h->p.lapl_chunk > 0 ? h->p.lapl_chunk : 32;
   int it = 0;
-  if (h->p.lapl_solver == 1 && h->mg.size() > 1) {
+  if (h->p.lapl_solver == 2) {
+    // flexible CG with the learned two-grid preconditioner (eager launches)
+    if (!h->net || h->mg.size() < 2)
+      return fail(FS_E_ARG, "learned solver: no network attached or grid of <= 64 cells%s");
+    launch_mg_coarsen(mg_args(h, 0), h->st);
+    MgPcgArgs ia = mgpcg_args(h, 1);
+    ia.part_out = h->part(0);
+    launch_mgpcg_init(ia, h->st);
+    const int nb_pcg = capped_grid(h->nbx * h->pnby);
+    if (launch_learned_cycle(h, 0, true, nb_pcg)) return FS_E_HIP;
+    const int lchunk = chunk > 8 ? chunk / 4 : chunk;
+    for (;;) {
+      for (int k = 0; k < lchunk; k++, it++)
+        if (launch_learned_iteration(h, it)) return FS_E_HIP;
+      HIP_OK(hipGetLastError());
+      if (sync_scalars(h)) return FS_E_HIP;
+      if (h->h_sc->status < 0) return check_status(h);
+      if (h->h_sc->done) break;
+    }
+  } else if (h->p.lapl_solver == 1 && h->mg.size() > 1) {
     // multigrid-preconditioned PCG (grids of <= 64 cells use Jacobi-PCG)
     for (size_t l = 0; l + 1 < h->mg.size(); l++) launch_mg_coarsen(mg_args(h, l), h->st);
     launch_mg_factor(mg_args(h, h->mg.size() - 1), h->d_mginv, h->st);
@@ -524,8 +592,7 @@ int lapl_solve_impl(fs_handle* h, int* lerr, fs_solve_stats* stats) {
     ia.part_out = h->part(0);
     launch_mgpcg_init(ia, h->st);
     const int nb_pcg = capped_grid(h->nbx * h->pnby);
-    launch_vcycle(h, 0, true, nb_pcg);
-    launch_mg_iteration(h, 0);
+    if (launch_vcycle(h, 0, true, nb_pcg) || launch_mg_iteration(h, 0)) return FS_E_HIP;
     it = 1;
     if (h->p.use_graphs && !h->mg_graph && make_mg_graph(h)) return FS_E_HIP;
     const int mg_chunk = chunk > 8 ? chunk / 4 : chunk;
@@ -533,7 +600,8 @@ int lapl_solve_impl(fs_handle* h, int* lerr, fs_solve_stats* stats) {
       if (h->p.use_graphs) {
         for (int k = 0; k < mg_chunk; k += 2, it += 2) HIP_OK(hipGraphLaunch(h->mg_graph, h->st));
       } else {
-        for (int k = 0; k < mg_chunk; k++, it++) launch_mg_iteration(h, it);
+        for (int k = 0; k < mg_chunk; k++, it++)
+          if (launch_mg_iteration(h, it)) return FS_E_HIP;
       }
       HIP_OK(hipGetLastError());
       if (sync_scalars(h)) return FS_E_HIP;
@@ -1257,8 +1325,111 @@ int fs_get_timers(const fs_handle* h, double* out4) {
 
 int fs_set_solver(fs_handle* h, int32_t solver) {
   if (!h) return fail(FS_E_ARG, "null handle%s");
-  if (solver < 0 || solver > 1) return fail(FS_E_ARG, "unknown Poisson solver%s");
+  if (solver < 0 || solver > 2) return fail(FS_E_ARG, "unknown Poisson solver%s");
   h->p.lapl_solver = solver;
+  return 0;
+}
+
+int fs_learn_create(int32_t capacity, int32_t H, int32_t W, uint64_t seed, fs_learn** out) {
+  if (!out || capacity < 0 || H <= 0 || W <= 0 || H % fsl::kAlign || W % fsl::kAlign)
+    return fail(FS_E_ARG, "bad argument (sides must be positive multiples of 8)%s");
+  fs_learn* n = new fs_learn();
+  HIP_OK(hipStreamCreate(&n->net.st));
+  if (fsl::net_init(&n->net, capacity, H, W, seed, n->net.st)) {
+    fsl::net_free(&n->net);
+    delete n;
+    return fail(FS_E_HIP, "out of device memory for the network%s");
+  }
+  *out = n;
+  return 0;
+}
+
+int fs_learn_destroy(fs_learn* n) {
+  if (!n) return 0;
+  (void)hipStreamSynchronize(n->net.st);
+  fsl::net_free(&n->net);
+  (void)hipStreamDestroy(n->net.st);
+  delete n;
+  return 0;
+}
+
+int64_t fs_learn_nparams(const fs_learn* n) { return n ? (int64_t)n->net.nparam : -1; }
+
+int fs_learn_get_weights(const fs_learn* n, float* out) {
+  if (!n || !out) return fail(FS_E_ARG, "null argument%s");
+  HIP_OK(hipMemcpy(out, n->net.w, n->net.nparam * sizeof(float), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int fs_learn_set_weights(fs_learn* n, const float* in) {
+  if (!n || !in) return fail(FS_E_ARG, "null argument%s");
+  HIP_OK(hipMemcpy(n->net.w, in, n->net.nparam * sizeof(float), hipMemcpyHostToDevice));
+  fsl::net_refresh(&n->net);
+  HIP_OK(hipStreamSynchronize(n->net.st));
+  return 0;
+}
+
+int fs_learn_samples(const fs_learn* n) { return n ? n->net.count : -1; }
+
+int fs_learn_harvest(fs_learn* n, fs_handle* h, int32_t* added) {
+  if (!n || !h) return fail(FS_E_ARG, "null argument%s");
+  if (h->mg.size() < 2) return fail(FS_E_ARG, "grid of <= 64 cells has no coarse level%s");
+  if (h->mg[1].nx > n->net.W || h->mg[1].ny > n->net.H)
+    return fail(FS_E_ARG, "coarse grid larger than the training grid%s");
+  const int before = n->net.count;
+  HIP_OK(hipStreamSynchronize(n->net.st));
+  launch_assembly(h);
+  for (size_t l = 0; l + 1 < h->mg.size(); l++) launch_mg_coarsen(mg_args(h, l), h->st);
+  launch_mg_factor(mg_args(h, h->mg.size() - 1), h->d_mginv, h->st);
+  MgPcgArgs ia = mgpcg_args(h, 1);
+  ia.part_out = h->part(0);
+  launch_mgpcg_init(ia, h->st);
+  const int nb_pcg = capped_grid(h->nbx * h->pnby);
+  if (launch_vcycle(h, 0, true, nb_pcg, &n->net)) return FS_E_HIP;
+  for (int it = 0; it < h->h_sc->maxit; it++) {
+    if (launch_mg_iteration(h, it, &n->net)) return FS_E_HIP;
+    if (h->h_sc->done || n->net.count >= n->net.cap) break;
+  }
+  HIP_OK(hipStreamSynchronize(h->st));
+  HIP_OK(hipGetLastError());
+  if (added) *added = n->net.count - before;
+  return 0;
+}
+
+int fs_learn_train(fs_learn* n, int32_t steps, int32_t batch, float lr, uint64_t seed,
+                   double* loss) {
+  if (!n || steps < 0 || batch <= 0) return fail(FS_E_ARG, "bad argument%s");
+  if (n->net.count == 0) return fail(FS_E_ARG, "empty training set%s");
+  std::vector<int> idx(batch);
+  uint64_t s = seed;
+  for (int k = 0; k < steps; k++) {
+    for (int b = 0; b < batch; b++) {
+      uint64_t z = (s += 0x9e3779b97f4a7c15ull);
+      z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+      z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+      idx[b] = (int)((z ^ (z >> 31)) % (uint64_t)n->net.count);
+    }
+    double l = 0.0;
+    if (fsl::train_step(&n->net, idx.data(), batch, lr, &l))
+      return fail(FS_E_HIP, "training step failed (batch too large?)%s");
+    if (loss) loss[k] = l;
+  }
+  return 0;
+}
+
+int fs_learn_loss_grad(fs_learn* n, int32_t B, int32_t H, int32_t W, const float* in5,
+                       const float* rhat, const float* op, double* loss, float* grad, float* out) {
+  if (!n || !in5 || !rhat || !op || !loss || B <= 0 || H <= 0 || W <= 0 || H % fsl::kAlign ||
+      W % fsl::kAlign)
+    return fail(FS_E_ARG, "bad argument%s");
+  if (fsl::loss_grad(&n->net, B, H, W, in5, rhat, op, loss, grad, out))
+    return fail(FS_E_HIP, "network evaluation failed%s");
+  return 0;
+}
+
+int fs_learn_attach(fs_handle* h, fs_learn* n) {
+  if (!h) return fail(FS_E_ARG, "null handle%s");
+  h->net = n;
   return 0;
 }
 

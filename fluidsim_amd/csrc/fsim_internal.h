@@ -18,6 +18,8 @@ struct DevScalars {
   double ts;
   // Poisson / PCG
   double cg_gam[2], cg_alpha[2], bnorm2, resid, tol, tres;
+  double cg_pq[2];   // p.q of the last two iterations (flexible CG)
+  double nn_scale;   // input scale of the learned coarse solver
   int32_t iters, done, maxit, pad_i;
   int32_t status, accept, attempts, replay_idx;
   int32_t replay_n, pad_j[3];
@@ -131,6 +133,8 @@ struct MgArgs {
   double* part_out;      // level-0 up: partials of gamma = r.u
   DevScalars* sc;
   Geo G;                 // level-0 metrics (volumes for the residual norm)
+  const double* q;       // level-0 up, flexible CG: q = M p of the last update (z.q partials)
+  double* part_scale;    // level-0 down, learned solver: partials of sum r_c^2 / d_c
   double oc;
   int32_t it, level0, init, pad_;
 };
@@ -172,7 +176,7 @@ struct MgPcgArgs {
   double* part_out;
   DevScalars* sc;
   Geo G;
-  int32_t nbx, nby, first, pad_;
+  int32_t nbx, nby, first, flex;  // flex: Polak-Ribiere beta (flexible CG)
 };
 
 constexpr int kTX = 64;  // tile width (one wave64 per tile row)

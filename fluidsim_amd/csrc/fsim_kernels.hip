@@ -1151,7 +1151,11 @@ __global__ void __launch_bounds__(kNT) k_mg_down(MgArgs A) {
   const MgLevel &F = A.F, &C = A.C;
   const int ntx = gs_tiles_x(F);
   const TileRange tr = tile_range(gs_ntiles(F));
-  if (tr.t0 >= tr.t1) return;  // block 0 always has tiles (it writes the scalars)
+  double sacc[3] = {0.0, 0.0, 0.0};  // sum r_c^2 / d_c for the learned coarse solver
+  if (tr.t0 >= tr.t1) {  // block 0 always has tiles (it writes the scalars)
+    if (A.part_scale) write_partials3(sacc, A.part_scale, s_red);
+    return;
+  }
   double pv[3];
   if (A.level0) load_partials3(A.part_in, A.nb_in, pv);
   gs_stage(F, s, (tr.t0 % ntx) * kGsTX, (tr.t0 / ntx) * kGsTY);
@@ -1220,9 +1224,15 @@ __global__ void __launch_bounds__(kNT) k_mg_down(MgArgs A) {
     __syncthreads();
     if ((fj & 1) == 0) {
       const int I = i0 / 2 + cI, J = (j0 + fj) / 2;
-      if (I < C.nx && J < C.ny) C.r[mg_at(C, I, J)] = s_res[fj][cI] + s_res[fj + 1][cI];
+      if (I < C.nx && J < C.ny) {
+        const size_t mc = mg_at(C, I, J);
+        const double rc = s_res[fj][cI] + s_res[fj + 1][cI];
+        C.r[mc] = rc;
+        if (A.part_scale) sacc[0] += rc * rc / C.d[mc];
+      }
     }
   }
+  if (A.part_scale) write_partials3(sacc, A.part_scale, s_red);
 }
 
 // Up: x2 = x1 + oc * e_c(aggregate) on the staged halo, post-smooth, write e.
@@ -1261,8 +1271,10 @@ __global__ void __launch_bounds__(kNT) k_mg_up(MgArgs A) {
       if (i < F.nx && j < F.ny) {
         const int b = kGsH + k / kGsTX, a = kGsH + k % kGsTX;
         const double e = s.x[b][a];
-        F.e[mg_at(F, i, j)] = e;
+        const size_t m = mg_at(F, i, j);
+        F.e[m] = e;
         acc[0] += s.r[b][a] * e;
+        if (A.q) acc[1] += A.q[m] * e;
       }
     }
   }
@@ -1509,7 +1521,11 @@ __global__ void __launch_bounds__(kNT) k_mgpcg_pq(MgPcgArgs A) {
   double tot[3];
   finish_partials3(pv, s_red, tot);
   const double gam = tot[0];
-  const double beta = A.first ? 0.0 : gam / sc->cg_gam[(A.it + 1) & 1];
+  // Fletcher-Reeves form for a fixed preconditioner; Polak-Ribiere for a
+  // varying one: beta = z.(r - r_old)/gamma_old = -(z.q_old)/(p.q)_old
+  const double beta = A.first  ? 0.0
+                      : A.flex ? -tot[1] / sc->cg_pq[(A.it + 1) & 1]
+                               : gam / sc->cg_gam[(A.it + 1) & 1];
   if (blockIdx.x == 0 && threadIdx.x == 0) sc->cg_gam[A.it & 1] = gam;
   double acc[3] = {0.0, 0.0, 0.0};
   for (int t = tr.t0; t < tr.t1; t++) {
@@ -1568,6 +1584,7 @@ __global__ void __launch_bounds__(kNT) k_mgpcg_xr(MgPcgArgs A) {
   double tot[3];
   finish_partials3(pv, s_red, tot);
   const double alpha = sc->cg_gam[A.it & 1] / tot[0];
+  if (blockIdx.x == 0 && threadIdx.x == 0) sc->cg_pq[A.it & 1] = tot[0];
   double acc[3] = {0.0, 0.0, 0.0};
   for (int t = tr.t0; t < tr.t1; t++) {
     if (t > tr.t0) xr_load(A, t, c);

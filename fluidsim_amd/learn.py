@@ -1,0 +1,97 @@
+"""The learned Poisson preconditioner (C-ABI fs_learn_*, declared in
+include/fsim.h).
+
+A 4-level U-Net (16/32/48/64 channels, 3x3 kernels, BF16 weights and
+activations, FP32 accumulation on the MFMA units) is the coarse solver of a
+two-grid preconditioner inside flexible CG (``lapl_solver = 2``).  It is
+trained on the device, by the library's own forward/backward kernels, with the
+A-norm energy loss over coarse residuals harvested from the simulator's
+multigrid-preconditioned solves.  The plain preconditioners stay available
+(``Discharge.set_solver(0 | 1)``) and every solve, learned or not, runs to the
+same FP64 residual.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import lib as _L
+
+
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class LearnedPreconditioner:
+    def __init__(self, capacity: int, H: int, W: int, seed: int = 0):
+        """Training set of `capacity` samples on an H x W coarse grid
+        (multiples of 8; the coarse grid of an n x m discharge is n/2 x m/2)."""
+        L = _L.load()
+        h = C.c_void_p()
+        _L.check(L.fs_learn_create(int(capacity), int(H), int(W), int(seed), C.byref(h)))
+        self.h, self.H, self.W = h, H, W
+
+    def close(self):
+        if self.h:
+            _L.load().fs_learn_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def nparams(self) -> int:
+        return int(_L.load().fs_learn_nparams(self.h))
+
+    @property
+    def samples(self) -> int:
+        return int(_L.load().fs_learn_samples(self.h))
+
+    def weights(self) -> np.ndarray:
+        w = np.zeros(self.nparams, dtype=np.float32)
+        _L.check(_L.load().fs_learn_get_weights(self.h, _fp(w)))
+        return w
+
+    def set_weights(self, w) -> None:
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        assert w.size == self.nparams
+        _L.check(_L.load().fs_learn_set_weights(self.h, _fp(w)))
+
+    def harvest(self, discharge) -> int:
+        """Samples from one multigrid solve of the discharge's current system."""
+        n = C.c_int32(0)
+        _L.check(_L.load().fs_learn_harvest(self.h, discharge.h, C.byref(n)))
+        return n.value
+
+    def train(self, steps: int, batch: int = 8, lr: float = 1e-3, seed: int = 0) -> np.ndarray:
+        loss = np.zeros(max(steps, 1))
+        _L.check(_L.load().fs_learn_train(self.h, int(steps), int(batch), float(lr), int(seed),
+                                         _dp(loss)))
+        return loss[:steps]
+
+    def loss_grad(self, in5, rhat, op):
+        """Loss, parameter gradient and e_hat of a host batch:
+        in5 [B, H, W, 5], rhat [B, H, W], op [B, 3, H, W]."""
+        in5 = np.ascontiguousarray(in5, dtype=np.float32)
+        rhat = np.ascontiguousarray(rhat, dtype=np.float32)
+        op = np.ascontiguousarray(op, dtype=np.float32)
+        B, H, W, _ = in5.shape
+        loss = np.zeros(1)
+        grad = np.zeros(self.nparams, dtype=np.float32)
+        out = np.zeros((B, H, W), dtype=np.float32)
+        _L.check(_L.load().fs_learn_loss_grad(self.h, B, H, W, _fp(in5), _fp(rhat), _fp(op),
+                                             _dp(loss), _fp(grad), _fp(out)))
+        return float(loss[0]), grad, out
+
+    def attach(self, discharge) -> None:
+        """Use this network in the discharge's solves with lapl_solver = 2."""
+        _L.check(_L.load().fs_learn_attach(discharge.h, self.h))
+        discharge._net = self  # keep the network alive while attached

@@ -25,7 +25,10 @@ EXPORTS = [
     "fs_ptcl_step", "fs_lapl_solve", "fs_accept", "fs_reject", "fs_run", "fs_set_replay",
     "fs_log_count", "fs_get_log", "fs_clear_log", "fs_rhs_sweep", "fs_circ_step",
     "fs_lapl_assemble", "fs_constants", "fs_eval_fits", "fs_eval_flux", "fs_eval_dfdph",
-    "fs_get_timers", "fs_set_timing", "fs_set_solver", "fs_time_kernel", "fs_copy_bench", "fs_last_error",
+    "fs_get_timers", "fs_set_timing", "fs_set_solver", "fs_time_kernel", "fs_copy_bench",
+    "fs_learn_create", "fs_learn_destroy", "fs_learn_nparams", "fs_learn_get_weights",
+    "fs_learn_set_weights", "fs_learn_samples", "fs_learn_harvest", "fs_learn_train",
+    "fs_learn_loss_grad", "fs_learn_attach", "fs_last_error",
 ]
 
 
@@ -77,6 +80,7 @@ def load():
     P = C.c_void_p
     D = C.POINTER(C.c_double)
     I = C.POINTER(C.c_int32)
+    F = C.POINTER(C.c_float)
     sig = {
         "fs_create": [C.POINTER(GridDesc), C.POINTER(Params), C.POINTER(P)],
         "fs_destroy": [P],
@@ -108,6 +112,16 @@ def load():
         "fs_set_solver": [P, C.c_int32],
         "fs_time_kernel": [P, C.c_int32, C.c_int32, D],
         "fs_copy_bench": [C.c_int64, C.c_int32, D],
+        "fs_learn_create": [C.c_int32, C.c_int32, C.c_int32, C.c_uint64, C.POINTER(P)],
+        "fs_learn_destroy": [P],
+        "fs_learn_nparams": [P],
+        "fs_learn_get_weights": [P, F],
+        "fs_learn_set_weights": [P, F],
+        "fs_learn_samples": [P],
+        "fs_learn_harvest": [P, P, I],
+        "fs_learn_train": [P, C.c_int32, C.c_int32, C.c_float, C.c_uint64, D],
+        "fs_learn_loss_grad": [P, C.c_int32, C.c_int32, C.c_int32, F, F, F, D, F, F],
+        "fs_learn_attach": [P, P],
         "fs_last_error": [],
     }
     for name, args in sig.items():
@@ -115,6 +129,7 @@ def load():
         f.argtypes = args
         f.restype = C.c_int
     L.fs_last_error.restype = C.c_char_p
+    L.fs_learn_nparams.restype = C.c_int64
     _lib = L
     return L
 

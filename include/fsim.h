@@ -74,7 +74,8 @@ typedef struct fs_params {
   int32_t lapl_maxit;
   int32_t lapl_chunk;     /* PCG iterations launched between convergence polls */
   int32_t debug_keep_k;   /* keep all five stage slopes for fs_get_field(FS_K*) */
-  int32_t lapl_solver;    /* 0 Jacobi-PCG, 1 multigrid-preconditioned PCG */
+  int32_t lapl_solver;    /* 0 Jacobi-PCG, 1 multigrid-preconditioned PCG, 2 learned
+                             two-grid flexible CG (fs_learn_attach) */
   int32_t use_graphs;     /* replay the PCG iterations from a captured HIP graph */
   int32_t pad_;
 } fs_params;
@@ -203,6 +204,42 @@ int fs_time_kernel(fs_handle* h, int32_t which, int32_t n, double* avg_ms);
  * HBM reference next to the kernel rooflines, and the calibration kernel for
  * the FETCH_SIZE / WRITE_SIZE counters. */
 int fs_copy_bench(int64_t bytes, int32_t reps, double* gbps);
+
+/* ---- learned Poisson preconditioner ------------------------------------
+ * No counterpart in the reference (its Poisson solve is PETSc's KSP with the
+ * default preconditioner, lapl_lib.F90:50-54, 104).  A 4-level convolutional
+ * U-Net (16/32/48/64 channels, 3x3 kernels, BF16 weights and activations,
+ * FP32 accumulation) serves as the coarse solver of a two-grid preconditioner
+ * (red-black Gauss-Seidel smoothing, 2x2 aggregation) inside flexible CG:
+ * lapl_solver = 2.  Its first application, to the right-hand side, is the
+ * solve's warm start.  It is trained on the device with the A-norm energy
+ * loss over residuals harvested from the simulator's own solves. */
+typedef struct fs_learn fs_learn;
+
+/* Training-set capacity (samples) and the coarse grid of the samples
+ * (H x W, multiples of 8; smaller grids are zero-padded). */
+int fs_learn_create(int32_t capacity, int32_t H, int32_t W, uint64_t seed, fs_learn** out);
+int fs_learn_destroy(fs_learn* n);
+int64_t fs_learn_nparams(const fs_learn* n);
+int fs_learn_get_weights(const fs_learn* n, float* out);   /* FP32 masters */
+int fs_learn_set_weights(fs_learn* n, const float* in);
+int fs_learn_samples(const fs_learn* n);
+/* Solve the handle's current Poisson system with the multigrid-preconditioned
+ * CG, one iteration at a time, appending the restricted, pre-smoothed coarse
+ * residual of every iteration (with the coarse operator) to the training set.
+ * phi is left unchanged. */
+int fs_learn_harvest(fs_learn* n, fs_handle* h, int32_t* added);
+/* `steps` Adam steps on random batches of `batch` samples; loss[steps]. */
+int fs_learn_train(fs_learn* n, int32_t steps, int32_t batch, float lr, uint64_t seed,
+                   double* loss);
+/* Loss and parameter gradient of a given batch (checks against a CPU model):
+ * in5[B][H][W][5] = r/(s d) and the four coupling ratios, rhat[B][H][W],
+ * op[B][3][H][W] = d, cx, cy.  grad[nparams], out[B][H][W] (e_hat) may be NULL. */
+int fs_learn_loss_grad(fs_learn* n, int32_t B, int32_t H, int32_t W, const float* in5,
+                       const float* rhat, const float* op, double* loss, float* grad, float* out);
+/* Use the network in the handle's later solves when lapl_solver = 2 (NULL
+ * detaches); the network must outlive the attachment. */
+int fs_learn_attach(fs_handle* h, fs_learn* n);
 
 const char* fs_last_error(void);
 
