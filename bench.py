@@ -49,7 +49,13 @@ def gpu_run(args):
     iters = [float(r[7]) for r in log if r[6] >= 0]
     elapsed = t1 - t0
     N = args.n * args.n
-    poisson = poisson_compare(d, args.solve_reps)
+    net, train = (None, None) if args.no_learned else learned_setup(args)
+    poisson = poisson_compare(d, args.solve_reps, net)
+    if net is not None:
+        poisson["learned"]["train"] = train
+        if args.learned_n2:
+            poisson["learned_at"] = {f"{args.learned_n2}x{args.learned_n2}":
+                                     solve_at(args.learned_n2, net, args.solve_reps)}
     # per-launch kernel durations on the live state (HIP events on the stream)
     k = {f"stage{s}": d.time_kernel(s, 50) for s in (1, 2, 3, 4, 5)}
     k["assembly"] = d.time_kernel(10, 50)
@@ -64,15 +70,38 @@ def gpu_run(args):
                 poisson=poisson)
 
 
-def poisson_compare(d, reps):
+def learned_setup(args):
+    """Harvest residuals from seeded runs of the 512^2 pulse workload, train the
+    network on the device (the library's own kernels) and return it."""
+    from fluidsim_amd.learn import LearnedPreconditioner, harvest_runs, train_schedule
+
+    net = LearnedPreconditioner(args.learn_samples, 64, 64, seed=1)
+    th = harvest_runs(net, 512, seeds=(0, 1, 2, 3), samples=args.learn_samples)
+    steps = args.learn_steps
+    loss, tt = train_schedule(net, ((steps // 2, 16, 1e-3), (steps - steps // 2, 16, 2.5e-4)))
+    return net, {"samples": net.samples, "harvest_s": th, "steps": steps, "batch": 16,
+                 "train_s": tt, "loss_first": float(loss[:50].mean()),
+                 "loss_last": float(loss[-50:].mean())}
+
+
+def poisson_compare(d, reps, net=None):
     """Time-to-1e-10 of the same assembled system (the final state of the run)
     with each preconditioner: wall seconds of fs_lapl_solve (assembly + solve
     + the true-residual check), median over `reps`, from the same phi."""
     import numpy as np
 
+    from fluidsim_amd.learn import LearnedPreconditioner
+
     ph0 = d.get("ph")
     out, sols = {}, {}
-    for name, solver in (("mg_pcg", 1), ("jacobi_pcg", 0)):
+    cases = [("mg_pcg", 1, None), ("jacobi_pcg", 0, None)]
+    if net is not None:
+        cases += [("learned", 2, net), ("learned_level_without_network", 2, False)]
+    for name, solver, nn in cases:
+        if nn:
+            nn.attach(d)
+        elif nn is False:
+            LearnedPreconditioner.detach(d)
         d.set_solver(solver)
         secs, its, res = [], [], []
         for _ in range(reps):
@@ -86,11 +115,27 @@ def poisson_compare(d, reps):
                      "true_rel_residual": float(max(res)), "converged": lerr == 0}
     d.set("ph", 1, ph0)
     d.set_solver(1)
-    a, b = sols["mg_pcg"][1:-1, 1:-1] - ph0[1:-1, 1:-1], sols["jacobi_pcg"][1:-1, 1:-1] - ph0[1:-1, 1:-1]
-    out["correction_rel_l2_mg_vs_jacobi"] = float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    b = sols["jacobi_pcg"][1:-1, 1:-1] - ph0[1:-1, 1:-1]
+    for name in sols:
+        if name != "jacobi_pcg":
+            a = sols[name][1:-1, 1:-1] - ph0[1:-1, 1:-1]
+            out[name]["correction_rel_l2_vs_jacobi"] = float(np.linalg.norm(a - b) / np.linalg.norm(b))
     out["plain_over_mg_time"] = out["jacobi_pcg"]["ms_median"] / out["mg_pcg"]["ms_median"]
-    out["learned"] = None
+    if net is not None:
+        out["plain_over_learned_time"] = out["jacobi_pcg"]["ms_median"] / out["learned"]["ms_median"]
+    else:
+        out["learned"] = None
     return out
+
+
+def solve_at(n, net, reps):
+    """The same comparison on the final system of a short n x n pulse run."""
+    from fluidsim_amd import Discharge, configs
+
+    d = Discharge(configs.config1(n))
+    d.init_gaussian(seed=0)
+    d.run(15)
+    return poisson_compare(d, reps, net)
 
 
 def cpu_baseline(args):
@@ -139,6 +184,11 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--solve-reps", type=int, default=3)
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned preconditioner")
+    ap.add_argument("--learn-samples", type=int, default=20000)
+    ap.add_argument("--learn-steps", type=int, default=3000)
+    ap.add_argument("--learned-n2", type=int, default=1024,
+                    help="second grid side for the learned/plain comparison (0: none)")
     ap.add_argument("--config", type=int, default=1, choices=(1, 2),
                     help="1: 512^2 pulse (the metric's workload); 2: 2048^2 DC, 5 cm, 1 Torr")
     args = ap.parse_args()
@@ -146,6 +196,7 @@ def main():
         args.n = 2048 if args.config == 2 else 512
     if args.config == 2:
         args.no_cpu = True  # a CPU step at 2048^2 takes about a minute; not sampled
+        args.no_learned = True
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU: one discharge per device; "
                                    "multi-GPU sharding is a design note only",

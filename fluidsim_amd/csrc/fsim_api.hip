@@ -152,6 +152,8 @@ struct fs_handle {
   size_t mg_first = 0;              // first level of the one-workgroup coarse kernel
   fs_learn* net = nullptr;          // learned coarse solver (lapl_solver = 2)
   hipGraphExec_t mg_graph = nullptr;  // two MG-PCG iterations (parity 1, then 0)
+  hipGraphExec_t learn_graph = nullptr;  // two learned-FCG iterations
+  const void* learn_graph_key = nullptr; // the network (or the handle: none) it was made for
   Geo G;
   std::vector<double> log;
   bool timing = false;
@@ -430,10 +432,19 @@ size_t mg_lds_first(const fs_handle* h) {
 }
 
 // Partial-sum buffers of the MG-PCG chain: 0 = |r/V|^2, 1 = r.u, 2 = p.q
-// With `harvest`, the coarse residual after the finest down kernel is
+// The level whose equation the learned coarse solver takes: the first level
+// below the finest with at most 64 x 64 cells.
+size_t learned_level(const fs_handle* h) {
+  size_t c = 1;
+  while (c + 1 < h->mg.size() && (size_t)h->mg[c].nx * h->mg[c].ny > 4096) c++;
+  return c;
+}
+
+// With `harvest`, the residual handed to the learned solver's level is
 // appended to its training set (eager use only: the stream is synchronised).
 int launch_vcycle(fs_handle* h, int it, bool init, int nb_rho, fsl::Net* harvest = nullptr) {
   const size_t L = h->mg.size(), f = h->mg_first;
+  const size_t hl = harvest ? learned_level(h) : 0;
   for (size_t l = 0; l < f; l++) {
     MgArgs a = mg_args(h, l);
     a.it = it;
@@ -441,9 +452,9 @@ int launch_vcycle(fs_handle* h, int it, bool init, int nb_rho, fsl::Net* harvest
     a.part_in = h->part(0);
     a.nb_in = nb_rho;
     launch_mg_down(a, h->st);
-    if (l == 0 && harvest) {
+    if (harvest && l + 1 == hl) {
       if (sync_scalars(h)) return FS_E_HIP;
-      if (!h->h_sc->done && fsl::net_store_sample(harvest, h->mg[1], h->st) < 0) return FS_E_HIP;
+      if (!h->h_sc->done && fsl::net_store_sample(harvest, h->mg[hl], h->st) < 0) return FS_E_HIP;
     }
   }
   MgCycleArgs c;
@@ -483,25 +494,38 @@ int launch_mg_iteration(fs_handle* h, int it, fsl::Net* harvest = nullptr) {
   return launch_vcycle(h, it, false, nb_pcg, harvest);
 }
 
-// Learned two-grid preconditioner: finest down kernel (pre-smoothing,
-// restriction, and the scale partials in buffer 2, free at this point of the
-// chain), the network as coarse solver, finest up kernel (prolongation without
-// over-correction, post-smoothing, partials of r.z and z.q).
+// Learned preconditioner: the multigrid cycle down to the learned level c
+// (red-black smoothing and aggregation on the finer levels; level c-1 also
+// writes the partial sums of r_c^2 / d_c that scale the network input, into
+// buffer 2, which is free at this point of the chain), the network as the
+// solver of level c (or a zero correction without a network), and the cycle
+// back up (no over-correction at level c-1; the finest level also writes the
+// partials of r.z and z.q for the flexible beta).
 int launch_learned_cycle(fs_handle* h, int it, bool init, int nb_rho) {
-  MgArgs a = mg_args(h, 0);
-  a.it = it;
-  a.init = init;
-  a.part_in = h->part(0);
-  a.nb_in = nb_rho;
-  a.part_scale = h->part(2);
-  launch_mg_down(a, h->st);
-  if (fsl::net_apply(&h->net->net, h->mg[1], h->part(2), mg_grid(h->mg[0]), h->d_sc, h->st))
-    return FS_E_HIP;
-  MgArgs u = mg_args(h, 0);
-  u.part_out = h->part(1);
-  u.oc = 1.0;
-  u.q = init ? nullptr : h->pois(PO_MQ);
-  launch_mg_up(u, h->st);
+  const size_t c = learned_level(h);
+  for (size_t l = 0; l < c; l++) {
+    MgArgs a = mg_args(h, l);
+    a.it = it;
+    a.init = init;
+    a.part_in = h->part(0);
+    a.nb_in = nb_rho;
+    if (l + 1 == c) a.part_scale = h->part(2);
+    launch_mg_down(a, h->st);
+  }
+  if (h->net) {
+    if (fsl::net_apply(&h->net->net, h->mg[c], h->part(2), mg_grid(h->mg[c - 1]), h->d_sc, h->st))
+      return FS_E_HIP;
+  } else {
+    const MgLevel& C = h->mg[c];
+    launch_fill(C.e, 0.0, (size_t)C.P * (C.ny + 2), h->st);
+  }
+  for (size_t l = c; l-- > 0;) {
+    MgArgs u = mg_args(h, l);
+    u.part_out = h->part(1);
+    if (l + 1 == c) u.oc = 1.0;
+    if (l == 0) u.q = init ? nullptr : h->pois(PO_MQ);
+    launch_mg_up(u, h->st);
+  }
   return 0;
 }
 
@@ -529,6 +553,23 @@ int make_mg_graph(fs_handle* h) {
   HIP_OK(hipStreamEndCapture(h->st, &g));
   HIP_OK(hipGraphInstantiate(&h->mg_graph, g, nullptr, nullptr, 0));
   HIP_OK(hipGraphDestroy(g));
+  return 0;
+}
+
+int make_learn_graph(fs_handle* h) {
+  if (h->learn_graph) HIP_OK(hipGraphExecDestroy(h->learn_graph));
+  h->learn_graph = nullptr;
+  hipGraph_t g = nullptr;
+  HIP_OK(hipStreamBeginCapture(h->st, hipStreamCaptureModeThreadLocal));
+  if (launch_learned_iteration(h, 1) || launch_learned_iteration(h, 2)) {
+    (void)hipStreamEndCapture(h->st, &g);
+    if (g) (void)hipGraphDestroy(g);
+    return FS_E_HIP;
+  }
+  HIP_OK(hipStreamEndCapture(h->st, &g));
+  HIP_OK(hipGraphInstantiate(&h->learn_graph, g, nullptr, nullptr, 0));
+  HIP_OK(hipGraphDestroy(g));
+  h->learn_graph_key = h->net ? (const void*)h->net : (const void*)h;
   return 0;
 }
 
@@ -566,19 +607,29 @@ int lapl_solve_impl(fs_handle* h, int* lerr, fs_solve_stats* stats) {
   const int chunk = h->p.lapl_chunk > 0 ? h->p.lapl_chunk : 32;
   int it = 0;
   if (h->p.lapl_solver == 2) {
-    // flexible CG with the learned two-grid preconditioner (eager launches)
-    if (!h->net || h->mg.size() < 2)
-      return fail(FS_E_ARG, "learned solver: no network attached or grid of <= 64 cells%s");
-    launch_mg_coarsen(mg_args(h, 0), h->st);
+    // flexible CG with the learned coarse solver (eager launches); without an
+    // attached network the learned level gets a zero correction
+    if (h->mg.size() < 2)
+      return fail(FS_E_ARG, "learned solver: grid of <= 64 cells has no coarse level%s");
+    for (size_t l = 0; l < learned_level(h); l++) launch_mg_coarsen(mg_args(h, l), h->st);
     MgPcgArgs ia = mgpcg_args(h, 1);
     ia.part_out = h->part(0);
     launch_mgpcg_init(ia, h->st);
     const int nb_pcg = capped_grid(h->nbx * h->pnby);
-    if (launch_learned_cycle(h, 0, true, nb_pcg)) return FS_E_HIP;
+    if (launch_learned_cycle(h, 0, true, nb_pcg) || launch_learned_iteration(h, 0))
+      return FS_E_HIP;
+    it = 1;
+    const void* key = h->net ? (const void*)h->net : (const void*)h;
+    if (h->p.use_graphs && (!h->learn_graph || h->learn_graph_key != key) && make_learn_graph(h))
+      return FS_E_HIP;
     const int lchunk = chunk > 8 ? chunk / 4 : chunk;
     for (;;) {
-      for (int k = 0; k < lchunk; k++, it++)
-        if (launch_learned_iteration(h, it)) return FS_E_HIP;
+      if (h->p.use_graphs) {
+        for (int k = 0; k < lchunk; k += 2, it += 2) HIP_OK(hipGraphLaunch(h->learn_graph, h->st));
+      } else {
+        for (int k = 0; k < lchunk; k++, it++)
+          if (launch_learned_iteration(h, it)) return FS_E_HIP;
+      }
       HIP_OK(hipGetLastError());
       if (sync_scalars(h)) return FS_E_HIP;
       if (h->h_sc->status < 0) return check_status(h);
@@ -910,6 +961,7 @@ int fs_destroy(fs_handle* h) {
   (void)hipFree(h->d_mg);
   (void)hipFree(h->d_mginv);
   if (h->mg_graph) (void)hipGraphExecDestroy(h->mg_graph);
+  if (h->learn_graph) (void)hipGraphExecDestroy(h->learn_graph);
   for (int e = 0; e < 8; e++)
     if (h->ev[e]) (void)hipEventDestroy(h->ev[e]);
   if (h->st) (void)hipStreamDestroy(h->st);
@@ -1235,6 +1287,8 @@ int fs_eval_dfdph(int32_t n, const double* a, double* out) {
 int fs_time_kernel(fs_handle* h, int32_t which, int32_t n, double* avg_ms) {
   if (!h || !avg_ms || n <= 0) return fail(FS_E_ARG, "bad argument%s");
   if (sync_scalars(h)) return FS_E_HIP;
+  const double tol0 = h->h_sc->tol;
+  const int32_t maxit0 = h->h_sc->maxit;
   h->h_sc->done = 0;  // let the PCG kernels run
   h->h_sc->maxit = 1 << 30;
   h->h_sc->tol = 0.0;
@@ -1261,6 +1315,14 @@ int fs_time_kernel(fs_handle* h, int32_t which, int32_t n, double* avg_ms) {
       launch_lapl_assemble(a, h->K, h->st);
     } else if (which == 11) {
       launch_pcg_it(pcg_args(h, 2 + (k & 1)), h->st);
+    } else if (which == 15) {
+      const size_t c = learned_level(h);
+      (void)fsl::net_apply(&h->net->net, h->mg[c], h->part(2), mg_grid(h->mg[c - 1]), h->d_sc,
+                           h->st);
+    } else if (which == 16) {
+      (void)launch_learned_iteration(h, 2 + (k & 1));
+    } else if (which == 17) {
+      launch_mg_iteration(h, 2 + (k & 1));
     } else if (which == 13 || which == 14) {
       MgArgs a = mg_args(h, 0);
       a.it = 2 + (k & 1);
@@ -1276,10 +1338,11 @@ int fs_time_kernel(fs_handle* h, int32_t which, int32_t n, double* avg_ms) {
       launch_pcg_init(ia, h->st);
     }
   };
-  if (!(which >= 1 && which <= 5) && !(which >= 10 && which <= 14))
+  if (!(which >= 1 && which <= 5) && !(which >= 10 && which <= 17))
     return fail(FS_E_ARG, "unknown kernel id%s");
-  if ((which == 13 || which == 14) && h->mg.size() < 2)
+  if (which >= 13 && h->mg.size() < 2)
     return fail(FS_E_ARG, "grid too small for a multigrid hierarchy%s");
+  if (which == 15 && !h->net) return fail(FS_E_ARG, "no network attached%s");
   launch(0);  // warm
   HIP_OK(hipEventRecord(h->ev[6], h->st));
   for (int k = 0; k < n; k++) launch(k);
@@ -1289,7 +1352,12 @@ int fs_time_kernel(fs_handle* h, int32_t which, int32_t n, double* avg_ms) {
   float ms = 0;
   HIP_OK(hipEventElapsedTime(&ms, h->ev[6], h->ev[7]));
   *avg_ms = (double)ms / n;
-  return 0;
+  // the solver settings come back; the fields do not (see the declaration)
+  if (sync_scalars(h)) return FS_E_HIP;
+  h->h_sc->tol = tol0;
+  h->h_sc->maxit = maxit0;
+  h->h_sc->done = 0;
+  return push_scalars(h);
 }
 
 int fs_copy_bench(int64_t bytes, int32_t reps, double* gbps) {
@@ -1374,8 +1442,9 @@ int fs_learn_samples(const fs_learn* n) { return n ? n->net.count : -1; }
 int fs_learn_harvest(fs_learn* n, fs_handle* h, int32_t* added) {
   if (!n || !h) return fail(FS_E_ARG, "null argument%s");
   if (h->mg.size() < 2) return fail(FS_E_ARG, "grid of <= 64 cells has no coarse level%s");
-  if (h->mg[1].nx > n->net.W || h->mg[1].ny > n->net.H)
-    return fail(FS_E_ARG, "coarse grid larger than the training grid%s");
+  const MgLevel& lc = h->mg[learned_level(h)];
+  if (lc.nx > n->net.W || lc.ny > n->net.H)
+    return fail(FS_E_ARG, "learned level larger than the training grid%s");
   const int before = n->net.count;
   HIP_OK(hipStreamSynchronize(n->net.st));
   launch_assembly(h);

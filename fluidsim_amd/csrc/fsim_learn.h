@@ -3,12 +3,12 @@
 //
 // The network is a 4-level convolutional U-Net (16/32/48/64 channels, 3x3
 // kernels, no biases, ReLU) with BF16 weights and activations and FP32
-// accumulation on the MFMA units.  It acts as the coarse solver of a two-grid
-// preconditioner inside flexible CG: the finest level is pre-smoothed and its
-// residual restricted to the 2x2-aggregated level exactly as in the multigrid
-// cycle; the network maps that coarse residual (plus the coarse operator's
-// coupling ratios) to the coarse correction, which is prolongated and
-// post-smoothed.  Activations are NHWC with the channel count a multiple of 16.
+// accumulation on the MFMA units.  It is the coarse solver of the multigrid
+// preconditioner inside flexible CG: the levels down to the first one of at
+// most 64 x 64 cells are smoothed and restricted as in the multigrid cycle;
+// the network maps that level's residual (plus its operator's coupling ratios)
+// to its correction, which is prolongated and post-smoothed on the way up.
+// Activations are NHWC with the channel count a multiple of 16.
 #pragma once
 
 #include <hip/hip_runtime.h>

@@ -74,10 +74,11 @@ struct ConvArgs {
 };
 
 // Four input channels c0..c0+3 of pixel (n, hh, ww) at the output resolution.
+template <int MODE>
 __device__ __forceinline__ bf4 conv_in4(const ConvArgs& A, int n, int hh, int ww, int c0) {
-  if (A.mode == kPlain)
+  if (MODE == kPlain)
     return *(const bf4*)(A.xa + (((size_t)n * A.H + hh) * A.W + ww) * A.ca + c0);
-  if (A.mode == kPool) {
+  if (MODE == kPool) {
     const size_t W2 = 2 * (size_t)A.W;
     const size_t b = (((size_t)n * 2 * A.H + 2 * hh) * W2 + 2 * ww) * A.ca + c0;
     const bf4 q0 = *(const bf4*)(A.xa + b), q1 = *(const bf4*)(A.xa + b + A.ca);
@@ -112,9 +113,12 @@ __device__ __forceinline__ float conv_in1(const ConvArgs& A, int p, int t, int c
   return bf2f(A.xb[(((size_t)n * A.H + hh) * A.W + ww) * A.cb + (c - A.ca)]);
 }
 
-// Forward (and input-gradient) 3x3 convolution: NT = cout / 16.
-template <int NT>
+// Forward (and input-gradient) 3x3 convolution with NT = cout / 16 output
+// tiles and CB = cin / 16 input-channel blocks, fully unrolled so that the
+// operand loads of all taps can be issued ahead of the MFMA chain.
+template <int MODE, int NT, int CB>
 __global__ void __launch_bounds__(256) k_conv(ConvArgs A) {
+  constexpr int cin = 16 * CB;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int HW = A.H * A.W, total = A.N * HW;
   const int p0 = (blockIdx.x * 4 + wave) * 16;
@@ -124,23 +128,26 @@ __global__ void __launch_bounds__(256) k_conv(ConvArgs A) {
   const bool pin = pa < total;
   const int n = pin ? pa / HW : 0, rem = pin ? pa % HW : 0;
   const int h = rem / A.W, w = rem % A.W;
-  const int cin = A.mode == kUpCat ? A.ca + A.cb : A.ca;
   f4 acc[NT];
 #pragma unroll
   for (int nt = 0; nt < NT; nt++) acc[nt] = (f4){0.f, 0.f, 0.f, 0.f};
+#pragma unroll
   for (int t = 0; t < 9; t++) {
     const int hh = h + t / 3 - 1, ww = w + t % 3 - 1;
     const bool ok = pin && hh >= 0 && hh < A.H && ww >= 0 && ww < A.W;
-    const uint16_t* wt = A.w + ((size_t)t * A.cout + col) * cin + 4 * kg;
-    for (int c = 0; c < cin; c += 16) {
-      bf4 a = {0, 0, 0, 0};
-      if (ok) a = conv_in4(A, n, hh, ww, c + 4 * kg);
+    const uint16_t* wt = A.w + ((size_t)t * (16 * NT) + col) * cin + 4 * kg;
+    bf4 a[CB], b[CB][NT];
 #pragma unroll
-      for (int nt = 0; nt < NT; nt++) {
-        const bf4 b = *(const bf4*)(wt + (size_t)nt * 16 * cin + c);
-        acc[nt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[nt], 0, 0, 0);
-      }
+    for (int cb = 0; cb < CB; cb++) {
+      a[cb] = ok ? conv_in4<MODE>(A, n, hh, ww, 16 * cb + 4 * kg) : (bf4){0, 0, 0, 0};
+#pragma unroll
+      for (int nt = 0; nt < NT; nt++) b[cb][nt] = *(const bf4*)(wt + (size_t)nt * 16 * cin + 16 * cb);
     }
+#pragma unroll
+    for (int cb = 0; cb < CB; cb++)
+#pragma unroll
+      for (int nt = 0; nt < NT; nt++)
+        acc[nt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[cb], b[cb][nt], acc[nt], 0, 0, 0);
   }
 #pragma unroll
   for (int nt = 0; nt < NT; nt++)
@@ -149,7 +156,7 @@ __global__ void __launch_bounds__(256) k_conv(ConvArgs A) {
       const int p = p0 + 4 * kg + i;
       if (p >= total) continue;
       const int co = nt * 16 + col;
-      const size_t o = (size_t)p * A.cout + co;
+      const size_t o = (size_t)p * (16 * NT) + co;
       float v = acc[nt][i];
       if (A.relu) v = fmaxf(v, 0.0f);
       if (A.mask && !(bf2f(A.mask[o]) > 0.0f)) v = 0.0f;
@@ -466,19 +473,26 @@ __global__ void k_nn_store(MgLevel L, int H, int W, float* out) {
     if ((x) != hipSuccess) return -1;   \
   } while (0)
 
-void launch_conv(const ConvArgs& a, hipStream_t st) {
+// The (mode, cout/16, cin/16) combinations of the forward and input-gradient
+// convolutions of the network.
+#define FSL_CONV_CASES(X)                                                              \
+  X(kPlain, 1, 1) X(kPlain, 2, 2) X(kPlain, 3, 3) X(kPlain, 4, 4) X(kPlain, 1, 2)      \
+  X(kPlain, 2, 3) X(kPlain, 3, 4) X(kPlain, 7, 3) X(kPlain, 5, 2) X(kPlain, 3, 1)      \
+  X(kPool, 2, 1) X(kPool, 3, 2) X(kPool, 4, 3) X(kUpCat, 3, 7) X(kUpCat, 2, 5)         \
+  X(kUpCat, 1, 3)
+
+int launch_conv(const ConvArgs& a, hipStream_t st) {
   const int total = a.N * a.H * a.W;
   const dim3 g((total + 63) / 64), b(256);
-  switch (a.cout / 16) {
-    case 1: hipLaunchKernelGGL(k_conv<1>, g, b, 0, st, a); break;
-    case 2: hipLaunchKernelGGL(k_conv<2>, g, b, 0, st, a); break;
-    case 3: hipLaunchKernelGGL(k_conv<3>, g, b, 0, st, a); break;
-    case 4: hipLaunchKernelGGL(k_conv<4>, g, b, 0, st, a); break;
-    case 5: hipLaunchKernelGGL(k_conv<5>, g, b, 0, st, a); break;
-    case 6: hipLaunchKernelGGL(k_conv<6>, g, b, 0, st, a); break;
-    case 7: hipLaunchKernelGGL(k_conv<7>, g, b, 0, st, a); break;
-    default: break;
+  const int nt = a.cout / 16, cb = (a.mode == kUpCat ? a.ca + a.cb : a.ca) / 16;
+#define FSL_CONV_LAUNCH(M, NT, CB)                                        \
+  if (a.mode == M && nt == NT && cb == CB) {                              \
+    hipLaunchKernelGGL((k_conv<M, NT, CB>), g, b, 0, st, a);              \
+    return 0;                                                             \
   }
+  FSL_CONV_CASES(FSL_CONV_LAUNCH)
+#undef FSL_CONV_LAUNCH
+  return -1;
 }
 
 // The forward convolution of layer l over activations a.
@@ -501,8 +515,10 @@ ConvArgs fwd_args(const Net* net, const Acts& a, int l) {
   return c;
 }
 
-void forward(const Net* net, const Acts& a, hipStream_t st) {
-  for (int l = 0; l < kLayers; l++) launch_conv(fwd_args(net, a, l), st);
+int forward(const Net* net, const Acts& a, hipStream_t st) {
+  for (int l = 0; l < kLayers; l++)
+    if (launch_conv(fwd_args(net, a, l), st)) return -1;
+  return 0;
 }
 
 int ceil_div(size_t a, size_t b) { return (int)((a + b - 1) / b); }
@@ -528,7 +544,7 @@ void wgrad(Net* net, const Acts& a, int l, hipStream_t st) {
 
 // Input gradient of layer l (l > 0): masked BF16 into dpre[l - 1] for plain
 // layers, FP32 into `dx` for pooling and up-concat layers.
-void dgrad(Net* net, const Acts& a, int l, float* dx, hipStream_t st) {
+int dgrad(Net* net, const Acts& a, int l, float* dx, hipStream_t st) {
   const LayerDesc& L = kNet[l];
   ConvArgs c;
   memset(&c, 0, sizeof(c));
@@ -548,7 +564,7 @@ void dgrad(Net* net, const Acts& a, int l, float* dx, hipStream_t st) {
   } else {
     c.yf = dx;
   }
-  launch_conv(c, st);
+  return launch_conv(c, st);
 }
 
 WOff woff(const Net* net) {
@@ -567,7 +583,7 @@ void refresh_bf16(Net* net, hipStream_t st) {
 }
 
 // Full backward from a.dout; parameter gradients into net->g.
-void backward(Net* net, const Acts& a, hipStream_t st) {
+int backward(Net* net, const Acts& a, hipStream_t st) {
   const int n0 = a.N * a.H * a.W;
   const float* wout = net->w + net->off[kLayers];
   hipLaunchKernelGGL(k_out_bwd, dim3(ceil_div(n0, 256)), dim3(256), 0, st, a.y[13], wout, a.dout,
@@ -588,17 +604,17 @@ void backward(Net* net, const Acts& a, hipStream_t st) {
     if (l == 0) break;
     const LayerDesc& L = kNet[l];
     if (L.mode == kPlain) {
-      dgrad(net, a, l, nullptr, st);
+      if (dgrad(net, a, l, nullptr, st)) return -1;
     } else if (L.mode == kUpCat) {
       const int lv = L.level;
-      dgrad(net, a, l, dxcat[lv], st);
+      if (dgrad(net, a, l, dxcat[lv], st)) return -1;
       const int Hc = a.H >> (lv + 1), Wc = a.W >> (lv + 1);
       const int total = a.N * Hc * Wc * L.ca;
       hipLaunchKernelGGL(k_up_bwd, dim3(ceil_div(total, 256)), dim3(256), 0, st, dxcat[lv],
                          L.ca + L.cb, a.y[l - 1], L.ca, a.N, Hc, Wc, a.dpre[l - 1]);
     } else {  // pooling layer at level lv + 1 reading encoder output l - 1 at level lv
       const int lv = L.level - 1;
-      dgrad(net, a, l, a.dx2, st);
+      if (dgrad(net, a, l, a.dx2, st)) return -1;
       const int c = L.ca, H = a.H >> lv, W = a.W >> lv;
       const int dl = 12 - 2 * lv;  // the decoder layer holding the skip
       const int total = a.N * H * W * c;
@@ -607,6 +623,7 @@ void backward(Net* net, const Acts& a, hipStream_t st) {
                          a.dpre[l - 1]);
     }
   }
+  return 0;
 }
 
 uint64_t splitmix(uint64_t& s) {
@@ -752,7 +769,7 @@ int net_apply(Net* net, const MgLevel& L, const double* part, int nb, DevScalars
   }
   hipLaunchKernelGGL(k_nn_input, dim3(ceil_div((size_t)H * W, 256)), dim3(256), 0, st, L, part,
                      nb, sc, H, W, a.in);
-  forward(net, a, st);
+  if (forward(net, a, st)) return -1;
   hipLaunchKernelGGL(k_nn_output, dim3(ceil_div((size_t)L.nx * L.ny, 256)), dim3(256), 0, st,
                      a.y[13], net->w + net->off[kLayers], L, sc, W);
   return hipGetLastError() == hipSuccess ? 0 : -1;
@@ -780,7 +797,7 @@ int train_step(Net* net, const int* idx_host, int B, float lr, double* loss) {
   HIPC(hipMemcpyAsync(d_idx, idx_host, B * sizeof(int), hipMemcpyHostToDevice, st));
   hipLaunchKernelGGL(k_gather, dim3(B), dim3(1024), 0, st, net->data, d_idx, net->H, net->W,
                      a.in, a.rhat, a.op, a.scale);
-  forward(net, a, st);
+  if (forward(net, a, st)) return -1;
   const int n0 = B * net->H * net->W;
   hipLaunchKernelGGL(k_out_fwd, dim3(ceil_div(n0, 256)), dim3(256), 0, st, a.y[13],
                      net->w + net->off[kLayers], a.out, n0);
@@ -788,7 +805,7 @@ int train_step(Net* net, const int* idx_host, int B, float lr, double* loss) {
   if (nbl > kLossSlots / 2 || B > kLossSlots) return -1;
   hipLaunchKernelGGL(k_loss, dim3(nbl), dim3(256), 0, st, a.out, a.rhat, a.op, B, net->H, net->W,
                      a.dout, net->lpart);
-  backward(net, a, st);
+  if (backward(net, a, st)) return -1;
   net->adam_t++;
   const float b1 = 0.9f, b2 = 0.999f;
   const float c1 = 1.0f - powf(b1, (float)net->adam_t), c2 = 1.0f - powf(b2, (float)net->adam_t);
@@ -823,14 +840,14 @@ int loss_grad(Net* net, int B, int H, int W, const float* in5, const float* rhat
   HIPC(hipMemcpyAsync(a.in, in.data(), in.size() * 2, hipMemcpyHostToDevice, st));
   HIPC(hipMemcpyAsync(a.rhat, rhat, n0 * 4, hipMemcpyHostToDevice, st));
   HIPC(hipMemcpyAsync(a.op, op, n0 * 3 * 4, hipMemcpyHostToDevice, st));
-  forward(net, a, st);
+  if (forward(net, a, st)) return -1;
   hipLaunchKernelGGL(k_out_fwd, dim3(ceil_div(n0, 256)), dim3(256), 0, st, a.y[13],
                      net->w + net->off[kLayers], a.out, (int)n0);
   const int nbl = ceil_div(n0, 256);
   if (nbl > kLossSlots / 2) return -1;
   hipLaunchKernelGGL(k_loss, dim3(nbl), dim3(256), 0, st, a.out, a.rhat, a.op, B, H, W, a.dout,
                      net->lpart);
-  backward(net, a, st);
+  if (backward(net, a, st)) return -1;
   std::vector<float> lp(nbl);
   HIPC(hipMemcpyAsync(lp.data(), net->lpart, nbl * 4, hipMemcpyDeviceToHost, st));
   if (grad) HIPC(hipMemcpyAsync(grad, net->g, net->nparam * 4, hipMemcpyDeviceToHost, st));

@@ -91,7 +91,49 @@ class LearnedPreconditioner:
                                              _dp(loss), _fp(grad), _fp(out)))
         return float(loss[0]), grad, out
 
+    @staticmethod
+    def detach(discharge) -> None:
+        """Learned solves of the discharge fall back to a zero correction on the
+        learned level (the method without the network)."""
+        _L.check(_L.load().fs_learn_attach(discharge.h, None))
+        discharge._net = None
+
     def attach(self, discharge) -> None:
         """Use this network in the discharge's solves with lapl_solver = 2."""
         _L.check(_L.load().fs_learn_attach(discharge.h, self.h))
         discharge._net = self  # keep the network alive while attached
+
+
+def harvest_runs(net: LearnedPreconditioner, n: int, seeds, samples: int, warm: int = 5,
+                 every: int = 2, config=None) -> float:
+    """Fill the training set with `samples` residuals from seeded runs of an
+    n x n discharge (the 512^2 pulse workload by default): after `warm` passes,
+    one multigrid solve is harvested every `every` passes.  Returns seconds."""
+    import time
+
+    from . import configs
+    from .discharge import Discharge
+
+    config = config or configs.config1
+    t0 = time.perf_counter()
+    target0 = net.samples
+    for k, seed in enumerate(seeds):
+        d = Discharge(config(n))
+        d.init_gaussian(seed=int(seed))
+        d.run(warm)
+        goal = target0 + samples * (k + 1) // len(seeds)
+        while net.samples < goal:
+            d.run(every)
+            if net.harvest(d) == 0:
+                break
+    return time.perf_counter() - t0
+
+
+def train_schedule(net: LearnedPreconditioner, phases=((1500, 16, 1e-3), (1500, 16, 2.5e-4)),
+                   seed: int = 0):
+    """Adam phases of (steps, batch, learning rate); returns (losses, seconds)."""
+    import time
+
+    t0 = time.perf_counter()
+    losses = [net.train(s, batch=b, lr=lr, seed=seed + k) for k, (s, b, lr) in enumerate(phases)]
+    return np.concatenate(losses), time.perf_counter() - t0

@@ -196,7 +196,9 @@ int fs_set_timing(fs_handle* h, int32_t on);
  * state is overwritten, so call it after the timed run.  which: 1..5 = the
  * fused stage kernel of that RK stage, 10 = Poisson assembly, 11/12 = the
  * Jacobi-PCG iteration and initialisation kernels, 13/14 = the multigrid down
- * and up kernels of the finest level. */
+ * and up kernels of the finest level, 15 = the attached network on the learned
+ * level, 16 = one learned flexible-CG iteration, 17 = one multigrid-PCG
+ * iteration (sequences of kernels, timed as a whole). */
 int fs_time_kernel(fs_handle* h, int32_t which, int32_t n, double* avg_ms);
 
 /* Device copy bandwidth: `reps` launches of a plain copy kernel moving
@@ -209,15 +211,17 @@ int fs_copy_bench(int64_t bytes, int32_t reps, double* gbps);
  * No counterpart in the reference (its Poisson solve is PETSc's KSP with the
  * default preconditioner, lapl_lib.F90:50-54, 104).  A 4-level convolutional
  * U-Net (16/32/48/64 channels, 3x3 kernels, BF16 weights and activations,
- * FP32 accumulation) serves as the coarse solver of a two-grid preconditioner
- * (red-black Gauss-Seidel smoothing, 2x2 aggregation) inside flexible CG:
- * lapl_solver = 2.  Its first application, to the right-hand side, is the
- * solve's warm start.  It is trained on the device with the A-norm energy
- * loss over residuals harvested from the simulator's own solves. */
+ * FP32 accumulation) is the coarse solver of the multigrid preconditioner
+ * (red-black Gauss-Seidel smoothing, 2x2 aggregation) on the first level of at
+ * most 64 x 64 cells, inside flexible CG: lapl_solver = 2.  Its first
+ * application, to the right-hand side, is the solve's warm start.  It is
+ * trained on the device with the A-norm energy loss over that level's
+ * residuals harvested from the simulator's own solves.  Without an attached
+ * network the level gets a zero correction (the method without the network). */
 typedef struct fs_learn fs_learn;
 
-/* Training-set capacity (samples) and the coarse grid of the samples
- * (H x W, multiples of 8; smaller grids are zero-padded). */
+/* Training-set capacity (samples) and the grid of the samples (H x W,
+ * multiples of 8, normally 64 x 64; smaller grids are zero-padded). */
 int fs_learn_create(int32_t capacity, int32_t H, int32_t W, uint64_t seed, fs_learn** out);
 int fs_learn_destroy(fs_learn* n);
 int64_t fs_learn_nparams(const fs_learn* n);
@@ -225,8 +229,8 @@ int fs_learn_get_weights(const fs_learn* n, float* out);   /* FP32 masters */
 int fs_learn_set_weights(fs_learn* n, const float* in);
 int fs_learn_samples(const fs_learn* n);
 /* Solve the handle's current Poisson system with the multigrid-preconditioned
- * CG, one iteration at a time, appending the restricted, pre-smoothed coarse
- * residual of every iteration (with the coarse operator) to the training set.
+ * CG, one iteration at a time, appending the residual that reaches the learned
+ * level in every cycle (with that level's operator) to the training set.
  * phi is left unchanged. */
 int fs_learn_harvest(fs_learn* n, fs_handle* h, int32_t* added);
 /* `steps` Adam steps on random batches of `batch` samples; loss[steps]. */
@@ -238,7 +242,8 @@ int fs_learn_train(fs_learn* n, int32_t steps, int32_t batch, float lr, uint64_t
 int fs_learn_loss_grad(fs_learn* n, int32_t B, int32_t H, int32_t W, const float* in5,
                        const float* rhat, const float* op, double* loss, float* grad, float* out);
 /* Use the network in the handle's later solves when lapl_solver = 2 (NULL
- * detaches); the network must outlive the attachment. */
+ * detaches: zero correction on the learned level); the network must outlive
+ * the attachment. */
 int fs_learn_attach(fs_handle* h, fs_learn* n);
 
 const char* fs_last_error(void);

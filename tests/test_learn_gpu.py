@@ -95,11 +95,13 @@ def test_learned_flexible_cg_reaches_the_plain_solution():
     fresh = LearnedPreconditioner(0, net.H, net.W, seed=1)  # same initial weights
     ph0 = d.get("ph")
     res = {}
-    for name, solver, nn in (("jacobi", 0, None), ("mg", 1, None), ("untrained", 2, fresh),
-                             ("learned", 2, net)):
+    for name, solver, nn in (("jacobi", 0, None), ("mg", 1, None), ("no network", 2, None),
+                             ("untrained", 2, fresh), ("learned", 2, net)):
         d.set("ph", 1, ph0)
         if nn is not None:
             nn.attach(d)
+        elif solver == 2:
+            LearnedPreconditioner.detach(d)
         d.set_solver(solver)
         lerr, st = d.lapl_solve()
         res[name] = (lerr, st.iters, st.resid, d.get("ph") - ph0)
@@ -109,6 +111,27 @@ def test_learned_flexible_cg_reaches_the_plain_solution():
     for name in res:
         assert res[name][0] == 0 and res[name][2] <= 1e-10, (name, res[name][:3])
     assert res["learned"][1] < res["untrained"][1]
+    assert res["learned"][1] < res["no network"][1]
     assert res["learned"][1] < res["jacobi"][1] / 3
     ref = res["mg"][3][1:-1, 1:-1]
     assert rel_l2(res["learned"][3][1:-1, 1:-1], ref) <= 1e-6
+
+
+def test_learned_graph_replay_equals_eager_launches():
+    """The learned iterations replayed from a captured HIP graph give bitwise
+    the same correction as the same kernels launched one by one."""
+    from fluidsim_amd import Discharge, configs
+
+    d, net = _harvested(steps=6)
+    net.train(50, batch=8, lr=2e-3, seed=1)
+    out = []
+    for graphs in (0, 1):
+        c = configs.config0(rtol=1e-10, solver=2, graphs=graphs)
+        e = Discharge(c)
+        e.init_gaussian(seed=0)
+        net.attach(e)
+        e.run(8)
+        out.append((e.get("ph"), e.get("ne", 1), e.log()))
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1])
+    assert np.array_equal(out[0][2], out[1][2])
