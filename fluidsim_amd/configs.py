@@ -62,3 +62,23 @@ def ic_defaults(p_torr=1.0) -> dict:
     """Gaussian blob of 1e8 cm^-3 peak, sigma = 0.1 cm, 1% noise (nondimensional)."""
     return dict(peak=1e14 * X0_3, sigma=1e-3 / X0, noise=0.01)
 
+
+
+def config3(n_members=256, base_seed=0, lo=64, hi=256, **kw) -> list:
+    """Parameter-sweep ensemble: sides drawn uniformly from lo..hi (independently,
+    non-square allowed), DC voltage U[150, 400] V, pressure log-uniform in
+    [0.5, 10] Torr, secondary emission U[0.05, 0.15]; member i's initial-noise
+    seed is base_seed + i (returned as cfg["seed"])."""
+    import numpy as np
+
+    rng = np.random.default_rng(12345 + base_seed)
+    out = []
+    for i in range(n_members):
+        nx, ny = (int(v) for v in rng.integers(lo, hi + 1, size=2))
+        volts = float(rng.uniform(150.0, 400.0))
+        p = float(np.exp(rng.uniform(np.log(0.5), np.log(10.0))))
+        gam = float(rng.uniform(0.05, 0.15))
+        c = base(nx, ny, p_torr=p, volts=volts, gam=gam, ratio_x=1.05 ** (64.0 / nx), **kw)
+        c["seed"] = base_seed + i
+        out.append(c)
+    return out

@@ -6,6 +6,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <chrono>
 #include <string>
 #include <vector>
@@ -432,6 +433,24 @@ size_t mg_lds_first(const fs_handle* h) {
 }
 
 // Partial-sum buffers of the MG-PCG chain: 0 = |r/V|^2, 1 = r.u, 2 = p.q
+// The one-workgroup part of the cycle: levels mg_first .. L-1.
+MgCycleArgs cycle_args(fs_handle* h) {
+  const size_t L = h->mg.size(), f = h->mg_first;
+  MgCycleArgs c;
+  memset(&c, 0, sizeof(c));
+  c.nq = (int)(L - f);
+  int off = 0;
+  for (int q = 0; q < c.nq; q++) {
+    c.lv[q] = h->mg[f + q];
+    c.off[q] = off;
+    off += (c.lv[q].nx + 2) * (c.lv[q].ny + 2);
+  }
+  c.inv = h->d_mginv;
+  c.sc = h->d_sc;
+  c.oc = kMgOver;
+  return c;
+}
+
 // The level whose equation the learned coarse solver takes: the first level
 // below the finest with at most 64 x 64 cells.
 size_t learned_level(const fs_handle* h) {
@@ -457,19 +476,7 @@ int launch_vcycle(fs_handle* h, int it, bool init, int nb_rho, fsl::Net* harvest
       if (!h->h_sc->done && fsl::net_store_sample(harvest, h->mg[hl], h->st) < 0) return FS_E_HIP;
     }
   }
-  MgCycleArgs c;
-  memset(&c, 0, sizeof(c));
-  c.nq = (int)(L - f);
-  int off = 0;
-  for (int q = 0; q < c.nq; q++) {
-    c.lv[q] = h->mg[f + q];
-    c.off[q] = off;
-    off += (c.lv[q].nx + 2) * (c.lv[q].ny + 2);
-  }
-  c.inv = h->d_mginv;
-  c.sc = h->d_sc;
-  c.oc = kMgOver;
-  launch_mg_coarse_cycle(c, h->st);
+  launch_mg_coarse_cycle(cycle_args(h), h->st);
   for (size_t l = f; l-- > 0;) {
     MgArgs a = mg_args(h, l);
     a.part_out = h->part(1);
@@ -573,7 +580,7 @@ int make_learn_graph(fs_handle* h) {
   return 0;
 }
 
-void launch_assembly(fs_handle* h) {
+LaplArgs lapl_args(fs_handle* h) {
   LaplArgs a;
   memset(&a, 0, sizeof(a));
   int l1 = h->alias ? h->bufA() : h->bufL1();
@@ -595,7 +602,28 @@ void launch_assembly(fs_handle* h) {
     a.bdbg = h->d_dbg;
     a.adbg = h->d_dbg + (size_t)h->bx * h->by;
   }
-  launch_lapl_assemble(a, h->K, h->st);
+  return a;
+}
+
+void launch_assembly(fs_handle* h) { launch_lapl_assemble(lapl_args(h), h->K, h->st); }
+
+// A pass's record of a ptcl_step attempt (the Poisson fields follow later).
+void log_attempt(fs_handle* h) {
+  DevScalars& s = *h->h_sc;
+  double rec[FS_REC_LEN] = {s.ts,      s.t,     s.dt_stage, s.err_cur, s.dt,
+                            (double)s.accept, -1.0, 0.0, 0.0, s.t_m};
+  h->log.insert(h->log.end(), rec, rec + FS_REC_LEN);
+}
+
+// The Poisson fields of the pending record, on the last ptcl_step attempt.
+void log_solve(fs_handle* h, int lerr) {
+  size_t n = h->log.size() / FS_REC_LEN;
+  if (n > 0 && h->log[(n - 1) * FS_REC_LEN + 6] < 0) {
+    double* rec = &h->log[(n - 1) * FS_REC_LEN];
+    rec[6] = lerr;
+    rec[7] = h->h_sc->iters;
+    rec[8] = h->h_sc->tres;
+  }
 }
 
 int lapl_solve_impl(fs_handle* h, int* lerr, fs_solve_stats* stats) {
@@ -697,14 +725,18 @@ int lapl_solve_impl(fs_handle* h, int* lerr, fs_solve_stats* stats) {
         std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   }
   // log the solve on the last ptcl_step attempt (as the oracle does)
-  size_t n = h->log.size() / FS_REC_LEN;
-  if (n > 0 && h->log[(n - 1) * FS_REC_LEN + 6] < 0) {
-    double* rec = &h->log[(n - 1) * FS_REC_LEN];
-    rec[6] = *lerr;
-    rec[7] = h->h_sc->iters;
-    rec[8] = h->h_sc->tres;
-  }
+  log_solve(h, *lerr);
   return 0;
+}
+
+CtrlArgs ctrl_args(fs_handle* h) {
+  CtrlArgs ca;
+  memset(&ca, 0, sizeof(ca));
+  ca.part = h->part(0);
+  ca.nblocks = h->nblk;
+  ca.sc = h->d_sc;
+  ca.replay = h->d_replay;
+  return ca;
 }
 
 int ptcl_step_impl(fs_handle* h, fs_step_info* info) {
@@ -719,12 +751,7 @@ int ptcl_step_impl(fs_handle* h, fs_step_info* info) {
       launch_circ(c, h->K, h->st);
     }
     time_mark(h, 1);
-    CtrlArgs ca;
-    ca.part = h->part(0);
-    ca.nblocks = h->nblk;
-    ca.sc = h->d_sc;
-    ca.replay = h->d_replay;
-    launch_ctrl(ca, h->st);
+    launch_ctrl(ctrl_args(h), h->st);
     HIP_OK(hipGetLastError());
     time_mark(h, 2);
     if (sync_scalars(h)) return FS_E_HIP;
@@ -737,9 +764,7 @@ int ptcl_step_impl(fs_handle* h, fs_step_info* info) {
     }
     attempts++;
     DevScalars& s = *h->h_sc;
-    double rec[FS_REC_LEN] = {s.ts,      s.t,     s.dt_stage, s.err_cur, s.dt,
-                              (double)s.accept, -1.0, 0.0, 0.0, s.t_m};
-    h->log.insert(h->log.end(), rec, rec + FS_REC_LEN);
+    log_attempt(h);
     if (int e = check_status(h)) return e;
     h->alias = false;
     if (s.accept) break;
@@ -776,6 +801,387 @@ int reject_impl(fs_handle* h) {
   if (copy_level(h, h->bufA(), h->bufL3())) return FS_E_HIP;
   h->alias = true;
   return 0;
+}
+
+
+// ---------------------------------------------------------------------------
+// Ensembles: independent discharges advanced in lockstep rounds by launches
+// that cover every member (EnsMap).  A round gives each member one ptcl_step
+// attempt (five stages, circuit, controller) or, once its attempt is accepted
+// (or before the first transport step), its Poisson solve and the accept /
+// reject of the pass -- exactly the sequence fs_run issues for the member
+// alone, so each member's fields are bitwise those of its solo run.
+struct EnsMapH {
+  int32_t* d = nullptr;  // inst[nblocks], first[n], nb[n]
+  EnsMap m{};
+};
+
+struct EnsRound {  // uploaded once per round
+  int32_t* sel;
+  uint8_t *ptcl, *pois, *begin, *acc, *rej;
+};
+
+}  // namespace
+
+struct fs_ensemble {
+  std::vector<fs_handle*> m;
+  std::vector<DevScalars*> own_d, own_h;  // members' own scalar buffers
+  std::vector<hipStream_t> own_st;
+  DevScalars* d_sc = nullptr;
+  DevScalars* h_sc = nullptr;
+  hipStream_t st = nullptr;
+  int n = 0, maxL = 0, maxF = 0;
+  // argument tables
+  FsPhys* t_phys = nullptr;
+  StageArgs* t_stage = nullptr;   // [(i*6 + rot*2 + alias)*5 + stage-1]
+  CircArgs* t_circ = nullptr;     // [i*5 + stage-1]
+  CtrlArgs* t_ctrl = nullptr;     // [i]
+  LaplArgs* t_lapl = nullptr;     // [i*6 + rot*2 + alias]
+  MgArgs* t_coarsen = nullptr;    // [i*kMgMaxLevels + l]
+  MgArgs* t_factor = nullptr;     // [i]
+  double** t_inv = nullptr;       // [i]
+  MgPcgArgs* t_init = nullptr;    // [i]
+  MgArgs* t_down = nullptr;       // [(i*kMgMaxLevels + l)*3 + v], v: init, odd, even
+  MgArgs* t_up = nullptr;         // [i*kMgMaxLevels + l]
+  MgCycleArgs* t_cc = nullptr;    // [i]
+  MgPcgArgs* t_pq = nullptr;      // [i*3 + v], v: first, odd, even
+  MgPcgArgs* t_xr = nullptr;      // [i*2 + parity]
+  PcgArgs* t_fin = nullptr;       // [i]
+  EnsMapH m_stage, m_one, m_pcg, m_coarsen[kMgMaxLevels], m_down[kMgMaxLevels], m_up[kMgMaxLevels];
+  char* d_round = nullptr;
+  char* h_round = nullptr;
+  size_t round_bytes = 0;
+  EnsRound r{};
+  std::vector<char> phase;        // 0 start of a pass, 1 ptcl attempt, 2 Poisson
+};
+
+namespace {
+
+bool bad(hipError_t e) { return e != hipSuccess; }
+
+template <class T>
+int upload_table(T** d, const std::vector<T>& h) {
+  if (*d) HIP_OK(hipFree(*d));
+  *d = nullptr;
+  if (h.empty()) return 0;
+  HIP_OK(hipMalloc(d, h.size() * sizeof(T)));
+  HIP_OK(hipMemcpy(*d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+  return 0;
+}
+
+// Block map from per-member block counts (0 = not in this launch).
+int make_map(EnsMapH& mh, const std::vector<int>& nb) {
+  const int n = (int)nb.size();
+  std::vector<int32_t> first(n), inst;
+  for (int i = 0; i < n; i++) {
+    first[i] = (int32_t)inst.size();
+    for (int b = 0; b < nb[i]; b++) inst.push_back(i);
+  }
+  std::vector<int32_t> all(inst);
+  all.insert(all.end(), first.begin(), first.end());
+  all.insert(all.end(), nb.begin(), nb.end());
+  if (mh.d) HIP_OK(hipFree(mh.d));
+  HIP_OK(hipMalloc(&mh.d, all.size() * sizeof(int32_t)));
+  HIP_OK(hipMemcpy(mh.d, all.data(), all.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+  mh.m.inst = mh.d;
+  mh.m.first = mh.d + inst.size();
+  mh.m.nb = mh.d + inst.size() + n;
+  mh.m.nblocks = (int32_t)inst.size();
+  return 0;
+}
+
+EnsMap with_mask(const EnsMapH& mh, const uint8_t* active) {
+  EnsMap m = mh.m;
+  m.active = active;
+  return m;
+}
+
+// Argument tables (they hold device pointers only; rebuilt when a run starts,
+// since a member's replay table may have changed).
+int ens_tables(fs_ensemble* E) {
+  const int n = E->n;
+  std::vector<FsPhys> phys(n);
+  std::vector<StageArgs> st(n * 30);
+  std::vector<CircArgs> ci(n * 5);
+  std::vector<CtrlArgs> ct(n);
+  std::vector<LaplArgs> la(n * 6);
+  std::vector<MgArgs> co(n * kMgMaxLevels), fa(n), dn(n * kMgMaxLevels * 3), up(n * kMgMaxLevels);
+  std::vector<double*> inv(n);
+  std::vector<MgPcgArgs> in(n), pq(n * 3), xr(n * 2);
+  std::vector<MgCycleArgs> cc(n);
+  std::vector<PcgArgs> fi(n);
+  for (int i = 0; i < n; i++) {
+    fs_handle* h = E->m[i];
+    phys[i] = h->K;
+    const int rot0 = h->rot;
+    const bool alias0 = h->alias;
+    for (int rot = 0; rot < 3; rot++)
+      for (int al = 0; al < 2; al++) {
+        h->rot = rot;
+        h->alias = al != 0;
+        for (int stg = 1; stg <= 5; stg++) st[(i * 6 + rot * 2 + al) * 5 + stg - 1] = stage_args(h, stg, 0);
+        la[i * 6 + rot * 2 + al] = lapl_args(h);
+      }
+    h->rot = rot0;
+    h->alias = alias0;
+    for (int stg = 1; stg <= 5; stg++) ci[i * 5 + stg - 1] = circ_args(h, stg);
+    ct[i] = ctrl_args(h);
+    const int L = (int)h->mg.size();
+    const int nb_pcg = capped_grid(h->nbx * h->pnby);
+    for (int l = 0; l + 1 < L; l++) co[i * kMgMaxLevels + l] = mg_args(h, l);
+    fa[i] = mg_args(h, L - 1);
+    inv[i] = h->d_mginv;
+    in[i] = mgpcg_args(h, 1);
+    in[i].part_out = h->part(0);
+    for (int l = 0; l < (int)h->mg_first; l++) {
+      for (int v = 0; v < 3; v++) {
+        MgArgs a = mg_args(h, l);
+        a.it = v == 1 ? 1 : 0;
+        a.init = v == 0;
+        a.part_in = h->part(0);
+        a.nb_in = nb_pcg;
+        dn[(i * kMgMaxLevels + l) * 3 + v] = a;
+      }
+      MgArgs u = mg_args(h, l);
+      u.part_out = h->part(1);
+      up[i * kMgMaxLevels + l] = u;
+    }
+    cc[i] = cycle_args(h);
+    for (int v = 0; v < 3; v++) {
+      MgPcgArgs a = mgpcg_args(h, v);  // it = 0 (first), 1 (odd), 2 (even)
+      a.part_in = h->part(1);
+      a.nb_in = mg_grid(h->mg[0]);
+      a.part_out = h->part(2);
+      pq[i * 3 + v] = a;
+    }
+    for (int par = 0; par < 2; par++) {
+      MgPcgArgs a = mgpcg_args(h, par);
+      a.part_in = h->part(2);
+      a.nb_in = nb_pcg;
+      a.part_out = h->part(0);
+      xr[i * 2 + par] = a;
+    }
+    PcgArgs f = pcg_args(h, 0);
+    f.part_out = h->part(2);
+    f.part_in = h->part(2);
+    fi[i] = f;
+  }
+  if (upload_table(&E->t_phys, phys) || upload_table(&E->t_stage, st) ||
+      upload_table(&E->t_circ, ci) || upload_table(&E->t_ctrl, ct) ||
+      upload_table(&E->t_lapl, la) || upload_table(&E->t_coarsen, co) ||
+      upload_table(&E->t_factor, fa) || upload_table(&E->t_inv, inv) ||
+      upload_table(&E->t_init, in) || upload_table(&E->t_down, dn) ||
+      upload_table(&E->t_up, up) || upload_table(&E->t_cc, cc) || upload_table(&E->t_pq, pq) ||
+      upload_table(&E->t_xr, xr) || upload_table(&E->t_fin, fi))
+    return FS_E_HIP;
+  return 0;
+}
+
+int ens_maps(fs_ensemble* E) {
+  const int n = E->n;
+  std::vector<int> stage(n), one(n, 1), pcg(n);
+  E->maxL = E->maxF = 0;
+  for (int i = 0; i < n; i++) {
+    fs_handle* h = E->m[i];
+    stage[i] = h->nbx * h->nby;
+    pcg[i] = capped_grid(h->nbx * h->pnby);
+    E->maxL = std::max(E->maxL, (int)h->mg.size());
+    E->maxF = std::max(E->maxF, (int)h->mg_first);
+  }
+  if (make_map(E->m_stage, stage) || make_map(E->m_one, one) || make_map(E->m_pcg, pcg))
+    return FS_E_HIP;
+  for (int l = 0; l < kMgMaxLevels; l++) {
+    std::vector<int> co(n, 0), dn(n, 0);
+    for (int i = 0; i < n; i++) {
+      fs_handle* h = E->m[i];
+      if (l + 1 < (int)h->mg.size()) {
+        const MgLevel& C = h->mg[l + 1];
+        co[i] = (C.nx * C.ny + 255) / 256;
+      }
+      if (l < (int)h->mg_first) dn[i] = mg_grid(h->mg[l]);
+    }
+    if (make_map(E->m_coarsen[l], co) || make_map(E->m_down[l], dn) || make_map(E->m_up[l], dn))
+      return FS_E_HIP;
+  }
+  return 0;
+}
+
+int ens_sync(fs_ensemble* E) {
+  HIP_OK(hipMemcpyAsync(E->h_sc, E->d_sc, E->n * sizeof(DevScalars), hipMemcpyDeviceToHost, E->st));
+  HIP_OK(hipStreamSynchronize(E->st));
+  return 0;
+}
+
+// The host side of the round buffer is rewritten after each upload, so the
+// copy completes before this returns (the stream is idle at each call).
+int ens_upload_round(fs_ensemble* E) {
+  HIP_OK(hipMemcpyAsync(E->d_round, E->h_round, E->round_bytes, hipMemcpyHostToDevice, E->st));
+  HIP_OK(hipStreamSynchronize(E->st));
+  return 0;
+}
+
+int ens_fail(fs_ensemble* E, int i) {
+  fs_handle* h = E->m[i];
+  int e = check_status(h);
+  if (e) {
+    std::string msg = "ensemble member " + std::to_string(i) + ": " + g_err;
+    g_err = msg;
+  }
+  return e;
+}
+
+// One V-cycle of every Poisson member: down over the tile levels, the
+// one-workgroup coarse cycle, up.  v: 0 init, 1 odd, 2 even iteration.
+void ens_vcycle(fs_ensemble* E, int v, const uint8_t* mask) {
+  for (int l = 0; l < E->maxF; l++)
+    launch_ens_mg_down(E->t_down, with_mask(E->m_down[l], mask),
+                       EnsSel{nullptr, kMgMaxLevels * 3, l * 3 + v}, E->st);
+  launch_ens_mg_coarse_cycle(E->t_cc, with_mask(E->m_one, mask), EnsSel{nullptr, 1, 0}, E->st);
+  for (int l = E->maxF; l-- > 0;)
+    launch_ens_mg_up(E->t_up, with_mask(E->m_up[l], mask), EnsSel{nullptr, kMgMaxLevels, l}, E->st);
+}
+
+void ens_iteration(fs_ensemble* E, int it, const uint8_t* mask) {
+  const int v = it == 0 ? 0 : (it & 1) ? 1 : 2;
+  launch_ens_mgpcg_pq(E->t_pq, with_mask(E->m_pcg, mask), EnsSel{nullptr, 3, v}, E->st);
+  launch_ens_mgpcg_xr(E->t_xr, with_mask(E->m_pcg, mask), EnsSel{nullptr, 2, it & 1}, E->st);
+  ens_vcycle(E, (it & 1) ? 1 : 2, mask);
+}
+
+int ens_round(fs_ensemble* E, std::vector<int>& left) {
+  const int n = E->n;
+  EnsRound& r = E->r;
+  char* hr = E->h_round;
+  int32_t* sel = (int32_t*)hr;
+  uint8_t* ptcl = (uint8_t*)(hr + 4 * n);
+  uint8_t *pois = ptcl + n, *begin = pois + n, *acc = begin + n, *rej = acc + n;
+  memset(ptcl, 0, 5 * n);
+  // passes that start now: begin_step, then a ptcl attempt (or, before the
+  // first transport step, straight to Poisson: main.F90:75)
+  int nbeg = 0;
+  for (int i = 0; i < n; i++)
+    if (left[i] > 0 && E->phase[i] == 0) {
+      begin[i] = 1;
+      nbeg++;
+      E->phase[i] = E->h_sc[i].ts > 1.0 ? 1 : 2;
+    }
+  int nptcl = 0;
+  for (int i = 0; i < n; i++) {
+    ptcl[i] = left[i] > 0 && E->phase[i] == 1;
+    nptcl += ptcl[i];
+    sel[i] = i * 6 + E->m[i]->rot * 2 + (E->m[i]->alias ? 1 : 0);
+  }
+  if (ens_upload_round(E)) return FS_E_HIP;
+  if (nbeg) launch_ens_scalars(E->d_sc, r.begin, n, 0, E->st);
+  if (nptcl) {
+    for (int stg = 1; stg <= 5; stg++) {
+      launch_ens_stage(E->t_stage, E->t_phys, with_mask(E->m_stage, r.ptcl),
+                       EnsSel{r.sel, 5, stg - 1}, E->st);
+      launch_ens_circ(E->t_circ, E->t_phys, with_mask(E->m_one, r.ptcl), EnsSel{nullptr, 5, stg - 1},
+                      E->st);
+    }
+    launch_ens_ctrl(E->t_ctrl, with_mask(E->m_one, r.ptcl), EnsSel{nullptr, 1, 0}, E->st);
+    HIP_OK(hipGetLastError());
+    if (ens_sync(E)) return FS_E_HIP;
+    for (int i = 0; i < n; i++) {
+      if (!ptcl[i]) continue;
+      fs_handle* h = E->m[i];
+      log_attempt(h);
+      if (int e = ens_fail(E, i)) return e;
+      h->alias = false;
+      if (E->h_sc[i].accept) {
+        E->phase[i] = 2;
+      } else if (copy_level(h, h->bufA(), h->bufL3())) {  // ptcl_lib.F90:76-93
+        return FS_E_HIP;
+      }
+    }
+  }
+  int npois = 0;
+  for (int i = 0; i < n; i++) {
+    pois[i] = left[i] > 0 && E->phase[i] == 2;
+    npois += pois[i];
+    sel[i] = i * 6 + E->m[i]->rot * 2 + (E->m[i]->alias ? 1 : 0);
+  }
+  if (!npois) return 0;
+  if (ens_upload_round(E)) return FS_E_HIP;
+  launch_ens_lapl(E->t_lapl, E->t_phys, with_mask(E->m_stage, r.pois), EnsSel{r.sel, 1, 0}, E->st);
+  for (int l = 0; l + 1 < E->maxL; l++)
+    launch_ens_mg_coarsen(E->t_coarsen, with_mask(E->m_coarsen[l], r.pois),
+                          EnsSel{nullptr, kMgMaxLevels, l}, E->st);
+  launch_ens_mg_factor(E->t_factor, E->t_inv, with_mask(E->m_one, r.pois), EnsSel{nullptr, 1, 0},
+                       E->st);
+  launch_ens_mgpcg_init(E->t_init, with_mask(E->m_pcg, r.pois), EnsSel{nullptr, 1, 0}, E->st);
+  ens_vcycle(E, 0, r.pois);
+  ens_iteration(E, 0, r.pois);
+  int it = 1;
+  for (;;) {
+    for (int k = 0; k < 8; k++, it++) ens_iteration(E, it, r.pois);
+    HIP_OK(hipGetLastError());
+    if (ens_sync(E)) return FS_E_HIP;
+    bool all = true;
+    for (int i = 0; i < n; i++) {
+      if (!pois[i]) continue;
+      if (E->h_sc[i].status < 0) return ens_fail(E, i);
+      all = all && E->h_sc[i].done;
+    }
+    if (all) break;
+  }
+  launch_ens_pcg_finish(E->t_fin, with_mask(E->m_pcg, r.pois), with_mask(E->m_one, r.pois),
+                        EnsSel{nullptr, 1, 0}, E->st);
+  if (ens_sync(E)) return FS_E_HIP;
+  for (int i = 0; i < n; i++) {
+    if (!pois[i]) continue;
+    fs_handle* h = E->m[i];
+    if (int e = ens_fail(E, i)) return e;
+    const int lerr = E->h_sc[i].done == 1 ? 0 : 1;
+    log_solve(h, lerr);
+    acc[i] = lerr == 0;
+    rej[i] = lerr != 0;
+  }
+  if (ens_upload_round(E)) return FS_E_HIP;
+  launch_ens_scalars(E->d_sc, r.acc, n, 1, E->st);
+  launch_ens_scalars(E->d_sc, r.rej, n, 2, E->st);
+  for (int i = 0; i < n; i++) {
+    if (!pois[i]) continue;
+    fs_handle* h = E->m[i];
+    if (acc[i]) {
+      h->rot = (h->rot + 1) % 3;
+    } else if (copy_level(h, h->bufA(), h->bufL3())) {
+      return FS_E_HIP;
+    }
+    h->alias = true;
+    E->phase[i] = 0;
+    left[i]--;
+  }
+  HIP_OK(hipGetLastError());
+  return ens_sync(E);
+}
+
+void ens_release(fs_ensemble* E) {
+  for (size_t i = 0; i < E->m.size(); i++) {
+    fs_handle* h = E->m[i];
+    if (i < E->own_d.size()) {
+      h->d_sc = E->own_d[i];
+      h->h_sc = E->own_h[i];
+    }
+    fs_destroy(h);
+  }
+  auto fr = [](void* p) {
+    if (p) (void)hipFree(p);
+  };
+  fr(E->t_phys); fr(E->t_stage); fr(E->t_circ); fr(E->t_ctrl); fr(E->t_lapl); fr(E->t_coarsen);
+  fr(E->t_factor); fr(E->t_inv); fr(E->t_init); fr(E->t_down); fr(E->t_up); fr(E->t_cc);
+  fr(E->t_pq); fr(E->t_xr); fr(E->t_fin); fr(E->d_sc); fr(E->d_round);
+  fr(E->m_stage.d); fr(E->m_one.d); fr(E->m_pcg.d);
+  for (int l = 0; l < kMgMaxLevels; l++) {
+    fr(E->m_coarsen[l].d);
+    fr(E->m_down[l].d);
+    fr(E->m_up[l].d);
+  }
+  if (E->h_sc) (void)hipHostFree(E->h_sc);
+  if (E->h_round) (void)hipHostFree(E->h_round);
+  if (E->st) (void)hipStreamDestroy(E->st);
+  delete E;
 }
 
 }  // namespace
@@ -1396,6 +1802,104 @@ int fs_set_solver(fs_handle* h, int32_t solver) {
   if (solver < 0 || solver > 2) return fail(FS_E_ARG, "unknown Poisson solver%s");
   h->p.lapl_solver = solver;
   return 0;
+}
+
+int fs_ensemble_create(int32_t n, const fs_grid_desc* g, const fs_params* p, fs_ensemble** out) {
+  if (n <= 0 || !g || !p || !out) return fail(FS_E_ARG, "bad argument%s");
+  fs_ensemble* E = new fs_ensemble();
+  E->n = n;
+  for (int i = 0; i < n; i++) {
+    if (p[i].lapl_solver != 1) {
+      ens_release(E);
+      return fail(FS_E_ARG, "ensemble members use the multigrid solver (lapl_solver 1)%s");
+    }
+    fs_handle* h = nullptr;
+    if (int e = fs_create(&g[i], &p[i], &h)) {
+      ens_release(E);
+      return e;
+    }
+    E->m.push_back(h);
+    if (h->mg.size() < 2) {
+      ens_release(E);
+      return fail(FS_E_ARG, "ensemble member grid of <= 64 cells%s");
+    }
+  }
+  // the members' scalars move into one array so that a round reads them with
+  // one copy; the members' graphs (made with the old address) are dropped
+  if (bad(hipStreamCreate(&E->st)) || bad(hipMalloc(&E->d_sc, n * sizeof(DevScalars))) ||
+      bad(hipHostMalloc(&E->h_sc, n * sizeof(DevScalars)))) {
+    ens_release(E);
+    return fail(FS_E_HIP, "out of memory for the ensemble%s");
+  }
+  for (int i = 0; i < n; i++) {
+    fs_handle* h = E->m[i];
+    if (sync_scalars(h)) {
+      ens_release(E);
+      return FS_E_HIP;
+    }
+    E->h_sc[i] = *h->h_sc;
+    E->own_d.push_back(h->d_sc);
+    E->own_h.push_back(h->h_sc);
+    h->d_sc = E->d_sc + i;
+    h->h_sc = E->h_sc + i;
+    if (h->mg_graph) (void)hipGraphExecDestroy(h->mg_graph);
+    h->mg_graph = nullptr;
+  }
+  E->round_bytes = (size_t)n * 9;
+  if (bad(hipMemcpy(E->d_sc, E->h_sc, n * sizeof(DevScalars), hipMemcpyHostToDevice)) ||
+      bad(hipMalloc(&E->d_round, E->round_bytes)) ||
+      bad(hipHostMalloc(&E->h_round, E->round_bytes)) || ens_maps(E)) {
+    ens_release(E);
+    return fail(FS_E_HIP, "ensemble set-up failed%s");
+  }
+  E->r.sel = (int32_t*)E->d_round;
+  E->r.ptcl = (uint8_t*)(E->d_round + 4 * n);
+  E->r.pois = E->r.ptcl + n;
+  E->r.begin = E->r.pois + n;
+  E->r.acc = E->r.begin + n;
+  E->r.rej = E->r.acc + n;
+  E->phase.assign(n, 0);
+  *out = E;
+  return 0;
+}
+
+int fs_ensemble_destroy(fs_ensemble* E) {
+  if (!E) return 0;
+  if (E->st) (void)hipStreamSynchronize(E->st);
+  ens_release(E);
+  return 0;
+}
+
+fs_handle* fs_ensemble_member(fs_ensemble* E, int32_t i) {
+  if (!E || i < 0 || i >= E->n) {
+    fail(FS_E_ARG, "no such ensemble member%s");
+    return nullptr;
+  }
+  return E->m[i];
+}
+
+int fs_ensemble_run(fs_ensemble* E, int32_t passes) {
+  if (!E || passes < 0) return fail(FS_E_ARG, "bad argument%s");
+  const int n = E->n;
+  for (fs_handle* h : E->m) HIP_OK(hipStreamSynchronize(h->st));
+  if (ens_tables(E)) return FS_E_HIP;
+  // members work on the ensemble's stream for the run (restart copies)
+  E->own_st.clear();
+  for (fs_handle* h : E->m) {
+    E->own_st.push_back(h->st);
+    h->st = E->st;
+  }
+  std::vector<int> left(n, passes);
+  int err = ens_sync(E);
+  while (!err) {
+    bool any = false;
+    for (int i = 0; i < n; i++) any = any || left[i] > 0;
+    if (!any) break;
+    err = ens_round(E, left);
+  }
+  if (!err && hipStreamSynchronize(E->st) != hipSuccess) err = fail(FS_E_HIP, "stream%s");
+  for (int i = 0; i < n; i++) E->m[i]->st = E->own_st[i];
+  return err;
 }
 
 int fs_learn_create(int32_t capacity, int32_t H, int32_t W, uint64_t seed, fs_learn** out) {

@@ -185,6 +185,48 @@ constexpr int kNT = kTX * kTY;
 constexpr int kPcgTY = 8;    // PCG tile height: each thread owns kPcgRows cells
 constexpr int kPcgRows = kPcgTY / kTY;
 
+// ---- ensemble launches --------------------------------------------------------
+// One launch covers the blocks of every ensemble member that takes part.  Block b
+// of a launch belongs to member inst[b] as that member's block b - first[member]
+// of nb[member] -- the shape of the member's own launch -- so every member
+// computes exactly what its own launch computes.
+struct EnsMap {
+  const int32_t* inst;
+  const int32_t* first;
+  const int32_t* nb;
+  const uint8_t* active;  // per member: takes part in this launch
+  int32_t nblocks;        // blocks of the launch
+};
+// A member's arguments: table[(sel ? sel[member] : member) * stride + off].
+struct EnsSel {
+  const int32_t* sel;
+  int32_t stride, off;
+};
+
+void launch_ens_stage(const StageArgs* t, const FsPhys* K, const EnsMap& m, const EnsSel& s,
+                      hipStream_t st);
+void launch_ens_circ(const CircArgs* t, const FsPhys* K, const EnsMap& m, const EnsSel& s,
+                     hipStream_t st);
+void launch_ens_ctrl(const CtrlArgs* t, const EnsMap& m, const EnsSel& s, hipStream_t st);
+// op: 0 begin step, 1 accept, 2 reject; one thread per member of mask
+void launch_ens_scalars(DevScalars* sc, const uint8_t* mask, int n, int op, hipStream_t st);
+void launch_ens_lapl(const LaplArgs* t, const FsPhys* K, const EnsMap& m, const EnsSel& s,
+                     hipStream_t st);
+void launch_ens_mg_coarsen(const MgArgs* t, const EnsMap& m, const EnsSel& s, hipStream_t st);
+void launch_ens_mg_factor(const MgArgs* t, double* const* inv, const EnsMap& m, const EnsSel& s,
+                          hipStream_t st);
+void launch_ens_mg_down(const MgArgs* t, const EnsMap& m, const EnsSel& s, hipStream_t st);
+void launch_ens_mg_up(const MgArgs* t, const EnsMap& m, const EnsSel& s, hipStream_t st);
+void launch_ens_mg_coarse_cycle(const MgCycleArgs* t, const EnsMap& m, const EnsSel& s,
+                                hipStream_t st);
+void launch_ens_mgpcg_init(const MgPcgArgs* t, const EnsMap& m, const EnsSel& s, hipStream_t st);
+void launch_ens_mgpcg_pq(const MgPcgArgs* t, const EnsMap& m, const EnsSel& s, hipStream_t st);
+void launch_ens_mgpcg_xr(const MgPcgArgs* t, const EnsMap& m, const EnsSel& s, hipStream_t st);
+// finish over map m (the members' capped PCG grids), then the true residual
+// with one block per member (map one, partial counts m.nb)
+void launch_ens_pcg_finish(const PcgArgs* t, const EnsMap& m, const EnsMap& one, const EnsSel& s,
+                           hipStream_t st);
+
 void launch_stage(const StageArgs& a, const FsPhys& K, hipStream_t s);
 void launch_circ(const CircArgs& a, const FsPhys& K, hipStream_t s);
 void launch_ctrl(const CtrlArgs& a, hipStream_t s);

@@ -41,9 +41,9 @@ __device__ __forceinline__ int xcd_tile(int b, int nb) {
 struct TileRange {
   int t0, t1;
 };
-__device__ __forceinline__ TileRange tile_range(int ntiles) {
-  const int b = xcd_tile(blockIdx.x, gridDim.x);
-  const int per = (ntiles + (int)gridDim.x - 1) / (int)gridDim.x;
+__device__ __forceinline__ TileRange tile_range(int ntiles, int bid, int nb) {
+  const int b = xcd_tile(bid, nb);
+  const int per = (ntiles + nb - 1) / nb;
   const int t0 = b * per;
   return {t0, t0 + per < ntiles ? t0 + per : ntiles};
 }
@@ -171,7 +171,7 @@ __device__ __forceinline__ Flux4 interior_face(const FsPhys& K, double E, double
 }
 
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(kNT) k_stage(StageArgs A, FsPhys K) {
+__device__ __forceinline__ void stage_body(const StageArgs& A, const FsPhys& K, int bid, int nb) {
   __shared__ double s_ni[kNH], s_ne[kNH], s_nt[kNH], s_nm[kNH], s_ph[kNH];
   __shared__ double s_te[kNH], s_mue[kNH], s_de[kNH], s_xa[kNH];
   __shared__ double s_f[4][kNF];
@@ -180,7 +180,7 @@ __global__ void __launch_bounds__(kNT) k_stage(StageArgs A, FsPhys K) {
 
   const Geo& G = A.G;
   const int bx = G.bx, by = G.by, P = G.P;
-  const int t = xcd_tile(blockIdx.x, gridDim.x);
+  const int t = xcd_tile(bid, nb);
   const int i0 = (t % A.nbx) * kTX, j0 = (t / A.nbx) * kTY;
   const double dt = A.sc->dt;
   if (threadIdx.x == 0) s_bad = 0;
@@ -394,7 +394,7 @@ __global__ void __launch_bounds__(kNT) k_stage(StageArgs A, FsPhys K) {
     block_sum<4>(acc, s_red);
     double tm = block_min(tmin, s_red);
     if (threadIdx.x == 0) {
-      double* p = A.part + (size_t)blockIdx.x * 8;
+      double* p = A.part + (size_t)bid * 8;
       p[0] = acc[0];
       p[1] = acc[1];
       p[2] = acc[2];
@@ -406,6 +406,8 @@ __global__ void __launch_bounds__(kNT) k_stage(StageArgs A, FsPhys K) {
   if (threadIdx.x == 0 && s_bad)
     atomicMin(&A.sc->status, s_bad == 1 ? FS_E_NEGDENS : FS_E_NAN);
 }
+
+__global__ void __launch_bounds__(kNT) k_stage(StageArgs A, FsPhys K) { stage_body(A, K, blockIdx.x, gridDim.x); }
 
 // ---------------------------------------------------------------------------
 __device__ double vsource(const CircArgs& C, const FsPhys& K, double t) {
@@ -443,7 +445,7 @@ __device__ double vsource(const CircArgs& C, const FsPhys& K, double t) {
   }
 }
 
-__global__ void __launch_bounds__(kNT) k_circ(CircArgs C, FsPhys K) {
+__device__ __forceinline__ void circ_body(const CircArgs& C, const FsPhys& K) {
   __shared__ double s_red[8 * kNW];
   __shared__ double s_vd;
   DevScalars* sc = C.sc;
@@ -483,8 +485,10 @@ __global__ void __launch_bounds__(kNT) k_circ(CircArgs C, FsPhys K) {
   }
 }
 
+__global__ void __launch_bounds__(kNT) k_circ(CircArgs C, FsPhys K) { circ_body(C, K); }
+
 // dt controller: ptcl_lib.F90:59-94 (plus replayed decisions)
-__global__ void __launch_bounds__(kNT) k_ctrl(CtrlArgs A) {
+__device__ __forceinline__ void ctrl_body(const CtrlArgs& A) {
   __shared__ double s_red[8 * kNW];
   double v[4] = {0, 0, 0, 0};
   double tm = 1e300;
@@ -531,6 +535,8 @@ __global__ void __launch_bounds__(kNT) k_ctrl(CtrlArgs A) {
   }
 }
 
+__global__ void __launch_bounds__(kNT) k_ctrl(CtrlArgs A) { ctrl_body(A); }
+
 __global__ void k_begin_step(DevScalars* sc) {
   sc->t_m = (double)1e9f;  // main.F90:74
   sc->attempts = 0;
@@ -570,14 +576,14 @@ __device__ __forceinline__ FaceJ lapl_face(const FsPhys& K, double E, double h, 
   return F;
 }
 
-__global__ void __launch_bounds__(kNT) k_lapl(LaplArgs A, FsPhys K) {
+__device__ __forceinline__ void lapl_body(const LaplArgs& A, const FsPhys& K, int bid, int nb) {
   __shared__ double s_ni[kNH], s_ne[kNH], s_ph[kNH], s_te[kNH], s_mue[kNH], s_de[kNH];
   __shared__ double s_f[4][kNF];
   __shared__ double s_red[8 * kNW];
   __shared__ int s_bad;
   const Geo& G = A.G;
   const int bx = G.bx, by = G.by, P = G.P;
-  const int t = xcd_tile(blockIdx.x, gridDim.x);
+  const int t = xcd_tile(bid, nb);
   const int i0 = (t % A.nbx) * kTX, j0 = (t / A.nbx) * kTY;
   const double dt = A.sc->dt;
   if (threadIdx.x == 0) s_bad = 0;
@@ -819,11 +825,13 @@ __global__ void __launch_bounds__(kNT) k_lapl(LaplArgs A, FsPhys K) {
   }
   if (threadIdx.x == 0 && s_bad == 1) atomicMin(&A.sc->status, FS_E_NEGDENS);
   if (threadIdx.x == 0 && s_bad == 3) atomicMin(&A.sc->status, FS_E_NANJ);
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+  if (bid == 0 && threadIdx.x == 0) {
     A.sc->done = 0;
     A.sc->iters = 0;
   }
 }
+
+__global__ void __launch_bounds__(kNT) k_lapl(LaplArgs A, FsPhys K) { lapl_body(A, K, blockIdx.x, gridDim.x); }
 
 // ---------------------------------------------------------------------------
 // PCG on M = -V*A (SPD), Chronopoulos-Gear form: one kernel per iteration and
@@ -872,12 +880,13 @@ __device__ __forceinline__ void reduce_partials3(const double* part, int nb, dou
   finish_partials3(v, s_red, tot);
 }
 
-__device__ __forceinline__ void write_partials3(double (&v)[3], double* part, double* s_red) {
+__device__ __forceinline__ void write_partials3(double (&v)[3], double* part, double* s_red,
+                                                int bid) {
   block_sum<3>(v, s_red);
   if (threadIdx.x == 0) {
-    part[3 * blockIdx.x] = v[0];
-    part[3 * blockIdx.x + 1] = v[1];
-    part[3 * blockIdx.x + 2] = v[2];
+    part[3 * bid] = v[0];
+    part[3 * bid + 1] = v[1];
+    part[3 * bid + 2] = v[2];
   }
 }
 
@@ -886,7 +895,7 @@ __global__ void __launch_bounds__(kNT) k_pcg_init(PcgArgs A) {
   __shared__ double s_red[8 * kNW];
   const Geo& G = A.G;
   const int P = G.P;
-  const TileRange tr = tile_range(A.nbx * A.nby);
+  const TileRange tr = tile_range(A.nbx * A.nby, blockIdx.x, gridDim.x);
   double acc[3] = {0.0, 0.0, 0.0};
   for (int t = tr.t0; t < tr.t1; t++) {
     for (int k = 0; k < kPcgRows; k++) {
@@ -910,7 +919,7 @@ __global__ void __launch_bounds__(kNT) k_pcg_init(PcgArgs A) {
       }
     }
   }
-  write_partials3(acc, A.part_out, s_red);
+  write_partials3(acc, A.part_out, s_red, blockIdx.x);
 }
 
 __global__ void __launch_bounds__(kNT) k_pcg_it(PcgArgs A) {
@@ -945,7 +954,7 @@ __global__ void __launch_bounds__(kNT) k_pcg_it(PcgArgs A) {
 
   const Geo& G = A.G;
   const int P = G.P;
-  const TileRange tr = tile_range(A.nbx * A.nby);
+  const TileRange tr = tile_range(A.nbx * A.nby, blockIdx.x, gridDim.x);
   const double *r = A.r_old, *w = A.w_old, *so = A.s_old, *d = A.dg;
   double acc[3] = {0.0, 0.0, 0.0};
   for (int t = tr.t0; t < tr.t1; t++) {
@@ -975,17 +984,17 @@ __global__ void __launch_bounds__(kNT) k_pcg_it(PcgArgs A) {
       }
     }
   }
-  write_partials3(acc, A.part_out, s_red);
+  write_partials3(acc, A.part_out, s_red, blockIdx.x);
 }
 
 // After the solve: ph += delta (lapl_lib.F90:112-116) when converged, and the
 // partials of the true residual |(rhs0 - M x)/V|^2.
-__global__ void __launch_bounds__(kNT) k_pcg_finish(PcgArgs A) {
+__device__ __forceinline__ void pcg_finish_body(const PcgArgs& A, int bid, int nb) {
   __shared__ double s_red[8 * kNW];
   DevScalars* sc = A.sc;
   const Geo& G = A.G;
   const int P = G.P;
-  const TileRange tr = tile_range(A.nbx * A.nby);
+  const TileRange tr = tile_range(A.nbx * A.nby, bid, nb);
   const bool ok = sc->done == 1;
   double acc[3] = {0.0, 0.0, 0.0};
   for (int t = tr.t0; t < tr.t1; t++) {
@@ -1004,11 +1013,13 @@ __global__ void __launch_bounds__(kNT) k_pcg_finish(PcgArgs A) {
       }
     }
   }
-  write_partials3(acc, A.part_out, s_red);
+  write_partials3(acc, A.part_out, s_red, bid);
 }
 
+__global__ void __launch_bounds__(kNT) k_pcg_finish(PcgArgs A) { pcg_finish_body(A, blockIdx.x, gridDim.x); }
+
 // Single block: true residual from the finish partials.
-__global__ void __launch_bounds__(kNT) k_pcg_tres(PcgArgs A, int nb) {
+__device__ __forceinline__ void pcg_tres_body(const PcgArgs& A, int nb) {
   __shared__ double s_red[8 * kNW];
   double tot[3];
   reduce_partials3(A.part_in, nb, s_red, tot);
@@ -1017,6 +1028,8 @@ __global__ void __launch_bounds__(kNT) k_pcg_tres(PcgArgs A, int nb) {
     sc->tres = sc->bnorm2 > 0 ? sqrt(tot[0] / sc->bnorm2) : sqrt(tot[0]);
   }
 }
+
+__global__ void __launch_bounds__(kNT) k_pcg_tres(PcgArgs A, int nb) { pcg_tres_body(A, nb); }
 
 // ---------------------------------------------------------------------------
 // Aggregation multigrid V(2,2)-cycle used as the PCG preconditioner.
@@ -1041,9 +1054,9 @@ __device__ __forceinline__ size_t mg_at(const MgLevel& L, int i, int j) {
   return (size_t)(j + 1) * L.P + (i + 1);
 }
 
-__global__ void __launch_bounds__(256) k_mg_coarsen(MgArgs A) {
+__device__ __forceinline__ void mg_coarsen_body(const MgArgs& A, int bid, int nb) {
   const MgLevel &F = A.F, &C = A.C;
-  int n = blockIdx.x * blockDim.x + threadIdx.x;
+  int n = bid * blockDim.x + threadIdx.x;
   if (n >= C.nx * C.ny) return;
   int I = n % C.nx, J = n / C.nx;
   int i0 = 2 * I, j0 = 2 * J;
@@ -1078,6 +1091,8 @@ __global__ void __launch_bounds__(256) k_mg_coarsen(MgArgs A) {
   C.cx[m] = cx;
   C.cy[m] = cy;
 }
+
+__global__ void __launch_bounds__(256) k_mg_coarsen(MgArgs A) { mg_coarsen_body(A, blockIdx.x, gridDim.x); }
 
 constexpr int kGsTX = 32, kGsTY = 16, kGsH = 4;  // tile and halo (four half-sweeps)
 constexpr int kGsW = kGsTX + 2 * kGsH, kGsR = kGsTY + 2 * kGsH;
@@ -1142,7 +1157,7 @@ __device__ __forceinline__ int gs_ntiles(const MgLevel& F) {
 // the coarse level.  Level 0 first checks convergence from the x/r-update
 // partials (every block identically); their loads are issued together with
 // the first tile's.
-__global__ void __launch_bounds__(kNT) k_mg_down(MgArgs A) {
+__device__ __forceinline__ void mg_down_body(const MgArgs& A, int bid, int nb) {
   __shared__ GsTile s;
   __shared__ double s_res[kGsTY][kGsTX / 2];
   __shared__ double s_red[8 * kNW];
@@ -1150,10 +1165,10 @@ __global__ void __launch_bounds__(kNT) k_mg_down(MgArgs A) {
   if (sc->done) return;
   const MgLevel &F = A.F, &C = A.C;
   const int ntx = gs_tiles_x(F);
-  const TileRange tr = tile_range(gs_ntiles(F));
+  const TileRange tr = tile_range(gs_ntiles(F), bid, nb);
   double sacc[3] = {0.0, 0.0, 0.0};  // sum r_c^2 / d_c for the learned coarse solver
   if (tr.t0 >= tr.t1) {  // block 0 always has tiles (it writes the scalars)
-    if (A.part_scale) write_partials3(sacc, A.part_scale, s_red);
+    if (A.part_scale) write_partials3(sacc, A.part_scale, s_red, bid);
     return;
   }
   double pv[3];
@@ -1164,7 +1179,7 @@ __global__ void __launch_bounds__(kNT) k_mg_down(MgArgs A) {
     finish_partials3(pv, s_red, tot);
     const double rho = tot[0];
     if (A.init) {
-      if (blockIdx.x == 0 && threadIdx.x == 0) {
+      if (bid == 0 && threadIdx.x == 0) {
         sc->bnorm2 = rho;
         sc->resid = rho > 0 ? 1.0 : 0.0;
         sc->iters = 0;
@@ -1180,7 +1195,7 @@ __global__ void __launch_bounds__(kNT) k_mg_down(MgArgs A) {
       const double res = sqrt(rho / sc->bnorm2);
       const bool conv = res <= sc->tol;
       const bool stop = conv || cur >= sc->maxit;
-      if (blockIdx.x == 0 && threadIdx.x == 0) {
+      if (bid == 0 && threadIdx.x == 0) {
         sc->itc[A.it & 1] = cur;
         sc->iters = cur;
         sc->resid = res;
@@ -1232,18 +1247,20 @@ __global__ void __launch_bounds__(kNT) k_mg_down(MgArgs A) {
       }
     }
   }
-  if (A.part_scale) write_partials3(sacc, A.part_scale, s_red);
+  if (A.part_scale) write_partials3(sacc, A.part_scale, s_red, bid);
 }
+
+__global__ void __launch_bounds__(kNT) k_mg_down(MgArgs A) { mg_down_body(A, blockIdx.x, gridDim.x); }
 
 // Up: x2 = x1 + oc * e_c(aggregate) on the staged halo, post-smooth, write e.
 // Level 0 writes u = B r and the partials of gamma = r.u.
-__global__ void __launch_bounds__(kNT) k_mg_up(MgArgs A) {
+__device__ __forceinline__ void mg_up_body(const MgArgs& A, int bid, int nb) {
   __shared__ GsTile s;
   __shared__ double s_red[8 * kNW];
   if (A.sc->done) return;
   const MgLevel &F = A.F, &C = A.C;
   const int ntx = gs_tiles_x(F);
-  const TileRange tr = tile_range(gs_ntiles(F));
+  const TileRange tr = tile_range(gs_ntiles(F), bid, nb);
   double acc[3] = {0.0, 0.0, 0.0};
   for (int t = tr.t0; t < tr.t1; t++) {
     const int i0 = (t % ntx) * kGsTX, j0 = (t / ntx) * kGsTY;
@@ -1278,15 +1295,17 @@ __global__ void __launch_bounds__(kNT) k_mg_up(MgArgs A) {
       }
     }
   }
-  if (A.level0) write_partials3(acc, A.part_out, s_red);
+  if (A.level0) write_partials3(acc, A.part_out, s_red, bid);
 }
+
+__global__ void __launch_bounds__(kNT) k_mg_up(MgArgs A) { mg_up_body(A, blockIdx.x, gridDim.x); }
 
 // Coarsest level (<= 64 cells): its operator is inverted once per solve by
 // Gauss-Jordan elimination (SPD, so no pivoting).  Thread (row, g) keeps the
 // four entries (row, g + 16 c) in registers; each pivot step publishes the
 // pivot row and column through LDS buffers alternated by step parity, so one
 // barrier per step suffices.  The inverse is stored column-major.
-__global__ void __launch_bounds__(1024) k_mg_factor(MgArgs A, double* inv) {
+__device__ __forceinline__ void mg_factor_body(const MgArgs& A, double* inv) {
   constexpr int kG = 1024 / kMgBottomMax, kC = kMgBottomMax / kG;
   __shared__ double s_prow[2][kMgBottomMax], s_pcol[2][kMgBottomMax];
   const MgLevel& L = A.F;
@@ -1334,12 +1353,14 @@ __global__ void __launch_bounds__(1024) k_mg_factor(MgArgs A, double* inv) {
   }
 }
 
+__global__ void __launch_bounds__(1024) k_mg_factor(MgArgs A, double* inv) { mg_factor_body(A, inv); }
+
 // Levels f .. L-1 (<= 1024 cells each, at most kMgLdsLevels) in one workgroup
 // of 1024 threads: thread t owns cell t of every level, keeps that cell's
 // operator row and right-hand side in registers, and the iterates live in LDS
 // with a zero ghost ring.  Same cycle as the tile kernels; level f's
 // correction is written back for the tile levels above.
-__global__ void __launch_bounds__(1024) k_mg_coarse_cycle(MgCycleArgs A) {
+__device__ __forceinline__ void mg_coarse_cycle_body(const MgCycleArgs& A) {
   __shared__ double s_x[kMgLdsX];
   __shared__ double s_r[1024];
   __shared__ double s_bp[1024 / kMgBottomMax][kMgBottomMax];
@@ -1453,11 +1474,13 @@ __global__ void __launch_bounds__(1024) k_mg_coarse_cycle(MgCycleArgs A) {
   }
 }
 
+__global__ void __launch_bounds__(1024) k_mg_coarse_cycle(MgCycleArgs A) { mg_coarse_cycle_body(A); }
+
 // MG-PCG: x = 0, r = rhs (from the assembly), p = 0, partials of |rhs/V|^2.
-__global__ void __launch_bounds__(kNT) k_mgpcg_init(MgPcgArgs A) {
+__device__ __forceinline__ void mgpcg_init_body(const MgPcgArgs& A, int bid, int nb) {
   __shared__ double s_red[8 * kNW];
   const Geo& G = A.G;
-  const TileRange tr = tile_range(A.nbx * A.nby);
+  const TileRange tr = tile_range(A.nbx * A.nby, bid, nb);
   double acc[3] = {0.0, 0.0, 0.0};
   for (int t = tr.t0; t < tr.t1; t++) {
     for (int k = 0; k < kPcgRows; k++) {
@@ -1473,8 +1496,10 @@ __global__ void __launch_bounds__(kNT) k_mgpcg_init(MgPcgArgs A) {
       }
     }
   }
-  write_partials3(acc, A.part_out, s_red);
+  write_partials3(acc, A.part_out, s_red, bid);
 }
+
+__global__ void __launch_bounds__(kNT) k_mgpcg_init(MgPcgArgs A) { mgpcg_init_body(A, blockIdx.x, gridDim.x); }
 
 // Cells of one 64x8 PCG tile held by a thread (kPcgRows of them).
 struct PqCells {
@@ -1509,11 +1534,11 @@ __device__ __forceinline__ void pq_load(const MgPcgArgs& A, int t, PqCells& c) {
 
 // p = u + beta p_old (neighbours recomputed), q = M p, partials of p.q.
 // The first tile's loads are issued before the reduction that yields beta.
-__global__ void __launch_bounds__(kNT) k_mgpcg_pq(MgPcgArgs A) {
+__device__ __forceinline__ void mgpcg_pq_body(const MgPcgArgs& A, int bid, int nb) {
   __shared__ double s_red[8 * kNW];
   DevScalars* sc = A.sc;
   if (sc->done) return;
-  const TileRange tr = tile_range(A.nbx * A.nby);
+  const TileRange tr = tile_range(A.nbx * A.nby, bid, nb);
   double pv[3];
   load_partials3(A.part_in, A.nb_in, pv);
   PqCells c;
@@ -1526,7 +1551,7 @@ __global__ void __launch_bounds__(kNT) k_mgpcg_pq(MgPcgArgs A) {
   const double beta = A.first  ? 0.0
                       : A.flex ? -tot[1] / sc->cg_pq[(A.it + 1) & 1]
                                : gam / sc->cg_gam[(A.it + 1) & 1];
-  if (blockIdx.x == 0 && threadIdx.x == 0) sc->cg_gam[A.it & 1] = gam;
+  if (bid == 0 && threadIdx.x == 0) sc->cg_gam[A.it & 1] = gam;
   double acc[3] = {0.0, 0.0, 0.0};
   for (int t = tr.t0; t < tr.t1; t++) {
     if (t > tr.t0) pq_load(A, t, c);
@@ -1543,8 +1568,10 @@ __global__ void __launch_bounds__(kNT) k_mgpcg_pq(MgPcgArgs A) {
       acc[0] += pn[0] * q;
     }
   }
-  write_partials3(acc, A.part_out, s_red);
+  write_partials3(acc, A.part_out, s_red, bid);
 }
+
+__global__ void __launch_bounds__(kNT) k_mgpcg_pq(MgPcgArgs A) { mgpcg_pq_body(A, blockIdx.x, gridDim.x); }
 
 struct XrCells {
   double x[kPcgRows], p[kPcgRows], r[kPcgRows], q[kPcgRows], vol[kPcgRows];
@@ -1572,11 +1599,11 @@ __device__ __forceinline__ void xr_load(const MgPcgArgs& A, int t, XrCells& c) {
 
 // alpha = gamma / p.q; x += alpha p; r -= alpha q; partials of |r/V|^2.
 // The first tile's loads are issued before the reduction that yields alpha.
-__global__ void __launch_bounds__(kNT) k_mgpcg_xr(MgPcgArgs A) {
+__device__ __forceinline__ void mgpcg_xr_body(const MgPcgArgs& A, int bid, int nb) {
   __shared__ double s_red[8 * kNW];
   DevScalars* sc = A.sc;
   if (sc->done) return;
-  const TileRange tr = tile_range(A.nbx * A.nby);
+  const TileRange tr = tile_range(A.nbx * A.nby, bid, nb);
   double pv[3];
   load_partials3(A.part_in, A.nb_in, pv);
   XrCells c;
@@ -1584,7 +1611,7 @@ __global__ void __launch_bounds__(kNT) k_mgpcg_xr(MgPcgArgs A) {
   double tot[3];
   finish_partials3(pv, s_red, tot);
   const double alpha = sc->cg_gam[A.it & 1] / tot[0];
-  if (blockIdx.x == 0 && threadIdx.x == 0) sc->cg_pq[A.it & 1] = tot[0];
+  if (bid == 0 && threadIdx.x == 0) sc->cg_pq[A.it & 1] = tot[0];
   double acc[3] = {0.0, 0.0, 0.0};
   for (int t = tr.t0; t < tr.t1; t++) {
     if (t > tr.t0) xr_load(A, t, c);
@@ -1598,8 +1625,10 @@ __global__ void __launch_bounds__(kNT) k_mgpcg_xr(MgPcgArgs A) {
       acc[0] += rn * rn;
     }
   }
-  write_partials3(acc, A.part_out, s_red);
+  write_partials3(acc, A.part_out, s_red, bid);
 }
+
+__global__ void __launch_bounds__(kNT) k_mgpcg_xr(MgPcgArgs A) { mgpcg_xr_body(A, blockIdx.x, gridDim.x); }
 
 __global__ void __launch_bounds__(256) k_copy(const double2* __restrict__ a,
                                              double2* __restrict__ b, size_t n) {
@@ -1639,6 +1668,103 @@ __global__ void k_eval_flux(FsPhys K, const double* a, double* out, int n, int d
   double f, df;
   sg_flux_dfdph(K, v[0], v[1], v[2], v[3], v[4], v[5], v[6], f, df);
   out[m] = dfdph ? df : sg_flux(K, v[0], v[1], v[2], v[3], v[4], v[5], v[6]);
+}
+
+// ---------------------------------------------------------------------------
+// Ensemble kernels: each block finds its member and its block index within
+// that member's own launch shape, then runs the same body as the solo kernel.
+__device__ __forceinline__ bool ens_slot(const EnsMap& m, int& i, int& bid, int& nb) {
+  i = m.inst[blockIdx.x];
+  if (!m.active[i]) return false;
+  bid = (int)blockIdx.x - m.first[i];
+  nb = m.nb[i];
+  return true;
+}
+
+template <class T>
+__device__ __forceinline__ const T& ens_arg(const T* t, const EnsSel& s, int i) {
+  return t[(size_t)(s.sel ? s.sel[i] : i) * s.stride + s.off];
+}
+
+__global__ void __launch_bounds__(kNT) kb_stage(const StageArgs* t, const FsPhys* K, EnsMap m,
+                                                EnsSel s) {
+  int i, bid, nb;
+  if (ens_slot(m, i, bid, nb)) stage_body(ens_arg(t, s, i), K[i], bid, nb);
+}
+__global__ void __launch_bounds__(kNT) kb_circ(const CircArgs* t, const FsPhys* K, EnsMap m,
+                                               EnsSel s) {
+  int i, bid, nb;
+  if (ens_slot(m, i, bid, nb)) circ_body(ens_arg(t, s, i), K[i]);
+}
+__global__ void __launch_bounds__(kNT) kb_ctrl(const CtrlArgs* t, EnsMap m, EnsSel s) {
+  int i, bid, nb;
+  if (ens_slot(m, i, bid, nb)) ctrl_body(ens_arg(t, s, i));
+}
+__global__ void kb_scalars(DevScalars* sc, const uint8_t* mask, int n, int op) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || !mask[i]) return;
+  DevScalars* c = sc + i;
+  if (op == 0) {
+    c->t_m = (double)1e9f;  // main.F90:74
+    c->attempts = 0;
+  } else if (op == 1) {  // main.F90:83-101
+    c->ts = c->ts + 1.0;
+    c->t = c->t + c->dt;
+    c->Vd_or = c->Vd_mi;
+    c->Vd_mi = c->Vd_pl;
+  } else {  // main.F90:103-121
+    c->dt = c->dt / 2.0;
+    c->Vd_pl = c->Vd_or;
+    c->Vd_mi = c->Vd_or;
+  }
+}
+__global__ void __launch_bounds__(kNT) kb_lapl(const LaplArgs* t, const FsPhys* K, EnsMap m,
+                                               EnsSel s) {
+  int i, bid, nb;
+  if (ens_slot(m, i, bid, nb)) lapl_body(ens_arg(t, s, i), K[i], bid, nb);
+}
+__global__ void __launch_bounds__(256) kb_mg_coarsen(const MgArgs* t, EnsMap m, EnsSel s) {
+  int i, bid, nb;
+  if (ens_slot(m, i, bid, nb)) mg_coarsen_body(ens_arg(t, s, i), bid, nb);
+}
+__global__ void __launch_bounds__(1024) kb_mg_factor(const MgArgs* t, double* const* inv,
+                                                     EnsMap m, EnsSel s) {
+  int i, bid, nb;
+  if (ens_slot(m, i, bid, nb)) mg_factor_body(ens_arg(t, s, i), inv[i]);
+}
+__global__ void __launch_bounds__(kNT) kb_mg_down(const MgArgs* t, EnsMap m, EnsSel s) {
+  int i, bid, nb;
+  if (ens_slot(m, i, bid, nb)) mg_down_body(ens_arg(t, s, i), bid, nb);
+}
+__global__ void __launch_bounds__(kNT) kb_mg_up(const MgArgs* t, EnsMap m, EnsSel s) {
+  int i, bid, nb;
+  if (ens_slot(m, i, bid, nb)) mg_up_body(ens_arg(t, s, i), bid, nb);
+}
+__global__ void __launch_bounds__(1024) kb_mg_coarse_cycle(const MgCycleArgs* t, EnsMap m,
+                                                           EnsSel s) {
+  int i, bid, nb;
+  if (ens_slot(m, i, bid, nb)) mg_coarse_cycle_body(ens_arg(t, s, i));
+}
+__global__ void __launch_bounds__(kNT) kb_mgpcg_init(const MgPcgArgs* t, EnsMap m, EnsSel s) {
+  int i, bid, nb;
+  if (ens_slot(m, i, bid, nb)) mgpcg_init_body(ens_arg(t, s, i), bid, nb);
+}
+__global__ void __launch_bounds__(kNT) kb_mgpcg_pq(const MgPcgArgs* t, EnsMap m, EnsSel s) {
+  int i, bid, nb;
+  if (ens_slot(m, i, bid, nb)) mgpcg_pq_body(ens_arg(t, s, i), bid, nb);
+}
+__global__ void __launch_bounds__(kNT) kb_mgpcg_xr(const MgPcgArgs* t, EnsMap m, EnsSel s) {
+  int i, bid, nb;
+  if (ens_slot(m, i, bid, nb)) mgpcg_xr_body(ens_arg(t, s, i), bid, nb);
+}
+__global__ void __launch_bounds__(kNT) kb_pcg_finish(const PcgArgs* t, EnsMap m, EnsSel s) {
+  int i, bid, nb;
+  if (ens_slot(m, i, bid, nb)) pcg_finish_body(ens_arg(t, s, i), bid, nb);
+}
+__global__ void __launch_bounds__(kNT) kb_pcg_tres(const PcgArgs* t, EnsMap one, EnsSel s,
+                                                   const int32_t* nbfin) {
+  int i, bid, nb;
+  if (ens_slot(one, i, bid, nb)) pcg_tres_body(ens_arg(t, s, i), nbfin[i]);
 }
 
 }  // namespace
@@ -1722,4 +1848,57 @@ void launch_eval_fits(const FsPhys& K, const double* T, double* out, int n, hipS
 void launch_eval_flux(const FsPhys& K, const double* a, double* out, int n, int dfdph,
                       hipStream_t s) {
   hipLaunchKernelGGL(k_eval_flux, dim3((n + 255) / 256), dim3(256), 0, s, K, a, out, n, dfdph);
+}
+
+void launch_ens_stage(const StageArgs* t, const FsPhys* K, const EnsMap& m, const EnsSel& s,
+                      hipStream_t st) {
+  if (m.nblocks) hipLaunchKernelGGL(kb_stage, dim3(m.nblocks), dim3(kNT), 0, st, t, K, m, s);
+}
+void launch_ens_circ(const CircArgs* t, const FsPhys* K, const EnsMap& m, const EnsSel& s,
+                     hipStream_t st) {
+  if (m.nblocks) hipLaunchKernelGGL(kb_circ, dim3(m.nblocks), dim3(kNT), 0, st, t, K, m, s);
+}
+void launch_ens_ctrl(const CtrlArgs* t, const EnsMap& m, const EnsSel& s, hipStream_t st) {
+  if (m.nblocks) hipLaunchKernelGGL(kb_ctrl, dim3(m.nblocks), dim3(kNT), 0, st, t, m, s);
+}
+void launch_ens_scalars(DevScalars* sc, const uint8_t* mask, int n, int op, hipStream_t st) {
+  if (n) hipLaunchKernelGGL(kb_scalars, dim3((n + 255) / 256), dim3(256), 0, st, sc, mask, n, op);
+}
+void launch_ens_lapl(const LaplArgs* t, const FsPhys* K, const EnsMap& m, const EnsSel& s,
+                     hipStream_t st) {
+  if (m.nblocks) hipLaunchKernelGGL(kb_lapl, dim3(m.nblocks), dim3(kNT), 0, st, t, K, m, s);
+}
+void launch_ens_mg_coarsen(const MgArgs* t, const EnsMap& m, const EnsSel& s, hipStream_t st) {
+  if (m.nblocks) hipLaunchKernelGGL(kb_mg_coarsen, dim3(m.nblocks), dim3(256), 0, st, t, m, s);
+}
+void launch_ens_mg_factor(const MgArgs* t, double* const* inv, const EnsMap& m, const EnsSel& s,
+                          hipStream_t st) {
+  if (m.nblocks)
+    hipLaunchKernelGGL(kb_mg_factor, dim3(m.nblocks), dim3(1024), 0, st, t, inv, m, s);
+}
+void launch_ens_mg_down(const MgArgs* t, const EnsMap& m, const EnsSel& s, hipStream_t st) {
+  if (m.nblocks) hipLaunchKernelGGL(kb_mg_down, dim3(m.nblocks), dim3(kNT), 0, st, t, m, s);
+}
+void launch_ens_mg_up(const MgArgs* t, const EnsMap& m, const EnsSel& s, hipStream_t st) {
+  if (m.nblocks) hipLaunchKernelGGL(kb_mg_up, dim3(m.nblocks), dim3(kNT), 0, st, t, m, s);
+}
+void launch_ens_mg_coarse_cycle(const MgCycleArgs* t, const EnsMap& m, const EnsSel& s,
+                                hipStream_t st) {
+  if (m.nblocks)
+    hipLaunchKernelGGL(kb_mg_coarse_cycle, dim3(m.nblocks), dim3(1024), 0, st, t, m, s);
+}
+void launch_ens_mgpcg_init(const MgPcgArgs* t, const EnsMap& m, const EnsSel& s, hipStream_t st) {
+  if (m.nblocks) hipLaunchKernelGGL(kb_mgpcg_init, dim3(m.nblocks), dim3(kNT), 0, st, t, m, s);
+}
+void launch_ens_mgpcg_pq(const MgPcgArgs* t, const EnsMap& m, const EnsSel& s, hipStream_t st) {
+  if (m.nblocks) hipLaunchKernelGGL(kb_mgpcg_pq, dim3(m.nblocks), dim3(kNT), 0, st, t, m, s);
+}
+void launch_ens_mgpcg_xr(const MgPcgArgs* t, const EnsMap& m, const EnsSel& s, hipStream_t st) {
+  if (m.nblocks) hipLaunchKernelGGL(kb_mgpcg_xr, dim3(m.nblocks), dim3(kNT), 0, st, t, m, s);
+}
+void launch_ens_pcg_finish(const PcgArgs* t, const EnsMap& m, const EnsMap& one, const EnsSel& s,
+                           hipStream_t st) {
+  if (m.nblocks) hipLaunchKernelGGL(kb_pcg_finish, dim3(m.nblocks), dim3(kNT), 0, st, t, m, s);
+  if (one.nblocks)
+    hipLaunchKernelGGL(kb_pcg_tres, dim3(one.nblocks), dim3(kNT), 0, st, t, one, s, m.nb);
 }

@@ -23,32 +23,45 @@ def _dp(a):
     return a.ctypes.data_as(C.POINTER(C.c_double))
 
 
+def descs(cfg: dict):
+    """fs_grid_desc and fs_params of a configuration dictionary."""
+    g = _L.GridDesc()
+    for name, _ in _L.GridDesc._fields_:
+        if name in cfg:
+            setattr(g, name, cfg[name])
+    p = _L.Params()
+    for name, _ in _L.Params._fields_:
+        if name in cfg:
+            setattr(p, name, cfg[name])
+    return g, p
+
+
 class Discharge:
     SCALARS = ["t", "dt", "err_prev", "t_m", "Vd_pl", "Vd_mi", "Vd_or", "Ida", "Idc",
                "Vsrc", "ts"]
 
-    def __init__(self, cfg: dict):
+    def __init__(self, cfg: dict, handle=None, owner=None):
+        """A discharge of configuration `cfg` (its own handle), or a view of an
+        existing handle (an ensemble member: `handle`, kept alive by `owner`)."""
         L = _L.load()
         self.cfg = dict(cfg)
-        g = _L.GridDesc()
-        for name, _ in _L.GridDesc._fields_:
-            if name in cfg:
-                setattr(g, name, cfg[name])
-        p = _L.Params()
-        for name, _ in _L.Params._fields_:
-            if name in cfg:
-                setattr(p, name, cfg[name])
-        self._g, self._p = g, p
-        h = C.c_void_p()
-        _L.check(L.fs_create(C.byref(g), C.byref(p), C.byref(h)))
+        self._g, self._p = descs(cfg)
+        self._owner = owner
+        if handle is None:
+            h = C.c_void_p()
+            _L.check(L.fs_create(C.byref(self._g), C.byref(self._p), C.byref(h)))
+            self._owns = True
+        else:
+            h = C.c_void_p(handle)
+            self._owns = False
         self.h = h
         self.nx, self.ny = int(cfg["nx"]), int(cfg["ny"])
         self.P = self.nx + 2
 
     def close(self):
-        if getattr(self, "h", None):
+        if getattr(self, "h", None) and getattr(self, "_owns", False):
             _L.load().fs_destroy(self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):
         try:

@@ -26,6 +26,7 @@ EXPORTS = [
     "fs_log_count", "fs_get_log", "fs_clear_log", "fs_rhs_sweep", "fs_circ_step",
     "fs_lapl_assemble", "fs_constants", "fs_eval_fits", "fs_eval_flux", "fs_eval_dfdph",
     "fs_get_timers", "fs_set_timing", "fs_set_solver", "fs_time_kernel", "fs_copy_bench",
+    "fs_ensemble_create", "fs_ensemble_destroy", "fs_ensemble_member", "fs_ensemble_run",
     "fs_learn_create", "fs_learn_destroy", "fs_learn_nparams", "fs_learn_get_weights",
     "fs_learn_set_weights", "fs_learn_samples", "fs_learn_harvest", "fs_learn_train",
     "fs_learn_loss_grad", "fs_learn_attach", "fs_last_error",
@@ -112,6 +113,10 @@ def load():
         "fs_set_solver": [P, C.c_int32],
         "fs_time_kernel": [P, C.c_int32, C.c_int32, D],
         "fs_copy_bench": [C.c_int64, C.c_int32, D],
+        "fs_ensemble_create": [C.c_int32, C.POINTER(GridDesc), C.POINTER(Params), C.POINTER(P)],
+        "fs_ensemble_destroy": [P],
+        "fs_ensemble_member": [P, C.c_int32],
+        "fs_ensemble_run": [P, C.c_int32],
         "fs_learn_create": [C.c_int32, C.c_int32, C.c_int32, C.c_uint64, C.POINTER(P)],
         "fs_learn_destroy": [P],
         "fs_learn_nparams": [P],
@@ -130,6 +135,7 @@ def load():
         f.restype = C.c_int
     L.fs_last_error.restype = C.c_char_p
     L.fs_learn_nparams.restype = C.c_int64
+    L.fs_ensemble_member.restype = C.c_void_p
     _lib = L
     return L
 

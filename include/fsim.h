@@ -207,6 +207,24 @@ int fs_time_kernel(fs_handle* h, int32_t which, int32_t n, double* avg_ms);
  * the FETCH_SIZE / WRITE_SIZE counters. */
 int fs_copy_bench(int64_t bytes, int32_t reps, double* gbps);
 
+/* ---- ensembles --------------------------------------------------------------
+ * No counterpart in the reference (one discharge per run).  n independent
+ * discharges (grid_desc g[i], params p[i]; multigrid solver) advanced by
+ * launches that each cover every member: a round gives each member one
+ * ptcl_step attempt or, after an accepted attempt, its Poisson solve and the
+ * accept/reject of the pass.  Each member has its own dt, controller,
+ * replay table and solver convergence, and its fields equal those of the same
+ * discharge run alone with fs_run, bitwise.  Members are ordinary handles
+ * (initial state, fields, scalars and logs through the fs_* calls above),
+ * owned by the ensemble. */
+typedef struct fs_ensemble fs_ensemble;
+
+int fs_ensemble_create(int32_t n, const fs_grid_desc* g, const fs_params* p, fs_ensemble** out);
+int fs_ensemble_destroy(fs_ensemble* e);
+fs_handle* fs_ensemble_member(fs_ensemble* e, int32_t i);
+/* Every member advances `passes` main-loop passes (fs_run semantics). */
+int fs_ensemble_run(fs_ensemble* e, int32_t passes);
+
 /* ---- learned Poisson preconditioner ------------------------------------
  * No counterpart in the reference (its Poisson solve is PETSc's KSP with the
  * default preconditioner, lapl_lib.F90:50-54, 104).  A 4-level convolutional

@@ -1,0 +1,61 @@
+"""GPU: ensembles.  Members advanced together by the ensemble's batched
+launches end bitwise where the same discharges end when run alone (fields,
+potential, scalars and the attempt log), across different grid sizes,
+pressures, voltages and secondary-emission coefficients."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _solo(c, passes):
+    from fluidsim_amd import Discharge
+
+    d = Discharge(c)
+    d.init_gaussian(seed=c["seed"])
+    d.run(passes)
+    return d
+
+
+@pytest.mark.parametrize("passes", [1, 14])
+def test_members_equal_their_solo_runs_bitwise(passes):
+    from fluidsim_amd import configs
+    from fluidsim_amd.ensemble import Ensemble
+
+    cfgs = configs.config3(5, base_seed=3, lo=64, hi=96)
+    E = Ensemble(cfgs)
+    for m, c in zip(E.members, cfgs):
+        m.init_gaussian(seed=c["seed"])
+    E.run(passes)
+    for i, (m, c) in enumerate(zip(E.members, cfgs)):
+        d = _solo(c, passes)
+        for f in ("ni", "ne", "nt", "nm"):
+            for lev in (1, 2, 3):
+                assert np.array_equal(m.get(f, lev), d.get(f, lev)), (i, f, lev)
+        assert np.array_equal(m.get("ph"), d.get("ph")), i
+        assert np.array_equal(m.log(), d.log()), i
+        assert m.scalars() == d.scalars(), i
+        print(i, (c["nx"], c["ny"]), "attempts", len(d.log()), "t", d.scalars()["t"])
+    E.close()
+
+
+def test_ensemble_runs_continue_like_one_run():
+    """Two calls of run(k) equal one call of run(2k), member by member."""
+    from fluidsim_amd import configs
+    from fluidsim_amd.ensemble import Ensemble
+
+    cfgs = configs.config3(3, base_seed=9, lo=64, hi=80)
+    out = []
+    for split in (False, True):
+        E = Ensemble(cfgs)
+        for m, c in zip(E.members, cfgs):
+            m.init_gaussian(seed=c["seed"])
+        if split:
+            E.run(5)
+            E.run(5)
+        else:
+            E.run(10)
+        out.append([m.get("ne", 1) for m in E.members] + [m.get("ph") for m in E.members])
+        E.close()
+    for a, b in zip(*out):
+        assert np.array_equal(a, b)
