@@ -138,6 +138,44 @@ def solve_at(n, net, reps):
     return poisson_compare(d, reps, net)
 
 
+def ensemble_run(args):
+    """The parameter-sweep ensemble: every member advances `steps` passes in the
+    ensemble's batched launches; aggregate accepted steps per second, next to
+    the same members run one after the other on their own (a sample)."""
+    from fluidsim_amd import Discharge, configs
+    from fluidsim_amd.ensemble import Ensemble
+
+    cfgs = configs.config3(args.members)
+    t0 = time.perf_counter()
+    E = Ensemble(cfgs)
+    for m, c in zip(E.members, cfgs):
+        m.init_gaussian(seed=c["seed"])
+    t_setup = time.perf_counter() - t0
+    E.run(args.warmup)
+    ts0 = [m.scalars()["ts"] for m in E.members]
+    t0 = time.perf_counter()
+    E.run(args.steps)
+    el = time.perf_counter() - t0
+    acc = sum(m.scalars()["ts"] - a for m, a in zip(E.members, ts0))
+    attempts = sum(len(m.log()) for m in E.members)
+    cells = sum(c["nx"] * c["ny"] for c in cfgs)
+    # solo reference: a sample of members, each run alone for the same passes
+    k = min(args.solo_sample, len(cfgs))
+    solo_t, solo_acc = 0.0, 0.0
+    for c in cfgs[:k]:
+        d = Discharge(c)
+        d.init_gaussian(seed=c["seed"])
+        d.run(args.warmup)
+        a = d.scalars()["ts"]
+        t1 = time.perf_counter()
+        d.run(args.steps)
+        solo_t += time.perf_counter() - t1
+        solo_acc += d.scalars()["ts"] - a
+    E.close()
+    return dict(elapsed=el, accepted=acc, attempts=attempts, cells=cells, setup_s=t_setup,
+                solo_rate=solo_acc / solo_t if solo_t > 0 else None, solo_sample=k)
+
+
 def cpu_baseline(args):
     """The CPU oracle (C restatement of the reference, OpenMP over rows) on a
     bounded sample: the first three passes untimed, then `cpu_steps` steps."""
@@ -189,8 +227,11 @@ def main():
     ap.add_argument("--learn-steps", type=int, default=3000)
     ap.add_argument("--learned-n2", type=int, default=1024,
                     help="second grid side for the learned/plain comparison (0: none)")
-    ap.add_argument("--config", type=int, default=1, choices=(1, 2),
-                    help="1: 512^2 pulse (the metric's workload); 2: 2048^2 DC, 5 cm, 1 Torr")
+    ap.add_argument("--config", type=int, default=1, choices=(1, 2, 3),
+                    help="1: 512^2 pulse (the metric's workload); 2: 2048^2 DC, 5 cm, 1 Torr; "
+                         "3: 256-member parameter-sweep ensemble")
+    ap.add_argument("--members", type=int, default=256)
+    ap.add_argument("--solo-sample", type=int, default=16)
     args = ap.parse_args()
     if args.n == 0:
         args.n = 2048 if args.config == 2 else 512
@@ -203,6 +244,24 @@ def main():
                           "n_gpus": args.gpus}))
         sys.exit(2)
 
+    if args.config == 3:
+        e = ensemble_run(args)
+        rate = e["accepted"] / e["elapsed"]
+        print(json.dumps({
+            "metric": "aggregate RK-Merson steps/s, 256-discharge parameter-sweep ensemble, FP64",
+            "value": rate, "unit": "steps/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": 1e3 * e["elapsed"] / max(args.steps, 1),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic (Gaussian + seeded 1% noise, member seed = id)",
+            "config": {"workload": f"{args.members} argon discharges, sides U{{64..256}}, "
+                                   "150-400 V DC, 0.5-10 Torr log-uniform, gamma 0.05-0.15",
+                       "members": args.members, "cells_total": e["cells"]},
+            "accepted_steps": e["accepted"], "ptcl_attempts": e["attempts"],
+            "member_passes_per_s": args.members * args.steps / e["elapsed"],
+            "solo_sequential_steps_per_s": e["solo_rate"], "solo_sample": e["solo_sample"],
+            "batched_over_sequential": rate / e["solo_rate"] if e["solo_rate"] else None,
+            "setup_s": e["setup_s"], "cpu_baseline": None}))
+        return
     r = gpu_run(args)
     steps_per_s = r["accepted"] / r["elapsed"]
     N = r["N"]
