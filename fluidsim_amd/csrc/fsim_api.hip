@@ -151,7 +151,14 @@ struct fs_handle {
   double* d_mg = nullptr;
   double* d_mginv = nullptr;        // dense inverse of the coarsest operator (<= 64 cells)
   size_t mg_first = 0;              // first level of the one-workgroup coarse kernel
+  int last_iters[3] = {0, 0, 0};    // iterations of the last solve, per solver
   fs_learn* net = nullptr;          // learned coarse solver (lapl_solver = 2)
+  // device-resident kernel arguments: stage [(rot*2 + alias)*3 + variant][stage],
+  // variant 0 step, 1 slopes only (debug copy), 2 slopes only (no copy);
+  // assembly [rot*2 + alias]; the physics constants
+  StageArgs* d_stab = nullptr;
+  LaplArgs* d_ltab = nullptr;
+  FsPhys* d_phys = nullptr;
   hipGraphExec_t mg_graph = nullptr;  // two MG-PCG iterations (parity 1, then 0)
   hipGraphExec_t learn_graph = nullptr;  // two learned-FCG iterations
   const void* learn_graph_key = nullptr; // the network (or the handle: none) it was made for
@@ -349,8 +356,8 @@ int build_mg(fs_handle* h) {
   std::vector<int> nxs, nys;
   int nx = h->bx, ny = h->by;
   size_t total = 0;
-  HIP_OK(hipMalloc(&h->d_mginv, 64 * 64 * sizeof(double)));
-  while ((size_t)nx * ny > 64) {
+  HIP_OK(hipMalloc(&h->d_mginv, kMgBottomMax * kMgBottomMax * sizeof(double)));
+  while ((size_t)nx * ny > (size_t)kMgBottomMax) {
     nx = (nx + 1) / 2;
     ny = (ny + 1) / 2;
     nxs.push_back(nx);
@@ -605,7 +612,43 @@ LaplArgs lapl_args(fs_handle* h) {
   return a;
 }
 
-void launch_assembly(fs_handle* h) { launch_lapl_assemble(lapl_args(h), h->K, h->st); }
+// The argument tables of the handle's stage and assembly kernels (rebuilt when
+// the scalars move, see the ensemble).
+int build_arg_tables(fs_handle* h) {
+  std::vector<StageArgs> st(6 * 15);
+  std::vector<LaplArgs> la(6);
+  const int rot0 = h->rot;
+  const bool alias0 = h->alias;
+  for (int rot = 0; rot < 3; rot++)
+    for (int al = 0; al < 2; al++) {
+      h->rot = rot;
+      h->alias = al != 0;
+      for (int v = 0; v < 3; v++)
+        for (int stg = 1; stg <= 5; stg++) {
+          StageArgs a = stage_args(h, stg, v == 0 ? 0 : 1);
+          if (v == 2)
+            for (int s = 0; s < 4; s++) a.kdbg[s] = nullptr;
+          st[((rot * 2 + al) * 3 + v) * 5 + stg - 1] = a;
+        }
+      la[rot * 2 + al] = lapl_args(h);
+    }
+  h->rot = rot0;
+  h->alias = alias0;
+  if (!h->d_stab) HIP_OK(hipMalloc(&h->d_stab, st.size() * sizeof(StageArgs)));
+  if (!h->d_ltab) HIP_OK(hipMalloc(&h->d_ltab, la.size() * sizeof(LaplArgs)));
+  if (!h->d_phys) HIP_OK(hipMalloc(&h->d_phys, sizeof(FsPhys)));
+  HIP_OK(hipMemcpy(h->d_stab, st.data(), st.size() * sizeof(StageArgs), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(h->d_ltab, la.data(), la.size() * sizeof(LaplArgs), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(h->d_phys, &h->K, sizeof(FsPhys), hipMemcpyHostToDevice));
+  return 0;
+}
+
+const StageArgs* stab(const fs_handle* h, int stage, int variant) {
+  return h->d_stab + ((h->rot * 2 + (h->alias ? 1 : 0)) * 3 + variant) * 5 + stage - 1;
+}
+const LaplArgs* ltab(const fs_handle* h) { return h->d_ltab + h->rot * 2 + (h->alias ? 1 : 0); }
+
+void launch_assembly(fs_handle* h) { launch_lapl_assemble(ltab(h), h->d_phys, h->nbx * h->nby, h->st); }
 
 // A pass's record of a ptcl_step attempt (the Poisson fields follow later).
 void log_attempt(fs_handle* h) {
@@ -624,6 +667,15 @@ void log_solve(fs_handle* h, int lerr) {
     rec[7] = h->h_sc->iters;
     rec[8] = h->h_sc->tres;
   }
+}
+
+// Iterations launched before the first convergence check: as many as the
+// previous solve took (an even count, for the two-iteration graphs), so that a
+// solve usually needs one host round trip; later checks every `dflt`.
+int first_chunk(int last, int dflt) {
+  if (last <= 0) return dflt;
+  const int n = last + 1;
+  return n + (n & 1);
 }
 
 int lapl_solve_impl(fs_handle* h, int* lerr, fs_solve_stats* stats) {
@@ -650,7 +702,7 @@ int lapl_solve_impl(fs_handle* h, int* lerr, fs_solve_stats* stats) {
     const void* key = h->net ? (const void*)h->net : (const void*)h;
     if (h->p.use_graphs && (!h->learn_graph || h->learn_graph_key != key) && make_learn_graph(h))
       return FS_E_HIP;
-    const int lchunk = chunk > 8 ? chunk / 4 : chunk;
+    int lchunk = first_chunk(h->last_iters[2], chunk > 8 ? chunk / 4 : chunk);
     for (;;) {
       if (h->p.use_graphs) {
         for (int k = 0; k < lchunk; k += 2, it += 2) HIP_OK(hipGraphLaunch(h->learn_graph, h->st));
@@ -658,6 +710,7 @@ int lapl_solve_impl(fs_handle* h, int* lerr, fs_solve_stats* stats) {
         for (int k = 0; k < lchunk; k++, it++)
           if (launch_learned_iteration(h, it)) return FS_E_HIP;
       }
+      lchunk = chunk > 8 ? chunk / 4 : chunk;
       HIP_OK(hipGetLastError());
       if (sync_scalars(h)) return FS_E_HIP;
       if (h->h_sc->status < 0) return check_status(h);
@@ -674,7 +727,7 @@ int lapl_solve_impl(fs_handle* h, int* lerr, fs_solve_stats* stats) {
     if (launch_vcycle(h, 0, true, nb_pcg) || launch_mg_iteration(h, 0)) return FS_E_HIP;
     it = 1;
     if (h->p.use_graphs && !h->mg_graph && make_mg_graph(h)) return FS_E_HIP;
-    const int mg_chunk = chunk > 8 ? chunk / 4 : chunk;
+    int mg_chunk = first_chunk(h->last_iters[1], chunk > 8 ? chunk / 4 : chunk);
     for (;;) {
       if (h->p.use_graphs) {
         for (int k = 0; k < mg_chunk; k += 2, it += 2) HIP_OK(hipGraphLaunch(h->mg_graph, h->st));
@@ -682,6 +735,7 @@ int lapl_solve_impl(fs_handle* h, int* lerr, fs_solve_stats* stats) {
         for (int k = 0; k < mg_chunk; k++, it++)
           if (launch_mg_iteration(h, it)) return FS_E_HIP;
       }
+      mg_chunk = chunk > 8 ? chunk / 4 : chunk;
       HIP_OK(hipGetLastError());
       if (sync_scalars(h)) return FS_E_HIP;
       if (h->h_sc->status < 0) return check_status(h);
@@ -693,7 +747,8 @@ int lapl_solve_impl(fs_handle* h, int* lerr, fs_solve_stats* stats) {
     ia.w_new = h->pois(PO_W0);  // w_0 is iteration 0's w_old
     launch_pcg_init(ia, h->st);
     for (;;) {
-      for (int k = 0; k < chunk; k++, it++) launch_pcg_it(pcg_args(h, it), h->st);
+      const int n = it == 0 ? first_chunk(h->last_iters[0], chunk) : chunk;
+      for (int k = 0; k < n; k++, it++) launch_pcg_it(pcg_args(h, it), h->st);
       HIP_OK(hipGetLastError());
       if (sync_scalars(h)) return FS_E_HIP;
       if (h->h_sc->status < 0) return check_status(h);
@@ -709,6 +764,8 @@ int lapl_solve_impl(fs_handle* h, int* lerr, fs_solve_stats* stats) {
   if (int e = check_status(h)) return e;
   int ok = h->h_sc->done == 1;
   *lerr = ok ? 0 : 1;
+  const int sv = h->p.lapl_solver == 2 ? 2 : (h->p.lapl_solver == 1 && h->mg.size() > 1) ? 1 : 0;
+  h->last_iters[sv] = h->h_sc->iters;
   if (h->timing) {
     float ms = 0;
     (void)hipEventElapsedTime(&ms, h->ev[4], h->ev[5]);
@@ -745,8 +802,7 @@ int ptcl_step_impl(fs_handle* h, fs_step_info* info) {
   for (;;) {
     time_mark(h, 0);
     for (int stage = 1; stage <= 5; stage++) {
-      StageArgs a = stage_args(h, stage, 0);
-      launch_stage(a, h->K, h->st);
+      launch_stage(stab(h, stage, 0), h->d_phys, h->nbx * h->nby, h->st);
       CircArgs c = circ_args(h, stage);
       launch_circ(c, h->K, h->st);
     }
@@ -1339,7 +1395,7 @@ int fs_create(const fs_grid_desc* g, const fs_params* p, fs_handle** out) {
     return FS_E_HIP;
   }
   h->stage_buf.resize(PL);
-  if (fs_init_uniform(h) != 0) {
+  if (build_arg_tables(h) || fs_init_uniform(h) != 0) {
     fs_destroy(h);
     return FS_E_HIP;
   }
@@ -1368,6 +1424,9 @@ int fs_destroy(fs_handle* h) {
   (void)hipFree(h->d_mginv);
   if (h->mg_graph) (void)hipGraphExecDestroy(h->mg_graph);
   if (h->learn_graph) (void)hipGraphExecDestroy(h->learn_graph);
+  (void)hipFree(h->d_stab);
+  (void)hipFree(h->d_ltab);
+  (void)hipFree(h->d_phys);
   for (int e = 0; e < 8; e++)
     if (h->ev[e]) (void)hipEventDestroy(h->ev[e]);
   if (h->st) (void)hipStreamDestroy(h->st);
@@ -1603,8 +1662,7 @@ int fs_clear_log(fs_handle* h) {
 int fs_rhs_sweep(fs_handle* h, int32_t stage) {
   if (!h || stage < 1 || stage > 5) return fail(FS_E_ARG, "bad argument%s");
   if (!h->d_kdbg) return fail(FS_E_ARG, "fs_rhs_sweep needs debug_keep_k%s");
-  StageArgs a = stage_args(h, stage, 1);
-  launch_stage(a, h->K, h->st);
+  launch_stage(stab(h, stage, 1), h->d_phys, h->nbx * h->nby, h->st);
   HIP_OK(hipGetLastError());
   if (sync_scalars(h)) return FS_E_HIP;
   return check_status(h);
@@ -1613,9 +1671,7 @@ int fs_rhs_sweep(fs_handle* h, int32_t stage) {
 int fs_circ_step(fs_handle* h, int32_t stage) {
   if (!h || stage < 1 || stage > 5) return fail(FS_E_ARG, "bad argument%s");
   // currents need the stage-kernel face terms of the current level 2
-  StageArgs a = stage_args(h, stage, 1);
-  for (int s = 0; s < 4; s++) a.kdbg[s] = nullptr;
-  launch_stage(a, h->K, h->st);
+  launch_stage(stab(h, stage, 2), h->d_phys, h->nbx * h->nby, h->st);
   CircArgs c = circ_args(h, stage);
   launch_circ(c, h->K, h->st);
   HIP_OK(hipGetLastError());
@@ -1626,24 +1682,7 @@ int fs_circ_step(fs_handle* h, int32_t stage) {
 int fs_lapl_assemble(fs_handle* h, double* b, double* A5) {
   if (!h || !b || !A5) return fail(FS_E_ARG, "null argument%s");
   if (!h->d_dbg) return fail(FS_E_ARG, "fs_lapl_assemble needs debug_keep_k%s");
-  LaplArgs a;
-  memset(&a, 0, sizeof(a));
-  int l1 = h->alias ? h->bufA() : h->bufL1();
-  for (int s = 0; s < 3; s++) a.n[s] = h->lev(l1, s);
-  a.ph = h->d_ph;
-  a.rhs0 = h->pois(PO_RHS0);
-  a.r = h->pois(PO_R0);
-  a.cx = h->pois(PO_CX);
-  a.cy = h->pois(PO_CY);
-  a.dg = h->pois(PO_DG);
-  a.bdbg = h->d_dbg;
-  a.adbg = h->d_dbg + (size_t)h->bx * h->by;
-  a.part = h->part(2);
-  a.sc = h->d_sc;
-  a.G = h->G;
-  a.nbx = h->nbx;
-  a.nby = h->nby;
-  launch_lapl_assemble(a, h->K, h->st);
+  launch_assembly(h);
   HIP_OK(hipGetLastError());
   size_t N = (size_t)h->bx * h->by;
   HIP_OK(hipMemcpyAsync(b, h->d_dbg, N * sizeof(double), hipMemcpyDeviceToHost, h->st));
@@ -1701,24 +1740,9 @@ int fs_time_kernel(fs_handle* h, int32_t which, int32_t n, double* avg_ms) {
   if (push_scalars(h)) return FS_E_HIP;
   auto launch = [&](int k) {
     if (which >= 1 && which <= 5) {
-      StageArgs a = stage_args(h, which, 0);
-      launch_stage(a, h->K, h->st);
+      launch_stage(stab(h, which, 0), h->d_phys, h->nbx * h->nby, h->st);
     } else if (which == 10) {
-      LaplArgs a;
-      memset(&a, 0, sizeof(a));
-      for (int s = 0; s < 3; s++) a.n[s] = h->lev(h->bufL1(), s);
-      a.ph = h->d_ph;
-      a.rhs0 = h->pois(PO_RHS0);
-      a.r = h->pois(PO_R0);
-      a.cx = h->pois(PO_CX);
-      a.cy = h->pois(PO_CY);
-      a.dg = h->pois(PO_DG);
-      a.part = h->part(2);
-      a.sc = h->d_sc;
-      a.G = h->G;
-      a.nbx = h->nbx;
-      a.nby = h->nby;
-      launch_lapl_assemble(a, h->K, h->st);
+      launch_assembly(h);
     } else if (which == 11) {
       launch_pcg_it(pcg_args(h, 2 + (k & 1)), h->st);
     } else if (which == 15) {
@@ -1844,6 +1868,12 @@ int fs_ensemble_create(int32_t n, const fs_grid_desc* g, const fs_params* p, fs_
     h->h_sc = E->h_sc + i;
     if (h->mg_graph) (void)hipGraphExecDestroy(h->mg_graph);
     h->mg_graph = nullptr;
+    if (h->learn_graph) (void)hipGraphExecDestroy(h->learn_graph);
+    h->learn_graph = nullptr;
+    if (build_arg_tables(h)) {
+      ens_release(E);
+      return FS_E_HIP;
+    }
   }
   E->round_bytes = (size_t)n * 9;
   if (bad(hipMemcpy(E->d_sc, E->h_sc, n * sizeof(DevScalars), hipMemcpyHostToDevice)) ||

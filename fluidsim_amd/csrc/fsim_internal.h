@@ -147,7 +147,7 @@ inline int capped_grid(int ntiles) { return ntiles < kMaxPartials ? ntiles : kMa
 
 // most levels a hierarchy may have
 constexpr int kMgMaxLevels = 16;
-constexpr int kMgBottomMax = 64;   // cells of the coarsest level (dense inverse)
+constexpr int kMgBottomMax = 16;   // cells of the coarsest level (dense inverse)
 constexpr int kMgLdsLevels = 4;    // levels run by the one-workgroup kernel
 constexpr int kMgLdsCells = 1024;  // cells of the finest of those
 constexpr int kMgLdsX = 4096;      // LDS doubles for their padded iterates
@@ -227,13 +227,14 @@ void launch_ens_mgpcg_xr(const MgPcgArgs* t, const EnsMap& m, const EnsSel& s, h
 void launch_ens_pcg_finish(const PcgArgs* t, const EnsMap& m, const EnsMap& one, const EnsSel& s,
                            hipStream_t st);
 
-void launch_stage(const StageArgs& a, const FsPhys& K, hipStream_t s);
+// a, K: device pointers (argument tables kept per handle)
+void launch_stage(const StageArgs* a, const FsPhys* K, int nblocks, hipStream_t s);
 void launch_circ(const CircArgs& a, const FsPhys& K, hipStream_t s);
 void launch_ctrl(const CtrlArgs& a, hipStream_t s);
 void launch_accept(DevScalars* sc, hipStream_t s);
 void launch_reject(DevScalars* sc, hipStream_t s);
 void launch_begin_step(DevScalars* sc, hipStream_t s);
-void launch_lapl_assemble(const LaplArgs& a, const FsPhys& K, hipStream_t s);
+void launch_lapl_assemble(const LaplArgs* a, const FsPhys* K, int nblocks, hipStream_t s);
 void launch_pcg_init(const PcgArgs& a, hipStream_t s);
 void launch_pcg_it(const PcgArgs& a, hipStream_t s);
 void launch_pcg_finish(const PcgArgs& a, hipStream_t s);

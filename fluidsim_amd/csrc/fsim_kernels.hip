@@ -171,6 +171,8 @@ __device__ __forceinline__ Flux4 interior_face(const FsPhys& K, double E, double
 }
 
 // ---------------------------------------------------------------------------
+// one instance per kernel: the LDS arrays declared here then belong to one kernel each
+template <int kInstance>
 __device__ __forceinline__ void stage_body(const StageArgs& A, const FsPhys& K, int bid, int nb) {
   __shared__ double s_ni[kNH], s_ne[kNH], s_nt[kNH], s_nm[kNH], s_ph[kNH];
   __shared__ double s_te[kNH], s_mue[kNH], s_de[kNH], s_xa[kNH];
@@ -407,7 +409,13 @@ __device__ __forceinline__ void stage_body(const StageArgs& A, const FsPhys& K, 
     atomicMin(&A.sc->status, s_bad == 1 ? FS_E_NEGDENS : FS_E_NAN);
 }
 
-__global__ void __launch_bounds__(kNT) k_stage(StageArgs A, FsPhys K) { stage_body(A, K, blockIdx.x, gridDim.x); }
+// Arguments from device memory: kept there per handle, they are read with
+// scalar loads where used (passed by value, the compiler keeps the whole
+// argument block live in SGPRs and spills it to VGPR lanes).
+__global__ void __launch_bounds__(kNT) k_stage(const StageArgs* __restrict__ A,
+                                               const FsPhys* __restrict__ K) {
+  stage_body<0>(*A, *K, blockIdx.x, gridDim.x);
+}
 
 // ---------------------------------------------------------------------------
 __device__ double vsource(const CircArgs& C, const FsPhys& K, double t) {
@@ -445,6 +453,7 @@ __device__ double vsource(const CircArgs& C, const FsPhys& K, double t) {
   }
 }
 
+template <int kInstance>
 __device__ __forceinline__ void circ_body(const CircArgs& C, const FsPhys& K) {
   __shared__ double s_red[8 * kNW];
   __shared__ double s_vd;
@@ -485,9 +494,10 @@ __device__ __forceinline__ void circ_body(const CircArgs& C, const FsPhys& K) {
   }
 }
 
-__global__ void __launch_bounds__(kNT) k_circ(CircArgs C, FsPhys K) { circ_body(C, K); }
+__global__ void __launch_bounds__(kNT) k_circ(CircArgs C, FsPhys K) { circ_body<0>(C, K); }
 
 // dt controller: ptcl_lib.F90:59-94 (plus replayed decisions)
+template <int kInstance>
 __device__ __forceinline__ void ctrl_body(const CtrlArgs& A) {
   __shared__ double s_red[8 * kNW];
   double v[4] = {0, 0, 0, 0};
@@ -535,7 +545,7 @@ __device__ __forceinline__ void ctrl_body(const CtrlArgs& A) {
   }
 }
 
-__global__ void __launch_bounds__(kNT) k_ctrl(CtrlArgs A) { ctrl_body(A); }
+__global__ void __launch_bounds__(kNT) k_ctrl(CtrlArgs A) { ctrl_body<0>(A); }
 
 __global__ void k_begin_step(DevScalars* sc) {
   sc->t_m = (double)1e9f;  // main.F90:74
@@ -576,6 +586,7 @@ __device__ __forceinline__ FaceJ lapl_face(const FsPhys& K, double E, double h, 
   return F;
 }
 
+template <int kInstance>
 __device__ __forceinline__ void lapl_body(const LaplArgs& A, const FsPhys& K, int bid, int nb) {
   __shared__ double s_ni[kNH], s_ne[kNH], s_ph[kNH], s_te[kNH], s_mue[kNH], s_de[kNH];
   __shared__ double s_f[4][kNF];
@@ -831,7 +842,10 @@ __device__ __forceinline__ void lapl_body(const LaplArgs& A, const FsPhys& K, in
   }
 }
 
-__global__ void __launch_bounds__(kNT) k_lapl(LaplArgs A, FsPhys K) { lapl_body(A, K, blockIdx.x, gridDim.x); }
+__global__ void __launch_bounds__(kNT) k_lapl(const LaplArgs* __restrict__ A,
+                                              const FsPhys* __restrict__ K) {
+  lapl_body<0>(*A, *K, blockIdx.x, gridDim.x);
+}
 
 // ---------------------------------------------------------------------------
 // PCG on M = -V*A (SPD), Chronopoulos-Gear form: one kernel per iteration and
@@ -989,6 +1003,7 @@ __global__ void __launch_bounds__(kNT) k_pcg_it(PcgArgs A) {
 
 // After the solve: ph += delta (lapl_lib.F90:112-116) when converged, and the
 // partials of the true residual |(rhs0 - M x)/V|^2.
+template <int kInstance>
 __device__ __forceinline__ void pcg_finish_body(const PcgArgs& A, int bid, int nb) {
   __shared__ double s_red[8 * kNW];
   DevScalars* sc = A.sc;
@@ -1016,9 +1031,10 @@ __device__ __forceinline__ void pcg_finish_body(const PcgArgs& A, int bid, int n
   write_partials3(acc, A.part_out, s_red, bid);
 }
 
-__global__ void __launch_bounds__(kNT) k_pcg_finish(PcgArgs A) { pcg_finish_body(A, blockIdx.x, gridDim.x); }
+__global__ void __launch_bounds__(kNT) k_pcg_finish(PcgArgs A) { pcg_finish_body<0>(A, blockIdx.x, gridDim.x); }
 
 // Single block: true residual from the finish partials.
+template <int kInstance>
 __device__ __forceinline__ void pcg_tres_body(const PcgArgs& A, int nb) {
   __shared__ double s_red[8 * kNW];
   double tot[3];
@@ -1029,7 +1045,7 @@ __device__ __forceinline__ void pcg_tres_body(const PcgArgs& A, int nb) {
   }
 }
 
-__global__ void __launch_bounds__(kNT) k_pcg_tres(PcgArgs A, int nb) { pcg_tres_body(A, nb); }
+__global__ void __launch_bounds__(kNT) k_pcg_tres(PcgArgs A, int nb) { pcg_tres_body<0>(A, nb); }
 
 // ---------------------------------------------------------------------------
 // Aggregation multigrid V(2,2)-cycle used as the PCG preconditioner.
@@ -1054,6 +1070,7 @@ __device__ __forceinline__ size_t mg_at(const MgLevel& L, int i, int j) {
   return (size_t)(j + 1) * L.P + (i + 1);
 }
 
+template <int kInstance>
 __device__ __forceinline__ void mg_coarsen_body(const MgArgs& A, int bid, int nb) {
   const MgLevel &F = A.F, &C = A.C;
   int n = bid * blockDim.x + threadIdx.x;
@@ -1092,7 +1109,7 @@ __device__ __forceinline__ void mg_coarsen_body(const MgArgs& A, int bid, int nb
   C.cy[m] = cy;
 }
 
-__global__ void __launch_bounds__(256) k_mg_coarsen(MgArgs A) { mg_coarsen_body(A, blockIdx.x, gridDim.x); }
+__global__ void __launch_bounds__(256) k_mg_coarsen(MgArgs A) { mg_coarsen_body<0>(A, blockIdx.x, gridDim.x); }
 
 constexpr int kGsTX = 32, kGsTY = 16, kGsH = 4;  // tile and halo (four half-sweeps)
 constexpr int kGsW = kGsTX + 2 * kGsH, kGsR = kGsTY + 2 * kGsH;
@@ -1157,6 +1174,7 @@ __device__ __forceinline__ int gs_ntiles(const MgLevel& F) {
 // the coarse level.  Level 0 first checks convergence from the x/r-update
 // partials (every block identically); their loads are issued together with
 // the first tile's.
+template <int kInstance>
 __device__ __forceinline__ void mg_down_body(const MgArgs& A, int bid, int nb) {
   __shared__ GsTile s;
   __shared__ double s_res[kGsTY][kGsTX / 2];
@@ -1250,10 +1268,11 @@ __device__ __forceinline__ void mg_down_body(const MgArgs& A, int bid, int nb) {
   if (A.part_scale) write_partials3(sacc, A.part_scale, s_red, bid);
 }
 
-__global__ void __launch_bounds__(kNT) k_mg_down(MgArgs A) { mg_down_body(A, blockIdx.x, gridDim.x); }
+__global__ void __launch_bounds__(kNT) k_mg_down(MgArgs A) { mg_down_body<0>(A, blockIdx.x, gridDim.x); }
 
 // Up: x2 = x1 + oc * e_c(aggregate) on the staged halo, post-smooth, write e.
 // Level 0 writes u = B r and the partials of gamma = r.u.
+template <int kInstance>
 __device__ __forceinline__ void mg_up_body(const MgArgs& A, int bid, int nb) {
   __shared__ GsTile s;
   __shared__ double s_red[8 * kNW];
@@ -1298,84 +1317,65 @@ __device__ __forceinline__ void mg_up_body(const MgArgs& A, int bid, int nb) {
   if (A.level0) write_partials3(acc, A.part_out, s_red, bid);
 }
 
-__global__ void __launch_bounds__(kNT) k_mg_up(MgArgs A) { mg_up_body(A, blockIdx.x, gridDim.x); }
+__global__ void __launch_bounds__(kNT) k_mg_up(MgArgs A) { mg_up_body<0>(A, blockIdx.x, gridDim.x); }
 
-// Coarsest level (<= 64 cells): its operator is inverted once per solve by
-// Gauss-Jordan elimination (SPD, so no pivoting).  Thread (row, g) keeps the
-// four entries (row, g + 16 c) in registers; each pivot step publishes the
-// pivot row and column through LDS buffers alternated by step parity, so one
-// barrier per step suffices.  The inverse is stored column-major.
+// Coarsest level (<= kMgBottomMax cells): its operator is inverted once per
+// solve by Gauss-Jordan elimination (SPD, so no pivoting).  Thread (row, col)
+// keeps entry (row, col) in a register; each pivot step publishes the pivot
+// row and column through LDS buffers alternated by step parity, so one barrier
+// per step suffices.  The inverse is stored column-major.
+template <int kInstance>
 __device__ __forceinline__ void mg_factor_body(const MgArgs& A, double* inv) {
-  constexpr int kG = 1024 / kMgBottomMax, kC = kMgBottomMax / kG;
-  __shared__ double s_prow[2][kMgBottomMax], s_pcol[2][kMgBottomMax];
+  constexpr int kB = kMgBottomMax;
+  __shared__ double s_prow[2][kB], s_pcol[2][kB];
   const MgLevel& L = A.F;
   const int n = L.nx * L.ny, nx = L.nx, t = threadIdx.x;
-  const int row = t % kMgBottomMax, g = t / kMgBottomMax;
-  double a[kC];
-#pragma unroll
-  for (int c = 0; c < kC; c++) {
-    const int k = g + kG * c;
-    double v = 0.0;
-    if (row < n && k < n) {
-      const size_t m = mg_at(L, row % nx, row / nx);
-      if (k == row) v = L.d[m];
-      else if (k == row - 1 && row % nx > 0) v = -L.cx[m - 1];
-      else if (k == row + 1 && row % nx + 1 < nx) v = -L.cx[m];
-      else if (k == row - nx) v = -L.cy[m - L.P];
-      else if (k == row + nx) v = -L.cy[m];
-    }
-    a[c] = v;
+  const int row = t % kB, col = t / kB;
+  double a = 0.0;
+  if (row < n && col < n) {
+    const size_t m = mg_at(L, row % nx, row / nx);
+    if (col == row) a = L.d[m];
+    else if (col == row - 1 && row % nx > 0) a = -L.cx[m - 1];
+    else if (col == row + 1 && row % nx + 1 < nx) a = -L.cx[m];
+    else if (col == row - nx) a = -L.cy[m - L.P];
+    else if (col == row + nx) a = -L.cy[m];
   }
   for (int p = 0; p < n; p++) {
     const int buf = p & 1;
-    if (row == p) {
-#pragma unroll
-      for (int c = 0; c < kC; c++) s_prow[buf][g + kG * c] = a[c];
-    }
-    if (g == p % kG) {
-#pragma unroll
-      for (int c = 0; c < kC; c++)
-        if (c == p / kG) s_pcol[buf][row] = a[c];
-    }
+    if (row == p) s_prow[buf][col] = a;
+    if (col == p) s_pcol[buf][row] = a;
     __syncthreads();
     const double piv = s_prow[buf][p], f = s_pcol[buf][row];
-#pragma unroll
-    for (int c = 0; c < kC; c++) {
-      const int k = g + kG * c;
-      const double pk = (k == p ? 1.0 : s_prow[buf][k]) / piv;
-      a[c] = row == p ? pk : (k == p ? 0.0 : a[c]) - f * pk;
-    }
+    const double pk = (col == p ? 1.0 : s_prow[buf][col]) / piv;
+    a = row == p ? pk : (col == p ? 0.0 : a) - f * pk;
   }
-#pragma unroll
-  for (int c = 0; c < kC; c++) {
-    const int k = g + kG * c;
-    if (row < n && k < n) inv[(size_t)k * n + row] = a[c];
-  }
+  if (row < n && col < n) inv[(size_t)col * n + row] = a;
 }
 
-__global__ void __launch_bounds__(1024) k_mg_factor(MgArgs A, double* inv) { mg_factor_body(A, inv); }
+__global__ void __launch_bounds__(kMgBottomMax * kMgBottomMax) k_mg_factor(MgArgs A, double* inv) {
+  mg_factor_body<0>(A, inv);
+}
 
 // Levels f .. L-1 (<= 1024 cells each, at most kMgLdsLevels) in one workgroup
 // of 1024 threads: thread t owns cell t of every level, keeps that cell's
 // operator row and right-hand side in registers, and the iterates live in LDS
 // with a zero ghost ring.  Same cycle as the tile kernels; level f's
 // correction is written back for the tile levels above.
+template <int kInstance>
 __device__ __forceinline__ void mg_coarse_cycle_body(const MgCycleArgs& A) {
   __shared__ double s_x[kMgLdsX];
   __shared__ double s_r[1024];
-  __shared__ double s_bp[1024 / kMgBottomMax][kMgBottomMax];
+  constexpr int kB = kMgBottomMax;
+  __shared__ double s_bp[kB][kB];
   if (A.sc->done) return;
   const int t = threadIdx.x;
   double cd[kMgLdsLevels], cl[kMgLdsLevels], cr[kMgLdsLevels], cb[kMgLdsLevels],
       ct[kMgLdsLevels], rr[kMgLdsLevels];
-  // the coarsest level's inverse, entries (row, g + 16 c) of thread (row, g)
-  constexpr int kG = 1024 / kMgBottomMax, kC = kMgBottomMax / kG;
+  // the coarsest level's inverse: entry (brow, bcol) in thread brow + kB * bcol
   const int nb = A.lv[A.nq - 1].nx * A.lv[A.nq - 1].ny;
-  const int brow = t % kMgBottomMax, bg = t / kMgBottomMax;
-  double binv[kC];
-#pragma unroll
-  for (int c = 0; c < kC; c++)
-    binv[c] = (brow < nb && bg + kG * c < nb) ? A.inv[(size_t)(bg + kG * c) * nb + brow] : 0.0;
+  const int brow = t % kB, bcol = t / kB;
+  const double binv =
+      (t < kB * kB && brow < nb && bcol < nb) ? A.inv[(size_t)bcol * nb + brow] : 0.0;
   for (int k = t; k < kMgLdsX; k += 1024) s_x[k] = 0.0;
 #pragma unroll
   for (int q = 0; q < kMgLdsLevels; q++) {
@@ -1428,18 +1428,14 @@ __device__ __forceinline__ void mg_coarse_cycle_body(const MgCycleArgs& A) {
         rr[q + 1 < kMgLdsLevels ? q + 1 : q] = v;
       }
       __syncthreads();
-    } else {  // coarsest level: x = inv r, partial sums over 16 column groups
+    } else {  // coarsest level: x = inv r (products in LDS, summed per row)
       if (act) s_r[t] = rr[q];
       __syncthreads();
-      double y = 0.0;
-#pragma unroll
-      for (int c = 0; c < kC; c++)
-        if (bg + kG * c < n) y += binv[c] * s_r[bg + kG * c];
-      s_bp[bg][brow] = y;
+      if (t < kB * kB) s_bp[bcol][brow] = bcol < n ? binv * s_r[bcol] : 0.0;
       __syncthreads();
       if (act) {
         double v = 0.0;
-        for (int g = 0; g < kG; g++) v += s_bp[g][t];
+        for (int c = 0; c < n; c++) v += s_bp[c][t];
         x[k] = v;
       }
       __syncthreads();
@@ -1474,9 +1470,10 @@ __device__ __forceinline__ void mg_coarse_cycle_body(const MgCycleArgs& A) {
   }
 }
 
-__global__ void __launch_bounds__(1024) k_mg_coarse_cycle(MgCycleArgs A) { mg_coarse_cycle_body(A); }
+__global__ void __launch_bounds__(1024) k_mg_coarse_cycle(MgCycleArgs A) { mg_coarse_cycle_body<0>(A); }
 
 // MG-PCG: x = 0, r = rhs (from the assembly), p = 0, partials of |rhs/V|^2.
+template <int kInstance>
 __device__ __forceinline__ void mgpcg_init_body(const MgPcgArgs& A, int bid, int nb) {
   __shared__ double s_red[8 * kNW];
   const Geo& G = A.G;
@@ -1499,7 +1496,7 @@ __device__ __forceinline__ void mgpcg_init_body(const MgPcgArgs& A, int bid, int
   write_partials3(acc, A.part_out, s_red, bid);
 }
 
-__global__ void __launch_bounds__(kNT) k_mgpcg_init(MgPcgArgs A) { mgpcg_init_body(A, blockIdx.x, gridDim.x); }
+__global__ void __launch_bounds__(kNT) k_mgpcg_init(MgPcgArgs A) { mgpcg_init_body<0>(A, blockIdx.x, gridDim.x); }
 
 // Cells of one 64x8 PCG tile held by a thread (kPcgRows of them).
 struct PqCells {
@@ -1534,6 +1531,7 @@ __device__ __forceinline__ void pq_load(const MgPcgArgs& A, int t, PqCells& c) {
 
 // p = u + beta p_old (neighbours recomputed), q = M p, partials of p.q.
 // The first tile's loads are issued before the reduction that yields beta.
+template <int kInstance>
 __device__ __forceinline__ void mgpcg_pq_body(const MgPcgArgs& A, int bid, int nb) {
   __shared__ double s_red[8 * kNW];
   DevScalars* sc = A.sc;
@@ -1571,7 +1569,7 @@ __device__ __forceinline__ void mgpcg_pq_body(const MgPcgArgs& A, int bid, int n
   write_partials3(acc, A.part_out, s_red, bid);
 }
 
-__global__ void __launch_bounds__(kNT) k_mgpcg_pq(MgPcgArgs A) { mgpcg_pq_body(A, blockIdx.x, gridDim.x); }
+__global__ void __launch_bounds__(kNT) k_mgpcg_pq(MgPcgArgs A) { mgpcg_pq_body<0>(A, blockIdx.x, gridDim.x); }
 
 struct XrCells {
   double x[kPcgRows], p[kPcgRows], r[kPcgRows], q[kPcgRows], vol[kPcgRows];
@@ -1599,6 +1597,7 @@ __device__ __forceinline__ void xr_load(const MgPcgArgs& A, int t, XrCells& c) {
 
 // alpha = gamma / p.q; x += alpha p; r -= alpha q; partials of |r/V|^2.
 // The first tile's loads are issued before the reduction that yields alpha.
+template <int kInstance>
 __device__ __forceinline__ void mgpcg_xr_body(const MgPcgArgs& A, int bid, int nb) {
   __shared__ double s_red[8 * kNW];
   DevScalars* sc = A.sc;
@@ -1628,7 +1627,7 @@ __device__ __forceinline__ void mgpcg_xr_body(const MgPcgArgs& A, int bid, int n
   write_partials3(acc, A.part_out, s_red, bid);
 }
 
-__global__ void __launch_bounds__(kNT) k_mgpcg_xr(MgPcgArgs A) { mgpcg_xr_body(A, blockIdx.x, gridDim.x); }
+__global__ void __launch_bounds__(kNT) k_mgpcg_xr(MgPcgArgs A) { mgpcg_xr_body<0>(A, blockIdx.x, gridDim.x); }
 
 __global__ void __launch_bounds__(256) k_copy(const double2* __restrict__ a,
                                              double2* __restrict__ b, size_t n) {
@@ -1689,16 +1688,16 @@ __device__ __forceinline__ const T& ens_arg(const T* t, const EnsSel& s, int i) 
 __global__ void __launch_bounds__(kNT) kb_stage(const StageArgs* t, const FsPhys* K, EnsMap m,
                                                 EnsSel s) {
   int i, bid, nb;
-  if (ens_slot(m, i, bid, nb)) stage_body(ens_arg(t, s, i), K[i], bid, nb);
+  if (ens_slot(m, i, bid, nb)) stage_body<1>(ens_arg(t, s, i), K[i], bid, nb);
 }
 __global__ void __launch_bounds__(kNT) kb_circ(const CircArgs* t, const FsPhys* K, EnsMap m,
                                                EnsSel s) {
   int i, bid, nb;
-  if (ens_slot(m, i, bid, nb)) circ_body(ens_arg(t, s, i), K[i]);
+  if (ens_slot(m, i, bid, nb)) circ_body<1>(ens_arg(t, s, i), K[i]);
 }
 __global__ void __launch_bounds__(kNT) kb_ctrl(const CtrlArgs* t, EnsMap m, EnsSel s) {
   int i, bid, nb;
-  if (ens_slot(m, i, bid, nb)) ctrl_body(ens_arg(t, s, i));
+  if (ens_slot(m, i, bid, nb)) ctrl_body<1>(ens_arg(t, s, i));
 }
 __global__ void kb_scalars(DevScalars* sc, const uint8_t* mask, int n, int op) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1721,50 +1720,50 @@ __global__ void kb_scalars(DevScalars* sc, const uint8_t* mask, int n, int op) {
 __global__ void __launch_bounds__(kNT) kb_lapl(const LaplArgs* t, const FsPhys* K, EnsMap m,
                                                EnsSel s) {
   int i, bid, nb;
-  if (ens_slot(m, i, bid, nb)) lapl_body(ens_arg(t, s, i), K[i], bid, nb);
+  if (ens_slot(m, i, bid, nb)) lapl_body<1>(ens_arg(t, s, i), K[i], bid, nb);
 }
 __global__ void __launch_bounds__(256) kb_mg_coarsen(const MgArgs* t, EnsMap m, EnsSel s) {
   int i, bid, nb;
-  if (ens_slot(m, i, bid, nb)) mg_coarsen_body(ens_arg(t, s, i), bid, nb);
+  if (ens_slot(m, i, bid, nb)) mg_coarsen_body<1>(ens_arg(t, s, i), bid, nb);
 }
-__global__ void __launch_bounds__(1024) kb_mg_factor(const MgArgs* t, double* const* inv,
+__global__ void __launch_bounds__(kMgBottomMax * kMgBottomMax) kb_mg_factor(const MgArgs* t, double* const* inv,
                                                      EnsMap m, EnsSel s) {
   int i, bid, nb;
-  if (ens_slot(m, i, bid, nb)) mg_factor_body(ens_arg(t, s, i), inv[i]);
+  if (ens_slot(m, i, bid, nb)) mg_factor_body<1>(ens_arg(t, s, i), inv[i]);
 }
 __global__ void __launch_bounds__(kNT) kb_mg_down(const MgArgs* t, EnsMap m, EnsSel s) {
   int i, bid, nb;
-  if (ens_slot(m, i, bid, nb)) mg_down_body(ens_arg(t, s, i), bid, nb);
+  if (ens_slot(m, i, bid, nb)) mg_down_body<1>(ens_arg(t, s, i), bid, nb);
 }
 __global__ void __launch_bounds__(kNT) kb_mg_up(const MgArgs* t, EnsMap m, EnsSel s) {
   int i, bid, nb;
-  if (ens_slot(m, i, bid, nb)) mg_up_body(ens_arg(t, s, i), bid, nb);
+  if (ens_slot(m, i, bid, nb)) mg_up_body<1>(ens_arg(t, s, i), bid, nb);
 }
 __global__ void __launch_bounds__(1024) kb_mg_coarse_cycle(const MgCycleArgs* t, EnsMap m,
                                                            EnsSel s) {
   int i, bid, nb;
-  if (ens_slot(m, i, bid, nb)) mg_coarse_cycle_body(ens_arg(t, s, i));
+  if (ens_slot(m, i, bid, nb)) mg_coarse_cycle_body<1>(ens_arg(t, s, i));
 }
 __global__ void __launch_bounds__(kNT) kb_mgpcg_init(const MgPcgArgs* t, EnsMap m, EnsSel s) {
   int i, bid, nb;
-  if (ens_slot(m, i, bid, nb)) mgpcg_init_body(ens_arg(t, s, i), bid, nb);
+  if (ens_slot(m, i, bid, nb)) mgpcg_init_body<1>(ens_arg(t, s, i), bid, nb);
 }
 __global__ void __launch_bounds__(kNT) kb_mgpcg_pq(const MgPcgArgs* t, EnsMap m, EnsSel s) {
   int i, bid, nb;
-  if (ens_slot(m, i, bid, nb)) mgpcg_pq_body(ens_arg(t, s, i), bid, nb);
+  if (ens_slot(m, i, bid, nb)) mgpcg_pq_body<1>(ens_arg(t, s, i), bid, nb);
 }
 __global__ void __launch_bounds__(kNT) kb_mgpcg_xr(const MgPcgArgs* t, EnsMap m, EnsSel s) {
   int i, bid, nb;
-  if (ens_slot(m, i, bid, nb)) mgpcg_xr_body(ens_arg(t, s, i), bid, nb);
+  if (ens_slot(m, i, bid, nb)) mgpcg_xr_body<1>(ens_arg(t, s, i), bid, nb);
 }
 __global__ void __launch_bounds__(kNT) kb_pcg_finish(const PcgArgs* t, EnsMap m, EnsSel s) {
   int i, bid, nb;
-  if (ens_slot(m, i, bid, nb)) pcg_finish_body(ens_arg(t, s, i), bid, nb);
+  if (ens_slot(m, i, bid, nb)) pcg_finish_body<1>(ens_arg(t, s, i), bid, nb);
 }
 __global__ void __launch_bounds__(kNT) kb_pcg_tres(const PcgArgs* t, EnsMap one, EnsSel s,
                                                    const int32_t* nbfin) {
   int i, bid, nb;
-  if (ens_slot(one, i, bid, nb)) pcg_tres_body(ens_arg(t, s, i), nbfin[i]);
+  if (ens_slot(one, i, bid, nb)) pcg_tres_body<1>(ens_arg(t, s, i), nbfin[i]);
 }
 
 }  // namespace
@@ -1775,8 +1774,8 @@ static inline int nblocks_of(const Geo& G, int* nbx, int* nby) {
   return *nbx * *nby;
 }
 
-void launch_stage(const StageArgs& a, const FsPhys& K, hipStream_t s) {
-  hipLaunchKernelGGL(k_stage, dim3(a.nbx * a.nby), dim3(kNT), 0, s, a, K);
+void launch_stage(const StageArgs* a, const FsPhys* K, int nblocks, hipStream_t s) {
+  hipLaunchKernelGGL(k_stage, dim3(nblocks), dim3(kNT), 0, s, a, K);
 }
 void launch_circ(const CircArgs& a, const FsPhys& K, hipStream_t s) {
   hipLaunchKernelGGL(k_circ, dim3(1), dim3(kNT), 0, s, a, K);
@@ -1793,8 +1792,8 @@ void launch_reject(DevScalars* sc, hipStream_t s) {
 void launch_begin_step(DevScalars* sc, hipStream_t s) {
   hipLaunchKernelGGL(k_begin_step, dim3(1), dim3(1), 0, s, sc);
 }
-void launch_lapl_assemble(const LaplArgs& a, const FsPhys& K, hipStream_t s) {
-  hipLaunchKernelGGL(k_lapl, dim3(a.nbx * a.nby), dim3(kNT), 0, s, a, K);
+void launch_lapl_assemble(const LaplArgs* a, const FsPhys* K, int nblocks, hipStream_t s) {
+  hipLaunchKernelGGL(k_lapl, dim3(nblocks), dim3(kNT), 0, s, a, K);
 }
 void launch_pcg_init(const PcgArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(k_pcg_init, dim3(capped_grid(a.nbx * a.nby)), dim3(kNT), 0, s, a);
@@ -1820,7 +1819,7 @@ void launch_mg_up(const MgArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(k_mg_up, dim3(mg_grid(a.F)), dim3(kNT), 0, s, a);
 }
 void launch_mg_factor(const MgArgs& a, double* inv, hipStream_t s) {
-  hipLaunchKernelGGL(k_mg_factor, dim3(1), dim3(1024), 0, s, a, inv);
+  hipLaunchKernelGGL(k_mg_factor, dim3(1), dim3(kMgBottomMax * kMgBottomMax), 0, s, a, inv);
 }
 void launch_mg_coarse_cycle(const MgCycleArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(k_mg_coarse_cycle, dim3(1), dim3(1024), 0, s, a);
@@ -1874,7 +1873,8 @@ void launch_ens_mg_coarsen(const MgArgs* t, const EnsMap& m, const EnsSel& s, hi
 void launch_ens_mg_factor(const MgArgs* t, double* const* inv, const EnsMap& m, const EnsSel& s,
                           hipStream_t st) {
   if (m.nblocks)
-    hipLaunchKernelGGL(kb_mg_factor, dim3(m.nblocks), dim3(1024), 0, st, t, inv, m, s);
+    hipLaunchKernelGGL(kb_mg_factor, dim3(m.nblocks), dim3(kMgBottomMax * kMgBottomMax), 0, st, t,
+                       inv, m, s);
 }
 void launch_ens_mg_down(const MgArgs* t, const EnsMap& m, const EnsSel& s, hipStream_t st) {
   if (m.nblocks) hipLaunchKernelGGL(kb_mg_down, dim3(m.nblocks), dim3(kNT), 0, st, t, m, s);

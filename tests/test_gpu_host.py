@@ -179,7 +179,15 @@ def test_graph_replay_equals_eager_launches(O):
 
 
 def test_tiny_grid_uses_jacobi_fallback(O):
-    """At <= 64 cells the hierarchy has one level; the solve must still work."""
+    """At <= 16 cells the hierarchy has one level; the solve must still work."""
+    c = _cfg(4, 3)
+    c["lapl_solver"] = 1
+    d, o, errs = _replayed_parity(O, c, 8)
+    assert max(errs.values()) <= 1e-10, errs
+
+
+def test_small_grid_two_level_hierarchy(O):
+    """17..64 cells: a two-level hierarchy whose coarse level is the dense solve."""
     c = _cfg(8, 6)
     c["lapl_solver"] = 1
     d, o, errs = _replayed_parity(O, c, 8)
