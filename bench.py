@@ -274,11 +274,12 @@ def main():
     t_stages = sum(kern[f"stage{s}"] for s in (1, 2, 3, 4, 5)) * 1e-3
     ach_s = bytes_step / t_stages / 1e9
     stage_traffic = profiled_traffic("k_stage", args.n)
-    # multigrid finest level: down reads r, d, cx, cy, writes x1 and the
-    # quarter-size coarse residual (5.25 planes); up reads x1, r, d, cx, cy and
-    # the coarse correction, writes e (6.25 planes)
-    ach_d = 5.25 * plane / (kern["mg_down0"] * 1e-3) / 1e9
-    ach_u = 6.25 * plane / (kern["mg_up0"] * 1e-3) / 1e9
+    # multigrid finest level with the fused CG update: down reads r, w, s, d,
+    # cx, cy, u, p, x, writes p, s, x, r, x1 and the quarter-size coarse
+    # residual (14.25 planes); up reads x1, r, d, cx, cy and the coarse
+    # correction, writes u and w = M u (7.25 planes)
+    ach_d = 14.25 * plane / (kern["mg_down0"] * 1e-3) / 1e9
+    ach_u = 7.25 * plane / (kern["mg_up0"] * 1e-3) / 1e9
     timers = r["timers"]
     tot_ms = r["elapsed"] * 1e3
     iters = r["iters"]

@@ -159,8 +159,8 @@ struct fs_handle {
   StageArgs* d_stab = nullptr;
   LaplArgs* d_ltab = nullptr;
   FsPhys* d_phys = nullptr;
-  hipGraphExec_t mg_graph = nullptr;  // two MG-PCG iterations (parity 1, then 0)
-  hipGraphExec_t learn_graph = nullptr;  // two learned-FCG iterations
+  hipGraphExec_t mg_graph[2] = {};    // one MG-PCG iteration, by iteration parity
+  hipGraphExec_t learn_graph[2] = {}; // one learned-FCG iteration, by parity
   const void* learn_graph_key = nullptr; // the network (or the handle: none) it was made for
   Geo G;
   std::vector<double> log;
@@ -424,24 +424,29 @@ MgPcgArgs mgpcg_args(fs_handle* h, int it) {
   return a;
 }
 
-// The first level run by the one-workgroup kernel: at most kMgLdsCells cells,
-// at most kMgLdsLevels levels from there down, padded iterates within kMgLdsX
-// doubles; never level 0 (its kernels carry the PCG reductions).
+// The first level run by the one-workgroup kernel: at most kMgLdsCells cells
+// (padded, at most kMgLdsR), the next at most 1024, at most kMgLdsLevels
+// levels from there down, padded iterates within kMgLdsX doubles; never level
+// 0 (its kernels carry the PCG reductions).
 size_t mg_lds_first(const fs_handle* h) {
   const size_t L = h->mg.size();
+  auto cells = [&](size_t q) { return (size_t)h->mg[q].nx * h->mg[q].ny; };
   for (size_t f = 1; f + 1 < L; f++) {
     size_t x = 0;
     for (size_t q = f; q < L; q++) x += (size_t)(h->mg[q].nx + 2) * (h->mg[q].ny + 2);
-    if ((size_t)h->mg[f].nx * h->mg[f].ny <= (size_t)kMgLdsCells && L - f <= (size_t)kMgLdsLevels &&
-        x <= (size_t)kMgLdsX)
+    const size_t padded = (size_t)(h->mg[f].nx + 2) * (h->mg[f].ny + 2);
+    if (cells(f) <= (size_t)kMgLdsCells && cells(f + 1) <= 1024 &&
+        L - f <= (size_t)kMgLdsLevels && x <= (size_t)kMgLdsX && padded <= (size_t)kMgLdsR)
       return f;
   }
   return L - 1;
 }
 
-// Partial-sum buffers of the MG-PCG chain: 0 = |r/V|^2, 1 = r.u, 2 = p.q
-// The one-workgroup part of the cycle: levels mg_first .. L-1.
-MgCycleArgs cycle_args(fs_handle* h) {
+// Partial-sum buffers of the MG-PCG chain: 0 = |r/V|^2, 1 = r.u and u.w
+// (flexible CG: r.z and z.q), 2 = p.q (flexible CG) or the learned input scale.
+// The one-workgroup part of V-cycle `it` (-1: the initial cycle): levels
+// mg_first .. L-1; when it starts at level 1 it also tests convergence.
+MgCycleArgs cycle_args(fs_handle* h, int it) {
   const size_t L = h->mg.size(), f = h->mg_first;
   MgCycleArgs c;
   memset(&c, 0, sizeof(c));
@@ -455,7 +460,73 @@ MgCycleArgs cycle_args(fs_handle* h) {
   c.inv = h->d_mginv;
   c.sc = h->d_sc;
   c.oc = kMgOver;
+  if (f == 1 && it >= 0) {
+    c.check = 1;
+    c.it = it;
+    c.part_in = h->part(0);
+    c.nb_in = mg_grid(h->mg[0]);
+  }
   return c;
+}
+
+// Level-0 residual buffers alternate by iteration parity: iteration it reads
+// r_buf(it) and writes r_buf(it + 1); the right-hand side starts in r_buf(0).
+double* r_buf(fs_handle* h, int k) { return h->pois((k & 1) ? PO_R1 : PO_R0); }
+
+MgCg cg_vectors(fs_handle* h, int it) {
+  MgCg c;
+  memset(&c, 0, sizeof(c));
+  c.x = h->pois(PO_X);
+  c.p = h->pois(PO_MP0);
+  c.u = h->pois(PO_MU);
+  c.w = h->pois(PO_W0);
+  c.s_old = h->pois((it & 1) ? PO_S1 : PO_S0);
+  c.s_new = h->pois((it & 1) ? PO_S0 : PO_S1);
+  c.r_old = r_buf(h, it);
+  c.on = 1;
+  c.first = it == 0;
+  return c;
+}
+
+// The down kernel of level l < mg_first in V-cycle `it` of the multigrid-PCG.
+// it = -1, the initial cycle: level 0 tests the right-hand side (partials of
+// the init kernel).  it >= 0: level 0 applies the CG update of iteration it
+// (partials of the previous cycle's level-0 up kernel) and level 1 tests the
+// updated residual.
+MgArgs mg_down_args(fs_handle* h, size_t l, int it) {
+  MgArgs a = mg_args(h, l);
+  const int nb0 = mg_grid(h->mg[0]);
+  if (l == 0 && it < 0) {
+    a.check = 1;
+    a.init = 1;
+    a.part_in = h->part(0);
+    a.nb_in = capped_grid(h->nbx * h->pnby);
+  } else if (l == 0) {
+    a.it = it;
+    a.F.r = r_buf(h, it + 1);
+    a.part_in = h->part(1);
+    a.nb_in = nb0;
+    a.part_out = h->part(0);
+    a.cg = cg_vectors(h, it);
+  } else if (l == 1 && it >= 0) {
+    a.check = 1;
+    a.it = it;
+    a.part_in = h->part(0);
+    a.nb_in = nb0;
+  }
+  return a;
+}
+
+// The up kernel of level l in V-cycle `it`: level 0 smooths the residual of
+// that cycle, writes u, w = M u and the partials of r.u and u.w.
+MgArgs mg_up_args(fs_handle* h, size_t l, int it) {
+  MgArgs a = mg_args(h, l);
+  if (l == 0) {
+    a.F.r = r_buf(h, it + 1);
+    a.part_out = h->part(1);
+    a.cg = cg_vectors(h, it < 0 ? 0 : it);
+  }
+  return a;
 }
 
 // The level whose equation the learned coarse solver takes: the first level
@@ -466,46 +537,24 @@ size_t learned_level(const fs_handle* h) {
   return c;
 }
 
-// With `harvest`, the residual handed to the learned solver's level is
-// appended to its training set (eager use only: the stream is synchronised).
-int launch_vcycle(fs_handle* h, int it, bool init, int nb_rho, fsl::Net* harvest = nullptr) {
-  const size_t L = h->mg.size(), f = h->mg_first;
+// V-cycle `it` of the multigrid-PCG (-1: the initial cycle; it >= 0: one
+// whole CG iteration, its update fused into the finest down kernel).  Only
+// the parity of `it` matters after it = 0.  With `harvest`, the residual
+// handed to the learned solver's level is appended to its training set (eager
+// use only: the stream is synchronised).
+int launch_vcycle(fs_handle* h, int it, fsl::Net* harvest = nullptr) {
+  const size_t f = h->mg_first;
   const size_t hl = harvest ? learned_level(h) : 0;
   for (size_t l = 0; l < f; l++) {
-    MgArgs a = mg_args(h, l);
-    a.it = it;
-    a.init = init;
-    a.part_in = h->part(0);
-    a.nb_in = nb_rho;
-    launch_mg_down(a, h->st);
+    launch_mg_down(mg_down_args(h, l, it), h->st);
     if (harvest && l + 1 == hl) {
       if (sync_scalars(h)) return FS_E_HIP;
       if (!h->h_sc->done && fsl::net_store_sample(harvest, h->mg[hl], h->st) < 0) return FS_E_HIP;
     }
   }
-  launch_mg_coarse_cycle(cycle_args(h), h->st);
-  for (size_t l = f; l-- > 0;) {
-    MgArgs a = mg_args(h, l);
-    a.part_out = h->part(1);
-    launch_mg_up(a, h->st);
-  }
+  launch_mg_coarse_cycle(cycle_args(h, it), h->st);
+  for (size_t l = f; l-- > 0;) launch_mg_up(mg_up_args(h, l, it), h->st);
   return 0;
-}
-
-// One MG-PCG iteration: p/q, x/r update, V-cycle (convergence is tested in
-// the V-cycle's first kernel).  Only the parity of `it` matters after it = 0.
-int launch_mg_iteration(fs_handle* h, int it, fsl::Net* harvest = nullptr) {
-  const int nb_pcg = capped_grid(h->nbx * h->pnby);
-  MgPcgArgs a = mgpcg_args(h, it);
-  a.part_in = h->part(1);
-  a.nb_in = mg_grid(h->mg[0]);
-  a.part_out = h->part(2);
-  launch_mgpcg_pq(a, h->st);
-  a.part_in = h->part(2);
-  a.nb_in = nb_pcg;
-  a.part_out = h->part(0);
-  launch_mgpcg_xr(a, h->st);
-  return launch_vcycle(h, it, false, nb_pcg, harvest);
 }
 
 // Learned preconditioner: the multigrid cycle down to the learned level c
@@ -521,6 +570,7 @@ int launch_learned_cycle(fs_handle* h, int it, bool init, int nb_rho) {
     MgArgs a = mg_args(h, l);
     a.it = it;
     a.init = init;
+    a.check = l == 0;
     a.part_in = h->part(0);
     a.nb_in = nb_rho;
     if (l + 1 == c) a.part_scale = h->part(2);
@@ -558,31 +608,40 @@ int launch_learned_iteration(fs_handle* h, int it) {
   return launch_learned_cycle(h, it, false, nb_pcg);
 }
 
-// Capture two iterations (parity 1 then 0) once per handle; every buffer is
-// fixed for the handle's lifetime, so the graph is replayed for all solves.
-int make_mg_graph(fs_handle* h) {
+// Capture one iteration of each parity (1 and 2) once per handle: only the
+// parity of the iteration number matters after the first, and every buffer is
+// fixed for the handle's lifetime, so the graphs serve all solves.
+int capture_iteration(fs_handle* h, hipGraphExec_t* out, int (*launch)(fs_handle*, int), int it) {
   hipGraph_t g = nullptr;
   HIP_OK(hipStreamBeginCapture(h->st, hipStreamCaptureModeThreadLocal));
-  if (launch_mg_iteration(h, 1) || launch_mg_iteration(h, 2)) return FS_E_HIP;
-  HIP_OK(hipStreamEndCapture(h->st, &g));
-  HIP_OK(hipGraphInstantiate(&h->mg_graph, g, nullptr, nullptr, 0));
-  HIP_OK(hipGraphDestroy(g));
-  return 0;
-}
-
-int make_learn_graph(fs_handle* h) {
-  if (h->learn_graph) HIP_OK(hipGraphExecDestroy(h->learn_graph));
-  h->learn_graph = nullptr;
-  hipGraph_t g = nullptr;
-  HIP_OK(hipStreamBeginCapture(h->st, hipStreamCaptureModeThreadLocal));
-  if (launch_learned_iteration(h, 1) || launch_learned_iteration(h, 2)) {
+  if (launch(h, it)) {
     (void)hipStreamEndCapture(h->st, &g);
     if (g) (void)hipGraphDestroy(g);
     return FS_E_HIP;
   }
   HIP_OK(hipStreamEndCapture(h->st, &g));
-  HIP_OK(hipGraphInstantiate(&h->learn_graph, g, nullptr, nullptr, 0));
+  HIP_OK(hipGraphInstantiate(out, g, nullptr, nullptr, 0));
   HIP_OK(hipGraphDestroy(g));
+  return 0;
+}
+
+int mg_iteration(fs_handle* h, int it) { return launch_vcycle(h, it); }
+
+int make_mg_graph(fs_handle* h) {
+  if (capture_iteration(h, &h->mg_graph[1], mg_iteration, 1) ||
+      capture_iteration(h, &h->mg_graph[0], mg_iteration, 2))
+    return FS_E_HIP;
+  return 0;
+}
+
+int make_learn_graph(fs_handle* h) {
+  for (int k = 0; k < 2; k++) {
+    if (h->learn_graph[k]) HIP_OK(hipGraphExecDestroy(h->learn_graph[k]));
+    h->learn_graph[k] = nullptr;
+  }
+  if (capture_iteration(h, &h->learn_graph[1], launch_learned_iteration, 1) ||
+      capture_iteration(h, &h->learn_graph[0], launch_learned_iteration, 2))
+    return FS_E_HIP;
   h->learn_graph_key = h->net ? (const void*)h->net : (const void*)h;
   return 0;
 }
@@ -670,13 +729,10 @@ void log_solve(fs_handle* h, int lerr) {
 }
 
 // Iterations launched before the first convergence check: as many as the
-// previous solve took (an even count, for the two-iteration graphs), so that a
-// solve usually needs one host round trip; later checks every `dflt`.
-int first_chunk(int last, int dflt) {
-  if (last <= 0) return dflt;
-  const int n = last + 1;
-  return n + (n & 1);
-}
+// previous solve took, so that a solve usually needs one host round trip and
+// launches no iteration past its last (an iteration after convergence costs
+// its kernels' launch latency).  Without a previous solve: `dflt`.
+int first_chunk(int last, int dflt) { return last > 0 ? last : dflt; }
 
 int lapl_solve_impl(fs_handle* h, int* lerr, fs_solve_stats* stats) {
   auto t0 = std::chrono::steady_clock::now();
@@ -686,6 +742,19 @@ int lapl_solve_impl(fs_handle* h, int* lerr, fs_solve_stats* stats) {
   time_mark(h, 5);
   const int chunk = h->p.lapl_chunk > 0 ? h->p.lapl_chunk : 32;
   int it = 0;
+  // After each chunk of iterations the finishing kernels run, then the host
+  // reads the scalars.  They change the potential only once the solve has
+  // stopped converged, so launching them before knowing saves a host round
+  // trip after the last chunk.
+  PcgArgs pa = pcg_args(h, 0);
+  pa.part_out = h->part(2);
+  pa.part_in = h->part(2);
+  auto end_chunk = [&]() -> int {
+    HIP_OK(hipGetLastError());
+    launch_pcg_finish(pa, h->st);
+    time_mark(h, 6);
+    return sync_scalars(h) ? FS_E_HIP : 0;
+  };
   if (h->p.lapl_solver == 2) {
     // flexible CG with the learned coarse solver (eager launches); without an
     // attached network the learned level gets a zero correction
@@ -700,19 +769,18 @@ int lapl_solve_impl(fs_handle* h, int* lerr, fs_solve_stats* stats) {
       return FS_E_HIP;
     it = 1;
     const void* key = h->net ? (const void*)h->net : (const void*)h;
-    if (h->p.use_graphs && (!h->learn_graph || h->learn_graph_key != key) && make_learn_graph(h))
+    if (h->p.use_graphs && (!h->learn_graph[0] || h->learn_graph_key != key) &&
+        make_learn_graph(h))
       return FS_E_HIP;
-    int lchunk = first_chunk(h->last_iters[2], chunk > 8 ? chunk / 4 : chunk);
-    for (;;) {
-      if (h->p.use_graphs) {
-        for (int k = 0; k < lchunk; k += 2, it += 2) HIP_OK(hipGraphLaunch(h->learn_graph, h->st));
-      } else {
-        for (int k = 0; k < lchunk; k++, it++)
-          if (launch_learned_iteration(h, it)) return FS_E_HIP;
+    // iteration 0 is launched already; then 2 more, then chunk / 4 at a time
+    int lchunk = std::max(1, first_chunk(h->last_iters[2], chunk > 8 ? chunk / 4 : chunk) - 1);
+    for (int round = 0;; round++) {
+      for (int k = 0; k < lchunk; k++, it++) {
+        if (h->p.use_graphs) HIP_OK(hipGraphLaunch(h->learn_graph[it & 1], h->st));
+        else if (launch_learned_iteration(h, it)) return FS_E_HIP;
       }
-      lchunk = chunk > 8 ? chunk / 4 : chunk;
-      HIP_OK(hipGetLastError());
-      if (sync_scalars(h)) return FS_E_HIP;
+      lchunk = round == 0 ? 2 : chunk > 8 ? chunk / 4 : chunk;
+      if (int e = end_chunk()) return e;
       if (h->h_sc->status < 0) return check_status(h);
       if (h->h_sc->done) break;
     }
@@ -723,21 +791,18 @@ int lapl_solve_impl(fs_handle* h, int* lerr, fs_solve_stats* stats) {
     MgPcgArgs ia = mgpcg_args(h, 1);  // zero the p that iteration 0 reads as p_old
     ia.part_out = h->part(0);
     launch_mgpcg_init(ia, h->st);
-    const int nb_pcg = capped_grid(h->nbx * h->pnby);
-    if (launch_vcycle(h, 0, true, nb_pcg) || launch_mg_iteration(h, 0)) return FS_E_HIP;
+    if (launch_vcycle(h, -1) || launch_vcycle(h, 0)) return FS_E_HIP;
     it = 1;
-    if (h->p.use_graphs && !h->mg_graph && make_mg_graph(h)) return FS_E_HIP;
-    int mg_chunk = first_chunk(h->last_iters[1], chunk > 8 ? chunk / 4 : chunk);
-    for (;;) {
-      if (h->p.use_graphs) {
-        for (int k = 0; k < mg_chunk; k += 2, it += 2) HIP_OK(hipGraphLaunch(h->mg_graph, h->st));
-      } else {
-        for (int k = 0; k < mg_chunk; k++, it++)
-          if (launch_mg_iteration(h, it)) return FS_E_HIP;
+    if (h->p.use_graphs && !h->mg_graph[0] && make_mg_graph(h)) return FS_E_HIP;
+    // iteration 0 is launched already; then 2 more, then chunk / 4 at a time
+    int mg_chunk = std::max(1, first_chunk(h->last_iters[1], chunk > 8 ? chunk / 4 : chunk) - 1);
+    for (int round = 0;; round++) {
+      for (int k = 0; k < mg_chunk; k++, it++) {
+        if (h->p.use_graphs) HIP_OK(hipGraphLaunch(h->mg_graph[it & 1], h->st));
+        else if (launch_vcycle(h, it)) return FS_E_HIP;
       }
-      mg_chunk = chunk > 8 ? chunk / 4 : chunk;
-      HIP_OK(hipGetLastError());
-      if (sync_scalars(h)) return FS_E_HIP;
+      mg_chunk = round == 0 ? 2 : chunk > 8 ? chunk / 4 : chunk;
+      if (int e = end_chunk()) return e;
       if (h->h_sc->status < 0) return check_status(h);
       if (h->h_sc->done) break;
     }
@@ -749,18 +814,11 @@ int lapl_solve_impl(fs_handle* h, int* lerr, fs_solve_stats* stats) {
     for (;;) {
       const int n = it == 0 ? first_chunk(h->last_iters[0], chunk) : chunk;
       for (int k = 0; k < n; k++, it++) launch_pcg_it(pcg_args(h, it), h->st);
-      HIP_OK(hipGetLastError());
-      if (sync_scalars(h)) return FS_E_HIP;
+      if (int e = end_chunk()) return e;
       if (h->h_sc->status < 0) return check_status(h);
       if (h->h_sc->done) break;
     }
   }
-  PcgArgs pa = pcg_args(h, 0);
-  pa.part_out = h->part(2);
-  pa.part_in = h->part(2);
-  launch_pcg_finish(pa, h->st);
-  time_mark(h, 6);
-  if (sync_scalars(h)) return FS_E_HIP;
   if (int e = check_status(h)) return e;
   int ok = h->h_sc->done == 1;
   *lerr = ok ? 0 : 1;
@@ -897,11 +955,10 @@ struct fs_ensemble {
   MgArgs* t_factor = nullptr;     // [i]
   double** t_inv = nullptr;       // [i]
   MgPcgArgs* t_init = nullptr;    // [i]
-  MgArgs* t_down = nullptr;       // [(i*kMgMaxLevels + l)*3 + v], v: init, odd, even
-  MgArgs* t_up = nullptr;         // [i*kMgMaxLevels + l]
-  MgCycleArgs* t_cc = nullptr;    // [i]
-  MgPcgArgs* t_pq = nullptr;      // [i*3 + v], v: first, odd, even
-  MgPcgArgs* t_xr = nullptr;      // [i*2 + parity]
+  // V-cycle variants v: 0 the initial cycle, 1 odd, 2 even, 3 first iteration
+  MgArgs* t_down = nullptr;       // [(i*kMgMaxLevels + l)*4 + v]
+  MgArgs* t_up = nullptr;         // [(i*kMgMaxLevels + l)*4 + v]
+  MgCycleArgs* t_cc = nullptr;    // [i*4 + v]
   PcgArgs* t_fin = nullptr;       // [i]
   EnsMapH m_stage, m_one, m_pcg, m_coarsen[kMgMaxLevels], m_down[kMgMaxLevels], m_up[kMgMaxLevels];
   char* d_round = nullptr;
@@ -961,10 +1018,10 @@ int ens_tables(fs_ensemble* E) {
   std::vector<CircArgs> ci(n * 5);
   std::vector<CtrlArgs> ct(n);
   std::vector<LaplArgs> la(n * 6);
-  std::vector<MgArgs> co(n * kMgMaxLevels), fa(n), dn(n * kMgMaxLevels * 3), up(n * kMgMaxLevels);
+  std::vector<MgArgs> co(n * kMgMaxLevels), fa(n), dn(n * kMgMaxLevels * 4), up(n * kMgMaxLevels * 4);
   std::vector<double*> inv(n);
-  std::vector<MgPcgArgs> in(n), pq(n * 3), xr(n * 2);
-  std::vector<MgCycleArgs> cc(n);
+  std::vector<MgPcgArgs> in(n);
+  std::vector<MgCycleArgs> cc(n * 4);
   std::vector<PcgArgs> fi(n);
   for (int i = 0; i < n; i++) {
     fs_handle* h = E->m[i];
@@ -983,39 +1040,18 @@ int ens_tables(fs_ensemble* E) {
     for (int stg = 1; stg <= 5; stg++) ci[i * 5 + stg - 1] = circ_args(h, stg);
     ct[i] = ctrl_args(h);
     const int L = (int)h->mg.size();
-    const int nb_pcg = capped_grid(h->nbx * h->pnby);
     for (int l = 0; l + 1 < L; l++) co[i * kMgMaxLevels + l] = mg_args(h, l);
     fa[i] = mg_args(h, L - 1);
     inv[i] = h->d_mginv;
     in[i] = mgpcg_args(h, 1);
     in[i].part_out = h->part(0);
-    for (int l = 0; l < (int)h->mg_first; l++) {
-      for (int v = 0; v < 3; v++) {
-        MgArgs a = mg_args(h, l);
-        a.it = v == 1 ? 1 : 0;
-        a.init = v == 0;
-        a.part_in = h->part(0);
-        a.nb_in = nb_pcg;
-        dn[(i * kMgMaxLevels + l) * 3 + v] = a;
+    static const int kVariantIt[4] = {-1, 1, 2, 0};
+    for (int v = 0; v < 4; v++) {
+      for (int l = 0; l < (int)h->mg_first; l++) {
+        dn[(i * kMgMaxLevels + l) * 4 + v] = mg_down_args(h, l, kVariantIt[v]);
+        up[(i * kMgMaxLevels + l) * 4 + v] = mg_up_args(h, l, kVariantIt[v]);
       }
-      MgArgs u = mg_args(h, l);
-      u.part_out = h->part(1);
-      up[i * kMgMaxLevels + l] = u;
-    }
-    cc[i] = cycle_args(h);
-    for (int v = 0; v < 3; v++) {
-      MgPcgArgs a = mgpcg_args(h, v);  // it = 0 (first), 1 (odd), 2 (even)
-      a.part_in = h->part(1);
-      a.nb_in = mg_grid(h->mg[0]);
-      a.part_out = h->part(2);
-      pq[i * 3 + v] = a;
-    }
-    for (int par = 0; par < 2; par++) {
-      MgPcgArgs a = mgpcg_args(h, par);
-      a.part_in = h->part(2);
-      a.nb_in = nb_pcg;
-      a.part_out = h->part(0);
-      xr[i * 2 + par] = a;
+      cc[i * 4 + v] = cycle_args(h, kVariantIt[v]);
     }
     PcgArgs f = pcg_args(h, 0);
     f.part_out = h->part(2);
@@ -1027,8 +1063,7 @@ int ens_tables(fs_ensemble* E) {
       upload_table(&E->t_lapl, la) || upload_table(&E->t_coarsen, co) ||
       upload_table(&E->t_factor, fa) || upload_table(&E->t_inv, inv) ||
       upload_table(&E->t_init, in) || upload_table(&E->t_down, dn) ||
-      upload_table(&E->t_up, up) || upload_table(&E->t_cc, cc) || upload_table(&E->t_pq, pq) ||
-      upload_table(&E->t_xr, xr) || upload_table(&E->t_fin, fi))
+      upload_table(&E->t_up, up) || upload_table(&E->t_cc, cc) || upload_table(&E->t_fin, fi))
     return FS_E_HIP;
   return 0;
 }
@@ -1087,21 +1122,20 @@ int ens_fail(fs_ensemble* E, int i) {
 }
 
 // One V-cycle of every Poisson member: down over the tile levels, the
-// one-workgroup coarse cycle, up.  v: 0 init, 1 odd, 2 even iteration.
+// one-workgroup coarse cycle, up.  v: 0 the initial cycle, then one CG
+// iteration each: 1 odd, 2 even, 3 the first.
 void ens_vcycle(fs_ensemble* E, int v, const uint8_t* mask) {
   for (int l = 0; l < E->maxF; l++)
     launch_ens_mg_down(E->t_down, with_mask(E->m_down[l], mask),
-                       EnsSel{nullptr, kMgMaxLevels * 3, l * 3 + v}, E->st);
-  launch_ens_mg_coarse_cycle(E->t_cc, with_mask(E->m_one, mask), EnsSel{nullptr, 1, 0}, E->st);
+                       EnsSel{nullptr, kMgMaxLevels * 4, l * 4 + v}, E->st);
+  launch_ens_mg_coarse_cycle(E->t_cc, with_mask(E->m_one, mask), EnsSel{nullptr, 4, v}, E->st);
   for (int l = E->maxF; l-- > 0;)
-    launch_ens_mg_up(E->t_up, with_mask(E->m_up[l], mask), EnsSel{nullptr, kMgMaxLevels, l}, E->st);
+    launch_ens_mg_up(E->t_up, with_mask(E->m_up[l], mask), EnsSel{nullptr, kMgMaxLevels * 4, l * 4 + v},
+                     l == 0, E->st);
 }
 
 void ens_iteration(fs_ensemble* E, int it, const uint8_t* mask) {
-  const int v = it == 0 ? 0 : (it & 1) ? 1 : 2;
-  launch_ens_mgpcg_pq(E->t_pq, with_mask(E->m_pcg, mask), EnsSel{nullptr, 3, v}, E->st);
-  launch_ens_mgpcg_xr(E->t_xr, with_mask(E->m_pcg, mask), EnsSel{nullptr, 2, it & 1}, E->st);
-  ens_vcycle(E, (it & 1) ? 1 : 2, mask);
+  ens_vcycle(E, it == 0 ? 3 : (it & 1) ? 1 : 2, mask);
 }
 
 int ens_round(fs_ensemble* E, std::vector<int>& left) {
@@ -1227,7 +1261,7 @@ void ens_release(fs_ensemble* E) {
   };
   fr(E->t_phys); fr(E->t_stage); fr(E->t_circ); fr(E->t_ctrl); fr(E->t_lapl); fr(E->t_coarsen);
   fr(E->t_factor); fr(E->t_inv); fr(E->t_init); fr(E->t_down); fr(E->t_up); fr(E->t_cc);
-  fr(E->t_pq); fr(E->t_xr); fr(E->t_fin); fr(E->d_sc); fr(E->d_round);
+  fr(E->t_fin); fr(E->d_sc); fr(E->d_round);
   fr(E->m_stage.d); fr(E->m_one.d); fr(E->m_pcg.d);
   for (int l = 0; l < kMgMaxLevels; l++) {
     fr(E->m_coarsen[l].d);
@@ -1422,8 +1456,10 @@ int fs_destroy(fs_handle* h) {
   (void)hipFree(h->d_replay);
   (void)hipFree(h->d_mg);
   (void)hipFree(h->d_mginv);
-  if (h->mg_graph) (void)hipGraphExecDestroy(h->mg_graph);
-  if (h->learn_graph) (void)hipGraphExecDestroy(h->learn_graph);
+  for (int k = 0; k < 2; k++) {
+    if (h->mg_graph[k]) (void)hipGraphExecDestroy(h->mg_graph[k]);
+    if (h->learn_graph[k]) (void)hipGraphExecDestroy(h->learn_graph[k]);
+  }
   (void)hipFree(h->d_stab);
   (void)hipFree(h->d_ltab);
   (void)hipFree(h->d_phys);
@@ -1752,15 +1788,11 @@ int fs_time_kernel(fs_handle* h, int32_t which, int32_t n, double* avg_ms) {
     } else if (which == 16) {
       (void)launch_learned_iteration(h, 2 + (k & 1));
     } else if (which == 17) {
-      launch_mg_iteration(h, 2 + (k & 1));
-    } else if (which == 13 || which == 14) {
-      MgArgs a = mg_args(h, 0);
-      a.it = 2 + (k & 1);
-      a.part_in = h->part(0);
-      a.nb_in = capped_grid(h->nbx * h->pnby);
-      a.part_out = h->part(1);
-      if (which == 13) launch_mg_down(a, h->st);
-      else launch_mg_up(a, h->st);
+      launch_vcycle(h, 2 + (k & 1));
+    } else if (which == 13) {
+      launch_mg_down(mg_down_args(h, 0, 2 + (k & 1)), h->st);
+    } else if (which == 14) {
+      launch_mg_up(mg_up_args(h, 0, 2 + (k & 1)), h->st);
     } else {
       PcgArgs ia = pcg_args(h, 0);
       ia.part_out = h->part(0);
@@ -1866,10 +1898,12 @@ int fs_ensemble_create(int32_t n, const fs_grid_desc* g, const fs_params* p, fs_
     E->own_h.push_back(h->h_sc);
     h->d_sc = E->d_sc + i;
     h->h_sc = E->h_sc + i;
-    if (h->mg_graph) (void)hipGraphExecDestroy(h->mg_graph);
-    h->mg_graph = nullptr;
-    if (h->learn_graph) (void)hipGraphExecDestroy(h->learn_graph);
-    h->learn_graph = nullptr;
+    for (int k = 0; k < 2; k++) {
+      if (h->mg_graph[k]) (void)hipGraphExecDestroy(h->mg_graph[k]);
+      h->mg_graph[k] = nullptr;
+      if (h->learn_graph[k]) (void)hipGraphExecDestroy(h->learn_graph[k]);
+      h->learn_graph[k] = nullptr;
+    }
     if (build_arg_tables(h)) {
       ens_release(E);
       return FS_E_HIP;
@@ -1987,10 +2021,9 @@ int fs_learn_harvest(fs_learn* n, fs_handle* h, int32_t* added) {
   MgPcgArgs ia = mgpcg_args(h, 1);
   ia.part_out = h->part(0);
   launch_mgpcg_init(ia, h->st);
-  const int nb_pcg = capped_grid(h->nbx * h->pnby);
-  if (launch_vcycle(h, 0, true, nb_pcg, &n->net)) return FS_E_HIP;
+  if (launch_vcycle(h, -1, &n->net)) return FS_E_HIP;
   for (int it = 0; it < h->h_sc->maxit; it++) {
-    if (launch_mg_iteration(h, it, &n->net)) return FS_E_HIP;
+    if (launch_vcycle(h, it, &n->net)) return FS_E_HIP;
     if (h->h_sc->done || n->net.count >= n->net.cap) break;
   }
   HIP_OK(hipStreamSynchronize(h->st));

@@ -125,18 +125,35 @@ struct MgLevel {
   int32_t nx, ny, P, pad_;
 };
 
+// Level-0 vectors of the multigrid-PCG in the Chronopoulos-Gear form: the
+// finest down kernel applies the CG update before smoothing the new residual,
+//   p = u + beta p,  s = w + beta s_old,  x += alpha p,  r = r_old - alpha s,
+// and the finest up kernel writes u = B r, w = M u and the partials of r.u
+// and u.w that give the next alpha and beta.
+struct MgCg {
+  double* x;
+  double* p;
+  const double* u;      // B r of the last cycle (level-0 e)
+  double* w;            // M u
+  const double* s_old;  // s = M p, double-buffered by iteration parity
+  double* s_new;
+  const double* r_old;  // residual before the update (F.r receives the new one)
+  int32_t on, first;    // first iteration: p = u, s = w
+};
+
 struct MgArgs {
   MgLevel F;             // fine level of this kernel
   MgLevel C;             // coarse level
-  const double* part_in; // level-0 down: residual partials of the x/r update
+  const double* part_in; // convergence test / CG update: the previous kernel's partials
   int32_t nb_in;
-  double* part_out;      // level-0 up: partials of gamma = r.u
+  double* part_out;      // level-0 up: partials of r.u (and u.w or z.q); level-0 CG down: |r/V|^2
   DevScalars* sc;
   Geo G;                 // level-0 metrics (volumes for the residual norm)
   const double* q;       // level-0 up, flexible CG: q = M p of the last update (z.q partials)
   double* part_scale;    // level-0 down, learned solver: partials of sum r_c^2 / d_c
   double oc;
-  int32_t it, level0, init, pad_;
+  int32_t it, level0, init, check;  // check: convergence test on part_in before the work
+  MgCg cg;
 };
 
 // Kernels whose per-block partial sums every block of the next kernel reads
@@ -148,9 +165,11 @@ inline int capped_grid(int ntiles) { return ntiles < kMaxPartials ? ntiles : kMa
 // most levels a hierarchy may have
 constexpr int kMgMaxLevels = 16;
 constexpr int kMgBottomMax = 16;   // cells of the coarsest level (dense inverse)
-constexpr int kMgLdsLevels = 4;    // levels run by the one-workgroup kernel
-constexpr int kMgLdsCells = 1024;  // cells of the finest of those
-constexpr int kMgLdsX = 4096;      // LDS doubles for their padded iterates
+constexpr int kMgLdsLevels = 5;    // levels run by the one-workgroup kernel
+constexpr int kMgLdsCells = 4096;  // cells of the finest of those (the others <= 1024)
+constexpr int kMgLdsC0 = kMgLdsCells / 1024;  // its cells per thread
+constexpr int kMgLdsX = 6144;      // LDS doubles for their padded iterates
+constexpr int kMgLdsR = 4608;      // LDS doubles for the padded residual of the finest
 
 struct MgCycleArgs {
   MgLevel lv[kMgLdsLevels];  // levels f .. L-1, the last one the coarsest
@@ -159,6 +178,8 @@ struct MgCycleArgs {
   const double* inv;         // dense inverse of the coarsest operator
   DevScalars* sc;
   double oc;
+  const double* part_in;     // check: |r/V|^2 partials of the level-0 CG update
+  int32_t nb_in, it, check, pad_;
 };
 
 struct MgPcgArgs {
@@ -216,12 +237,11 @@ void launch_ens_mg_coarsen(const MgArgs* t, const EnsMap& m, const EnsSel& s, hi
 void launch_ens_mg_factor(const MgArgs* t, double* const* inv, const EnsMap& m, const EnsSel& s,
                           hipStream_t st);
 void launch_ens_mg_down(const MgArgs* t, const EnsMap& m, const EnsSel& s, hipStream_t st);
-void launch_ens_mg_up(const MgArgs* t, const EnsMap& m, const EnsSel& s, hipStream_t st);
+// cg: the level-0 up kernel of the multigrid-PCG (wider halo, w = M u)
+void launch_ens_mg_up(const MgArgs* t, const EnsMap& m, const EnsSel& s, bool cg, hipStream_t st);
 void launch_ens_mg_coarse_cycle(const MgCycleArgs* t, const EnsMap& m, const EnsSel& s,
                                 hipStream_t st);
 void launch_ens_mgpcg_init(const MgPcgArgs* t, const EnsMap& m, const EnsSel& s, hipStream_t st);
-void launch_ens_mgpcg_pq(const MgPcgArgs* t, const EnsMap& m, const EnsSel& s, hipStream_t st);
-void launch_ens_mgpcg_xr(const MgPcgArgs* t, const EnsMap& m, const EnsSel& s, hipStream_t st);
 // finish over map m (the members' capped PCG grids), then the true residual
 // with one block per member (map one, partial counts m.nb)
 void launch_ens_pcg_finish(const PcgArgs* t, const EnsMap& m, const EnsMap& one, const EnsSel& s,

@@ -1111,28 +1111,32 @@ __device__ __forceinline__ void mg_coarsen_body(const MgArgs& A, int bid, int nb
 
 __global__ void __launch_bounds__(256) k_mg_coarsen(MgArgs A) { mg_coarsen_body<0>(A, blockIdx.x, gridDim.x); }
 
-constexpr int kGsTX = 32, kGsTY = 16, kGsH = 4;  // tile and halo (four half-sweeps)
-constexpr int kGsW = kGsTX + 2 * kGsH, kGsR = kGsTY + 2 * kGsH;
+constexpr int kGsTX = 32, kGsTY = 16;  // tile
+constexpr int kGsH = 4;                 // halo of the down kernels (four half-sweeps)
 static_assert((kGsTX / 2) * kGsTY == kNT, "one thread per horizontal cell pair");
 
-// A level's equation on one tile plus a halo of kGsH cells, staged in LDS:
+// A level's equation on one tile plus a halo of H cells, staged in LDS:
 // right-hand side, diagonal, right/upper couplings and the iterate.  Cells
 // outside the level carry r = 0, d = 1 and no couplings, so they stay zero.
+template <int H>
 struct GsTile {
-  double x[kGsR][kGsW];
-  double r[kGsR][kGsW];
-  double d[kGsR][kGsW];
-  double cx[kGsR][kGsW];
-  double cy[kGsR][kGsW];
+  static constexpr int W = kGsTX + 2 * H, R = kGsTY + 2 * H;
+  double x[R][W];
+  double r[R][W];
+  double d[R][W];
+  double cx[R][W];
+  double cy[R][W];
 };
 
 __device__ __forceinline__ int gs_tiles_x(const MgLevel& F) { return (F.nx + kGsTX - 1) / kGsTX; }
 
 // Stage r, d, cx, cy of the tile at (i0, j0) and its halo.
-__device__ __forceinline__ void gs_stage(const MgLevel& F, GsTile& s, int i0, int j0) {
-  for (int k = threadIdx.x; k < kGsR * kGsW; k += kNT) {
-    const int b = k / kGsW, a = k % kGsW;
-    const int i = i0 - kGsH + a, j = j0 - kGsH + b;
+template <int H>
+__device__ __forceinline__ void gs_stage(const MgLevel& F, GsTile<H>& s, int i0, int j0) {
+  constexpr int W = GsTile<H>::W, R = GsTile<H>::R;
+  for (int k = threadIdx.x; k < R * W; k += kNT) {
+    const int b = k / W, a = k % W;
+    const int i = i0 - H + a, j = j0 - H + b;
     double r = 0.0, d = 1.0, cx = 0.0, cy = 0.0;
     if (i >= 0 && i < F.nx && j >= 0 && j < F.ny) {
       const size_t m = mg_at(F, i, j);
@@ -1149,19 +1153,21 @@ __device__ __forceinline__ void gs_stage(const MgLevel& F, GsTile& s, int i0, in
 }
 
 // One Gauss-Seidel half-sweep over the cells of colour c ((i + j) & 1 == c)
-// within h <= kGsH - 1 cells of the tile at (i0, j0), entirely in LDS.
-__device__ __forceinline__ void gs_half(GsTile& s, int i0, int j0, int c, int h) {
-  const int w2 = (kGsTX + 2 * h) / 2, rows = kGsTY + 2 * h, a0 = kGsH - h;
+// within h <= H - 1 cells of the tile at (i0, j0), entirely in LDS.
+template <int H>
+__device__ __forceinline__ void gs_half(GsTile<H>& s, int i0, int j0, int c, int h) {
+  const int w2 = (kGsTX + 2 * h) / 2, rows = kGsTY + 2 * h, a0 = H - h;
   for (int k = threadIdx.x; k < rows * w2; k += kNT) {
     const int b = a0 + k / w2;
-    const int a = a0 + 2 * (k % w2) + ((c + i0 - kGsH + a0 + j0 - kGsH + b) & 1);
+    const int a = a0 + 2 * (k % w2) + ((c + i0 - H + a0 + j0 - H + b) & 1);
     s.x[b][a] = (s.r[b][a] + s.cx[b][a - 1] * s.x[b][a - 1] + s.cx[b][a] * s.x[b][a + 1] +
                  s.cy[b - 1][a] * s.x[b - 1][a] + s.cy[b][a] * s.x[b + 1][a]) /
                 s.d[b][a];
   }
 }
 
-__device__ __forceinline__ double gs_mx(const GsTile& s, int b, int a) {
+template <int H>
+__device__ __forceinline__ double gs_mx(const GsTile<H>& s, int b, int a) {
   return s.d[b][a] * s.x[b][a] - s.cx[b][a - 1] * s.x[b][a - 1] - s.cx[b][a] * s.x[b][a + 1] -
          s.cy[b - 1][a] * s.x[b - 1][a] - s.cy[b][a] * s.x[b + 1][a];
 }
@@ -1170,13 +1176,126 @@ __device__ __forceinline__ int gs_ntiles(const MgLevel& F) {
   return gs_tiles_x(F) * ((F.ny + kGsTY - 1) / kGsTY);
 }
 
+// Convergence test of the multigrid-PCG on rho = |r/V|^2: of the right-hand
+// side (init) or after the update of iteration `it`.  Every block decides
+// identically; `writer` (one thread) records the count, the residual and the
+// outcome.  True when the solve stops here.
+__device__ __forceinline__ bool mg_check(DevScalars* sc, double rho, int init, int it,
+                                         bool writer) {
+  if (init) {
+    if (writer) {
+      sc->bnorm2 = rho;
+      sc->resid = rho > 0 ? 1.0 : 0.0;
+      sc->iters = 0;
+      sc->itc[0] = 0;
+      sc->itc[1] = 0;
+      if (rho == 0.0) sc->done = 1;
+    }
+    return rho == 0.0;
+  }
+  // iteration count kept in two slots by parity: the other slot was written
+  // by the previous iteration, the writer fills this one
+  const int cur = sc->itc[(it + 1) & 1] + 1;
+  const double res = sqrt(rho / sc->bnorm2);
+  const bool conv = res <= sc->tol;
+  const bool stop = conv || cur >= sc->maxit;
+  if (writer) {
+    sc->itc[it & 1] = cur;
+    sc->iters = cur;
+    sc->resid = res;
+    if (conv) sc->done = 1;
+    else if (stop) sc->done = 2;
+  }
+  return stop;
+}
+
+// Level-0 CG vectors of the staged cells a thread holds in the down kernel.
+constexpr int kGsSlots = (GsTile<kGsH>::R * GsTile<kGsH>::W + kNT - 1) / kNT;
+struct CgCells {
+  double r[kGsSlots], w[kGsSlots], s[kGsSlots], u[kGsSlots], p[kGsSlots], x[kGsSlots];
+};
+
+// Loads of one tile for the CG update: the operator into LDS, the vectors
+// into registers (u, p, x only for the tile's own cells).
+__device__ __forceinline__ void cg_stage(const MgArgs& A, GsTile<kGsH>& s, int i0, int j0,
+                                         CgCells& c) {
+  constexpr int W = GsTile<kGsH>::W, R = GsTile<kGsH>::R;
+  const MgLevel& F = A.F;
+#pragma unroll
+  for (int e = 0; e < kGsSlots; e++) {
+    const int k = threadIdx.x + kNT * e;
+    c.r[e] = c.w[e] = c.s[e] = c.u[e] = c.p[e] = c.x[e] = 0.0;
+    if (k < R * W) {
+      const int b = k / W, a = k % W;
+      const int i = i0 - kGsH + a, j = j0 - kGsH + b;
+      double d = 1.0, cx = 0.0, cy = 0.0;
+      if (i >= 0 && i < F.nx && j >= 0 && j < F.ny) {
+        const size_t m = mg_at(F, i, j);
+        d = F.d[m];
+        cx = F.cx[m];
+        cy = F.cy[m];
+        c.r[e] = A.cg.r_old[m];
+        c.w[e] = A.cg.w[m];
+        if (!A.cg.first) c.s[e] = A.cg.s_old[m];
+        if (a >= kGsH && a < kGsH + kGsTX && b >= kGsH && b < kGsH + kGsTY) {
+          c.u[e] = A.cg.u[m];
+          if (!A.cg.first) c.p[e] = A.cg.p[m];
+          c.x[e] = A.cg.x[m];
+        }
+      }
+      s.d[b][a] = d;
+      s.cx[b][a] = cx;
+      s.cy[b][a] = cy;
+    }
+  }
+}
+
+// The CG update of the staged cells: the new residual goes to LDS (halo cells
+// recompute their owners' values exactly); the tile's own cells also write
+// p, s, x, r and add |r/V|^2 to acc.
+__device__ __forceinline__ void cg_update(const MgArgs& A, GsTile<kGsH>& s, int i0, int j0,
+                                          const CgCells& c, double alpha, double beta,
+                                          double& acc) {
+  constexpr int W = GsTile<kGsH>::W, R = GsTile<kGsH>::R;
+  const MgLevel& F = A.F;
+  const Geo& G = A.G;
+#pragma unroll
+  for (int e = 0; e < kGsSlots; e++) {
+    const int k = threadIdx.x + kNT * e;
+    if (k < R * W) {
+      const int b = k / W, a = k % W;
+      const int i = i0 - kGsH + a, j = j0 - kGsH + b;
+      double r = 0.0;
+      if (i >= 0 && i < F.nx && j >= 0 && j < F.ny) {
+        const double sn = A.cg.first ? c.w[e] : c.w[e] + beta * c.s[e];
+        r = c.r[e] - alpha * sn;
+        if (a >= kGsH && a < kGsH + kGsTX && b >= kGsH && b < kGsH + kGsTY) {
+          const size_t m = mg_at(F, i, j);
+          const double pn = A.cg.first ? c.u[e] : c.u[e] + beta * c.p[e];
+          A.cg.p[m] = pn;
+          A.cg.s_new[m] = sn;
+          A.cg.x[m] = c.x[e] + alpha * pn;
+          F.r[m] = r;
+          const double V = G.dlx[i] * G.dly[j] * (G.cyl ? G.r[j + 1] : 1.0);
+          const double rn = r / V;
+          acc += rn * rn;
+        }
+      }
+      s.r[b][a] = r;
+    }
+  }
+}
+
 // Down: pre-smooth M x = r on each tile, keep x1, restrict the residual into
-// the coarse level.  Level 0 first checks convergence from the x/r-update
-// partials (every block identically); their loads are issued together with
-// the first tile's.
+// the coarse level.  With `check` the kernel first tests convergence on the
+// partials of the previous kernel (every block identically); in the level-0
+// CG form it first applies the CG update (alpha and beta from the up kernel's
+// partials) and writes the partials of the new |r/V|^2, which the next
+// kernel tests.  The partial sums are loaded together with the first tile.
 template <int kInstance>
 __device__ __forceinline__ void mg_down_body(const MgArgs& A, int bid, int nb) {
-  __shared__ GsTile s;
+  constexpr int W = GsTile<kGsH>::W, R = GsTile<kGsH>::R;
+  __shared__ GsTile<kGsH> s;
   __shared__ double s_res[kGsTY][kGsTX / 2];
   __shared__ double s_red[8 * kNW];
   DevScalars* sc = A.sc;
@@ -1185,54 +1304,54 @@ __device__ __forceinline__ void mg_down_body(const MgArgs& A, int bid, int nb) {
   const int ntx = gs_tiles_x(F);
   const TileRange tr = tile_range(gs_ntiles(F), bid, nb);
   double sacc[3] = {0.0, 0.0, 0.0};  // sum r_c^2 / d_c for the learned coarse solver
+  double racc[3] = {0.0, 0.0, 0.0};  // sum |r/V|^2 after the CG update
   if (tr.t0 >= tr.t1) {  // block 0 always has tiles (it writes the scalars)
     if (A.part_scale) write_partials3(sacc, A.part_scale, s_red, bid);
+    if (A.cg.on) write_partials3(racc, A.part_out, s_red, bid);
     return;
   }
+  const bool red = A.check || A.cg.on;
   double pv[3];
-  if (A.level0) load_partials3(A.part_in, A.nb_in, pv);
-  gs_stage(F, s, (tr.t0 % ntx) * kGsTX, (tr.t0 / ntx) * kGsTY);
-  if (A.level0) {
+  if (red) load_partials3(A.part_in, A.nb_in, pv);
+  CgCells cc;
+  {
+    const int i0 = (tr.t0 % ntx) * kGsTX, j0 = (tr.t0 / ntx) * kGsTY;
+    if (A.cg.on) cg_stage(A, s, i0, j0, cc);
+    else gs_stage(F, s, i0, j0);
+  }
+  double alpha = 0.0, beta = 0.0;
+  if (red) {
     double tot[3];
     finish_partials3(pv, s_red, tot);
-    const double rho = tot[0];
-    if (A.init) {
-      if (bid == 0 && threadIdx.x == 0) {
-        sc->bnorm2 = rho;
-        sc->resid = rho > 0 ? 1.0 : 0.0;
-        sc->iters = 0;
-        sc->itc[0] = 0;
-        sc->itc[1] = 0;
-        if (rho == 0.0) sc->done = 1;
-      }
-      if (rho == 0.0) return;
+    if (A.check) {
+      if (mg_check(sc, tot[0], A.init, A.it, bid == 0 && threadIdx.x == 0)) return;
     } else {
-      // iteration count kept in two slots by parity: this kernel reads the
-      // other slot (written by the previous iteration) and block 0 writes its own
-      const int cur = sc->itc[(A.it + 1) & 1] + 1;
-      const double res = sqrt(rho / sc->bnorm2);
-      const bool conv = res <= sc->tol;
-      const bool stop = conv || cur >= sc->maxit;
-      if (bid == 0 && threadIdx.x == 0) {
-        sc->itc[A.it & 1] = cur;
-        sc->iters = cur;
-        sc->resid = res;
-        if (conv) sc->done = 1;
-        else if (stop) sc->done = 2;
+      // gamma = r.u, delta = u.M u; p.s = delta - beta gamma / alpha_old
+      const double gam = tot[0], del = tot[1];
+      if (A.cg.first) {
+        alpha = gam / del;
+      } else {
+        beta = gam / sc->cg_gam[(A.it + 1) & 1];
+        alpha = gam / (del - beta * gam / sc->cg_alpha[(A.it + 1) & 1]);
       }
-      if (stop) return;
+      if (bid == 0 && threadIdx.x == 0) {
+        sc->cg_gam[A.it & 1] = gam;
+        sc->cg_alpha[A.it & 1] = alpha;
+      }
     }
   }
   for (int t = tr.t0; t < tr.t1; t++) {
     const int i0 = (t % ntx) * kGsTX, j0 = (t / ntx) * kGsTY;
     if (t > tr.t0) {
       __syncthreads();
-      gs_stage(F, s, i0, j0);
+      if (A.cg.on) cg_stage(A, s, i0, j0, cc);
+      else gs_stage(F, s, i0, j0);
     }
+    if (A.cg.on) cg_update(A, s, i0, j0, cc, alpha, beta, racc[0]);
     __syncthreads();
     // first (red) half-sweep from zero is pointwise: the whole staged halo
-    for (int k = threadIdx.x; k < kGsR * kGsW; k += kNT) {
-      const int b = k / kGsW, a = k % kGsW;
+    for (int k = threadIdx.x; k < R * W; k += kNT) {
+      const int b = k / W, a = k % W;
       s.x[b][a] = ((i0 + a + j0 + b) & 1) == 0 ? s.r[b][a] / s.d[b][a] : 0.0;
     }
     __syncthreads();
@@ -1266,15 +1385,19 @@ __device__ __forceinline__ void mg_down_body(const MgArgs& A, int bid, int nb) {
     }
   }
   if (A.part_scale) write_partials3(sacc, A.part_scale, s_red, bid);
+  if (A.cg.on) write_partials3(racc, A.part_out, s_red, bid);
 }
 
 __global__ void __launch_bounds__(kNT) k_mg_down(MgArgs A) { mg_down_body<0>(A, blockIdx.x, gridDim.x); }
 
 // Up: x2 = x1 + oc * e_c(aggregate) on the staged halo, post-smooth, write e.
-// Level 0 writes u = B r and the partials of gamma = r.u.
-template <int kInstance>
+// Level 0 writes u = B r and the partials of r.u, and of z.q (flexible CG) or,
+// in the CG form, w = M u and the partials of u.w; M u needs u one cell past
+// the tile, so that kernel stages a halo of H = 5 and sweeps one cell wider.
+template <int kInstance, int H>
 __device__ __forceinline__ void mg_up_body(const MgArgs& A, int bid, int nb) {
-  __shared__ GsTile s;
+  constexpr int W = GsTile<H>::W, R = GsTile<H>::R;
+  __shared__ GsTile<H> s;
   __shared__ double s_red[8 * kNW];
   if (A.sc->done) return;
   const MgLevel &F = A.F, &C = A.C;
@@ -1285,39 +1408,45 @@ __device__ __forceinline__ void mg_up_body(const MgArgs& A, int bid, int nb) {
     const int i0 = (t % ntx) * kGsTX, j0 = (t / ntx) * kGsTY;
     if (t > tr.t0) __syncthreads();
     gs_stage(F, s, i0, j0);
-    for (int k = threadIdx.x; k < kGsR * kGsW; k += kNT) {
-      const int b = k / kGsW, a = k % kGsW;
-      const int i = i0 - kGsH + a, j = j0 - kGsH + b;
+    for (int k = threadIdx.x; k < R * W; k += kNT) {
+      const int b = k / W, a = k % W;
+      const int i = i0 - H + a, j = j0 - H + b;
       double v = 0.0;
       if (i >= 0 && i < F.nx && j >= 0 && j < F.ny)
         v = F.x1[mg_at(F, i, j)] + A.oc * C.e[mg_at(C, i >> 1, j >> 1)];
       s.x[b][a] = v;
     }
     __syncthreads();
-    gs_half(s, i0, j0, 1, 3);
+    gs_half(s, i0, j0, 1, H - 1);
     __syncthreads();
-    gs_half(s, i0, j0, 0, 2);
+    gs_half(s, i0, j0, 0, H - 2);
     __syncthreads();
-    gs_half(s, i0, j0, 1, 1);
+    gs_half(s, i0, j0, 1, H - 3);
     __syncthreads();
-    gs_half(s, i0, j0, 0, 0);
+    gs_half(s, i0, j0, 0, H - 4);
     __syncthreads();
     for (int k = threadIdx.x; k < kGsTX * kGsTY; k += kNT) {
       const int i = i0 + k % kGsTX, j = j0 + k / kGsTX;
       if (i < F.nx && j < F.ny) {
-        const int b = kGsH + k / kGsTX, a = kGsH + k % kGsTX;
+        const int b = H + k / kGsTX, a = H + k % kGsTX;
         const double e = s.x[b][a];
         const size_t m = mg_at(F, i, j);
         F.e[m] = e;
         acc[0] += s.r[b][a] * e;
         if (A.q) acc[1] += A.q[m] * e;
+        if (H > kGsH && A.cg.on) {
+          const double w = gs_mx(s, b, a);
+          A.cg.w[m] = w;
+          acc[1] += e * w;
+        }
       }
     }
   }
   if (A.level0) write_partials3(acc, A.part_out, s_red, bid);
 }
 
-__global__ void __launch_bounds__(kNT) k_mg_up(MgArgs A) { mg_up_body<0>(A, blockIdx.x, gridDim.x); }
+template <int H>
+__global__ void __launch_bounds__(kNT) k_mg_up(MgArgs A) { mg_up_body<0, H>(A, blockIdx.x, gridDim.x); }
 
 // Coarsest level (<= kMgBottomMax cells): its operator is inverted once per
 // solve by Gauss-Jordan elimination (SPD, so no pivoting).  Thread (row, col)
@@ -1356,120 +1485,278 @@ __global__ void __launch_bounds__(kMgBottomMax * kMgBottomMax) k_mg_factor(MgArg
   mg_factor_body<0>(A, inv);
 }
 
-// Levels f .. L-1 (<= 1024 cells each, at most kMgLdsLevels) in one workgroup
-// of 1024 threads: thread t owns cell t of every level, keeps that cell's
-// operator row and right-hand side in registers, and the iterates live in LDS
-// with a zero ghost ring.  Same cycle as the tile kernels; level f's
-// correction is written back for the tile levels above.
-template <int kInstance>
-__device__ __forceinline__ void mg_coarse_cycle_body(const MgCycleArgs& A) {
-  __shared__ double s_x[kMgLdsX];
-  __shared__ double s_r[1024];
-  constexpr int kB = kMgBottomMax;
-  __shared__ double s_bp[kB][kB];
-  if (A.sc->done) return;
-  const int t = threadIdx.x;
-  double cd[kMgLdsLevels], cl[kMgLdsLevels], cr[kMgLdsLevels], cb[kMgLdsLevels],
-      ct[kMgLdsLevels], rr[kMgLdsLevels];
-  // the coarsest level's inverse: entry (brow, bcol) in thread brow + kB * bcol
-  const int nb = A.lv[A.nq - 1].nx * A.lv[A.nq - 1].ny;
-  const int brow = t % kB, bcol = t / kB;
-  const double binv =
-      (t < kB * kB && brow < nb && bcol < nb) ? A.inv[(size_t)bcol * nb + brow] : 0.0;
-  for (int k = t; k < kMgLdsX; k += 1024) s_x[k] = 0.0;
+// Levels f .. L-1 in one workgroup of 1024 threads (at most kMgLdsLevels;
+// level f has at most kMgLdsCells cells, the others at most 1024).  Iterates
+// and residuals live in LDS in the levels' padded (nx+2)(ny+2) layout, which
+// is also their layout in HBM; operator rows and right-hand sides stay in
+// registers.  Level f is held as horizontal cell pairs (thread t owns pairs
+// t + 1024 p, p < kMgLdsPairs), one cell of each colour per pair, so each
+// half-sweep keeps every lane busy; the coarser levels hold one cell per
+// thread.  The sweeps multiply by the inverse diagonal.  Same cycle as the
+// tile kernels; level f's correction is written back for the levels above.
+constexpr int kMgLdsPairs = kMgLdsCells / 2048;
+struct PairRows {
+  double dinv[2][kMgLdsPairs], l[2][kMgLdsPairs], r[2][kMgLdsPairs], b[2][kMgLdsPairs],
+      t[2][kMgLdsPairs], rhs[2][kMgLdsPairs];
+  int ij[2][kMgLdsPairs];  // i | j << 16 of the cell of colour c, or -1
+};
+
+__device__ __forceinline__ int ij_k(int ij, int W) { return ((ij >> 16) + 1) * W + (ij & 0xffff) + 1; }
+__device__ __forceinline__ int ij_kc(int ij, int Wc) {
+  return ((ij >> 17) + 1) * Wc + ((ij & 0xffff) >> 1) + 1;
+}
+struct CellRows {
+  double dinv, l, r, b, t, rhs;
+  int k;   // padded index of cell t, or -1
+  int ij;  // i | j << 16
+};
+
+__device__ __forceinline__ void lds_pair_rows(const MgLevel& L, PairRows& R, int t) {
+  const int npr = (L.nx + 1) / 2, W = L.nx + 2;
 #pragma unroll
-  for (int q = 0; q < kMgLdsLevels; q++) {
-    cd[q] = 1.0;
-    cl[q] = cr[q] = cb[q] = ct[q] = rr[q] = 0.0;
-    if (q < A.nq) {
-      const MgLevel& L = A.lv[q];
-      if (t < L.nx * L.ny) {
-        const size_t m = mg_at(L, t % L.nx, t / L.nx), P = L.P;
-        cd[q] = L.d[m];
-        cl[q] = L.cx[m - 1];
-        cr[q] = L.cx[m];
-        cb[q] = L.cy[m - P];
-        ct[q] = L.cy[m];
-        if (q == 0) rr[q] = L.r[m];
-      }
-    }
-  }
-  __syncthreads();
+  for (int p = 0; p < kMgLdsPairs; p++) {
+    const int pp = t + 1024 * p;
 #pragma unroll
-  for (int q = 0; q < kMgLdsLevels; q++) {
-    if (q >= A.nq) break;
-    const MgLevel& L = A.lv[q];
-    const int n = L.nx * L.ny, W = L.nx + 2;
-    const int i = t % L.nx, j = t / L.nx, col = (i + j) & 1;
-    const bool act = t < n;
-    double* x = s_x + A.off[q];
-    const int k = (j + 1) * W + (i + 1);
-    if (q + 1 < A.nq) {
-      if (act && col == 0) x[k] = rr[q] / cd[q];
-      __syncthreads();
-      for (int s = 0; s < 3; s++) {  // black, red, black
-        if (act && col == ((s & 1) ^ 1))
-          x[k] = (rr[q] + cl[q] * x[k - 1] + cr[q] * x[k + 1] + cb[q] * x[k - W] +
-                  ct[q] * x[k + W]) /
-                 cd[q];
-        __syncthreads();
+    for (int c = 0; c < 2; c++) {
+      R.dinv[c][p] = 1.0;
+      R.l[c][p] = R.r[c][p] = R.b[c][p] = R.t[c][p] = R.rhs[c][p] = 0.0;
+      R.ij[c][p] = -1;
+      if (pp < npr * L.ny) {
+        const int j = pp / npr, i = 2 * (pp % npr) + (c ^ (j & 1));
+        if (i < L.nx) {
+          const int k = (j + 1) * W + i + 1;
+          R.dinv[c][p] = 1.0 / L.d[k];
+          R.l[c][p] = L.cx[k - 1];
+          R.r[c][p] = L.cx[k];
+          R.b[c][p] = L.cy[k - W];
+          R.t[c][p] = L.cy[k];
+          R.rhs[c][p] = L.r[k];
+          R.ij[c][p] = i | (j << 16);
+        }
       }
-      if (act)
-        s_r[t] = rr[q] - (cd[q] * x[k] - cl[q] * x[k - 1] - cr[q] * x[k + 1] -
-                          cb[q] * x[k - W] - ct[q] * x[k + W]);
-      __syncthreads();
-      const MgLevel& C = A.lv[q + 1];
-      if (q + 1 < kMgLdsLevels && t < C.nx * C.ny) {
-        const int I = t % C.nx, J = t / C.nx;
-        double v = 0.0;
-        for (int b = 0; b < 2; b++)
-          for (int a = 0; a < 2; a++)
-            if (2 * I + a < L.nx && 2 * J + b < L.ny) v += s_r[(2 * J + b) * L.nx + 2 * I + a];
-        rr[q + 1 < kMgLdsLevels ? q + 1 : q] = v;
-      }
-      __syncthreads();
-    } else {  // coarsest level: x = inv r (products in LDS, summed per row)
-      if (act) s_r[t] = rr[q];
-      __syncthreads();
-      if (t < kB * kB) s_bp[bcol][brow] = bcol < n ? binv * s_r[bcol] : 0.0;
-      __syncthreads();
-      if (act) {
-        double v = 0.0;
-        for (int c = 0; c < n; c++) v += s_bp[c][t];
-        x[k] = v;
-      }
-      __syncthreads();
-    }
-  }
-#pragma unroll
-  for (int q = kMgLdsLevels - 2; q >= 0; q--) {
-    if (q + 1 >= A.nq) continue;
-    const MgLevel &L = A.lv[q], &C = A.lv[q + 1];
-    const int n = L.nx * L.ny, W = L.nx + 2;
-    const int i = t % L.nx, j = t / L.nx, col = (i + j) & 1;
-    const bool act = t < n;
-    double* x = s_x + A.off[q];
-    const double* xc = s_x + A.off[q + 1];
-    const int k = (j + 1) * W + (i + 1);
-    if (act) x[k] += A.oc * xc[((j >> 1) + 1) * (C.nx + 2) + (i >> 1) + 1];
-    __syncthreads();
-    for (int s = 0; s < 4; s++) {  // black, red, black, red
-      if (act && col == ((s & 1) ^ 1))
-        x[k] = (rr[q] + cl[q] * x[k - 1] + cr[q] * x[k + 1] + cb[q] * x[k - W] +
-                ct[q] * x[k + W]) /
-               cd[q];
-      __syncthreads();
-    }
-  }
-  {
-    const MgLevel& L = A.lv[0];
-    if (t < L.nx * L.ny) {
-      const int i = t % L.nx, j = t / L.nx;
-      L.e[mg_at(L, i, j)] = s_x[A.off[0] + (j + 1) * (L.nx + 2) + i + 1];
     }
   }
 }
 
+__device__ __forceinline__ void lds_cell_rows(const MgLevel& L, CellRows& R, int t) {
+  const int W = L.nx + 2;
+  R.dinv = 1.0;
+  R.l = R.r = R.b = R.t = R.rhs = 0.0;
+  R.k = -1;
+  R.ij = 0;
+  if (t < L.nx * L.ny) {
+    const int i = t % L.nx, j = t / L.nx, k = (j + 1) * W + i + 1;
+    R.dinv = 1.0 / L.d[k];
+    R.l = L.cx[k - 1];
+    R.r = L.cx[k];
+    R.b = L.cy[k - W];
+    R.t = L.cy[k];
+    R.k = k;
+    R.ij = i | (j << 16);
+  }
+}
+
+__device__ __forceinline__ double lds_gs(double rhs, double l, double r, double b, double t,
+                                         double dinv, const double* x, int k, int W) {
+  return (rhs + l * x[k - 1] + r * x[k + 1] + b * x[k - W] + t * x[k + W]) * dinv;
+}
+__device__ __forceinline__ double lds_res(double rhs, double l, double r, double b, double t,
+                                          double dinv, const double* x, int k, int W) {
+  return rhs - (x[k] / dinv - l * x[k - 1] - r * x[k + 1] - b * x[k - W] - t * x[k + W]);
+}
+
+template <int COL>
+__device__ __forceinline__ void pair_half(const PairRows& R, double* x, int W) {
+#pragma unroll
+  for (int p = 0; p < kMgLdsPairs; p++) {
+    const int k = ij_k(R.ij[COL][p], W);
+    if (R.ij[COL][p] >= 0)
+      x[k] = lds_gs(R.rhs[COL][p], R.l[COL][p], R.r[COL][p], R.b[COL][p], R.t[COL][p],
+                    R.dinv[COL][p], x, k, W);
+  }
+  __syncthreads();
+}
+
+template <int COL>
+__device__ __forceinline__ void cell_half(const CellRows& R, double* x, int W) {
+  if (R.k >= 0 && (((R.ij & 0xffff) + (R.ij >> 16)) & 1) == COL)
+    x[R.k] = lds_gs(R.rhs, R.l, R.r, R.b, R.t, R.dinv, x, R.k, W);
+  __syncthreads();
+}
+
+// Pre-smoothing from zero (red pointwise, then black, red, black) and the
+// residual into s_r.
+__device__ __forceinline__ void pair_down(const PairRows& R, double* x, double* s_r, int W) {
+#pragma unroll
+  for (int p = 0; p < kMgLdsPairs; p++)
+    if (R.ij[0][p] >= 0) x[ij_k(R.ij[0][p], W)] = R.rhs[0][p] * R.dinv[0][p];
+  __syncthreads();
+  pair_half<1>(R, x, W);
+  pair_half<0>(R, x, W);
+  pair_half<1>(R, x, W);
+#pragma unroll
+  for (int c = 0; c < 2; c++)
+#pragma unroll
+    for (int p = 0; p < kMgLdsPairs; p++) {
+      const int k = ij_k(R.ij[c][p], W);
+      if (R.ij[c][p] >= 0)
+        s_r[k] = lds_res(R.rhs[c][p], R.l[c][p], R.r[c][p], R.b[c][p], R.t[c][p], R.dinv[c][p],
+                         x, k, W);
+    }
+  __syncthreads();
+}
+
+__device__ __forceinline__ void cell_down(const CellRows& R, double* x, double* s_r, int W) {
+  const bool red = R.k >= 0 && (((R.ij & 0xffff) + (R.ij >> 16)) & 1) == 0;
+  if (red) x[R.k] = R.rhs * R.dinv;
+  __syncthreads();
+  cell_half<1>(R, x, W);
+  cell_half<0>(R, x, W);
+  cell_half<1>(R, x, W);
+  if (R.k >= 0) s_r[R.k] = lds_res(R.rhs, R.l, R.r, R.b, R.t, R.dinv, x, R.k, W);
+  __syncthreads();
+}
+
+// Restriction of level L's residual (s_r) to the coarse cell held in C.
+__device__ __forceinline__ double lds_restrict(const MgLevel& L, const CellRows& C,
+                                               const double* s_r) {
+  double v = 0.0;
+  if (C.k >= 0) {
+    const int I = C.ij & 0xffff, J = C.ij >> 16, W = L.nx + 2;
+    for (int b = 0; b < 2; b++)
+      for (int a = 0; a < 2; a++)
+        if (2 * I + a < L.nx && 2 * J + b < L.ny) v += s_r[(2 * J + b + 1) * W + 2 * I + a + 1];
+  }
+  return v;
+}
+
+// Prolongation with over-correction and post-smoothing (black, red, black, red).
+__device__ __forceinline__ void pair_up(const PairRows& R, double* x, const double* xc, int W,
+                                        int Wc, double oc) {
+#pragma unroll
+  for (int c = 0; c < 2; c++)
+#pragma unroll
+    for (int p = 0; p < kMgLdsPairs; p++)
+      if (R.ij[c][p] >= 0) x[ij_k(R.ij[c][p], W)] += oc * xc[ij_kc(R.ij[c][p], Wc)];
+  __syncthreads();
+  pair_half<1>(R, x, W);
+  pair_half<0>(R, x, W);
+  pair_half<1>(R, x, W);
+  pair_half<0>(R, x, W);
+}
+
+__device__ __forceinline__ void cell_up(const CellRows& R, double* x, const double* xc, int W,
+                                        int Wc, double oc) {
+  if (R.k >= 0) {
+    const int i = R.ij & 0xffff, j = R.ij >> 16;
+    x[R.k] += oc * xc[((j >> 1) + 1) * Wc + (i >> 1) + 1];
+  }
+  __syncthreads();
+  cell_half<1>(R, x, W);
+  cell_half<0>(R, x, W);
+  cell_half<1>(R, x, W);
+  cell_half<0>(R, x, W);
+}
+
+template <int kInstance>
+__device__ __forceinline__ void mg_coarse_cycle_body(const MgCycleArgs& A) {
+  __shared__ double s_x[kMgLdsX];
+  __shared__ double s_r[kMgLdsR];
+  constexpr int kB = kMgBottomMax;
+  __shared__ double s_bp[kB][kB];
+  __shared__ double s_w[16];
+  if (A.sc->done) return;
+  const int t = threadIdx.x;
+  if (A.check) {  // convergence test on the level-0 CG update's |r/V|^2 partials
+    double v = 0.0;
+    for (int b = t; b < A.nb_in; b += 1024) v += A.part_in[3 * b];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if ((t & 63) == 0) s_w[t >> 6] = v;
+    __syncthreads();
+    double rho = 0.0;
+    for (int w = 0; w < 16; w++) rho += s_w[w];
+    if (mg_check(A.sc, rho, 0, A.it, t == 0)) return;
+  }
+  const int nq = A.nq;
+  const MgLevel& B = A.lv[nq - 1];  // coarsest
+  const int nb = B.nx * B.ny;
+  // the coarsest level's inverse: entry (brow, bcol) in thread brow + kB * bcol
+  const int brow = t % kB, bcol = t / kB;
+  const double binv =
+      (t < kB * kB && brow < nb && bcol < nb) ? A.inv[(size_t)bcol * nb + brow] : 0.0;
+  for (int k = t; k < kMgLdsX; k += 1024) s_x[k] = 0.0;
+  double rb = 0.0;  // right-hand side of the coarsest level at cell t
+  if (nq == 1) {
+    if (t < nb) rb = B.r[(t / B.nx + 1) * (B.nx + 2) + t % B.nx + 1];
+  } else {
+    PairRows R0;                        // level f
+    CellRows R1[kMgLdsLevels - 2];      // levels f+1 .. L-2
+    lds_pair_rows(A.lv[0], R0, t);
+#pragma unroll
+    for (int q = 1; q < kMgLdsLevels - 1; q++)
+      if (q < nq - 1) lds_cell_rows(A.lv[q], R1[q - 1], t);
+    CellRows Bc;  // the coarsest level's cell t (its position only)
+    lds_cell_rows(B, Bc, t);
+    __syncthreads();
+    // down: level f, then f+1 .. L-2, each restricted to the next
+    pair_down(R0, s_x + A.off[0], s_r, A.lv[0].nx + 2);
+    {
+      const double v = lds_restrict(A.lv[0], nq == 2 ? Bc : R1[0], s_r);
+      if (nq == 2) rb = v;
+      else R1[0].rhs = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 1; q < kMgLdsLevels - 1; q++) {
+      if (q + 1 >= nq) break;
+      cell_down(R1[q - 1], s_x + A.off[q], s_r, A.lv[q].nx + 2);
+      const double w = lds_restrict(A.lv[q], q + 2 == nq ? Bc : R1[q], s_r);
+      if (q + 2 == nq) rb = w;
+      else R1[q].rhs = w;
+      __syncthreads();
+    }
+    // coarsest level: x = inv r (products in LDS, summed per row)
+    if (t < nb) s_r[t] = rb;
+    __syncthreads();
+    if (t < kB * kB) s_bp[bcol][brow] = bcol < nb ? binv * s_r[bcol] : 0.0;
+    __syncthreads();
+    if (t < nb) {
+      double v = 0.0;
+      for (int c = 0; c < nb; c++) v += s_bp[c][t];
+      s_x[A.off[nq - 1] + Bc.k] = v;
+    }
+    __syncthreads();
+    // up: levels L-2 .. f+1, then level f
+#pragma unroll
+    for (int q = kMgLdsLevels - 2; q >= 1; q--) {
+      if (q + 1 >= nq) continue;
+      cell_up(R1[q - 1], s_x + A.off[q], s_x + A.off[q + 1], A.lv[q].nx + 2, A.lv[q + 1].nx + 2,
+              A.oc);
+    }
+    const MgLevel& L = A.lv[0];
+    pair_up(R0, s_x + A.off[0], s_x + A.off[1], L.nx + 2, A.lv[1].nx + 2, A.oc);
+#pragma unroll
+    for (int c = 0; c < 2; c++)
+#pragma unroll
+      for (int p = 0; p < kMgLdsPairs; p++)
+        if (R0.ij[c][p] >= 0) {
+          const int k = ij_k(R0.ij[c][p], L.nx + 2);
+          L.e[k] = s_x[A.off[0] + k];
+        }
+    return;
+  }
+  // a single level: the coarsest, solved directly
+  if (t < nb) s_r[t] = rb;
+  __syncthreads();
+  if (t < kB * kB) s_bp[bcol][brow] = bcol < nb ? binv * s_r[bcol] : 0.0;
+  __syncthreads();
+  if (t < nb) {
+    double v = 0.0;
+    for (int c = 0; c < nb; c++) v += s_bp[c][t];
+    B.e[(t / B.nx + 1) * (B.nx + 2) + t % B.nx + 1] = v;
+  }
+}
 __global__ void __launch_bounds__(1024) k_mg_coarse_cycle(MgCycleArgs A) { mg_coarse_cycle_body<0>(A); }
 
 // MG-PCG: x = 0, r = rhs (from the assembly), p = 0, partials of |rhs/V|^2.
@@ -1735,9 +2022,10 @@ __global__ void __launch_bounds__(kNT) kb_mg_down(const MgArgs* t, EnsMap m, Ens
   int i, bid, nb;
   if (ens_slot(m, i, bid, nb)) mg_down_body<1>(ens_arg(t, s, i), bid, nb);
 }
+template <int H>
 __global__ void __launch_bounds__(kNT) kb_mg_up(const MgArgs* t, EnsMap m, EnsSel s) {
   int i, bid, nb;
-  if (ens_slot(m, i, bid, nb)) mg_up_body<1>(ens_arg(t, s, i), bid, nb);
+  if (ens_slot(m, i, bid, nb)) mg_up_body<1, H>(ens_arg(t, s, i), bid, nb);
 }
 __global__ void __launch_bounds__(1024) kb_mg_coarse_cycle(const MgCycleArgs* t, EnsMap m,
                                                            EnsSel s) {
@@ -1747,14 +2035,6 @@ __global__ void __launch_bounds__(1024) kb_mg_coarse_cycle(const MgCycleArgs* t,
 __global__ void __launch_bounds__(kNT) kb_mgpcg_init(const MgPcgArgs* t, EnsMap m, EnsSel s) {
   int i, bid, nb;
   if (ens_slot(m, i, bid, nb)) mgpcg_init_body<1>(ens_arg(t, s, i), bid, nb);
-}
-__global__ void __launch_bounds__(kNT) kb_mgpcg_pq(const MgPcgArgs* t, EnsMap m, EnsSel s) {
-  int i, bid, nb;
-  if (ens_slot(m, i, bid, nb)) mgpcg_pq_body<1>(ens_arg(t, s, i), bid, nb);
-}
-__global__ void __launch_bounds__(kNT) kb_mgpcg_xr(const MgPcgArgs* t, EnsMap m, EnsSel s) {
-  int i, bid, nb;
-  if (ens_slot(m, i, bid, nb)) mgpcg_xr_body<1>(ens_arg(t, s, i), bid, nb);
 }
 __global__ void __launch_bounds__(kNT) kb_pcg_finish(const PcgArgs* t, EnsMap m, EnsSel s) {
   int i, bid, nb;
@@ -1816,7 +2096,8 @@ void launch_mg_down(const MgArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(k_mg_down, dim3(mg_grid(a.F)), dim3(kNT), 0, s, a);
 }
 void launch_mg_up(const MgArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL(k_mg_up, dim3(mg_grid(a.F)), dim3(kNT), 0, s, a);
+  if (a.cg.on) hipLaunchKernelGGL(k_mg_up<kGsH + 1>, dim3(mg_grid(a.F)), dim3(kNT), 0, s, a);
+  else hipLaunchKernelGGL(k_mg_up<kGsH>, dim3(mg_grid(a.F)), dim3(kNT), 0, s, a);
 }
 void launch_mg_factor(const MgArgs& a, double* inv, hipStream_t s) {
   hipLaunchKernelGGL(k_mg_factor, dim3(1), dim3(kMgBottomMax * kMgBottomMax), 0, s, a, inv);
@@ -1879,8 +2160,10 @@ void launch_ens_mg_factor(const MgArgs* t, double* const* inv, const EnsMap& m, 
 void launch_ens_mg_down(const MgArgs* t, const EnsMap& m, const EnsSel& s, hipStream_t st) {
   if (m.nblocks) hipLaunchKernelGGL(kb_mg_down, dim3(m.nblocks), dim3(kNT), 0, st, t, m, s);
 }
-void launch_ens_mg_up(const MgArgs* t, const EnsMap& m, const EnsSel& s, hipStream_t st) {
-  if (m.nblocks) hipLaunchKernelGGL(kb_mg_up, dim3(m.nblocks), dim3(kNT), 0, st, t, m, s);
+void launch_ens_mg_up(const MgArgs* t, const EnsMap& m, const EnsSel& s, bool cg, hipStream_t st) {
+  if (!m.nblocks) return;
+  if (cg) hipLaunchKernelGGL(kb_mg_up<kGsH + 1>, dim3(m.nblocks), dim3(kNT), 0, st, t, m, s);
+  else hipLaunchKernelGGL(kb_mg_up<kGsH>, dim3(m.nblocks), dim3(kNT), 0, st, t, m, s);
 }
 void launch_ens_mg_coarse_cycle(const MgCycleArgs* t, const EnsMap& m, const EnsSel& s,
                                 hipStream_t st) {
@@ -1889,12 +2172,6 @@ void launch_ens_mg_coarse_cycle(const MgCycleArgs* t, const EnsMap& m, const Ens
 }
 void launch_ens_mgpcg_init(const MgPcgArgs* t, const EnsMap& m, const EnsSel& s, hipStream_t st) {
   if (m.nblocks) hipLaunchKernelGGL(kb_mgpcg_init, dim3(m.nblocks), dim3(kNT), 0, st, t, m, s);
-}
-void launch_ens_mgpcg_pq(const MgPcgArgs* t, const EnsMap& m, const EnsSel& s, hipStream_t st) {
-  if (m.nblocks) hipLaunchKernelGGL(kb_mgpcg_pq, dim3(m.nblocks), dim3(kNT), 0, st, t, m, s);
-}
-void launch_ens_mgpcg_xr(const MgPcgArgs* t, const EnsMap& m, const EnsSel& s, hipStream_t st) {
-  if (m.nblocks) hipLaunchKernelGGL(kb_mgpcg_xr, dim3(m.nblocks), dim3(kNT), 0, st, t, m, s);
 }
 void launch_ens_pcg_finish(const PcgArgs* t, const EnsMap& m, const EnsMap& one, const EnsSel& s,
                            hipStream_t st) {

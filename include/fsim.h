@@ -196,7 +196,8 @@ int fs_set_timing(fs_handle* h, int32_t on);
  * state is overwritten, so call it after the timed run.  which: 1..5 = the
  * fused stage kernel of that RK stage, 10 = Poisson assembly, 11/12 = the
  * Jacobi-PCG iteration and initialisation kernels, 13/14 = the multigrid down
- * and up kernels of the finest level, 15 = the attached network on the learned
+ * and up kernels of the finest level (with the CG update and w = M u that the
+ * multigrid-PCG fuses into them), 15 = the attached network on the learned
  * level, 16 = one learned flexible-CG iteration, 17 = one multigrid-PCG
  * iteration (sequences of kernels, timed as a whole). */
 int fs_time_kernel(fs_handle* h, int32_t which, int32_t n, double* avg_ms);
