@@ -25,6 +25,7 @@ PEAK_HBM_GBS = 8000.0
 
 def gpu_run(args):
     from fluidsim_amd import Discharge, configs
+    from fluidsim_amd.lib import FsError
 
     if args.config == 2:
         cfg = configs.config2(args.n, rtol=args.rtol, chunk=args.chunk)
@@ -59,8 +60,16 @@ def gpu_run(args):
     # per-launch kernel durations on the live state (HIP events on the stream)
     k = {f"stage{s}": d.time_kernel(s, 50) for s in (1, 2, 3, 4, 5)}
     k["assembly"] = d.time_kernel(10, 50)
-    k["mg_down0"] = d.time_kernel(13, 200)
-    k["mg_up0"] = d.time_kernel(14, 200)
+    # multigrid-PCG: the down / up kernel of each tile level, the one-workgroup
+    # coarse kernel and one whole iteration
+    for lvl in range(16):
+        try:
+            k[f"mg_down{lvl}"] = d.time_kernel(20 + lvl, 200)
+            k[f"mg_up{lvl}"] = d.time_kernel(40 + lvl, 200)
+        except FsError:
+            break
+    k["mg_coarse"] = d.time_kernel(18, 200)
+    k["mg_iteration"] = d.time_kernel(17, 100)
     k["jacobi_pcg_it"] = d.time_kernel(11, 400)
     from fluidsim_amd.discharge import copy_bandwidth
 

@@ -118,6 +118,8 @@ void fs_phys_table(const FsPhys* K, double* o) {
 }
 
 // ---------------------------------------------------------------------------
+constexpr int kChainLog = 5;  // MG-PCG iteration graphs of 1, 2, 4, 8, 16 iterations
+
 struct fs_handle {
   fs_grid_desc g;
   fs_params p;
@@ -149,7 +151,8 @@ struct fs_handle {
   int pnby = 0, pnblk = 0;          // PCG tiles (64x8)
   std::vector<MgLevel> mg;          // multigrid hierarchy; level 0 aliases the PCG planes
   double* d_mg = nullptr;
-  double* d_mginv = nullptr;        // dense inverse of the coarsest operator (<= 64 cells)
+  double* d_mginv = nullptr;        // dense inverse of the coarsest operator
+  float* d_mgpk = nullptr;          // packed FP32 rows of the one-workgroup levels
   size_t mg_first = 0;              // first level of the one-workgroup coarse kernel
   int last_iters[3] = {0, 0, 0};    // iterations of the last solve, per solver
   fs_learn* net = nullptr;          // learned coarse solver (lapl_solver = 2)
@@ -159,7 +162,8 @@ struct fs_handle {
   StageArgs* d_stab = nullptr;
   LaplArgs* d_ltab = nullptr;
   FsPhys* d_phys = nullptr;
-  hipGraphExec_t mg_graph[2] = {};    // one MG-PCG iteration, by iteration parity
+  hipGraphExec_t mg_pro = nullptr;    // MG-PCG set-up and iteration 0
+  hipGraphExec_t mg_chain[2][kChainLog] = {};  // 2^m MG-PCG iterations from parity p
   hipGraphExec_t learn_graph[2] = {}; // one learned-FCG iteration, by parity
   const void* learn_graph_key = nullptr; // the network (or the handle: none) it was made for
   Geo G;
@@ -357,6 +361,7 @@ int build_mg(fs_handle* h) {
   int nx = h->bx, ny = h->by;
   size_t total = 0;
   HIP_OK(hipMalloc(&h->d_mginv, kMgBottomMax * kMgBottomMax * sizeof(double)));
+  HIP_OK(hipMalloc(&h->d_mgpk, kMgPackFloats * sizeof(float)));
   while ((size_t)nx * ny > (size_t)kMgBottomMax) {
     nx = (nx + 1) / 2;
     ny = (ny + 1) / 2;
@@ -458,6 +463,7 @@ MgCycleArgs cycle_args(fs_handle* h, int it) {
     off += (c.lv[q].nx + 2) * (c.lv[q].ny + 2);
   }
   c.inv = h->d_mginv;
+  c.pk = h->d_mgpk;
   c.sc = h->d_sc;
   c.oc = kMgOver;
   if (f == 1 && it >= 0) {
@@ -608,13 +614,14 @@ int launch_learned_iteration(fs_handle* h, int it) {
   return launch_learned_cycle(h, it, false, nb_pcg);
 }
 
-// Capture one iteration of each parity (1 and 2) once per handle: only the
-// parity of the iteration number matters after the first, and every buffer is
-// fixed for the handle's lifetime, so the graphs serve all solves.
-int capture_iteration(fs_handle* h, hipGraphExec_t* out, int (*launch)(fs_handle*, int), int it) {
+// Graphs are captured once per handle: every buffer is fixed for the
+// handle's lifetime, and after the first iteration only the parity of the
+// iteration number matters, so they serve all solves.
+template <class F>
+int capture(fs_handle* h, hipGraphExec_t* out, F&& launch) {
   hipGraph_t g = nullptr;
   HIP_OK(hipStreamBeginCapture(h->st, hipStreamCaptureModeThreadLocal));
-  if (launch(h, it)) {
+  if (launch()) {
     (void)hipStreamEndCapture(h->st, &g);
     if (g) (void)hipGraphDestroy(g);
     return FS_E_HIP;
@@ -625,13 +632,57 @@ int capture_iteration(fs_handle* h, hipGraphExec_t* out, int (*launch)(fs_handle
   return 0;
 }
 
-int mg_iteration(fs_handle* h, int it) { return launch_vcycle(h, it); }
+int capture_iteration(fs_handle* h, hipGraphExec_t* out, int (*launch)(fs_handle*, int), int it) {
+  return capture(h, out, [&]() { return launch(h, it); });
+}
 
-int make_mg_graph(fs_handle* h) {
-  if (capture_iteration(h, &h->mg_graph[1], mg_iteration, 1) ||
-      capture_iteration(h, &h->mg_graph[0], mg_iteration, 2))
-    return FS_E_HIP;
+// MG-PCG set-up: Galerkin hierarchy, coarsest inverse, x = 0, the initial
+// V-cycle and iteration 0.
+int mg_prologue(fs_handle* h) {
+  for (size_t l = 0; l + 1 < h->mg.size(); l++) launch_mg_coarsen(mg_args(h, l), h->st);
+  launch_mg_coarse_setup(cycle_args(h, -1), h->st);
+  MgPcgArgs ia = mgpcg_args(h, 1);
+  ia.part_out = h->part(0);
+  launch_mgpcg_init(ia, h->st);
+  return launch_vcycle(h, -1) || launch_vcycle(h, 0) ? FS_E_HIP : 0;
+}
+
+// MG-PCG iterations it .. it+n-1 (it >= 1): with graphs, as chains of 2^m
+// iterations (largest first), so a chunk costs a few graph launches.
+int launch_mg_chunk(fs_handle* h, int it, int n) {
+  while (n > 0) {
+    if (!h->p.use_graphs) {
+      if (launch_vcycle(h, it)) return FS_E_HIP;
+      it++, n--;
+      continue;
+    }
+    int m = kChainLog - 1;
+    while ((1 << m) > n) m--;
+    hipGraphExec_t& g = h->mg_chain[it & 1][m];
+    const int base = (it & 1) ? 1 : 2;
+    if (!g && capture(h, &g, [&]() {
+          for (int k = 0; k < (1 << m); k++)
+            if (launch_vcycle(h, base + k)) return FS_E_HIP;
+          return 0;
+        }))
+      return FS_E_HIP;
+    HIP_OK(hipGraphLaunch(g, h->st));
+    it += 1 << m;
+    n -= 1 << m;
+  }
   return 0;
+}
+
+void drop_graphs(fs_handle* h) {
+  auto drop = [](hipGraphExec_t& g) {
+    if (g) (void)hipGraphExecDestroy(g);
+    g = nullptr;
+  };
+  drop(h->mg_pro);
+  for (int k = 0; k < 2; k++) {
+    drop(h->learn_graph[k]);
+    for (int m = 0; m < kChainLog; m++) drop(h->mg_chain[k][m]);
+  }
 }
 
 int make_learn_graph(fs_handle* h) {
@@ -786,21 +837,19 @@ int lapl_solve_impl(fs_handle* h, int* lerr, fs_solve_stats* stats) {
     }
   } else if (h->p.lapl_solver == 1 && h->mg.size() > 1) {
     // multigrid-preconditioned PCG (grids of <= 64 cells use Jacobi-PCG)
-    for (size_t l = 0; l + 1 < h->mg.size(); l++) launch_mg_coarsen(mg_args(h, l), h->st);
-    launch_mg_factor(mg_args(h, h->mg.size() - 1), h->d_mginv, h->st);
-    MgPcgArgs ia = mgpcg_args(h, 1);  // zero the p that iteration 0 reads as p_old
-    ia.part_out = h->part(0);
-    launch_mgpcg_init(ia, h->st);
-    if (launch_vcycle(h, -1) || launch_vcycle(h, 0)) return FS_E_HIP;
+    if (!h->p.use_graphs) {
+      if (mg_prologue(h)) return FS_E_HIP;
+    } else {
+      if (!h->mg_pro && capture(h, &h->mg_pro, [&]() { return mg_prologue(h); }))
+        return FS_E_HIP;
+      HIP_OK(hipGraphLaunch(h->mg_pro, h->st));
+    }
     it = 1;
-    if (h->p.use_graphs && !h->mg_graph[0] && make_mg_graph(h)) return FS_E_HIP;
     // iteration 0 is launched already; then 2 more, then chunk / 4 at a time
     int mg_chunk = std::max(1, first_chunk(h->last_iters[1], chunk > 8 ? chunk / 4 : chunk) - 1);
     for (int round = 0;; round++) {
-      for (int k = 0; k < mg_chunk; k++, it++) {
-        if (h->p.use_graphs) HIP_OK(hipGraphLaunch(h->mg_graph[it & 1], h->st));
-        else if (launch_vcycle(h, it)) return FS_E_HIP;
-      }
+      if (launch_mg_chunk(h, it, mg_chunk)) return FS_E_HIP;
+      it += mg_chunk;
       mg_chunk = round == 0 ? 2 : chunk > 8 ? chunk / 4 : chunk;
       if (int e = end_chunk()) return e;
       if (h->h_sc->status < 0) return check_status(h);
@@ -952,8 +1001,6 @@ struct fs_ensemble {
   CtrlArgs* t_ctrl = nullptr;     // [i]
   LaplArgs* t_lapl = nullptr;     // [i*6 + rot*2 + alias]
   MgArgs* t_coarsen = nullptr;    // [i*kMgMaxLevels + l]
-  MgArgs* t_factor = nullptr;     // [i]
-  double** t_inv = nullptr;       // [i]
   MgPcgArgs* t_init = nullptr;    // [i]
   // V-cycle variants v: 0 the initial cycle, 1 odd, 2 even, 3 first iteration
   MgArgs* t_down = nullptr;       // [(i*kMgMaxLevels + l)*4 + v]
@@ -1018,8 +1065,7 @@ int ens_tables(fs_ensemble* E) {
   std::vector<CircArgs> ci(n * 5);
   std::vector<CtrlArgs> ct(n);
   std::vector<LaplArgs> la(n * 6);
-  std::vector<MgArgs> co(n * kMgMaxLevels), fa(n), dn(n * kMgMaxLevels * 4), up(n * kMgMaxLevels * 4);
-  std::vector<double*> inv(n);
+  std::vector<MgArgs> co(n * kMgMaxLevels), dn(n * kMgMaxLevels * 4), up(n * kMgMaxLevels * 4);
   std::vector<MgPcgArgs> in(n);
   std::vector<MgCycleArgs> cc(n * 4);
   std::vector<PcgArgs> fi(n);
@@ -1041,8 +1087,6 @@ int ens_tables(fs_ensemble* E) {
     ct[i] = ctrl_args(h);
     const int L = (int)h->mg.size();
     for (int l = 0; l + 1 < L; l++) co[i * kMgMaxLevels + l] = mg_args(h, l);
-    fa[i] = mg_args(h, L - 1);
-    inv[i] = h->d_mginv;
     in[i] = mgpcg_args(h, 1);
     in[i].part_out = h->part(0);
     static const int kVariantIt[4] = {-1, 1, 2, 0};
@@ -1061,7 +1105,6 @@ int ens_tables(fs_ensemble* E) {
   if (upload_table(&E->t_phys, phys) || upload_table(&E->t_stage, st) ||
       upload_table(&E->t_circ, ci) || upload_table(&E->t_ctrl, ct) ||
       upload_table(&E->t_lapl, la) || upload_table(&E->t_coarsen, co) ||
-      upload_table(&E->t_factor, fa) || upload_table(&E->t_inv, inv) ||
       upload_table(&E->t_init, in) || upload_table(&E->t_down, dn) ||
       upload_table(&E->t_up, up) || upload_table(&E->t_cc, cc) || upload_table(&E->t_fin, fi))
     return FS_E_HIP;
@@ -1198,8 +1241,7 @@ int ens_round(fs_ensemble* E, std::vector<int>& left) {
   for (int l = 0; l + 1 < E->maxL; l++)
     launch_ens_mg_coarsen(E->t_coarsen, with_mask(E->m_coarsen[l], r.pois),
                           EnsSel{nullptr, kMgMaxLevels, l}, E->st);
-  launch_ens_mg_factor(E->t_factor, E->t_inv, with_mask(E->m_one, r.pois), EnsSel{nullptr, 1, 0},
-                       E->st);
+  launch_ens_mg_coarse_setup(E->t_cc, with_mask(E->m_one, r.pois), EnsSel{nullptr, 4, 0}, E->st);
   launch_ens_mgpcg_init(E->t_init, with_mask(E->m_pcg, r.pois), EnsSel{nullptr, 1, 0}, E->st);
   ens_vcycle(E, 0, r.pois);
   ens_iteration(E, 0, r.pois);
@@ -1260,7 +1302,7 @@ void ens_release(fs_ensemble* E) {
     if (p) (void)hipFree(p);
   };
   fr(E->t_phys); fr(E->t_stage); fr(E->t_circ); fr(E->t_ctrl); fr(E->t_lapl); fr(E->t_coarsen);
-  fr(E->t_factor); fr(E->t_inv); fr(E->t_init); fr(E->t_down); fr(E->t_up); fr(E->t_cc);
+  fr(E->t_init); fr(E->t_down); fr(E->t_up); fr(E->t_cc);
   fr(E->t_fin); fr(E->d_sc); fr(E->d_round);
   fr(E->m_stage.d); fr(E->m_one.d); fr(E->m_pcg.d);
   for (int l = 0; l < kMgMaxLevels; l++) {
@@ -1456,10 +1498,8 @@ int fs_destroy(fs_handle* h) {
   (void)hipFree(h->d_replay);
   (void)hipFree(h->d_mg);
   (void)hipFree(h->d_mginv);
-  for (int k = 0; k < 2; k++) {
-    if (h->mg_graph[k]) (void)hipGraphExecDestroy(h->mg_graph[k]);
-    if (h->learn_graph[k]) (void)hipGraphExecDestroy(h->learn_graph[k]);
-  }
+  (void)hipFree(h->d_mgpk);
+  drop_graphs(h);
   (void)hipFree(h->d_stab);
   (void)hipFree(h->d_ltab);
   (void)hipFree(h->d_phys);
@@ -1793,6 +1833,12 @@ int fs_time_kernel(fs_handle* h, int32_t which, int32_t n, double* avg_ms) {
       launch_mg_down(mg_down_args(h, 0, 2 + (k & 1)), h->st);
     } else if (which == 14) {
       launch_mg_up(mg_up_args(h, 0, 2 + (k & 1)), h->st);
+    } else if (which == 18) {
+      launch_mg_coarse_cycle(cycle_args(h, 2 + (k & 1)), h->st);
+    } else if (which >= 20 && which < 40) {
+      launch_mg_down(mg_down_args(h, which - 20, 2 + (k & 1)), h->st);
+    } else if (which >= 40 && which < 60) {
+      launch_mg_up(mg_up_args(h, which - 40, 2 + (k & 1)), h->st);
     } else {
       PcgArgs ia = pcg_args(h, 0);
       ia.part_out = h->part(0);
@@ -1800,11 +1846,20 @@ int fs_time_kernel(fs_handle* h, int32_t which, int32_t n, double* avg_ms) {
       launch_pcg_init(ia, h->st);
     }
   };
-  if (!(which >= 1 && which <= 5) && !(which >= 10 && which <= 17))
+  const bool level_id = (which >= 20 && which < 60);
+  if (!(which >= 1 && which <= 5) && !(which >= 10 && which <= 18) && !level_id)
     return fail(FS_E_ARG, "unknown kernel id%s");
+  if (level_id && h->mg.size() > 1 && (size_t)(which % 20) >= h->mg_first)
+    return fail(FS_E_ARG, "no tile kernel on that multigrid level%s");
   if (which >= 13 && h->mg.size() < 2)
     return fail(FS_E_ARG, "grid too small for a multigrid hierarchy%s");
   if (which == 15 && !h->net) return fail(FS_E_ARG, "no network attached%s");
+  if (which == 11) {  // consistent partials for the Jacobi-PCG iterations
+    PcgArgs ia = pcg_args(h, 0);
+    ia.part_out = h->part(0);
+    ia.w_new = h->pois(PO_W0);
+    launch_pcg_init(ia, h->st);
+  }
   launch(0);  // warm
   HIP_OK(hipEventRecord(h->ev[6], h->st));
   for (int k = 0; k < n; k++) launch(k);
@@ -1898,12 +1953,7 @@ int fs_ensemble_create(int32_t n, const fs_grid_desc* g, const fs_params* p, fs_
     E->own_h.push_back(h->h_sc);
     h->d_sc = E->d_sc + i;
     h->h_sc = E->h_sc + i;
-    for (int k = 0; k < 2; k++) {
-      if (h->mg_graph[k]) (void)hipGraphExecDestroy(h->mg_graph[k]);
-      h->mg_graph[k] = nullptr;
-      if (h->learn_graph[k]) (void)hipGraphExecDestroy(h->learn_graph[k]);
-      h->learn_graph[k] = nullptr;
-    }
+    drop_graphs(h);
     if (build_arg_tables(h)) {
       ens_release(E);
       return FS_E_HIP;
@@ -2017,7 +2067,7 @@ int fs_learn_harvest(fs_learn* n, fs_handle* h, int32_t* added) {
   HIP_OK(hipStreamSynchronize(n->net.st));
   launch_assembly(h);
   for (size_t l = 0; l + 1 < h->mg.size(); l++) launch_mg_coarsen(mg_args(h, l), h->st);
-  launch_mg_factor(mg_args(h, h->mg.size() - 1), h->d_mginv, h->st);
+  launch_mg_coarse_setup(cycle_args(h, -1), h->st);
   MgPcgArgs ia = mgpcg_args(h, 1);
   ia.part_out = h->part(0);
   launch_mgpcg_init(ia, h->st);

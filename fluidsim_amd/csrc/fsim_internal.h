@@ -170,12 +170,18 @@ constexpr int kMgLdsCells = 4096;  // cells of the finest of those (the others <
 constexpr int kMgLdsC0 = kMgLdsCells / 1024;  // its cells per thread
 constexpr int kMgLdsX = 6144;      // LDS doubles for their padded iterates
 constexpr int kMgLdsR = 4608;      // LDS doubles for the padded residual of the finest
+constexpr int kMgLdsPairs = kMgLdsCells / 2048;  // horizontal cell pairs per thread on it
+// packed FP32 rows: 5 values per cell, 2 kMgLdsPairs cells of the finest level
+// and one of each other non-coarsest level per thread, 1024 threads
+constexpr int kMgPackSlots = 2 * kMgLdsPairs + kMgLdsLevels - 2;
+constexpr size_t kMgPackFloats = (size_t)kMgPackSlots * 5 * 1024;
 
 struct MgCycleArgs {
   MgLevel lv[kMgLdsLevels];  // levels f .. L-1, the last one the coarsest
   int32_t nq;                // how many of them
   int32_t off[kMgLdsLevels]; // LDS offset of each level's (nx+2)(ny+2) iterate
-  const double* inv;         // dense inverse of the coarsest operator
+  double* inv;               // dense inverse of the coarsest operator (set-up kernel)
+  float* pk;                 // packed FP32 operator rows of the other levels (set-up kernel)
   DevScalars* sc;
   double oc;
   const double* part_in;     // check: |r/V|^2 partials of the level-0 CG update
@@ -234,8 +240,8 @@ void launch_ens_scalars(DevScalars* sc, const uint8_t* mask, int n, int op, hipS
 void launch_ens_lapl(const LaplArgs* t, const FsPhys* K, const EnsMap& m, const EnsSel& s,
                      hipStream_t st);
 void launch_ens_mg_coarsen(const MgArgs* t, const EnsMap& m, const EnsSel& s, hipStream_t st);
-void launch_ens_mg_factor(const MgArgs* t, double* const* inv, const EnsMap& m, const EnsSel& s,
-                          hipStream_t st);
+void launch_ens_mg_coarse_setup(const MgCycleArgs* t, const EnsMap& m, const EnsSel& s,
+                                hipStream_t st);
 void launch_ens_mg_down(const MgArgs* t, const EnsMap& m, const EnsSel& s, hipStream_t st);
 // cg: the level-0 up kernel of the multigrid-PCG (wider halo, w = M u)
 void launch_ens_mg_up(const MgArgs* t, const EnsMap& m, const EnsSel& s, bool cg, hipStream_t st);
@@ -262,7 +268,7 @@ void launch_mg_coarsen(const MgArgs& a, hipStream_t s);
 int mg_grid(const MgLevel& F);  // blocks (= partials) of a down/up launch
 void launch_mg_down(const MgArgs& a, hipStream_t s);
 void launch_mg_up(const MgArgs& a, hipStream_t s);
-void launch_mg_factor(const MgArgs& a, double* inv, hipStream_t s);
+void launch_mg_coarse_setup(const MgCycleArgs& a, hipStream_t s);
 void launch_mg_coarse_cycle(const MgCycleArgs& a, hipStream_t s);
 void launch_mgpcg_init(const MgPcgArgs& a, hipStream_t s);
 void launch_mgpcg_pq(const MgPcgArgs& a, hipStream_t s);

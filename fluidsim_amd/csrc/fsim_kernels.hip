@@ -1448,53 +1448,96 @@ __device__ __forceinline__ void mg_up_body(const MgArgs& A, int bid, int nb) {
 template <int H>
 __global__ void __launch_bounds__(kNT) k_mg_up(MgArgs A) { mg_up_body<0, H>(A, blockIdx.x, gridDim.x); }
 
-// Coarsest level (<= kMgBottomMax cells): its operator is inverted once per
-// solve by Gauss-Jordan elimination (SPD, so no pivoting).  Thread (row, col)
-// keeps entry (row, col) in a register; each pivot step publishes the pivot
-// row and column through LDS buffers alternated by step parity, so one barrier
-// per step suffices.  The inverse is stored column-major.
+// Once per solve, for the one-workgroup levels f .. L-1 (1024 threads):
+//  * the coarsest level (<= kMgBottomMax cells) is inverted by Gauss-Jordan
+//    elimination (SPD, so no pivoting): thread (row, col) < kB^2 keeps entry
+//    (row, col) in a register; each pivot step publishes the pivot row and
+//    column through LDS buffers alternated by step parity, one barrier per
+//    step; the inverse is stored column-major;
+//  * the operator rows of the other levels are packed as FP32 (inverse
+//    diagonal, left, right, lower, upper coupling) in the order the cycle
+//    kernel's threads hold them (slot-major, 1024 threads per slot), so that
+//    the cycle loads half the bytes, coalesced.
 template <int kInstance>
-__device__ __forceinline__ void mg_factor_body(const MgArgs& A, double* inv) {
+__device__ __forceinline__ void mg_coarse_setup_body(const MgCycleArgs& A) {
   constexpr int kB = kMgBottomMax;
   __shared__ double s_prow[2][kB], s_pcol[2][kB];
-  const MgLevel& L = A.F;
-  const int n = L.nx * L.ny, nx = L.nx, t = threadIdx.x;
-  const int row = t % kB, col = t / kB;
-  double a = 0.0;
-  if (row < n && col < n) {
-    const size_t m = mg_at(L, row % nx, row / nx);
-    if (col == row) a = L.d[m];
-    else if (col == row - 1 && row % nx > 0) a = -L.cx[m - 1];
-    else if (col == row + 1 && row % nx + 1 < nx) a = -L.cx[m];
-    else if (col == row - nx) a = -L.cy[m - L.P];
-    else if (col == row + nx) a = -L.cy[m];
+  const int t = threadIdx.x;
+  {
+    const MgLevel& L = A.lv[A.nq - 1];
+    const int n = L.nx * L.ny, nx = L.nx;
+    const bool own = t < kB * kB;
+    const int row = t % kB, col = t / kB;
+    double a = 0.0;
+    if (own && row < n && col < n) {
+      const size_t m = mg_at(L, row % nx, row / nx);
+      if (col == row) a = L.d[m];
+      else if (col == row - 1 && row % nx > 0) a = -L.cx[m - 1];
+      else if (col == row + 1 && row % nx + 1 < nx) a = -L.cx[m];
+      else if (col == row - nx) a = -L.cy[m - L.P];
+      else if (col == row + nx) a = -L.cy[m];
+    }
+    for (int p = 0; p < n; p++) {
+      const int buf = p & 1;
+      if (own && row == p) s_prow[buf][col] = a;
+      if (own && col == p) s_pcol[buf][row] = a;
+      __syncthreads();
+      if (own) {
+        const double piv = s_prow[buf][p], f = s_pcol[buf][row];
+        const double pk = (col == p ? 1.0 : s_prow[buf][col]) / piv;
+        a = row == p ? pk : (col == p ? 0.0 : a) - f * pk;
+      }
+    }
+    if (own && row < n && col < n) A.inv[(size_t)col * n + row] = a;
   }
-  for (int p = 0; p < n; p++) {
-    const int buf = p & 1;
-    if (row == p) s_prow[buf][col] = a;
-    if (col == p) s_pcol[buf][row] = a;
-    __syncthreads();
-    const double piv = s_prow[buf][p], f = s_pcol[buf][row];
-    const double pk = (col == p ? 1.0 : s_prow[buf][col]) / piv;
-    a = row == p ? pk : (col == p ? 0.0 : a) - f * pk;
+  if (A.nq < 2) return;
+  auto put = [&](int slot, int k, const MgLevel& L, bool valid) {
+    float* q = A.pk + (size_t)slot * 5 * 1024 + t;
+    const int W = L.nx + 2;
+    q[0] = valid ? (float)(1.0 / L.d[k]) : 1.0f;
+    q[1024] = valid ? (float)L.cx[k - 1] : 0.0f;
+    q[2048] = valid ? (float)L.cx[k] : 0.0f;
+    q[3072] = valid ? (float)L.cy[k - W] : 0.0f;
+    q[4096] = valid ? (float)L.cy[k] : 0.0f;
+  };
+  {
+    const MgLevel& L = A.lv[0];
+    const int npr = (L.nx + 1) / 2, W = L.nx + 2;
+    for (int p = 0; p < kMgLdsPairs; p++)
+      for (int c = 0; c < 2; c++) {
+        const int pp = t + 1024 * p;
+        int k = 0;
+        bool valid = false;
+        if (pp < npr * L.ny) {
+          const int j = pp / npr, i = 2 * (pp % npr) + (c ^ (j & 1));
+          valid = i < L.nx;
+          k = (j + 1) * W + i + 1;
+        }
+        put(p * 2 + c, k, L, valid);
+      }
   }
-  if (row < n && col < n) inv[(size_t)col * n + row] = a;
+  for (int q = 1; q + 1 < A.nq; q++) {
+    const MgLevel& L = A.lv[q];
+    const bool valid = t < L.nx * L.ny;
+    const int k = valid ? (t / L.nx + 1) * (L.nx + 2) + t % L.nx + 1 : 0;
+    put(2 * kMgLdsPairs + q - 1, k, L, valid);
+  }
 }
 
-__global__ void __launch_bounds__(kMgBottomMax * kMgBottomMax) k_mg_factor(MgArgs A, double* inv) {
-  mg_factor_body<0>(A, inv);
+__global__ void __launch_bounds__(1024) k_mg_coarse_setup(MgCycleArgs A) {
+  mg_coarse_setup_body<0>(A);
 }
 
 // Levels f .. L-1 in one workgroup of 1024 threads (at most kMgLdsLevels;
 // level f has at most kMgLdsCells cells, the others at most 1024).  Iterates
 // and residuals live in LDS in the levels' padded (nx+2)(ny+2) layout, which
-// is also their layout in HBM; operator rows and right-hand sides stay in
-// registers.  Level f is held as horizontal cell pairs (thread t owns pairs
+// is also their layout in HBM; operator rows (the set-up kernel's FP32 table)
+// and right-hand sides stay in registers.  Level f is held as horizontal cell
+// pairs (thread t owns pairs
 // t + 1024 p, p < kMgLdsPairs), one cell of each colour per pair, so each
 // half-sweep keeps every lane busy; the coarser levels hold one cell per
 // thread.  The sweeps multiply by the inverse diagonal.  Same cycle as the
 // tile kernels; level f's correction is written back for the levels above.
-constexpr int kMgLdsPairs = kMgLdsCells / 2048;
 struct PairRows {
   double dinv[2][kMgLdsPairs], l[2][kMgLdsPairs], r[2][kMgLdsPairs], b[2][kMgLdsPairs],
       t[2][kMgLdsPairs], rhs[2][kMgLdsPairs];
@@ -1511,26 +1554,27 @@ struct CellRows {
   int ij;  // i | j << 16
 };
 
-__device__ __forceinline__ void lds_pair_rows(const MgLevel& L, PairRows& R, int t) {
+// Rows of level f (pairs) and of a coarser level q >= 1 (one cell) from the
+// packed FP32 table; right-hand sides of level f from its residual plane.
+__device__ __forceinline__ void lds_pair_rows(const MgLevel& L, PairRows& R, const float* pk, int t) {
   const int npr = (L.nx + 1) / 2, W = L.nx + 2;
 #pragma unroll
   for (int p = 0; p < kMgLdsPairs; p++) {
     const int pp = t + 1024 * p;
 #pragma unroll
     for (int c = 0; c < 2; c++) {
-      R.dinv[c][p] = 1.0;
-      R.l[c][p] = R.r[c][p] = R.b[c][p] = R.t[c][p] = R.rhs[c][p] = 0.0;
+      const float* q = pk + (size_t)(p * 2 + c) * 5 * 1024 + t;
+      R.dinv[c][p] = q[0];
+      R.l[c][p] = q[1024];
+      R.r[c][p] = q[2048];
+      R.b[c][p] = q[3072];
+      R.t[c][p] = q[4096];
+      R.rhs[c][p] = 0.0;
       R.ij[c][p] = -1;
       if (pp < npr * L.ny) {
         const int j = pp / npr, i = 2 * (pp % npr) + (c ^ (j & 1));
         if (i < L.nx) {
-          const int k = (j + 1) * W + i + 1;
-          R.dinv[c][p] = 1.0 / L.d[k];
-          R.l[c][p] = L.cx[k - 1];
-          R.r[c][p] = L.cx[k];
-          R.b[c][p] = L.cy[k - W];
-          R.t[c][p] = L.cy[k];
-          R.rhs[c][p] = L.r[k];
+          R.rhs[c][p] = L.r[(j + 1) * W + i + 1];
           R.ij[c][p] = i | (j << 16);
         }
       }
@@ -1538,20 +1582,33 @@ __device__ __forceinline__ void lds_pair_rows(const MgLevel& L, PairRows& R, int
   }
 }
 
-__device__ __forceinline__ void lds_cell_rows(const MgLevel& L, CellRows& R, int t) {
-  const int W = L.nx + 2;
+__device__ __forceinline__ void lds_cell_rows(const MgLevel& L, CellRows& R, const float* pk,
+                                              int slot, int t) {
+  const float* q = pk + (size_t)slot * 5 * 1024 + t;
+  R.dinv = q[0];
+  R.l = q[1024];
+  R.r = q[2048];
+  R.b = q[3072];
+  R.t = q[4096];
+  R.rhs = 0.0;
+  R.k = -1;
+  R.ij = 0;
+  if (t < L.nx * L.ny) {
+    const int i = t % L.nx, j = t / L.nx;
+    R.k = (j + 1) * (L.nx + 2) + i + 1;
+    R.ij = i | (j << 16);
+  }
+}
+
+// Position of cell t on the coarsest level (its rows are not needed).
+__device__ __forceinline__ void lds_cell_pos(const MgLevel& L, CellRows& R, int t) {
   R.dinv = 1.0;
   R.l = R.r = R.b = R.t = R.rhs = 0.0;
   R.k = -1;
   R.ij = 0;
   if (t < L.nx * L.ny) {
-    const int i = t % L.nx, j = t / L.nx, k = (j + 1) * W + i + 1;
-    R.dinv = 1.0 / L.d[k];
-    R.l = L.cx[k - 1];
-    R.r = L.cx[k];
-    R.b = L.cy[k - W];
-    R.t = L.cy[k];
-    R.k = k;
+    const int i = t % L.nx, j = t / L.nx;
+    R.k = (j + 1) * (L.nx + 2) + i + 1;
     R.ij = i | (j << 16);
   }
 }
@@ -1667,6 +1724,18 @@ __device__ __forceinline__ void mg_coarse_cycle_body(const MgCycleArgs& A) {
   __shared__ double s_w[16];
   if (A.sc->done) return;
   const int t = threadIdx.x;
+  const int nq = A.nq;
+  // operator rows first: their loads overlap the convergence test
+  PairRows R0;                    // level f
+  CellRows R1[kMgLdsLevels - 2];  // levels f+1 .. L-2
+  CellRows Bc;                    // the coarsest level's cell t (its position only)
+  if (nq > 1) {
+    lds_pair_rows(A.lv[0], R0, A.pk, t);
+#pragma unroll
+    for (int q = 1; q < kMgLdsLevels - 1; q++)
+      if (q < nq - 1) lds_cell_rows(A.lv[q], R1[q - 1], A.pk, 2 * kMgLdsPairs + q - 1, t);
+  }
+  lds_cell_pos(A.lv[nq - 1], Bc, t);
   if (A.check) {  // convergence test on the level-0 CG update's |r/V|^2 partials
     double v = 0.0;
     for (int b = t; b < A.nb_in; b += 1024) v += A.part_in[3 * b];
@@ -1678,7 +1747,6 @@ __device__ __forceinline__ void mg_coarse_cycle_body(const MgCycleArgs& A) {
     for (int w = 0; w < 16; w++) rho += s_w[w];
     if (mg_check(A.sc, rho, 0, A.it, t == 0)) return;
   }
-  const int nq = A.nq;
   const MgLevel& B = A.lv[nq - 1];  // coarsest
   const int nb = B.nx * B.ny;
   // the coarsest level's inverse: entry (brow, bcol) in thread brow + kB * bcol
@@ -1690,14 +1758,6 @@ __device__ __forceinline__ void mg_coarse_cycle_body(const MgCycleArgs& A) {
   if (nq == 1) {
     if (t < nb) rb = B.r[(t / B.nx + 1) * (B.nx + 2) + t % B.nx + 1];
   } else {
-    PairRows R0;                        // level f
-    CellRows R1[kMgLdsLevels - 2];      // levels f+1 .. L-2
-    lds_pair_rows(A.lv[0], R0, t);
-#pragma unroll
-    for (int q = 1; q < kMgLdsLevels - 1; q++)
-      if (q < nq - 1) lds_cell_rows(A.lv[q], R1[q - 1], t);
-    CellRows Bc;  // the coarsest level's cell t (its position only)
-    lds_cell_rows(B, Bc, t);
     __syncthreads();
     // down: level f, then f+1 .. L-2, each restricted to the next
     pair_down(R0, s_x + A.off[0], s_r, A.lv[0].nx + 2);
@@ -2013,10 +2073,9 @@ __global__ void __launch_bounds__(256) kb_mg_coarsen(const MgArgs* t, EnsMap m, 
   int i, bid, nb;
   if (ens_slot(m, i, bid, nb)) mg_coarsen_body<1>(ens_arg(t, s, i), bid, nb);
 }
-__global__ void __launch_bounds__(kMgBottomMax * kMgBottomMax) kb_mg_factor(const MgArgs* t, double* const* inv,
-                                                     EnsMap m, EnsSel s) {
+__global__ void __launch_bounds__(1024) kb_mg_coarse_setup(const MgCycleArgs* t, EnsMap m, EnsSel s) {
   int i, bid, nb;
-  if (ens_slot(m, i, bid, nb)) mg_factor_body<1>(ens_arg(t, s, i), inv[i]);
+  if (ens_slot(m, i, bid, nb)) mg_coarse_setup_body<1>(ens_arg(t, s, i));
 }
 __global__ void __launch_bounds__(kNT) kb_mg_down(const MgArgs* t, EnsMap m, EnsSel s) {
   int i, bid, nb;
@@ -2099,8 +2158,8 @@ void launch_mg_up(const MgArgs& a, hipStream_t s) {
   if (a.cg.on) hipLaunchKernelGGL(k_mg_up<kGsH + 1>, dim3(mg_grid(a.F)), dim3(kNT), 0, s, a);
   else hipLaunchKernelGGL(k_mg_up<kGsH>, dim3(mg_grid(a.F)), dim3(kNT), 0, s, a);
 }
-void launch_mg_factor(const MgArgs& a, double* inv, hipStream_t s) {
-  hipLaunchKernelGGL(k_mg_factor, dim3(1), dim3(kMgBottomMax * kMgBottomMax), 0, s, a, inv);
+void launch_mg_coarse_setup(const MgCycleArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(k_mg_coarse_setup, dim3(1), dim3(1024), 0, s, a);
 }
 void launch_mg_coarse_cycle(const MgCycleArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(k_mg_coarse_cycle, dim3(1), dim3(1024), 0, s, a);
@@ -2151,11 +2210,9 @@ void launch_ens_lapl(const LaplArgs* t, const FsPhys* K, const EnsMap& m, const 
 void launch_ens_mg_coarsen(const MgArgs* t, const EnsMap& m, const EnsSel& s, hipStream_t st) {
   if (m.nblocks) hipLaunchKernelGGL(kb_mg_coarsen, dim3(m.nblocks), dim3(256), 0, st, t, m, s);
 }
-void launch_ens_mg_factor(const MgArgs* t, double* const* inv, const EnsMap& m, const EnsSel& s,
-                          hipStream_t st) {
-  if (m.nblocks)
-    hipLaunchKernelGGL(kb_mg_factor, dim3(m.nblocks), dim3(kMgBottomMax * kMgBottomMax), 0, st, t,
-                       inv, m, s);
+void launch_ens_mg_coarse_setup(const MgCycleArgs* t, const EnsMap& m, const EnsSel& s,
+                                hipStream_t st) {
+  if (m.nblocks) hipLaunchKernelGGL(kb_mg_coarse_setup, dim3(m.nblocks), dim3(1024), 0, st, t, m, s);
 }
 void launch_ens_mg_down(const MgArgs* t, const EnsMap& m, const EnsSel& s, hipStream_t st) {
   if (m.nblocks) hipLaunchKernelGGL(kb_mg_down, dim3(m.nblocks), dim3(kNT), 0, st, t, m, s);

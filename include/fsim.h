@@ -199,7 +199,9 @@ int fs_set_timing(fs_handle* h, int32_t on);
  * and up kernels of the finest level (with the CG update and w = M u that the
  * multigrid-PCG fuses into them), 15 = the attached network on the learned
  * level, 16 = one learned flexible-CG iteration, 17 = one multigrid-PCG
- * iteration (sequences of kernels, timed as a whole). */
+ * iteration (sequences of kernels, timed as a whole), 18 = the one-workgroup
+ * coarse multigrid kernel, 20 + l / 40 + l = the down / up kernel of tile
+ * level l of the multigrid-PCG iteration. */
 int fs_time_kernel(fs_handle* h, int32_t which, int32_t n, double* avg_ms);
 
 /* Device copy bandwidth: `reps` launches of a plain copy kernel moving
