@@ -1116,14 +1116,16 @@ constexpr int kGsH = 4;                 // halo of the down kernels (four half-s
 static_assert((kGsTX / 2) * kGsTY == kNT, "one thread per horizontal cell pair");
 
 // A level's equation on one tile plus a halo of H cells, staged in LDS:
-// right-hand side, diagonal, right/upper couplings and the iterate.  Cells
-// outside the level carry r = 0, d = 1 and no couplings, so they stay zero.
+// right-hand side, diagonal and its inverse (the sweeps multiply by it),
+// right/upper couplings and the iterate.  Cells outside the level carry r = 0,
+// d = 1 and no couplings, so they stay zero.
 template <int H>
 struct GsTile {
   static constexpr int W = kGsTX + 2 * H, R = kGsTY + 2 * H;
   double x[R][W];
   double r[R][W];
   double d[R][W];
+  double dinv[R][W];
   double cx[R][W];
   double cy[R][W];
 };
@@ -1147,6 +1149,7 @@ __device__ __forceinline__ void gs_stage(const MgLevel& F, GsTile<H>& s, int i0,
     }
     s.r[b][a] = r;
     s.d[b][a] = d;
+    s.dinv[b][a] = 1.0 / d;
     s.cx[b][a] = cx;
     s.cy[b][a] = cy;
   }
@@ -1161,8 +1164,8 @@ __device__ __forceinline__ void gs_half(GsTile<H>& s, int i0, int j0, int c, int
     const int b = a0 + k / w2;
     const int a = a0 + 2 * (k % w2) + ((c + i0 - H + a0 + j0 - H + b) & 1);
     s.x[b][a] = (s.r[b][a] + s.cx[b][a - 1] * s.x[b][a - 1] + s.cx[b][a] * s.x[b][a + 1] +
-                 s.cy[b - 1][a] * s.x[b - 1][a] + s.cy[b][a] * s.x[b + 1][a]) /
-                s.d[b][a];
+                 s.cy[b - 1][a] * s.x[b - 1][a] + s.cy[b][a] * s.x[b + 1][a]) *
+                s.dinv[b][a];
   }
 }
 
@@ -1244,6 +1247,7 @@ __device__ __forceinline__ void cg_stage(const MgArgs& A, GsTile<kGsH>& s, int i
         }
       }
       s.d[b][a] = d;
+      s.dinv[b][a] = 1.0 / d;
       s.cx[b][a] = cx;
       s.cy[b][a] = cy;
     }
@@ -1352,7 +1356,7 @@ __device__ __forceinline__ void mg_down_body(const MgArgs& A, int bid, int nb) {
     // first (red) half-sweep from zero is pointwise: the whole staged halo
     for (int k = threadIdx.x; k < R * W; k += kNT) {
       const int b = k / W, a = k % W;
-      s.x[b][a] = ((i0 + a + j0 + b) & 1) == 0 ? s.r[b][a] / s.d[b][a] : 0.0;
+      s.x[b][a] = ((i0 + a + j0 + b) & 1) == 0 ? s.r[b][a] * s.dinv[b][a] : 0.0;
     }
     __syncthreads();
     gs_half(s, i0, j0, 1, 3);
@@ -1536,10 +1540,12 @@ __global__ void __launch_bounds__(1024) k_mg_coarse_setup(MgCycleArgs A) {
 // pairs (thread t owns pairs
 // t + 1024 p, p < kMgLdsPairs), one cell of each colour per pair, so each
 // half-sweep keeps every lane busy; the coarser levels hold one cell per
-// thread.  The sweeps multiply by the inverse diagonal.  Same cycle as the
-// tile kernels; level f's correction is written back for the levels above.
+// thread.  The sweeps multiply by the inverse diagonal.  These levels compute
+// in FP32 (a coarse correction needs no more; the outer CG and the finer
+// levels stay FP64).  Same cycle as the tile kernels; level f's correction is
+// written back in FP64 for the levels above.
 struct PairRows {
-  double dinv[2][kMgLdsPairs], l[2][kMgLdsPairs], r[2][kMgLdsPairs], b[2][kMgLdsPairs],
+  float dinv[2][kMgLdsPairs], l[2][kMgLdsPairs], r[2][kMgLdsPairs], b[2][kMgLdsPairs],
       t[2][kMgLdsPairs], rhs[2][kMgLdsPairs];
   int ij[2][kMgLdsPairs];  // i | j << 16 of the cell of colour c, or -1
 };
@@ -1549,7 +1555,7 @@ __device__ __forceinline__ int ij_kc(int ij, int Wc) {
   return ((ij >> 17) + 1) * Wc + ((ij & 0xffff) >> 1) + 1;
 }
 struct CellRows {
-  double dinv, l, r, b, t, rhs;
+  float dinv, l, r, b, t, rhs;
   int k;   // padded index of cell t, or -1
   int ij;  // i | j << 16
 };
@@ -1569,12 +1575,12 @@ __device__ __forceinline__ void lds_pair_rows(const MgLevel& L, PairRows& R, con
       R.r[c][p] = q[2048];
       R.b[c][p] = q[3072];
       R.t[c][p] = q[4096];
-      R.rhs[c][p] = 0.0;
+      R.rhs[c][p] = 0.0f;
       R.ij[c][p] = -1;
       if (pp < npr * L.ny) {
         const int j = pp / npr, i = 2 * (pp % npr) + (c ^ (j & 1));
         if (i < L.nx) {
-          R.rhs[c][p] = L.r[(j + 1) * W + i + 1];
+          R.rhs[c][p] = (float)L.r[(j + 1) * W + i + 1];
           R.ij[c][p] = i | (j << 16);
         }
       }
@@ -1590,7 +1596,7 @@ __device__ __forceinline__ void lds_cell_rows(const MgLevel& L, CellRows& R, con
   R.r = q[2048];
   R.b = q[3072];
   R.t = q[4096];
-  R.rhs = 0.0;
+  R.rhs = 0.0f;
   R.k = -1;
   R.ij = 0;
   if (t < L.nx * L.ny) {
@@ -1602,8 +1608,8 @@ __device__ __forceinline__ void lds_cell_rows(const MgLevel& L, CellRows& R, con
 
 // Position of cell t on the coarsest level (its rows are not needed).
 __device__ __forceinline__ void lds_cell_pos(const MgLevel& L, CellRows& R, int t) {
-  R.dinv = 1.0;
-  R.l = R.r = R.b = R.t = R.rhs = 0.0;
+  R.dinv = 1.0f;
+  R.l = R.r = R.b = R.t = R.rhs = 0.0f;
   R.k = -1;
   R.ij = 0;
   if (t < L.nx * L.ny) {
@@ -1613,17 +1619,17 @@ __device__ __forceinline__ void lds_cell_pos(const MgLevel& L, CellRows& R, int 
   }
 }
 
-__device__ __forceinline__ double lds_gs(double rhs, double l, double r, double b, double t,
-                                         double dinv, const double* x, int k, int W) {
+__device__ __forceinline__ float lds_gs(float rhs, float l, float r, float b, float t, float dinv,
+                                        const float* x, int k, int W) {
   return (rhs + l * x[k - 1] + r * x[k + 1] + b * x[k - W] + t * x[k + W]) * dinv;
 }
-__device__ __forceinline__ double lds_res(double rhs, double l, double r, double b, double t,
-                                          double dinv, const double* x, int k, int W) {
+__device__ __forceinline__ float lds_res(float rhs, float l, float r, float b, float t, float dinv,
+                                         const float* x, int k, int W) {
   return rhs - (x[k] / dinv - l * x[k - 1] - r * x[k + 1] - b * x[k - W] - t * x[k + W]);
 }
 
 template <int COL>
-__device__ __forceinline__ void pair_half(const PairRows& R, double* x, int W) {
+__device__ __forceinline__ void pair_half(const PairRows& R, float* x, int W) {
 #pragma unroll
   for (int p = 0; p < kMgLdsPairs; p++) {
     const int k = ij_k(R.ij[COL][p], W);
@@ -1635,7 +1641,7 @@ __device__ __forceinline__ void pair_half(const PairRows& R, double* x, int W) {
 }
 
 template <int COL>
-__device__ __forceinline__ void cell_half(const CellRows& R, double* x, int W) {
+__device__ __forceinline__ void cell_half(const CellRows& R, float* x, int W) {
   if (R.k >= 0 && (((R.ij & 0xffff) + (R.ij >> 16)) & 1) == COL)
     x[R.k] = lds_gs(R.rhs, R.l, R.r, R.b, R.t, R.dinv, x, R.k, W);
   __syncthreads();
@@ -1643,7 +1649,7 @@ __device__ __forceinline__ void cell_half(const CellRows& R, double* x, int W) {
 
 // Pre-smoothing from zero (red pointwise, then black, red, black) and the
 // residual into s_r.
-__device__ __forceinline__ void pair_down(const PairRows& R, double* x, double* s_r, int W) {
+__device__ __forceinline__ void pair_down(const PairRows& R, float* x, float* s_r, int W) {
 #pragma unroll
   for (int p = 0; p < kMgLdsPairs; p++)
     if (R.ij[0][p] >= 0) x[ij_k(R.ij[0][p], W)] = R.rhs[0][p] * R.dinv[0][p];
@@ -1663,7 +1669,7 @@ __device__ __forceinline__ void pair_down(const PairRows& R, double* x, double* 
   __syncthreads();
 }
 
-__device__ __forceinline__ void cell_down(const CellRows& R, double* x, double* s_r, int W) {
+__device__ __forceinline__ void cell_down(const CellRows& R, float* x, float* s_r, int W) {
   const bool red = R.k >= 0 && (((R.ij & 0xffff) + (R.ij >> 16)) & 1) == 0;
   if (red) x[R.k] = R.rhs * R.dinv;
   __syncthreads();
@@ -1675,9 +1681,9 @@ __device__ __forceinline__ void cell_down(const CellRows& R, double* x, double* 
 }
 
 // Restriction of level L's residual (s_r) to the coarse cell held in C.
-__device__ __forceinline__ double lds_restrict(const MgLevel& L, const CellRows& C,
-                                               const double* s_r) {
-  double v = 0.0;
+__device__ __forceinline__ float lds_restrict(const MgLevel& L, const CellRows& C,
+                                              const float* s_r) {
+  float v = 0.0f;
   if (C.k >= 0) {
     const int I = C.ij & 0xffff, J = C.ij >> 16, W = L.nx + 2;
     for (int b = 0; b < 2; b++)
@@ -1688,8 +1694,8 @@ __device__ __forceinline__ double lds_restrict(const MgLevel& L, const CellRows&
 }
 
 // Prolongation with over-correction and post-smoothing (black, red, black, red).
-__device__ __forceinline__ void pair_up(const PairRows& R, double* x, const double* xc, int W,
-                                        int Wc, double oc) {
+__device__ __forceinline__ void pair_up(const PairRows& R, float* x, const float* xc, int W,
+                                        int Wc, float oc) {
 #pragma unroll
   for (int c = 0; c < 2; c++)
 #pragma unroll
@@ -1702,8 +1708,8 @@ __device__ __forceinline__ void pair_up(const PairRows& R, double* x, const doub
   pair_half<0>(R, x, W);
 }
 
-__device__ __forceinline__ void cell_up(const CellRows& R, double* x, const double* xc, int W,
-                                        int Wc, double oc) {
+__device__ __forceinline__ void cell_up(const CellRows& R, float* x, const float* xc, int W,
+                                        int Wc, float oc) {
   if (R.k >= 0) {
     const int i = R.ij & 0xffff, j = R.ij >> 16;
     x[R.k] += oc * xc[((j >> 1) + 1) * Wc + (i >> 1) + 1];
@@ -1717,10 +1723,10 @@ __device__ __forceinline__ void cell_up(const CellRows& R, double* x, const doub
 
 template <int kInstance>
 __device__ __forceinline__ void mg_coarse_cycle_body(const MgCycleArgs& A) {
-  __shared__ double s_x[kMgLdsX];
-  __shared__ double s_r[kMgLdsR];
+  __shared__ float s_x[kMgLdsX];
+  __shared__ float s_r[kMgLdsR];
   constexpr int kB = kMgBottomMax;
-  __shared__ double s_bp[kB][kB];
+  __shared__ float s_bp[kB][kB];
   __shared__ double s_w[16];
   if (A.sc->done) return;
   const int t = threadIdx.x;
@@ -1751,18 +1757,18 @@ __device__ __forceinline__ void mg_coarse_cycle_body(const MgCycleArgs& A) {
   const int nb = B.nx * B.ny;
   // the coarsest level's inverse: entry (brow, bcol) in thread brow + kB * bcol
   const int brow = t % kB, bcol = t / kB;
-  const double binv =
-      (t < kB * kB && brow < nb && bcol < nb) ? A.inv[(size_t)bcol * nb + brow] : 0.0;
-  for (int k = t; k < kMgLdsX; k += 1024) s_x[k] = 0.0;
-  double rb = 0.0;  // right-hand side of the coarsest level at cell t
+  const float binv =
+      (t < kB * kB && brow < nb && bcol < nb) ? (float)A.inv[(size_t)bcol * nb + brow] : 0.0f;
+  for (int k = t; k < kMgLdsX; k += 1024) s_x[k] = 0.0f;
+  float rb = 0.0f;  // right-hand side of the coarsest level at cell t
   if (nq == 1) {
-    if (t < nb) rb = B.r[(t / B.nx + 1) * (B.nx + 2) + t % B.nx + 1];
+    if (t < nb) rb = (float)B.r[(t / B.nx + 1) * (B.nx + 2) + t % B.nx + 1];
   } else {
     __syncthreads();
     // down: level f, then f+1 .. L-2, each restricted to the next
     pair_down(R0, s_x + A.off[0], s_r, A.lv[0].nx + 2);
     {
-      const double v = lds_restrict(A.lv[0], nq == 2 ? Bc : R1[0], s_r);
+      const float v = lds_restrict(A.lv[0], nq == 2 ? Bc : R1[0], s_r);
       if (nq == 2) rb = v;
       else R1[0].rhs = v;
     }
@@ -1771,7 +1777,7 @@ __device__ __forceinline__ void mg_coarse_cycle_body(const MgCycleArgs& A) {
     for (int q = 1; q < kMgLdsLevels - 1; q++) {
       if (q + 1 >= nq) break;
       cell_down(R1[q - 1], s_x + A.off[q], s_r, A.lv[q].nx + 2);
-      const double w = lds_restrict(A.lv[q], q + 2 == nq ? Bc : R1[q], s_r);
+      const float w = lds_restrict(A.lv[q], q + 2 == nq ? Bc : R1[q], s_r);
       if (q + 2 == nq) rb = w;
       else R1[q].rhs = w;
       __syncthreads();
@@ -1779,10 +1785,10 @@ __device__ __forceinline__ void mg_coarse_cycle_body(const MgCycleArgs& A) {
     // coarsest level: x = inv r (products in LDS, summed per row)
     if (t < nb) s_r[t] = rb;
     __syncthreads();
-    if (t < kB * kB) s_bp[bcol][brow] = bcol < nb ? binv * s_r[bcol] : 0.0;
+    if (t < kB * kB) s_bp[bcol][brow] = bcol < nb ? binv * s_r[bcol] : 0.0f;
     __syncthreads();
     if (t < nb) {
-      double v = 0.0;
+      float v = 0.0f;
       for (int c = 0; c < nb; c++) v += s_bp[c][t];
       s_x[A.off[nq - 1] + Bc.k] = v;
     }
@@ -1792,29 +1798,29 @@ __device__ __forceinline__ void mg_coarse_cycle_body(const MgCycleArgs& A) {
     for (int q = kMgLdsLevels - 2; q >= 1; q--) {
       if (q + 1 >= nq) continue;
       cell_up(R1[q - 1], s_x + A.off[q], s_x + A.off[q + 1], A.lv[q].nx + 2, A.lv[q + 1].nx + 2,
-              A.oc);
+              (float)A.oc);
     }
     const MgLevel& L = A.lv[0];
-    pair_up(R0, s_x + A.off[0], s_x + A.off[1], L.nx + 2, A.lv[1].nx + 2, A.oc);
+    pair_up(R0, s_x + A.off[0], s_x + A.off[1], L.nx + 2, A.lv[1].nx + 2, (float)A.oc);
 #pragma unroll
     for (int c = 0; c < 2; c++)
 #pragma unroll
       for (int p = 0; p < kMgLdsPairs; p++)
         if (R0.ij[c][p] >= 0) {
           const int k = ij_k(R0.ij[c][p], L.nx + 2);
-          L.e[k] = s_x[A.off[0] + k];
+          L.e[k] = (double)s_x[A.off[0] + k];
         }
     return;
   }
   // a single level: the coarsest, solved directly
   if (t < nb) s_r[t] = rb;
   __syncthreads();
-  if (t < kB * kB) s_bp[bcol][brow] = bcol < nb ? binv * s_r[bcol] : 0.0;
+  if (t < kB * kB) s_bp[bcol][brow] = bcol < nb ? binv * s_r[bcol] : 0.0f;
   __syncthreads();
   if (t < nb) {
-    double v = 0.0;
+    float v = 0.0f;
     for (int c = 0; c < nb; c++) v += s_bp[c][t];
-    B.e[(t / B.nx + 1) * (B.nx + 2) + t % B.nx + 1] = v;
+    B.e[(t / B.nx + 1) * (B.nx + 2) + t % B.nx + 1] = (double)v;
   }
 }
 __global__ void __launch_bounds__(1024) k_mg_coarse_cycle(MgCycleArgs A) { mg_coarse_cycle_body<0>(A); }
