@@ -163,6 +163,7 @@ struct fs_handle {
   LaplArgs* d_ltab = nullptr;
   FsPhys* d_phys = nullptr;
   hipGraphExec_t mg_pro = nullptr;    // MG-PCG set-up and iteration 0
+  int mg_it = 0, mg_chunk = 0;        // next MG-PCG iteration / chunk of a solve in progress
   hipGraphExec_t mg_chain[2][kChainLog] = {};  // 2^m MG-PCG iterations from parity p
   hipGraphExec_t learn_graph[2] = {}; // one learned-FCG iteration, by parity
   const void* learn_graph_key = nullptr; // the network (or the handle: none) it was made for
@@ -726,7 +727,7 @@ LaplArgs lapl_args(fs_handle* h) {
 // the scalars move, see the ensemble).
 int build_arg_tables(fs_handle* h) {
   std::vector<StageArgs> st(6 * 15);
-  std::vector<LaplArgs> la(6);
+  std::vector<LaplArgs> la(12);  // [gate * 6 + rot * 2 + alias]
   const int rot0 = h->rot;
   const bool alias0 = h->alias;
   for (int rot = 0; rot < 3; rot++)
@@ -741,6 +742,8 @@ int build_arg_tables(fs_handle* h) {
           st[((rot * 2 + al) * 3 + v) * 5 + stg - 1] = a;
         }
       la[rot * 2 + al] = lapl_args(h);
+      la[6 + rot * 2 + al] = la[rot * 2 + al];
+      la[6 + rot * 2 + al].gate = 1;
     }
   h->rot = rot0;
   h->alias = alias0;
@@ -756,9 +759,13 @@ int build_arg_tables(fs_handle* h) {
 const StageArgs* stab(const fs_handle* h, int stage, int variant) {
   return h->d_stab + ((h->rot * 2 + (h->alias ? 1 : 0)) * 3 + variant) * 5 + stage - 1;
 }
-const LaplArgs* ltab(const fs_handle* h) { return h->d_ltab + h->rot * 2 + (h->alias ? 1 : 0); }
+const LaplArgs* ltab(const fs_handle* h, bool gated = false) {
+  return h->d_ltab + (gated ? 6 : 0) + h->rot * 2 + (h->alias ? 1 : 0);
+}
 
-void launch_assembly(fs_handle* h) { launch_lapl_assemble(ltab(h), h->d_phys, h->nbx * h->nby, h->st); }
+void launch_assembly(fs_handle* h, bool gated = false) {
+  launch_lapl_assemble(ltab(h, gated), h->d_phys, h->nbx * h->nby, h->st);
+}
 
 // A pass's record of a ptcl_step attempt (the Poisson fields follow later).
 void log_attempt(fs_handle* h) {
@@ -785,12 +792,23 @@ void log_solve(fs_handle* h, int lerr) {
 // its kernels' launch latency).  Without a previous solve: `dflt`.
 int first_chunk(int last, int dflt) { return last > 0 ? last : dflt; }
 
-int lapl_solve_impl(fs_handle* h, int* lerr, fs_solve_stats* stats) {
+// Multigrid-PCG with graphs can be issued ahead of the host: after the
+// controller's kernel (mode 1) the assembly, the set-up and the first chunk
+// with its finishing kernels are enqueued, the assembly skipping the whole
+// solve on the device unless the attempt is accepted; after the host has read
+// the scalars, mode 2 continues that solve.  Mode 0: a whole solve.
+bool can_issue_ahead(const fs_handle* h) {
+  return h->p.use_graphs && h->p.lapl_solver == 1 && h->mg.size() > 1;
+}
+
+int lapl_solve_impl(fs_handle* h, int* lerr, fs_solve_stats* stats, int mode = 0) {
   auto t0 = std::chrono::steady_clock::now();
-  time_mark(h, 4);
-  launch_assembly(h);
-  HIP_OK(hipGetLastError());
-  time_mark(h, 5);
+  if (mode != 2) {
+    time_mark(h, 4);
+    launch_assembly(h, mode == 1);
+    HIP_OK(hipGetLastError());
+    time_mark(h, 5);
+  }
   const int chunk = h->p.lapl_chunk > 0 ? h->p.lapl_chunk : 32;
   int it = 0;
   // After each chunk of iterations the finishing kernels run, then the host
@@ -800,11 +818,11 @@ int lapl_solve_impl(fs_handle* h, int* lerr, fs_solve_stats* stats) {
   PcgArgs pa = pcg_args(h, 0);
   pa.part_out = h->part(2);
   pa.part_in = h->part(2);
-  auto end_chunk = [&]() -> int {
+  auto end_chunk = [&](bool read = true) -> int {
     HIP_OK(hipGetLastError());
     launch_pcg_finish(pa, h->st);
     time_mark(h, 6);
-    return sync_scalars(h) ? FS_E_HIP : 0;
+    return read && sync_scalars(h) ? FS_E_HIP : 0;
   };
   if (h->p.lapl_solver == 2) {
     // flexible CG with the learned coarse solver (eager launches); without an
@@ -837,23 +855,33 @@ int lapl_solve_impl(fs_handle* h, int* lerr, fs_solve_stats* stats) {
     }
   } else if (h->p.lapl_solver == 1 && h->mg.size() > 1) {
     // multigrid-preconditioned PCG (grids of <= 64 cells use Jacobi-PCG)
-    if (!h->p.use_graphs) {
-      if (mg_prologue(h)) return FS_E_HIP;
-    } else {
-      if (!h->mg_pro && capture(h, &h->mg_pro, [&]() { return mg_prologue(h); }))
-        return FS_E_HIP;
-      HIP_OK(hipGraphLaunch(h->mg_pro, h->st));
+    int& mg_it = h->mg_it;
+    int& mg_chunk = h->mg_chunk;
+    if (mode != 2) {
+      if (!h->p.use_graphs) {
+        if (mg_prologue(h)) return FS_E_HIP;
+      } else {
+        if (!h->mg_pro && capture(h, &h->mg_pro, [&]() { return mg_prologue(h); }))
+          return FS_E_HIP;
+        HIP_OK(hipGraphLaunch(h->mg_pro, h->st));
+      }
+      // iteration 0 is launched already; then as many as the last solve took,
+      // then 2 more, then chunk / 4 at a time
+      mg_it = 1;
+      mg_chunk = std::max(1, first_chunk(h->last_iters[1], chunk > 8 ? chunk / 4 : chunk) - 1);
+      if (launch_mg_chunk(h, mg_it, mg_chunk)) return FS_E_HIP;
+      mg_it += mg_chunk;
+      mg_chunk = 2;
+      if (int e = end_chunk(mode == 0)) return e;
+      if (mode == 1) return 0;
     }
-    it = 1;
-    // iteration 0 is launched already; then 2 more, then chunk / 4 at a time
-    int mg_chunk = std::max(1, first_chunk(h->last_iters[1], chunk > 8 ? chunk / 4 : chunk) - 1);
-    for (int round = 0;; round++) {
-      if (launch_mg_chunk(h, it, mg_chunk)) return FS_E_HIP;
-      it += mg_chunk;
-      mg_chunk = round == 0 ? 2 : chunk > 8 ? chunk / 4 : chunk;
-      if (int e = end_chunk()) return e;
+    for (;;) {
       if (h->h_sc->status < 0) return check_status(h);
       if (h->h_sc->done) break;
+      if (launch_mg_chunk(h, mg_it, mg_chunk)) return FS_E_HIP;
+      mg_it += mg_chunk;
+      mg_chunk = chunk > 8 ? chunk / 4 : chunk;
+      if (int e = end_chunk()) return e;
     }
   } else {
     PcgArgs ia = pcg_args(h, 0);
@@ -903,7 +931,10 @@ CtrlArgs ctrl_args(fs_handle* h) {
   return ca;
 }
 
-int ptcl_step_impl(fs_handle* h, fs_step_info* info) {
+// ahead: after each attempt's controller kernel, issue the Poisson solve that
+// follows an accepted attempt (lapl_solve_impl mode 1) before reading the
+// controller's decision.
+int ptcl_step_impl(fs_handle* h, fs_step_info* info, bool ahead = false) {
   launch_begin_step(h->d_sc, h->st);
   int attempts = 0;
   for (;;) {
@@ -917,6 +948,11 @@ int ptcl_step_impl(fs_handle* h, fs_step_info* info) {
     launch_ctrl(ctrl_args(h), h->st);
     HIP_OK(hipGetLastError());
     time_mark(h, 2);
+    if (ahead) {
+      int lerr_ahead = 0;
+      h->alias = false;  // the solve reads level 1 (as below, after the read)
+      if (int e = lapl_solve_impl(h, &lerr_ahead, nullptr, 1)) return e;
+    }
     if (sync_scalars(h)) return FS_E_HIP;
     if (h->timing) {
       float ms = 0;
@@ -1685,21 +1721,28 @@ int fs_reject(fs_handle* h) {
   return reject_impl(h);
 }
 
+// The host copy of the scalars is refreshed by the reads inside each pass (the
+// controller's decision, the end of the solve); between passes only ts, which
+// decides whether the next pass starts with a transport step, is advanced on
+// the host as k_accept advances it on the device, and one read at the end
+// leaves the copy current.
 int fs_run(fs_handle* h, int32_t n_iters) {
   if (!h) return fail(FS_E_ARG, "null handle%s");
   for (int it = 0; it < n_iters; it++) {
+    int lerr = 0;
     if (h->h_sc->ts > 1.0) {  // main.F90:75
-      if (int e = ptcl_step_impl(h, nullptr)) return e;
+      const bool ahead = can_issue_ahead(h);
+      if (int e = ptcl_step_impl(h, nullptr, ahead)) return e;
+      if (int e = lapl_solve_impl(h, &lerr, nullptr, ahead ? 2 : 0)) return e;
     } else {
       launch_begin_step(h->d_sc, h->st);
+      if (int e = lapl_solve_impl(h, &lerr, nullptr)) return e;
     }
-    int lerr = 0;
-    if (int e = lapl_solve_impl(h, &lerr, nullptr)) return e;
     int e = lerr == 0 ? accept_impl(h) : reject_impl(h);
     if (e) return e;
-    if (sync_scalars(h)) return FS_E_HIP;
+    if (lerr == 0) h->h_sc->ts = h->h_sc->ts + 1.0;
   }
-  return 0;
+  return sync_scalars(h) ? FS_E_HIP : 0;
 }
 
 int fs_set_replay(fs_handle* h, const double* recs, int32_t n) {

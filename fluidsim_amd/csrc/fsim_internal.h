@@ -90,6 +90,7 @@ struct LaplArgs {
   DevScalars* sc;
   Geo G;
   int32_t nbx, nby;
+  int32_t gate, pad_;   // gate: skip the solve (done = 3) unless the last attempt was accepted
 };
 
 struct PcgArgs {

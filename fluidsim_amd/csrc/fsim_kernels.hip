@@ -592,6 +592,10 @@ __device__ __forceinline__ void lapl_body(const LaplArgs& A, const FsPhys& K, in
   __shared__ double s_f[4][kNF];
   __shared__ double s_red[8 * kNW];
   __shared__ int s_bad;
+  if (A.gate && A.sc->accept == 0) {  // the attempt was rejected: no solve
+    if (bid == 0 && threadIdx.x == 0) A.sc->done = 3;
+    return;
+  }
   const Geo& G = A.G;
   const int bx = G.bx, by = G.by, P = G.P;
   const int t = xcd_tile(bid, nb);
