@@ -28,9 +28,9 @@ def gpu_run(args):
     from fluidsim_amd.lib import FsError
 
     if args.config == 2:
-        cfg = configs.config2(args.n, rtol=args.rtol, chunk=args.chunk)
+        cfg = configs.config2(args.n, rtol=args.rtol, chunk=args.chunk, warm=int(not args.cold))
     else:
-        cfg = configs.config1(args.n, rtol=args.rtol, chunk=args.chunk)
+        cfg = configs.config1(args.n, rtol=args.rtol, chunk=args.chunk, warm=int(not args.cold))
     d = Discharge(cfg)
     d.init_gaussian(seed=0)
     # warm-up: the two Poisson-only passes (main.F90:75) and the first steps,
@@ -155,6 +155,8 @@ def ensemble_run(args):
     from fluidsim_amd.ensemble import Ensemble
 
     cfgs = configs.config3(args.members)
+    for c in cfgs:
+        c["lapl_warm"] = int(not args.cold)
     t0 = time.perf_counter()
     E = Ensemble(cfgs)
     for m, c in zip(E.members, cfgs):
@@ -232,6 +234,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--solve-reps", type=int, default=3)
     ap.add_argument("--no-learned", action="store_true", help="skip the learned preconditioner")
+    ap.add_argument("--cold", action="store_true",
+                    help="start every multigrid-PCG solve of the run from zero (no warm start)")
     ap.add_argument("--learn-samples", type=int, default=20000)
     ap.add_argument("--learn-steps", type=int, default=3000)
     ap.add_argument("--learned-n2", type=int, default=1024,
@@ -264,7 +268,8 @@ def main():
             "data": "synthetic (Gaussian + seeded 1% noise, member seed = id)",
             "config": {"workload": f"{args.members} argon discharges, sides U{{64..256}}, "
                                    "150-400 V DC, 0.5-10 Torr log-uniform, gamma 0.05-0.15",
-                       "members": args.members, "cells_total": e["cells"]},
+                       "members": args.members, "cells_total": e["cells"],
+                       "poisson_initial_guess": "zero" if args.cold else "previous pass's correction"},
             "accepted_steps": e["accepted"], "ptcl_attempts": e["attempts"],
             "member_passes_per_s": args.members * args.steps / e["elapsed"],
             "solo_sequential_steps_per_s": e["solo_rate"], "solo_sample": e["solo_sample"],
@@ -283,11 +288,12 @@ def main():
     t_stages = sum(kern[f"stage{s}"] for s in (1, 2, 3, 4, 5)) * 1e-3
     ach_s = bytes_step / t_stages / 1e9
     stage_traffic = profiled_traffic("k_stage", args.n)
-    # multigrid finest level with the fused CG update: down reads r, w, s, d,
-    # cx, cy, u, p, x, writes p, s, x, r, x1 and the quarter-size coarse
-    # residual (14.25 planes); up reads x1, r, d, cx, cy and the coarse
+    # multigrid finest level with the fused CG update, timed in its
+    # first-iteration form: down reads r, w, d, cx, cy, u, x, writes p, s, x, r,
+    # x1 and the quarter-size coarse residual (12.25 planes; later iterations
+    # also read s and p: 14.25); up reads x1, r, d, cx, cy and the coarse
     # correction, writes u and w = M u (7.25 planes)
-    ach_d = 14.25 * plane / (kern["mg_down0"] * 1e-3) / 1e9
+    ach_d = 12.25 * plane / (kern["mg_down0"] * 1e-3) / 1e9
     ach_u = 7.25 * plane / (kern["mg_up0"] * 1e-3) / 1e9
     timers = r["timers"]
     tot_ms = r["elapsed"] * 1e3
@@ -311,7 +317,8 @@ def main():
                                 f"1 Torr, 300 V through 5e4 ohm"),
                    "nx": args.n, "ny": args.n, "poisson_rtol": args.rtol,
                    "poisson_solver": "CG preconditioned by an aggregation multigrid V(2,2) "
-                                     "cycle with red-black Gauss-Seidel smoothing"},
+                                     "cycle with red-black Gauss-Seidel smoothing",
+                   "poisson_initial_guess": "zero" if args.cold else "previous pass's correction"},
         "accepted_steps": r["accepted"],
         "ptcl_attempts": r["attempts"],
         "pcg_iters_mean": sum(iters) / max(len(iters), 1),

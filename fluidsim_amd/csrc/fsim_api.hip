@@ -162,7 +162,9 @@ struct fs_handle {
   StageArgs* d_stab = nullptr;
   LaplArgs* d_ltab = nullptr;
   FsPhys* d_phys = nullptr;
-  hipGraphExec_t mg_pro = nullptr;    // MG-PCG set-up and iteration 0
+  hipGraphExec_t mg_pro[2] = {};      // MG-PCG set-up and iteration 0 (from zero, warm)
+  bool warm_ok = false;               // x holds the last converged MG-PCG correction
+  bool moved = false;                 // an accepted transport step since the last solve
   int mg_it = 0, mg_chunk = 0;        // next MG-PCG iteration / chunk of a solve in progress
   hipGraphExec_t mg_chain[2][kChainLog] = {};  // 2^m MG-PCG iterations from parity p
   hipGraphExec_t learn_graph[2] = {}; // one learned-FCG iteration, by parity
@@ -637,13 +639,14 @@ int capture_iteration(fs_handle* h, hipGraphExec_t* out, int (*launch)(fs_handle
   return capture(h, out, [&]() { return launch(h, it); });
 }
 
-// MG-PCG set-up: Galerkin hierarchy, coarsest inverse, x = 0, the initial
-// V-cycle and iteration 0.
-int mg_prologue(fs_handle* h) {
+// MG-PCG set-up: Galerkin hierarchy, coarsest inverse, the start (x = 0, or
+// with `warm` the x in place), the initial V-cycle and iteration 0.
+int mg_prologue(fs_handle* h, bool warm) {
   for (size_t l = 0; l + 1 < h->mg.size(); l++) launch_mg_coarsen(mg_args(h, l), h->st);
   launch_mg_coarse_setup(cycle_args(h, -1), h->st);
   MgPcgArgs ia = mgpcg_args(h, 1);
   ia.part_out = h->part(0);
+  ia.warm = warm;
   launch_mgpcg_init(ia, h->st);
   return launch_vcycle(h, -1) || launch_vcycle(h, 0) ? FS_E_HIP : 0;
 }
@@ -679,7 +682,8 @@ void drop_graphs(fs_handle* h) {
     if (g) (void)hipGraphExecDestroy(g);
     g = nullptr;
   };
-  drop(h->mg_pro);
+  drop(h->mg_pro[0]);
+  drop(h->mg_pro[1]);
   for (int k = 0; k < 2; k++) {
     drop(h->learn_graph[k]);
     for (int m = 0; m < kChainLog; m++) drop(h->mg_chain[k][m]);
@@ -858,12 +862,15 @@ int lapl_solve_impl(fs_handle* h, int* lerr, fs_solve_stats* stats, int mode = 0
     int& mg_it = h->mg_it;
     int& mg_chunk = h->mg_chunk;
     if (mode != 2) {
+      // the last correction is a useful start only for the next pass's solve:
+      // densities have moved since (not for a repeated or a Poisson-only solve)
+      const bool warm = h->p.lapl_warm && h->warm_ok && h->moved;
       if (!h->p.use_graphs) {
-        if (mg_prologue(h)) return FS_E_HIP;
+        if (mg_prologue(h, warm)) return FS_E_HIP;
       } else {
-        if (!h->mg_pro && capture(h, &h->mg_pro, [&]() { return mg_prologue(h); }))
-          return FS_E_HIP;
-        HIP_OK(hipGraphLaunch(h->mg_pro, h->st));
+        hipGraphExec_t& g = h->mg_pro[warm ? 1 : 0];
+        if (!g && capture(h, &g, [&]() { return mg_prologue(h, warm); })) return FS_E_HIP;
+        HIP_OK(hipGraphLaunch(g, h->st));
       }
       // iteration 0 is launched already; then as many as the last solve took,
       // then 2 more, then chunk / 4 at a time
@@ -901,6 +908,8 @@ int lapl_solve_impl(fs_handle* h, int* lerr, fs_solve_stats* stats, int mode = 0
   *lerr = ok ? 0 : 1;
   const int sv = h->p.lapl_solver == 2 ? 2 : (h->p.lapl_solver == 1 && h->mg.size() > 1) ? 1 : 0;
   h->last_iters[sv] = h->h_sc->iters;
+  h->warm_ok = sv == 1 && ok;
+  h->moved = false;
   if (h->timing) {
     float ms = 0;
     (void)hipEventElapsedTime(&ms, h->ev[4], h->ev[5]);
@@ -951,6 +960,7 @@ int ptcl_step_impl(fs_handle* h, fs_step_info* info, bool ahead = false) {
     if (ahead) {
       int lerr_ahead = 0;
       h->alias = false;  // the solve reads level 1 (as below, after the read)
+      h->moved = true;   // it runs only if this attempt is accepted
       if (int e = lapl_solve_impl(h, &lerr_ahead, nullptr, 1)) return e;
     }
     if (sync_scalars(h)) return FS_E_HIP;
@@ -966,7 +976,10 @@ int ptcl_step_impl(fs_handle* h, fs_step_info* info, bool ahead = false) {
     log_attempt(h);
     if (int e = check_status(h)) return e;
     h->alias = false;
-    if (s.accept) break;
+    if (s.accept) {
+      h->moved = true;
+      break;
+    }
     // internal restart, ptcl_lib.F90:76-93: n1 = n2 = n3 (level 1 is rewritten at stage 5)
     if (copy_level(h, h->bufA(), h->bufL3())) return FS_E_HIP;
   }
@@ -1017,6 +1030,7 @@ struct EnsMapH {
 
 struct EnsRound {  // uploaded once per round
   int32_t* sel;
+  int32_t* sel2;  // the MG-PCG start of each member: i * 2 + warm
   uint8_t *ptcl, *pois, *begin, *acc, *rej;
 };
 
@@ -1037,7 +1051,7 @@ struct fs_ensemble {
   CtrlArgs* t_ctrl = nullptr;     // [i]
   LaplArgs* t_lapl = nullptr;     // [i*6 + rot*2 + alias]
   MgArgs* t_coarsen = nullptr;    // [i*kMgMaxLevels + l]
-  MgPcgArgs* t_init = nullptr;    // [i]
+  MgPcgArgs* t_init = nullptr;    // [i * 2 + warm]
   // V-cycle variants v: 0 the initial cycle, 1 odd, 2 even, 3 first iteration
   MgArgs* t_down = nullptr;       // [(i*kMgMaxLevels + l)*4 + v]
   MgArgs* t_up = nullptr;         // [(i*kMgMaxLevels + l)*4 + v]
@@ -1102,7 +1116,7 @@ int ens_tables(fs_ensemble* E) {
   std::vector<CtrlArgs> ct(n);
   std::vector<LaplArgs> la(n * 6);
   std::vector<MgArgs> co(n * kMgMaxLevels), dn(n * kMgMaxLevels * 4), up(n * kMgMaxLevels * 4);
-  std::vector<MgPcgArgs> in(n);
+  std::vector<MgPcgArgs> in(2 * n);
   std::vector<MgCycleArgs> cc(n * 4);
   std::vector<PcgArgs> fi(n);
   for (int i = 0; i < n; i++) {
@@ -1123,8 +1137,11 @@ int ens_tables(fs_ensemble* E) {
     ct[i] = ctrl_args(h);
     const int L = (int)h->mg.size();
     for (int l = 0; l + 1 < L; l++) co[i * kMgMaxLevels + l] = mg_args(h, l);
-    in[i] = mgpcg_args(h, 1);
-    in[i].part_out = h->part(0);
+    for (int w = 0; w < 2; w++) {
+      in[i * 2 + w] = mgpcg_args(h, 1);
+      in[i * 2 + w].part_out = h->part(0);
+      in[i * 2 + w].warm = w;
+    }
     static const int kVariantIt[4] = {-1, 1, 2, 0};
     for (int v = 0; v < 4; v++) {
       for (int l = 0; l < (int)h->mg_first; l++) {
@@ -1222,7 +1239,7 @@ int ens_round(fs_ensemble* E, std::vector<int>& left) {
   EnsRound& r = E->r;
   char* hr = E->h_round;
   int32_t* sel = (int32_t*)hr;
-  uint8_t* ptcl = (uint8_t*)(hr + 4 * n);
+  uint8_t* ptcl = (uint8_t*)(hr + 8 * n);
   uint8_t *pois = ptcl + n, *begin = pois + n, *acc = begin + n, *rej = acc + n;
   memset(ptcl, 0, 5 * n);
   // passes that start now: begin_step, then a ptcl attempt (or, before the
@@ -1260,16 +1277,19 @@ int ens_round(fs_ensemble* E, std::vector<int>& left) {
       h->alias = false;
       if (E->h_sc[i].accept) {
         E->phase[i] = 2;
+        h->moved = true;
       } else if (copy_level(h, h->bufA(), h->bufL3())) {  // ptcl_lib.F90:76-93
         return FS_E_HIP;
       }
     }
   }
   int npois = 0;
+  int32_t* sel2 = sel + n;
   for (int i = 0; i < n; i++) {
     pois[i] = left[i] > 0 && E->phase[i] == 2;
     npois += pois[i];
     sel[i] = i * 6 + E->m[i]->rot * 2 + (E->m[i]->alias ? 1 : 0);
+    sel2[i] = i * 2 + (E->m[i]->p.lapl_warm && E->m[i]->warm_ok && E->m[i]->moved ? 1 : 0);
   }
   if (!npois) return 0;
   if (ens_upload_round(E)) return FS_E_HIP;
@@ -1278,7 +1298,7 @@ int ens_round(fs_ensemble* E, std::vector<int>& left) {
     launch_ens_mg_coarsen(E->t_coarsen, with_mask(E->m_coarsen[l], r.pois),
                           EnsSel{nullptr, kMgMaxLevels, l}, E->st);
   launch_ens_mg_coarse_setup(E->t_cc, with_mask(E->m_one, r.pois), EnsSel{nullptr, 4, 0}, E->st);
-  launch_ens_mgpcg_init(E->t_init, with_mask(E->m_pcg, r.pois), EnsSel{nullptr, 1, 0}, E->st);
+  launch_ens_mgpcg_init(E->t_init, with_mask(E->m_pcg, r.pois), EnsSel{r.sel2, 1, 0}, E->st);
   ens_vcycle(E, 0, r.pois);
   ens_iteration(E, 0, r.pois);
   int it = 1;
@@ -1303,6 +1323,8 @@ int ens_round(fs_ensemble* E, std::vector<int>& left) {
     if (int e = ens_fail(E, i)) return e;
     const int lerr = E->h_sc[i].done == 1 ? 0 : 1;
     log_solve(h, lerr);
+    h->warm_ok = lerr == 0;
+    h->moved = false;
     acc[i] = lerr == 0;
     rej[i] = lerr != 0;
   }
@@ -1580,6 +1602,7 @@ static double* field_dev(const fs_handle* h, int field, int level, bool for_writ
 
 int fs_set_field(fs_handle* h, int32_t field, int32_t level, const double* host, int32_t ld) {
   if (!h || !host) return fail(FS_E_ARG, "null argument%s");
+  h->warm_ok = false;
   if (ld < h->P) return fail(FS_E_ARG, "ld < nx+2%s");
   double* d = field_dev(h, field, level, true);
   if (!d) return fail(FS_E_ARG, "bad field/level%s");
@@ -1850,6 +1873,7 @@ int fs_eval_dfdph(int32_t n, const double* a, double* out) {
 
 int fs_time_kernel(fs_handle* h, int32_t which, int32_t n, double* avg_ms) {
   if (!h || !avg_ms || n <= 0) return fail(FS_E_ARG, "bad argument%s");
+  h->warm_ok = false;  // the timed launches overwrite the solver's vectors
   if (sync_scalars(h)) return FS_E_HIP;
   const double tol0 = h->h_sc->tol;
   const int32_t maxit0 = h->h_sc->maxit;
@@ -1871,9 +1895,11 @@ int fs_time_kernel(fs_handle* h, int32_t which, int32_t n, double* avg_ms) {
     } else if (which == 16) {
       (void)launch_learned_iteration(h, 2 + (k & 1));
     } else if (which == 17) {
-      launch_vcycle(h, 2 + (k & 1));
+      launch_vcycle(h, 2 + k);
     } else if (which == 13) {
-      launch_mg_down(mg_down_args(h, 0, 2 + (k & 1)), h->st);
+      // first-iteration form (beta = 0, p and s_old not read): repeated on the
+      // same partials it stays a valid update, unlike the recurrence form
+      launch_mg_down(mg_down_args(h, 0, 0), h->st);
     } else if (which == 14) {
       launch_mg_up(mg_up_args(h, 0, 2 + (k & 1)), h->st);
     } else if (which == 18) {
@@ -1903,9 +1929,20 @@ int fs_time_kernel(fs_handle* h, int32_t which, int32_t n, double* avg_ms) {
     ia.w_new = h->pois(PO_W0);
     launch_pcg_init(ia, h->st);
   }
-  launch(0);  // warm
+  if (which == 13 || which == 14 || which == 17 || which == 18 || level_id) {
+    // a consistent multigrid-PCG state: start, the initial cycle, iterations 0 and 1
+    for (size_t l = 0; l + 1 < h->mg.size(); l++) launch_mg_coarsen(mg_args(h, l), h->st);
+    launch_mg_coarse_setup(cycle_args(h, -1), h->st);
+    MgPcgArgs ia = mgpcg_args(h, 1);
+    ia.part_out = h->part(0);
+    launch_mgpcg_init(ia, h->st);
+    if (launch_vcycle(h, -1) || launch_vcycle(h, 0) || launch_vcycle(h, 1)) return FS_E_HIP;
+  }
+  // iteration numbers keep alternating in parity through the warm-up launch
+  int seq = 0;
+  launch(seq++);  // warm
   HIP_OK(hipEventRecord(h->ev[6], h->st));
-  for (int k = 0; k < n; k++) launch(k);
+  for (int k = 0; k < n; k++) launch(seq++);
   HIP_OK(hipEventRecord(h->ev[7], h->st));
   HIP_OK(hipEventSynchronize(h->ev[7]));
   HIP_OK(hipGetLastError());
@@ -1955,6 +1992,7 @@ int fs_set_solver(fs_handle* h, int32_t solver) {
   if (!h) return fail(FS_E_ARG, "null handle%s");
   if (solver < 0 || solver > 2) return fail(FS_E_ARG, "unknown Poisson solver%s");
   h->p.lapl_solver = solver;
+  h->warm_ok = false;
   return 0;
 }
 
@@ -2002,7 +2040,7 @@ int fs_ensemble_create(int32_t n, const fs_grid_desc* g, const fs_params* p, fs_
       return FS_E_HIP;
     }
   }
-  E->round_bytes = (size_t)n * 9;
+  E->round_bytes = (size_t)n * 13;
   if (bad(hipMemcpy(E->d_sc, E->h_sc, n * sizeof(DevScalars), hipMemcpyHostToDevice)) ||
       bad(hipMalloc(&E->d_round, E->round_bytes)) ||
       bad(hipHostMalloc(&E->h_round, E->round_bytes)) || ens_maps(E)) {
@@ -2010,7 +2048,8 @@ int fs_ensemble_create(int32_t n, const fs_grid_desc* g, const fs_params* p, fs_
     return fail(FS_E_HIP, "ensemble set-up failed%s");
   }
   E->r.sel = (int32_t*)E->d_round;
-  E->r.ptcl = (uint8_t*)(E->d_round + 4 * n);
+  E->r.sel2 = E->r.sel + n;
+  E->r.ptcl = (uint8_t*)(E->d_round + 8 * n);
   E->r.pois = E->r.ptcl + n;
   E->r.begin = E->r.pois + n;
   E->r.acc = E->r.begin + n;
@@ -2106,6 +2145,7 @@ int fs_learn_harvest(fs_learn* n, fs_handle* h, int32_t* added) {
   const MgLevel& lc = h->mg[learned_level(h)];
   if (lc.nx > n->net.W || lc.ny > n->net.H)
     return fail(FS_E_ARG, "learned level larger than the training grid%s");
+  h->warm_ok = false;
   const int before = n->net.count;
   HIP_OK(hipStreamSynchronize(n->net.st));
   launch_assembly(h);

@@ -205,6 +205,7 @@ struct MgPcgArgs {
   DevScalars* sc;
   Geo G;
   int32_t nbx, nby, first, flex;  // flex: Polak-Ribiere beta (flexible CG)
+  int32_t warm, pad_;             // init: start from the x in place (r = rhs - M x)
 };
 
 constexpr int kTX = 64;  // tile width (one wave64 per tile row)

@@ -1187,18 +1187,20 @@ __device__ __forceinline__ int gs_ntiles(const MgLevel& F) {
 // side (init) or after the update of iteration `it`.  Every block decides
 // identically; `writer` (one thread) records the count, the residual and the
 // outcome.  True when the solve stops here.
-__device__ __forceinline__ bool mg_check(DevScalars* sc, double rho, int init, int it,
+__device__ __forceinline__ bool mg_check(DevScalars* sc, double rho, double b2, int init, int it,
                                          bool writer) {
-  if (init) {
+  if (init) {  // b2 = |rhs/V|^2; rho < b2 after a warm start
+    const double res = b2 > 0.0 ? sqrt(rho / b2) : 0.0;
+    const bool conv = b2 == 0.0 || res <= sc->tol;
     if (writer) {
-      sc->bnorm2 = rho;
-      sc->resid = rho > 0 ? 1.0 : 0.0;
+      sc->bnorm2 = b2;
+      sc->resid = res;
       sc->iters = 0;
       sc->itc[0] = 0;
       sc->itc[1] = 0;
-      if (rho == 0.0) sc->done = 1;
+      if (conv) sc->done = 1;
     }
-    return rho == 0.0;
+    return conv;
   }
   // iteration count kept in two slots by parity: the other slot was written
   // by the previous iteration, the writer fills this one
@@ -1332,15 +1334,19 @@ __device__ __forceinline__ void mg_down_body(const MgArgs& A, int bid, int nb) {
     double tot[3];
     finish_partials3(pv, s_red, tot);
     if (A.check) {
-      if (mg_check(sc, tot[0], A.init, A.it, bid == 0 && threadIdx.x == 0)) return;
+      if (mg_check(sc, tot[0], tot[1], A.init, A.it, bid == 0 && threadIdx.x == 0)) return;
     } else {
       // gamma = r.u, delta = u.M u; p.s = delta - beta gamma / alpha_old
       const double gam = tot[0], del = tot[1];
-      if (A.cg.first) {
-        alpha = gam / del;
-      } else {
+      double ps = del;
+      if (!A.cg.first) {
         beta = gam / sc->cg_gam[(A.it + 1) & 1];
-        alpha = gam / (del - beta * gam / sc->cg_alpha[(A.it + 1) & 1]);
+        ps = del - beta * gam / sc->cg_alpha[(A.it + 1) & 1];
+      }
+      alpha = gam / ps;
+      if (!(ps > 0.0) || !isfinite(alpha)) {  // breakdown: stop, not converged
+        if (bid == 0 && threadIdx.x == 0) sc->done = 2;
+        return;
       }
       if (bid == 0 && threadIdx.x == 0) {
         sc->cg_gam[A.it & 1] = gam;
@@ -1544,12 +1550,14 @@ __global__ void __launch_bounds__(1024) k_mg_coarse_setup(MgCycleArgs A) {
 // pairs (thread t owns pairs
 // t + 1024 p, p < kMgLdsPairs), one cell of each colour per pair, so each
 // half-sweep keeps every lane busy; the coarser levels hold one cell per
-// thread.  The sweeps multiply by the inverse diagonal.  These levels compute
-// in FP32 (a coarse correction needs no more; the outer CG and the finer
-// levels stay FP64).  Same cycle as the tile kernels; level f's correction is
-// written back in FP64 for the levels above.
+// thread.  The sweeps multiply by the inverse diagonal.  The operator rows
+// are FP32 values but the arithmetic is FP64: the cycle must stay a symmetric
+// operator to rounding level, which the CG recurrences (fused, Chronopoulos-
+// Gear) rely on; with FP32 arithmetic here, warm-started solves broke down near
+// convergence.  Same cycle as the tile kernels; level f's correction is
+// written back for the levels above.
 struct PairRows {
-  float dinv[2][kMgLdsPairs], l[2][kMgLdsPairs], r[2][kMgLdsPairs], b[2][kMgLdsPairs],
+  double dinv[2][kMgLdsPairs], l[2][kMgLdsPairs], r[2][kMgLdsPairs], b[2][kMgLdsPairs],
       t[2][kMgLdsPairs], rhs[2][kMgLdsPairs];
   int ij[2][kMgLdsPairs];  // i | j << 16 of the cell of colour c, or -1
 };
@@ -1559,7 +1567,7 @@ __device__ __forceinline__ int ij_kc(int ij, int Wc) {
   return ((ij >> 17) + 1) * Wc + ((ij & 0xffff) >> 1) + 1;
 }
 struct CellRows {
-  float dinv, l, r, b, t, rhs;
+  double dinv, l, r, b, t, rhs;
   int k;   // padded index of cell t, or -1
   int ij;  // i | j << 16
 };
@@ -1579,12 +1587,12 @@ __device__ __forceinline__ void lds_pair_rows(const MgLevel& L, PairRows& R, con
       R.r[c][p] = q[2048];
       R.b[c][p] = q[3072];
       R.t[c][p] = q[4096];
-      R.rhs[c][p] = 0.0f;
+      R.rhs[c][p] = 0.0;
       R.ij[c][p] = -1;
       if (pp < npr * L.ny) {
         const int j = pp / npr, i = 2 * (pp % npr) + (c ^ (j & 1));
         if (i < L.nx) {
-          R.rhs[c][p] = (float)L.r[(j + 1) * W + i + 1];
+          R.rhs[c][p] = (double)L.r[(j + 1) * W + i + 1];
           R.ij[c][p] = i | (j << 16);
         }
       }
@@ -1600,7 +1608,7 @@ __device__ __forceinline__ void lds_cell_rows(const MgLevel& L, CellRows& R, con
   R.r = q[2048];
   R.b = q[3072];
   R.t = q[4096];
-  R.rhs = 0.0f;
+  R.rhs = 0.0;
   R.k = -1;
   R.ij = 0;
   if (t < L.nx * L.ny) {
@@ -1612,8 +1620,8 @@ __device__ __forceinline__ void lds_cell_rows(const MgLevel& L, CellRows& R, con
 
 // Position of cell t on the coarsest level (its rows are not needed).
 __device__ __forceinline__ void lds_cell_pos(const MgLevel& L, CellRows& R, int t) {
-  R.dinv = 1.0f;
-  R.l = R.r = R.b = R.t = R.rhs = 0.0f;
+  R.dinv = 1.0;
+  R.l = R.r = R.b = R.t = R.rhs = 0.0;
   R.k = -1;
   R.ij = 0;
   if (t < L.nx * L.ny) {
@@ -1623,17 +1631,17 @@ __device__ __forceinline__ void lds_cell_pos(const MgLevel& L, CellRows& R, int 
   }
 }
 
-__device__ __forceinline__ float lds_gs(float rhs, float l, float r, float b, float t, float dinv,
-                                        const float* x, int k, int W) {
+__device__ __forceinline__ double lds_gs(double rhs, double l, double r, double b, double t, double dinv,
+                                        const double* x, int k, int W) {
   return (rhs + l * x[k - 1] + r * x[k + 1] + b * x[k - W] + t * x[k + W]) * dinv;
 }
-__device__ __forceinline__ float lds_res(float rhs, float l, float r, float b, float t, float dinv,
-                                         const float* x, int k, int W) {
+__device__ __forceinline__ double lds_res(double rhs, double l, double r, double b, double t, double dinv,
+                                         const double* x, int k, int W) {
   return rhs - (x[k] / dinv - l * x[k - 1] - r * x[k + 1] - b * x[k - W] - t * x[k + W]);
 }
 
 template <int COL>
-__device__ __forceinline__ void pair_half(const PairRows& R, float* x, int W) {
+__device__ __forceinline__ void pair_half(const PairRows& R, double* x, int W) {
 #pragma unroll
   for (int p = 0; p < kMgLdsPairs; p++) {
     const int k = ij_k(R.ij[COL][p], W);
@@ -1645,7 +1653,7 @@ __device__ __forceinline__ void pair_half(const PairRows& R, float* x, int W) {
 }
 
 template <int COL>
-__device__ __forceinline__ void cell_half(const CellRows& R, float* x, int W) {
+__device__ __forceinline__ void cell_half(const CellRows& R, double* x, int W) {
   if (R.k >= 0 && (((R.ij & 0xffff) + (R.ij >> 16)) & 1) == COL)
     x[R.k] = lds_gs(R.rhs, R.l, R.r, R.b, R.t, R.dinv, x, R.k, W);
   __syncthreads();
@@ -1653,7 +1661,7 @@ __device__ __forceinline__ void cell_half(const CellRows& R, float* x, int W) {
 
 // Pre-smoothing from zero (red pointwise, then black, red, black) and the
 // residual into s_r.
-__device__ __forceinline__ void pair_down(const PairRows& R, float* x, float* s_r, int W) {
+__device__ __forceinline__ void pair_down(const PairRows& R, double* x, double* s_r, int W) {
 #pragma unroll
   for (int p = 0; p < kMgLdsPairs; p++)
     if (R.ij[0][p] >= 0) x[ij_k(R.ij[0][p], W)] = R.rhs[0][p] * R.dinv[0][p];
@@ -1673,7 +1681,7 @@ __device__ __forceinline__ void pair_down(const PairRows& R, float* x, float* s_
   __syncthreads();
 }
 
-__device__ __forceinline__ void cell_down(const CellRows& R, float* x, float* s_r, int W) {
+__device__ __forceinline__ void cell_down(const CellRows& R, double* x, double* s_r, int W) {
   const bool red = R.k >= 0 && (((R.ij & 0xffff) + (R.ij >> 16)) & 1) == 0;
   if (red) x[R.k] = R.rhs * R.dinv;
   __syncthreads();
@@ -1685,9 +1693,9 @@ __device__ __forceinline__ void cell_down(const CellRows& R, float* x, float* s_
 }
 
 // Restriction of level L's residual (s_r) to the coarse cell held in C.
-__device__ __forceinline__ float lds_restrict(const MgLevel& L, const CellRows& C,
-                                              const float* s_r) {
-  float v = 0.0f;
+__device__ __forceinline__ double lds_restrict(const MgLevel& L, const CellRows& C,
+                                              const double* s_r) {
+  double v = 0.0;
   if (C.k >= 0) {
     const int I = C.ij & 0xffff, J = C.ij >> 16, W = L.nx + 2;
     for (int b = 0; b < 2; b++)
@@ -1698,8 +1706,8 @@ __device__ __forceinline__ float lds_restrict(const MgLevel& L, const CellRows& 
 }
 
 // Prolongation with over-correction and post-smoothing (black, red, black, red).
-__device__ __forceinline__ void pair_up(const PairRows& R, float* x, const float* xc, int W,
-                                        int Wc, float oc) {
+__device__ __forceinline__ void pair_up(const PairRows& R, double* x, const double* xc, int W,
+                                        int Wc, double oc) {
 #pragma unroll
   for (int c = 0; c < 2; c++)
 #pragma unroll
@@ -1712,8 +1720,8 @@ __device__ __forceinline__ void pair_up(const PairRows& R, float* x, const float
   pair_half<0>(R, x, W);
 }
 
-__device__ __forceinline__ void cell_up(const CellRows& R, float* x, const float* xc, int W,
-                                        int Wc, float oc) {
+__device__ __forceinline__ void cell_up(const CellRows& R, double* x, const double* xc, int W,
+                                        int Wc, double oc) {
   if (R.k >= 0) {
     const int i = R.ij & 0xffff, j = R.ij >> 16;
     x[R.k] += oc * xc[((j >> 1) + 1) * Wc + (i >> 1) + 1];
@@ -1727,10 +1735,10 @@ __device__ __forceinline__ void cell_up(const CellRows& R, float* x, const float
 
 template <int kInstance>
 __device__ __forceinline__ void mg_coarse_cycle_body(const MgCycleArgs& A) {
-  __shared__ float s_x[kMgLdsX];
-  __shared__ float s_r[kMgLdsR];
+  __shared__ double s_x[kMgLdsX];
+  __shared__ double s_r[kMgLdsR];
   constexpr int kB = kMgBottomMax;
-  __shared__ float s_bp[kB][kB];
+  __shared__ double s_bp[kB][kB];
   __shared__ double s_w[16];
   if (A.sc->done) return;
   const int t = threadIdx.x;
@@ -1755,24 +1763,24 @@ __device__ __forceinline__ void mg_coarse_cycle_body(const MgCycleArgs& A) {
     __syncthreads();
     double rho = 0.0;
     for (int w = 0; w < 16; w++) rho += s_w[w];
-    if (mg_check(A.sc, rho, 0, A.it, t == 0)) return;
+    if (mg_check(A.sc, rho, 0.0, 0, A.it, t == 0)) return;
   }
   const MgLevel& B = A.lv[nq - 1];  // coarsest
   const int nb = B.nx * B.ny;
   // the coarsest level's inverse: entry (brow, bcol) in thread brow + kB * bcol
   const int brow = t % kB, bcol = t / kB;
-  const float binv =
-      (t < kB * kB && brow < nb && bcol < nb) ? (float)A.inv[(size_t)bcol * nb + brow] : 0.0f;
-  for (int k = t; k < kMgLdsX; k += 1024) s_x[k] = 0.0f;
-  float rb = 0.0f;  // right-hand side of the coarsest level at cell t
+  const double binv =
+      (t < kB * kB && brow < nb && bcol < nb) ? (double)A.inv[(size_t)bcol * nb + brow] : 0.0;
+  for (int k = t; k < kMgLdsX; k += 1024) s_x[k] = 0.0;
+  double rb = 0.0;  // right-hand side of the coarsest level at cell t
   if (nq == 1) {
-    if (t < nb) rb = (float)B.r[(t / B.nx + 1) * (B.nx + 2) + t % B.nx + 1];
+    if (t < nb) rb = (double)B.r[(t / B.nx + 1) * (B.nx + 2) + t % B.nx + 1];
   } else {
     __syncthreads();
     // down: level f, then f+1 .. L-2, each restricted to the next
     pair_down(R0, s_x + A.off[0], s_r, A.lv[0].nx + 2);
     {
-      const float v = lds_restrict(A.lv[0], nq == 2 ? Bc : R1[0], s_r);
+      const double v = lds_restrict(A.lv[0], nq == 2 ? Bc : R1[0], s_r);
       if (nq == 2) rb = v;
       else R1[0].rhs = v;
     }
@@ -1781,7 +1789,7 @@ __device__ __forceinline__ void mg_coarse_cycle_body(const MgCycleArgs& A) {
     for (int q = 1; q < kMgLdsLevels - 1; q++) {
       if (q + 1 >= nq) break;
       cell_down(R1[q - 1], s_x + A.off[q], s_r, A.lv[q].nx + 2);
-      const float w = lds_restrict(A.lv[q], q + 2 == nq ? Bc : R1[q], s_r);
+      const double w = lds_restrict(A.lv[q], q + 2 == nq ? Bc : R1[q], s_r);
       if (q + 2 == nq) rb = w;
       else R1[q].rhs = w;
       __syncthreads();
@@ -1789,10 +1797,10 @@ __device__ __forceinline__ void mg_coarse_cycle_body(const MgCycleArgs& A) {
     // coarsest level: x = inv r (products in LDS, summed per row)
     if (t < nb) s_r[t] = rb;
     __syncthreads();
-    if (t < kB * kB) s_bp[bcol][brow] = bcol < nb ? binv * s_r[bcol] : 0.0f;
+    if (t < kB * kB) s_bp[bcol][brow] = bcol < nb ? binv * s_r[bcol] : 0.0;
     __syncthreads();
     if (t < nb) {
-      float v = 0.0f;
+      double v = 0.0;
       for (int c = 0; c < nb; c++) v += s_bp[c][t];
       s_x[A.off[nq - 1] + Bc.k] = v;
     }
@@ -1802,10 +1810,10 @@ __device__ __forceinline__ void mg_coarse_cycle_body(const MgCycleArgs& A) {
     for (int q = kMgLdsLevels - 2; q >= 1; q--) {
       if (q + 1 >= nq) continue;
       cell_up(R1[q - 1], s_x + A.off[q], s_x + A.off[q + 1], A.lv[q].nx + 2, A.lv[q + 1].nx + 2,
-              (float)A.oc);
+              A.oc);
     }
     const MgLevel& L = A.lv[0];
-    pair_up(R0, s_x + A.off[0], s_x + A.off[1], L.nx + 2, A.lv[1].nx + 2, (float)A.oc);
+    pair_up(R0, s_x + A.off[0], s_x + A.off[1], L.nx + 2, A.lv[1].nx + 2, A.oc);
 #pragma unroll
     for (int c = 0; c < 2; c++)
 #pragma unroll
@@ -1819,21 +1827,23 @@ __device__ __forceinline__ void mg_coarse_cycle_body(const MgCycleArgs& A) {
   // a single level: the coarsest, solved directly
   if (t < nb) s_r[t] = rb;
   __syncthreads();
-  if (t < kB * kB) s_bp[bcol][brow] = bcol < nb ? binv * s_r[bcol] : 0.0f;
+  if (t < kB * kB) s_bp[bcol][brow] = bcol < nb ? binv * s_r[bcol] : 0.0;
   __syncthreads();
   if (t < nb) {
-    float v = 0.0f;
+    double v = 0.0;
     for (int c = 0; c < nb; c++) v += s_bp[c][t];
     B.e[(t / B.nx + 1) * (B.nx + 2) + t % B.nx + 1] = (double)v;
   }
 }
 __global__ void __launch_bounds__(1024) k_mg_coarse_cycle(MgCycleArgs A) { mg_coarse_cycle_body<0>(A); }
 
-// MG-PCG: x = 0, r = rhs (from the assembly), p = 0, partials of |rhs/V|^2.
+// MG-PCG start: r = rhs (from the assembly) with x = 0, or with `warm`
+// r = rhs - M x for the x in place; p = 0; partials of |r/V|^2 and |rhs/V|^2.
 template <int kInstance>
 __device__ __forceinline__ void mgpcg_init_body(const MgPcgArgs& A, int bid, int nb) {
   __shared__ double s_red[8 * kNW];
   const Geo& G = A.G;
+  const int P = G.P;
   const TileRange tr = tile_range(A.nbx * A.nby, bid, nb);
   double acc[3] = {0.0, 0.0, 0.0};
   for (int t = tr.t0; t < tr.t1; t++) {
@@ -1841,12 +1851,21 @@ __device__ __forceinline__ void mgpcg_init_body(const MgPcgArgs& A, int bid, int
       const int ci = (t % A.nbx) * kTX + (int)(threadIdx.x % kTX);
       const int cj = (t / A.nbx) * kPcgTY + (int)(threadIdx.x / kTX) + k * (kPcgTY / kPcgRows);
       if (ci < G.bx && cj < G.by) {
-        size_t m = (size_t)(cj + 1) * G.P + (ci + 1);
-        A.x[m] = 0.0;
+        size_t m = (size_t)(cj + 1) * P + (ci + 1);
+        const double rhs = A.r[m];
+        double r = rhs;
+        if (A.warm) {
+          const double* x = A.x;
+          r = rhs - (A.dg[m] * x[m] - A.cx[m - 1] * x[m - 1] - A.cx[m] * x[m + 1] -
+                     A.cy[m - P] * x[m - P] - A.cy[m] * x[m + P]);
+        }
         A.p_new[m] = 0.0;
-        double V = G.dlx[ci] * G.dly[cj] * (G.cyl ? G.r[cj + 1] : 1.0);
-        double rv = A.r[m] / V;
+        const double V = G.dlx[ci] * G.dly[cj] * (G.cyl ? G.r[cj + 1] : 1.0);
+        const double rv = r / V, bv = rhs / V;
         acc[0] += rv * rv;
+        acc[1] += bv * bv;
+        if (A.warm) A.r[m] = r;  // x is only read in this kernel
+        else A.x[m] = 0.0;       // x is not read in this kernel
       }
     }
   }

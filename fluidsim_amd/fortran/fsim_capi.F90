@@ -20,7 +20,7 @@ module fsim_capi
     real(c_double) :: p_torr, gam, vset, res_ohm
     integer(c_int32_t) :: rf, circuit
     real(c_double) :: t_rise, t_width, t_fall, dt0, lapl_rtol
-    integer(c_int32_t) :: lapl_maxit, lapl_chunk, debug_keep_k, lapl_solver, use_graphs, pad_
+    integer(c_int32_t) :: lapl_maxit, lapl_chunk, debug_keep_k, lapl_solver, use_graphs, lapl_warm
   end type
 
   type, bind(C) :: fs_step_info

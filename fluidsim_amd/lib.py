@@ -46,7 +46,7 @@ class Params(C.Structure):
                 ("t_rise", C.c_double), ("t_width", C.c_double), ("t_fall", C.c_double),
                 ("dt0", C.c_double), ("lapl_rtol", C.c_double), ("lapl_maxit", C.c_int32),
                 ("lapl_chunk", C.c_int32), ("debug_keep_k", C.c_int32), ("lapl_solver", C.c_int32),
-                ("use_graphs", C.c_int32), ("pad_", C.c_int32)]
+                ("use_graphs", C.c_int32), ("lapl_warm", C.c_int32)]
 
 
 class StepInfo(C.Structure):

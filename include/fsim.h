@@ -77,7 +77,9 @@ typedef struct fs_params {
   int32_t lapl_solver;    /* 0 Jacobi-PCG, 1 multigrid-preconditioned PCG, 2 learned
                              two-grid flexible CG (fs_learn_attach) */
   int32_t use_graphs;     /* replay the PCG iterations from a captured HIP graph */
-  int32_t pad_;
+  int32_t lapl_warm;      /* 1: a multigrid-PCG solve starts from the previous converged
+                             correction instead of zero (the reference's PETSc KSP starts
+                             from zero); fs_set_field and fs_set_solver reset it */
 } fs_params;
 
 /* Per-attempt controller record (replay and log), 10 doubles:
@@ -197,7 +199,8 @@ int fs_set_timing(fs_handle* h, int32_t on);
  * fused stage kernel of that RK stage, 10 = Poisson assembly, 11/12 = the
  * Jacobi-PCG iteration and initialisation kernels, 13/14 = the multigrid down
  * and up kernels of the finest level (with the CG update and w = M u that the
- * multigrid-PCG fuses into them), 15 = the attached network on the learned
+ * multigrid-PCG fuses into them; the down kernel in its first-iteration form),
+ * 15 = the attached network on the learned
  * level, 16 = one learned flexible-CG iteration, 17 = one multigrid-PCG
  * iteration (sequences of kernels, timed as a whole), 18 = the one-workgroup
  * coarse multigrid kernel, 20 + l / 40 + l = the down / up kernel of tile

@@ -17,12 +17,14 @@ def _solo(c, passes):
     return d
 
 
-@pytest.mark.parametrize("passes", [1, 14])
-def test_members_equal_their_solo_runs_bitwise(passes):
+@pytest.mark.parametrize("passes,warm", [(1, 0), (14, 0), (14, 1)])
+def test_members_equal_their_solo_runs_bitwise(passes, warm):
     from fluidsim_amd import configs
     from fluidsim_amd.ensemble import Ensemble
 
     cfgs = configs.config3(5, base_seed=3, lo=64, hi=96)
+    for c in cfgs:
+        c["lapl_warm"] = warm
     E = Ensemble(cfgs)
     for m, c in zip(E.members, cfgs):
         m.init_gaussian(seed=c["seed"])
