@@ -105,6 +105,11 @@ void fs_phys_init(FsPhys* K, double p, double gam) {
   K->log_s_lo = log(5e-1);
   K->log_s_hi = log(16.0);
   K->exp_m100 = exp(-100.0);
+  K->c_rate = K->t0 / K->x0_3;
+  K->c_rate6 = K->t0 / (1.0e6 * K->x0_3);
+  K->c_nu = K->ninf * K->t0 / K->x0_3;
+  K->c_beta = 1e-13 * K->t0 / K->x0_3;
+  K->me_mi = K->me / K->mi;
 }
 
 void fs_phys_table(const FsPhys* K, double* o) {
@@ -134,7 +139,7 @@ struct fs_handle {
   double* d_k = nullptr;    // k1,k3,k4 x 4 species x PL
   double* d_kdbg = nullptr; // 5 stages x 4 species x PL (debug)
   double* d_ph = nullptr;
-  double* d_geo = nullptr;  // dx,dlx,dy,dly,r
+  double* d_geo = nullptr;  // dx,dlx,dy,dly,r and 1/dx,1/dlx,1/dy,1/dly
   int8_t* d_types = nullptr;  // ltype, rtype, gl, gr (by each)
   double* d_dr = nullptr;     // dra, drc
   double* d_cur = nullptr;
@@ -1460,7 +1465,7 @@ int fs_create(const fs_grid_desc* g, const fs_params* p, fs_handle** out) {
       bad(hipMalloc(&h->d_lev, 16 * PL * sizeof(double))) ||
       bad(hipMalloc(&h->d_k, 12 * PL * sizeof(double))) ||
       bad(hipMalloc(&h->d_ph, PL * sizeof(double))) ||
-      bad(hipMalloc(&h->d_geo, (3 * (size_t)bx + 3 * (size_t)by + 6) * sizeof(double))) ||
+      bad(hipMalloc(&h->d_geo, (5 * (size_t)bx + 5 * (size_t)by + 10) * sizeof(double))) ||
       bad(hipMalloc(&h->d_types, 4 * (size_t)by)) ||
       bad(hipMalloc(&h->d_dr, 2 * (size_t)by * sizeof(double))) ||
       bad(hipMalloc(&h->d_cur, 2 * (size_t)by * sizeof(double))) ||
@@ -1487,6 +1492,8 @@ int fs_create(const fs_grid_desc* g, const fs_params* p, fs_handle** out) {
   geo.insert(geo.end(), h->dy.begin(), h->dy.end());
   geo.insert(geo.end(), h->dly.begin(), h->dly.end());
   geo.insert(geo.end(), h->r.begin(), h->r.end());
+  for (const std::vector<double>* v : {&h->dx, &h->dlx, &h->dy, &h->dly})
+    for (double a : *v) geo.push_back(1.0 / a);
   (void)hipMemcpyAsync(h->d_geo, geo.data(), geo.size() * sizeof(double), hipMemcpyHostToDevice,
                        h->st);
   (void)hipMemcpyAsync(h->d_types, types.data(), types.size(), hipMemcpyHostToDevice, h->st);
@@ -1497,6 +1504,10 @@ int fs_create(const fs_grid_desc* g, const fs_params* p, fs_handle** out) {
   h->G.dy = h->G.dlx + bx;
   h->G.dly = h->G.dy + (by + 1);
   h->G.r = h->G.dly + by;
+  h->G.rdx = h->G.r + (by + 2);
+  h->G.rdlx = h->G.rdx + (bx + 1);
+  h->G.rdy = h->G.rdlx + bx;
+  h->G.rdly = h->G.rdy + (by + 1);
   h->G.ltype = h->d_types;
   h->G.rtype = h->d_types + by;
   h->G.bx = bx;

@@ -37,6 +37,10 @@ struct Geo {
   const double* r;    // by+2  (Fortran r(k+1))
   const int8_t* ltype;  // by: type_x of the first cell of each row (-2 / -1)
   const int8_t* rtype;  // by: type_x of the last cell of each row (2 / 1)
+  const double* rdx;    // 1/dx, 1/dlx, 1/dy, 1/dly
+  const double* rdlx;
+  const double* rdy;
+  const double* rdly;
   int32_t bx, by, P, cyl;
 };
 

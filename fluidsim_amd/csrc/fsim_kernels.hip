@@ -154,19 +154,19 @@ __device__ __forceinline__ Flux4 electrode_right(const FsPhys& K, double E, doub
 }
 
 // interior face between cells L and R (ptcl_lib.F90:131-161)
-__device__ __forceinline__ Flux4 interior_face(const FsPhys& K, double E, double h, double mueL,
-                                               double mueR, double deL, double deR, double niL,
-                                               double niR, double neL, double neR, double ntL,
-                                               double ntR, double nmL, double nmR) {
+// rh = 1/h.
+__device__ __forceinline__ Flux4 interior_face(const FsPhys& K, double E, double h, double rh,
+                                               double mueL, double mueR, double deL, double deR,
+                                               double niL, double niR, double neL, double neR,
+                                               double ntL, double ntR, double nmL, double nmR) {
   Flux4 F;
   double mue = 5e-1 * (mueL + mueR);
   double de = 5e-1 * (deL + deR);
   double mut = K.c53 * mue;
   double dt = K.c53 * de;
-  F.i = sg_flux(K, E, h, 1.0, K.mui, K.Di, niL, niR);
-  F.e = sg_flux(K, E, h, -1.0, mue, de, neL, neR);
-  F.t = sg_flux(K, E, h, -1.0, mut, dt, ntL, ntR);
-  F.m = -(K.Dm * (nmR - nmL) / h);
+  F.i = sg_flux(K, E, h, rh, 1.0, K.mui, K.Di, niL, niR);
+  sg_flux_et(K, E, h, rh, mue, de, mut, dt, neL, neR, ntL, ntR, F.e, F.t);
+  F.m = -(K.Dm * (nmR - nmL) * rh);
   return F;
 }
 
@@ -231,8 +231,8 @@ __device__ __forceinline__ void stage_body(const StageArgs& A, const FsPhys& K, 
     Flux4 F = {0, 0, 0, 0};
     if (cj < by && fi <= bx) {
       int hL = (fy + 1) * kHX + fx, hR = hL + 1;
-      double h = G.dx[fi];
-      double E = -(s_ph[hR] - s_ph[hL]) / h;
+      double h = G.dx[fi], rh = G.rdx[fi];
+      double E = -(s_ph[hR] - s_ph[hL]) * rh;
       if (fi == 0) {
         if (G.ltype[cj] == -2)
           F = electrode_left(K, E, s_ph[hL], s_ph[hR], s_te[hR], s_mue[hR], s_ni[hR], s_ne[hR],
@@ -242,8 +242,8 @@ __device__ __forceinline__ void stage_body(const StageArgs& A, const FsPhys& K, 
           F = electrode_right(K, E, s_ph[hL], s_ph[hR], s_te[hL], s_mue[hL], s_ni[hL],
                               s_ne[hL], s_nt[hL], s_nm[hL]);
       } else {
-        F = interior_face(K, E, h, s_mue[hL], s_mue[hR], s_de[hL], s_de[hR], s_ni[hL], s_ni[hR],
-                          s_ne[hL], s_ne[hR], s_nt[hL], s_nt[hR], s_nm[hL], s_nm[hR]);
+        F = interior_face(K, E, h, rh, s_mue[hL], s_mue[hR], s_de[hL], s_de[hR], s_ni[hL],
+                          s_ni[hR], s_ne[hL], s_ne[hR], s_nt[hL], s_nt[hR], s_nm[hL], s_nm[hR]);
         if (fi == 1) A.cur[cj] = F.i - F.e;            // Ida face (circ_lib.F90:87-118)
         if (fi == bx - 1) A.cur[by + cj] = F.i - F.e;  // Idc face (circ_lib.F90:120-151)
       }
@@ -265,19 +265,19 @@ __device__ __forceinline__ void stage_body(const StageArgs& A, const FsPhys& K, 
   if (valid) {
     typx = ci == 0 ? G.ltype[cj] : (ci == bx - 1 ? G.rtype[cj] : 0);
     int fL = ty * (kTX + 1) + tx, fR = fL + 1;
-    double dlx = G.dlx[ci];
+    double rdlx = G.rdlx[ci];
 #pragma unroll
-    for (int s = 0; s < 4; s++) dfx[s] = (s_f[s][fR] - s_f[s][fL]) / dlx;
+    for (int s = 0; s < 4; s++) dfx[s] = (s_f[s][fR] - s_f[s][fL]) * rdlx;
     double exL, exR;
     if (typx == -1) {
       exL = 0.0;
-      exR = -(s_ph[hc + 1] - s_ph[hc]) / G.dx[ci + 1];
+      exR = -(s_ph[hc + 1] - s_ph[hc]) * G.rdx[ci + 1];
     } else if (typx == 1) {
-      exL = -(s_ph[hc] - s_ph[hc - 1]) / G.dx[ci];
+      exL = -(s_ph[hc] - s_ph[hc - 1]) * G.rdx[ci];
       exR = 0.0;
     } else {
-      exL = -(s_ph[hc] - s_ph[hc - 1]) / G.dx[ci];
-      exR = -(s_ph[hc + 1] - s_ph[hc]) / G.dx[ci + 1];
+      exL = -(s_ph[hc] - s_ph[hc - 1]) * G.rdx[ci];
+      exR = -(s_ph[hc + 1] - s_ph[hc]) * G.rdx[ci + 1];
     }
     feX = (s_f[1][fR] + s_f[1][fL]) / 2.0;
     exA = (exL + exR) / 2.0;
@@ -295,10 +295,10 @@ __device__ __forceinline__ void stage_body(const StageArgs& A, const FsPhys& K, 
     Flux4 F = {0, 0, 0, 0};
     if (fci < bx && fj > 0 && fj < by) {  // faces on y = 0 and y = w carry no flux
       int hL = fy * kHX + fx + 1, hR = hL + kHX;
-      double h = G.dy[fj];
-      double E = -(s_ph[hR] - s_ph[hL]) / h;
-      F = interior_face(K, E, h, s_mue[hL], s_mue[hR], s_de[hL], s_de[hR], s_ni[hL], s_ni[hR],
-                        s_ne[hL], s_ne[hR], s_nt[hL], s_nt[hR], s_nm[hL], s_nm[hR]);
+      double h = G.dy[fj], rh = G.rdy[fj];
+      double E = -(s_ph[hR] - s_ph[hL]) * rh;
+      F = interior_face(K, E, h, rh, s_mue[hL], s_mue[hR], s_de[hL], s_de[hR], s_ni[hL],
+                        s_ni[hR], s_ne[hL], s_ne[hR], s_nt[hL], s_nt[hR], s_nm[hL], s_nm[hR]);
     }
     s_f[0][f] = F.i;
     s_f[1][f] = F.e;
@@ -312,28 +312,29 @@ __device__ __forceinline__ void stage_body(const StageArgs& A, const FsPhys& K, 
   double tmin = 1e300;
   if (valid) {
     int fB = ty * kTX + tx, fT = fB + kTX;
-    double dly = G.dly[cj];
+    double rdly = G.rdly[cj];
     double dfy[4];
     if (G.cyl) {
       double rj = G.r[cj + 1], rp = G.r[cj + 2], rm = G.r[cj];
+      double w = rdly / rj;
 #pragma unroll
       for (int s = 0; s < 4; s++)
-        dfy[s] = (s_f[s][fT] * (rp + rj) / 2.0 - s_f[s][fB] * (rj + rm) / 2.0) / dly / rj;
+        dfy[s] = (s_f[s][fT] * (rp + rj) / 2.0 - s_f[s][fB] * (rj + rm) / 2.0) * w;
     } else {
 #pragma unroll
-      for (int s = 0; s < 4; s++) dfy[s] = (s_f[s][fT] - s_f[s][fB]) / dly;
+      for (int s = 0; s < 4; s++) dfy[s] = (s_f[s][fT] - s_f[s][fB]) * rdly;
     }
     int typy = cj == 0 ? -1 : (cj == by - 1 ? 1 : 0);
     double eyB, eyT;
     if (typy == -1) {
       eyB = 0.0;
-      eyT = -(s_ph[hc + kHX] - s_ph[hc]) / G.dy[cj + 1];
+      eyT = -(s_ph[hc + kHX] - s_ph[hc]) * G.rdy[cj + 1];
     } else if (typy == 1) {
-      eyB = -(s_ph[hc] - s_ph[hc - kHX]) / G.dy[cj];
+      eyB = -(s_ph[hc] - s_ph[hc - kHX]) * G.rdy[cj];
       eyT = 0.0;
     } else {
-      eyB = -(s_ph[hc] - s_ph[hc - kHX]) / G.dy[cj];
-      eyT = -(s_ph[hc + kHX] - s_ph[hc]) / G.dy[cj + 1];
+      eyB = -(s_ph[hc] - s_ph[hc - kHX]) * G.rdy[cj];
+      eyT = -(s_ph[hc + kHX] - s_ph[hc]) * G.rdy[cj + 1];
     }
     double feY = (s_f[1][fT] + s_f[1][fB]) / 2.0;
     double eyA = (eyB + eyT) / 2.0;
@@ -348,7 +349,7 @@ __device__ __forceinline__ void stage_body(const StageArgs& A, const FsPhys& K, 
     double term_sm = R.k_ex * ninf * ne - R.k_si * nm * ne - R.k_sc * nm * ne - K.k_r * nm * ne -
                      K.kmp2 * nm2 - K.k2q_ninf * nm - K.k3q_ninf2 * nm;
     double st1 = -(feX * exA + feY * eyA);
-    double st2 = -(ne * R.nu * K.me / K.mi * (te / 1.5 - K.kTe));
+    double st2 = -(ne * R.nu * K.me_mi * (te * (1.0 / 1.5) - K.kTe));
     double st3 = -(K.H_ir * R.k_ir * ninf * ne) - K.H_si * R.k_si * nm * ne -
                  K.H_ex * R.k_ex * ninf * ne;
     double kv[4];
@@ -370,10 +371,10 @@ __device__ __forceinline__ void stage_body(const StageArgs& A, const FsPhys& K, 
         double n3 = A.n3[s][m], kk = kv[s], nw;
         double nmin = s == 2 ? K.nmin_t : K.n_zero;
         if (stage == 1) {
-          nw = n3 + kk * dt / 3.0;
+          nw = n3 + kk * dt * (1.0 / 3.0);
           A.k1[s][m] = kk;
         } else if (stage == 2) {
-          nw = n3 + dt * (A.k1[s][m] + kk) / 6.0;
+          nw = n3 + dt * (A.k1[s][m] + kk) * (1.0 / 6.0);
         } else if (stage == 3) {
           nw = n3 + dt * (A.k1[s][m] + kk * 3.0) / 8.0;
           A.k3[s][m] = kk;
@@ -382,8 +383,9 @@ __device__ __forceinline__ void stage_body(const StageArgs& A, const FsPhys& K, 
           A.k4[s][m] = kk;
         } else {
           double k1 = A.k1[s][m], k3 = A.k3[s][m], k4 = A.k4[s][m];
-          nw = n3 + dt * (k1 + k4 * 4.0 + kk) / 6.0;
-          double e = fabs(dt * (k1 * 2.0 / 30.0 - k3 * 3.0 / 10.0 + k4 * 4.0 / 15.0 - kk / 30.0));
+          nw = n3 + dt * (k1 + k4 * 4.0 + kk) * (1.0 / 6.0);
+          double e = fabs(dt * (k1 * (2.0 / 30.0) - k3 * (3.0 / 10.0) + k4 * (4.0 / 15.0) -
+                                kk * (1.0 / 30.0)));
           double q = e / (1e-3 + 1e-3 * fabs(n3));
           acc[s] = q * q;
         }
@@ -2042,7 +2044,7 @@ __global__ void k_eval_flux(FsPhys K, const double* a, double* out, int n, int d
   const double* v = a + 7 * (size_t)m;
   double f, df;
   sg_flux_dfdph(K, v[0], v[1], v[2], v[3], v[4], v[5], v[6], f, df);
-  out[m] = dfdph ? df : sg_flux(K, v[0], v[1], v[2], v[3], v[4], v[5], v[6]);
+  out[m] = dfdph ? df : sg_flux(K, v[0], v[1], 1.0 / v[1], v[2], v[3], v[4], v[5], v[6]);
 }
 
 // ---------------------------------------------------------------------------

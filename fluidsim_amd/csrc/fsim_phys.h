@@ -1,10 +1,17 @@
 // fsim_phys.h -- physical constants and per-cell / per-face physics of the
 // fluidsim plasma-transport path, as HIP device functions for gfx950.
 //
-// Every function keeps the reference's operation order (ptcl_props.F90,
-// lapl_lib.F90) so results agree with a left-to-right Fortran evaluation up to
-// the ulp differences of the device exp/log; the library is compiled with
-// -ffp-contract=off so no FMA contraction reorders the roundings.
+// The functions keep the reference's operation order (ptcl_props.F90,
+// lapl_lib.F90), compiled with -ffp-contract=off (no FMA contraction), except
+// where the stage kernel trades a division or a transcendental for a
+// multiplication, each a difference of a few ulp from a left-to-right Fortran
+// evaluation: the rate fits multiply by precomputed products of their unit
+// conversions, beta uses exp10 for 10**y, and the stage kernel multiplies by
+// reciprocal grid spacings and shares one SG exponential between electrons and
+// electron energy (their exponent arguments are equal up to rounding:
+// mut/Dt = mue/De).  The Poisson assembly and the mobility/diffusion fits keep
+// the reference's order exactly: the assembled residual is a small difference
+// of large terms, where ulp changes in E or in the mobility show at 1e-9.
 //
 // Hoisting (bit-identical to the reference's repeated calls, because the same
 // expression on the same operands gives the same value):
@@ -37,6 +44,13 @@ struct FsPhys {
   // host-libm logs of the clamp bounds that bind in the rate fits
   double log_kex_lo, log_kir_lo, log_s_lo, log_s_hi;
   double exp_m100;    // exp(-100), the SG exponent clamp (ptcl_props.F90:159)
+  // products of the rate fits' unit conversions, formed once (the fits
+  // multiply exp(poly) by them instead of dividing by x0^3, 1e6 each time)
+  double c_rate;      // t0 / x0^3              (k_ex, k_ir)
+  double c_rate6;     // t0 / (1e6 x0^3)        (k_sc, k_si)
+  double c_nu;        // ninf * t0 / x0^3       (nu)
+  double c_beta;      // 1e-13 * t0 / x0^3      (beta)
+  double me_mi;       // me / mi
 };
 
 // Host: fill FsPhys for runtime pressure and gamma (props.F90:24-44,
@@ -134,7 +148,7 @@ __device__ __forceinline__ double kex_of_x(const FsPhys& K, double x) {
   else
     s = F32(-150.266369415) + F32(158.67012941) * x + F32(-85.5163470769) * x2 +
         F32(23.0655081891) * x3 + F32(-3.0908688844) * x4 + F32(0.16403226902) * x5;
-  return exp(s) * K.t0 / K.x0_3;
+  return exp(s) * K.c_rate;
 }
 
 // get_k_ir (ptcl_props.F90:347-374); x = log(max(1.37, min(157, T*ph0)))
@@ -147,7 +161,7 @@ __device__ __forceinline__ double kir_of_x(const FsPhys& K, double x) {
   else
     s = F32(-168.925304837) + F32(177.113452524) * x + F32(-92.3047166553) * x2 +
         F32(24.1814225482) * x3 + F32(-3.15463258104) * x4 + F32(0.16331188192) * x5;
-  return exp(s) * K.t0 / K.x0_3;
+  return exp(s) * K.c_rate;
 }
 
 // get_nu (ptcl_props.F90:376-395).  The reference's real exponents x**3. ..
@@ -159,7 +173,7 @@ __device__ __forceinline__ double nu_of_x(const FsPhys& K, double x) {
              F32(0.129397015353) * x3 + F32(-0.0414865809044) * x4 +
              F32(-0.0582934303409) * x5 + F32(0.0309832277826) * x6 +
              F32(-0.00542014733763) * x7 + F32(0.000325615321708) * x8;
-  return exp(s) / K.x0_3 * K.ninf * K.t0;
+  return exp(s) * K.c_nu;
 }
 
 // get_k_sc / get_k_si (ptcl_props.F90:397-429); x = log(min(16, max(0.5, T*ph0)))
@@ -167,20 +181,20 @@ __device__ __forceinline__ double ksc_of_x(const FsPhys& K, double x) {
   double x2 = x * x, x3 = x2 * x, x4 = x2 * x2, x5 = x2 * x3;
   double s = F32(-21.4827864151) + F32(0.457356923276) * x + F32(-0.555439231606) * x2 +
              F32(1.27257798891) * x3 + F32(-0.67840685073) * x4 + F32(0.10591014464) * x5;
-  return exp(s) / 1.0e6 / K.x0_3 * K.t0;
+  return exp(s) * K.c_rate6;
 }
 __device__ __forceinline__ double ksi_of_x(const FsPhys& K, double x) {
   double x2 = x * x, x3 = x2 * x, x4 = x2 * x2;
   double s = F32(-43.1347385848) + F32(43.9905424566) * x + F32(-28.1169537586) * x2 +
              F32(8.28853856817) * x3 + F32(-0.931626144207) * x4;
-  return exp(s) / 1.0e6 / K.x0_3 * K.t0;
+  return exp(s) * K.c_rate6;
 }
 
 // get_beta (ptcl_props.F90:182-192), v = T*ph0
 __device__ __forceinline__ double beta_of_v(const FsPhys& K, double v) {
-  double w = v / F32(1.5) * 11.604505e3;
+  double w = v * (11.604505e3 / F32(1.5));
   double x = log10(w < 5e4 ? w : 5e4);
-  return pow(10.0, 0.944 - 0.665 * (x - F32(2.48))) * 1e-13 * K.t0 / K.x0_3;
+  return exp10(0.944 - 0.665 * (x - F32(2.48))) * K.c_beta;
 }
 
 // The six reaction-rate fits at one cell from v = T*ph0 and xa = fit_x(v).
@@ -203,15 +217,36 @@ __device__ __forceinline__ Rates cell_rates(const FsPhys& K, double v, double xa
 
 // Scharfetter-Gummel face flux (ptcl_props.F90:142-165); qs = +1 ions, -1
 // electrons/energy.  One exponential: exp(max(-|arg|, -100)).
-__device__ __forceinline__ double sg_flux(const FsPhys& K, double E, double h, double qs,
-                                          double mu, double D, double nl, double nr) {
+// rh = 1/h.
+__device__ __forceinline__ double sg_flux(const FsPhys& K, double E, double h, double rh,
+                                          double qs, double mu, double D, double nl, double nr) {
   double v = qs * mu * E;
   double arg = v * h / D;
-  if (fabs(qs * E) < K.ftol) return D * (nl - nr) / h;
+  if (fabs(qs * E) < K.ftol) return D * (nl - nr) * rh;
   double na = arg > 0 ? -arg : arg;
   double ex = na < -100.0 ? K.exp_m100 : exp(na);
   double num = arg > 0 ? (nl - nr * ex) : (nr - nl * ex);
   return v * num / (1.0 - ex);
+}
+
+// Electron and electron-energy fluxes of one face (qs = -1; mut = c53 mue,
+// Dt = c53 De): one exponential of the electron argument for both.
+__device__ __forceinline__ void sg_flux_et(const FsPhys& K, double E, double h, double rh,
+                                           double mue, double de, double mut, double dt,
+                                           double neL, double neR, double ntL, double ntR,
+                                           double& fe, double& ft) {
+  if (fabs(E) < K.ftol) {
+    fe = de * (neL - neR) * rh;
+    ft = dt * (ntL - ntR) * rh;
+    return;
+  }
+  const double ve = -(mue * E), vt = -(mut * E);
+  const double arg = ve * h / de;
+  const double na = arg > 0 ? -arg : arg;
+  const double ex = na < -100.0 ? K.exp_m100 : exp(na);
+  const double om = 1.0 / (1.0 - ex);
+  fe = ve * (arg > 0 ? (neL - neR * ex) : (neR - neL * ex)) * om;
+  ft = vt * (arg > 0 ? (ntL - ntR * ex) : (ntR - ntL * ex)) * om;
 }
 
 // SG flux and its derivative w.r.t. the drift velocity (dFdPh,
