@@ -439,8 +439,8 @@ MgPcgArgs mgpcg_args(fs_handle* h, int it) {
 
 // The first level run by the one-workgroup kernel: at most kMgLdsCells cells
 // (padded, at most kMgLdsR), the next at most 1024, at most kMgLdsLevels
-// levels from there down, padded iterates within kMgLdsX doubles; never level
-// 0 (its kernels carry the PCG reductions).
+// levels from there down, padded iterates within kMgLdsX doubles (those below
+// it within kMgSetupX); never level 0 (its kernels carry the PCG reductions).
 size_t mg_lds_first(const fs_handle* h) {
   const size_t L = h->mg.size();
   auto cells = [&](size_t q) { return (size_t)h->mg[q].nx * h->mg[q].ny; };
@@ -449,7 +449,8 @@ size_t mg_lds_first(const fs_handle* h) {
     for (size_t q = f; q < L; q++) x += (size_t)(h->mg[q].nx + 2) * (h->mg[q].ny + 2);
     const size_t padded = (size_t)(h->mg[f].nx + 2) * (h->mg[f].ny + 2);
     if (cells(f) <= (size_t)kMgLdsCells && cells(f + 1) <= 1024 &&
-        L - f <= (size_t)kMgLdsLevels && x <= (size_t)kMgLdsX && padded <= (size_t)kMgLdsR)
+        L - f <= (size_t)kMgLdsLevels && x <= (size_t)kMgLdsX && padded <= (size_t)kMgLdsR &&
+        x - padded <= (size_t)kMgSetupX)
       return f;
   }
   return L - 1;
@@ -647,7 +648,7 @@ int capture_iteration(fs_handle* h, hipGraphExec_t* out, int (*launch)(fs_handle
 // MG-PCG set-up: Galerkin hierarchy, coarsest inverse, the start (x = 0, or
 // with `warm` the x in place), the initial V-cycle and iteration 0.
 int mg_prologue(fs_handle* h, bool warm) {
-  for (size_t l = 0; l + 1 < h->mg.size(); l++) launch_mg_coarsen(mg_args(h, l), h->st);
+  for (size_t l = 0; l < h->mg_first; l++) launch_mg_coarsen(mg_args(h, l), h->st);
   launch_mg_coarse_setup(cycle_args(h, -1), h->st);
   MgPcgArgs ia = mgpcg_args(h, 1);
   ia.part_out = h->part(0);
@@ -1186,7 +1187,7 @@ int ens_maps(fs_ensemble* E) {
     std::vector<int> co(n, 0), dn(n, 0);
     for (int i = 0; i < n; i++) {
       fs_handle* h = E->m[i];
-      if (l + 1 < (int)h->mg.size()) {
+      if (l < (int)h->mg_first) {  // the set-up kernel coarsens the levels below
         const MgLevel& C = h->mg[l + 1];
         co[i] = (C.nx * C.ny + 255) / 256;
       }
@@ -1299,7 +1300,7 @@ int ens_round(fs_ensemble* E, std::vector<int>& left) {
   if (!npois) return 0;
   if (ens_upload_round(E)) return FS_E_HIP;
   launch_ens_lapl(E->t_lapl, E->t_phys, with_mask(E->m_stage, r.pois), EnsSel{r.sel, 1, 0}, E->st);
-  for (int l = 0; l + 1 < E->maxL; l++)
+  for (int l = 0; l < E->maxF; l++)
     launch_ens_mg_coarsen(E->t_coarsen, with_mask(E->m_coarsen[l], r.pois),
                           EnsSel{nullptr, kMgMaxLevels, l}, E->st);
   launch_ens_mg_coarse_setup(E->t_cc, with_mask(E->m_one, r.pois), EnsSel{nullptr, 4, 0}, E->st);
@@ -1942,7 +1943,7 @@ int fs_time_kernel(fs_handle* h, int32_t which, int32_t n, double* avg_ms) {
   }
   if (which == 13 || which == 14 || which == 17 || which == 18 || level_id) {
     // a consistent multigrid-PCG state: start, the initial cycle, iterations 0 and 1
-    for (size_t l = 0; l + 1 < h->mg.size(); l++) launch_mg_coarsen(mg_args(h, l), h->st);
+    for (size_t l = 0; l < h->mg_first; l++) launch_mg_coarsen(mg_args(h, l), h->st);
     launch_mg_coarse_setup(cycle_args(h, -1), h->st);
     MgPcgArgs ia = mgpcg_args(h, 1);
     ia.part_out = h->part(0);
@@ -2160,7 +2161,7 @@ int fs_learn_harvest(fs_learn* n, fs_handle* h, int32_t* added) {
   const int before = n->net.count;
   HIP_OK(hipStreamSynchronize(n->net.st));
   launch_assembly(h);
-  for (size_t l = 0; l + 1 < h->mg.size(); l++) launch_mg_coarsen(mg_args(h, l), h->st);
+  for (size_t l = 0; l < h->mg_first; l++) launch_mg_coarsen(mg_args(h, l), h->st);
   launch_mg_coarse_setup(cycle_args(h, -1), h->st);
   MgPcgArgs ia = mgpcg_args(h, 1);
   ia.part_out = h->part(0);

@@ -175,6 +175,7 @@ constexpr int kMgLdsCells = 4096;  // cells of the finest of those (the others <
 constexpr int kMgLdsC0 = kMgLdsCells / 1024;  // its cells per thread
 constexpr int kMgLdsX = 6144;      // LDS doubles for their padded iterates
 constexpr int kMgLdsR = 4608;      // LDS doubles for the padded residual of the finest
+constexpr int kMgSetupX = 2048;    // LDS doubles per coefficient for the levels below it
 constexpr int kMgLdsPairs = kMgLdsCells / 2048;  // horizontal cell pairs per thread on it
 // packed FP32 rows: 5 values per cell, 2 kMgLdsPairs cells of the finest level
 // and one of each other non-coarsest level per thread, 1024 threads

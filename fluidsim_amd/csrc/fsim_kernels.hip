@@ -1076,43 +1076,59 @@ __device__ __forceinline__ size_t mg_at(const MgLevel& L, int i, int j) {
   return (size_t)(j + 1) * L.P + (i + 1);
 }
 
+// The Galerkin coefficients of coarse cell (I, J) from fine planes d, cx, cy
+// (padded, row pitch P, nx x ny cells); coarse size NX x NY.
+struct Coarse3 {
+  double d, cx, cy;
+};
+__device__ __forceinline__ Coarse3 coarsen_cell(const double* fd, const double* fcx,
+                                                const double* fcy, int P, int nx, int ny, int NX,
+                                                int NY, int I, int J) {
+  auto at = [&](int i, int j) { return (size_t)(j + 1) * P + (i + 1); };
+  int i0 = 2 * I, j0 = 2 * J;
+  bool hx = i0 + 1 < nx, hy = j0 + 1 < ny;
+  double d = fd[at(i0, j0)];
+  double internal = 0.0;
+  if (hx) {
+    d += fd[at(i0 + 1, j0)];
+    internal += fcx[at(i0, j0)];
+  }
+  if (hy) {
+    d += fd[at(i0, j0 + 1)];
+    internal += fcy[at(i0, j0)];
+  }
+  if (hx && hy) {
+    d += fd[at(i0 + 1, j0 + 1)];
+    internal += fcx[at(i0, j0 + 1)] + fcy[at(i0 + 1, j0)];
+  }
+  Coarse3 c;
+  c.d = d - 2.0 * internal;
+  // faces leaving the aggregate to the right / upwards (zero at the domain edge)
+  int ir = hx ? i0 + 1 : i0, ju = hy ? j0 + 1 : j0;
+  c.cx = 0.0;
+  c.cy = 0.0;
+  if (I + 1 < NX) {
+    c.cx = fcx[at(ir, j0)];
+    if (hy) c.cx += fcx[at(ir, j0 + 1)];
+  }
+  if (J + 1 < NY) {
+    c.cy = fcy[at(i0, ju)];
+    if (hx) c.cy += fcy[at(i0 + 1, ju)];
+  }
+  return c;
+}
+
 template <int kInstance>
 __device__ __forceinline__ void mg_coarsen_body(const MgArgs& A, int bid, int nb) {
   const MgLevel &F = A.F, &C = A.C;
   int n = bid * blockDim.x + threadIdx.x;
   if (n >= C.nx * C.ny) return;
   int I = n % C.nx, J = n / C.nx;
-  int i0 = 2 * I, j0 = 2 * J;
-  bool hx = i0 + 1 < F.nx, hy = j0 + 1 < F.ny;
-  double d = F.d[mg_at(F, i0, j0)];
-  double internal = 0.0;
-  if (hx) {
-    d += F.d[mg_at(F, i0 + 1, j0)];
-    internal += F.cx[mg_at(F, i0, j0)];
-  }
-  if (hy) {
-    d += F.d[mg_at(F, i0, j0 + 1)];
-    internal += F.cy[mg_at(F, i0, j0)];
-  }
-  if (hx && hy) {
-    d += F.d[mg_at(F, i0 + 1, j0 + 1)];
-    internal += F.cx[mg_at(F, i0, j0 + 1)] + F.cy[mg_at(F, i0 + 1, j0)];
-  }
+  const Coarse3 c = coarsen_cell(F.d, F.cx, F.cy, F.P, F.nx, F.ny, C.nx, C.ny, I, J);
   size_t m = mg_at(C, I, J);
-  C.d[m] = d - 2.0 * internal;
-  // faces leaving the aggregate to the right / upwards (zero at the domain edge)
-  int ir = hx ? i0 + 1 : i0, ju = hy ? j0 + 1 : j0;
-  double cx = 0.0, cy = 0.0;
-  if (I + 1 < C.nx) {
-    cx = F.cx[mg_at(F, ir, j0)];
-    if (hy) cx += F.cx[mg_at(F, ir, j0 + 1)];
-  }
-  if (J + 1 < C.ny) {
-    cy = F.cy[mg_at(F, i0, ju)];
-    if (hx) cy += F.cy[mg_at(F, i0 + 1, ju)];
-  }
-  C.cx[m] = cx;
-  C.cy[m] = cy;
+  C.d[m] = c.d;
+  C.cx[m] = c.cx;
+  C.cy[m] = c.cy;
 }
 
 __global__ void __launch_bounds__(256) k_mg_coarsen(MgArgs A) { mg_coarsen_body<0>(A, blockIdx.x, gridDim.x); }
@@ -1465,6 +1481,8 @@ template <int H>
 __global__ void __launch_bounds__(kNT) k_mg_up(MgArgs A) { mg_up_body<0, H>(A, blockIdx.x, gridDim.x); }
 
 // Once per solve, for the one-workgroup levels f .. L-1 (1024 threads):
+//  * levels f+1 .. L-1 are coarsened from level f (written by the tile
+//    kernels) in LDS, and also stored;
 //  * the coarsest level (<= kMgBottomMax cells) is inverted by Gauss-Jordan
 //    elimination (SPD, so no pivoting): thread (row, col) < kB^2 keeps entry
 //    (row, col) in a register; each pivot step publishes the pivot row and
@@ -1478,20 +1496,53 @@ template <int kInstance>
 __device__ __forceinline__ void mg_coarse_setup_body(const MgCycleArgs& A) {
   constexpr int kB = kMgBottomMax;
   __shared__ double s_prow[2][kB], s_pcol[2][kB];
+  // Galerkin coefficients of levels f+1 .. L-1, coarsened here from level f
+  // (whose tile kernel wrote it) and kept in LDS (padded planes at s_off[q])
+  __shared__ double s_c[3][kMgSetupX];
+  __shared__ int s_off[kMgLdsLevels];
   const int t = threadIdx.x;
+  if (t == 0) {
+    int o = 0;
+    for (int q = 1; q < A.nq; q++) {
+      s_off[q] = o;
+      o += (A.lv[q].nx + 2) * (A.lv[q].ny + 2);
+    }
+  }
+  for (int k = t; k < kMgSetupX; k += 1024) s_c[0][k] = s_c[1][k] = s_c[2][k] = 0.0;
+  __syncthreads();
+  auto lvd = [&](int q) -> const double* { return q == 0 ? A.lv[0].d : s_c[0] + s_off[q]; };
+  auto lvx = [&](int q) -> const double* { return q == 0 ? A.lv[0].cx : s_c[1] + s_off[q]; };
+  auto lvy = [&](int q) -> const double* { return q == 0 ? A.lv[0].cy : s_c[2] + s_off[q]; };
+  for (int q = 0; q + 1 < A.nq; q++) {
+    const MgLevel &F = A.lv[q], &C = A.lv[q + 1];
+    for (int n = t; n < C.nx * C.ny; n += 1024) {
+      const int I = n % C.nx, J = n / C.nx;
+      const Coarse3 c = coarsen_cell(lvd(q), lvx(q), lvy(q), F.nx + 2, F.nx, F.ny, C.nx, C.ny, I, J);
+      const size_t m = mg_at(C, I, J);
+      s_c[0][s_off[q + 1] + m] = c.d;
+      s_c[1][s_off[q + 1] + m] = c.cx;
+      s_c[2][s_off[q + 1] + m] = c.cy;
+      C.d[m] = c.d;
+      C.cx[m] = c.cx;
+      C.cy[m] = c.cy;
+    }
+    __syncthreads();
+  }
   {
-    const MgLevel& L = A.lv[A.nq - 1];
+    const int qb = A.nq - 1;
+    const MgLevel& L = A.lv[qb];
+    const double *Ld = lvd(qb), *Lx = lvx(qb), *Ly = lvy(qb);
     const int n = L.nx * L.ny, nx = L.nx;
     const bool own = t < kB * kB;
     const int row = t % kB, col = t / kB;
     double a = 0.0;
     if (own && row < n && col < n) {
       const size_t m = mg_at(L, row % nx, row / nx);
-      if (col == row) a = L.d[m];
-      else if (col == row - 1 && row % nx > 0) a = -L.cx[m - 1];
-      else if (col == row + 1 && row % nx + 1 < nx) a = -L.cx[m];
-      else if (col == row - nx) a = -L.cy[m - L.P];
-      else if (col == row + nx) a = -L.cy[m];
+      if (col == row) a = Ld[m];
+      else if (col == row - 1 && row % nx > 0) a = -Lx[m - 1];
+      else if (col == row + 1 && row % nx + 1 < nx) a = -Lx[m];
+      else if (col == row - nx) a = -Ly[m - L.P];
+      else if (col == row + nx) a = -Ly[m];
     }
     for (int p = 0; p < n; p++) {
       const int buf = p & 1;
@@ -1507,14 +1558,15 @@ __device__ __forceinline__ void mg_coarse_setup_body(const MgCycleArgs& A) {
     if (own && row < n && col < n) A.inv[(size_t)col * n + row] = a;
   }
   if (A.nq < 2) return;
-  auto put = [&](int slot, int k, const MgLevel& L, bool valid) {
+  auto put = [&](int slot, int k, int lq, bool valid) {
     float* q = A.pk + (size_t)slot * 5 * 1024 + t;
-    const int W = L.nx + 2;
-    q[0] = valid ? (float)(1.0 / L.d[k]) : 1.0f;
-    q[1024] = valid ? (float)L.cx[k - 1] : 0.0f;
-    q[2048] = valid ? (float)L.cx[k] : 0.0f;
-    q[3072] = valid ? (float)L.cy[k - W] : 0.0f;
-    q[4096] = valid ? (float)L.cy[k] : 0.0f;
+    const int W = A.lv[lq].nx + 2;
+    const double *Ld = lvd(lq), *Lx = lvx(lq), *Ly = lvy(lq);
+    q[0] = valid ? (float)(1.0 / Ld[k]) : 1.0f;
+    q[1024] = valid ? (float)Lx[k - 1] : 0.0f;
+    q[2048] = valid ? (float)Lx[k] : 0.0f;
+    q[3072] = valid ? (float)Ly[k - W] : 0.0f;
+    q[4096] = valid ? (float)Ly[k] : 0.0f;
   };
   {
     const MgLevel& L = A.lv[0];
@@ -1529,14 +1581,14 @@ __device__ __forceinline__ void mg_coarse_setup_body(const MgCycleArgs& A) {
           valid = i < L.nx;
           k = (j + 1) * W + i + 1;
         }
-        put(p * 2 + c, k, L, valid);
+        put(p * 2 + c, k, 0, valid);
       }
   }
   for (int q = 1; q + 1 < A.nq; q++) {
     const MgLevel& L = A.lv[q];
     const bool valid = t < L.nx * L.ny;
     const int k = valid ? (t / L.nx + 1) * (L.nx + 2) + t % L.nx + 1 : 0;
-    put(2 * kMgLdsPairs + q - 1, k, L, valid);
+    put(2 * kMgLdsPairs + q - 1, k, q, valid);
   }
 }
 
