@@ -437,8 +437,8 @@ MgPcgArgs mgpcg_args(fs_handle* h, int it) {
   return a;
 }
 
-// The first level run by the one-workgroup kernel: at most kMgLdsCells cells
-// (padded, at most kMgLdsR), the next at most 1024, at most kMgLdsLevels
+// The first level run by the one-workgroup kernel: at most kMgLdsCells cells,
+// the next at most 1024 (padded, at most kMgLdsR), at most kMgLdsLevels
 // levels from there down, padded iterates within kMgLdsX doubles (those below
 // it within kMgSetupX); never level 0 (its kernels carry the PCG reductions).
 size_t mg_lds_first(const fs_handle* h) {
@@ -448,8 +448,9 @@ size_t mg_lds_first(const fs_handle* h) {
     size_t x = 0;
     for (size_t q = f; q < L; q++) x += (size_t)(h->mg[q].nx + 2) * (h->mg[q].ny + 2);
     const size_t padded = (size_t)(h->mg[f].nx + 2) * (h->mg[f].ny + 2);
+    const size_t padded1 = (size_t)(h->mg[f + 1].nx + 2) * (h->mg[f + 1].ny + 2);
     if (cells(f) <= (size_t)kMgLdsCells && cells(f + 1) <= 1024 &&
-        L - f <= (size_t)kMgLdsLevels && x <= (size_t)kMgLdsX && padded <= (size_t)kMgLdsR &&
+        L - f <= (size_t)kMgLdsLevels && x <= (size_t)kMgLdsX && padded1 <= (size_t)kMgLdsR &&
         x - padded <= (size_t)kMgSetupX)
       return f;
   }

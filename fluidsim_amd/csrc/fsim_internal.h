@@ -172,14 +172,13 @@ constexpr int kMgMaxLevels = 16;
 constexpr int kMgBottomMax = 16;   // cells of the coarsest level (dense inverse)
 constexpr int kMgLdsLevels = 5;    // levels run by the one-workgroup kernel
 constexpr int kMgLdsCells = 4096;  // cells of the finest of those (the others <= 1024)
-constexpr int kMgLdsC0 = kMgLdsCells / 1024;  // its cells per thread
 constexpr int kMgLdsX = 6144;      // LDS doubles for their padded iterates
-constexpr int kMgLdsR = 4608;      // LDS doubles for the padded residual of the finest
+constexpr int kMgLdsR = 4608;      // LDS doubles for the padded residuals of the others
 constexpr int kMgSetupX = 2048;    // LDS doubles per coefficient for the levels below it
-constexpr int kMgLdsPairs = kMgLdsCells / 2048;  // horizontal cell pairs per thread on it
-// packed FP32 rows: 5 values per cell, 2 kMgLdsPairs cells of the finest level
-// and one of each other non-coarsest level per thread, 1024 threads
-constexpr int kMgPackSlots = 2 * kMgLdsPairs + kMgLdsLevels - 2;
+// packed FP32 rows: 5 values per cell, the 4 cells of a 2x2 aggregate of the
+// finest level and one cell of each other non-coarsest level per thread, 1024
+// threads
+constexpr int kMgPackSlots = 4 + kMgLdsLevels - 2;
 constexpr size_t kMgPackFloats = (size_t)kMgPackSlots * 5 * 1024;
 
 struct MgCycleArgs {

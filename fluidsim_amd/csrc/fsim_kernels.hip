@@ -1568,27 +1568,22 @@ __device__ __forceinline__ void mg_coarse_setup_body(const MgCycleArgs& A) {
     q[3072] = valid ? (float)Ly[k - W] : 0.0f;
     q[4096] = valid ? (float)Ly[k] : 0.0f;
   };
-  {
-    const MgLevel& L = A.lv[0];
-    const int npr = (L.nx + 1) / 2, W = L.nx + 2;
-    for (int p = 0; p < kMgLdsPairs; p++)
-      for (int c = 0; c < 2; c++) {
-        const int pp = t + 1024 * p;
-        int k = 0;
-        bool valid = false;
-        if (pp < npr * L.ny) {
-          const int j = pp / npr, i = 2 * (pp % npr) + (c ^ (j & 1));
-          valid = i < L.nx;
-          k = (j + 1) * W + i + 1;
-        }
-        put(p * 2 + c, k, 0, valid);
-      }
+  {  // level f by aggregates: child c = a + 2b of cell t of level f+1 in slot c
+    const MgLevel &L = A.lv[0], &C = A.lv[1];
+    const int W = L.nx + 2;
+    const bool own = t < C.nx * C.ny;
+    const int I = own ? t % C.nx : 0, J = own ? t / C.nx : 0;
+    for (int c = 0; c < 4; c++) {
+      const int i = 2 * I + (c & 1), j = 2 * J + (c >> 1);
+      const bool valid = own && i < L.nx && j < L.ny;
+      put(c, valid ? (j + 1) * W + i + 1 : 0, 0, valid);
+    }
   }
   for (int q = 1; q + 1 < A.nq; q++) {
     const MgLevel& L = A.lv[q];
     const bool valid = t < L.nx * L.ny;
     const int k = valid ? (t / L.nx + 1) * (L.nx + 2) + t % L.nx + 1 : 0;
-    put(2 * kMgLdsPairs + q - 1, k, q, valid);
+    put(4 + q - 1, k, q, valid);
   }
 }
 
@@ -1597,59 +1592,63 @@ __global__ void __launch_bounds__(1024) k_mg_coarse_setup(MgCycleArgs A) {
 }
 
 // Levels f .. L-1 in one workgroup of 1024 threads (at most kMgLdsLevels;
-// level f has at most kMgLdsCells cells, the others at most 1024).  Iterates
-// and residuals live in LDS in the levels' padded (nx+2)(ny+2) layout, which
-// is also their layout in HBM; operator rows (the set-up kernel's FP32 table)
-// and right-hand sides stay in registers.  Level f is held as horizontal cell
-// pairs (thread t owns pairs
-// t + 1024 p, p < kMgLdsPairs), one cell of each colour per pair, so each
-// half-sweep keeps every lane busy; the coarser levels hold one cell per
-// thread.  The sweeps multiply by the inverse diagonal.  The operator rows
-// are FP32 values but the arithmetic is FP64: the cycle must stay a symmetric
+// level f has at most kMgLdsCells cells, the next at most 1024).  Iterates and
+// residuals live in LDS in the levels' padded (nx+2)(ny+2) layout, which is
+// also their layout in HBM; operator rows (the set-up kernel's FP32 table) and
+// right-hand sides stay in registers.  Level f is held by aggregates: thread t
+// owns the 2x2 children of cell t of level f+1, two of each colour, with their
+// iterates in registers as well, so half of each update's neighbours are its
+// own registers, and the restriction to and prolongation from level f+1 stay
+// in the thread (it holds cell t there); the coarser levels hold one cell per
+// thread.  The sweeps multiply by the inverse diagonal.  The operator rows are
+// FP32 values but the arithmetic is FP64: the cycle must stay a symmetric
 // operator to rounding level, which the CG recurrences (fused, Chronopoulos-
 // Gear) rely on; with FP32 arithmetic here, warm-started solves broke down near
 // convergence.  Same cycle as the tile kernels; level f's correction is
 // written back for the levels above.
-struct PairRows {
-  double dinv[2][kMgLdsPairs], l[2][kMgLdsPairs], r[2][kMgLdsPairs], b[2][kMgLdsPairs],
-      t[2][kMgLdsPairs], rhs[2][kMgLdsPairs];
-  int ij[2][kMgLdsPairs];  // i | j << 16 of the cell of colour c, or -1
+//
+// Children c = a + 2b at (2I + a, 2J + b): colour (a + b) & 1, i.e. 0 and 3
+// are red, 1 and 2 black.
+struct AggRows {
+  float dinv[4], l[4], r[4], b[4], t[4];  // held as the table's FP32 values
+  double rhs[4], x[4];
+  int k0;    // padded index of child 0, or -1 when thread t has no aggregate
+  int mask;  // bit c: child c lies inside the level
 };
-
-__device__ __forceinline__ int ij_k(int ij, int W) { return ((ij >> 16) + 1) * W + (ij & 0xffff) + 1; }
-__device__ __forceinline__ int ij_kc(int ij, int Wc) {
-  return ((ij >> 17) + 1) * Wc + ((ij & 0xffff) >> 1) + 1;
-}
 struct CellRows {
-  double dinv, l, r, b, t, rhs;
+  float dinv, l, r, b, t;
+  double rhs;
   int k;   // padded index of cell t, or -1
   int ij;  // i | j << 16
 };
 
-// Rows of level f (pairs) and of a coarser level q >= 1 (one cell) from the
-// packed FP32 table; right-hand sides of level f from its residual plane.
-__device__ __forceinline__ void lds_pair_rows(const MgLevel& L, PairRows& R, const float* pk, int t) {
-  const int npr = (L.nx + 1) / 2, W = L.nx + 2;
+// Rows of level f (aggregates) and of a coarser level q >= 1 (one cell) from
+// the packed FP32 table; right-hand sides of level f from its residual plane.
+__device__ __forceinline__ void lds_agg_rows(const MgLevel& L, const MgLevel& C, AggRows& R,
+                                             const float* pk, int t) {
+  const int W = L.nx + 2;
+  R.k0 = -1;
+  R.mask = 0;
+  int I = 0, J = 0;
+  if (t < C.nx * C.ny) {
+    I = t % C.nx;
+    J = t / C.nx;
+    R.k0 = (2 * J + 1) * W + 2 * I + 1;
+  }
 #pragma unroll
-  for (int p = 0; p < kMgLdsPairs; p++) {
-    const int pp = t + 1024 * p;
-#pragma unroll
-    for (int c = 0; c < 2; c++) {
-      const float* q = pk + (size_t)(p * 2 + c) * 5 * 1024 + t;
-      R.dinv[c][p] = q[0];
-      R.l[c][p] = q[1024];
-      R.r[c][p] = q[2048];
-      R.b[c][p] = q[3072];
-      R.t[c][p] = q[4096];
-      R.rhs[c][p] = 0.0;
-      R.ij[c][p] = -1;
-      if (pp < npr * L.ny) {
-        const int j = pp / npr, i = 2 * (pp % npr) + (c ^ (j & 1));
-        if (i < L.nx) {
-          R.rhs[c][p] = (double)L.r[(j + 1) * W + i + 1];
-          R.ij[c][p] = i | (j << 16);
-        }
-      }
+  for (int c = 0; c < 4; c++) {
+    const int a = c & 1, b = c >> 1;
+    const float* q = pk + (size_t)c * 5 * 1024 + t;
+    R.dinv[c] = q[0];
+    R.l[c] = q[1024];
+    R.r[c] = q[2048];
+    R.b[c] = q[3072];
+    R.t[c] = q[4096];
+    R.rhs[c] = 0.0;
+    R.x[c] = 0.0;
+    if (R.k0 >= 0 && 2 * I + a < L.nx && 2 * J + b < L.ny) {
+      R.mask |= 1 << c;
+      R.rhs[c] = L.r[R.k0 + a + b * W];
     }
   }
 }
@@ -1674,8 +1673,9 @@ __device__ __forceinline__ void lds_cell_rows(const MgLevel& L, CellRows& R, con
 
 // Position of cell t on the coarsest level (its rows are not needed).
 __device__ __forceinline__ void lds_cell_pos(const MgLevel& L, CellRows& R, int t) {
-  R.dinv = 1.0;
-  R.l = R.r = R.b = R.t = R.rhs = 0.0;
+  R.dinv = 1.0f;
+  R.l = R.r = R.b = R.t = 0.0f;
+  R.rhs = 0.0;
   R.k = -1;
   R.ij = 0;
   if (t < L.nx * L.ny) {
@@ -1694,44 +1694,112 @@ __device__ __forceinline__ double lds_res(double rhs, double l, double r, double
   return rhs - (x[k] / dinv - l * x[k - 1] - r * x[k + 1] - b * x[k - W] - t * x[k + W]);
 }
 
+// Neighbour values of child c (left, right, below, above): the other colour's
+// children of the same aggregate from registers, the rest from LDS.
+__device__ __forceinline__ void agg_nb(const AggRows& R, const double* x, int W, int c,
+                                       double& L, double& Rt, double& B, double& T) {
+  const int k = R.k0;
+  if (c == 0) {
+    L = x[k - 1];
+    Rt = R.x[1];
+    B = x[k - W];
+    T = R.x[2];
+  } else if (c == 1) {
+    L = R.x[0];
+    Rt = x[k + 2];
+    B = x[k + 1 - W];
+    T = R.x[3];
+  } else if (c == 2) {
+    L = x[k + W - 1];
+    Rt = R.x[3];
+    B = R.x[0];
+    T = x[k + 2 * W];
+  } else {
+    L = R.x[2];
+    Rt = x[k + W + 2];
+    B = R.x[1];
+    T = x[k + 2 * W + 1];
+  }
+}
+
+__device__ __forceinline__ int agg_k(const AggRows& R, int W, int c) {
+  return R.k0 + (c & 1) + (c >> 1) * W;
+}
+
+// One half-sweep of colour COL over the aggregates (children 0, 3 or 1, 2).
 template <int COL>
-__device__ __forceinline__ void pair_half(const PairRows& R, double* x, int W) {
+__device__ __forceinline__ void agg_half(AggRows& R, double* x, int W) {
+  constexpr int c0 = COL == 0 ? 0 : 1, c1 = COL == 0 ? 3 : 2;
+  double v[2];
 #pragma unroll
-  for (int p = 0; p < kMgLdsPairs; p++) {
-    const int k = ij_k(R.ij[COL][p], W);
-    if (R.ij[COL][p] >= 0)
-      x[k] = lds_gs(R.rhs[COL][p], R.l[COL][p], R.r[COL][p], R.b[COL][p], R.t[COL][p],
-                    R.dinv[COL][p], x, k, W);
+  for (int q = 0; q < 2; q++) {
+    const int c = q == 0 ? c0 : c1;
+    v[q] = R.x[c];
+    if (R.mask & (1 << c)) {
+      double L, Rt, B, T;
+      agg_nb(R, x, W, c, L, Rt, B, T);
+      v[q] = (R.rhs[c] + R.l[c] * L + R.r[c] * Rt + R.b[c] * B + R.t[c] * T) * R.dinv[c];
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 2; q++) {
+    const int c = q == 0 ? c0 : c1;
+    if (R.mask & (1 << c)) {
+      R.x[c] = v[q];
+      x[agg_k(R, W, c)] = v[q];
+    }
   }
   __syncthreads();
+}
+
+// Pre-smoothing from zero (red pointwise, then black, red, black); returns the
+// aggregate's residual sum, the right-hand side of cell t of level f+1.
+__device__ __forceinline__ double agg_down(AggRows& R, double* x, int W) {
+#pragma unroll
+  for (int c = 0; c < 4; c += 3)
+    if (R.mask & (1 << c)) {
+      R.x[c] = R.rhs[c] * R.dinv[c];
+      x[agg_k(R, W, c)] = R.x[c];
+    }
+  __syncthreads();
+  agg_half<1>(R, x, W);
+  agg_half<0>(R, x, W);
+  agg_half<1>(R, x, W);
+  double sum = 0.0;
+#pragma unroll
+  for (int c = 0; c < 4; c++)
+    if (R.mask & (1 << c)) {
+      double L, Rt, B, T;
+      agg_nb(R, x, W, c, L, Rt, B, T);
+      sum += R.rhs[c] - (R.x[c] / R.dinv[c] - R.l[c] * L - R.r[c] * Rt - R.b[c] * B - R.t[c] * T);
+    }
+  return sum;
+}
+
+// Prolongation of e (the correction of cell t of level f+1) with
+// over-correction, post-smoothing (black, red, black, red), and the result.
+__device__ __forceinline__ void agg_up(AggRows& R, double* x, int W, double e, double oc,
+                                       double* out) {
+#pragma unroll
+  for (int c = 0; c < 4; c++)
+    if (R.mask & (1 << c)) {
+      R.x[c] += oc * e;
+      x[agg_k(R, W, c)] = R.x[c];
+    }
+  __syncthreads();
+  agg_half<1>(R, x, W);
+  agg_half<0>(R, x, W);
+  agg_half<1>(R, x, W);
+  agg_half<0>(R, x, W);
+#pragma unroll
+  for (int c = 0; c < 4; c++)
+    if (R.mask & (1 << c)) out[agg_k(R, W, c)] = R.x[c];
 }
 
 template <int COL>
 __device__ __forceinline__ void cell_half(const CellRows& R, double* x, int W) {
   if (R.k >= 0 && (((R.ij & 0xffff) + (R.ij >> 16)) & 1) == COL)
     x[R.k] = lds_gs(R.rhs, R.l, R.r, R.b, R.t, R.dinv, x, R.k, W);
-  __syncthreads();
-}
-
-// Pre-smoothing from zero (red pointwise, then black, red, black) and the
-// residual into s_r.
-__device__ __forceinline__ void pair_down(const PairRows& R, double* x, double* s_r, int W) {
-#pragma unroll
-  for (int p = 0; p < kMgLdsPairs; p++)
-    if (R.ij[0][p] >= 0) x[ij_k(R.ij[0][p], W)] = R.rhs[0][p] * R.dinv[0][p];
-  __syncthreads();
-  pair_half<1>(R, x, W);
-  pair_half<0>(R, x, W);
-  pair_half<1>(R, x, W);
-#pragma unroll
-  for (int c = 0; c < 2; c++)
-#pragma unroll
-    for (int p = 0; p < kMgLdsPairs; p++) {
-      const int k = ij_k(R.ij[c][p], W);
-      if (R.ij[c][p] >= 0)
-        s_r[k] = lds_res(R.rhs[c][p], R.l[c][p], R.r[c][p], R.b[c][p], R.t[c][p], R.dinv[c][p],
-                         x, k, W);
-    }
   __syncthreads();
 }
 
@@ -1759,21 +1827,6 @@ __device__ __forceinline__ double lds_restrict(const MgLevel& L, const CellRows&
   return v;
 }
 
-// Prolongation with over-correction and post-smoothing (black, red, black, red).
-__device__ __forceinline__ void pair_up(const PairRows& R, double* x, const double* xc, int W,
-                                        int Wc, double oc) {
-#pragma unroll
-  for (int c = 0; c < 2; c++)
-#pragma unroll
-    for (int p = 0; p < kMgLdsPairs; p++)
-      if (R.ij[c][p] >= 0) x[ij_k(R.ij[c][p], W)] += oc * xc[ij_kc(R.ij[c][p], Wc)];
-  __syncthreads();
-  pair_half<1>(R, x, W);
-  pair_half<0>(R, x, W);
-  pair_half<1>(R, x, W);
-  pair_half<0>(R, x, W);
-}
-
 __device__ __forceinline__ void cell_up(const CellRows& R, double* x, const double* xc, int W,
                                         int Wc, double oc) {
   if (R.k >= 0) {
@@ -1798,14 +1851,14 @@ __device__ __forceinline__ void mg_coarse_cycle_body(const MgCycleArgs& A) {
   const int t = threadIdx.x;
   const int nq = A.nq;
   // operator rows first: their loads overlap the convergence test
-  PairRows R0;                    // level f
+  AggRows R0;                     // level f
   CellRows R1[kMgLdsLevels - 2];  // levels f+1 .. L-2
   CellRows Bc;                    // the coarsest level's cell t (its position only)
   if (nq > 1) {
-    lds_pair_rows(A.lv[0], R0, A.pk, t);
+    lds_agg_rows(A.lv[0], A.lv[1], R0, A.pk, t);
 #pragma unroll
     for (int q = 1; q < kMgLdsLevels - 1; q++)
-      if (q < nq - 1) lds_cell_rows(A.lv[q], R1[q - 1], A.pk, 2 * kMgLdsPairs + q - 1, t);
+      if (q < nq - 1) lds_cell_rows(A.lv[q], R1[q - 1], A.pk, 4 + q - 1, t);
   }
   lds_cell_pos(A.lv[nq - 1], Bc, t);
   if (A.check) {  // convergence test on the level-0 CG update's |r/V|^2 partials
@@ -1831,14 +1884,12 @@ __device__ __forceinline__ void mg_coarse_cycle_body(const MgCycleArgs& A) {
     if (t < nb) rb = (double)B.r[(t / B.nx + 1) * (B.nx + 2) + t % B.nx + 1];
   } else {
     __syncthreads();
-    // down: level f, then f+1 .. L-2, each restricted to the next
-    pair_down(R0, s_x + A.off[0], s_r, A.lv[0].nx + 2);
+    // down: level f (its restriction is cell t's own), then f+1 .. L-2
     {
-      const double v = lds_restrict(A.lv[0], nq == 2 ? Bc : R1[0], s_r);
-      if (nq == 2) rb = v;
-      else R1[0].rhs = v;
+      const double v = agg_down(R0, s_x + A.off[0], A.lv[0].nx + 2);
+      if (nq == 2) rb = t < nb ? v : 0.0;
+      else R1[0].rhs = R1[0].k >= 0 ? v : 0.0;
     }
-    __syncthreads();
 #pragma unroll
     for (int q = 1; q < kMgLdsLevels - 1; q++) {
       if (q + 1 >= nq) break;
@@ -1866,16 +1917,12 @@ __device__ __forceinline__ void mg_coarse_cycle_body(const MgCycleArgs& A) {
       cell_up(R1[q - 1], s_x + A.off[q], s_x + A.off[q + 1], A.lv[q].nx + 2, A.lv[q + 1].nx + 2,
               A.oc);
     }
+    // level f: the correction of cell t of level f+1 was last written by this
+    // thread (cell t is its own there)
     const MgLevel& L = A.lv[0];
-    pair_up(R0, s_x + A.off[0], s_x + A.off[1], L.nx + 2, A.lv[1].nx + 2, A.oc);
-#pragma unroll
-    for (int c = 0; c < 2; c++)
-#pragma unroll
-      for (int p = 0; p < kMgLdsPairs; p++)
-        if (R0.ij[c][p] >= 0) {
-          const int k = ij_k(R0.ij[c][p], L.nx + 2);
-          L.e[k] = (double)s_x[A.off[0] + k];
-        }
+    const double e = Bc.k >= 0 && nq == 2 ? s_x[A.off[1] + Bc.k]
+                     : (nq > 2 && R1[0].k >= 0 ? s_x[A.off[1] + R1[0].k] : 0.0);
+    agg_up(R0, s_x + A.off[0], L.nx + 2, e, A.oc, L.e);
     return;
   }
   // a single level: the coarsest, solved directly
