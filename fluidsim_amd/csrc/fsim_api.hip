@@ -110,6 +110,16 @@ void fs_phys_init(FsPhys* K, double p, double gam) {
   K->c_nu = K->ninf * K->t0 / K->x0_3;
   K->c_beta = 1e-13 * K->t0 / K->x0_3;
   K->me_mi = K->me / K->mi;
+  K->c_mue = K->x0 / K->ninf * K->t0 * K->ph0;
+  K->c_de = K->x0 / K->ninf * K->t0;
+  K->r_Di = 1.0 / K->Di;
+  {
+    const long double ln10 = logl(10.0L);
+    K->beta_a = (double)(ln10 * (0.944L + 0.665L * (long double)2.48f) -
+                         0.665L * logl(11.604505e3L / 1.5L));
+    const double x = log10(5e4);
+    K->beta_hi = exp10(0.944 - 0.665 * (x - (double)2.48f)) * K->c_beta;
+  }
 }
 
 void fs_phys_table(const FsPhys* K, double* o) {

@@ -28,6 +28,23 @@ constexpr int kNFX = (kTX + 1) * kTY, kNFY = kTX * (kTY + 1);
 constexpr int kNF = kNFX > kNFY ? kNFX : kNFY;
 constexpr int kNW = kNT / 64;
 
+// Device pointers held in argument blocks in device memory are generic
+// pointers, and accesses through them compile to flat instructions, whose
+// completion every LDS wait also waits for.  These casts make them global
+// accesses (all such buffers are device allocations).
+template <class T>
+__device__ __forceinline__ __attribute__((address_space(1))) T* to_global(T* p) {
+  return (__attribute__((address_space(1))) T*)p;
+}
+template <class T>
+__device__ __forceinline__ T gld(const T* p, size_t i) {
+  return to_global(p)[i];
+}
+template <class T>
+__device__ __forceinline__ void gst(T* p, size_t i, T v) {
+  to_global(p)[i] = v;
+}
+
 // Blocks are dispatched round-robin over the 8 XCDs; give each XCD a
 // contiguous band of tiles so that halo rows are shared inside one L2.
 __device__ __forceinline__ int xcd_tile(int b, int nb) {
@@ -164,7 +181,7 @@ __device__ __forceinline__ Flux4 interior_face(const FsPhys& K, double E, double
   double de = 5e-1 * (deL + deR);
   double mut = K.c53 * mue;
   double dt = K.c53 * de;
-  F.i = sg_flux(K, E, h, rh, 1.0, K.mui, K.Di, niL, niR);
+  F.i = sg_flux_ion(K, E, h, rh, niL, niR);
   sg_flux_et(K, E, h, rh, mue, de, mut, dt, neL, neR, ntL, ntR, F.e, F.t);
   F.m = -(K.Dm * (nmR - nmL) * rh);
   return F;
@@ -195,11 +212,11 @@ __device__ __forceinline__ void stage_body(const StageArgs& A, const FsPhys& K, 
     double vi = 0, ve = 0, vt = 0, vm = 0, vp = 0, te = 0, mue = 0, de = 0, xa = 0;
     if (ci <= bx && cj <= by) {
       size_t m = (size_t)(cj + 1) * P + (ci + 1);
-      vi = A.rd[0][m];
-      ve = A.rd[1][m];
-      vt = A.rd[2][m];
-      vm = A.rd[3][m];
-      vp = A.ph[m];
+      vi = gld(A.rd[0], m);
+      ve = gld(A.rd[1], m);
+      vt = gld(A.rd[2], m);
+      vm = gld(A.rd[3], m);
+      vp = gld(A.ph, m);
       if (ci >= 0 && ci < bx && cj >= 0 && cj < by) {
         if (vt >= 0.0 && ve >= 0.0) {
           te = vt / ve;  // get_Te, ptcl_props.F90:167-180
@@ -208,8 +225,8 @@ __device__ __forceinline__ void stage_body(const StageArgs& A, const FsPhys& K, 
           s_bad = 1;
         }
         xa = fit_x(te * K.ph0);
-        mue = mue_of_x(K, xa);
-        de = de_of_x(K, xa);
+        mue = mue_of_x_folded(K, xa);
+        de = de_of_x_folded(K, xa);
       }
     }
     s_ni[h] = vi;
@@ -222,30 +239,48 @@ __device__ __forceinline__ void stage_body(const StageArgs& A, const FsPhys& K, 
     s_de[h] = de;
     s_xa[h] = xa;
   }
+  // this cell's RK base and earlier slopes, loaded here so that their latency
+  // overlaps the flux phases
+  const int tx = threadIdx.x % kTX, ty = threadIdx.x / kTX;
+  const int ci = i0 + tx, cj = j0 + ty;
+  const bool valid = (ci < bx) && (cj < by);
+  const size_t mc = (size_t)(cj + 1) * P + (ci + 1);
+  const int stage = A.stage;
+  double pn3[4] = {0, 0, 0, 0}, pk1[4] = {0, 0, 0, 0}, pk3[4] = {0, 0, 0, 0},
+         pk4[4] = {0, 0, 0, 0};
+  if (valid && !A.rhs_only) {
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+      pn3[s] = gld(A.n3[s], mc);
+      if (stage >= 2) pk1[s] = gld((const double*)A.k1[s], mc);
+      if (stage >= 4) pk3[s] = gld((const double*)A.k3[s], mc);
+      if (stage == 5) pk4[s] = gld((const double*)A.k4[s], mc);
+    }
+  }
   __syncthreads();
 
   // ---- phase 2: x faces (left face of cells i0..i0+TX, rows j0..j0+TY-1)
   for (int f = threadIdx.x; f < kNFX; f += kNT) {
     int fx = f % (kTX + 1), fy = f / (kTX + 1);
-    int fi = i0 + fx, cj = j0 + fy;
+    int fi = i0 + fx, fj = j0 + fy;
     Flux4 F = {0, 0, 0, 0};
-    if (cj < by && fi <= bx) {
+    if (fj < by && fi <= bx) {
       int hL = (fy + 1) * kHX + fx, hR = hL + 1;
-      double h = G.dx[fi], rh = G.rdx[fi];
+      double h = gld(G.dx, fi), rh = gld(G.rdx, fi);
       double E = -(s_ph[hR] - s_ph[hL]) * rh;
       if (fi == 0) {
-        if (G.ltype[cj] == -2)
+        if (gld(G.ltype, fj) == -2)
           F = electrode_left(K, E, s_ph[hL], s_ph[hR], s_te[hR], s_mue[hR], s_ni[hR], s_ne[hR],
                              s_nt[hR], s_nm[hR]);
       } else if (fi == bx) {
-        if (G.rtype[cj] == 2)
+        if (gld(G.rtype, fj) == 2)
           F = electrode_right(K, E, s_ph[hL], s_ph[hR], s_te[hL], s_mue[hL], s_ni[hL],
                               s_ne[hL], s_nt[hL], s_nm[hL]);
       } else {
         F = interior_face(K, E, h, rh, s_mue[hL], s_mue[hR], s_de[hL], s_de[hR], s_ni[hL],
                           s_ni[hR], s_ne[hL], s_ne[hR], s_nt[hL], s_nt[hR], s_nm[hL], s_nm[hR]);
-        if (fi == 1) A.cur[cj] = F.i - F.e;            // Ida face (circ_lib.F90:87-118)
-        if (fi == bx - 1) A.cur[by + cj] = F.i - F.e;  // Idc face (circ_lib.F90:120-151)
+        if (fi == 1) gst(A.cur, fj, F.i - F.e);            // Ida face (circ_lib.F90:87-118)
+        if (fi == bx - 1) gst(A.cur, by + fj, F.i - F.e);  // Idc face (circ_lib.F90:120-151)
       }
     }
     s_f[0][f] = F.i;
@@ -256,28 +291,25 @@ __device__ __forceinline__ void stage_body(const StageArgs& A, const FsPhys& K, 
   __syncthreads();
 
   // ---- phase 3a: x divergence per cell (kept in registers)
-  const int tx = threadIdx.x % kTX, ty = threadIdx.x / kTX;
-  const int ci = i0 + tx, cj = j0 + ty;
-  const bool valid = (ci < bx) && (cj < by);
   const int hc = (ty + 1) * kHX + tx + 1;
   double dfx[4] = {0, 0, 0, 0}, feX = 0, exA = 0, mue2 = 0;
   int typx = 0;
   if (valid) {
-    typx = ci == 0 ? G.ltype[cj] : (ci == bx - 1 ? G.rtype[cj] : 0);
+    typx = ci == 0 ? gld(G.ltype, cj) : (ci == bx - 1 ? gld(G.rtype, cj) : 0);
     int fL = ty * (kTX + 1) + tx, fR = fL + 1;
-    double rdlx = G.rdlx[ci];
+    double rdlx = gld(G.rdlx, ci);
 #pragma unroll
     for (int s = 0; s < 4; s++) dfx[s] = (s_f[s][fR] - s_f[s][fL]) * rdlx;
     double exL, exR;
     if (typx == -1) {
       exL = 0.0;
-      exR = -(s_ph[hc + 1] - s_ph[hc]) * G.rdx[ci + 1];
+      exR = -(s_ph[hc + 1] - s_ph[hc]) * gld(G.rdx, ci + 1);
     } else if (typx == 1) {
-      exL = -(s_ph[hc] - s_ph[hc - 1]) * G.rdx[ci];
+      exL = -(s_ph[hc] - s_ph[hc - 1]) * gld(G.rdx, ci);
       exR = 0.0;
     } else {
-      exL = -(s_ph[hc] - s_ph[hc - 1]) * G.rdx[ci];
-      exR = -(s_ph[hc + 1] - s_ph[hc]) * G.rdx[ci + 1];
+      exL = -(s_ph[hc] - s_ph[hc - 1]) * gld(G.rdx, ci);
+      exR = -(s_ph[hc + 1] - s_ph[hc]) * gld(G.rdx, ci + 1);
     }
     feX = (s_f[1][fR] + s_f[1][fL]) / 2.0;
     exA = (exL + exR) / 2.0;
@@ -295,7 +327,7 @@ __device__ __forceinline__ void stage_body(const StageArgs& A, const FsPhys& K, 
     Flux4 F = {0, 0, 0, 0};
     if (fci < bx && fj > 0 && fj < by) {  // faces on y = 0 and y = w carry no flux
       int hL = fy * kHX + fx + 1, hR = hL + kHX;
-      double h = G.dy[fj], rh = G.rdy[fj];
+      double h = gld(G.dy, fj), rh = gld(G.rdy, fj);
       double E = -(s_ph[hR] - s_ph[hL]) * rh;
       F = interior_face(K, E, h, rh, s_mue[hL], s_mue[hR], s_de[hL], s_de[hR], s_ni[hL],
                         s_ni[hR], s_ne[hL], s_ne[hR], s_nt[hL], s_nt[hR], s_nm[hL], s_nm[hR]);
@@ -312,10 +344,10 @@ __device__ __forceinline__ void stage_body(const StageArgs& A, const FsPhys& K, 
   double tmin = 1e300;
   if (valid) {
     int fB = ty * kTX + tx, fT = fB + kTX;
-    double rdly = G.rdly[cj];
+    double rdly = gld(G.rdly, cj);
     double dfy[4];
     if (G.cyl) {
-      double rj = G.r[cj + 1], rp = G.r[cj + 2], rm = G.r[cj];
+      double rj = gld(G.r, cj + 1), rp = gld(G.r, cj + 2), rm = gld(G.r, cj);
       double w = rdly / rj;
 #pragma unroll
       for (int s = 0; s < 4; s++)
@@ -328,13 +360,13 @@ __device__ __forceinline__ void stage_body(const StageArgs& A, const FsPhys& K, 
     double eyB, eyT;
     if (typy == -1) {
       eyB = 0.0;
-      eyT = -(s_ph[hc + kHX] - s_ph[hc]) * G.rdy[cj + 1];
+      eyT = -(s_ph[hc + kHX] - s_ph[hc]) * gld(G.rdy, cj + 1);
     } else if (typy == 1) {
-      eyB = -(s_ph[hc] - s_ph[hc - kHX]) * G.rdy[cj];
+      eyB = -(s_ph[hc] - s_ph[hc - kHX]) * gld(G.rdy, cj);
       eyT = 0.0;
     } else {
-      eyB = -(s_ph[hc] - s_ph[hc - kHX]) * G.rdy[cj];
-      eyT = -(s_ph[hc + kHX] - s_ph[hc]) * G.rdy[cj + 1];
+      eyB = -(s_ph[hc] - s_ph[hc - kHX]) * gld(G.rdy, cj);
+      eyT = -(s_ph[hc + kHX] - s_ph[hc]) * gld(G.rdy, cj + 1);
     }
     double feY = (s_f[1][fT] + s_f[1][fB]) / 2.0;
     double eyA = (eyB + eyT) / 2.0;
@@ -359,51 +391,50 @@ __device__ __forceinline__ void stage_body(const StageArgs& A, const FsPhys& K, 
     kv[3] = -dfx[3] - dfy[3] + term_sm;
     if (isnan(kv[0]) || isnan(kv[1]) || isnan(kv[2]) || isnan(kv[3])) s_bad = 2;
 
-    size_t m = (size_t)(cj + 1) * P + (ci + 1);
+    const size_t m = mc;
     if (A.kdbg[0]) {
 #pragma unroll
-      for (int s = 0; s < 4; s++) A.kdbg[s][m] = kv[s];
+      for (int s = 0; s < 4; s++) gst(A.kdbg[s], m, kv[s]);
     }
     if (!A.rhs_only) {
-      const int stage = A.stage;
 #pragma unroll
       for (int s = 0; s < 4; s++) {
-        double n3 = A.n3[s][m], kk = kv[s], nw;
+        double n3 = pn3[s], kk = kv[s], nw;
         double nmin = s == 2 ? K.nmin_t : K.n_zero;
         if (stage == 1) {
           nw = n3 + kk * dt * (1.0 / 3.0);
-          A.k1[s][m] = kk;
+          gst(A.k1[s], m, kk);
         } else if (stage == 2) {
-          nw = n3 + dt * (A.k1[s][m] + kk) * (1.0 / 6.0);
+          nw = n3 + dt * (pk1[s] + kk) * (1.0 / 6.0);
         } else if (stage == 3) {
-          nw = n3 + dt * (A.k1[s][m] + kk * 3.0) / 8.0;
-          A.k3[s][m] = kk;
+          nw = n3 + dt * (pk1[s] + kk * 3.0) / 8.0;
+          gst(A.k3[s], m, kk);
         } else if (stage == 4) {
-          nw = n3 + dt * (A.k1[s][m] - A.k3[s][m] * 3.0 + kk * 4.0) / 2.0;
-          A.k4[s][m] = kk;
+          nw = n3 + dt * (pk1[s] - pk3[s] * 3.0 + kk * 4.0) / 2.0;
+          gst(A.k4[s], m, kk);
         } else {
-          double k1 = A.k1[s][m], k3 = A.k3[s][m], k4 = A.k4[s][m];
+          double k1 = pk1[s], k3 = pk3[s], k4 = pk4[s];
           nw = n3 + dt * (k1 + k4 * 4.0 + kk) * (1.0 / 6.0);
           double e = fabs(dt * (k1 * (2.0 / 30.0) - k3 * (3.0 / 10.0) + k4 * (4.0 / 15.0) -
                                 kk * (1.0 / 30.0)));
           double q = e / (1e-3 + 1e-3 * fabs(n3));
           acc[s] = q * q;
         }
-        A.wr[s][m] = nw > nmin ? nw : nmin;
+        gst(A.wr[s], m, nw > nmin ? nw : nmin);
       }
       if (stage == 5) tmin = 1.0 / (mue2 * ne + K.mui * ni);
     }
   }
-  if (!A.rhs_only && A.stage == 5) {
+  if (!A.rhs_only && stage == 5) {
     block_sum<4>(acc, s_red);
     double tm = block_min(tmin, s_red);
     if (threadIdx.x == 0) {
-      double* p = A.part + (size_t)bid * 8;
-      p[0] = acc[0];
-      p[1] = acc[1];
-      p[2] = acc[2];
-      p[3] = acc[3];
-      p[4] = tm;
+      const size_t p = (size_t)bid * 8;
+      gst(A.part, p + 0, acc[0]);
+      gst(A.part, p + 1, acc[1]);
+      gst(A.part, p + 2, acc[2]);
+      gst(A.part, p + 3, acc[3]);
+      gst(A.part, p + 4, tm);
     }
   }
   __syncthreads();
@@ -657,7 +688,7 @@ __device__ __forceinline__ void lapl_body(const LaplArgs& A, const FsPhys& K, in
           F.pe = -(a * mue * ne / h) - K.gam * F.pi;  // dfxe_mi
         }
       } else if (fi == bx) {
-        if (G.rtype[cj] == 2) {  // lapl_lib.F90:224-240 and :445-456
+        if (gld(G.rtype, cj) == 2) {  // lapl_lib.F90:224-240 and :445-456
           double a = s_ph[hR] > s_ph[hL] ? 1.0 : 0.0;
           double te = s_te[hL], mue = s_mue[hL], ni = s_ni[hL], ne = s_ne[hL];
           double ve = sqrt(K.ve_num * te / K.ve_den) * K.t0 / K.x0;

@@ -51,6 +51,13 @@ struct FsPhys {
   double c_nu;        // ninf * t0 / x0^3       (nu)
   double c_beta;      // 1e-13 * t0 / x0^3      (beta)
   double me_mi;       // me / mi
+  // the stage kernel's folded forms (each a few ulp from the reference's
+  // left-to-right evaluation)
+  double c_mue;       // x0 / ninf * t0 * ph0   (get_mue's unit conversion)
+  double c_de;        // x0 / ninf * t0         (get_De's)
+  double r_Di;        // 1 / Di                 (ion SG exponent)
+  double beta_a;      // ln10 (0.944 + 0.665 * 2.48) - 0.665 ln(11604.505 / 1.5)
+  double beta_hi;     // beta at the w = 5e4 clamp
 };
 
 // Host: fill FsPhys for runtime pressure and gamma (props.F90:24-44,
@@ -88,17 +95,21 @@ __device__ __forceinline__ double fit_x(double v) {
   return log(2.34e-1 > m ? 2.34e-1 : m);
 }
 
-// get_mue from x (ptcl_props.F90:194-221)
-__device__ __forceinline__ double mue_of_x(const FsPhys& K, double x) {
+// get_mue from x (ptcl_props.F90:194-221): exponent, then the value
+__device__ __forceinline__ double mue_poly(const FsPhys& K, double x) {
   double x2 = x * x, x3 = x2 * x, x4 = x2 * x2, x5 = x2 * x3;
-  double s;
   if (x < K.log5119)
-    s = F32(58.1133016145) + F32(-0.984082217962) * x + F32(-0.164770900119) * x2 +
-        F32(-0.142058042584) * x3 + F32(-0.0637079234081) * x4 + F32(0.0436642742558) * x5;
-  else
-    s = F32(70.7846754548) + F32(-22.7558138237) * x + F32(12.8366493242) * x2 +
-        F32(-3.57242763244) * x3 + F32(0.486276623664) * x4 + F32(-0.0259045697422) * x5;
-  return exp(s) * K.x0 / K.ninf * K.t0 * K.ph0;
+    return F32(58.1133016145) + F32(-0.984082217962) * x + F32(-0.164770900119) * x2 +
+           F32(-0.142058042584) * x3 + F32(-0.0637079234081) * x4 + F32(0.0436642742558) * x5;
+  return F32(70.7846754548) + F32(-22.7558138237) * x + F32(12.8366493242) * x2 +
+         F32(-3.57242763244) * x3 + F32(0.486276623664) * x4 + F32(-0.0259045697422) * x5;
+}
+__device__ __forceinline__ double mue_of_x(const FsPhys& K, double x) {
+  return exp(mue_poly(K, x)) * K.x0 / K.ninf * K.t0 * K.ph0;
+}
+// the stage kernel's form: one product with the folded conversion
+__device__ __forceinline__ double mue_of_x_folded(const FsPhys& K, double x) {
+  return exp(mue_poly(K, x)) * K.c_mue;
 }
 
 // get_mut from x (ptcl_props.F90:223-250)
@@ -115,15 +126,20 @@ __device__ __forceinline__ double mut_of_x(const FsPhys& K, double x) {
 }
 
 // get_De from x (ptcl_props.F90:268-291)
-__device__ __forceinline__ double de_of_x(const FsPhys& K, double x) {
+__device__ __forceinline__ double de_poly(double x) {
   Pow11 q = pow_chain(x);
-  double s = F32(57.7067018813) + F32(-0.0699892875381) * x + F32(-0.117645585949) * q.x2 +
-             F32(0.105390295278) * q.x3 + F32(-0.102862612604) * q.x4 +
-             F32(-0.0469171521686) * q.x5 + F32(0.0584908312121) * q.x6 +
-             F32(0.000578601715687) * q.x7 + F32(-0.0122860884883) * q.x8 +
-             F32(0.00426793748856) * q.x9 + F32(-0.000590000082557) * q.x10 +
-             F32(3.00706533201e-05) * q.x11;
-  return exp(s) * K.x0 / K.ninf * K.t0;
+  return F32(57.7067018813) + F32(-0.0699892875381) * x + F32(-0.117645585949) * q.x2 +
+         F32(0.105390295278) * q.x3 + F32(-0.102862612604) * q.x4 +
+         F32(-0.0469171521686) * q.x5 + F32(0.0584908312121) * q.x6 +
+         F32(0.000578601715687) * q.x7 + F32(-0.0122860884883) * q.x8 +
+         F32(0.00426793748856) * q.x9 + F32(-0.000590000082557) * q.x10 +
+         F32(3.00706533201e-05) * q.x11;
+}
+__device__ __forceinline__ double de_of_x(const FsPhys& K, double x) {
+  return exp(de_poly(x)) * K.x0 / K.ninf * K.t0;
+}
+__device__ __forceinline__ double de_of_x_folded(const FsPhys& K, double x) {
+  return exp(de_poly(x)) * K.c_de;
 }
 
 // get_Dt from x (ptcl_props.F90:293-316)
@@ -190,11 +206,15 @@ __device__ __forceinline__ double ksi_of_x(const FsPhys& K, double x) {
   return exp(s) * K.c_rate6;
 }
 
-// get_beta (ptcl_props.F90:182-192), v = T*ph0
-__device__ __forceinline__ double beta_of_v(const FsPhys& K, double v) {
+// get_beta (ptcl_props.F90:182-192), v = T*ph0, xa = fit_x(v):
+// 10**(0.944 - 0.665 (log10(w) - 2.48)) with w = v 11604.505/1.5 clamped at
+// 5e4 is exp(beta_a - 0.665 ln v) below the clamp, and ln v is xa there unless
+// v < 0.234 (fit_x's lower clamp).  Above the clamp the value is a constant.
+__device__ __forceinline__ double beta_of_v(const FsPhys& K, double v, double xa) {
   double w = v * (11.604505e3 / F32(1.5));
-  double x = log10(w < 5e4 ? w : 5e4);
-  return exp10(0.944 - 0.665 * (x - F32(2.48))) * K.c_beta;
+  if (!(w < 5e4)) return K.beta_hi;
+  double lv = v >= 2.34e-1 ? xa : log(v);
+  return exp(K.beta_a - 0.665 * lv) * K.c_beta;
 }
 
 // The six reaction-rate fits at one cell from v = T*ph0 and xa = fit_x(v).
@@ -211,7 +231,7 @@ __device__ __forceinline__ Rates cell_rates(const FsPhys& K, double v, double xa
   R.k_si = ksi_of_x(K, x_s);
   R.k_ex = kex_of_x(K, x_ex);
   R.nu = nu_of_x(K, xa);
-  R.beta = beta_of_v(K, v);
+  R.beta = beta_of_v(K, v, xa);
   return R;
 }
 
@@ -223,6 +243,19 @@ __device__ __forceinline__ double sg_flux(const FsPhys& K, double E, double h, d
   double v = qs * mu * E;
   double arg = v * h / D;
   if (fabs(qs * E) < K.ftol) return D * (nl - nr) * rh;
+  double na = arg > 0 ? -arg : arg;
+  double ex = na < -100.0 ? K.exp_m100 : exp(na);
+  double num = arg > 0 ? (nl - nr * ex) : (nr - nl * ex);
+  return v * num / (1.0 - ex);
+}
+
+// Ion flux (qs = +1, mu = mui, D = Di) with the exponent argument formed by
+// multiplying by 1/Di.
+__device__ __forceinline__ double sg_flux_ion(const FsPhys& K, double E, double h, double rh,
+                                              double nl, double nr) {
+  double v = K.mui * E;
+  double arg = v * h * K.r_Di;
+  if (fabs(E) < K.ftol) return K.Di * (nl - nr) * rh;
   double na = arg > 0 ? -arg : arg;
   double ex = na < -100.0 ? K.exp_m100 : exp(na);
   double num = arg > 0 ? (nl - nr * ex) : (nr - nl * ex);
