@@ -1240,7 +1240,7 @@ int ens_fail(fs_ensemble* E, int i) {
 void ens_vcycle(fs_ensemble* E, int v, const uint8_t* mask) {
   for (int l = 0; l < E->maxF; l++)
     launch_ens_mg_down(E->t_down, with_mask(E->m_down[l], mask),
-                       EnsSel{nullptr, kMgMaxLevels * 4, l * 4 + v}, E->st);
+                       EnsSel{nullptr, kMgMaxLevels * 4, l * 4 + v}, l == 0 && v != 0, E->st);
   launch_ens_mg_coarse_cycle(E->t_cc, with_mask(E->m_one, mask), EnsSel{nullptr, 4, v}, E->st);
   for (int l = E->maxF; l-- > 0;)
     launch_ens_mg_up(E->t_up, with_mask(E->m_up[l], mask), EnsSel{nullptr, kMgMaxLevels * 4, l * 4 + v},

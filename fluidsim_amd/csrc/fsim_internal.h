@@ -248,7 +248,9 @@ void launch_ens_lapl(const LaplArgs* t, const FsPhys* K, const EnsMap& m, const 
 void launch_ens_mg_coarsen(const MgArgs* t, const EnsMap& m, const EnsSel& s, hipStream_t st);
 void launch_ens_mg_coarse_setup(const MgCycleArgs* t, const EnsMap& m, const EnsSel& s,
                                 hipStream_t st);
-void launch_ens_mg_down(const MgArgs* t, const EnsMap& m, const EnsSel& s, hipStream_t st);
+// cg: the members' level-0 CG form (cg.on in every selected entry)
+void launch_ens_mg_down(const MgArgs* t, const EnsMap& m, const EnsSel& s, bool cg,
+                        hipStream_t st);
 // cg: the level-0 up kernel of the multigrid-PCG (wider halo, w = M u)
 void launch_ens_mg_up(const MgArgs* t, const EnsMap& m, const EnsSel& s, bool cg, hipStream_t st);
 void launch_ens_mg_coarse_cycle(const MgCycleArgs* t, const EnsMap& m, const EnsSel& s,

@@ -900,13 +900,37 @@ __global__ void __launch_bounds__(kNT) k_lapl(const LaplArgs* __restrict__ A,
 // Sum the previous kernel's 3 partials per block, identically in every block.
 // Partial sums of the previous kernel: this thread's share (loads only, so
 // that a kernel can issue them together with its own loads) ...
-__device__ __forceinline__ void load_partials3(const double* part, int nb, double (&v)[3]) {
-  v[0] = v[1] = v[2] = 0.0;
-  for (int b = threadIdx.x; b < nb; b += kNT) {
-    v[0] += part[3 * b];
-    v[1] += part[3 * b + 1];
-    v[2] += part[3 * b + 2];
+// (nb <= kMaxPartials).  In two parts, so that a kernel can issue other
+// loads between them: every load is issued before the first sum (threads past
+// nb load the first partial and do not add it).
+struct Part3 {
+  double a[kMaxPartials / kNT][3];
+};
+__device__ __forceinline__ void part3_load(const double* part, int nb, Part3& P) {
+#pragma unroll
+  for (int q = 0; q < kMaxPartials / kNT; q++) {
+    const int b = threadIdx.x + kNT * q;
+    const size_t o = b < nb ? 3 * (size_t)b : 0;
+    P.a[q][0] = gld(part, o);
+    P.a[q][1] = gld(part, o + 1);
+    P.a[q][2] = gld(part, o + 2);
   }
+}
+__device__ __forceinline__ void part3_sum(const Part3& P, int nb, double (&v)[3]) {
+  __builtin_amdgcn_sched_barrier(0);  // keep the sums (and their waits) here
+  v[0] = v[1] = v[2] = 0.0;
+#pragma unroll
+  for (int q = 0; q < kMaxPartials / kNT; q++)
+    if (threadIdx.x + kNT * q < nb) {
+      v[0] += P.a[q][0];
+      v[1] += P.a[q][1];
+      v[2] += P.a[q][2];
+    }
+}
+__device__ __forceinline__ void load_partials3(const double* part, int nb, double (&v)[3]) {
+  Part3 P;
+  part3_load(part, nb, P);
+  part3_sum(P, nb, v);
 }
 
 // ... and the block-wide total, identical in every block.
@@ -1185,27 +1209,64 @@ struct GsTile {
 
 __device__ __forceinline__ int gs_tiles_x(const MgLevel& F) { return (F.nx + kGsTX - 1) / kGsTX; }
 
-// Stage r, d, cx, cy of the tile at (i0, j0) and its halo.
-template <int H>
-__device__ __forceinline__ void gs_stage(const MgLevel& F, GsTile<H>& s, int i0, int j0) {
+// Stage r, d, cx, cy of the tile at (i0, j0) and its halo; with kUp also the
+// iterate x1 + oc * e_c(parent) of the up kernel.  In two parts: gs_load issues
+// every load of the tile into registers (cells outside the level load the
+// padded plane's first element and discard it), gs_store writes LDS, so the
+// staging costs one memory round trip and a kernel can issue other loads
+// between the two.
+template <int H, bool kUp>
+struct GsRegs {
+  static constexpr int NS = (GsTile<H>::R * GsTile<H>::W + kNT - 1) / kNT;
+  double r[NS], d[NS], cx[NS], cy[NS], x[NS], e[NS];
+  bool in[NS];
+};
+template <int H, bool kUp>
+__device__ __forceinline__ void gs_load(const MgLevel& F, int i0, int j0, const MgLevel& C,
+                                        GsRegs<H, kUp>& g) {
   constexpr int W = GsTile<H>::W, R = GsTile<H>::R;
-  for (int k = threadIdx.x; k < R * W; k += kNT) {
+#pragma unroll
+  for (int q = 0; q < g.NS; q++) {
+    const int k = threadIdx.x + kNT * q;
     const int b = k / W, a = k % W;
     const int i = i0 - H + a, j = j0 - H + b;
-    double r = 0.0, d = 1.0, cx = 0.0, cy = 0.0;
-    if (i >= 0 && i < F.nx && j >= 0 && j < F.ny) {
-      const size_t m = mg_at(F, i, j);
-      r = F.r[m];
-      d = F.d[m];
-      cx = F.cx[m];
-      cy = F.cy[m];
+    g.in[q] = k < R * W && i >= 0 && i < F.nx && j >= 0 && j < F.ny;
+    const size_t m = g.in[q] ? mg_at(F, i, j) : 0;
+    g.r[q] = gld(F.r, m);
+    g.d[q] = gld(F.d, m);
+    g.cx[q] = gld(F.cx, m);
+    g.cy[q] = gld(F.cy, m);
+    if (kUp) {
+      g.x[q] = gld(F.x1, m);
+      g.e[q] = gld(C.e, g.in[q] ? mg_at(C, i >> 1, j >> 1) : 0);
     }
-    s.r[b][a] = r;
-    s.d[b][a] = d;
-    s.dinv[b][a] = 1.0 / d;
-    s.cx[b][a] = cx;
-    s.cy[b][a] = cy;
   }
+}
+template <int H, bool kUp>
+__device__ __forceinline__ void gs_store(const GsRegs<H, kUp>& g, GsTile<H>& s, double oc) {
+  constexpr int W = GsTile<H>::W, R = GsTile<H>::R;
+#pragma unroll
+  for (int q = 0; q < g.NS; q++) {
+    const int k = threadIdx.x + kNT * q;
+    if (k < R * W) {
+      const int b = k / W, a = k % W;
+      const bool in = g.in[q];
+      const double dq = in ? g.d[q] : 1.0;
+      s.r[b][a] = in ? g.r[q] : 0.0;
+      s.d[b][a] = dq;
+      s.dinv[b][a] = 1.0 / dq;
+      s.cx[b][a] = in ? g.cx[q] : 0.0;
+      s.cy[b][a] = in ? g.cy[q] : 0.0;
+      if (kUp) s.x[b][a] = in ? g.x[q] + oc * g.e[q] : 0.0;
+    }
+  }
+}
+template <int H, bool kUp>
+__device__ __forceinline__ void gs_stage(const MgLevel& F, GsTile<H>& s, int i0, int j0,
+                                         const MgLevel& C, double oc) {
+  GsRegs<H, kUp> g;
+  gs_load<H, kUp>(F, i0, j0, C, g);
+  gs_store<H, kUp>(g, s, oc);
 }
 
 // One Gauss-Seidel half-sweep over the cells of colour c ((i + j) & 1 == c)
@@ -1267,46 +1328,74 @@ __device__ __forceinline__ bool mg_check(DevScalars* sc, double rho, double b2, 
   return stop;
 }
 
-// Level-0 CG vectors of the staged cells a thread holds in the down kernel.
+// Level-0 CG vectors of the staged cells a thread holds in the down kernel,
+// and the volume of its own cells.
 constexpr int kGsSlots = (GsTile<kGsH>::R * GsTile<kGsH>::W + kNT - 1) / kNT;
 struct CgCells {
   double r[kGsSlots], w[kGsSlots], s[kGsSlots], u[kGsSlots], p[kGsSlots], x[kGsSlots];
+  double vol[kGsSlots];
 };
 
 // Loads of one tile for the CG update: the operator into LDS, the vectors
-// into registers (u, p, x only for the tile's own cells).
-__device__ __forceinline__ void cg_stage(const MgArgs& A, GsTile<kGsH>& s, int i0, int j0,
-                                         CgCells& c) {
+// into registers (u, p, x and the volume only for the tile's own cells).  As
+// gs_load / gs_store: cg_load issues every load (cells outside the level load
+// the padded planes' first element and discard it), cg_store writes LDS.
+struct CgRegs {
+  double d[kGsSlots], cx[kGsSlots], cy[kGsSlots], vx[kGsSlots], vy[kGsSlots], vr[kGsSlots];
+  bool in[kGsSlots], own[kGsSlots];
+};
+__device__ __forceinline__ void cg_load(const MgArgs& A, int i0, int j0, CgCells& c, CgRegs& g) {
   constexpr int W = GsTile<kGsH>::W, R = GsTile<kGsH>::R;
   const MgLevel& F = A.F;
+  const Geo& G = A.G;
 #pragma unroll
   for (int e = 0; e < kGsSlots; e++) {
     const int k = threadIdx.x + kNT * e;
-    c.r[e] = c.w[e] = c.s[e] = c.u[e] = c.p[e] = c.x[e] = 0.0;
+    const int b = k / W, a = k % W;
+    const int i = i0 - kGsH + a, j = j0 - kGsH + b;
+    const bool in = k < R * W && i >= 0 && i < F.nx && j >= 0 && j < F.ny;
+    const bool own = in && a >= kGsH && a < kGsH + kGsTX && b >= kGsH && b < kGsH + kGsTY;
+    g.in[e] = in;
+    g.own[e] = own;
+    const size_t m = in ? mg_at(F, i, j) : 0;
+    const size_t mo = own ? m : 0;
+    g.d[e] = gld(F.d, m);
+    g.cx[e] = gld(F.cx, m);
+    g.cy[e] = gld(F.cy, m);
+    c.r[e] = gld((const double*)A.cg.r_old, m);
+    c.w[e] = gld((const double*)A.cg.w, m);
+    c.s[e] = A.cg.first ? 0.0 : gld((const double*)A.cg.s_old, m);
+    c.u[e] = gld((const double*)A.cg.u, mo);
+    c.p[e] = A.cg.first ? 0.0 : gld((const double*)A.cg.p, mo);
+    c.x[e] = gld((const double*)A.cg.x, mo);
+    g.vx[e] = gld(G.dlx, own ? i : 0);
+    g.vy[e] = gld(G.dly, own ? j : 0);
+    g.vr[e] = G.cyl ? gld(G.r, own ? j + 1 : 0) : 1.0;
+  }
+}
+__device__ __forceinline__ void cg_store(const CgRegs& g, GsTile<kGsH>& s, CgCells& c) {
+  constexpr int W = GsTile<kGsH>::W, R = GsTile<kGsH>::R;
+#pragma unroll
+  for (int e = 0; e < kGsSlots; e++) {
+    const int k = threadIdx.x + kNT * e;
+    if (!g.in[e]) c.r[e] = c.w[e] = c.s[e] = 0.0;
+    if (!g.own[e]) c.u[e] = c.p[e] = c.x[e] = 0.0;
+    c.vol[e] = g.vx[e] * g.vy[e] * g.vr[e];
     if (k < R * W) {
       const int b = k / W, a = k % W;
-      const int i = i0 - kGsH + a, j = j0 - kGsH + b;
-      double d = 1.0, cx = 0.0, cy = 0.0;
-      if (i >= 0 && i < F.nx && j >= 0 && j < F.ny) {
-        const size_t m = mg_at(F, i, j);
-        d = F.d[m];
-        cx = F.cx[m];
-        cy = F.cy[m];
-        c.r[e] = A.cg.r_old[m];
-        c.w[e] = A.cg.w[m];
-        if (!A.cg.first) c.s[e] = A.cg.s_old[m];
-        if (a >= kGsH && a < kGsH + kGsTX && b >= kGsH && b < kGsH + kGsTY) {
-          c.u[e] = A.cg.u[m];
-          if (!A.cg.first) c.p[e] = A.cg.p[m];
-          c.x[e] = A.cg.x[m];
-        }
-      }
-      s.d[b][a] = d;
-      s.dinv[b][a] = 1.0 / d;
-      s.cx[b][a] = cx;
-      s.cy[b][a] = cy;
+      const double dq = g.in[e] ? g.d[e] : 1.0;
+      s.d[b][a] = dq;
+      s.dinv[b][a] = 1.0 / dq;
+      s.cx[b][a] = g.in[e] ? g.cx[e] : 0.0;
+      s.cy[b][a] = g.in[e] ? g.cy[e] : 0.0;
     }
   }
+}
+__device__ __forceinline__ void cg_stage(const MgArgs& A, GsTile<kGsH>& s, int i0, int j0,
+                                         CgCells& c) {
+  CgRegs g;
+  cg_load(A, i0, j0, c, g);
+  cg_store(g, s, c);
 }
 
 // The CG update of the staged cells: the new residual goes to LDS (halo cells
@@ -1317,7 +1406,6 @@ __device__ __forceinline__ void cg_update(const MgArgs& A, GsTile<kGsH>& s, int 
                                           double& acc) {
   constexpr int W = GsTile<kGsH>::W, R = GsTile<kGsH>::R;
   const MgLevel& F = A.F;
-  const Geo& G = A.G;
 #pragma unroll
   for (int e = 0; e < kGsSlots; e++) {
     const int k = threadIdx.x + kNT * e;
@@ -1335,8 +1423,7 @@ __device__ __forceinline__ void cg_update(const MgArgs& A, GsTile<kGsH>& s, int 
           A.cg.s_new[m] = sn;
           A.cg.x[m] = c.x[e] + alpha * pn;
           F.r[m] = r;
-          const double V = G.dlx[i] * G.dly[j] * (G.cyl ? G.r[j + 1] : 1.0);
-          const double rn = r / V;
+          const double rn = r / c.vol[e];
           acc += rn * rn;
         }
       }
@@ -1351,7 +1438,7 @@ __device__ __forceinline__ void cg_update(const MgArgs& A, GsTile<kGsH>& s, int 
 // CG form it first applies the CG update (alpha and beta from the up kernel's
 // partials) and writes the partials of the new |r/V|^2, which the next
 // kernel tests.  The partial sums are loaded together with the first tile.
-template <int kInstance>
+template <int kInstance, bool kCg>
 __device__ __forceinline__ void mg_down_body(const MgArgs& A, int bid, int nb) {
   constexpr int W = GsTile<kGsH>::W, R = GsTile<kGsH>::R;
   __shared__ GsTile<kGsH> s;
@@ -1366,18 +1453,29 @@ __device__ __forceinline__ void mg_down_body(const MgArgs& A, int bid, int nb) {
   double racc[3] = {0.0, 0.0, 0.0};  // sum |r/V|^2 after the CG update
   if (tr.t0 >= tr.t1) {  // block 0 always has tiles (it writes the scalars)
     if (A.part_scale) write_partials3(sacc, A.part_scale, s_red, bid);
-    if (A.cg.on) write_partials3(racc, A.part_out, s_red, bid);
+    if (kCg) write_partials3(racc, A.part_out, s_red, bid);
     return;
   }
-  const bool red = A.check || A.cg.on;
-  double pv[3];
-  if (red) load_partials3(A.part_in, A.nb_in, pv);
+  const bool red = A.check || kCg;
+  // the first tile's loads, then the partials', then the LDS stores
   CgCells cc;
+  Part3 pl;
   {
     const int i0 = (tr.t0 % ntx) * kGsTX, j0 = (tr.t0 / ntx) * kGsTY;
-    if (A.cg.on) cg_stage(A, s, i0, j0, cc);
-    else gs_stage(F, s, i0, j0);
+    if (kCg) {
+      CgRegs g;
+      cg_load(A, i0, j0, cc, g);
+      if (red) part3_load(A.part_in, A.nb_in, pl);
+      cg_store(g, s, cc);
+    } else {
+      GsRegs<kGsH, false> g;
+      gs_load<kGsH, false>(F, i0, j0, C, g);
+      if (red) part3_load(A.part_in, A.nb_in, pl);
+      gs_store<kGsH, false>(g, s, 0.0);
+    }
   }
+  double pv[3];
+  if (red) part3_sum(pl, A.nb_in, pv);
   double alpha = 0.0, beta = 0.0;
   if (red) {
     double tot[3];
@@ -1407,10 +1505,10 @@ __device__ __forceinline__ void mg_down_body(const MgArgs& A, int bid, int nb) {
     const int i0 = (t % ntx) * kGsTX, j0 = (t / ntx) * kGsTY;
     if (t > tr.t0) {
       __syncthreads();
-      if (A.cg.on) cg_stage(A, s, i0, j0, cc);
-      else gs_stage(F, s, i0, j0);
+      if (kCg) cg_stage(A, s, i0, j0, cc);
+      else gs_stage<kGsH, false>(F, s, i0, j0, C, 0.0);
     }
-    if (A.cg.on) cg_update(A, s, i0, j0, cc, alpha, beta, racc[0]);
+    if (kCg) cg_update(A, s, i0, j0, cc, alpha, beta, racc[0]);
     __syncthreads();
     // first (red) half-sweep from zero is pointwise: the whole staged halo
     for (int k = threadIdx.x; k < R * W; k += kNT) {
@@ -1448,10 +1546,15 @@ __device__ __forceinline__ void mg_down_body(const MgArgs& A, int bid, int nb) {
     }
   }
   if (A.part_scale) write_partials3(sacc, A.part_scale, s_red, bid);
-  if (A.cg.on) write_partials3(racc, A.part_out, s_red, bid);
+  if (kCg) write_partials3(racc, A.part_out, s_red, bid);
 }
 
-__global__ void __launch_bounds__(kNT) k_mg_down(MgArgs A) { mg_down_body<0>(A, blockIdx.x, gridDim.x); }
+// kCg: the level-0 CG form (its own instance: the CG vectors it keeps in
+// registers would otherwise limit the occupancy of the other levels' launches)
+template <bool kCg>
+__global__ void __launch_bounds__(kNT) k_mg_down(MgArgs A) {
+  mg_down_body<0, kCg>(A, blockIdx.x, gridDim.x);
+}
 
 // Up: x2 = x1 + oc * e_c(aggregate) on the staged halo, post-smooth, write e.
 // Level 0 writes u = B r and the partials of r.u, and of z.q (flexible CG) or,
@@ -1470,15 +1573,7 @@ __device__ __forceinline__ void mg_up_body(const MgArgs& A, int bid, int nb) {
   for (int t = tr.t0; t < tr.t1; t++) {
     const int i0 = (t % ntx) * kGsTX, j0 = (t / ntx) * kGsTY;
     if (t > tr.t0) __syncthreads();
-    gs_stage(F, s, i0, j0);
-    for (int k = threadIdx.x; k < R * W; k += kNT) {
-      const int b = k / W, a = k % W;
-      const int i = i0 - H + a, j = j0 - H + b;
-      double v = 0.0;
-      if (i >= 0 && i < F.nx && j >= 0 && j < F.ny)
-        v = F.x1[mg_at(F, i, j)] + A.oc * C.e[mg_at(C, i >> 1, j >> 1)];
-      s.x[b][a] = v;
-    }
+    gs_stage<H, true>(F, s, i0, j0, C, A.oc);
     __syncthreads();
     gs_half(s, i0, j0, 1, H - 1);
     __syncthreads();
@@ -2238,9 +2333,10 @@ __global__ void __launch_bounds__(1024) kb_mg_coarse_setup(const MgCycleArgs* t,
   int i, bid, nb;
   if (ens_slot(m, i, bid, nb)) mg_coarse_setup_body<1>(ens_arg(t, s, i));
 }
+template <bool kCg>
 __global__ void __launch_bounds__(kNT) kb_mg_down(const MgArgs* t, EnsMap m, EnsSel s) {
   int i, bid, nb;
-  if (ens_slot(m, i, bid, nb)) mg_down_body<1>(ens_arg(t, s, i), bid, nb);
+  if (ens_slot(m, i, bid, nb)) mg_down_body<1, kCg>(ens_arg(t, s, i), bid, nb);
 }
 template <int H>
 __global__ void __launch_bounds__(kNT) kb_mg_up(const MgArgs* t, EnsMap m, EnsSel s) {
@@ -2313,7 +2409,8 @@ int mg_grid(const MgLevel& F) {
   return capped_grid(((F.nx + kGsTX - 1) / kGsTX) * ((F.ny + kGsTY - 1) / kGsTY));
 }
 void launch_mg_down(const MgArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL(k_mg_down, dim3(mg_grid(a.F)), dim3(kNT), 0, s, a);
+  if (a.cg.on) hipLaunchKernelGGL(k_mg_down<true>, dim3(mg_grid(a.F)), dim3(kNT), 0, s, a);
+  else hipLaunchKernelGGL(k_mg_down<false>, dim3(mg_grid(a.F)), dim3(kNT), 0, s, a);
 }
 void launch_mg_up(const MgArgs& a, hipStream_t s) {
   if (a.cg.on) hipLaunchKernelGGL(k_mg_up<kGsH + 1>, dim3(mg_grid(a.F)), dim3(kNT), 0, s, a);
@@ -2375,8 +2472,11 @@ void launch_ens_mg_coarse_setup(const MgCycleArgs* t, const EnsMap& m, const Ens
                                 hipStream_t st) {
   if (m.nblocks) hipLaunchKernelGGL(kb_mg_coarse_setup, dim3(m.nblocks), dim3(1024), 0, st, t, m, s);
 }
-void launch_ens_mg_down(const MgArgs* t, const EnsMap& m, const EnsSel& s, hipStream_t st) {
-  if (m.nblocks) hipLaunchKernelGGL(kb_mg_down, dim3(m.nblocks), dim3(kNT), 0, st, t, m, s);
+void launch_ens_mg_down(const MgArgs* t, const EnsMap& m, const EnsSel& s, bool cg,
+                        hipStream_t st) {
+  if (!m.nblocks) return;
+  if (cg) hipLaunchKernelGGL(kb_mg_down<true>, dim3(m.nblocks), dim3(kNT), 0, st, t, m, s);
+  else hipLaunchKernelGGL(kb_mg_down<false>, dim3(m.nblocks), dim3(kNT), 0, st, t, m, s);
 }
 void launch_ens_mg_up(const MgArgs* t, const EnsMap& m, const EnsSel& s, bool cg, hipStream_t st) {
   if (!m.nblocks) return;
