@@ -1131,6 +1131,57 @@ __device__ __forceinline__ size_t mg_at(const MgLevel& L, int i, int j) {
   return (size_t)(j + 1) * L.P + (i + 1);
 }
 
+// A kernel argument is loaded where it is first used; a kernel that first
+// tests a flag in device memory would then load its arguments one dependent
+// round trip after another behind that test.  hold() takes values as scalar
+// operands of an empty asm statement, so that their loads are all issued
+// before it (together with the flag's).
+__device__ __forceinline__ void hold(const void* p) { asm volatile("" ::"s"(p)); }
+__device__ __forceinline__ void hold(int v) { asm volatile("" ::"s"(v)); }
+__device__ __forceinline__ void hold(double v) { asm volatile("" ::"s"(v)); }
+__device__ __forceinline__ void hold_level(const MgLevel& L) {
+  hold(L.d);
+  hold(L.cx);
+  hold(L.cy);
+  hold(L.r);
+  hold(L.x1);
+  hold(L.e);
+  hold(L.nx);
+  hold(L.ny);
+  hold(L.P);
+}
+// the arguments of the tile kernels (kernel-argument copies only: the
+// ensemble kernels read theirs from a table)
+__device__ __forceinline__ void hold_mg_args(const MgArgs& A, bool cg) {
+  hold_level(A.F);
+  hold_level(A.C);
+  hold(A.part_in);
+  hold(A.nb_in);
+  hold(A.part_out);
+  hold(A.part_scale);
+  hold(A.q);
+  hold(A.oc);
+  hold(A.check);
+  hold(A.it);
+  hold(A.level0);
+  hold(A.cg.on);
+  hold(A.cg.w);
+  if (cg) {
+    hold(A.cg.x);
+    hold(A.cg.p);
+    hold(A.cg.u);
+    hold(A.cg.w);
+    hold(A.cg.s_old);
+    hold(A.cg.s_new);
+    hold(A.cg.r_old);
+    hold(A.cg.first);
+    hold(A.G.dlx);
+    hold(A.G.dly);
+    hold(A.G.r);
+    hold(A.G.cyl);
+  }
+}
+
 // The Galerkin coefficients of coarse cell (I, J) from fine planes d, cx, cy
 // (padded, row pitch P, nx x ny cells); coarse size NX x NY.
 struct Coarse3 {
@@ -1293,15 +1344,28 @@ __device__ __forceinline__ int gs_ntiles(const MgLevel& F) {
   return gs_tiles_x(F) * ((F.ny + kGsTY - 1) / kGsTY);
 }
 
+// The solver state a convergence test reads, loaded at the start of the
+// kernel (with its arguments) rather than after the reduction that needs it.
+struct CheckState {
+  double tol, bnorm2;
+  int32_t itc_prev, maxit;
+};
+__device__ __forceinline__ CheckState check_state(const DevScalars* sc, int it) {
+  return {sc->tol, sc->bnorm2, sc->itc[(it + 1) & 1], sc->maxit};
+}
+__device__ __forceinline__ void hold(const CheckState& c) {
+  asm volatile("" ::"s"(c.tol), "s"(c.bnorm2), "s"(c.itc_prev), "s"(c.maxit));
+}
+
 // Convergence test of the multigrid-PCG on rho = |r/V|^2: of the right-hand
 // side (init) or after the update of iteration `it`.  Every block decides
 // identically; `writer` (one thread) records the count, the residual and the
 // outcome.  True when the solve stops here.
-__device__ __forceinline__ bool mg_check(DevScalars* sc, double rho, double b2, int init, int it,
-                                         bool writer) {
+__device__ __forceinline__ bool mg_check(DevScalars* sc, const CheckState& cs, double rho,
+                                         double b2, int init, int it, bool writer) {
   if (init) {  // b2 = |rhs/V|^2; rho < b2 after a warm start
     const double res = b2 > 0.0 ? sqrt(rho / b2) : 0.0;
-    const bool conv = b2 == 0.0 || res <= sc->tol;
+    const bool conv = b2 == 0.0 || res <= cs.tol;
     if (writer) {
       sc->bnorm2 = b2;
       sc->resid = res;
@@ -1314,10 +1378,10 @@ __device__ __forceinline__ bool mg_check(DevScalars* sc, double rho, double b2, 
   }
   // iteration count kept in two slots by parity: the other slot was written
   // by the previous iteration, the writer fills this one
-  const int cur = sc->itc[(it + 1) & 1] + 1;
-  const double res = sqrt(rho / sc->bnorm2);
-  const bool conv = res <= sc->tol;
-  const bool stop = conv || cur >= sc->maxit;
+  const int cur = cs.itc_prev + 1;
+  const double res = sqrt(rho / cs.bnorm2);
+  const bool conv = res <= cs.tol;
+  const bool stop = conv || cur >= cs.maxit;
   if (writer) {
     sc->itc[it & 1] = cur;
     sc->iters = cur;
@@ -1445,7 +1509,23 @@ __device__ __forceinline__ void mg_down_body(const MgArgs& A, int bid, int nb) {
   __shared__ double s_res[kGsTY][kGsTX / 2];
   __shared__ double s_red[8 * kNW];
   DevScalars* sc = A.sc;
-  if (sc->done) return;
+  CheckState cs{};
+  double gam_old = 1.0, alpha_old = 1.0;  // the previous CG iteration's gamma and alpha
+  {
+    const int done = sc->done;
+    if (A.check) cs = check_state(sc, A.it);
+    if (kCg && !A.cg.first) {
+      gam_old = sc->cg_gam[(A.it + 1) & 1];
+      alpha_old = sc->cg_alpha[(A.it + 1) & 1];
+    }
+    if (kInstance == 0) {
+      hold_mg_args(A, kCg);
+      hold(nb);
+      hold(cs);
+      asm volatile("" ::"s"(gam_old), "s"(alpha_old));
+    }
+    if (done) return;
+  }
   const MgLevel &F = A.F, &C = A.C;
   const int ntx = gs_tiles_x(F);
   const TileRange tr = tile_range(gs_ntiles(F), bid, nb);
@@ -1481,14 +1561,14 @@ __device__ __forceinline__ void mg_down_body(const MgArgs& A, int bid, int nb) {
     double tot[3];
     finish_partials3(pv, s_red, tot);
     if (A.check) {
-      if (mg_check(sc, tot[0], tot[1], A.init, A.it, bid == 0 && threadIdx.x == 0)) return;
+      if (mg_check(sc, cs, tot[0], tot[1], A.init, A.it, bid == 0 && threadIdx.x == 0)) return;
     } else {
       // gamma = r.u, delta = u.M u; p.s = delta - beta gamma / alpha_old
       const double gam = tot[0], del = tot[1];
       double ps = del;
       if (!A.cg.first) {
-        beta = gam / sc->cg_gam[(A.it + 1) & 1];
-        ps = del - beta * gam / sc->cg_alpha[(A.it + 1) & 1];
+        beta = gam / gam_old;
+        ps = del - beta * gam / alpha_old;
       }
       alpha = gam / ps;
       if (!(ps > 0.0) || !isfinite(alpha)) {  // breakdown: stop, not converged
@@ -1565,7 +1645,14 @@ __device__ __forceinline__ void mg_up_body(const MgArgs& A, int bid, int nb) {
   constexpr int W = GsTile<H>::W, R = GsTile<H>::R;
   __shared__ GsTile<H> s;
   __shared__ double s_red[8 * kNW];
-  if (A.sc->done) return;
+  {
+    const int done = A.sc->done;
+    if (kInstance == 0) {
+      hold_mg_args(A, false);
+      hold(nb);
+    }
+    if (done) return;
+  }
   const MgLevel &F = A.F, &C = A.C;
   const int ntx = gs_tiles_x(F);
   const TileRange tr = tile_range(gs_ntiles(F), bid, nb);
@@ -1770,12 +1857,12 @@ __device__ __forceinline__ void lds_agg_rows(const MgLevel& L, const MgLevel& C,
     R.r[c] = q[2048];
     R.b[c] = q[3072];
     R.t[c] = q[4096];
-    R.rhs[c] = 0.0;
     R.x[c] = 0.0;
-    if (R.k0 >= 0 && 2 * I + a < L.nx && 2 * J + b < L.ny) {
-      R.mask |= 1 << c;
-      R.rhs[c] = L.r[R.k0 + a + b * W];
-    }
+    const bool in = R.k0 >= 0 && 2 * I + a < L.nx && 2 * J + b < L.ny;
+    // loaded for every child (the padded plane's first element outside the
+    // level): only children with their mask bit set are ever read
+    R.rhs[c] = gld(L.r, in ? R.k0 + a + b * W : 0);
+    if (in) R.mask |= 1 << c;
   }
 }
 
@@ -1973,37 +2060,54 @@ __device__ __forceinline__ void mg_coarse_cycle_body(const MgCycleArgs& A) {
   constexpr int kB = kMgBottomMax;
   __shared__ double s_bp[kB][kB];
   __shared__ double s_w[16];
-  if (A.sc->done) return;
+  CheckState cs{};
+  {
+    const int done = A.sc->done;
+    if (A.check) cs = check_state(A.sc, A.it);
+    if (kInstance == 0) hold(cs);
+    static_assert(kMgLdsLevels == 5, "the hold below lists five levels");
+    if (kInstance == 0)  // one statement: one round of argument loads
+      asm volatile("" ::"s"(A.lv[0].r), "s"(A.lv[0].e), "s"(A.lv[0].nx), "s"(A.lv[0].ny),
+                   "s"(A.lv[1].nx), "s"(A.lv[1].ny), "s"(A.lv[2].nx), "s"(A.lv[2].ny),
+                   "s"(A.lv[3].nx), "s"(A.lv[3].ny), "s"(A.lv[4].nx), "s"(A.lv[4].ny),
+                   "s"(A.off[0]), "s"(A.off[1]), "s"(A.off[2]), "s"(A.off[3]), "s"(A.off[4]),
+                   "s"(A.nq), "s"(A.inv), "s"(A.pk), "s"(A.oc), "s"(A.part_in), "s"(A.nb_in),
+                   "s"(A.it), "s"(A.check));
+    if (done) return;
+  }
   const int t = threadIdx.x;
   const int nq = A.nq;
-  // operator rows first: their loads overlap the convergence test
+  // The coarsest level's inverse: entry (brow, bcol) in thread brow + kB * bcol,
+  // loaded with the partial (nb_in <= 1024: one per thread) and the operator
+  // rows, whose loads overlap the convergence test.  Every load is issued
+  // unconditionally (rows of levels the hierarchy does not have are loaded
+  // from the table and never used; without the test the partial is read from
+  // the inverse), so that they all share one wait.
+  const MgLevel& B = A.lv[nq - 1];  // coarsest
+  const int nb = B.nx * B.ny;
+  const int brow = t % kB, bcol = t / kB;
+  const bool own_b = t < kB * kB && brow < nb && bcol < nb;
+  const double inv_v = gld((const double*)A.inv, own_b ? (size_t)bcol * nb + brow : 0);
+  const double* pin = A.check ? A.part_in : (const double*)A.inv;
+  const double part_v = gld(pin, A.check && t < A.nb_in ? 3 * (size_t)t : 0);
   AggRows R0;                     // level f
   CellRows R1[kMgLdsLevels - 2];  // levels f+1 .. L-2
   CellRows Bc;                    // the coarsest level's cell t (its position only)
-  if (nq > 1) {
-    lds_agg_rows(A.lv[0], A.lv[1], R0, A.pk, t);
+  lds_agg_rows(A.lv[0], A.lv[1], R0, A.pk, t);
 #pragma unroll
-    for (int q = 1; q < kMgLdsLevels - 1; q++)
-      if (q < nq - 1) lds_cell_rows(A.lv[q], R1[q - 1], A.pk, 4 + q - 1, t);
-  }
+  for (int q = 1; q < kMgLdsLevels - 1; q++) lds_cell_rows(A.lv[q], R1[q - 1], A.pk, 4 + q - 1, t);
   lds_cell_pos(A.lv[nq - 1], Bc, t);
   if (A.check) {  // convergence test on the level-0 CG update's |r/V|^2 partials
-    double v = 0.0;
-    for (int b = t; b < A.nb_in; b += 1024) v += A.part_in[3 * b];
+    double v = t < A.nb_in ? part_v : 0.0;
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
     if ((t & 63) == 0) s_w[t >> 6] = v;
     __syncthreads();
     double rho = 0.0;
     for (int w = 0; w < 16; w++) rho += s_w[w];
-    if (mg_check(A.sc, rho, 0.0, 0, A.it, t == 0)) return;
+    if (mg_check(A.sc, cs, rho, 0.0, 0, A.it, t == 0)) return;
   }
-  const MgLevel& B = A.lv[nq - 1];  // coarsest
-  const int nb = B.nx * B.ny;
-  // the coarsest level's inverse: entry (brow, bcol) in thread brow + kB * bcol
-  const int brow = t % kB, bcol = t / kB;
-  const double binv =
-      (t < kB * kB && brow < nb && bcol < nb) ? (double)A.inv[(size_t)bcol * nb + brow] : 0.0;
+  const double binv = own_b ? inv_v : 0.0;
   for (int k = t; k < kMgLdsX; k += 1024) s_x[k] = 0.0;
   double rb = 0.0;  // right-hand side of the coarsest level at cell t
   if (nq == 1) {
