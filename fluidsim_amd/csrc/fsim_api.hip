@@ -180,6 +180,7 @@ struct fs_handle {
   hipGraphExec_t mg_pro[2] = {};      // MG-PCG set-up and iteration 0 (from zero, warm)
   bool warm_ok = false;               // x holds the last converged MG-PCG correction
   bool moved = false;                 // an accepted transport step since the last solve
+  bool solve_warm = false;            // the MG-PCG solve in flight started from the last correction
   int mg_it = 0, mg_chunk = 0;        // next MG-PCG iteration / chunk of a solve in progress
   hipGraphExec_t mg_chain[2][kChainLog] = {};  // 2^m MG-PCG iterations from parity p
   hipGraphExec_t learn_graph[2] = {}; // one learned-FCG iteration, by parity
@@ -527,6 +528,7 @@ MgArgs mg_down_args(fs_handle* h, size_t l, int it) {
     a.init = 1;
     a.part_in = h->part(0);
     a.nb_in = capped_grid(h->nbx * h->pnby);
+    a.xs = h->pois(PO_X);
   } else if (l == 0) {
     a.it = it;
     a.F.r = r_buf(h, it + 1);
@@ -882,6 +884,7 @@ int lapl_solve_impl(fs_handle* h, int* lerr, fs_solve_stats* stats, int mode = 0
       // the last correction is a useful start only for the next pass's solve:
       // densities have moved since (not for a repeated or a Poisson-only solve)
       const bool warm = h->p.lapl_warm && h->warm_ok && h->moved;
+      h->solve_warm = warm;
       if (!h->p.use_graphs) {
         if (mg_prologue(h, warm)) return FS_E_HIP;
       } else {
@@ -922,6 +925,13 @@ int lapl_solve_impl(fs_handle* h, int* lerr, fs_solve_stats* stats, int mode = 0
   }
   if (int e = check_status(h)) return e;
   int ok = h->h_sc->done == 1;
+  if (!ok && h->solve_warm && h->p.lapl_solver == 1 && h->mg.size() > 1) {
+    // a warm start that did not converge (the potential is unchanged): the
+    // same system again from zero
+    h->solve_warm = false;
+    h->warm_ok = false;
+    return lapl_solve_impl(h, lerr, stats, 0);
+  }
   *lerr = ok ? 0 : 1;
   const int sv = h->p.lapl_solver == 2 ? 2 : (h->p.lapl_solver == 1 && h->mg.size() > 1) ? 1 : 0;
   h->last_iters[sv] = h->h_sc->iters;

@@ -20,6 +20,10 @@ struct DevScalars {
   double cg_gam[2], cg_alpha[2], bnorm2, resid, tol, tres;
   double cg_pq[2];   // p.q of the last two iterations (flexible CG)
   double nn_scale;   // input scale of the learned coarse solver
+  // warm start: x holds the correction of a solve after a transport step of
+  // dt_warm; the next warm start begins from warm_f * x, warm_f = dt_stage /
+  // dt_warm (the correction grows with the step)
+  double dt_warm, warm_f;
   int32_t iters, done, maxit, pad_i;
   int32_t status, accept, attempts, replay_idx;
   int32_t replay_n, pad_j[3];
@@ -159,6 +163,7 @@ struct MgArgs {
   double oc;
   int32_t it, level0, init, check;  // check: convergence test on part_in before the work
   MgCg cg;
+  double* xs;            // level-0 initial test: the iterate, scaled here by warm_f
 };
 
 // Kernels whose per-block partial sums every block of the next kernel reads

@@ -1166,6 +1166,7 @@ __device__ __forceinline__ void hold_mg_args(const MgArgs& A, bool cg) {
   hold(A.level0);
   hold(A.cg.on);
   hold(A.cg.w);
+  hold(A.xs);
   if (cg) {
     hold(A.cg.x);
     hold(A.cg.p);
@@ -1324,6 +1325,7 @@ __device__ __forceinline__ void gs_stage(const MgLevel& F, GsTile<H>& s, int i0,
 // within h <= H - 1 cells of the tile at (i0, j0), entirely in LDS.
 template <int H>
 __device__ __forceinline__ void gs_half(GsTile<H>& s, int i0, int j0, int c, int h) {
+  #pragma clang fp contract(fast)  // smoother arithmetic: FMAs allowed
   const int w2 = (kGsTX + 2 * h) / 2, rows = kGsTY + 2 * h, a0 = H - h;
   for (int k = threadIdx.x; k < rows * w2; k += kNT) {
     const int b = a0 + k / w2;
@@ -1336,6 +1338,7 @@ __device__ __forceinline__ void gs_half(GsTile<H>& s, int i0, int j0, int c, int
 
 template <int H>
 __device__ __forceinline__ double gs_mx(const GsTile<H>& s, int b, int a) {
+  #pragma clang fp contract(fast)  // smoother arithmetic: FMAs allowed
   return s.d[b][a] * s.x[b][a] - s.cx[b][a - 1] * s.x[b][a - 1] - s.cx[b][a] * s.x[b][a + 1] -
          s.cy[b - 1][a] * s.x[b - 1][a] - s.cy[b][a] * s.x[b + 1][a];
 }
@@ -1511,9 +1514,11 @@ __device__ __forceinline__ void mg_down_body(const MgArgs& A, int bid, int nb) {
   DevScalars* sc = A.sc;
   CheckState cs{};
   double gam_old = 1.0, alpha_old = 1.0;  // the previous CG iteration's gamma and alpha
+  double wf = 1.0;                        // warm-start scale of x (initial test)
   {
     const int done = sc->done;
     if (A.check) cs = check_state(sc, A.it);
+    if (A.xs) wf = sc->warm_f;
     if (kCg && !A.cg.first) {
       gam_old = sc->cg_gam[(A.it + 1) & 1];
       alpha_old = sc->cg_alpha[(A.it + 1) & 1];
@@ -1522,13 +1527,32 @@ __device__ __forceinline__ void mg_down_body(const MgArgs& A, int bid, int nb) {
       hold_mg_args(A, kCg);
       hold(nb);
       hold(cs);
-      asm volatile("" ::"s"(gam_old), "s"(alpha_old));
+      asm volatile("" ::"s"(gam_old), "s"(alpha_old), "s"(wf));
     }
     if (done) return;
   }
   const MgLevel &F = A.F, &C = A.C;
   const int ntx = gs_tiles_x(F);
   const TileRange tr = tile_range(gs_ntiles(F), bid, nb);
+  if (A.xs) {
+    // the initial test of MG-PCG: the start is wf x (the init kernel formed
+    // r = rhs - wf M x); scaled here, before the test may stop the solve, since
+    // the init kernel's blocks read x across tiles; and this solve's step
+    // length becomes the reference of the next warm start
+    if (wf != 1.0) {
+      for (int t = tr.t0; t < tr.t1; t++) {
+        const int i0 = (t % ntx) * kGsTX, j0 = (t / ntx) * kGsTY;
+        for (int k = threadIdx.x; k < kGsTX * kGsTY; k += kNT) {
+          const int i = i0 + k % kGsTX, j = j0 + k / kGsTX;
+          if (i < F.nx && j < F.ny) {
+            const size_t m = mg_at(F, i, j);
+            A.xs[m] = wf * A.xs[m];
+          }
+        }
+      }
+    }
+    if (bid == 0 && threadIdx.x == 0) sc->dt_warm = sc->dt_stage;
+  }
   double sacc[3] = {0.0, 0.0, 0.0};  // sum r_c^2 / d_c for the learned coarse solver
   double racc[3] = {0.0, 0.0, 0.0};  // sum |r/V|^2 after the CG update
   if (tr.t0 >= tr.t1) {  // block 0 always has tiles (it writes the scalars)
@@ -1564,11 +1588,18 @@ __device__ __forceinline__ void mg_down_body(const MgArgs& A, int bid, int nb) {
       if (mg_check(sc, cs, tot[0], tot[1], A.init, A.it, bid == 0 && threadIdx.x == 0)) return;
     } else {
       // gamma = r.u, delta = u.M u; p.s = delta - beta gamma / alpha_old
+      // (0 < p.s <= delta in exact arithmetic).  Near convergence the
+      // difference can lose every digit; then the direction restarts from u
+      // (beta = 0, p.s = delta), as in the first iteration.
       const double gam = tot[0], del = tot[1];
       double ps = del;
       if (!A.cg.first) {
         beta = gam / gam_old;
         ps = del - beta * gam / alpha_old;
+        if (!(ps > 0.0) || !isfinite(beta)) {
+          beta = 0.0;
+          ps = del;
+        }
       }
       alpha = gam / ps;
       if (!(ps > 0.0) || !isfinite(alpha)) {  // breakdown: stop, not converged
@@ -1900,10 +1931,12 @@ __device__ __forceinline__ void lds_cell_pos(const MgLevel& L, CellRows& R, int 
 
 __device__ __forceinline__ double lds_gs(double rhs, double l, double r, double b, double t, double dinv,
                                         const double* x, int k, int W) {
+  #pragma clang fp contract(fast)  // smoother arithmetic: FMAs allowed
   return (rhs + l * x[k - 1] + r * x[k + 1] + b * x[k - W] + t * x[k + W]) * dinv;
 }
 __device__ __forceinline__ double lds_res(double rhs, double l, double r, double b, double t, double dinv,
                                          const double* x, int k, int W) {
+  #pragma clang fp contract(fast)  // smoother arithmetic: FMAs allowed
   return rhs - (x[k] / dinv - l * x[k - 1] - r * x[k + 1] - b * x[k - W] - t * x[k + W]);
 }
 
@@ -1942,6 +1975,7 @@ __device__ __forceinline__ int agg_k(const AggRows& R, int W, int c) {
 // One half-sweep of colour COL over the aggregates (children 0, 3 or 1, 2).
 template <int COL>
 __device__ __forceinline__ void agg_half(AggRows& R, double* x, int W) {
+  #pragma clang fp contract(fast)  // smoother arithmetic: FMAs allowed
   constexpr int c0 = COL == 0 ? 0 : 1, c1 = COL == 0 ? 3 : 2;
   double v[2];
 #pragma unroll
@@ -1968,6 +2002,7 @@ __device__ __forceinline__ void agg_half(AggRows& R, double* x, int W) {
 // Pre-smoothing from zero (red pointwise, then black, red, black); returns the
 // aggregate's residual sum, the right-hand side of cell t of level f+1.
 __device__ __forceinline__ double agg_down(AggRows& R, double* x, int W) {
+  #pragma clang fp contract(fast)  // smoother arithmetic: FMAs allowed
 #pragma unroll
   for (int c = 0; c < 4; c += 3)
     if (R.mask & (1 << c)) {
@@ -1993,6 +2028,7 @@ __device__ __forceinline__ double agg_down(AggRows& R, double* x, int W) {
 // over-correction, post-smoothing (black, red, black, red), and the result.
 __device__ __forceinline__ void agg_up(AggRows& R, double* x, int W, double e, double oc,
                                        double* out) {
+  #pragma clang fp contract(fast)  // smoother arithmetic: FMAs allowed
 #pragma unroll
   for (int c = 0; c < 4; c++)
     if (R.mask & (1 << c)) {
@@ -2011,12 +2047,14 @@ __device__ __forceinline__ void agg_up(AggRows& R, double* x, int W, double e, d
 
 template <int COL>
 __device__ __forceinline__ void cell_half(const CellRows& R, double* x, int W) {
+  #pragma clang fp contract(fast)  // smoother arithmetic: FMAs allowed
   if (R.k >= 0 && (((R.ij & 0xffff) + (R.ij >> 16)) & 1) == COL)
     x[R.k] = lds_gs(R.rhs, R.l, R.r, R.b, R.t, R.dinv, x, R.k, W);
   __syncthreads();
 }
 
 __device__ __forceinline__ void cell_down(const CellRows& R, double* x, double* s_r, int W) {
+  #pragma clang fp contract(fast)  // smoother arithmetic: FMAs allowed
   const bool red = R.k >= 0 && (((R.ij & 0xffff) + (R.ij >> 16)) & 1) == 0;
   if (red) x[R.k] = R.rhs * R.dinv;
   __syncthreads();
@@ -2042,6 +2080,7 @@ __device__ __forceinline__ double lds_restrict(const MgLevel& L, const CellRows&
 
 __device__ __forceinline__ void cell_up(const CellRows& R, double* x, const double* xc, int W,
                                         int Wc, double oc) {
+  #pragma clang fp contract(fast)  // smoother arithmetic: FMAs allowed
   if (R.k >= 0) {
     const int i = R.ij & 0xffff, j = R.ij >> 16;
     x[R.k] += oc * xc[((j >> 1) + 1) * Wc + (i >> 1) + 1];
@@ -2176,6 +2215,15 @@ __device__ __forceinline__ void mgpcg_init_body(const MgPcgArgs& A, int bid, int
   const Geo& G = A.G;
   const int P = G.P;
   const TileRange tr = tile_range(A.nbx * A.nby, bid, nb);
+  // warm: start from f x, f = dt_stage / dt_warm (the initial convergence test
+  // kernel scales x itself: other blocks read it here)
+  double f = 1.0;
+  if (A.warm) {
+    const double dw = A.sc->dt_warm, ds = A.sc->dt_stage;
+    const double q = ds / dw;
+    if (dw > 0.0 && ds > 0.0 && isfinite(q)) f = q;
+  }
+  if (bid == 0 && threadIdx.x == 0) A.sc->warm_f = f;
   double acc[3] = {0.0, 0.0, 0.0};
   for (int t = tr.t0; t < tr.t1; t++) {
     for (int k = 0; k < kPcgRows; k++) {
@@ -2187,8 +2235,8 @@ __device__ __forceinline__ void mgpcg_init_body(const MgPcgArgs& A, int bid, int
         double r = rhs;
         if (A.warm) {
           const double* x = A.x;
-          r = rhs - (A.dg[m] * x[m] - A.cx[m - 1] * x[m - 1] - A.cx[m] * x[m + 1] -
-                     A.cy[m - P] * x[m - P] - A.cy[m] * x[m + P]);
+          r = rhs - f * (A.dg[m] * x[m] - A.cx[m - 1] * x[m - 1] - A.cx[m] * x[m + 1] -
+                         A.cy[m - P] * x[m - P] - A.cy[m] * x[m + P]);
         }
         A.p_new[m] = 0.0;
         const double V = G.dlx[ci] * G.dly[cj] * (G.cyl ? G.r[cj + 1] : 1.0);
