@@ -20,10 +20,14 @@ struct DevScalars {
   double cg_gam[2], cg_alpha[2], bnorm2, resid, tol, tres;
   double cg_pq[2];   // p.q of the last two iterations (flexible CG)
   double nn_scale;   // input scale of the learned coarse solver
-  // warm start: x holds the correction of a solve after a transport step of
-  // dt_warm; the next warm start begins from warm_f * x, warm_f = dt_stage /
-  // dt_warm (the correction grows with the step)
-  double dt_warm, warm_f;
+  // warm start: x holds the correction of the last solve, after a transport
+  // step of dt_warm, x2 (when x2_ok) the one before, after dt_warm2.  The next
+  // solve starts from warm_a x + warm_b x2: the corrections per unit step
+  // length extrapolated linearly to the new step (or, without x2, x scaled by
+  // the step-length ratio).  warm_now: the solve in flight starts warm.
+  double dt_warm, dt_warm2, warm_a, warm_b;
+  int32_t warm_now, x2_ok;
+  int32_t itmax, pad_w;  // iteration limit of the MG-PCG solve in flight (set by its init kernel)
   int32_t iters, done, maxit, pad_i;
   int32_t status, accept, attempts, replay_idx;
   int32_t replay_n, pad_j[3];
@@ -163,7 +167,9 @@ struct MgArgs {
   double oc;
   int32_t it, level0, init, check;  // check: convergence test on part_in before the work
   MgCg cg;
-  double* xs;            // level-0 initial test: the iterate, scaled here by warm_f
+  double* xs;            // level-0 initial test: the iterate x and the previous
+  double* xs2;           // correction x2, advanced here (warm_a, warm_b)
+  const double* rhs0;    // level-0 initial test: the right-hand side (zero start)
 };
 
 // Kernels whose per-block partial sums every block of the next kernel reads
@@ -215,6 +221,7 @@ struct MgPcgArgs {
   Geo G;
   int32_t nbx, nby, first, flex;  // flex: Polak-Ribiere beta (flexible CG)
   int32_t warm, pad_;             // init: start from the x in place (r = rhs - M x)
+  const double* x2;               // init: the correction before x
 };
 
 constexpr int kTX = 64;  // tile width (one wave64 per tile row)

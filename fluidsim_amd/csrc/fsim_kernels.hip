@@ -1167,6 +1167,7 @@ __device__ __forceinline__ void hold_mg_args(const MgArgs& A, bool cg) {
   hold(A.cg.on);
   hold(A.cg.w);
   hold(A.xs);
+  hold(A.xs2);
   if (cg) {
     hold(A.cg.x);
     hold(A.cg.p);
@@ -1354,7 +1355,7 @@ struct CheckState {
   int32_t itc_prev, maxit;
 };
 __device__ __forceinline__ CheckState check_state(const DevScalars* sc, int it) {
-  return {sc->tol, sc->bnorm2, sc->itc[(it + 1) & 1], sc->maxit};
+  return {sc->tol, sc->bnorm2, sc->itc[(it + 1) & 1], sc->itmax};
 }
 __device__ __forceinline__ void hold(const CheckState& c) {
   asm volatile("" ::"s"(c.tol), "s"(c.bnorm2), "s"(c.itc_prev), "s"(c.maxit));
@@ -1514,11 +1515,16 @@ __device__ __forceinline__ void mg_down_body(const MgArgs& A, int bid, int nb) {
   DevScalars* sc = A.sc;
   CheckState cs{};
   double gam_old = 1.0, alpha_old = 1.0;  // the previous CG iteration's gamma and alpha
-  double wf = 1.0;                        // warm-start scale of x (initial test)
+  double wa = 1.0, wb = 0.0;  // warm start x = wa x + wb x2 (initial test)
+  int wn = 0;
   {
     const int done = sc->done;
     if (A.check) cs = check_state(sc, A.it);
-    if (A.xs) wf = sc->warm_f;
+    if (A.xs) {
+      wa = sc->warm_a;
+      wb = sc->warm_b;
+      wn = sc->warm_now;
+    }
     if (kCg && !A.cg.first) {
       gam_old = sc->cg_gam[(A.it + 1) & 1];
       alpha_old = sc->cg_alpha[(A.it + 1) & 1];
@@ -1527,7 +1533,7 @@ __device__ __forceinline__ void mg_down_body(const MgArgs& A, int bid, int nb) {
       hold_mg_args(A, kCg);
       hold(nb);
       hold(cs);
-      asm volatile("" ::"s"(gam_old), "s"(alpha_old), "s"(wf));
+      asm volatile("" ::"s"(gam_old), "s"(alpha_old), "s"(wa), "s"(wb), "s"(wn));
     }
     if (done) return;
   }
@@ -1535,23 +1541,30 @@ __device__ __forceinline__ void mg_down_body(const MgArgs& A, int bid, int nb) {
   const int ntx = gs_tiles_x(F);
   const TileRange tr = tile_range(gs_ntiles(F), bid, nb);
   if (A.xs) {
-    // the initial test of MG-PCG: the start is wf x (the init kernel formed
-    // r = rhs - wf M x); scaled here, before the test may stop the solve, since
-    // the init kernel's blocks read x across tiles; and this solve's step
-    // length becomes the reference of the next warm start
-    if (wf != 1.0) {
+    // the initial test of MG-PCG.  A warm start is wa x + wb x2 (the init
+    // kernel formed r = rhs - M (wa x + wb x2)); it is formed here, before the
+    // test may stop the solve, since the init kernel's blocks read x and x2
+    // across tiles; x2 takes the last correction.  This solve's step length
+    // becomes the reference of the next warm start.
+    if (wn) {
       for (int t = tr.t0; t < tr.t1; t++) {
         const int i0 = (t % ntx) * kGsTX, j0 = (t / ntx) * kGsTY;
         for (int k = threadIdx.x; k < kGsTX * kGsTY; k += kNT) {
           const int i = i0 + k % kGsTX, j = j0 + k / kGsTX;
           if (i < F.nx && j < F.ny) {
             const size_t m = mg_at(F, i, j);
-            A.xs[m] = wf * A.xs[m];
+            const double x = A.xs[m], y = A.xs2[m];
+            A.xs2[m] = x;
+            A.xs[m] = wb != 0.0 ? wa * x + wb * y : wa * x;
           }
         }
       }
     }
-    if (bid == 0 && threadIdx.x == 0) sc->dt_warm = sc->dt_stage;
+    if (bid == 0 && threadIdx.x == 0) {
+      sc->dt_warm2 = sc->dt_warm;
+      sc->dt_warm = sc->dt_stage;
+      sc->x2_ok = wn;
+    }
   }
   double sacc[3] = {0.0, 0.0, 0.0};  // sum r_c^2 / d_c for the learned coarse solver
   double racc[3] = {0.0, 0.0, 0.0};  // sum |r/V|^2 after the CG update
@@ -1581,10 +1594,16 @@ __device__ __forceinline__ void mg_down_body(const MgArgs& A, int bid, int nb) {
   double pv[3];
   if (red) part3_sum(pl, A.nb_in, pv);
   double alpha = 0.0, beta = 0.0;
+  bool zero_start = false;  // a warm start no better than zero, discarded
   if (red) {
     double tot[3];
     finish_partials3(pv, s_red, tot);
     if (A.check) {
+      // (every block decides identically from the same partials)
+      if (A.xs && wn && !(tot[0] < tot[1])) {
+        zero_start = true;
+        tot[0] = tot[1];
+      }
       if (mg_check(sc, cs, tot[0], tot[1], A.init, A.it, bid == 0 && threadIdx.x == 0)) return;
     } else {
       // gamma = r.u, delta = u.M u; p.s = delta - beta gamma / alpha_old
@@ -1612,12 +1631,29 @@ __device__ __forceinline__ void mg_down_body(const MgArgs& A, int bid, int nb) {
       }
     }
   }
+  if (zero_start) {
+    // x = 0 and r = rhs on this block's cells; every block stages r from rhs
+    // (the neighbours' own r may not be rewritten yet)
+    for (int t = tr.t0; t < tr.t1; t++) {
+      const int i0 = (t % ntx) * kGsTX, j0 = (t / ntx) * kGsTY;
+      for (int k = threadIdx.x; k < kGsTX * kGsTY; k += kNT) {
+        const int i = i0 + k % kGsTX, j = j0 + k / kGsTX;
+        if (i < F.nx && j < F.ny) {
+          const size_t m = mg_at(F, i, j);
+          A.xs[m] = 0.0;
+          F.r[m] = A.rhs0[m];
+        }
+      }
+    }
+  }
+  MgLevel Fs = F;  // the level as staged (r from rhs after a discarded start)
+  if (zero_start) Fs.r = const_cast<double*>(A.rhs0);
   for (int t = tr.t0; t < tr.t1; t++) {
     const int i0 = (t % ntx) * kGsTX, j0 = (t / ntx) * kGsTY;
-    if (t > tr.t0) {
+    if (t > tr.t0 || zero_start) {
       __syncthreads();
       if (kCg) cg_stage(A, s, i0, j0, cc);
-      else gs_stage<kGsH, false>(F, s, i0, j0, C, 0.0);
+      else gs_stage<kGsH, false>(Fs, s, i0, j0, C, 0.0);
     }
     if (kCg) cg_update(A, s, i0, j0, cc, alpha, beta, racc[0]);
     __syncthreads();
@@ -2215,15 +2251,41 @@ __device__ __forceinline__ void mgpcg_init_body(const MgPcgArgs& A, int bid, int
   const Geo& G = A.G;
   const int P = G.P;
   const TileRange tr = tile_range(A.nbx * A.nby, bid, nb);
-  // warm: start from f x, f = dt_stage / dt_warm (the initial convergence test
-  // kernel scales x itself: other blocks read it here)
-  double f = 1.0;
+  // warm: start from a x + b x2 (the initial convergence test kernel forms it
+  // in place, since other blocks read x and x2 here).  With g = correction /
+  // step length sampled at the midpoints of the last two steps (dt1 = dt_warm,
+  // dt2 = dt_warm2), extrapolated to the midpoint of this one (dt = dt_stage):
+  // c = (dt + dt1) / (dt1 + dt2), a = (1 + c) dt / dt1, b = -c dt / dt2;
+  // without x2, a = dt / dt1 (at most kWarmRatio), b = 0.  The extrapolation
+  // only for step-length ratios dt / dt1 and dt1 / dt2 within [1/kWarmRatio,
+  // kWarmRatio]: a correction of a much shorter step (after a rejected one)
+  // carries the solver's tolerance in a large relative error, which the
+  // extrapolation would amplify.  A warm solve stops after kWarmIters
+  // iterations (its caller then solves from zero).
+  constexpr double kWarmRatio = 3.0;
+  constexpr int kWarmIters = 64;
+  auto moderate = [](double q) { return q >= 1.0 / kWarmRatio && q <= kWarmRatio; };
+  double a = 1.0, b = 0.0;
   if (A.warm) {
-    const double dw = A.sc->dt_warm, ds = A.sc->dt_stage;
-    const double q = ds / dw;
-    if (dw > 0.0 && ds > 0.0 && isfinite(q)) f = q;
+    const DevScalars* sc = A.sc;
+    const double dt = sc->dt_stage, dt1 = sc->dt_warm, dt2 = sc->dt_warm2;
+    const double a1 = dt / dt1;
+    if (dt > 0.0 && dt1 > 0.0 && isfinite(a1)) {
+      a = fmin(a1, kWarmRatio);
+      const double q = dt1 / dt2;
+      if (moderate(a1) && sc->x2_ok && dt2 > 0.0 && moderate(q)) {
+        const double c = (dt + dt1) / (dt1 + dt2);
+        a = (1.0 + c) * a1;
+        b = -c * (dt / dt2);
+      }
+    }
   }
-  if (bid == 0 && threadIdx.x == 0) A.sc->warm_f = f;
+  if (bid == 0 && threadIdx.x == 0) {
+    A.sc->warm_a = a;
+    A.sc->warm_b = b;
+    A.sc->warm_now = A.warm;
+    A.sc->itmax = A.warm ? min(A.sc->maxit, kWarmIters) : A.sc->maxit;
+  }
   double acc[3] = {0.0, 0.0, 0.0};
   for (int t = tr.t0; t < tr.t1; t++) {
     for (int k = 0; k < kPcgRows; k++) {
@@ -2235,8 +2297,13 @@ __device__ __forceinline__ void mgpcg_init_body(const MgPcgArgs& A, int bid, int
         double r = rhs;
         if (A.warm) {
           const double* x = A.x;
-          r = rhs - f * (A.dg[m] * x[m] - A.cx[m - 1] * x[m - 1] - A.cx[m] * x[m + 1] -
-                         A.cy[m - P] * x[m - P] - A.cy[m] * x[m + P]);
+          const double* y = A.x2;
+          double mx = a * (A.dg[m] * x[m] - A.cx[m - 1] * x[m - 1] - A.cx[m] * x[m + 1] -
+                           A.cy[m - P] * x[m - P] - A.cy[m] * x[m + P]);
+          if (b != 0.0)
+            mx += b * (A.dg[m] * y[m] - A.cx[m - 1] * y[m - 1] - A.cx[m] * y[m + 1] -
+                       A.cy[m - P] * y[m - P] - A.cy[m] * y[m + P]);
+          r = rhs - mx;
         }
         A.p_new[m] = 0.0;
         const double V = G.dlx[ci] * G.dly[cj] * (G.cyl ? G.r[cj + 1] : 1.0);
