@@ -269,7 +269,7 @@ def main():
             "config": {"workload": f"{args.members} argon discharges, sides U{{64..256}}, "
                                    "150-400 V DC, 0.5-10 Torr log-uniform, gamma 0.05-0.15",
                        "members": args.members, "cells_total": e["cells"],
-                       "poisson_initial_guess": "zero" if args.cold else "previous pass's correction"},
+                       "poisson_initial_guess": "zero" if args.cold else "extrapolated from the last two passes' corrections"},
             "accepted_steps": e["accepted"], "ptcl_attempts": e["attempts"],
             "member_passes_per_s": args.members * args.steps / e["elapsed"],
             "solo_sequential_steps_per_s": e["solo_rate"], "solo_sample": e["solo_sample"],
@@ -318,7 +318,7 @@ def main():
                    "nx": args.n, "ny": args.n, "poisson_rtol": args.rtol,
                    "poisson_solver": "CG preconditioned by an aggregation multigrid V(2,2) "
                                      "cycle with red-black Gauss-Seidel smoothing",
-                   "poisson_initial_guess": "zero" if args.cold else "previous pass's correction"},
+                   "poisson_initial_guess": "zero" if args.cold else "extrapolated from the last two passes' corrections"},
         "accepted_steps": r["accepted"],
         "ptcl_attempts": r["attempts"],
         "pcg_iters_mean": sum(iters) / max(len(iters), 1),

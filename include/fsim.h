@@ -77,9 +77,10 @@ typedef struct fs_params {
   int32_t lapl_solver;    /* 0 Jacobi-PCG, 1 multigrid-preconditioned PCG, 2 learned
                              two-grid flexible CG (fs_learn_attach) */
   int32_t use_graphs;     /* replay the PCG iterations from a captured HIP graph */
-  int32_t lapl_warm;      /* 1: a multigrid-PCG solve starts from the previous converged
-                             correction instead of zero (the reference's PETSc KSP starts
-                             from zero); fs_set_field and fs_set_solver reset it */
+  int32_t lapl_warm;      /* 1: a multigrid-PCG solve starts from the last two converged
+                             corrections extrapolated to the new step, instead of zero
+                             (the reference's PETSc KSP starts from zero); fs_set_field
+                             and fs_set_solver reset it */
 } fs_params;
 
 /* Per-attempt controller record (replay and log), 10 doubles:
