@@ -224,7 +224,7 @@ __device__ __forceinline__ void stage_body(const StageArgs& A, const FsPhys& K, 
           te = 1e-8;
           s_bad = 1;
         }
-        xa = fit_x(te * K.ph0);
+        xa = fit_x_stage(te * K.ph0);
         mue = mue_of_x_folded(K, xa);
         de = de_of_x_folded(K, xa);
       }
@@ -2465,8 +2465,8 @@ __global__ void k_fill(double* p, double v, size_t n) {
 __global__ void k_eval_fits(FsPhys K, const double* T, double* out, int n) {
   int m = blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= n) return;
-  double t = T[m], v = t * K.ph0, xa = fit_x(v);
-  Rates R = cell_rates(K, v, xa);
+  double t = T[m], v = t * K.ph0, xa = fit_x(v), xs = fit_x_stage(v);
+  Rates R = cell_rates(K, v, xs);  // as the stage kernel evaluates them
   double* o = out + 12 * (size_t)m;
   o[0] = mue_of_x(K, xa);
   o[1] = mut_of_x(K, xa);
@@ -2478,8 +2478,8 @@ __global__ void k_eval_fits(FsPhys K, const double* T, double* out, int n) {
   o[7] = R.k_sc;
   o[8] = R.k_si;
   o[9] = R.beta;
-  o[10] = 0.0;
-  o[11] = 0.0;
+  o[10] = mue_of_x_folded(K, xs);  // the stage kernel's mobility and diffusion
+  o[11] = de_of_x_folded(K, xs);
 }
 
 __global__ void k_eval_flux(FsPhys K, const double* a, double* out, int n, int dfdph) {

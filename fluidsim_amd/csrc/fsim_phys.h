@@ -9,7 +9,8 @@
 // conversions, beta uses exp10 for 10**y, and the stage kernel multiplies by
 // reciprocal grid spacings and shares one SG exponential between electrons and
 // electron energy (their exponent arguments are equal up to rounding:
-// mut/Dt = mue/De).  The Poisson assembly and the mobility/diffusion fits keep
+// mut/Dt = mue/De).  The stage kernel's logarithm is fs_log below (within one
+// ulp, like the library's).  The Poisson assembly and the mobility/diffusion fits keep
 // the reference's order exactly: the assembled residual is a small difference
 // of large terms, where ulp changes in E or in the mobility show at 1e-9.
 //
@@ -69,6 +70,49 @@ void fs_phys_table(const FsPhys* K, double* out40);
 #define F32(v) ((double)(v##f))
 
 // ---------------------------------------------------------------------------
+// log for the stage kernel's arguments in fewer operations than the general
+// library function (no special cases: normal positive arguments); within one
+// ulp of the exact value and correctly rounded for nearly all arguments (a CPU
+// test compares it with long-double libm).
+//
+// log: v = 2^e m with m in [sqrt(1/2), sqrt(2)), f = m - 1 (exact); with
+// s = f / (2 + f), log m = 2 artanh(s) = f - (f^2/2 - s (f^2/2 + R)),
+// R = 2 s^2/3 + 2 s^4/5 + ... (the Taylor series of artanh to s^21; |s| <= 0.1716,
+// truncation below 2^-56 relative), so the leading term f is exact and the
+// rounding errors sit in a correction at most a fifth of the result;
+// log v = e ln2 + log m (the order of the sums as in Sun's fdlibm).  For normal
+// v > 0.
+__host__ __device__ inline double fs_log(double v) {
+  constexpr double kLn2Hi = 0.6931471803691238;
+  constexpr double kLn2Lo = 1.9082149292705877e-10;
+  int e;
+  double m = frexp(v, &e);  // [0.5, 1)
+  if (m < 0.7071067811865476) {
+    m = m * 2.0;
+    e = e - 1;
+  }
+  const double f = m - 1.0;
+  const double s = f / (2.0 + f);
+  const double z = s * s;
+  double q = 2.0 / 21.0;
+  q = fma(q, z, 2.0 / 19.0);
+  q = fma(q, z, 2.0 / 17.0);
+  q = fma(q, z, 2.0 / 15.0);
+  q = fma(q, z, 2.0 / 13.0);
+  q = fma(q, z, 2.0 / 11.0);
+  q = fma(q, z, 2.0 / 9.0);
+  q = fma(q, z, 2.0 / 7.0);
+  q = fma(q, z, 2.0 / 5.0);
+  q = fma(q, z, 2.0 / 3.0);
+  const double R = z * q;
+  const double hfsq = 0.5 * f * f;
+  const double de = (double)e;
+  // e ln2_lo joins the correction before f, and e ln2_hi (exact) comes last:
+  // log v = e ln2_hi - ((f^2/2 - (s (f^2/2 + R) + e ln2_lo)) - f)
+  return de * kLn2Hi - ((hfsq - (s * (hfsq + R) + de * kLn2Lo)) - f);
+}
+
+// ---------------------------------------------------------------------------
 // integer powers by gfortran's optimal power tree: x^n = x^(n-t) * x^t
 struct Pow11 {
   double x, x2, x3, x4, x5, x6, x7, x8, x9, x10, x11;
@@ -93,6 +137,11 @@ __device__ __forceinline__ Pow11 pow_chain(double x) {
 __device__ __forceinline__ double fit_x(double v) {
   double m = v < 1.57e2 ? v : 1.57e2;
   return log(2.34e-1 > m ? 2.34e-1 : m);
+}
+// the stage kernel's form (fs_log)
+__device__ __forceinline__ double fit_x_stage(double v) {
+  double m = v < 1.57e2 ? v : 1.57e2;
+  return fs_log(2.34e-1 > m ? 2.34e-1 : m);
 }
 
 // get_mue from x (ptcl_props.F90:194-221): exponent, then the value

@@ -177,7 +177,8 @@ int fs_lapl_assemble(fs_handle* h, double* b, double* A5);
 /* Constant table (40 entries, the order of the CPU oracle's table). */
 int fs_constants(double p_torr, double gam, double* out40);
 /* Device evaluation of the fits for known-answer tests: out[12*n]:
- * mue, mut, De, Dt, k_ex, k_ir, nu, k_sc, k_si, beta, 0, 0 */
+ * mue, mut, De, Dt (the Poisson assembly's forms), k_ex, k_ir, nu, k_sc, k_si,
+ * beta, mue, De (the rates and the last two as the stage kernel evaluates them) */
 int fs_eval_fits(double p_torr, int32_t n, const double* T, double* out);
 /* Device SG flux / dFdPh: args7 = {E, dh, q, mu, D, nl, nr} per entry. */
 int fs_eval_flux(int32_t n, const double* args7, double* out);

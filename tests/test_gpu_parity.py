@@ -39,6 +39,11 @@ def test_fits_match_oracle(O):
         for col in range(10):
             r = np.abs(a[:, col] - b[:, col]) / np.abs(b[:, col])
             assert r.max() < 1e-11, (p, col, r.max())
+        # the stage kernel's own mobility and diffusion (folded conversions,
+        # its exp and log) against the reference's get_mue and get_De
+        for col, ref in ((10, 0), (11, 2)):
+            r = np.abs(a[:, col] - b[:, ref]) / np.abs(b[:, ref])
+            assert r.max() < 1e-11, (p, col, r.max())
 
 
 def test_sg_flux_and_dfdph_match_oracle(O):
