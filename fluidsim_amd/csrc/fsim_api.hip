@@ -134,6 +134,7 @@ void fs_phys_table(const FsPhys* K, double* o) {
 
 // ---------------------------------------------------------------------------
 constexpr int kChainLog = 5;  // MG-PCG iteration graphs of 1, 2, 4, 8, 16 iterations
+constexpr int kSolveGraphs = 4;  // whole-first-chunk solve graphs: 1 .. 4 iterations after 0
 
 struct fs_handle {
   fs_grid_desc g;
@@ -178,6 +179,9 @@ struct fs_handle {
   LaplArgs* d_ltab = nullptr;
   FsPhys* d_phys = nullptr;
   hipGraphExec_t mg_pro[2] = {};      // MG-PCG set-up and iteration 0 (from zero, warm)
+  // set-up, iteration 0, k = 1 .. kSolveGraphs further iterations and the
+  // finishing kernels in one graph (from zero, warm)
+  hipGraphExec_t mg_solve[2][kSolveGraphs] = {};
   bool warm_ok = false;               // x holds the last converged MG-PCG correction
   bool moved = false;                 // an accepted transport step since the last solve
   bool solve_warm = false;            // the MG-PCG solve in flight started from the last correction
@@ -709,6 +713,7 @@ void drop_graphs(fs_handle* h) {
   drop(h->mg_pro[0]);
   drop(h->mg_pro[1]);
   for (int k = 0; k < 2; k++) {
+    for (int c = 0; c < kSolveGraphs; c++) drop(h->mg_solve[k][c]);
     drop(h->learn_graph[k]);
     for (int m = 0; m < kChainLog; m++) drop(h->mg_chain[k][m]);
   }
@@ -890,21 +895,39 @@ int lapl_solve_impl(fs_handle* h, int* lerr, fs_solve_stats* stats, int mode = 0
       // densities have moved since (not for a repeated or a Poisson-only solve)
       const bool warm = h->p.lapl_warm && h->warm_ok && h->moved;
       h->solve_warm = warm;
-      if (!h->p.use_graphs) {
-        if (mg_prologue(h, warm)) return FS_E_HIP;
-      } else {
-        hipGraphExec_t& g = h->mg_pro[warm ? 1 : 0];
-        if (!g && capture(h, &g, [&]() { return mg_prologue(h, warm); })) return FS_E_HIP;
-        HIP_OK(hipGraphLaunch(g, h->st));
-      }
-      // iteration 0 is launched already; then as many as the last solve took,
-      // then 2 more, then chunk / 4 at a time
+      // iteration 0 comes with the set-up; then as many as the last solve
+      // took, then 2 more, then chunk / 4 at a time.  A short first chunk is
+      // one graph with the set-up and the finishing kernels (each graph
+      // boundary costs microseconds of idle device).
       mg_it = 1;
       mg_chunk = std::max(1, first_chunk(h->last_iters[1], chunk > 8 ? chunk / 4 : chunk) - 1);
-      if (launch_mg_chunk(h, mg_it, mg_chunk)) return FS_E_HIP;
+      if (h->p.use_graphs && mg_chunk <= kSolveGraphs) {
+        hipGraphExec_t& g = h->mg_solve[warm ? 1 : 0][mg_chunk - 1];
+        const int n = mg_chunk;
+        if (!g && capture(h, &g, [&]() {
+              if (mg_prologue(h, warm)) return FS_E_HIP;
+              for (int k = 0; k < n; k++)
+                if (launch_vcycle(h, 1 + k)) return FS_E_HIP;
+              launch_pcg_finish(pa, h->st);
+              return 0;
+            }))
+          return FS_E_HIP;
+        HIP_OK(hipGraphLaunch(g, h->st));
+        time_mark(h, 6);
+        if (mode == 0 && sync_scalars(h)) return FS_E_HIP;
+      } else {
+        if (!h->p.use_graphs) {
+          if (mg_prologue(h, warm)) return FS_E_HIP;
+        } else {
+          hipGraphExec_t& g = h->mg_pro[warm ? 1 : 0];
+          if (!g && capture(h, &g, [&]() { return mg_prologue(h, warm); })) return FS_E_HIP;
+          HIP_OK(hipGraphLaunch(g, h->st));
+        }
+        if (launch_mg_chunk(h, mg_it, mg_chunk)) return FS_E_HIP;
+        if (int e = end_chunk(mode == 0)) return e;
+      }
       mg_it += mg_chunk;
       mg_chunk = 2;
-      if (int e = end_chunk(mode == 0)) return e;
       if (mode == 1) return 0;
     }
     for (;;) {
