@@ -1347,28 +1347,45 @@ int ens_round(fs_ensemble* E, std::vector<int>& left) {
     sel2[i] = i * 2 + (E->m[i]->p.lapl_warm && E->m[i]->warm_ok && E->m[i]->moved ? 1 : 0);
   }
   if (!npois) return 0;
-  if (ens_upload_round(E)) return FS_E_HIP;
-  launch_ens_lapl(E->t_lapl, E->t_phys, with_mask(E->m_stage, r.pois), EnsSel{r.sel, 1, 0}, E->st);
-  for (int l = 0; l < E->maxF; l++)
-    launch_ens_mg_coarsen(E->t_coarsen, with_mask(E->m_coarsen[l], r.pois),
-                          EnsSel{nullptr, kMgMaxLevels, l}, E->st);
-  launch_ens_mg_coarse_setup(E->t_cc, with_mask(E->m_one, r.pois), EnsSel{nullptr, 4, 0}, E->st);
-  launch_ens_mgpcg_init(E->t_init, with_mask(E->m_pcg, r.pois), EnsSel{r.sel2, 1, 0}, E->st);
-  ens_vcycle(E, 0, r.pois);
-  ens_iteration(E, 0, r.pois);
-  int it = 1;
-  for (;;) {
-    for (int k = 0; k < 8; k++, it++) ens_iteration(E, it, r.pois);
-    HIP_OK(hipGetLastError());
-    if (ens_sync(E)) return FS_E_HIP;
-    bool all = true;
-    for (int i = 0; i < n; i++) {
-      if (!pois[i]) continue;
-      if (E->h_sc[i].status < 0) return ens_fail(E, i);
-      all = all && E->h_sc[i].done;
+  // The members' solves; then, as a member's own solve does, the warm solves
+  // that did not converge again from zero (the pois mask then holds just
+  // those members, and is restored for the finishing kernels).
+  std::vector<uint8_t> all_pois(pois, pois + n);
+  for (int round = 0; round < 2; round++) {
+    if (ens_upload_round(E)) return FS_E_HIP;
+    launch_ens_lapl(E->t_lapl, E->t_phys, with_mask(E->m_stage, r.pois), EnsSel{r.sel, 1, 0}, E->st);
+    for (int l = 0; l < E->maxF; l++)
+      launch_ens_mg_coarsen(E->t_coarsen, with_mask(E->m_coarsen[l], r.pois),
+                            EnsSel{nullptr, kMgMaxLevels, l}, E->st);
+    launch_ens_mg_coarse_setup(E->t_cc, with_mask(E->m_one, r.pois), EnsSel{nullptr, 4, 0}, E->st);
+    launch_ens_mgpcg_init(E->t_init, with_mask(E->m_pcg, r.pois), EnsSel{r.sel2, 1, 0}, E->st);
+    ens_vcycle(E, 0, r.pois);
+    ens_iteration(E, 0, r.pois);
+    int it = 1;
+    for (;;) {
+      for (int k = 0; k < 8; k++, it++) ens_iteration(E, it, r.pois);
+      HIP_OK(hipGetLastError());
+      if (ens_sync(E)) return FS_E_HIP;
+      bool all = true;
+      for (int i = 0; i < n; i++) {
+        if (!pois[i]) continue;
+        if (E->h_sc[i].status < 0) return ens_fail(E, i);
+        all = all && E->h_sc[i].done;
+      }
+      if (all) break;
     }
-    if (all) break;
+    if (round == 1) break;
+    int nretry = 0;
+    for (int i = 0; i < n; i++) {
+      const bool warm = (sel2[i] & 1) != 0;
+      pois[i] = pois[i] && warm && E->h_sc[i].done != 1;
+      nretry += pois[i];
+      sel2[i] = i * 2;  // from zero
+    }
+    if (!nretry) break;
   }
+  memcpy(pois, all_pois.data(), n);
+  if (ens_upload_round(E)) return FS_E_HIP;
   launch_ens_pcg_finish(E->t_fin, with_mask(E->m_pcg, r.pois), with_mask(E->m_one, r.pois),
                         EnsSel{nullptr, 1, 0}, E->st);
   if (ens_sync(E)) return FS_E_HIP;
