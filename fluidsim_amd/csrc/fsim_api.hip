@@ -163,7 +163,7 @@ struct fs_handle {
   int nreplay = 0;
   int rot = 0;        // level rotation: A = buf[rot], L1 = buf[rot+1], L3 = buf[rot+2]
   bool alias = true;  // level 1 currently equals level 2 (after accept/reject/init)
-  int nbx = 0, nby = 0, nblk = 0;   // stage / assembly tiles (64x4)
+  int nbx = 0, nby = 0, nblk = 0;   // stage / assembly tiles (64x8)
   int pnby = 0, pnblk = 0;          // PCG tiles (64x8)
   std::vector<MgLevel> mg;          // multigrid hierarchy; level 0 aliases the PCG planes
   double* d_mg = nullptr;
@@ -953,12 +953,23 @@ int lapl_solve_impl(fs_handle* h, int* lerr, fs_solve_stats* stats, int mode = 0
   }
   if (int e = check_status(h)) return e;
   int ok = h->h_sc->done == 1;
-  if (!ok && h->solve_warm && h->p.lapl_solver == 1 && h->mg.size() > 1) {
-    // a warm start that did not converge (the potential is unchanged): the
-    // same system again from zero
-    h->solve_warm = false;
-    h->warm_ok = false;
-    return lapl_solve_impl(h, lerr, stats, 0);
+  if (!ok && h->p.lapl_solver == 1 && h->mg.size() > 1) {
+    if (h->solve_warm) {
+      // a warm start that did not converge (the potential is unchanged): the
+      // same system again from zero
+      h->solve_warm = false;
+      h->warm_ok = false;
+      return lapl_solve_impl(h, lerr, stats, 0);
+    }
+    // From zero too: the same system by Jacobi-PCG before the step is
+    // rejected.  dFdPh's cancellation at a near-zero field can turn a face
+    // coupling negative and the operator indefinite; the multigrid-PCG
+    // iteration steps along negative curvature, but a smoother that amplifies
+    // the negative mode strongly can still keep it from converging.
+    h->p.lapl_solver = 0;
+    const int e = lapl_solve_impl(h, lerr, stats, 0);
+    h->p.lapl_solver = 1;
+    return e;
   }
   *lerr = ok ? 0 : 1;
   const int sv = h->p.lapl_solver == 2 ? 2 : (h->p.lapl_solver == 1 && h->mg.size() > 1) ? 1 : 0;
@@ -1517,7 +1528,7 @@ int fs_create(const fs_grid_desc* g, const fs_params* p, fs_handle** out) {
   }
   h->nblk = 0;
   h->nbx = (bx + kTX - 1) / kTX;
-  h->nby = (by + kTY - 1) / kTY;
+  h->nby = (by + kSTY - 1) / kSTY;
   h->nblk = h->nbx * h->nby;
   h->pnby = (by + kPcgTY - 1) / kPcgTY;
   h->pnblk = h->nbx * h->pnby;

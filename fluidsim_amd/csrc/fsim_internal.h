@@ -227,6 +227,10 @@ struct MgPcgArgs {
 constexpr int kTX = 64;  // tile width (one wave64 per tile row)
 constexpr int kTY = 4;   // tile height
 constexpr int kNT = kTX * kTY;
+// The stage and assembly kernels stage a tile plus a one-cell halo: taller
+// tiles (64x8, 512 threads) recompute fewer halo cells per cell.
+constexpr int kSTY = 8;
+constexpr int kSNT = kTX * kSTY;
 constexpr int kPcgTY = 8;    // PCG tile height: each thread owns kPcgRows cells
 constexpr int kPcgRows = kPcgTY / kTY;
 

@@ -2,7 +2,7 @@
 //
 //   k_stage      one RK-Merson stage: ptclEqn (ptcl_lib.F90:99-550) fused with
 //                rk_step (ptcl_props.F90:71-138) and the embedded error sum.
-//                64x4 tiles; densities, potential and per-cell transport
+//                64x8 tiles; densities, potential and per-cell transport
 //                properties of the tile plus a one-cell halo are staged in
 //                LDS, each face flux is computed once and shared by the two
 //                cells it separates.
@@ -23,10 +23,12 @@
 
 namespace {
 
-constexpr int kHX = kTX + 2, kHY = kTY + 2, kNH = kHX * kHY;
-constexpr int kNFX = (kTX + 1) * kTY, kNFY = kTX * (kTY + 1);
+// stage / assembly tiles (kTX x kSTY cells, kSNT threads) and their halo
+constexpr int kHX = kTX + 2, kHY = kSTY + 2, kNH = kHX * kHY;
+constexpr int kNFX = (kTX + 1) * kSTY, kNFY = kTX * (kSTY + 1);
 constexpr int kNF = kNFX > kNFY ? kNFX : kNFY;
 constexpr int kNW = kNT / 64;
+constexpr int kSNW = kSNT / 64;
 
 // Device pointers held in argument blocks in device memory are generic
 // pointers, and accesses through them compile to flat instructions, whose
@@ -65,7 +67,8 @@ __device__ __forceinline__ TileRange tile_range(int ntiles, int bid, int nb) {
   return {t0, t0 + per < ntiles ? t0 + per : ntiles};
 }
 
-template <int NV>
+// Block-wide sums (s_red: NV * NW doubles) and minimum of NW waves.
+template <int NV, int NW = kNW>
 __device__ __forceinline__ void block_sum(double (&v)[NV], double* s_red) {
 #pragma unroll
   for (int k = 0; k < NV; k++) {
@@ -77,18 +80,19 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double* s_red) {
   int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   if (lane == 0)
 #pragma unroll
-    for (int k = 0; k < NV; k++) s_red[k * kNW + w] = v[k];
+    for (int k = 0; k < NV; k++) s_red[k * NW + w] = v[k];
   __syncthreads();
   if (threadIdx.x == 0)
 #pragma unroll
     for (int k = 0; k < NV; k++) {
-      double a = s_red[k * kNW];
-      for (int q = 1; q < kNW; q++) a += s_red[k * kNW + q];
+      double a = s_red[k * NW];
+      for (int q = 1; q < NW; q++) a += s_red[k * NW + q];
       v[k] = a;
     }
   __syncthreads();
 }
 
+template <int NW = kNW>
 __device__ __forceinline__ double block_min(double v, double* s_red) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_down(v, off, 64));
@@ -96,7 +100,7 @@ __device__ __forceinline__ double block_min(double v, double* s_red) {
   if (lane == 0) s_red[w] = v;
   __syncthreads();
   if (threadIdx.x == 0)
-    for (int q = 1; q < kNW; q++) v = fmin(v, s_red[q]);
+    for (int q = 1; q < NW; q++) v = fmin(v, s_red[q]);
   __syncthreads();
   return v;
 }
@@ -194,19 +198,19 @@ __device__ __forceinline__ void stage_body(const StageArgs& A, const FsPhys& K, 
   __shared__ double s_ni[kNH], s_ne[kNH], s_nt[kNH], s_nm[kNH], s_ph[kNH];
   __shared__ double s_te[kNH], s_mue[kNH], s_de[kNH], s_xa[kNH];
   __shared__ double s_f[4][kNF];
-  __shared__ double s_red[8 * kNW];
+  __shared__ double s_red[8 * kSNW];
   __shared__ int s_bad;
 
   const Geo& G = A.G;
   const int bx = G.bx, by = G.by, P = G.P;
   const int t = xcd_tile(bid, nb);
-  const int i0 = (t % A.nbx) * kTX, j0 = (t / A.nbx) * kTY;
+  const int i0 = (t % A.nbx) * kTX, j0 = (t / A.nbx) * kSTY;
   const double dt = A.sc->dt;
   if (threadIdx.x == 0) s_bad = 0;
   __syncthreads();
 
   // ---- phase 1: stage densities + potential of tile and halo; cell properties
-  for (int h = threadIdx.x; h < kNH; h += kNT) {
+  for (int h = threadIdx.x; h < kNH; h += kSNT) {
     int hx = h % kHX, hy = h / kHX;
     int ci = i0 - 1 + hx, cj = j0 - 1 + hy;
     double vi = 0, ve = 0, vt = 0, vm = 0, vp = 0, te = 0, mue = 0, de = 0, xa = 0;
@@ -260,7 +264,7 @@ __device__ __forceinline__ void stage_body(const StageArgs& A, const FsPhys& K, 
   __syncthreads();
 
   // ---- phase 2: x faces (left face of cells i0..i0+TX, rows j0..j0+TY-1)
-  for (int f = threadIdx.x; f < kNFX; f += kNT) {
+  for (int f = threadIdx.x; f < kNFX; f += kSNT) {
     int fx = f % (kTX + 1), fy = f / (kTX + 1);
     int fi = i0 + fx, fj = j0 + fy;
     Flux4 F = {0, 0, 0, 0};
@@ -321,7 +325,7 @@ __device__ __forceinline__ void stage_body(const StageArgs& A, const FsPhys& K, 
   __syncthreads();
 
   // ---- phase 2b: y faces (bottom face of rows j0..j0+TY, columns i0..i0+TX-1)
-  for (int f = threadIdx.x; f < kNFY; f += kNT) {
+  for (int f = threadIdx.x; f < kNFY; f += kSNT) {
     int fx = f % kTX, fy = f / kTX;
     int fci = i0 + fx, fj = j0 + fy;
     Flux4 F = {0, 0, 0, 0};
@@ -426,8 +430,8 @@ __device__ __forceinline__ void stage_body(const StageArgs& A, const FsPhys& K, 
     }
   }
   if (!A.rhs_only && stage == 5) {
-    block_sum<4>(acc, s_red);
-    double tm = block_min(tmin, s_red);
+    block_sum<4, kSNW>(acc, s_red);
+    double tm = block_min<kSNW>(tmin, s_red);
     if (threadIdx.x == 0) {
       const size_t p = (size_t)bid * 8;
       gst(A.part, p + 0, acc[0]);
@@ -445,7 +449,7 @@ __device__ __forceinline__ void stage_body(const StageArgs& A, const FsPhys& K, 
 // Arguments from device memory: kept there per handle, they are read with
 // scalar loads where used (passed by value, the compiler keeps the whole
 // argument block live in SGPRs and spills it to VGPR lanes).
-__global__ void __launch_bounds__(kNT) k_stage(const StageArgs* __restrict__ A,
+__global__ void __launch_bounds__(kSNT, 4) k_stage(const StageArgs* __restrict__ A,
                                                const FsPhys* __restrict__ K) {
   stage_body<0>(*A, *K, blockIdx.x, gridDim.x);
 }
@@ -623,7 +627,7 @@ template <int kInstance>
 __device__ __forceinline__ void lapl_body(const LaplArgs& A, const FsPhys& K, int bid, int nb) {
   __shared__ double s_ni[kNH], s_ne[kNH], s_ph[kNH], s_te[kNH], s_mue[kNH], s_de[kNH];
   __shared__ double s_f[4][kNF];
-  __shared__ double s_red[8 * kNW];
+  __shared__ double s_red[8 * kSNW];
   __shared__ int s_bad;
   if (A.gate && A.sc->accept == 0) {  // the attempt was rejected: no solve
     if (bid == 0 && threadIdx.x == 0) A.sc->done = 3;
@@ -632,12 +636,12 @@ __device__ __forceinline__ void lapl_body(const LaplArgs& A, const FsPhys& K, in
   const Geo& G = A.G;
   const int bx = G.bx, by = G.by, P = G.P;
   const int t = xcd_tile(bid, nb);
-  const int i0 = (t % A.nbx) * kTX, j0 = (t / A.nbx) * kTY;
+  const int i0 = (t % A.nbx) * kTX, j0 = (t / A.nbx) * kSTY;
   const double dt = A.sc->dt;
   if (threadIdx.x == 0) s_bad = 0;
   __syncthreads();
 
-  for (int h = threadIdx.x; h < kNH; h += kNT) {
+  for (int h = threadIdx.x; h < kNH; h += kSNT) {
     int hx = h % kHX, hy = h / kHX;
     int ci = i0 - 1 + hx, cj = j0 - 1 + hy;
     double vi = 0, ve = 0, vp = 0, te = 0, mue = 0, de = 0;
@@ -669,7 +673,7 @@ __device__ __forceinline__ void lapl_body(const LaplArgs& A, const FsPhys& K, in
   __syncthreads();
 
   // x faces: fluxes (laplEqn) and Jacobian terms (laplJ)
-  for (int f = threadIdx.x; f < kNFX; f += kNT) {
+  for (int f = threadIdx.x; f < kNFX; f += kSNT) {
     int fx = f % (kTX + 1), fy = f / (kTX + 1);
     int fi = i0 + fx, cj = j0 + fy;
     FaceJ F = {0, 0, 0, 0};
@@ -766,7 +770,7 @@ __device__ __forceinline__ void lapl_body(const LaplArgs& A, const FsPhys& K, in
   __syncthreads();
 
   // y faces (those on y = 0 and y = w carry nothing: type_y = -1 / 1)
-  for (int f = threadIdx.x; f < kNFY; f += kNT) {
+  for (int f = threadIdx.x; f < kNFY; f += kSNT) {
     int fx = f % kTX, fy = f / kTX;
     int fci = i0 + fx, fj = j0 + fy;
     FaceJ F = {0, 0, 0, 0};
@@ -879,7 +883,7 @@ __device__ __forceinline__ void lapl_body(const LaplArgs& A, const FsPhys& K, in
   }
 }
 
-__global__ void __launch_bounds__(kNT) k_lapl(const LaplArgs* __restrict__ A,
+__global__ void __launch_bounds__(kSNT, 4) k_lapl(const LaplArgs* __restrict__ A,
                                               const FsPhys* __restrict__ K) {
   lapl_body<0>(*A, *K, blockIdx.x, gridDim.x);
 }
@@ -1607,9 +1611,14 @@ __device__ __forceinline__ void mg_down_body(const MgArgs& A, int bid, int nb) {
       if (mg_check(sc, cs, tot[0], tot[1], A.init, A.it, bid == 0 && threadIdx.x == 0)) return;
     } else {
       // gamma = r.u, delta = u.M u; p.s = delta - beta gamma / alpha_old
-      // (0 < p.s <= delta in exact arithmetic).  Near convergence the
-      // difference can lose every digit; then the direction restarts from u
-      // (beta = 0, p.s = delta), as in the first iteration.
+      // (0 < p.s <= delta in exact arithmetic for a definite operator).  Near
+      // convergence the difference can lose every digit; then the direction
+      // restarts from u (beta = 0, p.s = delta), as in the first iteration.
+      // The assembled operator is not always definite: dFdPh's two terms
+      // cancel at a near-zero field (lapl_lib.F90:625-631) and can leave a
+      // face coupling of the wrong sign.  A direction of negative curvature
+      // (p.s < 0 after the restart) is then stepped along (alpha < 0), as CG
+      // does on a symmetric indefinite system; only p.s = 0 stops the solve.
       const double gam = tot[0], del = tot[1];
       double ps = del;
       if (!A.cg.first) {
@@ -1621,7 +1630,7 @@ __device__ __forceinline__ void mg_down_body(const MgArgs& A, int bid, int nb) {
         }
       }
       alpha = gam / ps;
-      if (!(ps > 0.0) || !isfinite(alpha)) {  // breakdown: stop, not converged
+      if (ps == 0.0 || !isfinite(alpha)) {  // breakdown: stop, not converged
         if (bid == 0 && threadIdx.x == 0) sc->done = 2;
         return;
       }
@@ -2507,7 +2516,7 @@ __device__ __forceinline__ const T& ens_arg(const T* t, const EnsSel& s, int i) 
   return t[(size_t)(s.sel ? s.sel[i] : i) * s.stride + s.off];
 }
 
-__global__ void __launch_bounds__(kNT) kb_stage(const StageArgs* t, const FsPhys* K, EnsMap m,
+__global__ void __launch_bounds__(kSNT, 4) kb_stage(const StageArgs* t, const FsPhys* K, EnsMap m,
                                                 EnsSel s) {
   int i, bid, nb;
   if (ens_slot(m, i, bid, nb)) stage_body<1>(ens_arg(t, s, i), K[i], bid, nb);
@@ -2539,7 +2548,7 @@ __global__ void kb_scalars(DevScalars* sc, const uint8_t* mask, int n, int op) {
     c->Vd_mi = c->Vd_or;
   }
 }
-__global__ void __launch_bounds__(kNT) kb_lapl(const LaplArgs* t, const FsPhys* K, EnsMap m,
+__global__ void __launch_bounds__(kSNT, 4) kb_lapl(const LaplArgs* t, const FsPhys* K, EnsMap m,
                                                EnsSel s) {
   int i, bid, nb;
   if (ens_slot(m, i, bid, nb)) lapl_body<1>(ens_arg(t, s, i), K[i], bid, nb);
@@ -2585,12 +2594,12 @@ __global__ void __launch_bounds__(kNT) kb_pcg_tres(const PcgArgs* t, EnsMap one,
 
 static inline int nblocks_of(const Geo& G, int* nbx, int* nby) {
   *nbx = (G.bx + kTX - 1) / kTX;
-  *nby = (G.by + kTY - 1) / kTY;
+  *nby = (G.by + kSTY - 1) / kSTY;
   return *nbx * *nby;
 }
 
 void launch_stage(const StageArgs* a, const FsPhys* K, int nblocks, hipStream_t s) {
-  hipLaunchKernelGGL(k_stage, dim3(nblocks), dim3(kNT), 0, s, a, K);
+  hipLaunchKernelGGL(k_stage, dim3(nblocks), dim3(kSNT), 0, s, a, K);
 }
 void launch_circ(const CircArgs& a, const FsPhys& K, hipStream_t s) {
   hipLaunchKernelGGL(k_circ, dim3(1), dim3(kNT), 0, s, a, K);
@@ -2608,7 +2617,7 @@ void launch_begin_step(DevScalars* sc, hipStream_t s) {
   hipLaunchKernelGGL(k_begin_step, dim3(1), dim3(1), 0, s, sc);
 }
 void launch_lapl_assemble(const LaplArgs* a, const FsPhys* K, int nblocks, hipStream_t s) {
-  hipLaunchKernelGGL(k_lapl, dim3(nblocks), dim3(kNT), 0, s, a, K);
+  hipLaunchKernelGGL(k_lapl, dim3(nblocks), dim3(kSNT), 0, s, a, K);
 }
 void launch_pcg_init(const PcgArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(k_pcg_init, dim3(capped_grid(a.nbx * a.nby)), dim3(kNT), 0, s, a);
@@ -2668,7 +2677,7 @@ void launch_eval_flux(const FsPhys& K, const double* a, double* out, int n, int 
 
 void launch_ens_stage(const StageArgs* t, const FsPhys* K, const EnsMap& m, const EnsSel& s,
                       hipStream_t st) {
-  if (m.nblocks) hipLaunchKernelGGL(kb_stage, dim3(m.nblocks), dim3(kNT), 0, st, t, K, m, s);
+  if (m.nblocks) hipLaunchKernelGGL(kb_stage, dim3(m.nblocks), dim3(kSNT), 0, st, t, K, m, s);
 }
 void launch_ens_circ(const CircArgs* t, const FsPhys* K, const EnsMap& m, const EnsSel& s,
                      hipStream_t st) {
@@ -2682,7 +2691,7 @@ void launch_ens_scalars(DevScalars* sc, const uint8_t* mask, int n, int op, hipS
 }
 void launch_ens_lapl(const LaplArgs* t, const FsPhys* K, const EnsMap& m, const EnsSel& s,
                      hipStream_t st) {
-  if (m.nblocks) hipLaunchKernelGGL(kb_lapl, dim3(m.nblocks), dim3(kNT), 0, st, t, K, m, s);
+  if (m.nblocks) hipLaunchKernelGGL(kb_lapl, dim3(m.nblocks), dim3(kSNT), 0, st, t, K, m, s);
 }
 void launch_ens_mg_coarsen(const MgArgs* t, const EnsMap& m, const EnsSel& s, hipStream_t st) {
   if (m.nblocks) hipLaunchKernelGGL(kb_mg_coarsen, dim3(m.nblocks), dim3(256), 0, st, t, m, s);
