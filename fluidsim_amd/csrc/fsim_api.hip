@@ -7,6 +7,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <string>
 #include <vector>
@@ -157,6 +158,11 @@ struct fs_handle {
   double* d_part = nullptr;
   DevScalars* d_sc = nullptr;
   DevScalars* h_sc = nullptr;  // pinned mirror
+  DevScalars* h_pub = nullptr; // host-mapped copy written by k_publish (and its device address)
+  DevScalars* d_pub = nullptr;
+  uint32_t* d_pubcnt = nullptr;  // publications run (device)
+  uint32_t pub_tok = 0;          // publications enqueued (host)
+  bool spec_taken = false;       // the next pass's k_begin_step and stage 1 are enqueued and run
   double* d_pois = nullptr;    // rhs0 r0 cx cy dg x p r1 w0 w1 s0 s1 (kPois planes)
   double* d_dbg = nullptr;     // b (N) + A5 (5N)
   double* d_replay = nullptr;
@@ -247,10 +253,45 @@ int upload(fs_handle* h, double* dst, const double* src, size_t n) {
   return 0;
 }
 
-int sync_scalars(fs_handle* h) {
-  HIP_OK(hipMemcpyAsync(h->h_sc, h->d_sc, sizeof(DevScalars), hipMemcpyDeviceToHost, h->st));
-  HIP_OK(hipStreamSynchronize(h->st));
+// Waits until the last publication enqueued (k_publish) has reached host
+// memory, and copies it to h_sc.  The poll asks the stream every 1024 rounds,
+// so that an error, or a stream that drained without publishing (which would
+// be a bug), ends the wait.
+int wait_published(fs_handle* h) {
+  const uint32_t tok = h->pub_tok;
+  const volatile uint32_t* seq = &h->h_pub->pub_seq;
+  for (unsigned k = 1; *seq != tok; k++) {
+    if ((k & 1023) == 0) {
+      const hipError_t q = hipStreamQuery(h->st);
+      if (q == hipErrorNotReady) continue;
+      if (q != hipSuccess) HIP_OK(q);
+      if (*seq != tok) return fail(FS_E_HIP, "scalars were not published%s");
+      break;
+    }
+  }
+  std::atomic_thread_fence(std::memory_order_acquire);
+  memcpy(h->h_sc, const_cast<const DevScalars*>(h->h_pub), sizeof(DevScalars));
   return 0;
+}
+
+void enqueue_publish(fs_handle* h) {
+  launch_publish(h->d_sc, h->d_pub, h->d_pubcnt, h->st);
+  h->pub_tok++;
+}
+
+// Reads the device scalars once the work enqueued so far has run.  With
+// host-mapped memory (and no timing events to read) a kernel publishes them:
+// no copy command and no stream synchronisation on the path from the last
+// kernel to the next launch.
+int sync_scalars(fs_handle* h) {
+  if (!h->h_pub || h->timing) {
+    HIP_OK(hipMemcpyAsync(h->h_sc, h->d_sc, sizeof(DevScalars), hipMemcpyDeviceToHost, h->st));
+    HIP_OK(hipStreamSynchronize(h->st));
+    return 0;
+  }
+  enqueue_publish(h);
+  HIP_OK(hipGetLastError());
+  return wait_published(h);
 }
 
 int push_scalars(fs_handle* h) {
@@ -789,8 +830,11 @@ int build_arg_tables(fs_handle* h) {
   return 0;
 }
 
+const StageArgs* stab_at(const fs_handle* h, int rot, bool alias, int stage, int variant) {
+  return h->d_stab + ((rot * 2 + (alias ? 1 : 0)) * 3 + variant) * 5 + stage - 1;
+}
 const StageArgs* stab(const fs_handle* h, int stage, int variant) {
-  return h->d_stab + ((h->rot * 2 + (h->alias ? 1 : 0)) * 3 + variant) * 5 + stage - 1;
+  return stab_at(h, h->rot, h->alias, stage, variant);
 }
 const LaplArgs* ltab(const fs_handle* h, bool gated = false) {
   return h->d_ltab + (gated ? 6 : 0) + h->rot * 2 + (h->alias ? 1 : 0);
@@ -1008,14 +1052,23 @@ CtrlArgs ctrl_args(fs_handle* h) {
 
 // ahead: after each attempt's controller kernel, issue the Poisson solve that
 // follows an accepted attempt (lapl_solve_impl mode 1) before reading the
-// controller's decision.
-int ptcl_step_impl(fs_handle* h, fs_step_info* info, bool ahead = false) {
-  launch_begin_step(h->d_sc, h->st);
+// controller's decision.  spec_next (another pass follows in the same run):
+// behind the solve's first chunk, also issue the next pass's accept, begin and
+// first stage, gated on the device by k_accept_spec; the host reads the same
+// scalars and knows whether they ran.  The device does not then wait for the
+// host between passes that converge in their first chunk.
+int ptcl_step_impl(fs_handle* h, fs_step_info* info, bool ahead = false,
+                   bool spec_next = false) {
+  const bool pre = h->spec_taken;  // this pass's begin and stage 1 are issued already
+  h->spec_taken = false;
+  spec_next = spec_next && ahead && h->h_pub && !h->timing;
+  if (!pre) launch_begin_step(h->d_sc, h->st);
   int attempts = 0;
   for (;;) {
     time_mark(h, 0);
     for (int stage = 1; stage <= 5; stage++) {
-      launch_stage(stab(h, stage, 0), h->d_phys, h->nbx * h->nby, h->st);
+      if (!(pre && attempts == 0 && stage == 1))
+        launch_stage(stab(h, stage, 0), h->d_phys, h->nbx * h->nby, h->st);
       CircArgs c = circ_args(h, stage);
       launch_circ(c, h->K, h->st);
     }
@@ -1029,7 +1082,19 @@ int ptcl_step_impl(fs_handle* h, fs_step_info* info, bool ahead = false) {
       h->moved = true;   // it runs only if this attempt is accepted
       if (int e = lapl_solve_impl(h, &lerr_ahead, nullptr, 1)) return e;
     }
-    if (sync_scalars(h)) return FS_E_HIP;
+    if (spec_next) {
+      enqueue_publish(h);
+      launch_accept_spec(h->d_sc, h->st);
+      launch_begin_step(h->d_sc, h->st, 1);
+      launch_stage(stab_at(h, (h->rot + 1) % 3, true, 1, 0), h->d_phys, h->nbx * h->nby, h->st,
+                   1);
+      HIP_OK(hipGetLastError());
+      if (wait_published(h)) return FS_E_HIP;
+      const DevScalars& s = *h->h_sc;  // what k_accept_spec reads
+      h->spec_taken = s.status >= 0 && s.accept == 1 && s.done == 1;
+    } else if (sync_scalars(h)) {
+      return FS_E_HIP;
+    }
     if (h->timing) {
       float ms = 0;
       (void)hipEventElapsedTime(&ms, h->ev[0], h->ev[1]);
@@ -1066,7 +1131,7 @@ int ptcl_step_impl(fs_handle* h, fs_step_info* info, bool ahead = false) {
 }
 
 int accept_impl(fs_handle* h) {
-  launch_accept(h->d_sc, h->st);
+  if (!h->spec_taken) launch_accept(h->d_sc, h->st);  // else k_accept_spec did it
   HIP_OK(hipGetLastError());
   h->rot = (h->rot + 1) % 3;
   h->alias = true;
@@ -1550,10 +1615,15 @@ int fs_create(const fs_grid_desc* g, const fs_params* p, fs_handle** out) {
       bad(hipMalloc(&h->d_part, 3 * ((size_t)h->nblk + 64) * 8 * sizeof(double))) ||
       bad(hipMalloc(&h->d_sc, sizeof(DevScalars))) ||
       bad(hipHostMalloc(&h->h_sc, sizeof(DevScalars), hipHostMallocDefault)) ||
+      bad(hipHostMalloc(&h->h_pub, sizeof(DevScalars),
+                        hipHostMallocMapped | hipHostMallocCoherent)) ||
+      bad(hipHostGetDevicePointer((void**)&h->d_pub, h->h_pub, 0)) ||
+      bad(hipMalloc(&h->d_pubcnt, 256)) || bad(hipMemset(h->d_pubcnt, 0, 256)) ||
       bad(hipMalloc(&h->d_pois, kPois * PL * sizeof(double)))) {
     fs_destroy(h);
     return FS_E_HIP;
   }
+  memset(h->h_pub, 0, sizeof(DevScalars));  // publication count 0, as *d_pubcnt
   if (p->debug_keep_k) {
     if (bad(hipMalloc(&h->d_kdbg, 20 * PL * sizeof(double))) ||
         bad(hipMalloc(&h->d_dbg, 6 * (size_t)bx * by * sizeof(double)))) {
@@ -1640,6 +1710,8 @@ int fs_destroy(fs_handle* h) {
   (void)hipFree(h->d_part);
   (void)hipFree(h->d_sc);
   (void)hipHostFree(h->h_sc);
+  if (h->h_pub) (void)hipHostFree(h->h_pub);
+  (void)hipFree(h->d_pubcnt);
   (void)hipFree(h->d_pois);
   (void)hipFree(h->d_dbg);
   (void)hipFree(h->d_replay);
@@ -1844,7 +1916,7 @@ int fs_run(fs_handle* h, int32_t n_iters) {
     int lerr = 0;
     if (h->h_sc->ts > 1.0) {  // main.F90:75
       const bool ahead = can_issue_ahead(h);
-      if (int e = ptcl_step_impl(h, nullptr, ahead)) return e;
+      if (int e = ptcl_step_impl(h, nullptr, ahead, it + 1 < n_iters)) return e;
       if (int e = lapl_solve_impl(h, &lerr, nullptr, ahead ? 2 : 0)) return e;
     } else {
       launch_begin_step(h->d_sc, h->st);

@@ -33,6 +33,8 @@ struct DevScalars {
   int32_t replay_n, pad_j[3];
   int32_t itc[2], pad_k[2];   // MG-PCG iteration count by parity
   uint32_t ticket[8];
+  uint32_t pub_seq;  // host-memory copy only: the publication count (k_publish)
+  int32_t spec;      // k_accept_spec accepted the last pass: its gated followers run
 };
 
 // Grid metrics on the device, 0-based over interior cells ci in [0,bx), cj in
@@ -278,12 +280,15 @@ void launch_ens_pcg_finish(const PcgArgs* t, const EnsMap& m, const EnsMap& one,
                            hipStream_t st);
 
 // a, K: device pointers (argument tables kept per handle)
-void launch_stage(const StageArgs* a, const FsPhys* K, int nblocks, hipStream_t s);
+void launch_stage(const StageArgs* a, const FsPhys* K, int nblocks, hipStream_t s,
+                  int gated = 0);
 void launch_circ(const CircArgs& a, const FsPhys& K, hipStream_t s);
 void launch_ctrl(const CtrlArgs& a, hipStream_t s);
 void launch_accept(DevScalars* sc, hipStream_t s);
 void launch_reject(DevScalars* sc, hipStream_t s);
-void launch_begin_step(DevScalars* sc, hipStream_t s);
+void launch_begin_step(DevScalars* sc, hipStream_t s, int gated = 0);
+void launch_accept_spec(DevScalars* sc, hipStream_t s);
+void launch_publish(const DevScalars* sc, DevScalars* pub, uint32_t* cnt, hipStream_t s);
 void launch_lapl_assemble(const LaplArgs* a, const FsPhys* K, int nblocks, hipStream_t s);
 void launch_pcg_init(const PcgArgs& a, hipStream_t s);
 void launch_pcg_it(const PcgArgs& a, hipStream_t s);

@@ -449,8 +449,10 @@ __device__ __forceinline__ void stage_body(const StageArgs& A, const FsPhys& K, 
 // Arguments from device memory: kept there per handle, they are read with
 // scalar loads where used (passed by value, the compiler keeps the whole
 // argument block live in SGPRs and spills it to VGPR lanes).
+// gated: as k_begin_step.
 __global__ void __launch_bounds__(kSNT, 4) k_stage(const StageArgs* __restrict__ A,
-                                               const FsPhys* __restrict__ K) {
+                                               const FsPhys* __restrict__ K, int gated) {
+  if (gated && A->sc->spec == 0) return;
   stage_body<0>(*A, *K, blockIdx.x, gridDim.x);
 }
 
@@ -584,16 +586,53 @@ __device__ __forceinline__ void ctrl_body(const CtrlArgs& A) {
 
 __global__ void __launch_bounds__(kNT) k_ctrl(CtrlArgs A) { ctrl_body<0>(A); }
 
-__global__ void k_begin_step(DevScalars* sc) {
+// gated: issued before the host has read the last pass's outcome; runs only
+// if k_accept_spec accepted that pass.
+__global__ void k_begin_step(DevScalars* sc, int gated) {
+  if (gated && sc->spec == 0) return;
   sc->t_m = (double)1e9f;  // main.F90:74
   sc->attempts = 0;
 }
 
-__global__ void k_accept(DevScalars* sc) {  // main.F90:83-101
+// Copies the scalars to host-mapped memory and then, after a system-scope
+// fence, the count of publications so far (kept on the device in *cnt, so that
+// a publication captured in a graph counts too): the host polls for the count
+// it expects instead of synchronising the stream, then reads the copy (one
+// wave, vector stores).
+__global__ void __launch_bounds__(64) k_publish(const DevScalars* sc, DevScalars* pub,
+                                                uint32_t* cnt) {
+  constexpr int kWords = sizeof(DevScalars) / sizeof(uint64_t);
+  static_assert(sizeof(DevScalars) % sizeof(uint64_t) == 0, "DevScalars is copied as 64-bit words");
+  const uint32_t seq = *cnt + 1u;
+  const uint64_t* src = reinterpret_cast<const uint64_t*>(sc);
+  uint64_t* dst = reinterpret_cast<uint64_t*>(pub);
+  for (int k = threadIdx.x; k < kWords; k += 64) dst[k] = src[k];
+  __threadfence_system();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    *cnt = seq;
+    *reinterpret_cast<volatile uint32_t*>(&pub->pub_seq) = seq;
+    __threadfence_system();
+  }
+}
+
+__device__ __forceinline__ void accept_body(DevScalars* sc) {  // main.F90:83-101
   sc->ts = sc->ts + 1.0;
   sc->t = sc->t + sc->dt;
   sc->Vd_or = sc->Vd_mi;
   sc->Vd_mi = sc->Vd_pl;
+}
+
+__global__ void k_accept(DevScalars* sc) { accept_body(sc); }
+
+// The accept of a pass, issued before the host has read the pass's outcome:
+// applied when the host accepts the pass on the scalars this kernel reads (no
+// error, the attempt accepted, its solve converged in the chunk just run), and
+// recorded in spec for the gated kernels of the next pass that follow it.
+__global__ void k_accept_spec(DevScalars* sc) {
+  const int go = sc->status >= 0 && sc->accept == 1 && sc->done == 1;
+  sc->spec = go;
+  if (go) accept_body(sc);
 }
 
 __global__ void k_reject(DevScalars* sc) {  // main.F90:103-121
@@ -2598,8 +2637,8 @@ static inline int nblocks_of(const Geo& G, int* nbx, int* nby) {
   return *nbx * *nby;
 }
 
-void launch_stage(const StageArgs* a, const FsPhys* K, int nblocks, hipStream_t s) {
-  hipLaunchKernelGGL(k_stage, dim3(nblocks), dim3(kSNT), 0, s, a, K);
+void launch_stage(const StageArgs* a, const FsPhys* K, int nblocks, hipStream_t s, int gated) {
+  hipLaunchKernelGGL(k_stage, dim3(nblocks), dim3(kSNT), 0, s, a, K, gated);
 }
 void launch_circ(const CircArgs& a, const FsPhys& K, hipStream_t s) {
   hipLaunchKernelGGL(k_circ, dim3(1), dim3(kNT), 0, s, a, K);
@@ -2613,8 +2652,14 @@ void launch_accept(DevScalars* sc, hipStream_t s) {
 void launch_reject(DevScalars* sc, hipStream_t s) {
   hipLaunchKernelGGL(k_reject, dim3(1), dim3(1), 0, s, sc);
 }
-void launch_begin_step(DevScalars* sc, hipStream_t s) {
-  hipLaunchKernelGGL(k_begin_step, dim3(1), dim3(1), 0, s, sc);
+void launch_begin_step(DevScalars* sc, hipStream_t s, int gated) {
+  hipLaunchKernelGGL(k_begin_step, dim3(1), dim3(1), 0, s, sc, gated);
+}
+void launch_accept_spec(DevScalars* sc, hipStream_t s) {
+  hipLaunchKernelGGL(k_accept_spec, dim3(1), dim3(1), 0, s, sc);
+}
+void launch_publish(const DevScalars* sc, DevScalars* pub, uint32_t* cnt, hipStream_t s) {
+  hipLaunchKernelGGL(k_publish, dim3(1), dim3(64), 0, s, sc, pub, cnt);
 }
 void launch_lapl_assemble(const LaplArgs* a, const FsPhys* K, int nblocks, hipStream_t s) {
   hipLaunchKernelGGL(k_lapl, dim3(nblocks), dim3(kSNT), 0, s, a, K);

@@ -253,3 +253,27 @@ def test_indefinite_operator_solved_by_multigrid_pcg(O):
     assert st.iters <= 40, st.iters
     e = rel_l2(d.get("ph")[1:-1, 1 : nx + 1] - ph[1:-1, 1 : nx + 1], x)
     assert e <= 1e-8, e
+
+
+def test_run_issuing_passes_ahead_equals_single_passes(O):
+    """Within one fs_run call the next pass's accept, begin and first stage are
+    issued behind each solve's first chunk, gated on the device by that pass's
+    outcome; one pass per call issues nothing ahead.  Both must give the same
+    fields, scalars and log bit for bit, over passes whose solves need more
+    than their first chunk (the gate closed) and passes that need only it."""
+    from fluidsim_amd import Discharge, configs
+
+    c = configs.config1(128, rtol=1e-10, chunk=32, warm=1)
+    a, b = Discharge(c), Discharge(c)
+    a.init_gaussian(seed=3)
+    b.init_gaussian(seed=3)
+    a.run(40)
+    for _ in range(40):
+        b.run(1)
+    for f in ("ni", "ne", "nt", "nm", "ph"):
+        assert np.array_equal(a.get(f, 1), b.get(f, 1)), f
+    assert a.scalars() == b.scalars()
+    la, lb = a.log(), b.log()
+    assert np.array_equal(la, lb)
+    iters = la[la[:, 6] >= 0, 7]
+    assert len(set(iters.tolist())) > 1, iters  # first chunks both long enough and too short
