@@ -607,6 +607,12 @@ MgArgs mg_up_args(fs_handle* h, size_t l, int it) {
   return a;
 }
 
+// Multigrid levels the assembly coarsens itself (those of the tile levels
+// whose aggregates lie inside one assembly tile); k_mg_coarsen makes the rest.
+size_t lapl_coarse_levels(const fs_handle* h) {
+  return h->mg.size() > 1 ? std::min<size_t>(kLaplCoarse, h->mg_first) : 0;
+}
+
 // The level whose equation the learned coarse solver takes: the first level
 // below the finest with at most 64 x 64 cells.
 size_t learned_level(const fs_handle* h) {
@@ -711,7 +717,8 @@ int capture_iteration(fs_handle* h, hipGraphExec_t* out, int (*launch)(fs_handle
 // MG-PCG set-up: Galerkin hierarchy, coarsest inverse, the start (x = 0, or
 // with `warm` the x in place), the initial V-cycle and iteration 0.
 int mg_prologue(fs_handle* h, bool warm) {
-  for (size_t l = 0; l < h->mg_first; l++) launch_mg_coarsen(mg_args(h, l), h->st);
+  for (size_t l = lapl_coarse_levels(h); l < h->mg_first; l++)
+    launch_mg_coarsen(mg_args(h, l), h->st);
   launch_mg_coarse_setup(cycle_args(h, -1), h->st);
   MgPcgArgs ia = mgpcg_args(h, 1);
   ia.part_out = h->part(0);
@@ -794,6 +801,8 @@ LaplArgs lapl_args(fs_handle* h) {
     a.bdbg = h->d_dbg;
     a.adbg = h->d_dbg + (size_t)h->bx * h->by;
   }
+  a.ncl = (int32_t)lapl_coarse_levels(h);
+  for (int l = 0; l < a.ncl; l++) a.cl[l] = h->mg[l + 1];
   return a;
 }
 
@@ -906,7 +915,8 @@ int lapl_solve_impl(fs_handle* h, int* lerr, fs_solve_stats* stats, int mode = 0
     // attached network the learned level gets a zero correction
     if (h->mg.size() < 2)
       return fail(FS_E_ARG, "learned solver: grid of <= 64 cells has no coarse level%s");
-    for (size_t l = 0; l < learned_level(h); l++) launch_mg_coarsen(mg_args(h, l), h->st);
+    for (size_t l = lapl_coarse_levels(h); l < learned_level(h); l++)
+      launch_mg_coarsen(mg_args(h, l), h->st);
     MgPcgArgs ia = mgpcg_args(h, 1);
     ia.part_out = h->part(0);
     launch_mgpcg_init(ia, h->st);

@@ -90,6 +90,16 @@ struct CtrlArgs {
   const double* replay;  // FS_REC_LEN per record
 };
 
+struct MgLevel {
+  double* d;    // diagonal of the (Galerkin) operator
+  double* cx;   // coupling to the right neighbour
+  double* cy;   // coupling to the upper neighbour
+  double* r;    // residual handed down to this level (level 0: the PCG residual)
+  double* x1;   // pre-smoothed iterate
+  double* e;    // correction returned to the finer level (level 0: u = B r)
+  int32_t nx, ny, P, pad_;
+};
+
 struct LaplArgs {
   const double* n[3];   // level-1 ni, ne, nt
   const double* ph;
@@ -105,6 +115,11 @@ struct LaplArgs {
   Geo G;
   int32_t nbx, nby;
   int32_t gate, pad_;   // gate: skip the solve (done = 3) unless the last attempt was accepted
+  // multigrid levels 1 .. ncl, whose 2^l x 2^l aggregates lie inside one
+  // assembly tile (l <= kLaplCoarse): their Galerkin operators are coarsened by
+  // the assembly kernel from the tile it has just assembled
+  MgLevel cl[3];
+  int32_t ncl, pad2_;
 };
 
 struct PcgArgs {
@@ -130,15 +145,6 @@ struct PcgArgs {
 
 // One level of the aggregation multigrid hierarchy: (nx+2)*(ny+2) planes
 // with zero ghosts (d ghost = 1) so stencils need no bounds checks.
-struct MgLevel {
-  double* d;    // diagonal of the (Galerkin) operator
-  double* cx;   // coupling to the right neighbour
-  double* cy;   // coupling to the upper neighbour
-  double* r;    // residual handed down to this level (level 0: the PCG residual)
-  double* x1;   // pre-smoothed iterate
-  double* e;    // correction returned to the finer level (level 0: u = B r)
-  int32_t nx, ny, P, pad_;
-};
 
 // Level-0 vectors of the multigrid-PCG in the Chronopoulos-Gear form: the
 // finest down kernel applies the CG update before smoothing the new residual,
@@ -233,6 +239,9 @@ constexpr int kNT = kTX * kTY;
 // tiles (64x8, 512 threads) recompute fewer halo cells per cell.
 constexpr int kSTY = 8;
 constexpr int kSNT = kTX * kSTY;
+constexpr int kLaplCoarse = 3;  // log2(kSTY): coarse levels an assembly tile holds whole
+static_assert(kSTY == (1 << kLaplCoarse), "assembly tile height");
+static_assert(sizeof(LaplArgs::cl) == kLaplCoarse * sizeof(MgLevel), "LaplArgs::cl");
 constexpr int kPcgTY = 8;    // PCG tile height: each thread owns kPcgRows cells
 constexpr int kPcgRows = kPcgTY / kTY;
 

@@ -27,6 +27,9 @@ namespace {
 constexpr int kHX = kTX + 2, kHY = kSTY + 2, kNH = kHX * kHY;
 constexpr int kNFX = (kTX + 1) * kSTY, kNFY = kTX * (kSTY + 1);
 constexpr int kNF = kNFX > kNFY ? kNFX : kNFY;
+// the assembly stages its tile's levels 0 .. kLaplCoarse (3 planes each) in
+// its face buffer s_f[4][kNF]
+static_assert(3 * (kSNT + kSNT / 4 + kSNT / 16 + kSNT / 64) <= 4 * kNF, "assembly LDS levels");
 constexpr int kNW = kNT / 64;
 constexpr int kSNW = kSNT / 64;
 
@@ -641,6 +644,55 @@ __global__ void k_reject(DevScalars* sc) {  // main.F90:103-121
   sc->Vd_mi = sc->Vd_or;
 }
 
+__device__ __forceinline__ size_t mg_at(const MgLevel& L, int i, int j) {
+  return (size_t)(j + 1) * L.P + (i + 1);
+}
+
+// Galerkin coarsening (P = piecewise-constant 2x2 aggregation, R = P^T) of
+// aggregate (I, J) of an nx x ny level into an NX x NY level; at(i, j) is the
+// index of fine cell (i, j) in the coefficient arrays.
+struct Coarse3 {
+  double d, cx, cy;
+};
+
+template <class At>
+__device__ __forceinline__ Coarse3 coarsen_agg(const double* fd, const double* fcx,
+                                               const double* fcy, At at, int nx, int ny, int NX,
+                                               int NY, int I, int J) {
+  int i0 = 2 * I, j0 = 2 * J;
+  bool hx = i0 + 1 < nx, hy = j0 + 1 < ny;
+  double d = fd[at(i0, j0)];
+  double internal = 0.0;
+  if (hx) {
+    d += fd[at(i0 + 1, j0)];
+    internal += fcx[at(i0, j0)];
+  }
+  if (hy) {
+    d += fd[at(i0, j0 + 1)];
+    internal += fcy[at(i0, j0)];
+  }
+  if (hx && hy) {
+    d += fd[at(i0 + 1, j0 + 1)];
+    internal += fcx[at(i0, j0 + 1)] + fcy[at(i0 + 1, j0)];
+  }
+  Coarse3 c;
+  c.d = d - 2.0 * internal;
+  // faces leaving the aggregate to the right / upwards (zero at the domain edge)
+  int ir = hx ? i0 + 1 : i0, ju = hy ? j0 + 1 : j0;
+  c.cx = 0.0;
+  c.cy = 0.0;
+  if (I + 1 < NX) {
+    c.cx = fcx[at(ir, j0)];
+    if (hy) c.cx += fcx[at(ir, j0 + 1)];
+  }
+  if (J + 1 < NY) {
+    c.cy = fcy[at(i0, ju)];
+    if (hx) c.cy += fcy[at(i0 + 1, ju)];
+  }
+  return c;
+}
+
+
 // ---------------------------------------------------------------------------
 // Poisson assembly.  Per x/y face: ion and electron SG fluxes (laplEqn) and
 // P = (mu/h)*dFdPh (laplJ) from one shared exponential per species.
@@ -757,6 +809,7 @@ __device__ __forceinline__ void lapl_body(const LaplArgs& A, const FsPhys& K, in
   const int ci = i0 + tx, cj = j0 + ty;
   const bool valid = (ci < bx) && (cj < by);
   const int hc = (ty + 1) * kHX + tx + 1;
+  double o_d = 0.0, o_cx = 0.0, o_cy = 0.0;  // this cell's operator row, as stored
   double dflxi_x = 0, dflxe_x = 0, dfdx = 0, Cx = 0, aW = 0, aE = 0, gR = 0;
   int typx = 0;
   if (valid) {
@@ -897,11 +950,14 @@ __device__ __forceinline__ void lapl_body(const LaplArgs& A, const FsPhys& K, in
     double V = dlx * dly * rw;
     double rhs = V * b;  // -V*A*delta = V*b
     double dgv = -V * C;
+    o_d = dgv;
+    o_cx = typx <= 0 ? gR * dly * rw : 0.0;
+    o_cy = typy <= 0 ? gT * dlx * (G.cyl ? (rp + rj) / 2.0 : 1.0) : 0.0;
     A.rhs0[m] = rhs;
     A.r[m] = rhs;
-    A.dg[m] = dgv;
-    A.cx[m] = typx <= 0 ? gR * dly * rw : 0.0;
-    A.cy[m] = typy <= 0 ? gT * dlx * (G.cyl ? (rp + rj) / 2.0 : 1.0) : 0.0;
+    A.dg[m] = o_d;
+    A.cx[m] = o_cx;
+    A.cy[m] = o_cy;
     if (A.bdbg) {
       size_t n = (size_t)cj * bx + ci;
       A.bdbg[n] = b;
@@ -913,6 +969,48 @@ __device__ __forceinline__ void lapl_body(const LaplArgs& A, const FsPhys& K, in
       a5[4] = C;
     }
     if (isnan(b)) s_bad = 4;
+  }
+  if (A.ncl > 0) {
+    // Multigrid levels 1 .. ncl over this tile's 2^q x 2^q aggregates: the
+    // Galerkin coarsening of k_mg_coarsen (coarsen_agg, same operands and
+    // order), from the rows just assembled, staged level by level in LDS in the
+    // face buffer (free once every cell has read its faces).
+    double* s_lv = &s_f[0][0];
+    __syncthreads();
+    s_lv[threadIdx.x] = o_d;
+    s_lv[kSNT + threadIdx.x] = o_cx;
+    s_lv[2 * kSNT + threadIdx.x] = o_cy;
+    __syncthreads();
+    int off = 0, fnx = bx, fny = by;
+    for (int q = 1; q <= A.ncl; q++) {
+      const int wf = kTX >> (q - 1), nf = wf * (kSTY >> (q - 1));  // the tile on level q - 1
+      const int wc = kTX >> q, nc = wc * (kSTY >> q);              // ... and on level q
+      const int fi0 = i0 >> (q - 1), fj0 = j0 >> (q - 1);
+      const MgLevel& C = A.cl[q - 1];
+      const double* fd = s_lv + off;
+      double* cd = s_lv + off + 3 * nf;
+      const int t = threadIdx.x;
+      if (t < nc) {
+        const int I = (fi0 >> 1) + t % wc, J = (fj0 >> 1) + t / wc;
+        Coarse3 c = {0.0, 0.0, 0.0};
+        if (I < C.nx && J < C.ny) {
+          c = coarsen_agg(fd, fd + nf, fd + 2 * nf,
+                          [&](int i, int j) { return (j - fj0) * wf + (i - fi0); }, fnx, fny,
+                          C.nx, C.ny, I, J);
+          const size_t m = mg_at(C, I, J);
+          C.d[m] = c.d;
+          C.cx[m] = c.cx;
+          C.cy[m] = c.cy;
+        }
+        cd[t] = c.d;
+        cd[nc + t] = c.cx;
+        cd[2 * nc + t] = c.cy;
+      }
+      __syncthreads();
+      off += 3 * nf;
+      fnx = C.nx;
+      fny = C.ny;
+    }
   }
   if (threadIdx.x == 0 && s_bad == 1) atomicMin(&A.sc->status, FS_E_NEGDENS);
   if (threadIdx.x == 0 && s_bad == 3) atomicMin(&A.sc->status, FS_E_NANJ);
@@ -1170,9 +1268,6 @@ __global__ void __launch_bounds__(kNT) k_pcg_tres(PcgArgs A, int nb) { pcg_tres_
 // levels run inside one workgroup with their iterates in LDS, and the
 // coarsest (<= 64 cells) is solved with a dense inverse.
 
-__device__ __forceinline__ size_t mg_at(const MgLevel& L, int i, int j) {
-  return (size_t)(j + 1) * L.P + (i + 1);
-}
 
 // A kernel argument is loaded where it is first used; a kernel that first
 // tests a flag in device memory would then load its arguments one dependent
@@ -1229,44 +1324,11 @@ __device__ __forceinline__ void hold_mg_args(const MgArgs& A, bool cg) {
 
 // The Galerkin coefficients of coarse cell (I, J) from fine planes d, cx, cy
 // (padded, row pitch P, nx x ny cells); coarse size NX x NY.
-struct Coarse3 {
-  double d, cx, cy;
-};
 __device__ __forceinline__ Coarse3 coarsen_cell(const double* fd, const double* fcx,
                                                 const double* fcy, int P, int nx, int ny, int NX,
                                                 int NY, int I, int J) {
-  auto at = [&](int i, int j) { return (size_t)(j + 1) * P + (i + 1); };
-  int i0 = 2 * I, j0 = 2 * J;
-  bool hx = i0 + 1 < nx, hy = j0 + 1 < ny;
-  double d = fd[at(i0, j0)];
-  double internal = 0.0;
-  if (hx) {
-    d += fd[at(i0 + 1, j0)];
-    internal += fcx[at(i0, j0)];
-  }
-  if (hy) {
-    d += fd[at(i0, j0 + 1)];
-    internal += fcy[at(i0, j0)];
-  }
-  if (hx && hy) {
-    d += fd[at(i0 + 1, j0 + 1)];
-    internal += fcx[at(i0, j0 + 1)] + fcy[at(i0 + 1, j0)];
-  }
-  Coarse3 c;
-  c.d = d - 2.0 * internal;
-  // faces leaving the aggregate to the right / upwards (zero at the domain edge)
-  int ir = hx ? i0 + 1 : i0, ju = hy ? j0 + 1 : j0;
-  c.cx = 0.0;
-  c.cy = 0.0;
-  if (I + 1 < NX) {
-    c.cx = fcx[at(ir, j0)];
-    if (hy) c.cx += fcx[at(ir, j0 + 1)];
-  }
-  if (J + 1 < NY) {
-    c.cy = fcy[at(i0, ju)];
-    if (hx) c.cy += fcy[at(i0 + 1, ju)];
-  }
-  return c;
+  return coarsen_agg(fd, fcx, fcy, [&](int i, int j) { return (size_t)(j + 1) * P + (i + 1); },
+                     nx, ny, NX, NY, I, J);
 }
 
 template <int kInstance>
