@@ -1063,8 +1063,9 @@ CtrlArgs ctrl_args(fs_handle* h) {
 // ahead: after each attempt's controller kernel, issue the Poisson solve that
 // follows an accepted attempt (lapl_solve_impl mode 1) before reading the
 // controller's decision.  spec_next (another pass follows in the same run):
-// behind the solve's first chunk, also issue the next pass's accept, begin and
-// first stage, gated on the device by k_accept_spec; the host reads the same
+// behind the solve's first chunk, also issue the next pass's accept and begin
+// (k_accept_spec, which applies them only if the pass is accepted) and its
+// first stage, gated on k_accept_spec's decision; the host reads the same
 // scalars and knows whether they ran.  The device does not then wait for the
 // host between passes that converge in their first chunk.
 int ptcl_step_impl(fs_handle* h, fs_step_info* info, bool ahead = false,
@@ -1095,7 +1096,6 @@ int ptcl_step_impl(fs_handle* h, fs_step_info* info, bool ahead = false,
     if (spec_next) {
       enqueue_publish(h);
       launch_accept_spec(h->d_sc, h->st);
-      launch_begin_step(h->d_sc, h->st, 1);
       launch_stage(stab_at(h, (h->rot + 1) % 3, true, 1, 0), h->d_phys, h->nbx * h->nby, h->st,
                    1);
       HIP_OK(hipGetLastError());

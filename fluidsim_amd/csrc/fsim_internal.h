@@ -34,7 +34,7 @@ struct DevScalars {
   int32_t itc[2], pad_k[2];   // MG-PCG iteration count by parity
   uint32_t ticket[8];
   uint32_t pub_seq;  // host-memory copy only: the publication count (k_publish)
-  int32_t spec;      // k_accept_spec accepted the last pass: its gated followers run
+  int32_t spec;      // k_accept_spec accepted the last pass: the gated stage kernel runs
 };
 
 // Grid metrics on the device, 0-based over interior cells ci in [0,bx), cj in
@@ -295,7 +295,7 @@ void launch_circ(const CircArgs& a, const FsPhys& K, hipStream_t s);
 void launch_ctrl(const CtrlArgs& a, hipStream_t s);
 void launch_accept(DevScalars* sc, hipStream_t s);
 void launch_reject(DevScalars* sc, hipStream_t s);
-void launch_begin_step(DevScalars* sc, hipStream_t s, int gated = 0);
+void launch_begin_step(DevScalars* sc, hipStream_t s);
 void launch_accept_spec(DevScalars* sc, hipStream_t s);
 void launch_publish(const DevScalars* sc, DevScalars* pub, uint32_t* cnt, hipStream_t s);
 void launch_lapl_assemble(const LaplArgs* a, const FsPhys* K, int nblocks, hipStream_t s);

@@ -452,7 +452,8 @@ __device__ __forceinline__ void stage_body(const StageArgs& A, const FsPhys& K, 
 // Arguments from device memory: kept there per handle, they are read with
 // scalar loads where used (passed by value, the compiler keeps the whole
 // argument block live in SGPRs and spills it to VGPR lanes).
-// gated: as k_begin_step.
+// gated: issued before the host has read the last pass's outcome; runs only
+// if k_accept_spec accepted that pass.
 __global__ void __launch_bounds__(kSNT, 4) k_stage(const StageArgs* __restrict__ A,
                                                const FsPhys* __restrict__ K, int gated) {
   if (gated && A->sc->spec == 0) return;
@@ -589,13 +590,12 @@ __device__ __forceinline__ void ctrl_body(const CtrlArgs& A) {
 
 __global__ void __launch_bounds__(kNT) k_ctrl(CtrlArgs A) { ctrl_body<0>(A); }
 
-// gated: issued before the host has read the last pass's outcome; runs only
-// if k_accept_spec accepted that pass.
-__global__ void k_begin_step(DevScalars* sc, int gated) {
-  if (gated && sc->spec == 0) return;
+__device__ __forceinline__ void begin_step_body(DevScalars* sc) {
   sc->t_m = (double)1e9f;  // main.F90:74
   sc->attempts = 0;
 }
+
+__global__ void k_begin_step(DevScalars* sc) { begin_step_body(sc); }
 
 // Copies the scalars to host-mapped memory and then, after a system-scope
 // fence, the count of publications so far (kept on the device in *cnt, so that
@@ -628,14 +628,18 @@ __device__ __forceinline__ void accept_body(DevScalars* sc) {  // main.F90:83-10
 
 __global__ void k_accept(DevScalars* sc) { accept_body(sc); }
 
-// The accept of a pass, issued before the host has read the pass's outcome:
-// applied when the host accepts the pass on the scalars this kernel reads (no
-// error, the attempt accepted, its solve converged in the chunk just run), and
-// recorded in spec for the gated kernels of the next pass that follow it.
+// The accept of a pass and the begin of the next, issued before the host has
+// read the pass's outcome: applied when the host accepts the pass on the
+// scalars this kernel reads (no error, the attempt accepted, its solve
+// converged in the chunk just run), and recorded in spec for the gated stage
+// kernel that follows.
 __global__ void k_accept_spec(DevScalars* sc) {
   const int go = sc->status >= 0 && sc->accept == 1 && sc->done == 1;
   sc->spec = go;
-  if (go) accept_body(sc);
+  if (go) {
+    accept_body(sc);
+    begin_step_body(sc);
+  }
 }
 
 __global__ void k_reject(DevScalars* sc) {  // main.F90:103-121
@@ -2714,8 +2718,8 @@ void launch_accept(DevScalars* sc, hipStream_t s) {
 void launch_reject(DevScalars* sc, hipStream_t s) {
   hipLaunchKernelGGL(k_reject, dim3(1), dim3(1), 0, s, sc);
 }
-void launch_begin_step(DevScalars* sc, hipStream_t s, int gated) {
-  hipLaunchKernelGGL(k_begin_step, dim3(1), dim3(1), 0, s, sc, gated);
+void launch_begin_step(DevScalars* sc, hipStream_t s) {
+  hipLaunchKernelGGL(k_begin_step, dim3(1), dim3(1), 0, s, sc);
 }
 void launch_accept_spec(DevScalars* sc, hipStream_t s) {
   hipLaunchKernelGGL(k_accept_spec, dim3(1), dim3(1), 0, s, sc);
