@@ -154,7 +154,7 @@ struct fs_handle {
   double* d_geo = nullptr;  // dx,dlx,dy,dly,r and 1/dx,1/dlx,1/dy,1/dly
   int8_t* d_types = nullptr;  // ltype, rtype, gl, gr (by each)
   double* d_dr = nullptr;     // dra, drc
-  double* d_cur = nullptr;
+  double* d_cur = nullptr;  // electrode face currents, 2*by per stage parity
   double* d_part = nullptr;
   DevScalars* d_sc = nullptr;
   DevScalars* h_sc = nullptr;  // pinned mirror
@@ -311,7 +311,16 @@ void time_mark(fs_handle* h, int e) {
   if (h->timing) (void)hipEventRecord(h->ev[e], h->st);
 }
 
-StageArgs stage_args(fs_handle* h, int stage, int rhs_only) {
+// The electrode face currents of a stage (Ida terms, then Idc terms), by stage
+// parity: a stage launch that also does the previous stage's circuit update
+// reads that stage's currents while writing its own.
+double* cur_buf(fs_handle* h, int stage) { return h->d_cur + (size_t)(stage & 1) * 2 * h->by; }
+
+CircArgs circ_args(fs_handle* h, int stage);
+
+// fuse_circ: the launch of stages 2-5 also does the previous stage's circuit
+// update (StageArgs::cc)
+StageArgs stage_args(fs_handle* h, int stage, int rhs_only, bool fuse_circ = false) {
   StageArgs a;
   memset(&a, 0, sizeof(a));
   // stage 1 reads A writes B; 2 reads B writes A; 3 A->B; 4 B->A; 5 A->L1
@@ -328,7 +337,7 @@ StageArgs stage_args(fs_handle* h, int stage, int rhs_only) {
     a.kdbg[s] = h->d_kdbg ? h->d_kdbg + ((size_t)(stage - 1) * 4 + s) * h->PL : nullptr;
   }
   a.ph = h->d_ph;
-  a.cur = h->d_cur;
+  a.cur = cur_buf(h, stage);
   a.part = h->part(0);
   a.sc = h->d_sc;
   a.G = h->G;
@@ -336,6 +345,7 @@ StageArgs stage_args(fs_handle* h, int stage, int rhs_only) {
   a.rhs_only = rhs_only;
   a.nbx = h->nbx;
   a.nby = h->nby;
+  if (fuse_circ && stage >= 2) a.cc = circ_args(h, stage - 1);
   return a;
 }
 
@@ -343,7 +353,7 @@ CircArgs circ_args(fs_handle* h, int stage) {
   CircArgs c;
   memset(&c, 0, sizeof(c));
   c.ph = h->d_ph;
-  c.cur = h->d_cur;
+  c.cur = cur_buf(h, stage);
   c.dra = h->d_dr;
   c.drc = h->d_dr + h->by;
   c.gl = h->d_types + 2 * h->by;
@@ -819,7 +829,7 @@ int build_arg_tables(fs_handle* h) {
       h->alias = al != 0;
       for (int v = 0; v < 3; v++)
         for (int stg = 1; stg <= 5; stg++) {
-          StageArgs a = stage_args(h, stg, v == 0 ? 0 : 1);
+          StageArgs a = stage_args(h, stg, v == 0 ? 0 : 1, v == 0);
           if (v == 2)
             for (int s = 0; s < 4; s++) a.kdbg[s] = nullptr;
           st[((rot * 2 + al) * 3 + v) * 5 + stg - 1] = a;
@@ -1077,14 +1087,11 @@ int ptcl_step_impl(fs_handle* h, fs_step_info* info, bool ahead = false,
   int attempts = 0;
   for (;;) {
     time_mark(h, 0);
-    for (int stage = 1; stage <= 5; stage++) {
+    for (int stage = 1; stage <= 5; stage++)  // (stages 2-5 also do circuit updates 1-4)
       if (!(pre && attempts == 0 && stage == 1))
         launch_stage(stab(h, stage, 0), h->d_phys, h->nbx * h->nby, h->st);
-      CircArgs c = circ_args(h, stage);
-      launch_circ(c, h->K, h->st);
-    }
     time_mark(h, 1);
-    launch_ctrl(ctrl_args(h), h->st);
+    launch_circ_ctrl(circ_args(h, 5), h->K, ctrl_args(h), h->st);
     HIP_OK(hipGetLastError());
     time_mark(h, 2);
     if (ahead) {
@@ -1621,7 +1628,7 @@ int fs_create(const fs_grid_desc* g, const fs_params* p, fs_handle** out) {
       bad(hipMalloc(&h->d_geo, (5 * (size_t)bx + 5 * (size_t)by + 10) * sizeof(double))) ||
       bad(hipMalloc(&h->d_types, 4 * (size_t)by)) ||
       bad(hipMalloc(&h->d_dr, 2 * (size_t)by * sizeof(double))) ||
-      bad(hipMalloc(&h->d_cur, 2 * (size_t)by * sizeof(double))) ||
+      bad(hipMalloc(&h->d_cur, 4 * (size_t)by * sizeof(double))) ||
       bad(hipMalloc(&h->d_part, 3 * ((size_t)h->nblk + 64) * 8 * sizeof(double))) ||
       bad(hipMalloc(&h->d_sc, sizeof(DevScalars))) ||
       bad(hipHostMalloc(&h->h_sc, sizeof(DevScalars), hipHostMallocDefault)) ||
@@ -1676,7 +1683,7 @@ int fs_create(const fs_grid_desc* g, const fs_params* p, fs_handle** out) {
   // recomputation at the boundary yields p = 0
   (void)hipMemsetAsync(h->d_k, 0, 12 * PL * sizeof(double), h->st);
   (void)hipMemsetAsync(h->d_ph, 0, PL * sizeof(double), h->st);
-  (void)hipMemsetAsync(h->d_cur, 0, 2 * (size_t)by * sizeof(double), h->st);
+  (void)hipMemsetAsync(h->d_cur, 0, 4 * (size_t)by * sizeof(double), h->st);
   (void)hipMemsetAsync(h->d_pois, 0, kPois * PL * sizeof(double), h->st);
   launch_fill(h->pois(PO_DG), 1.0, PL, h->st);
   if (build_mg(h)) {

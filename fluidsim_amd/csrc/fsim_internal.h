@@ -55,6 +55,18 @@ struct Geo {
 };
 
 // species order everywhere: 0 ni, 1 ne, 2 nt, 3 nm
+struct CircArgs {
+  double* ph;
+  const double* cur;
+  const double* dra;    // by: current weights at the left electrode face
+  const double* drc;    // by: current weights at the right electrode face
+  const int8_t* gl;     // by: ghost potential set on the left
+  const int8_t* gr;     // by: ghost potential set on the right
+  DevScalars* sc;
+  double vset, res, t_rise, t_width, t_fall;
+  int32_t rf, circuit, stage, by, P, bx;
+};
+
 struct StageArgs {
   const double* rd[4];  // level-2 densities read by this stage
   double* wr[4];        // stage 1-4: next predictor, stage 5: level 1
@@ -69,19 +81,12 @@ struct StageArgs {
   DevScalars* sc;
   Geo G;
   int32_t stage, rhs_only, nbx, nby;
+  // cc.stage > 0: the circuit update of that (the previous) stage is done by
+  // this launch before its fluxes, as k_circ would (the circuit of stage 5
+  // goes with the controller, k_circ_ctrl)
+  CircArgs cc;
 };
 
-struct CircArgs {
-  double* ph;
-  const double* cur;
-  const double* dra;    // by: current weights at the left electrode face
-  const double* drc;    // by: current weights at the right electrode face
-  const int8_t* gl;     // by: ghost potential set on the left
-  const int8_t* gr;     // by: ghost potential set on the right
-  DevScalars* sc;
-  double vset, res, t_rise, t_width, t_fall;
-  int32_t rf, circuit, stage, by, P, bx;
-};
 
 struct CtrlArgs {
   const double* part;
@@ -293,6 +298,7 @@ void launch_stage(const StageArgs* a, const FsPhys* K, int nblocks, hipStream_t 
                   int gated = 0);
 void launch_circ(const CircArgs& a, const FsPhys& K, hipStream_t s);
 void launch_ctrl(const CtrlArgs& a, hipStream_t s);
+void launch_circ_ctrl(const CircArgs& c, const FsPhys& K, const CtrlArgs& a, hipStream_t s);
 void launch_accept(DevScalars* sc, hipStream_t s);
 void launch_reject(DevScalars* sc, hipStream_t s);
 void launch_begin_step(DevScalars* sc, hipStream_t s);

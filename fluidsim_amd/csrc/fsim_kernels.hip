@@ -143,6 +143,51 @@ __device__ bool last_block_sum(double (&v)[NV], double* part, uint32_t* ticket, 
   return true;
 }
 
+__device__ double vsource(const CircArgs& C, const FsPhys& K, double t) {
+  const double Vmax = C.vset, pi = K.pi;
+  switch (C.rf) {
+    case 1: return Vmax * sin(pi * t);
+    case 2: {
+      if (t < 2.5e-1) {
+        double s = sin(4.0 * pi * t);
+        return Vmax * (s * (s * s));
+      }
+      return 0.0;
+    }
+    case 3: {
+      if (t < 2.3e-1) {
+        double s = sin(4.0 * pi * t);
+        return Vmax * (s * s);
+      }
+      if (t < 3.35e-1) return Vmax * (sin(3.81 * pi * (t + 5.85e-2)) * 7.19e-1 + 2.81e-1);
+      if (t < 8.2e-1) {
+        double s = sin(3.81 * pi * (t + 1.2e-1) / 3.5);
+        return Vmax * ((s * (s * s)) * (-4.38e-1));
+      }
+      return 0.0;
+    }
+    case 4: {
+      double tr = C.t_rise, tw = C.t_width, tf = C.t_fall;
+      if (t < tr) return Vmax * (t / tr);
+      if (t < tr + tw) return Vmax;
+      double te = tr + tw + tf;
+      if (t < te) return Vmax * ((te - t) / tf);
+      return 0.0;
+    }
+    default: return Vmax;
+  }
+}
+
+// The electrode voltage after the circuit update of `stage` (Vd_mi for stages
+// 1-4, Vd_pl for 5), circ_lib.F90:158-174, from kv[1..stage].
+__device__ __forceinline__ double circ_vd(int stage, double Vd_or, double dt, const double* kv) {
+  if (stage == 1) return Vd_or + dt * kv[1] / 3.0;
+  if (stage == 2) return Vd_or + dt * (kv[1] + kv[2]) / 6.0;
+  if (stage == 3) return Vd_or + dt * (kv[1] + kv[3] * 3.0) / 8.0;
+  if (stage == 4) return Vd_or + dt * (kv[1] - kv[3] * 3.0 + kv[4] * 4.0) / 2.0;
+  return Vd_or + dt * (kv[1] + kv[4] * 4.0 + kv[5]) / 6.0;
+}
+
 // ---------------------------------------------------------------------------
 // Electrode boundary fluxes, ptcl_lib.F90:187-213 (left) and :247-272 (right)
 struct Flux4 {
@@ -210,6 +255,18 @@ __device__ __forceinline__ void stage_body(const StageArgs& A, const FsPhys& K, 
   const int i0 = (t % A.nbx) * kTX, j0 = (t / A.nbx) * kSTY;
   const double dt = A.sc->dt;
   if (threadIdx.x == 0) s_bad = 0;
+  // The previous stage's circuit update (cc.stage > 0), k_circ's arithmetic in
+  // every block: the electrode currents are summed as k_circ's 256 threads sum
+  // them (the other threads add zeros), the update is made after phase 1, and
+  // the ghost potential of the electrode rows in this tile's halo is set in
+  // LDS from it (k_circ_ctrl writes the final one to memory after stage 5).
+  const int cstage = kInstance == 0 ? A.cc.stage : 0;
+  double cv[2] = {0.0, 0.0};
+  if (cstage > 0 && threadIdx.x < kNT)
+    for (int j = threadIdx.x; j < A.cc.by; j += kNT) {
+      cv[0] += gld(A.cc.dra, j) * gld(A.cc.cur, j);
+      cv[1] += gld(A.cc.drc, j) * gld(A.cc.cur, A.cc.by + j);
+    }
   __syncthreads();
 
   // ---- phase 1: stage densities + potential of tile and halo; cell properties
@@ -262,6 +319,44 @@ __device__ __forceinline__ void stage_body(const StageArgs& A, const FsPhys& K, 
       if (stage >= 2) pk1[s] = gld((const double*)A.k1[s], mc);
       if (stage >= 4) pk3[s] = gld((const double*)A.k3[s], mc);
       if (stage == 5) pk4[s] = gld((const double*)A.k4[s], mc);
+    }
+  }
+  if (cstage > 0) {
+    __shared__ double s_vd;
+    block_sum<2, kSNW>(cv, s_red);
+    if (threadIdx.x == 0) {
+      const CircArgs& C = A.cc;
+      DevScalars* sc = A.sc;
+      const int k = cstage;
+      const double Vsrc = vsource(C, K, sc->t);
+      const bool pin = C.rf != 1 && C.circuit == 0;  // circ_lib.F90:176-177
+      double kv[6];
+      for (int q = 1; q < k; q++) kv[q] = sc->kv[q];
+      // Vd_mi as the previous update left it (stored only from update 2 on:
+      // the other blocks still read the stored value during update 1)
+      const double vprev = k == 1 ? sc->Vd_mi : (pin ? Vsrc : circ_vd(k - 1, sc->Vd_or, dt, kv));
+      kv[k] = -(cv[1] - (Vsrc - vprev) / C.res) / K.Cap;
+      const double vd = pin ? Vsrc : circ_vd(k, sc->Vd_or, dt, kv);
+      s_vd = vd;
+      if (bid == 0) {
+        sc->kv[k] = kv[k];
+        if (k > 1) sc->Vd_mi = vd;
+        if (pin) sc->Vd_pl = Vsrc;
+        sc->Vsrc = Vsrc;
+        sc->Ida = cv[0];
+        sc->Idc = cv[1];
+      }
+    }
+    __syncthreads();
+    const double vd = s_vd;
+    for (int q = threadIdx.x; q < 2 * kHY; q += kSNT) {  // circ_lib.F90:180-198
+      const int hy = q % kHY, cj = j0 - 1 + hy;
+      if (cj < 0 || cj >= by) continue;
+      if (q < kHY) {
+        if (i0 == 0 && gld(A.cc.gl, cj)) s_ph[hy * kHX] = vd / 2.0;
+      } else if (i0 + kTX >= bx && gld(A.cc.gr, cj)) {
+        s_ph[hy * kHX + bx - i0 + 1] = vd / (-2.0);
+      }
     }
   }
   __syncthreads();
@@ -461,40 +556,7 @@ __global__ void __launch_bounds__(kSNT, 4) k_stage(const StageArgs* __restrict__
 }
 
 // ---------------------------------------------------------------------------
-__device__ double vsource(const CircArgs& C, const FsPhys& K, double t) {
-  const double Vmax = C.vset, pi = K.pi;
-  switch (C.rf) {
-    case 1: return Vmax * sin(pi * t);
-    case 2: {
-      if (t < 2.5e-1) {
-        double s = sin(4.0 * pi * t);
-        return Vmax * (s * (s * s));
-      }
-      return 0.0;
-    }
-    case 3: {
-      if (t < 2.3e-1) {
-        double s = sin(4.0 * pi * t);
-        return Vmax * (s * s);
-      }
-      if (t < 3.35e-1) return Vmax * (sin(3.81 * pi * (t + 5.85e-2)) * 7.19e-1 + 2.81e-1);
-      if (t < 8.2e-1) {
-        double s = sin(3.81 * pi * (t + 1.2e-1) / 3.5);
-        return Vmax * ((s * (s * s)) * (-4.38e-1));
-      }
-      return 0.0;
-    }
-    case 4: {
-      double tr = C.t_rise, tw = C.t_width, tf = C.t_fall;
-      if (t < tr) return Vmax * (t / tr);
-      if (t < tr + tw) return Vmax;
-      double te = tr + tw + tf;
-      if (t < te) return Vmax * ((te - t) / tf);
-      return 0.0;
-    }
-    default: return Vmax;
-  }
-}
+
 
 template <int kInstance>
 __device__ __forceinline__ void circ_body(const CircArgs& C, const FsPhys& K) {
@@ -513,12 +575,9 @@ __device__ __forceinline__ void circ_body(const CircArgs& C, const FsPhys& K) {
     double Ida = v[0], Idc = v[1];
     double* kv = sc->kv;
     kv[stage] = -(Idc - (Vsrc - sc->Vd_mi) / C.res) / K.Cap;
-    double dt = sc->dt;
-    if (stage == 1) sc->Vd_mi = sc->Vd_or + dt * kv[1] / 3.0;
-    else if (stage == 2) sc->Vd_mi = sc->Vd_or + dt * (kv[1] + kv[2]) / 6.0;
-    else if (stage == 3) sc->Vd_mi = sc->Vd_or + dt * (kv[1] + kv[3] * 3.0) / 8.0;
-    else if (stage == 4) sc->Vd_mi = sc->Vd_or + dt * (kv[1] - kv[3] * 3.0 + kv[4] * 4.0) / 2.0;
-    else sc->Vd_pl = sc->Vd_or + dt * (kv[1] + kv[4] * 4.0 + kv[5]) / 6.0;
+    const double vd = circ_vd(stage, sc->Vd_or, sc->dt, kv);
+    if (stage <= 4) sc->Vd_mi = vd;
+    else sc->Vd_pl = vd;
     if (C.rf != 1 && C.circuit == 0) {  // circ_lib.F90:176-177
       sc->Vd_pl = Vsrc;
       sc->Vd_mi = Vsrc;
@@ -589,6 +648,13 @@ __device__ __forceinline__ void ctrl_body(const CtrlArgs& A) {
 }
 
 __global__ void __launch_bounds__(kNT) k_ctrl(CtrlArgs A) { ctrl_body<0>(A); }
+
+// The circuit update of stage 5 and the controller in one launch.
+__global__ void __launch_bounds__(kNT) k_circ_ctrl(CircArgs C, FsPhys K, CtrlArgs A) {
+  circ_body<0>(C, K);
+  __syncthreads();
+  ctrl_body<0>(A);
+}
 
 __device__ __forceinline__ void begin_step_body(DevScalars* sc) {
   sc->t_m = (double)1e9f;  // main.F90:74
@@ -2711,6 +2777,9 @@ void launch_circ(const CircArgs& a, const FsPhys& K, hipStream_t s) {
 }
 void launch_ctrl(const CtrlArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(k_ctrl, dim3(1), dim3(kNT), 0, s, a);
+}
+void launch_circ_ctrl(const CircArgs& c, const FsPhys& K, const CtrlArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(k_circ_ctrl, dim3(1), dim3(kNT), 0, s, c, K, a);
 }
 void launch_accept(DevScalars* sc, hipStream_t s) {
   hipLaunchKernelGGL(k_accept, dim3(1), dim3(1), 0, s, sc);
