@@ -256,11 +256,14 @@ __device__ __forceinline__ void stage_body(const StageArgs& A, const FsPhys& K, 
   const double dt = A.sc->dt;
   if (threadIdx.x == 0) s_bad = 0;
   // The previous stage's circuit update (cc.stage > 0), k_circ's arithmetic in
-  // every block: the electrode currents are summed as k_circ's 256 threads sum
+  // each block that needs it: the electrode currents are summed as k_circ's 256 threads sum
   // them (the other threads add zeros), the update is made after phase 1, and
   // the ghost potential of the electrode rows in this tile's halo is set in
   // LDS from it (k_circ_ctrl writes the final one to memory after stage 5).
-  const int cstage = kInstance == 0 ? A.cc.stage : 0;
+  // (only the blocks that use it: those with an electrode ghost column in the
+  // halo, and block 0, which records the scalars)
+  const int cstage =
+      kInstance == 0 && (bid == 0 || i0 == 0 || i0 + kTX >= bx) ? A.cc.stage : 0;
   double cv[2] = {0.0, 0.0};
   if (cstage > 0 && threadIdx.x < kNT)
     for (int j = threadIdx.x; j < A.cc.by; j += kNT) {
