@@ -297,7 +297,6 @@ void launch_ens_pcg_finish(const PcgArgs* t, const EnsMap& m, const EnsMap& one,
 void launch_stage(const StageArgs* a, const FsPhys* K, int nblocks, hipStream_t s,
                   int gated = 0);
 void launch_circ(const CircArgs& a, const FsPhys& K, hipStream_t s);
-void launch_ctrl(const CtrlArgs& a, hipStream_t s);
 void launch_circ_ctrl(const CircArgs& c, const FsPhys& K, const CtrlArgs& a, hipStream_t s);
 void launch_accept(DevScalars* sc, hipStream_t s);
 void launch_reject(DevScalars* sc, hipStream_t s);

@@ -1,21 +1,23 @@
 // fsim_kernels.hip -- gfx950 kernels of the plasma-transport hot path.
 //
 //   k_stage      one RK-Merson stage: ptclEqn (ptcl_lib.F90:99-550) fused with
-//                rk_step (ptcl_props.F90:71-138) and the embedded error sum.
-//                64x8 tiles; densities, potential and per-cell transport
-//                properties of the tile plus a one-cell halo are staged in
-//                LDS, each face flux is computed once and shared by the two
-//                cells it separates.
-//   k_circ       circ_step (circ_lib.F90:19-210): electrode currents, RC
-//                circuit stage, ghost potential.
-//   k_ctrl       the dt controller of ptcl_step (ptcl_lib.F90:59-94).
+//                rk_step (ptcl_props.F90:71-138) and the embedded error sum;
+//                stages 2-5 first make the previous stage's circuit update
+//                (circ_step, circ_lib.F90:19-210).  64x8 tiles; densities,
+//                potential and per-cell transport properties of the tile plus
+//                a one-cell halo are staged in LDS, each face flux is computed
+//                once and shared by the two cells it separates.
+//   k_circ_ctrl  the circuit update of stage 5 and the dt controller of
+//                ptcl_step (ptcl_lib.F90:59-94); k_circ alone serves the
+//                ensemble and fs_circ_step.
 //   k_lapl       lapl_solve assembly (laplEqn + laplJ, lapl_lib.F90:123-609)
-//                into the volume-symmetrised operator -V*A (3 planes).
-//   k_pcg_a/b    matrix-free Jacobi-PCG iteration (replaces KSPSolve,
-//                lapl_lib.F90:103) with deterministic last-block reductions.
+//                into the volume-symmetrised operator -V*A (3 planes), and the
+//                multigrid levels whose aggregates lie inside a tile.
+//   k_mg_*       the multigrid-preconditioned CG that replaces KSPSolve
+//                (lapl_lib.F90:103); k_pcg_* the Jacobi-preconditioned one.
 //
 // Reductions are deterministic: per-block partials in a fixed tree order,
-// then the last block (atomic ticket) sums the partials in a fixed order.
+// which every block of the next kernel sums in the same fixed order.
 #include <hip/hip_runtime.h>
 #include <string.h>
 
@@ -108,41 +110,7 @@ __device__ __forceinline__ double block_min(double v, double* s_red) {
   return v;
 }
 
-// Writes this block's NV partials, then lets the last block to arrive sum all
-// blocks' partials in block order.  Returns true in the last block, where
-// `tot` holds the totals (valid in thread 0).
-template <int NV>
-__device__ bool last_block_sum(double (&v)[NV], double* part, uint32_t* ticket, double (&tot)[NV],
-                               double* s_red) {
-  __shared__ int s_last;
-  block_sum<NV>(v, s_red);
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int k = 0; k < NV; k++)
-      __hip_atomic_store(&part[blockIdx.x * NV + k], v[k], __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_AGENT);
-    __threadfence();
-    uint32_t t = atomicAdd(ticket, 1u);
-    s_last = (t == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (!s_last) return false;
-  __threadfence();
-  double acc[NV];
-#pragma unroll
-  for (int k = 0; k < NV; k++) acc[k] = 0.0;
-  // fixed order: thread t owns blocks t, t+NT, ... (sequential), then a tree
-  for (int b = threadIdx.x; b < (int)gridDim.x; b += kNT)
-#pragma unroll
-    for (int k = 0; k < NV; k++)
-      acc[k] += __hip_atomic_load(&part[b * NV + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  block_sum<NV>(acc, s_red);
-#pragma unroll
-  for (int k = 0; k < NV; k++) tot[k] = acc[k];
-  if (threadIdx.x == 0) *ticket = 0u;
-  return true;
-}
-
+// The source voltage at time t for the waveform C.rf (circ_lib.F90:26-50).
 __device__ double vsource(const CircArgs& C, const FsPhys& K, double t) {
   const double Vmax = C.vset, pi = K.pi;
   switch (C.rf) {
@@ -649,8 +617,6 @@ __device__ __forceinline__ void ctrl_body(const CtrlArgs& A) {
     sc->Vd_mi = sc->Vd_or;
   }
 }
-
-__global__ void __launch_bounds__(kNT) k_ctrl(CtrlArgs A) { ctrl_body<0>(A); }
 
 // The circuit update of stage 5 and the controller in one launch.
 __global__ void __launch_bounds__(kNT) k_circ_ctrl(CircArgs C, FsPhys K, CtrlArgs A) {
@@ -2777,9 +2743,6 @@ void launch_stage(const StageArgs* a, const FsPhys* K, int nblocks, hipStream_t 
 }
 void launch_circ(const CircArgs& a, const FsPhys& K, hipStream_t s) {
   hipLaunchKernelGGL(k_circ, dim3(1), dim3(kNT), 0, s, a, K);
-}
-void launch_ctrl(const CtrlArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL(k_ctrl, dim3(1), dim3(kNT), 0, s, a);
 }
 void launch_circ_ctrl(const CircArgs& c, const FsPhys& K, const CtrlArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(k_circ_ctrl, dim3(1), dim3(kNT), 0, s, c, K, a);
