@@ -212,13 +212,13 @@ struct fs_handle {
   double* part(int which) const { return d_part + (size_t)which * 8 * ((size_t)nblk + 64); }
 };
 
-constexpr int kPois = 18;
+constexpr int kPois = 19;
 enum { PO_RHS0 = 0, PO_R0 = 1, PO_CX = 2, PO_CY = 3, PO_DG = 4, PO_X = 5, PO_P = 6, PO_R1 = 7,
        PO_W0 = 8, PO_W1 = 9, PO_S0 = 10, PO_S1 = 11,
        // multigrid-preconditioned PCG: p (double-buffered), q, u = B r, pre-smoothed x1
        PO_MP0 = 12, PO_MP1 = 13, PO_MQ = 14, PO_MU = 15, PO_MX1 = 16,
        // the MG-PCG correction before the last (second-order warm start)
-       PO_X2 = 17 };
+       PO_X2 = 17, PO_X3 = 18 };
 constexpr double kMgOver = 1.8;    // scale of the piecewise-constant coarse correction
 
 namespace {
@@ -496,6 +496,7 @@ MgPcgArgs mgpcg_args(fs_handle* h, int it) {
   a.q = h->pois(PO_MQ);
   a.x = h->pois(PO_X);
   a.x2 = h->pois(PO_X2);
+  a.x3 = h->pois(PO_X3);
   a.r = h->pois(PO_R0);
   a.sc = h->d_sc;
   a.G = h->G;
@@ -588,6 +589,7 @@ MgArgs mg_down_args(fs_handle* h, size_t l, int it) {
     a.nb_in = capped_grid(h->nbx * h->pnby);
     a.xs = h->pois(PO_X);
     a.xs2 = h->pois(PO_X2);
+    a.xs3 = h->pois(PO_X3);
     a.rhs0 = h->pois(PO_RHS0);
   } else if (l == 0) {
     a.it = it;

@@ -25,8 +25,12 @@ struct DevScalars {
   // solve starts from warm_a x + warm_b x2: the corrections per unit step
   // length extrapolated linearly to the new step (or, without x2, x scaled by
   // the step-length ratio).  warm_now: the solve in flight starts warm.
+  // While the step length still changes, x3 (when x3_ok, after dt_warm3)
+  // makes the extrapolation quadratic: + warm_c x3.
   double dt_warm, dt_warm2, warm_a, warm_b;
+  double dt_warm3, warm_c;
   int32_t warm_now, x2_ok;
+  int32_t x3_ok, pad_x;
   int32_t itmax, pad_w;  // iteration limit of the MG-PCG solve in flight (set by its init kernel)
   int32_t iters, done, maxit, pad_i;
   int32_t status, accept, attempts, replay_idx;
@@ -182,6 +186,7 @@ struct MgArgs {
   MgCg cg;
   double* xs;            // level-0 initial test: the iterate x and the previous
   double* xs2;           // correction x2, advanced here (warm_a, warm_b)
+  double* xs3;           // correction x3, advanced here (warm_c)
   const double* rhs0;    // level-0 initial test: the right-hand side (zero start)
 };
 
@@ -235,6 +240,7 @@ struct MgPcgArgs {
   int32_t nbx, nby, first, flex;  // flex: Polak-Ribiere beta (flexible CG)
   int32_t warm, pad_;             // init: start from the x in place (r = rhs - M x)
   const double* x2;               // init: the correction before x
+  const double* x3;               // init: the one before x2
 };
 
 constexpr int kTX = 64;  // tile width (one wave64 per tile row)
