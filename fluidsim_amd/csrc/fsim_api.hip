@@ -194,7 +194,11 @@ struct fs_handle {
   int mg_it = 0, mg_chunk = 0;        // next MG-PCG iteration / chunk of a solve in progress
   hipGraphExec_t mg_chain[2][kChainLog] = {};  // 2^m MG-PCG iterations from parity p
   hipGraphExec_t learn_graph[2] = {}; // one learned-FCG iteration, by parity
-  const void* learn_graph_key = nullptr; // the network (or the handle: none) it was made for
+  // what the learned graphs were captured over: the network (or the handle:
+  // none attached) and the allocation number of its inference activations,
+  // which net_apply reallocates when a handle of another level size uses it
+  const void* learn_graph_net = nullptr;
+  uint64_t learn_graph_acts = 0;
   Geo G;
   std::vector<double> log;
   bool timing = false;
@@ -787,7 +791,8 @@ int make_learn_graph(fs_handle* h) {
   if (capture_iteration(h, &h->learn_graph[1], launch_learned_iteration, 1) ||
       capture_iteration(h, &h->learn_graph[0], launch_learned_iteration, 2))
     return FS_E_HIP;
-  h->learn_graph_key = h->net ? (const void*)h->net : (const void*)h;
+  h->learn_graph_net = h->net ? (const void*)h->net : (const void*)h;
+  h->learn_graph_acts = h->net ? h->net->net.infer.id : 0;
   return 0;
 }
 
@@ -937,7 +942,9 @@ int lapl_solve_impl(fs_handle* h, int* lerr, fs_solve_stats* stats, int mode = 0
       return FS_E_HIP;
     it = 1;
     const void* key = h->net ? (const void*)h->net : (const void*)h;
-    if (h->p.use_graphs && (!h->learn_graph[0] || h->learn_graph_key != key) &&
+    const uint64_t acts = h->net ? h->net->net.infer.id : 0;
+    if (h->p.use_graphs &&
+        (!h->learn_graph[0] || h->learn_graph_net != key || h->learn_graph_acts != acts) &&
         make_learn_graph(h))
       return FS_E_HIP;
     // iteration 0 is launched already; then 2 more, then chunk / 4 at a time
@@ -2378,6 +2385,14 @@ int fs_learn_loss_grad(fs_learn* n, int32_t B, int32_t H, int32_t W, const float
 
 int fs_learn_attach(fs_handle* h, fs_learn* n) {
   if (!h) return fail(FS_E_ARG, "null handle%s");
+  // learned-iteration graphs captured over the previous network's buffers are
+  // never replayed again (a new network may sit at a freed one's address)
+  for (int k = 0; k < 2; k++) {
+    if (h->learn_graph[k]) (void)hipGraphExecDestroy(h->learn_graph[k]);
+    h->learn_graph[k] = nullptr;
+  }
+  h->learn_graph_net = nullptr;
+  h->learn_graph_acts = 0;
   h->net = n;
   return 0;
 }

@@ -47,6 +47,7 @@ struct Acts {
   float* op = nullptr;              // d, cx, cy [N][3][H][W] (loss operator)
   float* scale = nullptr;           // s per sample
   size_t bytes = 0;
+  uint64_t id = 0;                  // allocation number, unique in the process (0: none)
 };
 
 struct Net {

@@ -14,6 +14,8 @@
 // gradient are rounded to BF16 (round to nearest even); products and sums are
 // FP32; weights are FP32 masters with BF16 copies.
 #include <hip/hip_runtime.h>
+
+#include <atomic>
 #include <math.h>
 #include <string.h>
 
@@ -661,6 +663,9 @@ int acts_alloc(Acts& a, int N, int H, int W) {
   for (size_t b : sz) total += (b + 255) / 256 * 256;
   char* base = nullptr;
   if (hipMalloc(&base, total) != hipSuccess) return -1;
+  // graphs captured over these buffers are keyed by this number
+  static std::atomic<uint64_t> next_id{1};
+  a.id = next_id.fetch_add(1);
   a.N = N;
   a.H = H;
   a.W = W;

@@ -50,16 +50,21 @@ def test_device_network_matches_cpu_reference():
     e, acts = ref.forward(in5.astype(np.float64))
     L, dout = ref.loss(e, rhat.astype(np.float64), op.astype(np.float64))
     g = ref.backward(dout, acts)
+    # measured on the MI355X: output 1.4e-6, loss 7e-7, layer gradients
+    # 2e-9 .. 3.7e-7 relative; the bounds sit just above.  FP32 accumulation
+    # order is the only difference left (both sides round to BF16 at the same
+    # points); on the CPU network, leaving out every BF16 rounding moves the
+    # output by 6e-3 and transposing one layer's 3x3 taps by 0.1 .. 0.9.
     print("out rel L2", rel_l2(out, e), "loss", loss, L)
-    assert rel_l2(out, e) <= 2e-3
-    assert loss == pytest.approx(L, rel=2e-3, abs=1e-6)
+    assert rel_l2(out, e) <= 5e-6
+    assert loss == pytest.approx(L, rel=5e-6, abs=1e-9)
     off, n = U.offsets()
     for l in range(15):
         hi = off[l + 1] if l < 14 else n
         err = rel_l2(grad[off[l]:hi], g[off[l]:hi])
         print(f"layer {l} grad rel L2 {err:.2e}")
-        assert err <= 2e-2, (l, err)
-    assert rel_l2(grad, g) <= 1e-2
+        assert err <= 1.5e-6, (l, err)
+    assert rel_l2(grad, g) <= 1e-6
 
 
 def _harvested(nx=64, steps=24, seed=0):
@@ -135,3 +140,32 @@ def test_learned_graph_replay_equals_eager_launches():
     assert np.array_equal(out[0][0], out[1][0])
     assert np.array_equal(out[0][1], out[1][1])
     assert np.array_equal(out[0][2], out[1][2])
+
+
+def test_learned_graphs_follow_reallocated_activations():
+    """One network shared by two discharges whose learned levels differ in size
+    (32x32 and 24x24 cells), alternating passes: every switch makes the network
+    reallocate its inference activations, so graphs captured over one
+    allocation must be captured again, never replayed over freed buffers.
+    Graph replay (graphs = 1) equals eager launches (graphs = 0) bitwise."""
+    from fluidsim_amd import Discharge, configs
+
+    d, net = _harvested(steps=6)
+    net.train(50, batch=8, lr=2e-3, seed=1)
+    out = {}
+    for graphs in (0, 1):
+        hs = []
+        for nx in (64, 48):
+            c = configs.config0(rtol=1e-10, solver=2, graphs=graphs)
+            c["nx"] = c["ny"] = nx
+            e = Discharge(c)
+            e.init_gaussian(seed=0)
+            net.attach(e)
+            hs.append(e)
+        for _ in range(3):
+            for e in hs:
+                e.run(2)
+        out[graphs] = [(e.get("ph"), e.get("ne", 1), e.log()) for e in hs]
+    for a, b in zip(out[0], out[1]):
+        for x, y in zip(a, b):
+            assert np.array_equal(x, y)
