@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -2056,6 +2057,52 @@ static int eval_flux_common(int32_t n, const double* a, double* out, int dfdph) 
 int fs_eval_flux(int32_t n, const double* a, double* out) { return eval_flux_common(n, a, out, 0); }
 int fs_eval_dfdph(int32_t n, const double* a, double* out) {
   return eval_flux_common(n, a, out, 1);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Staging of fs_eval_flux_noalloc: a file-scope device array instead of the
+// per-call hipMalloc of eval_flux_common.  Each device holds its own copy for
+// the life of the process; its address comes from hipGetSymbolAddress once per
+// device (the array's name is a host-side stand-in, never a device address).
+namespace {
+constexpr size_t kNoAllocScratchBytes = (size_t)2 << 20;  // 2 MiB per device
+constexpr int32_t kNoAllocMaxN = (int32_t)(kNoAllocScratchBytes / (8 * sizeof(double)));
+constexpr int kNoAllocMaxDevices = 64;
+std::mutex g_noalloc_mu;
+double* g_noalloc_ptr[kNoAllocMaxDevices];  // device address of `scratch`, per device
+}  // namespace
+
+static __device__ __align__(256) double scratch[kNoAllocScratchBytes / sizeof(double)];
+
+extern "C" {
+
+int fs_eval_flux_noalloc(int32_t n, const double* a, double* out, int32_t dfdph) {
+  if (n < 0 || n > kNoAllocMaxN)
+    return fail(FS_E_ARG, "fs_eval_flux_noalloc: n must lie in 0..%s%d", "", kNoAllocMaxN);
+  if (n == 0) return 0;
+  if (!a || !out) return fail(FS_E_ARG, "null argument%s");
+  int dev = -1;
+  HIP_OK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kNoAllocMaxDevices) return fail(FS_E_ARG, "device id out of range%s");
+  // calls in this process are serialised: they share the device's one array
+  std::lock_guard<std::mutex> lock(g_noalloc_mu);
+  double* ptr = g_noalloc_ptr[dev];
+  if (!ptr) {
+    HIP_OK(hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) return fail(FS_E_HIP, "no device address for the staging array%s");
+    g_noalloc_ptr[dev] = ptr;
+  }
+  FsPhys K;
+  fs_phys_init(&K, 10.0, 0.1);
+  double* dA = ptr;                    // 7 n doubles of arguments
+  double* dO = ptr + 7 * (size_t)n;    // n doubles of results
+  HIP_OK(hipMemcpy(dA, a, 7 * (size_t)n * sizeof(double), hipMemcpyHostToDevice));
+  launch_eval_flux(K, dA, dO, n, dfdph ? 1 : 0, nullptr);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipMemcpy(out, dO, (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
+  return 0;
 }
 
 int fs_time_kernel(fs_handle* h, int32_t which, int32_t n, double* avg_ms) {

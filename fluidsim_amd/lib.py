@@ -25,6 +25,7 @@ EXPORTS = [
     "fs_ptcl_step", "fs_lapl_solve", "fs_accept", "fs_reject", "fs_run", "fs_set_replay",
     "fs_log_count", "fs_get_log", "fs_clear_log", "fs_rhs_sweep", "fs_circ_step",
     "fs_lapl_assemble", "fs_constants", "fs_eval_fits", "fs_eval_flux", "fs_eval_dfdph",
+    "fs_eval_flux_noalloc",
     "fs_get_timers", "fs_set_timing", "fs_set_solver", "fs_time_kernel", "fs_copy_bench",
     "fs_ensemble_create", "fs_ensemble_destroy", "fs_ensemble_member", "fs_ensemble_run",
     "fs_learn_create", "fs_learn_destroy", "fs_learn_nparams", "fs_learn_get_weights",
@@ -108,6 +109,7 @@ def load():
         "fs_eval_fits": [C.c_double, C.c_int32, D, D],
         "fs_eval_flux": [C.c_int32, D, D],
         "fs_eval_dfdph": [C.c_int32, D, D],
+        "fs_eval_flux_noalloc": [C.c_int32, D, D, C.c_int32],
         "fs_get_timers": [P, D],
         "fs_set_timing": [P, C.c_int32],
         "fs_set_solver": [P, C.c_int32],

@@ -184,6 +184,16 @@ int fs_eval_fits(double p_torr, int32_t n, const double* T, double* out);
 int fs_eval_flux(int32_t n, const double* args7, double* out);
 int fs_eval_dfdph(int32_t n, const double* args7, double* out);
 
+/* fs_eval_flux (dfdph = 0) or fs_eval_dfdph (dfdph = 1) without any device
+ * allocation.  No counterpart in the reference: this is for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  The device staging lives in a
+ * fixed 2 MiB array per GPU, held until the process ends, so n is at most
+ * 32768 (larger n: FS_E_ARG, nothing launched).  Every call on one GPU
+ * shares that array: calls from threads of one process are serialised, and
+ * results equal fs_eval_flux / fs_eval_dfdph bitwise. */
+int fs_eval_flux_noalloc(int32_t n, const double* args7, double* out, int32_t dfdph);
+
 /* Select the Poisson preconditioner of later fs_lapl_solve calls (0 Jacobi,
  * 1 multigrid), as fs_params.lapl_solver does at creation.  The reference
  * takes its Krylov method and preconditioner from the runtime options database
