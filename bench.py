@@ -59,6 +59,10 @@ def gpu_run(args):
     iters = [float(r[7]) for r in log if r[6] >= 0]
     elapsed = t1 - t0
     N = args.n * args.n
+    # the state after the timed steps: the CPU baseline and the same-method
+    # leg below start from it
+    snap = snapshot(d)
+    same = None if args.no_cpu else same_method_gpu(d, snap, args)
     net, train = (None, None) if args.no_learned else learned_setup(args)
     poisson = poisson_compare(d, args.solve_reps, net)
     if net is not None:
@@ -85,7 +89,39 @@ def gpu_run(args):
     copy_gbs = copy_bandwidth(1 << 30, 20)
     return dict(cfg=cfg, elapsed=elapsed, accepted=accepted, attempts=attempts,
                 iters=iters, timers=timers, kernels=k, N=N, s1=s1, copy_gbs=copy_gbs,
-                poisson=poisson)
+                poisson=poisson, snap=snap, same=same)
+
+
+def snapshot(d):
+    """Fields (levels 1-3 of the four species, the potential) and scalars."""
+    return {"fields": {(n, lev): d.get(n, lev) for n in ("ni", "ne", "nt", "nm")
+                       for lev in (1, 2, 3)},
+            "ph": d.get("ph"), "scalars": d.scalars()}
+
+
+def restore(x, snap):
+    """Put a snapshot into a Discharge or an oracle instance (same interface)."""
+    for (n, lev), a in snap["fields"].items():
+        x.set(n, lev, a)
+    x.set("ph", 1, snap["ph"])
+    x.set_scalars(snap["scalars"])
+
+
+def same_method_gpu(d, snap, args, passes=5):
+    """The device with the CPU baseline's Poisson method: Jacobi-PCG from zero to
+    the same tolerance, `passes` passes from the snapshot; the handle is then
+    returned to the snapshot and to multigrid-PCG."""
+    restore(d, snap)
+    d.set_solver(0)
+    s0 = d.scalars()["ts"]
+    t0 = time.perf_counter()
+    d.run(passes)
+    el = time.perf_counter() - t0
+    acc = d.scalars()["ts"] - s0
+    d.set_solver(1)
+    restore(d, snap)
+    return {"poisson": f"Jacobi-PCG from zero to {args.rtol:g}", "passes": passes,
+            "value": acc / el, "unit": "steps/s"}
 
 
 def learned_setup(args):
@@ -196,25 +232,82 @@ def ensemble_run(args):
                 solo_rate=solo_acc / solo_t if solo_t > 0 else None, solo_sample=k)
 
 
-def cpu_baseline(args):
-    """The CPU oracle (C restatement of the reference, OpenMP over rows) on a
-    bounded sample: the first three passes untimed, then `cpu_steps` steps."""
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(args, snap):
+    """The CPU oracle (the C restatement of the reference, OpenMP over rows,
+    bitwise equal to its serial run) on a bounded sample of the same workload,
+    from the GPU run's state after its timed steps:
+      transport  one ptcl_step attempt (five RK-Merson stages with ptclEqn,
+                 circuit and controller; a replay record accepts it): median
+                 of `cpu_reps` on all threads, and once on one thread;
+      Poisson    one lapl_solve to the same tolerance from the state that
+                 attempt leaves, with BiCGStab + ILU(0) (the reference's KSP
+                 IBCGS with PETSc's default preconditioner) and with
+                 Jacobi-PCG, all threads.
+    value = 1 / (transport + the faster Poisson), all threads."""
+    import numpy as np
+
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from fluidsim_amd import configs
     from oracle import oracle as O
 
     nth = args.cpu_threads or min(os.cpu_count() or 1, 16)
     cfg = configs.config1(args.n, rtol=args.rtol)
-    ocfg = dict(cfg, rtol=args.rtol, maxit=cfg["lapl_maxit"], solver=2, nthreads=nth)
-    o = O.Oracle(ocfg)
-    o.init_gaussian(seed=0)
-    o.run(3)
-    t0 = time.perf_counter()
-    o.run(args.cpu_steps)
-    dt = time.perf_counter() - t0
-    return dict(value=args.cpu_steps / dt, unit="steps/s", cores=nth, kind="port",
-                sample=f"{args.cpu_steps} steps of the {args.n}x{args.n} workload after 3 "
-                       f"untimed passes; Jacobi-PCG Poisson to 1e-10, {nth} OpenMP threads")
+    dt = snap["scalars"]["dt"]
+    rec = np.zeros((1, 10))
+    rec[0, 4], rec[0, 5] = dt, 1.0
+
+    def oracle(solver, threads):
+        o = O.Oracle(dict(cfg, rtol=args.rtol, maxit=cfg["lapl_maxit"], solver=solver,
+                          nthreads=threads))
+        restore(o, snap)
+        return o
+
+    def transport(o):
+        restore(o, snap)
+        o.set_replay(rec)
+        t0 = time.perf_counter()
+        st = o.ptcl_step()
+        t = time.perf_counter() - t0
+        if st != 0:
+            raise RuntimeError(f"oracle ptcl_step status {st}")
+        return t
+
+    def poisson(o):
+        t0 = time.perf_counter()
+        st, lerr, stats = o.lapl_solve()
+        t = time.perf_counter() - t0
+        return t, int(stats[0]), lerr
+
+    ob = oracle(1, nth)
+    t_all = sorted(transport(ob) for _ in range(args.cpu_reps))[args.cpu_reps // 2]
+    t_bi, it_bi, lerr_bi = poisson(ob)
+    oj = oracle(2, nth)
+    transport(oj)
+    t_ja, it_ja, lerr_ja = poisson(oj)
+    t_ser = transport(oracle(1, 1))
+    t_pois = min(t_bi, t_ja)
+    return dict(value=1.0 / (t_all + t_pois), unit="steps/s", cores=nth, kind="port",
+                sample=(f"from the {args.n}x{args.n} state after the timed GPU steps: one "
+                        f"ptcl_step attempt (median of {args.cpu_reps}) + one Poisson solve to "
+                        f"{args.rtol:g} (the faster of BiCGStab+ILU(0) and Jacobi-PCG), "
+                        f"{nth} OpenMP threads"),
+                nproc=os.cpu_count(), cpu_model=cpu_model(),
+                transport_s=t_all, transport_s_serial=t_ser,
+                poisson_s={"bicgstab_ilu0": t_bi, "jacobi_pcg": t_ja},
+                poisson_iters={"bicgstab_ilu0": it_bi, "jacobi_pcg": it_ja},
+                poisson_converged={"bicgstab_ilu0": lerr_bi == 0, "jacobi_pcg": lerr_ja == 0},
+                value_jacobi=1.0 / (t_all + t_ja),
+                value_serial_transport_estimate=1.0 / (t_ser + t_pois))
 
 
 def profiled_traffic(kernel: str, n: int):
@@ -238,7 +331,7 @@ def main():
     ap.add_argument("--n", type=int, default=0, help="grid side (default 512, 2048 for config 2)")
     ap.add_argument("--rtol", type=float, default=1e-10)
     ap.add_argument("--chunk", type=int, default=32)
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-reps", type=int, default=3)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--solve-reps", type=int, default=3)
@@ -350,7 +443,11 @@ def main():
     if args.config == 2:
         line["metric"] = "RK-Merson steps/s at 2048^2 FP64 (large single instance)"
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(args)
+        cb = cpu_baseline(args, r["snap"])
+        line["cpu_baseline"] = cb
+        same = r["same"]
+        same["over_cpu_jacobi"] = same["value"] / cb["value_jacobi"]
+        line["gpu_same_poisson_method"] = same
     else:
         line["cpu_baseline"] = None
     print(json.dumps(line))

@@ -254,7 +254,7 @@ __device__ __forceinline__ void stage_body(const StageArgs& A, const FsPhys& K, 
       vp = gld(A.ph, m);
       if (ci >= 0 && ci < bx && cj >= 0 && cj < by) {
         if (vt >= 0.0 && ve >= 0.0) {
-          te = vt / ve;  // get_Te, ptcl_props.F90:167-180
+          te = fs_div(vt, ve);  // get_Te, ptcl_props.F90:167-180
         } else {
           te = 1e-8;
           s_bad = 1;
@@ -490,12 +490,12 @@ __device__ __forceinline__ void stage_body(const StageArgs& A, const FsPhys& K, 
           nw = n3 + dt * (k1 + k4 * 4.0 + kk) * (1.0 / 6.0);
           double e = fabs(dt * (k1 * (2.0 / 30.0) - k3 * (3.0 / 10.0) + k4 * (4.0 / 15.0) -
                                 kk * (1.0 / 30.0)));
-          double q = e / (1e-3 + 1e-3 * fabs(n3));
+          double q = fs_div(e, 1e-3 + 1e-3 * fabs(n3));
           acc[s] = q * q;
         }
         gst(A.wr[s], m, nw > nmin ? nw : nmin);
       }
-      if (stage == 5) tmin = 1.0 / (mue2 * ne + K.mui * ni);
+      if (stage == 5) tmin = fs_rcp(mue2 * ne + K.mui * ni);
     }
   }
   if (!A.rhs_only && stage == 5) {

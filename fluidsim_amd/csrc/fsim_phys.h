@@ -1,18 +1,18 @@
 // fsim_phys.h -- physical constants and per-cell / per-face physics of the
 // fluidsim plasma-transport path, as HIP device functions for gfx950.
 //
-// The functions keep the reference's operation order (ptcl_props.F90,
-// lapl_lib.F90), compiled with -ffp-contract=off (no FMA contraction), except
-// where the stage kernel trades a division or a transcendental for a
-// multiplication, each a difference of a few ulp from a left-to-right Fortran
-// evaluation: the rate fits multiply by precomputed products of their unit
-// conversions, beta uses exp10 for 10**y, and the stage kernel multiplies by
-// reciprocal grid spacings and shares one SG exponential between electrons and
+// The Poisson assembly's functions keep the reference's operation order
+// (ptcl_props.F90, lapl_lib.F90) exactly, compiled with -ffp-contract=off (no
+// FMA contraction): the assembled residual is a small difference of large
+// terms, where ulp changes in E or in the mobility show at 1e-9.  The stage
+// kernel, bound by FP64 instruction issue, uses shorter forms, each within a
+// few ulp of a left-to-right Fortran evaluation (host tests bound each one):
+// the fits in Horner form with FMA, its own exp (fs_exp), log (fs_log) and
+// Newton-refined division (fs_div, fs_rcp); the rate fits multiply by
+// precomputed products of their unit conversions, beta is exp of the cell's
+// log; reciprocal grid spacings; one SG exponential shared by electrons and
 // electron energy (their exponent arguments are equal up to rounding:
-// mut/Dt = mue/De).  The stage kernel's logarithm is fs_log below (within one
-// ulp, like the library's).  The Poisson assembly and the mobility/diffusion fits keep
-// the reference's order exactly: the assembled residual is a small difference
-// of large terms, where ulp changes in E or in the mobility show at 1e-9.
+// mut/Dt = mue/De).
 //
 // Hoisting (bit-identical to the reference's repeated calls, because the same
 // expression on the same operands gives the same value):
@@ -70,6 +70,79 @@ void fs_phys_table(const FsPhys* K, double* out40);
 #define F32(v) ((double)(v##f))
 
 // ---------------------------------------------------------------------------
+// Short arithmetic for the stage kernel.  Its cost is FP64 VALU issue: every
+// wave64 instruction holds a SIMD for 4 cycles, a coefficient moved into a
+// VGPR costs two more, so the forms below keep constants in SGPRs and drop the
+// special-value handling that the kernel's argument ranges never need.
+
+// a*b + c with c uniform across the wave (a constant or a kernel-wide value):
+// one VOP3 v_fma_f64 reading c from an SGPR pair.  (Left to itself the
+// compiler turns a Horner step into v_fmac_f64, whose accumulator must first
+// be copied into VGPRs: three instructions instead of one.)  c must not vary
+// across lanes.
+__host__ __device__ inline double fs_fma_c(double a, double b, double c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  double d;
+  asm("v_fma_f64 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "s"(c));
+  return d;
+#else
+  return fma(a, b, c);
+#endif
+}
+
+// 1/b refined from the hardware estimate (v_rcp_f64) by two Newton steps
+// r <- r + r (1 - b r), each squaring the relative error; a/b is then
+// q = a r corrected once by q + r (a - b q) (Markstein's final step).  Within
+// one ulp for normal operands with a normal quotient (a host test checks the
+// iteration from a float-precision seed); no special values.
+__host__ __device__ inline double fs_rcp(double b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  double r = __builtin_amdgcn_rcp(b);
+#else
+  double r = (double)(1.0f / (float)b);
+#endif
+  double e = fma(-b, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-b, r, 1.0);
+  r = fma(r, e, r);
+  return r;
+}
+__host__ __device__ inline double fs_div(double a, double b) {
+  const double r = fs_rcp(b);
+  const double q = a * r;
+  return fma(fma(-b, q, a), r, q);
+}
+
+// exp(y) for |y| < 700: y = k ln2 + r with ln2 split in two (Cody and Waite;
+// the high part has 32 significant bits, so k ln2_hi is exact), |r| <= ln2/2,
+// exp(r) by its Taylor series through r^13/13! (truncation below 5e-18
+// relative) in Horner form, the result 2^k exp(r).  Within about one ulp; no
+// overflow, underflow or NaN handling (the stage kernel's arguments lie in
+// [-200, 100]).
+__host__ __device__ inline double fs_exp(double y) {
+  constexpr double kLog2e = 1.4426950408889634;
+  constexpr double kLn2Hi = 6.93147180369123816490e-01;
+  constexpr double kLn2Lo = 1.90821492927058770002e-10;
+  const double k = rint(y * kLog2e);
+  double r = fma(-k, kLn2Hi, y);
+  r = fma(-k, kLn2Lo, r);
+  double p = fs_fma_c(1.0 / 6227020800.0, r, 1.0 / 479001600.0);  // 1/13!, 1/12!
+  p = fs_fma_c(p, r, 1.0 / 39916800.0);
+  p = fs_fma_c(p, r, 1.0 / 3628800.0);
+  p = fs_fma_c(p, r, 1.0 / 362880.0);
+  p = fs_fma_c(p, r, 1.0 / 40320.0);
+  p = fs_fma_c(p, r, 1.0 / 5040.0);
+  p = fs_fma_c(p, r, 1.0 / 720.0);
+  p = fs_fma_c(p, r, 1.0 / 120.0);
+  p = fs_fma_c(p, r, 1.0 / 24.0);
+  p = fs_fma_c(p, r, 1.0 / 6.0);
+  p = fs_fma_c(p, r, 0.5);
+  p = fs_fma_c(p, r, 1.0);
+  p = fs_fma_c(p, r, 1.0);
+  return ldexp(p, (int)k);
+}
+
+// ---------------------------------------------------------------------------
 // log for the stage kernel's arguments in fewer operations than the general
 // library function (no special cases: normal positive arguments); within one
 // ulp of the exact value and correctly rounded for nearly all arguments (a CPU
@@ -92,18 +165,17 @@ __host__ __device__ inline double fs_log(double v) {
     e = e - 1;
   }
   const double f = m - 1.0;
-  const double s = f / (2.0 + f);
+  const double s = fs_div(f, 2.0 + f);
   const double z = s * s;
-  double q = 2.0 / 21.0;
-  q = fma(q, z, 2.0 / 19.0);
-  q = fma(q, z, 2.0 / 17.0);
-  q = fma(q, z, 2.0 / 15.0);
-  q = fma(q, z, 2.0 / 13.0);
-  q = fma(q, z, 2.0 / 11.0);
-  q = fma(q, z, 2.0 / 9.0);
-  q = fma(q, z, 2.0 / 7.0);
-  q = fma(q, z, 2.0 / 5.0);
-  q = fma(q, z, 2.0 / 3.0);
+  double q = fs_fma_c(2.0 / 21.0, z, 2.0 / 19.0);
+  q = fs_fma_c(q, z, 2.0 / 17.0);
+  q = fs_fma_c(q, z, 2.0 / 15.0);
+  q = fs_fma_c(q, z, 2.0 / 13.0);
+  q = fs_fma_c(q, z, 2.0 / 11.0);
+  q = fs_fma_c(q, z, 2.0 / 9.0);
+  q = fs_fma_c(q, z, 2.0 / 7.0);
+  q = fs_fma_c(q, z, 2.0 / 5.0);
+  q = fs_fma_c(q, z, 2.0 / 3.0);
   const double R = z * q;
   const double hfsq = 0.5 * f * f;
   const double de = (double)e;
@@ -117,7 +189,7 @@ __host__ __device__ inline double fs_log(double v) {
 struct Pow11 {
   double x, x2, x3, x4, x5, x6, x7, x8, x9, x10, x11;
 };
-__device__ __forceinline__ Pow11 pow_chain(double x) {
+__host__ __device__ inline Pow11 pow_chain(double x) {
   Pow11 q;
   q.x = x;
   q.x2 = x * x;
@@ -145,7 +217,7 @@ __device__ __forceinline__ double fit_x_stage(double v) {
 }
 
 // get_mue from x (ptcl_props.F90:194-221): exponent, then the value
-__device__ __forceinline__ double mue_poly(const FsPhys& K, double x) {
+__host__ __device__ inline double mue_poly(const FsPhys& K, double x) {
   double x2 = x * x, x3 = x2 * x, x4 = x2 * x2, x5 = x2 * x3;
   if (x < K.log5119)
     return F32(58.1133016145) + F32(-0.984082217962) * x + F32(-0.164770900119) * x2 +
@@ -156,9 +228,25 @@ __device__ __forceinline__ double mue_poly(const FsPhys& K, double x) {
 __device__ __forceinline__ double mue_of_x(const FsPhys& K, double x) {
   return exp(mue_poly(K, x)) * K.x0 / K.ninf * K.t0 * K.ph0;
 }
-// the stage kernel's form: one product with the folded conversion
+// The stage kernel's forms of the fits: the same polynomials in Horner form
+// with FMA (the exponent moves by a few ulp of the largest power-sum term from
+// the reference's evaluation order; a host test bounds the difference), fs_exp,
+// and one product with the folded unit conversion.
+__host__ __device__ inline double mue_poly_h(double x, double log5119) {
+  double lo = fs_fma_c(F32(0.0436642742558), x, F32(-0.0637079234081));
+  lo = fs_fma_c(lo, x, F32(-0.142058042584));
+  lo = fs_fma_c(lo, x, F32(-0.164770900119));
+  lo = fs_fma_c(lo, x, F32(-0.984082217962));
+  lo = fs_fma_c(lo, x, F32(58.1133016145));
+  double hi = fs_fma_c(F32(-0.0259045697422), x, F32(0.486276623664));
+  hi = fs_fma_c(hi, x, F32(-3.57242763244));
+  hi = fs_fma_c(hi, x, F32(12.8366493242));
+  hi = fs_fma_c(hi, x, F32(-22.7558138237));
+  hi = fs_fma_c(hi, x, F32(70.7846754548));
+  return x < log5119 ? lo : hi;
+}
 __device__ __forceinline__ double mue_of_x_folded(const FsPhys& K, double x) {
-  return exp(mue_poly(K, x)) * K.c_mue;
+  return fs_exp(mue_poly_h(x, K.log5119)) * K.c_mue;
 }
 
 // get_mut from x (ptcl_props.F90:223-250)
@@ -175,7 +263,7 @@ __device__ __forceinline__ double mut_of_x(const FsPhys& K, double x) {
 }
 
 // get_De from x (ptcl_props.F90:268-291)
-__device__ __forceinline__ double de_poly(double x) {
+__host__ __device__ inline double de_poly(double x) {
   Pow11 q = pow_chain(x);
   return F32(57.7067018813) + F32(-0.0699892875381) * x + F32(-0.117645585949) * q.x2 +
          F32(0.105390295278) * q.x3 + F32(-0.102862612604) * q.x4 +
@@ -187,8 +275,21 @@ __device__ __forceinline__ double de_poly(double x) {
 __device__ __forceinline__ double de_of_x(const FsPhys& K, double x) {
   return exp(de_poly(x)) * K.x0 / K.ninf * K.t0;
 }
+__host__ __device__ inline double de_poly_h(double x) {
+  double q = fs_fma_c(F32(3.00706533201e-05), x, F32(-0.000590000082557));
+  q = fs_fma_c(q, x, F32(0.00426793748856));
+  q = fs_fma_c(q, x, F32(-0.0122860884883));
+  q = fs_fma_c(q, x, F32(0.000578601715687));
+  q = fs_fma_c(q, x, F32(0.0584908312121));
+  q = fs_fma_c(q, x, F32(-0.0469171521686));
+  q = fs_fma_c(q, x, F32(-0.102862612604));
+  q = fs_fma_c(q, x, F32(0.105390295278));
+  q = fs_fma_c(q, x, F32(-0.117645585949));
+  q = fs_fma_c(q, x, F32(-0.0699892875381));
+  return fs_fma_c(q, x, F32(57.7067018813));
+}
 __device__ __forceinline__ double de_of_x_folded(const FsPhys& K, double x) {
-  return exp(de_poly(x)) * K.c_de;
+  return fs_exp(de_poly_h(x)) * K.c_de;
 }
 
 // get_Dt from x (ptcl_props.F90:293-316)
@@ -203,56 +304,115 @@ __device__ __forceinline__ double dt_of_x(const FsPhys& K, double x) {
   return exp(s) * K.x0 / K.ninf * K.t0;
 }
 
+// The rate fits as the stage kernel evaluates them (Horner with FMA, fs_exp,
+// one product with the folded conversion); the reference's power-sum forms
+// are kept below for the host test that bounds the difference.
 // get_k_ex (ptcl_props.F90:318-345); x = log(max(0.9611, min(157, T*ph0)))
+__host__ __device__ inline double kex_poly_h(double x, double log5667) {
+  double lo = fs_fma_c(F32(-1.44763942525), x, F32(7.51502551486));
+  lo = fs_fma_c(lo, x, F32(-10.5053010711));
+  lo = fs_fma_c(lo, x, F32(-1.05188982824));
+  lo = fs_fma_c(lo, x, F32(17.1472529739));
+  lo = fs_fma_c(lo, x, F32(-54.4513453969));
+  double hi = fs_fma_c(F32(0.16403226902), x, F32(-3.0908688844));
+  hi = fs_fma_c(hi, x, F32(23.0655081891));
+  hi = fs_fma_c(hi, x, F32(-85.5163470769));
+  hi = fs_fma_c(hi, x, F32(158.67012941));
+  hi = fs_fma_c(hi, x, F32(-150.266369415));
+  return x < log5667 ? lo : hi;
+}
 __device__ __forceinline__ double kex_of_x(const FsPhys& K, double x) {
-  double x2 = x * x, x3 = x2 * x, x4 = x2 * x2, x5 = x2 * x3;
-  double s;
-  if (x < K.log5667)
-    s = F32(-54.4513453969) + F32(17.1472529739) * x + F32(-1.05188982824) * x2 +
-        F32(-10.5053010711) * x3 + F32(7.51502551486) * x4 + F32(-1.44763942525) * x5;
-  else
-    s = F32(-150.266369415) + F32(158.67012941) * x + F32(-85.5163470769) * x2 +
-        F32(23.0655081891) * x3 + F32(-3.0908688844) * x4 + F32(0.16403226902) * x5;
-  return exp(s) * K.c_rate;
+  return fs_exp(kex_poly_h(x, K.log5667)) * K.c_rate;
 }
 
 // get_k_ir (ptcl_props.F90:347-374); x = log(max(1.37, min(157, T*ph0)))
+__host__ __device__ inline double kir_poly_h(double x, double log75) {
+  double lo = fs_fma_c(F32(-14.9080865672), x, F32(81.9325486193));
+  lo = fs_fma_c(lo, x, F32(-161.378167125));
+  lo = fs_fma_c(lo, x, F32(138.743842326));
+  lo = fs_fma_c(lo, x, F32(-30.1312514721));
+  lo = fs_fma_c(lo, x, F32(-73.27080258));
+  double hi = fs_fma_c(F32(0.16331188192), x, F32(-3.15463258104));
+  hi = fs_fma_c(hi, x, F32(24.1814225482));
+  hi = fs_fma_c(hi, x, F32(-92.3047166553));
+  hi = fs_fma_c(hi, x, F32(177.113452524));
+  hi = fs_fma_c(hi, x, F32(-168.925304837));
+  return x < log75 ? lo : hi;
+}
 __device__ __forceinline__ double kir_of_x(const FsPhys& K, double x) {
-  double x2 = x * x, x3 = x2 * x, x4 = x2 * x2, x5 = x2 * x3;
-  double s;
-  if (x < K.log75)
-    s = F32(-73.27080258) + F32(-30.1312514721) * x + F32(138.743842326) * x2 +
-        F32(-161.378167125) * x3 + F32(81.9325486193) * x4 + F32(-14.9080865672) * x5;
-  else
-    s = F32(-168.925304837) + F32(177.113452524) * x + F32(-92.3047166553) * x2 +
-        F32(24.1814225482) * x3 + F32(-3.15463258104) * x4 + F32(0.16331188192) * x5;
-  return exp(s) * K.c_rate;
+  return fs_exp(kir_poly_h(x, K.log75)) * K.c_rate;
 }
 
-// get_nu (ptcl_props.F90:376-395).  The reference's real exponents x**3. ..
-// x**8. are libm pow() calls; here they are power-tree products (<= 2 ulp apart).
+// get_nu (ptcl_props.F90:376-395)
+__host__ __device__ inline double nu_poly_h(double x) {
+  double q = fs_fma_c(F32(0.000325615321708), x, F32(-0.00542014733763));
+  q = fs_fma_c(q, x, F32(0.0309832277826));
+  q = fs_fma_c(q, x, F32(-0.0582934303409));
+  q = fs_fma_c(q, x, F32(-0.0414865809044));
+  q = fs_fma_c(q, x, F32(0.129397015353));
+  q = fs_fma_c(q, x, F32(0.00936933121094));
+  q = fs_fma_c(q, x, F32(1.45173283977));
+  return fs_fma_c(q, x, F32(-32.275912575));
+}
 __device__ __forceinline__ double nu_of_x(const FsPhys& K, double x) {
-  double x2 = x * x, x3 = x2 * x, x4 = x2 * x2, x5 = x2 * x3, x6 = x3 * x3, x7 = x3 * x4,
-         x8 = x4 * x4;
-  double s = F32(-32.275912575) + F32(1.45173283977) * x + F32(0.00936933121094) * x2 +
-             F32(0.129397015353) * x3 + F32(-0.0414865809044) * x4 +
-             F32(-0.0582934303409) * x5 + F32(0.0309832277826) * x6 +
-             F32(-0.00542014733763) * x7 + F32(0.000325615321708) * x8;
-  return exp(s) * K.c_nu;
+  return fs_exp(nu_poly_h(x)) * K.c_nu;
 }
 
 // get_k_sc / get_k_si (ptcl_props.F90:397-429); x = log(min(16, max(0.5, T*ph0)))
+__host__ __device__ inline double ksc_poly_h(double x) {
+  double q = fs_fma_c(F32(0.10591014464), x, F32(-0.67840685073));
+  q = fs_fma_c(q, x, F32(1.27257798891));
+  q = fs_fma_c(q, x, F32(-0.555439231606));
+  q = fs_fma_c(q, x, F32(0.457356923276));
+  return fs_fma_c(q, x, F32(-21.4827864151));
+}
+__host__ __device__ inline double ksi_poly_h(double x) {
+  double q = fs_fma_c(F32(-0.931626144207), x, F32(8.28853856817));
+  q = fs_fma_c(q, x, F32(-28.1169537586));
+  q = fs_fma_c(q, x, F32(43.9905424566));
+  return fs_fma_c(q, x, F32(-43.1347385848));
+}
 __device__ __forceinline__ double ksc_of_x(const FsPhys& K, double x) {
-  double x2 = x * x, x3 = x2 * x, x4 = x2 * x2, x5 = x2 * x3;
-  double s = F32(-21.4827864151) + F32(0.457356923276) * x + F32(-0.555439231606) * x2 +
-             F32(1.27257798891) * x3 + F32(-0.67840685073) * x4 + F32(0.10591014464) * x5;
-  return exp(s) * K.c_rate6;
+  return fs_exp(ksc_poly_h(x)) * K.c_rate6;
 }
 __device__ __forceinline__ double ksi_of_x(const FsPhys& K, double x) {
+  return fs_exp(ksi_poly_h(x)) * K.c_rate6;
+}
+
+// The reference's power-sum exponents of the rate fits (left to right, powers
+// by the power tree), for the host test of the Horner forms above.
+__host__ __device__ inline double kex_poly_ref(double x, double log5667) {
+  double x2 = x * x, x3 = x2 * x, x4 = x2 * x2, x5 = x2 * x3;
+  if (x < log5667)
+    return F32(-54.4513453969) + F32(17.1472529739) * x + F32(-1.05188982824) * x2 +
+           F32(-10.5053010711) * x3 + F32(7.51502551486) * x4 + F32(-1.44763942525) * x5;
+  return F32(-150.266369415) + F32(158.67012941) * x + F32(-85.5163470769) * x2 +
+         F32(23.0655081891) * x3 + F32(-3.0908688844) * x4 + F32(0.16403226902) * x5;
+}
+__host__ __device__ inline double kir_poly_ref(double x, double log75) {
+  double x2 = x * x, x3 = x2 * x, x4 = x2 * x2, x5 = x2 * x3;
+  if (x < log75)
+    return F32(-73.27080258) + F32(-30.1312514721) * x + F32(138.743842326) * x2 +
+           F32(-161.378167125) * x3 + F32(81.9325486193) * x4 + F32(-14.9080865672) * x5;
+  return F32(-168.925304837) + F32(177.113452524) * x + F32(-92.3047166553) * x2 +
+         F32(24.1814225482) * x3 + F32(-3.15463258104) * x4 + F32(0.16331188192) * x5;
+}
+__host__ __device__ inline double nu_poly_ref(double x) {
+  double x2 = x * x, x3 = x2 * x, x4 = x2 * x2, x5 = x2 * x3, x6 = x3 * x3, x7 = x3 * x4,
+         x8 = x4 * x4;
+  return F32(-32.275912575) + F32(1.45173283977) * x + F32(0.00936933121094) * x2 +
+         F32(0.129397015353) * x3 + F32(-0.0414865809044) * x4 + F32(-0.0582934303409) * x5 +
+         F32(0.0309832277826) * x6 + F32(-0.00542014733763) * x7 + F32(0.000325615321708) * x8;
+}
+__host__ __device__ inline double ksc_poly_ref(double x) {
+  double x2 = x * x, x3 = x2 * x, x4 = x2 * x2, x5 = x2 * x3;
+  return F32(-21.4827864151) + F32(0.457356923276) * x + F32(-0.555439231606) * x2 +
+         F32(1.27257798891) * x3 + F32(-0.67840685073) * x4 + F32(0.10591014464) * x5;
+}
+__host__ __device__ inline double ksi_poly_ref(double x) {
   double x2 = x * x, x3 = x2 * x, x4 = x2 * x2;
-  double s = F32(-43.1347385848) + F32(43.9905424566) * x + F32(-28.1169537586) * x2 +
-             F32(8.28853856817) * x3 + F32(-0.931626144207) * x4;
-  return exp(s) * K.c_rate6;
+  return F32(-43.1347385848) + F32(43.9905424566) * x + F32(-28.1169537586) * x2 +
+         F32(8.28853856817) * x3 + F32(-0.931626144207) * x4;
 }
 
 // get_beta (ptcl_props.F90:182-192), v = T*ph0, xa = fit_x(v):
@@ -263,7 +423,7 @@ __device__ __forceinline__ double beta_of_v(const FsPhys& K, double v, double xa
   double w = v * (11.604505e3 / F32(1.5));
   if (!(w < 5e4)) return K.beta_hi;
   double lv = v >= 2.34e-1 ? xa : log(v);
-  return exp(K.beta_a - 0.665 * lv) * K.c_beta;
+  return fs_exp(K.beta_a - 0.665 * lv) * K.c_beta;
 }
 
 // The six reaction-rate fits at one cell from v = T*ph0 and xa = fit_x(v).
@@ -295,25 +455,32 @@ __device__ __forceinline__ double sg_flux(const FsPhys& K, double E, double h, d
   double na = arg > 0 ? -arg : arg;
   double ex = na < -100.0 ? K.exp_m100 : exp(na);
   double num = arg > 0 ? (nl - nr * ex) : (nr - nl * ex);
-  return v * num / (1.0 - ex);
+  double om = 1.0 - ex;
+  // |arg| below 2^-53: the reference divides by zero here and stops on the
+  // NaN; the a -> 0 limit of the flux is Fick's
+  return om == 0.0 ? D * (nl - nr) * rh : v * num / om;
 }
 
+// The stage kernel's SG fluxes, with fs_exp and the Newton-refined division
+// (the face's 1 - exp(-|arg|) carries their ulp as eps/|arg| relative, as it
+// carries the library exp's).
 // Ion flux (qs = +1, mu = mui, D = Di) with the exponent argument formed by
 // multiplying by 1/Di.
-__device__ __forceinline__ double sg_flux_ion(const FsPhys& K, double E, double h, double rh,
+__host__ __device__ inline double sg_flux_ion(const FsPhys& K, double E, double h, double rh,
                                               double nl, double nr) {
   double v = K.mui * E;
   double arg = v * h * K.r_Di;
   if (fabs(E) < K.ftol) return K.Di * (nl - nr) * rh;
   double na = arg > 0 ? -arg : arg;
-  double ex = na < -100.0 ? K.exp_m100 : exp(na);
+  double ex = na < -100.0 ? K.exp_m100 : fs_exp(na);
   double num = arg > 0 ? (nl - nr * ex) : (nr - nl * ex);
-  return v * num / (1.0 - ex);
+  const double om = 1.0 - ex;
+  return om == 0.0 ? K.Di * (nl - nr) * rh : fs_div(v * num, om);  // a -> 0: Fick
 }
 
 // Electron and electron-energy fluxes of one face (qs = -1; mut = c53 mue,
 // Dt = c53 De): one exponential of the electron argument for both.
-__device__ __forceinline__ void sg_flux_et(const FsPhys& K, double E, double h, double rh,
+__host__ __device__ inline void sg_flux_et(const FsPhys& K, double E, double h, double rh,
                                            double mue, double de, double mut, double dt,
                                            double neL, double neR, double ntL, double ntR,
                                            double& fe, double& ft) {
@@ -323,12 +490,17 @@ __device__ __forceinline__ void sg_flux_et(const FsPhys& K, double E, double h, 
     return;
   }
   const double ve = -(mue * E), vt = -(mut * E);
-  const double arg = ve * h / de;
+  const double arg = fs_div(ve * h, de);
   const double na = arg > 0 ? -arg : arg;
-  const double ex = na < -100.0 ? K.exp_m100 : exp(na);
-  const double om = 1.0 / (1.0 - ex);
+  const double ex = na < -100.0 ? K.exp_m100 : fs_exp(na);
+  const double om1 = 1.0 - ex;
+  const double om = fs_rcp(om1);
   fe = ve * (arg > 0 ? (neL - neR * ex) : (neR - neL * ex)) * om;
   ft = vt * (arg > 0 ? (ntL - ntR * ex) : (ntR - ntL * ex)) * om;
+  if (om1 == 0.0) {  // |arg| below 2^-53: the a -> 0 (Fick) limit
+    fe = de * (neL - neR) * rh;
+    ft = dt * (ntL - ntR) * rh;
+  }
 }
 
 // SG flux and its derivative w.r.t. the drift velocity (dFdPh,
@@ -342,7 +514,9 @@ __device__ __forceinline__ void sg_flux_dfdph(const FsPhys& K, double E, double 
   double na = arg > 0 ? -arg : arg;
   double ef = exp(na);                         // dFdPh: exp(-|a|), unclamped
   double ex = na < -100.0 ? K.exp_m100 : ef;   // get_Flux: exp(max(-|a|, -100))
-  if (fabs(qs * E) < K.ftol) {
+  if (fabs(qs * E) < K.ftol || 1.0 - ex == 0.0) {
+    // (|arg| below 2^-53: the reference divides by zero and stops; the
+    // a -> 0 limit is Fick's flux)
     flux = D * (nl - nr) / h;
   } else {
     double num = arg > 0 ? (nl - nr * ex) : (nr - nl * ex);
@@ -354,5 +528,6 @@ __device__ __forceinline__ void sg_flux_dfdph(const FsPhys& K, double E, double 
     double om = 1.0 - ef;
     double t1 = arg > 0 ? (nl - nr * ef) / om : (nr - nl * ef) / om;
     df = t1 + v * ef * (nr - nl) / (om * om) * h / D;
+    if (om == 0.0) df = 5e-1 * (nl + nr);  // the a -> 0 limit of dF/dv
   }
 }

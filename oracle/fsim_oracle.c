@@ -275,11 +275,17 @@ static double get_flux(const oconsts* k, double E, double dh, int q, double mu, 
   double v = (double)q * mu * E;
   double arg = v * dh / D;
   if (fabs((double)q * E) < tol) return D * (nl - nr) / dh;
+  /* DIFF: where 1 - exp(-|arg|) rounds to 0 (|arg| below 2^-53, a field of
+   * rounding size that passes the |qE| < tol test) the reference divides by
+   * zero and stops on the NaN (ptcl_lib.F90:530-549); here the a -> 0 limit,
+   * Fick's flux, as in the device kernels */
   if (arg > 0) {
     double ex = exp(fmax(-arg, -100.0));
+    if (1.0 - ex == 0.0) return D * (nl - nr) / dh;
     return v * (nl - nr * ex) / (1.0 - ex);
   } else {
     double ex = exp(fmax(arg, -100.0));
+    if (1.0 - ex == 0.0) return D * (nl - nr) / dh;
     return v * (nr - nl * ex) / (1.0 - ex);
   }
 }
@@ -297,9 +303,11 @@ static double dfdph(double E_pl, double E_mi, int q, double mu, double D, double
   } else if (a_mi > 0) {
     double ex = exp(-a_mi), om = 1.0 - ex;
     df = (n1 - n2 * ex) / om + v_mi * ex * (n2 - n1) / (om * om) * dx / D;
+    if (om == 0.0) df = 5e-1 * (n1 + n2); /* DIFF: the a -> 0 limit (see get_flux) */
   } else {
     double ex = exp(a_mi), om = 1.0 - ex;
     df = (n2 - n1 * ex) / om + v_mi * ex * (n2 - n1) / (om * om) * dx / D;
+    if (om == 0.0) df = 5e-1 * (n1 + n2);
   }
   if (isnan(df)) g_nan_dfdph = 1; /* reference: "flx is NaN" + stop */
   return df;

@@ -170,6 +170,34 @@ def test_sg_flux_limits(O):
     assert big == v * (5.0 - 2.0 * np.exp(-100.0)) / (1.0 - np.exp(-100.0))
 
 
+def test_sg_degenerate_argument_takes_the_limit(O):
+    """|qE| and |v| pass the reference's 1e-12 tests but |arg| < 2^-53, so
+    1 - exp(-|arg|) is exactly 0: the reference divides by zero there and
+    stops; the oracle (like the kernels) takes the a -> 0 limits, Fick's flux
+    D (nL - nR) / h and dF/dv = (nL + nR) / 2 (the Bernoulli form's limit)."""
+    import mpmath as mp
+
+    D, h, mu, nl, nr, E = 1e3, 1e-3, 1e-3, 5.0, 2.0, 1e-8     # |v| = 1e-11, arg = 1e-17
+    for q in (1.0, -1.0):
+        assert abs(q * mu * E * h / D) < 2.0 ** -53
+        assert O.eval_flux([[E, h, q, mu, D, nl, nr]])[0] == D * (nl - nr) / h
+        assert O.eval_dfdph([[E, h, q, mu, D, nl, nr]])[0] == 0.5 * (nl + nr)
+    # the limit of the exact derivative (50 digits) at a small argument
+    mp.mp.dps = 80
+    v = mp.mpf("1e-6")                                          # a = 1e-12
+
+    def flux_v(v):
+        a_ = v * mp.mpf(h) / mp.mpf(D)
+        return v * (mp.mpf(nl) - mp.mpf(nr) * mp.exp(-a_)) / (1 - mp.exp(-a_))
+
+    dv = mp.mpf("1e-26")
+    assert abs(float((flux_v(v + dv) - flux_v(v - dv)) / (2 * dv)) - 0.5 * (nl + nr)) < 1e-10
+    # just above the degenerate band the reference's own formula stays (finite)
+    E2 = 1e-3                                                   # arg = 1e-15
+    f = O.eval_flux([[E2, h, 1.0, mu, D, nl, nr]])[0]
+    assert np.isfinite(f) and f != D * (nl - nr) / h
+
+
 def test_dfdph_is_derivative_of_sg_flux(O):
     import mpmath as mp
 

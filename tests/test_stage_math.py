@@ -1,5 +1,7 @@
-"""The stage kernel's logarithm (fs_log in fsim_phys.h), compiled for the host
-from the same header and compared with long-double libm."""
+"""The stage kernel's short arithmetic (fsim_phys.h), compiled for the host from
+the same header: fs_log and fs_exp against long-double libm, fs_div / fs_rcp
+against long-double division, and the Horner-form fit exponents against the
+reference's power-sum order."""
 import os
 import re
 import subprocess
@@ -31,9 +33,42 @@ def report(tmp_path_factory):
     return rows
 
 
+def _rows(report, fn):
+    return {k: v for k, v in report.items() if k[0] == fn}
+
+
 def test_fs_log_within_one_ulp(report):
-    assert len(report) == 3
-    for key, (ulp, wrong) in report.items():
+    rows = _rows(report, "log")
+    assert len(rows) == 3
+    for key, (ulp, wrong) in rows.items():
         # within one ulp everywhere, correctly rounded for nearly all arguments
         assert ulp < 1.0, (key, ulp)
         assert wrong < 0.08, (key, wrong)
+
+
+def test_fs_exp_within_one_ulp(report):
+    rows = _rows(report, "exp")
+    assert len(rows) == 3
+    for key, (ulp, wrong) in rows.items():
+        assert ulp < 1.0, (key, ulp)
+        assert wrong < 0.1, (key, wrong)
+
+
+def test_newton_division_and_reciprocal(report):
+    for fn in ("div", "rcp"):
+        (key, (ulp, wrong)), = _rows(report, fn).items()
+        assert ulp <= 0.5 + 1e-9 and wrong < 0.01, (key, ulp, wrong)
+
+
+# largest |Horner - power sum| of each fit's exponent, in units of 2^-52 (the
+# relative difference of the fit itself): the terms of the power sums reach
+# 8e3 against values of 20 .. 170, so the two orders round differently by up
+# to 6e-12; the device fit parity test bounds the stage fits at 1e-11
+FIT_BOUND = {"fit_mue": 2048, "fit_De": 32768, "fit_kex": 8192, "fit_kir": 8192,
+             "fit_nu": 2048, "fit_ksc": 1024, "fit_ksi": 1024}
+
+
+def test_horner_fits_against_the_reference_order(report):
+    for name, bound in FIT_BOUND.items():
+        (key, (diff, _)), = _rows(report, name).items()
+        assert diff <= bound, (key, diff)
