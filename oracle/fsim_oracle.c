@@ -327,6 +327,8 @@ struct osim {
   double *ph, *ph_mi, *Em;
   double t, dt, err_prev, t_m;
   double Vmax, Vsrc, Vd_pl, Vd_mi, Vd_or, Ida, Idc, Cap, Res, kv[6];
+  double *sig_pl, *sig_mi, *sig_or; /* wall surface charge (bx+2), 1-based: sfc_lib.F90:7 */
+  double *ks;                       /* its stage slopes (bx+2) x 6 */
   int ts, status;
   /* Poisson work */
   double *rhs, *A5, *sol, *wk, *part;
@@ -399,10 +401,10 @@ osim* o_create(const ocfg* c) {
   for (int j = 1; j <= by + 2; j++) s->r[j] = s->y[j];                       /* props.F90:165-166 */
   s->type_x = calloc((size_t)bx * by, sizeof(int));
   s->type_y = calloc((size_t)bx * by, sizeof(int));
-  for (int j = 1; j <= by; j++) /* props.F90:186-218 (rwall is out of scope) */
+  for (int j = 1; j <= by; j++) /* props.F90:186-218 */
     for (int i = 1; i <= bx; i++) {
       if (j == 1) TY(i, j) = -1;
-      if (j == by) TY(i, j) = 1;
+      if (j == by) TY(i, j) = c->rwall ? 3 : 1; /* props.F90:192-198 */
       int el = (s->y[j] <= c->ew) || (s->y[j] >= c->gap);
       if (i == 1) TX(i, j) = el ? -2 : -1;
       if (i == bx) TX(i, j) = el ? 2 : 1;
@@ -419,6 +421,11 @@ osim* o_create(const ocfg* c) {
   s->ph = calloc(PL, sizeof(double));
   s->ph_mi = calloc(PL, sizeof(double));
   s->Em = calloc(by + 3, sizeof(double)); /* em_lib.F90:104: Em = 0 */
+  /* sfc_init (sfc_lib.F90:82-90): sig_pl = sig_mi = sig_or = 0 */
+  s->sig_pl = calloc(bx + 3, sizeof(double));
+  s->sig_mi = calloc(bx + 3, sizeof(double));
+  s->sig_or = calloc(bx + 3, sizeof(double));
+  s->ks = calloc((size_t)(bx + 3) * 6, sizeof(double));
   size_t N = (size_t)bx * by;
   s->rhs = calloc(N, sizeof(double));
   s->A5 = calloc(5 * N, sizeof(double));
@@ -446,6 +453,7 @@ void o_destroy(osim* s) {
   free(s->ni); free(s->ne); free(s->nt); free(s->nm);
   free(s->ki); free(s->ke); free(s->kt); free(s->km);
   free(s->ph); free(s->ph_mi); free(s->Em);
+  free(s->sig_pl); free(s->sig_mi); free(s->sig_or); free(s->ks);
   free(s->rhs); free(s->A5); free(s->sol); free(s->wk); free(s->part);
   free(s->replay); free(s->log);
   free(s);
@@ -674,13 +682,16 @@ static void ptcl_eqn(osim* s, int i, int j, int stage, double* tm_out) {
     }
   }
 
-  /* Y-dir: ptcl_lib.F90:285-443 (ny > 1 always here; rwall out of scope) */
+  /* Y-dir: ptcl_lib.F90:285-443 (ny > 1 always here) */
   if (ty == -1) {
     Ey[1] = 0.0;
     Ey[2] = -(PH(i, j + 1) - PH(i, j)) / dy[j];
   } else if (ty == 1) {
     Ey[1] = -(PH(i, j) - PH(i, j - 1)) / dy[j - 1];
     Ey[2] = 0.0;
+  } else if (ty == 3) { /* dielectric wall: ptcl_lib.F90:292-294 */
+    Ey[1] = -(PH(i, j) - PH(i, j - 1)) / dy[j - 1];
+    Ey[2] = -s->sig_mi[i];
   } else {
     Ey[1] = -(PH(i, j) - PH(i, j - 1)) / dy[j - 1];
     Ey[2] = -(PH(i, j + 1) - PH(i, j)) / dy[j];
@@ -726,8 +737,19 @@ static void ptcl_eqn(osim* s, int i, int j, int stage, double* tm_out) {
     fluxe_y[1] = get_flux(K, Ey[1], dy[j - 1], -1, mue[1], De[1], NE(i, j - 1), NE(i, j));
     fluxt_y[1] = get_flux(K, Ey[1], dy[j - 1], -1, mut[1], Dt[1], NT(i, j - 1), NT(i, j));
     fluxm_y[1] = -Dm * (NM(i, j) - NM(i, j - 1)) / dy[j - 1];
-    /* type_y 3 (rwall) and 2 (never assigned) are out of scope */
-    fluxi_y[2] = 0.0; fluxe_y[2] = 0.0; fluxt_y[2] = 0.0; fluxm_y[2] = 0.0;
+    if (ty == 3) { /* flux into the dielectric: ptcl_lib.F90:393-409 (type 2 is never assigned) */
+      Te[2] = get_Te(NT(i, j), NE(i, j));
+      mue[2] = get_mue(K, Te[2]);
+      De[2] = get_De(K, Te[2]);
+      mut[2] = get_mut(K, Te[2]);
+      Dt[2] = get_Dt(K, Te[2]);
+      fluxi_y[2] = get_flux(K, Ey[2], dy[j], 1, mui, Di, NI(i, j), 0.0);
+      fluxe_y[2] = get_flux(K, Ey[2], dy[j], -1, mue[2], De[2], NE(i, j), 0.0);
+      fluxt_y[2] = get_flux(K, Ey[2], dy[j], -1, mut[2], Dt[2], NT(i, j), 0.0);
+      fluxm_y[2] = Dm * NM(i, j) / dy[j];
+    } else {
+      fluxi_y[2] = 0.0; fluxe_y[2] = 0.0; fluxt_y[2] = 0.0; fluxm_y[2] = 0.0;
+    }
   }
 
   /* divergence: ptcl_lib.F90:446-480 */
@@ -969,6 +991,7 @@ int o_ptcl_step(osim* s) {
     o_rhs_sweep(s, stage);
     if (s->status < 0) return s->status;
     o_circ_step(s, stage);
+    if (s->c.rwall) o_sfc_step(s, stage); /* ptcl_lib.F90:52 */
     double e[4];
     o_rk_all(s, stage, e);
     if (stage == 5) {
@@ -995,9 +1018,64 @@ int o_ptcl_step(osim* s) {
         copy_level(s, 2, 3);
         s->Vd_pl = s->Vd_or;
         s->Vd_mi = s->Vd_or;
+        for (int i = 1; i <= s->bx + 2; i++) s->sig_pl[i] = s->sig_mi[i] = s->sig_or[i];
       }
     }
   }
+  return 0;
+}
+
+/* sfc_step: sfc_lib.F90:20-79, the dielectric wall's surface charge, Merson
+ * stages with the circuit's tableau (ks = ion minus electron flux into the
+ * wall).  Two departures from the text, both in the reference's own call:
+ *  - ptcl_step calls sfc_step(g, stage, ni(:,:,2), ne(:,:,2), nt(:,:,2))
+ *    (ptcl_lib.F90:52) against the dummy order (g, stage, ne, ni, nt), so
+ *    inside it `ne` is the ion density and `ni` the electron density: kept
+ *    (Te = nt / n_ion, the "ion" flux from the electron density and so on);
+ *  - the row test reads type_y(i-1, by+1), one column past the (bx, by)
+ *    array: undefined in the reference; here the top row type_y(i-1, by),
+ *    the row the wall belongs to (props.F90:192-198). */
+void o_sfc_step(osim* s, int stage) {
+  const oconsts* K = &s->k;
+  const int bx = s->bx, by = s->by, j = by + 1;
+#define NE_D(a, b) A3(s->ni, a, b, 2) /* dummy ne <- actual ni */
+#define NI_D(a, b) A3(s->ne, a, b, 2) /* dummy ni <- actual ne */
+#define NT_D(a, b) A3(s->nt, a, b, 2)
+  for (int i = 2; i <= bx + 1; i++) {
+    if (TY(i - 1, by) != 3) continue;
+    const double Te = get_Te(NT_D(i, j), NE_D(i, j));
+    const double mue = get_mue(K, Te), De = get_De(K, Te);
+    const double fluxi = get_flux(K, -s->sig_mi[i], s->dy[j], 1, K->mui, K->Di, NI_D(i, j), 0.0);
+    const double fluxe = get_flux(K, -s->sig_mi[i], s->dy[j], -1, mue, De, NE_D(i, j), 0.0);
+    s->ks[(size_t)i * 6 + stage] = fluxi - fluxe;
+  }
+#undef NE_D
+#undef NI_D
+#undef NT_D
+  const double dt = s->dt;
+  for (int i = 2; i <= bx + 1; i++) {
+    if (TY(i - 1, by) != 3) continue;
+    const double* k = s->ks + (size_t)i * 6;
+    if (stage == 1) s->sig_mi[i] = s->sig_or[i] + dt * k[1] / 3.0;
+    else if (stage == 2) s->sig_mi[i] = s->sig_or[i] + dt * (k[1] + k[2]) / 6.0;
+    else if (stage == 3) s->sig_mi[i] = s->sig_or[i] + dt * (k[1] + k[3] * 3.0) / 8.0;
+    else if (stage == 4)
+      s->sig_mi[i] = s->sig_or[i] + dt * (k[1] - k[3] * 3.0 + k[4] * 4.0) / 2.0;
+    else s->sig_pl[i] = s->sig_or[i] + dt * (k[1] + k[4] * 4.0 + k[5]) / 6.0;
+  }
+  /* the halo exchange of sig (sfc_lib.F90:60-72) is MPI_PROC_NULL at one rank */
+}
+
+int o_get_sig(const osim* s, int which, double* out) {
+  const double* a = which == 0 ? s->sig_pl : which == 1 ? s->sig_mi : which == 2 ? s->sig_or : 0;
+  if (!a) return -1;
+  memcpy(out, a + 1, (size_t)(s->bx + 2) * sizeof(double));
+  return 0;
+}
+int o_set_sig(osim* s, int which, const double* in) {
+  double* a = which == 0 ? s->sig_pl : which == 1 ? s->sig_mi : which == 2 ? s->sig_or : 0;
+  if (!a) return -1;
+  memcpy(a + 1, in, (size_t)(s->bx + 2) * sizeof(double));
   return 0;
 }
 
@@ -1005,7 +1083,7 @@ int o_ptcl_step(osim* s) {
 /* laplEqn: lapl_lib.F90:123-379                                             */
 /* ------------------------------------------------------------------------ */
 static double lapl_eqn(osim* s, int i, int j, const double* ne, const double* ni,
-                       const double* nt) {
+                       const double* nt, double sig) {
   const oconsts* K = &s->k;
   double dfdx = 0, dfdy = 0, dflxe_x = 0, dflxe_y = 0, dflxi_x = 0, dflxi_y = 0;
   double Ex[3] = {0, 0, 0}, Ey[3] = {0, 0, 0}, mue[3] = {0, 0, 0}, De[3] = {0, 0, 0};
@@ -1084,6 +1162,9 @@ static double lapl_eqn(osim* s, int i, int j, const double* ne, const double* ni
   } else if (ty == 1) {
     Ey[1] = -(PH(i, j) - PH(i, j - 1)) / dy[j - 1];
     Ey[2] = 0.0;
+  } else if (ty == 3) { /* dielectric wall: lapl_lib.F90:263-265 */
+    Ey[1] = -(PH(i, j) - PH(i, j - 1)) / dy[j - 1];
+    Ey[2] = -sig;
   } else {
     Ey[1] = -(PH(i, j) - PH(i, j - 1)) / dy[j - 1];
     Ey[2] = -(PH(i, j + 1) - PH(i, j)) / dy[j];
@@ -1116,8 +1197,16 @@ static double lapl_eqn(osim* s, int i, int j, const double* ne, const double* ni
     De[1] = 5e-1 * (get_De(K, Te[1]) + get_De(K, Te[2]));
     flxi[1] = get_flux(K, Ey[1], dy[j - 1], 1, mui, Di, NI(i, j - 1), NI(i, j));
     flxe[1] = get_flux(K, Ey[1], dy[j - 1], -1, mue[1], De[1], NE(i, j - 1), NE(i, j));
-    flxi[2] = 0.0;
-    flxe[2] = 0.0;
+    if (ty == 3) { /* lapl_lib.F90:324-335 */
+      Te[2] = get_Te(NT(i, j), NE(i, j));
+      mue[2] = get_mue(K, Te[2]);
+      De[2] = get_De(K, Te[2]);
+      flxi[2] = get_flux(K, Ey[2], dy[j], 1, mui, Di, NI(i, j), 0.0);
+      flxe[2] = get_flux(K, Ey[2], dy[j], -1, mue[2], De[2], NE(i, j), 0.0);
+    } else {
+      flxi[2] = 0.0;
+      flxe[2] = 0.0;
+    }
   }
   if (s->c.cyl) {
     dfdy = -(Ey[2] * (r[j + 1] + r[j]) / 2.0 - Ey[1] * (r[j] + r[j - 1]) / 2.0) / s->dly[j - 1] /
@@ -1289,7 +1378,10 @@ int o_lapl_assemble(osim* s, const double* ne, const double* ni, const double* n
   for (int j = 2; j <= by + 1; j++)
     for (int i = 2; i <= bx + 1; i++) {
       size_t n = (size_t)(i - 2) + (size_t)(j - 2) * bx; /* props.F90:229-230 */
-      b[n] = lapl_eqn(s, i, j, ne, ni, nt);
+      /* sig0 = sig(i) on the dielectric row, else 0 (lapl_lib.F90:72-76); main
+       * passes sig_pl (main.F90:80) */
+      const double sig0 = TY(i - 1, j - 1) == 3 ? s->sig_pl[i] : 0.0;
+      b[n] = lapl_eqn(s, i, j, ne, ni, nt, sig0);
       lapl_j(s, i, j, ne, ni, nt, A5 + 5 * n);
     }
   if (g_nan_dfdph) return -5;
@@ -1551,6 +1643,10 @@ void o_accept(osim* s) {
   copy_level(s, 2, 1);
   s->Vd_or = s->Vd_mi;
   s->Vd_mi = s->Vd_pl;
+  for (int i = 1; i <= s->bx + 2; i++) { /* main.F90:100-101 */
+    s->sig_or[i] = s->sig_mi[i];
+    s->sig_mi[i] = s->sig_pl[i];
+  }
 }
 
 /* main.F90:103-121 */
@@ -1560,6 +1656,7 @@ void o_reject(osim* s) {
   copy_level(s, 2, 3);
   s->Vd_pl = s->Vd_or;
   s->Vd_mi = s->Vd_or;
+  for (int i = 1; i <= s->bx + 2; i++) s->sig_pl[i] = s->sig_mi[i] = s->sig_or[i];
 }
 
 /* n_iters passes of the main loop body: main.F90:68-121 */

@@ -10,6 +10,7 @@
  *   ptcl_props.F90   fits, SG flux, RK-Merson     (ptcl_props.F90:5-429)
  *   ptcl_lib.F90     RHS sweep, stage loop, dt    (ptcl_lib.F90:15-550)
  *   circ_lib.F90     electrode potential/circuit  (circ_lib.F90:19-223)
+ *   sfc_lib.F90      dielectric wall charge       (sfc_lib.F90:20-90)
  *   lapl_lib.F90     semi-implicit Poisson        (lapl_lib.F90:57-636)
  *   main.F90         accept/reject, rotation      (main.F90:65-121)
  * All arrays use the reference's layout: (bx+2, by+2[, level]) FP64, column
@@ -41,6 +42,7 @@ typedef struct ocfg {
   double rtol;               /* Poisson relative residual */
   int32_t maxit;
   int32_t nthreads;          /* OpenMP threads (1 = serial) */
+  int32_t rwall;             /* dielectric top wall with surface charge (props.F90:47) */
 } ocfg;
 
 typedef struct osim osim;
@@ -65,6 +67,10 @@ void  o_set_scalars(osim* s, const double* in11);
 
 void  o_rhs_sweep(osim* s, int stage);          /* ptclEqn over interior */
 void  o_circ_step(osim* s, int stage);
+void  o_sfc_step(osim* s, int stage);           /* wall charge (rwall) */
+/* wall surface charge sig(bx+2), which: 0 sig_pl, 1 sig_mi, 2 sig_or */
+int   o_get_sig(const osim* s, int which, double* out);
+int   o_set_sig(osim* s, int which, const double* in);
 void  o_rk_all(osim* s, int stage, double* nerr4);
 int   o_ptcl_step(osim* s);                     /* <0 fatal */
 int   o_lapl_assemble(osim* s, const double* ne, const double* ni, const double* nt,

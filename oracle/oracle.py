@@ -39,7 +39,7 @@ class OCfg(C.Structure):
         ("res_ohm", C.c_double), ("t_rise", C.c_double), ("t_width", C.c_double),
         ("t_fall", C.c_double), ("circuit", C.c_int32), ("solver", C.c_int32),
         ("dt0", C.c_double), ("rtol", C.c_double), ("maxit", C.c_int32),
-        ("nthreads", C.c_int32),
+        ("nthreads", C.c_int32), ("rwall", C.c_int32),
     ]
 
 
@@ -74,6 +74,9 @@ def lib():
         L.o_set_scalars.argtypes = [P, D]
         L.o_rhs_sweep.argtypes = [P, C.c_int]
         L.o_circ_step.argtypes = [P, C.c_int]
+        L.o_sfc_step.argtypes = [P, C.c_int]
+        L.o_get_sig.argtypes = [P, C.c_int, D]
+        L.o_set_sig.argtypes = [P, C.c_int, D]
         L.o_rk_all.argtypes = [P, C.c_int, D]
         L.o_ptcl_step.argtypes = [P]
         L.o_lapl_assemble.argtypes = [P, D, D, D, D, D]
@@ -220,6 +223,21 @@ class Oracle:
 
     def circ_step(self, stage: int):
         lib().o_circ_step(self.h, stage)
+
+    def sfc_step(self, stage: int):
+        lib().o_sfc_step(self.h, stage)
+
+    SIG = {"sig_pl": 0, "sig_mi": 1, "sig_or": 2}
+
+    def get_sig(self, which: str = "sig_pl") -> np.ndarray:
+        """Wall surface charge at x-nodes 1..bx+2 (ghosts included)."""
+        out = np.zeros(self.bx + 2)
+        lib().o_get_sig(self.h, self.SIG[which], _dp(out))
+        return out
+
+    def set_sig(self, which: str, arr) -> None:
+        a = np.ascontiguousarray(arr, dtype=np.float64).reshape(self.bx + 2)
+        lib().o_set_sig(self.h, self.SIG[which], _dp(a))
 
     def rk_all(self, stage: int) -> np.ndarray:
         e = np.zeros(4)
