@@ -148,10 +148,17 @@ struct fs_handle {
   std::vector<int32_t> type_x, type_y;
   // device
   hipStream_t st = nullptr;
+  // a second stream for the multigrid set-up kernel, which runs beside the
+  // start kernel and the first tile-level kernels of a solve (fork and join
+  // events; inside a capture the pair makes a parallel branch of the graph)
+  hipStream_t st2 = nullptr;
+  hipEvent_t fork_ev = nullptr, join_ev = nullptr;
+  bool join_pending = false;  // the next coarse-cycle launch waits for join_ev
   double* d_lev = nullptr;  // 4 buffers x 4 species x PL
   double* d_k = nullptr;    // k1,k3,k4 x 4 species x PL
   double* d_kdbg = nullptr; // 5 stages x 4 species x PL (debug)
   double* d_ph = nullptr;
+  double* d_sig = nullptr;  // dielectric wall: sig_pl, sig_mi, sig_or, ks(1..5), bx+2 each
   double* d_geo = nullptr;  // dx,dlx,dy,dly,r and 1/dx,1/dlx,1/dy,1/dly
   int8_t* d_types = nullptr;  // ltype, rtype, gl, gr (by each)
   double* d_dr = nullptr;     // dra, drc
@@ -191,6 +198,7 @@ struct fs_handle {
   hipGraphExec_t mg_solve[2][kSolveGraphs] = {};
   bool warm_ok = false;               // x holds the last converged MG-PCG correction
   bool moved = false;                 // an accepted transport step since the last solve
+  bool in_ens = false;                // a member of an ensemble (its tables are the ensemble's)
   bool solve_warm = false;            // the MG-PCG solve in flight started from the last correction
   int mg_it = 0, mg_chunk = 0;        // next MG-PCG iteration / chunk of a solve in progress
   hipGraphExec_t mg_chain[2][kChainLog] = {};  // 2^m MG-PCG iterations from parity p
@@ -653,6 +661,10 @@ int launch_vcycle(fs_handle* h, int it, fsl::Net* harvest = nullptr) {
       if (!h->h_sc->done && fsl::net_store_sample(harvest, h->mg[hl], h->st) < 0) return FS_E_HIP;
     }
   }
+  if (h->join_pending) {  // the set-up kernel forked by mg_prologue
+    h->join_pending = false;
+    HIP_OK(hipStreamWaitEvent(h->st, h->join_ev, 0));
+  }
   launch_mg_coarse_cycle(cycle_args(h, it), h->st);
   for (size_t l = f; l-- > 0;) launch_mg_up(mg_up_args(h, l, it), h->st);
   return 0;
@@ -736,7 +748,16 @@ int capture_iteration(fs_handle* h, hipGraphExec_t* out, int (*launch)(fs_handle
 int mg_prologue(fs_handle* h, bool warm) {
   for (size_t l = lapl_coarse_levels(h); l < h->mg_first; l++)
     launch_mg_coarsen(mg_args(h, l), h->st);
-  launch_mg_coarse_setup(cycle_args(h, -1), h->st);
+  // The one-workgroup set-up (coarser operators, bottom inverse, packed rows)
+  // reads only the first one-workgroup level's operator and feeds only the
+  // coarse-cycle kernel: it runs on the second stream while the start kernel
+  // and the tile levels' down kernels run on the first (joined in
+  // launch_vcycle before the coarse cycle).
+  HIP_OK(hipEventRecord(h->fork_ev, h->st));
+  HIP_OK(hipStreamWaitEvent(h->st2, h->fork_ev, 0));
+  launch_mg_coarse_setup(cycle_args(h, -1), h->st2);
+  HIP_OK(hipEventRecord(h->join_ev, h->st2));
+  h->join_pending = true;
   MgPcgArgs ia = mgpcg_args(h, 1);
   ia.part_out = h->part(0);
   ia.warm = warm;
@@ -1604,7 +1625,7 @@ int fs_create(const fs_grid_desc* g, const fs_params* p, fs_handle** out) {
     for (int i = 0; i < bx; i++) {
       size_t n = (size_t)j * bx + i;
       if (j == 0) h->type_y[n] = -1;
-      if (j == by - 1) h->type_y[n] = 1;
+      if (j == by - 1) h->type_y[n] = g->rwall ? 3 : 1;  // props.F90:192-198
       if (i == 0) h->type_x[n] = types[j];
       if (i == bx - 1) h->type_x[n] = types[by + j];
     }
@@ -1632,6 +1653,9 @@ int fs_create(const fs_grid_desc* g, const fs_params* p, fs_handle** out) {
   };
   size_t PL = h->PL;
   if (bad(hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking)) ||
+      bad(hipStreamCreateWithFlags(&h->st2, hipStreamNonBlocking)) ||
+      bad(hipEventCreateWithFlags(&h->fork_ev, hipEventDisableTiming)) ||
+      bad(hipEventCreateWithFlags(&h->join_ev, hipEventDisableTiming)) ||
       bad(hipMalloc(&h->d_lev, 16 * PL * sizeof(double))) ||
       bad(hipMalloc(&h->d_k, 12 * PL * sizeof(double))) ||
       bad(hipMalloc(&h->d_ph, PL * sizeof(double))) ||
@@ -1646,11 +1670,14 @@ int fs_create(const fs_grid_desc* g, const fs_params* p, fs_handle** out) {
                         hipHostMallocMapped | hipHostMallocCoherent)) ||
       bad(hipHostGetDevicePointer((void**)&h->d_pub, h->h_pub, 0)) ||
       bad(hipMalloc(&h->d_pubcnt, 256)) || bad(hipMemset(h->d_pubcnt, 0, 256)) ||
-      bad(hipMalloc(&h->d_pois, kPois * PL * sizeof(double)))) {
+      bad(hipMalloc(&h->d_pois, kPois * PL * sizeof(double))) ||
+      (g->rwall && bad(hipMalloc(&h->d_sig, 8 * ((size_t)bx + 2) * sizeof(double))))) {
     fs_destroy(h);
     return FS_E_HIP;
   }
   memset(h->h_pub, 0, sizeof(DevScalars));  // publication count 0, as *d_pubcnt
+  // sfc_init (sfc_lib.F90:82-90): the wall starts uncharged
+  if (h->d_sig) (void)hipMemsetAsync(h->d_sig, 0, 8 * ((size_t)bx + 2) * sizeof(double), h->st);
   if (p->debug_keep_k) {
     if (bad(hipMalloc(&h->d_kdbg, 20 * PL * sizeof(double))) ||
         bad(hipMalloc(&h->d_dbg, 6 * (size_t)bx * by * sizeof(double)))) {
@@ -1689,6 +1716,8 @@ int fs_create(const fs_grid_desc* g, const fs_params* p, fs_handle** out) {
   h->G.by = by;
   h->G.P = h->P;
   h->G.cyl = g->cyl;
+  h->G.rwall = g->rwall ? 1 : 0;
+  h->G.sig = h->d_sig;
   // zero state; PCG ghosts: r = p = x = 0 and diagonal 1 so that neighbour
   // recomputation at the boundary yields p = 0
   (void)hipMemsetAsync(h->d_k, 0, 12 * PL * sizeof(double), h->st);
@@ -1710,6 +1739,8 @@ int fs_create(const fs_grid_desc* g, const fs_params* p, fs_handle** out) {
   s.Vd_pl = s.Vd_mi = s.Vd_or = s.Vsrc = p->vset;
   s.tol = p->lapl_rtol > 0 ? p->lapl_rtol : 1e-10;
   s.maxit = p->lapl_maxit > 0 ? p->lapl_maxit : 20000;
+  s.sig = h->d_sig;
+  s.sig_n = h->d_sig ? bx + 2 : 0;
   if (push_scalars(h)) {
     fs_destroy(h);
     return FS_E_HIP;
@@ -1730,6 +1761,7 @@ int fs_destroy(fs_handle* h) {
   (void)hipFree(h->d_k);
   (void)hipFree(h->d_kdbg);
   (void)hipFree(h->d_ph);
+  (void)hipFree(h->d_sig);
   (void)hipFree(h->d_geo);
   (void)hipFree(h->d_types);
   (void)hipFree(h->d_dr);
@@ -1751,6 +1783,12 @@ int fs_destroy(fs_handle* h) {
   (void)hipFree(h->d_phys);
   for (int e = 0; e < 8; e++)
     if (h->ev[e]) (void)hipEventDestroy(h->ev[e]);
+  if (h->fork_ev) (void)hipEventDestroy(h->fork_ev);
+  if (h->join_ev) (void)hipEventDestroy(h->join_ev);
+  if (h->st2) {
+    (void)hipStreamSynchronize(h->st2);
+    (void)hipStreamDestroy(h->st2);
+  }
   if (h->st) (void)hipStreamDestroy(h->st);
   delete h;
   return 0;
@@ -1788,16 +1826,30 @@ static double* field_dev(const fs_handle* h, int field, int level, bool for_writ
   return nullptr;
 }
 
+// the wall's surface charge (FS_SIG): level 1 sig_pl, 2 sig_mi, 3 sig_or, nx+2
+// entries (x nodes; the ghost ends are carried but never used)
+static double* sig_dev(const fs_handle* h, int level) {
+  if (!h->d_sig || level < 1 || level > 3) return nullptr;
+  const int which = level == 1 ? kSigPl : level == 2 ? kSigMi : kSigOr;
+  return h->d_sig + (size_t)which * (h->bx + 2);
+}
+
 int fs_set_field(fs_handle* h, int32_t field, int32_t level, const double* host, int32_t ld) {
   if (!h || !host) return fail(FS_E_ARG, "null argument%s");
   h->warm_ok = false;
+  if (field == FS_SIG) {
+    double* d = sig_dev(h, level);
+    if (!d) return fail(FS_E_ARG, "no dielectric wall, or bad level%s");
+    return upload(h, d, host, (size_t)h->bx + 2);
+  }
   if (ld < h->P) return fail(FS_E_ARG, "ld < nx+2%s");
   double* d = field_dev(h, field, level, true);
   if (!d) return fail(FS_E_ARG, "bad field/level%s");
   for (int j = 0; j < h->by + 2; j++)
     memcpy(&h->stage_buf[(size_t)j * h->P], host + (size_t)j * ld, h->P * sizeof(double));
-  if (field <= FS_NM && level == 1 && h->alias) {
-    // level 1 stops aliasing level 2: give it its own copy first
+  if (field <= FS_NM && (level == 1 || level == 2) && h->alias) {
+    // level 1 stops aliasing level 2 (a write to either would change both):
+    // give it its own copy first
     for (int s = 0; s < 4; s++)
       HIP_OK(hipMemcpyAsync(h->lev(h->bufL1(), s), h->lev(h->bufA(), s), h->PL * sizeof(double),
                             hipMemcpyDeviceToDevice, h->st));
@@ -1808,6 +1860,14 @@ int fs_set_field(fs_handle* h, int32_t field, int32_t level, const double* host,
 
 int fs_get_field(const fs_handle* h, int32_t field, int32_t level, double* host, int32_t ld) {
   if (!h || !host) return fail(FS_E_ARG, "null argument%s");
+  if (field == FS_SIG) {
+    const double* d = sig_dev(h, level);
+    if (!d) return fail(FS_E_ARG, "no dielectric wall, or bad level%s");
+    HIP_OK(hipMemcpyAsync(host, d, ((size_t)h->bx + 2) * sizeof(double), hipMemcpyDeviceToHost,
+                          h->st));
+    HIP_OK(hipStreamSynchronize(h->st));
+    return 0;
+  }
   if (ld < h->P) return fail(FS_E_ARG, "ld < nx+2%s");
   double* d = field_dev(h, field, level, false);
   if (!d) return fail(FS_E_ARG, "bad field/level%s");
@@ -1888,6 +1948,18 @@ int fs_get_scalars(const fs_handle* h, double* o) {
                   s.Vd_or, s.Ida, s.Idc,      s.Vsrc, s.ts};
   memcpy(o, v, sizeof(v));
   return 0;
+}
+
+int fs_circ_init(fs_handle* h, double vset) {  // circ_lib.F90:213-223
+  if (!h) return fail(FS_E_ARG, "null handle%s");
+  if (h->in_ens) return fail(FS_E_ARG, "circ_init on an ensemble member%s");
+  if (sync_scalars(h)) return FS_E_HIP;
+  h->p.vset = vset;
+  // the stage launches carry the circuit's amplitude in their argument tables
+  if (build_arg_tables(h)) return FS_E_HIP;
+  DevScalars& s = *h->h_sc;
+  s.Vsrc = s.Vd_pl = s.Vd_mi = s.Vd_or = vset;
+  return push_scalars(h);
 }
 
 int fs_set_scalars(fs_handle* h, const double* o) {
@@ -2232,6 +2304,8 @@ int fs_set_solver(fs_handle* h, int32_t solver) {
 
 int fs_ensemble_create(int32_t n, const fs_grid_desc* g, const fs_params* p, fs_ensemble** out) {
   if (n <= 0 || !g || !p || !out) return fail(FS_E_ARG, "bad argument%s");
+  for (int i = 0; i < n; i++)
+    if (g[i].rwall) return fail(FS_E_ARG, "ensemble members have no dielectric wall%s");
   fs_ensemble* E = new fs_ensemble();
   E->n = n;
   for (int i = 0; i < n; i++) {
@@ -2268,6 +2342,7 @@ int fs_ensemble_create(int32_t n, const fs_grid_desc* g, const fs_params* p, fs_
     E->own_h.push_back(h->h_sc);
     h->d_sc = E->d_sc + i;
     h->h_sc = E->h_sc + i;
+    h->in_ens = true;
     drop_graphs(h);
     if (build_arg_tables(h)) {
       ens_release(E);

@@ -35,6 +35,11 @@ struct DevScalars {
   int32_t iters, done, maxit, pad_i;
   int32_t status, accept, attempts, replay_idx;
   int32_t replay_n, pad_j[3];
+  // dielectric wall (rwall): the surface-charge arrays (sig_pl, sig_mi, sig_or
+  // of sig_n = bx+2 entries each, Geo::sig) for the accept, reject and
+  // restart copies; sig_n = 0 without the wall
+  double* sig;
+  int32_t sig_n, pad_s;
   int32_t itc[2], pad_k[2];   // MG-PCG iteration count by parity
   uint32_t ticket[8];
   uint32_t pub_seq;  // host-memory copy only: the publication count (k_publish)
@@ -56,7 +61,13 @@ struct Geo {
   const double* rdy;
   const double* rdly;
   int32_t bx, by, P, cyl;
+  // dielectric top wall (props.F90:47, 192-198): the top row is type_y = 3.
+  // sig: 8 arrays of bx+2 (x-node index i+1 for cell i): sig_pl, sig_mi,
+  // sig_or (sfc_lib.F90:7), then the surface-charge slopes of stages 1-5
+  double* sig;
+  int32_t rwall, pad_g;
 };
+constexpr int kSigPl = 0, kSigMi = 1, kSigOr = 2, kSigK = 2;  // ks of stage s: kSigK + s
 
 // species order everywhere: 0 ni, 1 ne, 2 nt, 3 nm
 struct CircArgs {

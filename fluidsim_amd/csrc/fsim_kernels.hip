@@ -404,6 +404,18 @@ __device__ __forceinline__ void stage_body(const StageArgs& A, const FsPhys& K, 
       double E = -(s_ph[hR] - s_ph[hL]) * rh;
       F = interior_face(K, E, h, rh, s_mue[hL], s_mue[hR], s_de[hL], s_de[hR], s_ni[hL],
                         s_ni[hR], s_ne[hL], s_ne[hR], s_nt[hL], s_nt[hR], s_nm[hL], s_nm[hR]);
+    } else if (G.rwall && fci < bx && fj == by) {
+      // the dielectric wall above the top row: E = -sig_mi, fluxes into a zero
+      // density, energy transport by get_mut / get_Dt (ptcl_lib.F90:292-294,
+      // 393-409)
+      const int hL = fy * kHX + fx + 1;
+      const double h = gld(G.dy, by), rh = gld(G.rdy, by);
+      const double E = -gld(G.sig + (size_t)kSigMi * (bx + 2), fci + 1);
+      const double xa = s_xa[hL];
+      F.i = sg_flux_ion(K, E, h, rh, s_ni[hL], 0.0);
+      F.e = sg_flux(K, E, h, rh, -1.0, s_mue[hL], s_de[hL], s_ne[hL], 0.0);
+      F.t = sg_flux(K, E, h, rh, -1.0, mut_of_x(K, xa), dt_of_x(K, xa), s_nt[hL], 0.0);
+      F.m = K.Dm * s_nm[hL] / h;
     }
     s_f[0][f] = F.i;
     s_f[1][f] = F.e;
@@ -429,7 +441,7 @@ __device__ __forceinline__ void stage_body(const StageArgs& A, const FsPhys& K, 
 #pragma unroll
       for (int s = 0; s < 4; s++) dfy[s] = (s_f[s][fT] - s_f[s][fB]) * rdly;
     }
-    int typy = cj == 0 ? -1 : (cj == by - 1 ? 1 : 0);
+    int typy = cj == 0 ? -1 : (cj == by - 1 ? (G.rwall ? 3 : 1) : 0);
     double eyB, eyT;
     if (typy == -1) {
       eyB = 0.0;
@@ -437,6 +449,9 @@ __device__ __forceinline__ void stage_body(const StageArgs& A, const FsPhys& K, 
     } else if (typy == 1) {
       eyB = -(s_ph[hc] - s_ph[hc - kHX]) * gld(G.rdy, cj);
       eyT = 0.0;
+    } else if (typy == 3) {  // ptcl_lib.F90:292-294
+      eyB = -(s_ph[hc] - s_ph[hc - kHX]) * gld(G.rdy, cj);
+      eyT = -gld(G.sig + (size_t)kSigMi * (bx + 2), ci + 1);
     } else {
       eyB = -(s_ph[hc] - s_ph[hc - kHX]) * gld(G.rdy, cj);
       eyT = -(s_ph[hc + kHX] - s_ph[hc]) * gld(G.rdy, cj + 1);
@@ -444,6 +459,7 @@ __device__ __forceinline__ void stage_body(const StageArgs& A, const FsPhys& K, 
     double feY = (s_f[1][fT] + s_f[1][fB]) / 2.0;
     double eyA = (eyB + eyT) / 2.0;
     if (typy <= 0) mue2 = 5e-1 * (s_mue[hc] + s_mue[hc + kHX]);
+    else if (typy == 3) mue2 = s_mue[hc];  // get_mue(Te(2)) of the wall branch (:398)
 
     // rates and sources, ptcl_lib.F90:483-514 (Em = 0: its +0 term is dropped)
     double te = s_te[hc];
@@ -496,6 +512,29 @@ __device__ __forceinline__ void stage_body(const StageArgs& A, const FsPhys& K, 
         gst(A.wr[s], m, nw > nmin ? nw : nmin);
       }
       if (stage == 5) tmin = fs_rcp(mue2 * ne + K.mui * ni);
+      if (typy == 3) {
+        // sfc_step (sfc_lib.F90:20-58) for this column, from the stage
+        // densities and sig_mi before its update, with the reference's
+        // argument binding: ptcl_step passes (ni, ne, nt) to the dummies
+        // (ne, ni, nt) (ptcl_lib.F90:52), so Te = nt / ni, the "ion" flux
+        // carries ne and the "electron" flux ni; Merson's tableau as the
+        // circuit's
+        double* sg = G.sig;
+        const size_t W = (size_t)bx + 2, ic = (size_t)ci + 1;
+        const double E = -gld(sg + kSigMi * W, ic);
+        const double h = gld(G.dy, by), rh = gld(G.rdy, by);
+        const double tes = fs_div(s_nt[hc], ni);
+        const double xs = fit_x_stage(tes * K.ph0);
+        const double ks = sg_flux_ion(K, E, h, rh, ne, 0.0) -
+                          sg_flux(K, E, h, rh, -1.0, mue_of_x_folded(K, xs),
+                                  de_of_x_folded(K, xs), ni, 0.0);
+        double kq[6];
+        for (int q = 1; q < stage; q++) kq[q] = gld(sg + (kSigK + q) * W, ic);
+        kq[stage] = ks;
+        gst(sg + (kSigK + stage) * W, ic, ks);
+        const double v = circ_vd(stage, gld(sg + kSigOr * W, ic), dt, kq);
+        gst(sg + (stage == 5 ? kSigPl : kSigMi) * W, ic, v);
+      }
     }
   }
   if (!A.rhs_only && stage == 5) {
@@ -618,11 +657,39 @@ __device__ __forceinline__ void ctrl_body(const CtrlArgs& A) {
   }
 }
 
+// The wall charge's rotation (accept: sig_or <- sig_mi <- sig_pl, main.F90:
+// 100-101) and restore (reject, restart: sig_pl = sig_mi = sig_or, main.F90:
+// 117-118, ptcl_lib.F90:90-91); every thread of the launch takes a share.
+__device__ __forceinline__ void sig_accept(const DevScalars* sc) {
+  const int n = sc->sig_n;
+  double* s = sc->sig;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    s[kSigOr * n + i] = s[kSigMi * n + i];
+    s[kSigMi * n + i] = s[kSigPl * n + i];
+  }
+}
+__device__ __forceinline__ void sig_restore(const DevScalars* sc) {
+  const int n = sc->sig_n;
+  double* s = sc->sig;
+  for (int i = threadIdx.x; i < n; i += blockDim.x)
+    s[kSigPl * n + i] = s[kSigMi * n + i] = s[kSigOr * n + i];
+}
+
+// After the controller: a restart restores the wall charge too (every thread
+// of the launch copies a share, on the decision thread 0 made)
+__device__ __forceinline__ void ctrl_sig_restore(const CtrlArgs& A) {
+  __shared__ int s_restart;
+  if (threadIdx.x == 0) s_restart = A.sc->accept == 0;
+  __syncthreads();
+  if (s_restart) sig_restore(A.sc);
+}
+
 // The circuit update of stage 5 and the controller in one launch.
 __global__ void __launch_bounds__(kNT) k_circ_ctrl(CircArgs C, FsPhys K, CtrlArgs A) {
   circ_body<0>(C, K);
   __syncthreads();
   ctrl_body<0>(A);
+  ctrl_sig_restore(A);
 }
 
 __device__ __forceinline__ void begin_step_body(DevScalars* sc) {
@@ -661,15 +728,22 @@ __device__ __forceinline__ void accept_body(DevScalars* sc) {  // main.F90:83-10
   sc->Vd_mi = sc->Vd_pl;
 }
 
-__global__ void k_accept(DevScalars* sc) { accept_body(sc); }
+__global__ void __launch_bounds__(kNT) k_accept(DevScalars* sc) {
+  sig_accept(sc);
+  if (threadIdx.x == 0) accept_body(sc);
+}
 
 // The accept of a pass and the begin of the next, issued before the host has
 // read the pass's outcome: applied when the host accepts the pass on the
 // scalars this kernel reads (no error, the attempt accepted, its solve
 // converged in the chunk just run), and recorded in spec for the gated stage
 // kernel that follows.
-__global__ void k_accept_spec(DevScalars* sc) {
+__global__ void __launch_bounds__(kNT) k_accept_spec(DevScalars* sc) {
+  // (the flags read here are not written by this kernel)
   const int go = sc->status >= 0 && sc->accept == 1 && sc->done == 1;
+  if (go) sig_accept(sc);
+  __syncthreads();
+  if (threadIdx.x != 0) return;
   sc->spec = go;
   if (go) {
     accept_body(sc);
@@ -677,7 +751,9 @@ __global__ void k_accept_spec(DevScalars* sc) {
   }
 }
 
-__global__ void k_reject(DevScalars* sc) {  // main.F90:103-121
+__global__ void __launch_bounds__(kNT) k_reject(DevScalars* sc) {  // main.F90:103-121
+  sig_restore(sc);
+  if (threadIdx.x != 0) return;
   sc->dt = sc->dt / 2.0;
   sc->Vd_pl = sc->Vd_or;
   sc->Vd_mi = sc->Vd_or;
@@ -912,6 +988,16 @@ __device__ __forceinline__ void lapl_body(const LaplArgs& A, const FsPhys& K, in
       F = lapl_face(K, E, h, s_mue[hL], s_mue[hR], s_de[hL], s_de[hR], s_ni[hL], s_ni[hR],
                     s_ne[hL], s_ne[hR]);
       if (isnan(F.pi) || isnan(F.pe)) s_bad = 3;
+    } else if (G.rwall && fci < bx && fj == by) {
+      // the dielectric wall: E = -sig_pl, fluxes into a zero density with the
+      // top cell's own mobility (lapl_lib.F90:263-265, 324-335); its Jacobian
+      // terms are zero (lapl_lib.F90:548-561)
+      const int hL = fy * kHX + fx + 1;
+      const double h = G.dy[by];
+      const double E = -G.sig[(size_t)kSigPl * (bx + 2) + fci + 1];
+      double dfi, dfe;
+      sg_flux_dfdph(K, E, h, 1.0, K.mui, K.Di, s_ni[hL], 0.0, F.fi, dfi);
+      sg_flux_dfdph(K, E, h, -1.0, s_mue[hL], s_de[hL], s_ne[hL], 0.0, F.fe, dfe);
     }
     s_f[0][f] = F.fi;
     s_f[1][f] = F.fe;
@@ -924,7 +1010,7 @@ __device__ __forceinline__ void lapl_body(const LaplArgs& A, const FsPhys& K, in
     int fB = ty * kTX + tx, fT = fB + kTX;
     double dly = G.dly[cj], hB = G.dy[cj], hT = G.dy[cj + 1];
     double rj = G.r[cj + 1], rp = G.r[cj + 2], rm = G.r[cj];
-    int typy = cj == 0 ? -1 : (cj == by - 1 ? 1 : 0);
+    int typy = cj == 0 ? -1 : (cj == by - 1 ? (G.rwall ? 3 : 1) : 0);
     double eyB, eyT;
     if (typy == -1) {
       eyB = 0.0;
@@ -932,6 +1018,9 @@ __device__ __forceinline__ void lapl_body(const LaplArgs& A, const FsPhys& K, in
     } else if (typy == 1) {
       eyB = -(s_ph[hc] - s_ph[hc - kHX]) / hB;
       eyT = 0.0;
+    } else if (typy == 3) {  // Ey(2) = -sig (lapl_lib.F90:263-265)
+      eyB = -(s_ph[hc] - s_ph[hc - kHX]) / hB;
+      eyT = -G.sig[(size_t)kSigPl * (bx + 2) + ci + 1];
     } else {
       eyB = -(s_ph[hc] - s_ph[hc - kHX]) / hB;
       eyT = -(s_ph[hc + kHX] - s_ph[hc]) / hT;
@@ -2777,16 +2866,16 @@ void launch_circ_ctrl(const CircArgs& c, const FsPhys& K, const CtrlArgs& a, hip
   hipLaunchKernelGGL(k_circ_ctrl, dim3(1), dim3(kNT), 0, s, c, K, a);
 }
 void launch_accept(DevScalars* sc, hipStream_t s) {
-  hipLaunchKernelGGL(k_accept, dim3(1), dim3(1), 0, s, sc);
+  hipLaunchKernelGGL(k_accept, dim3(1), dim3(kNT), 0, s, sc);
 }
 void launch_reject(DevScalars* sc, hipStream_t s) {
-  hipLaunchKernelGGL(k_reject, dim3(1), dim3(1), 0, s, sc);
+  hipLaunchKernelGGL(k_reject, dim3(1), dim3(kNT), 0, s, sc);
 }
 void launch_begin_step(DevScalars* sc, hipStream_t s) {
   hipLaunchKernelGGL(k_begin_step, dim3(1), dim3(1), 0, s, sc);
 }
 void launch_accept_spec(DevScalars* sc, hipStream_t s) {
-  hipLaunchKernelGGL(k_accept_spec, dim3(1), dim3(1), 0, s, sc);
+  hipLaunchKernelGGL(k_accept_spec, dim3(1), dim3(kNT), 0, s, sc);
 }
 void launch_publish(const DevScalars* sc, DevScalars* pub, uint32_t* cnt, hipStream_t s) {
   hipLaunchKernelGGL(k_publish, dim3(1), dim3(64), 0, s, sc, pub, cnt);

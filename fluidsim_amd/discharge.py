@@ -95,6 +95,20 @@ class Discharge:
         a = np.ascontiguousarray(arr, dtype=np.float64).reshape(self.ny + 2, self.P)
         _L.check(_L.load().fs_set_field(self.h, _L.FIELDS[name], level, _dp(a), self.P))
 
+    def get_sig(self, which: str = "sig_pl") -> np.ndarray:
+        """The dielectric wall's surface charge (rwall), x nodes 1..nx+2."""
+        out = np.zeros(self.nx + 2)
+        _L.check(_L.load().fs_get_field(self.h, _L.FS_SIG, _L.SIG_LEVELS[which], _dp(out), 0))
+        return out
+
+    def set_sig(self, which: str, arr) -> None:
+        a = np.ascontiguousarray(arr, dtype=np.float64).reshape(self.nx + 2)
+        _L.check(_L.load().fs_set_field(self.h, _L.FS_SIG, _L.SIG_LEVELS[which], _dp(a), 0))
+
+    def circ_init(self, vset: float) -> None:
+        """circ_init(Vset): source amplitude and circuit voltages (nondimensional)."""
+        _L.check(_L.load().fs_circ_init(self.h, vset))
+
     def ptcl_init(self):
         _L.check(_L.load().fs_init_uniform(self.h))
 

@@ -11,9 +11,10 @@ import os
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libfsim_hip.so")
 
-FS_NI, FS_NE, FS_NT, FS_NM, FS_PH, FS_KI, FS_KE, FS_KT, FS_KM = range(9)
+FS_NI, FS_NE, FS_NT, FS_NM, FS_PH, FS_KI, FS_KE, FS_KT, FS_KM, FS_SIG = range(10)
 FIELDS = {"ni": FS_NI, "ne": FS_NE, "nt": FS_NT, "nm": FS_NM, "ph": FS_PH,
           "ki": FS_KI, "ke": FS_KE, "kt": FS_KT, "km": FS_KM}
+SIG_LEVELS = {"sig_pl": 1, "sig_mi": 2, "sig_or": 3}
 FS_REC_LEN = 10
 STATUS = {0: "ok", 1: "not converged", -1: "bad argument", -2: "negative density",
           -3: "NaN in stage slope", -4: "minimum timestep", -5: "NaN in dFdPh",
@@ -21,7 +22,7 @@ STATUS = {0: "ok", 1: "not converged", -1: "bad argument", -2: "negative density
 
 EXPORTS = [
     "fs_create", "fs_destroy", "fs_get_grid", "fs_set_field", "fs_get_field",
-    "fs_init_uniform", "fs_init_gaussian", "fs_get_scalars", "fs_set_scalars",
+    "fs_init_uniform", "fs_init_gaussian", "fs_circ_init", "fs_get_scalars", "fs_set_scalars",
     "fs_ptcl_step", "fs_lapl_solve", "fs_accept", "fs_reject", "fs_run", "fs_set_replay",
     "fs_log_count", "fs_get_log", "fs_clear_log", "fs_rhs_sweep", "fs_circ_step",
     "fs_lapl_assemble", "fs_constants", "fs_eval_fits", "fs_eval_flux", "fs_eval_dfdph",
@@ -38,7 +39,7 @@ class GridDesc(C.Structure):
     _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("grid_x", C.c_int32),
                 ("grid_y", C.c_int32), ("ratio_x", C.c_double), ("ratio_y", C.c_double),
                 ("l", C.c_double), ("w", C.c_double), ("ew", C.c_double), ("gap", C.c_double),
-                ("cyl", C.c_int32), ("pad_", C.c_int32)]
+                ("cyl", C.c_int32), ("rwall", C.c_int32)]
 
 
 class Params(C.Structure):
@@ -91,6 +92,7 @@ def load():
         "fs_get_field": [P, C.c_int32, C.c_int32, D, C.c_int32],
         "fs_init_uniform": [P],
         "fs_init_gaussian": [P, C.c_uint64, C.c_double, C.c_double, C.c_double],
+        "fs_circ_init": [P, C.c_double],
         "fs_get_scalars": [P, D],
         "fs_set_scalars": [P, D],
         "fs_ptcl_step": [P, C.POINTER(StepInfo)],

@@ -46,6 +46,8 @@ extern "C" {
 #define FS_KE 6
 #define FS_KT 7
 #define FS_KM 8
+#define FS_SIG 9 /* dielectric wall charge sig(nx+2): level 1 sig_pl, 2 sig_mi, 3 sig_or
+                    (sfc_lib.F90:7); handles with rwall only; ld is not used */
 
 /* Grid description: the arguments of g_init (props.F90:52) plus the build's
  * geometric spacing.  Lengths are nondimensional (units of x0 = 1e-4 m). */
@@ -55,7 +57,9 @@ typedef struct fs_grid_desc {
   double ratio_x, ratio_y; /* geometric spacing ratio (grid kind 2) */
   double l, w, ew, gap;    /* as g_init */
   int32_t cyl;             /* r-z axisymmetric when 1 (props.F90:47) */
-  int32_t pad_;
+  int32_t rwall;           /* 1: the top wall (y = w) is a dielectric that collects
+                              surface charge (props.F90:47, 192-198; sfc_lib.F90);
+                              not for ensemble members */
 } fs_grid_desc;
 
 /* Run parameters: compile-time constants of the reference made runtime
@@ -127,6 +131,10 @@ int fs_init_uniform(fs_handle* h);
 /* Gaussian blob + seeded multiplicative noise on ne/ni (all levels, ghosts
  * included): n = peak*exp(-r^2/(2 sigma^2))*(1+noise*u) + n_zero. */
 int fs_init_gaussian(fs_handle* h, uint64_t seed, double peak, double sigma, double noise);
+
+/* circ_init(Vset) (circ_lib.F90:213-223): the source amplitude Vmax and the
+ * voltages Vsrc, Vd_pl, Vd_mi, Vd_or all take Vset (nondimensional). */
+int fs_circ_init(fs_handle* h, double vset);
 
 /* Scalars: t, dt, err_prev, t_m, Vd_pl, Vd_mi, Vd_or, Ida, Idc, Vsrc, ts. */
 int fs_get_scalars(const fs_handle* h, double* out11);

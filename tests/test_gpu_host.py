@@ -48,6 +48,48 @@ def test_gaussian_initial_state_matches_oracle_bitwise(O):
     assert np.array_equal(d.get("nt", 2), o.get("nt", 2))
 
 
+def test_level_writes_keep_levels_apart():
+    """After an accept the device holds levels 1 and 2 in one buffer; a write
+    of either level gives level 1 its own copy first, so the other keeps its
+    values (a host that rotates its arrays, as the reference's main loop does,
+    writes them one at a time)."""
+    from fluidsim_amd import Discharge
+
+    d = Discharge(_cfg(20, 14))
+    d.init_gaussian(seed=5)
+    d.run(3)
+    for name in ("ne", "nt"):
+        l1, l3 = d.get(name, 1), d.get(name, 3)
+        x = l1 * 1.5
+        d.set(name, 2, x)
+        assert np.array_equal(d.get(name, 2), x)
+        assert np.array_equal(d.get(name, 1), l1)
+        assert np.array_equal(d.get(name, 3), l3)
+    d.run(2)
+    d.set("ni", 1, d.get("ni", 1) * 0.5)
+    assert not np.array_equal(d.get("ni", 1), d.get("ni", 2))
+
+
+def test_circ_init_sets_amplitude_and_voltages():
+    """circ_init(Vset) (circ_lib.F90:213-223) on a handle created with
+    vset = 0 equals creating it with vset."""
+    from fluidsim_amd import Discharge
+
+    c = _cfg(20, 14, volts=250.0)
+    a = Discharge(c)
+    b = Discharge(dict(c, vset=0.0))
+    b.circ_init(c["vset"])
+    sb = b.scalars()
+    for k in ("Vd_pl", "Vd_mi", "Vd_or", "Vsrc"):
+        assert sb[k] == c["vset"], k
+    for d in (a, b):
+        d.init_gaussian(seed=2)
+        d.run(4)
+    assert np.array_equal(a.get("ne", 1), b.get("ne", 1))
+    assert np.array_equal(a.get("ph"), b.get("ph"))
+    assert a.scalars()["Vd_pl"] == b.scalars()["Vd_pl"]
+
+
 def test_negative_density_is_fatal(O):
     from fluidsim_amd import Discharge, lib
 
