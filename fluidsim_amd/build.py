@@ -40,8 +40,8 @@ def build_lib(verbose: bool = False) -> str:
     return LIB
 
 
-FORTRAN_SOURCES = ["fsim_capi.F90", "props.F90", "circ_lib.F90", "ptcl_lib.F90",
-                   "lapl_lib.F90", "main.F90"]
+FORTRAN_SOURCES = ["fsim_capi.F90", "dev_mirror.F90", "props.F90", "circ_lib.F90",
+                   "sfc_lib.F90", "ptcl_lib.F90", "lapl_lib.F90", "main.F90"]
 
 
 def build_fortran(verbose: bool = False) -> str | None:

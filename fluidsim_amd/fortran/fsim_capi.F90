@@ -1,19 +1,20 @@
 ! fsim_capi.F90 -- ISO_C_BINDING declarations of libfsim_hip.so (include/fsim.h).
 !
 ! The derived types mirror the C structs field by field (bind(C) layout); the
-! interfaces are the C entry points.  The Fortran modules props, ptcl_lib,
-! lapl_lib and circ_lib call only these.
+! interfaces are the C entry points.  The Fortran modules props, dev_mirror,
+! circ_lib, sfc_lib, ptcl_lib and lapl_lib call only these.
 module fsim_capi
   use iso_c_binding
   implicit none
 
   integer(c_int), parameter :: FS_NI = 0, FS_NE = 1, FS_NT = 2, FS_NM = 3, FS_PH = 4
+  integer(c_int), parameter :: FS_SIG = 9
   integer(c_int), parameter :: FS_REC_LEN = 10
 
   type, bind(C) :: fs_grid_desc
     integer(c_int32_t) :: nx, ny, grid_x, grid_y
     real(c_double) :: ratio_x, ratio_y, l, w, ew, gap
-    integer(c_int32_t) :: cyl, pad_
+    integer(c_int32_t) :: cyl, rwall
   end type
 
   type, bind(C) :: fs_params
@@ -69,6 +70,11 @@ module fsim_capi
       type(c_ptr), value :: h
       integer(c_int64_t), value :: seed
       real(c_double), value :: peak, sigma, noise
+    end function
+    integer(c_int) function fs_circ_init(h, vset) bind(C, name="fs_circ_init")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), value :: vset
     end function
     integer(c_int) function fs_get_scalars(h, out) bind(C, name="fs_get_scalars")
       import :: c_int, c_ptr, c_double
