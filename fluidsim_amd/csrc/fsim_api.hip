@@ -148,12 +148,6 @@ struct fs_handle {
   std::vector<int32_t> type_x, type_y;
   // device
   hipStream_t st = nullptr;
-  // a second stream for the multigrid set-up kernel, which runs beside the
-  // start kernel and the first tile-level kernels of a solve (fork and join
-  // events; inside a capture the pair makes a parallel branch of the graph)
-  hipStream_t st2 = nullptr;
-  hipEvent_t fork_ev = nullptr, join_ev = nullptr;
-  bool join_pending = false;  // the next coarse-cycle launch waits for join_ev
   double* d_lev = nullptr;  // 4 buffers x 4 species x PL
   double* d_k = nullptr;    // k1,k3,k4 x 4 species x PL
   double* d_kdbg = nullptr; // 5 stages x 4 species x PL (debug)
@@ -661,10 +655,6 @@ int launch_vcycle(fs_handle* h, int it, fsl::Net* harvest = nullptr) {
       if (!h->h_sc->done && fsl::net_store_sample(harvest, h->mg[hl], h->st) < 0) return FS_E_HIP;
     }
   }
-  if (h->join_pending) {  // the set-up kernel forked by mg_prologue
-    h->join_pending = false;
-    HIP_OK(hipStreamWaitEvent(h->st, h->join_ev, 0));
-  }
   launch_mg_coarse_cycle(cycle_args(h, it), h->st);
   for (size_t l = f; l-- > 0;) launch_mg_up(mg_up_args(h, l, it), h->st);
   return 0;
@@ -748,16 +738,10 @@ int capture_iteration(fs_handle* h, hipGraphExec_t* out, int (*launch)(fs_handle
 int mg_prologue(fs_handle* h, bool warm) {
   for (size_t l = lapl_coarse_levels(h); l < h->mg_first; l++)
     launch_mg_coarsen(mg_args(h, l), h->st);
-  // The one-workgroup set-up (coarser operators, bottom inverse, packed rows)
-  // reads only the first one-workgroup level's operator and feeds only the
-  // coarse-cycle kernel: it runs on the second stream while the start kernel
-  // and the tile levels' down kernels run on the first (joined in
-  // launch_vcycle before the coarse cycle).
-  HIP_OK(hipEventRecord(h->fork_ev, h->st));
-  HIP_OK(hipStreamWaitEvent(h->st2, h->fork_ev, 0));
-  launch_mg_coarse_setup(cycle_args(h, -1), h->st2);
-  HIP_OK(hipEventRecord(h->join_ev, h->st2));
-  h->join_pending = true;
+  // (the one-workgroup set-up stays on the handle's stream: run beside the
+  // start kernel on a second stream, the graph's parallel branch and its join
+  // cost more than the 13 us it would hide, measured on the 512^2 pulse)
+  launch_mg_coarse_setup(cycle_args(h, -1), h->st);
   MgPcgArgs ia = mgpcg_args(h, 1);
   ia.part_out = h->part(0);
   ia.warm = warm;
@@ -1242,6 +1226,16 @@ struct fs_ensemble {
   size_t round_bytes = 0;
   EnsRound r{};
   std::vector<char> phase;        // 0 start of a pass, 1 ptcl attempt, 2 Poisson
+  // A round's fixed launch sequences as HIP graphs (when every member has
+  // use_graphs), captured on first use in a run: each run uploads its
+  // argument tables anew, so each run drops them.  Which members a launch
+  // covers is read from the round buffer on the device.
+  bool graphs = false;
+  hipGraphExec_t g_ptcl = nullptr;   // begin, five stages with their circuit updates, controller
+  hipGraphExec_t g_solve = nullptr;  // assembly, coarsening, set-up, start, cycles to iteration 8
+  hipGraphExec_t g_iter = nullptr;   // eight iterations (from an odd one)
+  hipGraphExec_t g_fin = nullptr;    // the finishing kernels
+  hipGraphExec_t g_end = nullptr;    // accept / reject of the members' passes
 };
 
 namespace {
@@ -1413,6 +1407,40 @@ void ens_iteration(fs_ensemble* E, int it, const uint8_t* mask) {
   ens_vcycle(E, it == 0 ? 3 : (it & 1) ? 1 : 2, mask);
 }
 
+void ens_drop_graphs(fs_ensemble* E) {
+  for (hipGraphExec_t* g : {&E->g_ptcl, &E->g_solve, &E->g_iter, &E->g_fin, &E->g_end}) {
+    if (*g) (void)hipGraphExecDestroy(*g);
+    *g = nullptr;
+  }
+}
+
+// Runs `launch` (kernel launches on E->st only) eagerly, or as the graph *g,
+// captured on its first use.
+template <class F>
+int ens_launch(fs_ensemble* E, hipGraphExec_t* g, F&& launch) {
+  if (!E->graphs) {
+    launch();
+    HIP_OK(hipGetLastError());
+    return 0;
+  }
+  if (!*g) {
+    hipGraph_t gr = nullptr;
+    HIP_OK(hipStreamBeginCapture(E->st, hipStreamCaptureModeThreadLocal));
+    launch();
+    const hipError_t le = hipGetLastError();
+    const hipError_t ce = hipStreamEndCapture(E->st, &gr);
+    if (le != hipSuccess || ce != hipSuccess) {
+      if (gr) (void)hipGraphDestroy(gr);
+      return fail(FS_E_HIP, "ensemble graph capture: %s", hipGetErrorString(le != hipSuccess ? le : ce));
+    }
+    const hipError_t ie = hipGraphInstantiate(g, gr, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(gr);
+    HIP_OK(ie);
+  }
+  HIP_OK(hipGraphLaunch(*g, E->st));
+  return 0;
+}
+
 int ens_round(fs_ensemble* E, std::vector<int>& left) {
   const int n = E->n;
   EnsRound& r = E->r;
@@ -1437,16 +1465,19 @@ int ens_round(fs_ensemble* E, std::vector<int>& left) {
     sel[i] = i * 6 + E->m[i]->rot * 2 + (E->m[i]->alias ? 1 : 0);
   }
   if (ens_upload_round(E)) return FS_E_HIP;
-  if (nbeg) launch_ens_scalars(E->d_sc, r.begin, n, 0, E->st);
   if (nptcl) {
-    for (int stg = 1; stg <= 5; stg++) {
-      launch_ens_stage(E->t_stage, E->t_phys, with_mask(E->m_stage, r.ptcl),
-                       EnsSel{r.sel, 5, stg - 1}, E->st);
-      launch_ens_circ(E->t_circ, E->t_phys, with_mask(E->m_one, r.ptcl), EnsSel{nullptr, 5, stg - 1},
-                      E->st);
-    }
-    launch_ens_ctrl(E->t_ctrl, with_mask(E->m_one, r.ptcl), EnsSel{nullptr, 1, 0}, E->st);
-    HIP_OK(hipGetLastError());
+    // (the begin kernel changes only the members marked in r.begin)
+    if (ens_launch(E, &E->g_ptcl, [&]() {
+          launch_ens_scalars(E->d_sc, r.begin, n, 0, E->st);
+          for (int stg = 1; stg <= 5; stg++) {
+            launch_ens_stage(E->t_stage, E->t_phys, with_mask(E->m_stage, r.ptcl),
+                             EnsSel{r.sel, 5, stg - 1}, E->st);
+            launch_ens_circ(E->t_circ, E->t_phys, with_mask(E->m_one, r.ptcl),
+                            EnsSel{nullptr, 5, stg - 1}, E->st);
+          }
+          launch_ens_ctrl(E->t_ctrl, with_mask(E->m_one, r.ptcl), EnsSel{nullptr, 1, 0}, E->st);
+        }))
+      return FS_E_HIP;
     if (ens_sync(E)) return FS_E_HIP;
     for (int i = 0; i < n; i++) {
       if (!ptcl[i]) continue;
@@ -1461,6 +1492,9 @@ int ens_round(fs_ensemble* E, std::vector<int>& left) {
         return FS_E_HIP;
       }
     }
+  }
+  else if (nbeg) {
+    launch_ens_scalars(E->d_sc, r.begin, n, 0, E->st);
   }
   int npois = 0;
   int32_t* sel2 = sel + n;
@@ -1477,18 +1511,27 @@ int ens_round(fs_ensemble* E, std::vector<int>& left) {
   std::vector<uint8_t> all_pois(pois, pois + n);
   for (int round = 0; round < 2; round++) {
     if (ens_upload_round(E)) return FS_E_HIP;
-    launch_ens_lapl(E->t_lapl, E->t_phys, with_mask(E->m_stage, r.pois), EnsSel{r.sel, 1, 0}, E->st);
-    for (int l = 0; l < E->maxF; l++)
-      launch_ens_mg_coarsen(E->t_coarsen, with_mask(E->m_coarsen[l], r.pois),
-                            EnsSel{nullptr, kMgMaxLevels, l}, E->st);
-    launch_ens_mg_coarse_setup(E->t_cc, with_mask(E->m_one, r.pois), EnsSel{nullptr, 4, 0}, E->st);
-    launch_ens_mgpcg_init(E->t_init, with_mask(E->m_pcg, r.pois), EnsSel{r.sel2, 1, 0}, E->st);
-    ens_vcycle(E, 0, r.pois);
-    ens_iteration(E, 0, r.pois);
-    int it = 1;
-    for (;;) {
-      for (int k = 0; k < 8; k++, it++) ens_iteration(E, it, r.pois);
-      HIP_OK(hipGetLastError());
+    // the set-up and iterations 0..8, then eight at a time (from odd ones)
+    if (ens_launch(E, &E->g_solve, [&]() {
+          launch_ens_lapl(E->t_lapl, E->t_phys, with_mask(E->m_stage, r.pois), EnsSel{r.sel, 1, 0},
+                          E->st);
+          for (int l = 0; l < E->maxF; l++)
+            launch_ens_mg_coarsen(E->t_coarsen, with_mask(E->m_coarsen[l], r.pois),
+                                  EnsSel{nullptr, kMgMaxLevels, l}, E->st);
+          launch_ens_mg_coarse_setup(E->t_cc, with_mask(E->m_one, r.pois), EnsSel{nullptr, 4, 0},
+                                     E->st);
+          launch_ens_mgpcg_init(E->t_init, with_mask(E->m_pcg, r.pois), EnsSel{r.sel2, 1, 0},
+                                E->st);
+          ens_vcycle(E, 0, r.pois);
+          for (int it = 0; it <= 8; it++) ens_iteration(E, it, r.pois);
+        }))
+      return FS_E_HIP;
+    for (bool first = true;; first = false) {
+      if (!first &&
+          ens_launch(E, &E->g_iter, [&]() {
+            for (int k = 0; k < 8; k++) ens_iteration(E, 9 + k, r.pois);
+          }))
+        return FS_E_HIP;
       if (ens_sync(E)) return FS_E_HIP;
       bool all = true;
       for (int i = 0; i < n; i++) {
@@ -1510,8 +1553,11 @@ int ens_round(fs_ensemble* E, std::vector<int>& left) {
   }
   memcpy(pois, all_pois.data(), n);
   if (ens_upload_round(E)) return FS_E_HIP;
-  launch_ens_pcg_finish(E->t_fin, with_mask(E->m_pcg, r.pois), with_mask(E->m_one, r.pois),
-                        EnsSel{nullptr, 1, 0}, E->st);
+  if (ens_launch(E, &E->g_fin, [&]() {
+        launch_ens_pcg_finish(E->t_fin, with_mask(E->m_pcg, r.pois), with_mask(E->m_one, r.pois),
+                              EnsSel{nullptr, 1, 0}, E->st);
+      }))
+    return FS_E_HIP;
   if (ens_sync(E)) return FS_E_HIP;
   for (int i = 0; i < n; i++) {
     if (!pois[i]) continue;
@@ -1525,8 +1571,11 @@ int ens_round(fs_ensemble* E, std::vector<int>& left) {
     rej[i] = lerr != 0;
   }
   if (ens_upload_round(E)) return FS_E_HIP;
-  launch_ens_scalars(E->d_sc, r.acc, n, 1, E->st);
-  launch_ens_scalars(E->d_sc, r.rej, n, 2, E->st);
+  if (ens_launch(E, &E->g_end, [&]() {
+        launch_ens_scalars(E->d_sc, r.acc, n, 1, E->st);
+        launch_ens_scalars(E->d_sc, r.rej, n, 2, E->st);
+      }))
+    return FS_E_HIP;
   for (int i = 0; i < n; i++) {
     if (!pois[i]) continue;
     fs_handle* h = E->m[i];
@@ -1544,6 +1593,7 @@ int ens_round(fs_ensemble* E, std::vector<int>& left) {
 }
 
 void ens_release(fs_ensemble* E) {
+  ens_drop_graphs(E);
   for (size_t i = 0; i < E->m.size(); i++) {
     fs_handle* h = E->m[i];
     if (i < E->own_d.size()) {
@@ -1653,9 +1703,6 @@ int fs_create(const fs_grid_desc* g, const fs_params* p, fs_handle** out) {
   };
   size_t PL = h->PL;
   if (bad(hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking)) ||
-      bad(hipStreamCreateWithFlags(&h->st2, hipStreamNonBlocking)) ||
-      bad(hipEventCreateWithFlags(&h->fork_ev, hipEventDisableTiming)) ||
-      bad(hipEventCreateWithFlags(&h->join_ev, hipEventDisableTiming)) ||
       bad(hipMalloc(&h->d_lev, 16 * PL * sizeof(double))) ||
       bad(hipMalloc(&h->d_k, 12 * PL * sizeof(double))) ||
       bad(hipMalloc(&h->d_ph, PL * sizeof(double))) ||
@@ -1783,12 +1830,6 @@ int fs_destroy(fs_handle* h) {
   (void)hipFree(h->d_phys);
   for (int e = 0; e < 8; e++)
     if (h->ev[e]) (void)hipEventDestroy(h->ev[e]);
-  if (h->fork_ev) (void)hipEventDestroy(h->fork_ev);
-  if (h->join_ev) (void)hipEventDestroy(h->join_ev);
-  if (h->st2) {
-    (void)hipStreamSynchronize(h->st2);
-    (void)hipStreamDestroy(h->st2);
-  }
   if (h->st) (void)hipStreamDestroy(h->st);
   delete h;
   return 0;
@@ -2387,7 +2428,10 @@ int fs_ensemble_run(fs_ensemble* E, int32_t passes) {
   if (!E || passes < 0) return fail(FS_E_ARG, "bad argument%s");
   const int n = E->n;
   for (fs_handle* h : E->m) HIP_OK(hipStreamSynchronize(h->st));
+  ens_drop_graphs(E);  // they hold the previous run's tables
   if (ens_tables(E)) return FS_E_HIP;
+  E->graphs = true;
+  for (fs_handle* h : E->m) E->graphs = E->graphs && h->p.use_graphs;
   // members work on the ensemble's stream for the run (restart copies)
   E->own_st.clear();
   for (fs_handle* h : E->m) {
