@@ -19,7 +19,7 @@ DT0 = float(np.float32(5.0e-6))  # main.F90:37
 def base(nx, ny, *, l_m=1e-2, w_m=1e-2, p_torr=1.0, volts=200.0, res_ohm=1e5, rf=0,
          circuit=0, grid_x=2, ratio_x=1.05, grid_y=0, gam=0.1, cyl=0, rtol=1e-10,
          maxit=20000, t_rise=0.0, t_width=0.0, t_fall=0.0, chunk=32, debug=0,
-         solver=1, graphs=1, warm=0, rwall=0) -> dict:
+         solver=1, graphs=1, warm=0, rwall=0, stage_tile=0) -> dict:
     """A parallel-plate argon discharge: electrodes at x = 0 and x = l over the
     full width w (ew = w), a Neumann wall at y = 0 and at y = w either a
     Neumann wall or, with rwall = 1, a dielectric wall that collects surface
@@ -32,6 +32,7 @@ def base(nx, ny, *, l_m=1e-2, w_m=1e-2, p_torr=1.0, volts=200.0, res_ohm=1e5, rf
         vset=volts / PH0, res_ohm=res_ohm, t_rise=t_rise, t_width=t_width, t_fall=t_fall,
         circuit=circuit, dt0=DT0, lapl_rtol=rtol, lapl_maxit=maxit, lapl_chunk=chunk,
         debug_keep_k=debug, lapl_solver=solver, use_graphs=graphs, lapl_warm=warm,
+        stage_tile=stage_tile,
     )
 
 

@@ -9,8 +9,10 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <map>
 #include <mutex>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "fsim.h"
@@ -171,7 +173,9 @@ struct fs_handle {
   int nreplay = 0;
   int rot = 0;        // level rotation: A = buf[rot], L1 = buf[rot+1], L3 = buf[rot+2]
   bool alias = true;  // level 1 currently equals level 2 (after accept/reject/init)
-  int nbx = 0, nby = 0, nblk = 0;   // stage / assembly tiles (64x8)
+  int nbx = 0, nby = 0, nblk = 0;   // assembly tiles (64x8)
+  int sty = 8;                      // stage tile height (4, 8 or 16: fs_params.stage_tile)
+  size_t npart = 0;                 // entries of one partial-sum buffer
   int pnby = 0, pnblk = 0;          // PCG tiles (64x8)
   std::vector<MgLevel> mg;          // multigrid hierarchy; level 0 aliases the PCG planes
   double* d_mg = nullptr;
@@ -216,7 +220,7 @@ struct fs_handle {
   int bufB() const { return 3; }
   double* kk(int which, int sp) const { return d_k + ((size_t)which * 4 + sp) * PL; }
   double* pois(int which) const { return d_pois + (size_t)which * PL; }
-  double* part(int which) const { return d_part + (size_t)which * 8 * ((size_t)nblk + 64); }
+  double* part(int which) const { return d_part + (size_t)which * 8 * npart; }
 };
 
 constexpr int kPois = 19;
@@ -1079,7 +1083,7 @@ CtrlArgs ctrl_args(fs_handle* h) {
   CtrlArgs ca;
   memset(&ca, 0, sizeof(ca));
   ca.part = h->part(0);
-  ca.nblocks = h->nblk;
+  ca.nblocks = h->nbx * ((h->by + kStageRows - 1) / kStageRows);  // the stage kernel's row groups
   ca.sc = h->d_sc;
   ca.replay = h->d_replay;
   return ca;
@@ -1104,7 +1108,7 @@ int ptcl_step_impl(fs_handle* h, fs_step_info* info, bool ahead = false,
     time_mark(h, 0);
     for (int stage = 1; stage <= 5; stage++)  // (stages 2-5 also do circuit updates 1-4)
       if (!(pre && attempts == 0 && stage == 1))
-        launch_stage(stab(h, stage, 0), h->d_phys, h->nbx * h->nby, h->st);
+        launch_stage(stab(h, stage, 0), h->d_phys, h->nbx, h->by, h->sty, h->st);
     time_mark(h, 1);
     launch_circ_ctrl(circ_args(h, 5), h->K, ctrl_args(h), h->st);
     HIP_OK(hipGetLastError());
@@ -1118,7 +1122,7 @@ int ptcl_step_impl(fs_handle* h, fs_step_info* info, bool ahead = false,
     if (spec_next) {
       enqueue_publish(h);
       launch_accept_spec(h->d_sc, h->st);
-      launch_stage(stab_at(h, (h->rot + 1) % 3, true, 1, 0), h->d_phys, h->nbx * h->nby, h->st,
+      launch_stage(stab_at(h, (h->rot + 1) % 3, true, 1, 0), h->d_phys, h->nbx, h->by, h->sty, h->st,
                    1);
       HIP_OK(hipGetLastError());
       if (wait_published(h)) return FS_E_HIP;
@@ -1197,6 +1201,83 @@ struct EnsRound {  // uploaded once per round
   uint8_t *ptcl, *pois, *begin, *acc, *rej;
 };
 
+// ---------------------------------------------------------------------------
+// The stage kernel's tile height (fs_params.stage_tile).  With 0 the handle
+// times the three heights on its own grid and keeps the fastest; the choice is
+// cached per grid shape for the process.  The stage kernel's results do not
+// depend on the height (its error sums are reduced per row segment of kTX
+// cells, the same tree for every height), only its speed does.
+constexpr int kStageTiles[3] = {4, 8, 16};
+std::mutex g_tile_mu;
+std::map<std::tuple<int, int, int, int>, int> g_tile_cache;  // (nx, ny, cyl, rwall) -> height
+// set while an ensemble creates its members: the ensemble times its own
+// batched launch instead
+thread_local bool t_tile_deferred = false;
+
+bool valid_stage_tile(int ty) { return ty == 4 || ty == 8 || ty == 16; }
+
+// Average time of `reps` launches made by `launch` on stream st (after two
+// untimed ones).
+template <class F>
+int time_launches(hipStream_t st, int reps, F&& launch, float* ms) {
+  hipEvent_t a = nullptr, b = nullptr;
+  HIP_OK(hipEventCreate(&a));
+  HIP_OK(hipEventCreate(&b));
+  launch();
+  launch();
+  (void)hipEventRecord(a, st);
+  for (int k = 0; k < reps; k++) launch();
+  (void)hipEventRecord(b, st);
+  const hipError_t e = hipEventSynchronize(b);
+  float t = 0.0f;
+  if (e == hipSuccess) (void)hipEventElapsedTime(&t, a, b);
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  HIP_OK(e);
+  HIP_OK(hipGetLastError());
+  *ms = t / reps;
+  return 0;
+}
+
+int choose_stage_tile(fs_handle* h) {
+  const int req = h->p.stage_tile;
+  if (req != 0 && !valid_stage_tile(req)) return fail(FS_E_ARG, "stage_tile must be 0, 4, 8 or 16%s");
+  if (req != 0 || t_tile_deferred) {
+    h->sty = req != 0 ? req : 8;
+    return 0;
+  }
+  const auto key = std::make_tuple(h->bx, h->by, h->g.cyl, h->g.rwall);
+  {
+    std::lock_guard<std::mutex> lk(g_tile_mu);
+    auto it = g_tile_cache.find(key);
+    if (it != g_tile_cache.end()) {
+      h->sty = it->second;
+      return 0;
+    }
+  }
+  // right-hand-side launches of stage 1 on the initial state: they write
+  // only the electrode-face currents, cleared afterwards
+  float best = 0.0f;
+  int bt = 8;
+  for (int ty : kStageTiles) {
+    float ms = 0.0f;
+    if (time_launches(h->st, 10, [&]() {
+          launch_stage(stab(h, 1, 2), h->d_phys, h->nbx, h->by, ty, h->st);
+        }, &ms))
+      return FS_E_HIP;
+    if (ty == kStageTiles[0] || ms < best) {
+      best = ms;
+      bt = ty;
+    }
+  }
+  HIP_OK(hipMemsetAsync(h->d_cur, 0, 4 * (size_t)h->by * sizeof(double), h->st));
+  HIP_OK(hipStreamSynchronize(h->st));
+  h->sty = bt;
+  std::lock_guard<std::mutex> lk(g_tile_mu);
+  g_tile_cache[key] = bt;
+  return 0;
+}
+
 }  // namespace
 
 struct fs_ensemble {
@@ -1221,6 +1302,8 @@ struct fs_ensemble {
   MgCycleArgs* t_cc = nullptr;    // [i*4 + v]
   PcgArgs* t_fin = nullptr;       // [i]
   EnsMapH m_stage, m_one, m_pcg, m_coarsen[kMgMaxLevels], m_down[kMgMaxLevels], m_up[kMgMaxLevels];
+  EnsMapH m_st;                   // the stage kernel's tiles (kTX x sty) of every member
+  int sty = 8;                    // the stage kernel's tile height
   char* d_round = nullptr;
   char* h_round = nullptr;
   size_t round_bytes = 0;
@@ -1348,7 +1431,10 @@ int ens_maps(fs_ensemble* E) {
     E->maxL = std::max(E->maxL, (int)h->mg.size());
     E->maxF = std::max(E->maxF, (int)h->mg_first);
   }
-  if (make_map(E->m_stage, stage) || make_map(E->m_one, one) || make_map(E->m_pcg, pcg))
+  std::vector<int> st(n);
+  for (int i = 0; i < n; i++) st[i] = stage_blocks(E->m[i]->nbx, E->m[i]->by, E->sty);
+  if (make_map(E->m_stage, stage) || make_map(E->m_one, one) || make_map(E->m_pcg, pcg) ||
+      make_map(E->m_st, st))
     return FS_E_HIP;
   for (int l = 0; l < kMgMaxLevels; l++) {
     std::vector<int> co(n, 0), dn(n, 0);
@@ -1364,6 +1450,57 @@ int ens_maps(fs_ensemble* E) {
       return FS_E_HIP;
   }
   return 0;
+}
+
+// The ensemble's stage tile height: the one requested, or the fastest of the
+// three for the batched launch over every member (right-hand-side launches of
+// stage 1 on the members' initial states; they write only the electrode-face
+// currents, cleared afterwards).
+int ens_choose_stage_tile(fs_ensemble* E, int req) {
+  const int n = E->n;
+  if (req != 0) {
+    E->sty = req;
+    return ens_maps(E);
+  }
+  std::vector<StageArgs> st(n);
+  std::vector<FsPhys> phys(n);
+  for (int i = 0; i < n; i++) {
+    st[i] = stage_args(E->m[i], 1, 1);
+    for (int s = 0; s < 4; s++) st[i].kdbg[s] = nullptr;
+    phys[i] = E->m[i]->K;
+  }
+  StageArgs* d_st = nullptr;
+  FsPhys* d_phys = nullptr;
+  uint8_t* d_on = nullptr;
+  int err = upload_table(&d_st, st) || upload_table(&d_phys, phys) ? FS_E_HIP : 0;
+  if (!err && (hipMalloc(&d_on, n) != hipSuccess || hipMemset(d_on, 1, n) != hipSuccess))
+    err = fail(FS_E_HIP, "ensemble tile choice: allocation%s");
+  float best = 0.0f;
+  int bt = 8;
+  for (int ty : kStageTiles) {
+    if (err) break;
+    E->sty = ty;
+    float ms = 0.0f;
+    if ((err = ens_maps(E))) break;
+    const EnsMap m = with_mask(E->m_st, d_on);
+    if ((err = time_launches(E->st, 5, [&]() {
+           launch_ens_stage(d_st, d_phys, m, EnsSel{nullptr, 1, 0}, ty, E->st);
+         }, &ms)))
+      break;
+    if (ty == kStageTiles[0] || ms < best) {
+      best = ms;
+      bt = ty;
+    }
+  }
+  if (d_st) (void)hipFree(d_st);
+  if (d_phys) (void)hipFree(d_phys);
+  if (d_on) (void)hipFree(d_on);
+  if (err) return err;
+  for (fs_handle* h : E->m)
+    HIP_OK(hipMemsetAsync(h->d_cur, 0, 4 * (size_t)h->by * sizeof(double), E->st));
+  HIP_OK(hipStreamSynchronize(E->st));
+  E->sty = bt;
+  return ens_maps(E);
 }
 
 int ens_sync(fs_ensemble* E) {
@@ -1470,8 +1607,8 @@ int ens_round(fs_ensemble* E, std::vector<int>& left) {
     if (ens_launch(E, &E->g_ptcl, [&]() {
           launch_ens_scalars(E->d_sc, r.begin, n, 0, E->st);
           for (int stg = 1; stg <= 5; stg++) {
-            launch_ens_stage(E->t_stage, E->t_phys, with_mask(E->m_stage, r.ptcl),
-                             EnsSel{r.sel, 5, stg - 1}, E->st);
+            launch_ens_stage(E->t_stage, E->t_phys, with_mask(E->m_st, r.ptcl),
+                             EnsSel{r.sel, 5, stg - 1}, E->sty, E->st);
             launch_ens_circ(E->t_circ, E->t_phys, with_mask(E->m_one, r.ptcl),
                             EnsSel{nullptr, 5, stg - 1}, E->st);
           }
@@ -1608,7 +1745,7 @@ void ens_release(fs_ensemble* E) {
   fr(E->t_phys); fr(E->t_stage); fr(E->t_circ); fr(E->t_ctrl); fr(E->t_lapl); fr(E->t_coarsen);
   fr(E->t_init); fr(E->t_down); fr(E->t_up); fr(E->t_cc);
   fr(E->t_fin); fr(E->d_sc); fr(E->d_round);
-  fr(E->m_stage.d); fr(E->m_one.d); fr(E->m_pcg.d);
+  fr(E->m_stage.d); fr(E->m_one.d); fr(E->m_pcg.d); fr(E->m_st.d);
   for (int l = 0; l < kMgMaxLevels; l++) {
     fr(E->m_coarsen[l].d);
     fr(E->m_down[l].d);
@@ -1695,6 +1832,9 @@ int fs_create(const fs_grid_desc* g, const fs_params* p, fs_handle** out) {
   h->nblk = h->nbx * h->nby;
   h->pnby = (by + kPcgTY - 1) / kPcgTY;
   h->pnblk = h->nbx * h->pnby;
+  // a partial-sum buffer holds a value set per block of any of the handle's
+  // launches, and the stage kernel's per-row-segment error sums
+  h->npart = (size_t)std::max(h->nblk, h->nbx * by) + 64;
 
   auto bad = [&](hipError_t e) {
     if (e == hipSuccess) return false;
@@ -1710,7 +1850,7 @@ int fs_create(const fs_grid_desc* g, const fs_params* p, fs_handle** out) {
       bad(hipMalloc(&h->d_types, 4 * (size_t)by)) ||
       bad(hipMalloc(&h->d_dr, 2 * (size_t)by * sizeof(double))) ||
       bad(hipMalloc(&h->d_cur, 4 * (size_t)by * sizeof(double))) ||
-      bad(hipMalloc(&h->d_part, 3 * ((size_t)h->nblk + 64) * 8 * sizeof(double))) ||
+      bad(hipMalloc(&h->d_part, 3 * h->npart * 8 * sizeof(double))) ||
       bad(hipMalloc(&h->d_sc, sizeof(DevScalars))) ||
       bad(hipHostMalloc(&h->h_sc, sizeof(DevScalars), hipHostMallocDefault)) ||
       bad(hipHostMalloc(&h->h_pub, sizeof(DevScalars),
@@ -1793,7 +1933,7 @@ int fs_create(const fs_grid_desc* g, const fs_params* p, fs_handle** out) {
     return FS_E_HIP;
   }
   h->stage_buf.resize(PL);
-  if (build_arg_tables(h) || fs_init_uniform(h) != 0) {
+  if (build_arg_tables(h) || fs_init_uniform(h) != 0 || choose_stage_tile(h)) {
     fs_destroy(h);
     return FS_E_HIP;
   }
@@ -1991,6 +2131,11 @@ int fs_get_scalars(const fs_handle* h, double* o) {
   return 0;
 }
 
+int fs_stage_tile(const fs_handle* h) {
+  if (!h) return fail(FS_E_ARG, "null handle%s");
+  return h->sty;
+}
+
 int fs_circ_init(fs_handle* h, double vset) {  // circ_lib.F90:213-223
   if (!h) return fail(FS_E_ARG, "null handle%s");
   if (h->in_ens) return fail(FS_E_ARG, "circ_init on an ensemble member%s");
@@ -2105,7 +2250,7 @@ int fs_clear_log(fs_handle* h) {
 int fs_rhs_sweep(fs_handle* h, int32_t stage) {
   if (!h || stage < 1 || stage > 5) return fail(FS_E_ARG, "bad argument%s");
   if (!h->d_kdbg) return fail(FS_E_ARG, "fs_rhs_sweep needs debug_keep_k%s");
-  launch_stage(stab(h, stage, 1), h->d_phys, h->nbx * h->nby, h->st);
+  launch_stage(stab(h, stage, 1), h->d_phys, h->nbx, h->by, h->sty, h->st);
   HIP_OK(hipGetLastError());
   if (sync_scalars(h)) return FS_E_HIP;
   return check_status(h);
@@ -2114,7 +2259,7 @@ int fs_rhs_sweep(fs_handle* h, int32_t stage) {
 int fs_circ_step(fs_handle* h, int32_t stage) {
   if (!h || stage < 1 || stage > 5) return fail(FS_E_ARG, "bad argument%s");
   // currents need the stage-kernel face terms of the current level 2
-  launch_stage(stab(h, stage, 2), h->d_phys, h->nbx * h->nby, h->st);
+  launch_stage(stab(h, stage, 2), h->d_phys, h->nbx, h->by, h->sty, h->st);
   CircArgs c = circ_args(h, stage);
   launch_circ(c, h->K, h->st);
   HIP_OK(hipGetLastError());
@@ -2230,7 +2375,7 @@ int fs_time_kernel(fs_handle* h, int32_t which, int32_t n, double* avg_ms) {
   if (push_scalars(h)) return FS_E_HIP;
   auto launch = [&](int k) {
     if (which >= 1 && which <= 5) {
-      launch_stage(stab(h, which, 0), h->d_phys, h->nbx * h->nby, h->st);
+      launch_stage(stab(h, which, 0), h->d_phys, h->nbx, h->by, h->sty, h->st);
     } else if (which == 10) {
       launch_assembly(h);
     } else if (which == 11) {
@@ -2345,6 +2490,8 @@ int fs_set_solver(fs_handle* h, int32_t solver) {
 
 int fs_ensemble_create(int32_t n, const fs_grid_desc* g, const fs_params* p, fs_ensemble** out) {
   if (n <= 0 || !g || !p || !out) return fail(FS_E_ARG, "bad argument%s");
+  if (p[0].stage_tile != 0 && !valid_stage_tile(p[0].stage_tile))
+    return fail(FS_E_ARG, "stage_tile must be 0, 4, 8 or 16%s");
   for (int i = 0; i < n; i++)
     if (g[i].rwall) return fail(FS_E_ARG, "ensemble members have no dielectric wall%s");
   fs_ensemble* E = new fs_ensemble();
@@ -2355,9 +2502,12 @@ int fs_ensemble_create(int32_t n, const fs_grid_desc* g, const fs_params* p, fs_
       return fail(FS_E_ARG, "ensemble members use the multigrid solver (lapl_solver 1)%s");
     }
     fs_handle* h = nullptr;
-    if (int e = fs_create(&g[i], &p[i], &h)) {
+    t_tile_deferred = true;
+    const int ce = fs_create(&g[i], &p[i], &h);
+    t_tile_deferred = false;
+    if (ce) {
       ens_release(E);
-      return e;
+      return ce;
     }
     E->m.push_back(h);
     if (h->mg.size() < 2) {
@@ -2393,7 +2543,8 @@ int fs_ensemble_create(int32_t n, const fs_grid_desc* g, const fs_params* p, fs_
   E->round_bytes = (size_t)n * 13;
   if (bad(hipMemcpy(E->d_sc, E->h_sc, n * sizeof(DevScalars), hipMemcpyHostToDevice)) ||
       bad(hipMalloc(&E->d_round, E->round_bytes)) ||
-      bad(hipHostMalloc(&E->h_round, E->round_bytes)) || ens_maps(E)) {
+      bad(hipHostMalloc(&E->h_round, E->round_bytes)) || ens_maps(E) ||
+      ens_choose_stage_tile(E, p[0].stage_tile)) {
     ens_release(E);
     return fail(FS_E_HIP, "ensemble set-up failed%s");
   }
@@ -2407,6 +2558,11 @@ int fs_ensemble_create(int32_t n, const fs_grid_desc* g, const fs_params* p, fs_
   E->phase.assign(n, 0);
   *out = E;
   return 0;
+}
+
+int fs_ensemble_stage_tile(const fs_ensemble* E) {
+  if (!E) return fail(FS_E_ARG, "null ensemble%s");
+  return E->sty;
 }
 
 int fs_ensemble_destroy(fs_ensemble* E) {

@@ -261,6 +261,7 @@ constexpr int kNT = kTX * kTY;
 // tiles (64x8, 512 threads) recompute fewer halo cells per cell.
 constexpr int kSTY = 8;
 constexpr int kSNT = kTX * kSTY;
+constexpr int kStageRows = 4;  // rows of the stage kernel's error-sum groups
 constexpr int kLaplCoarse = 3;  // log2(kSTY): coarse levels an assembly tile holds whole
 static_assert(kSTY == (1 << kLaplCoarse), "assembly tile height");
 static_assert(sizeof(LaplArgs::cl) == kLaplCoarse * sizeof(MgLevel), "LaplArgs::cl");
@@ -285,7 +286,7 @@ struct EnsSel {
   int32_t stride, off;
 };
 
-void launch_ens_stage(const StageArgs* t, const FsPhys* K, const EnsMap& m, const EnsSel& s,
+void launch_ens_stage(const StageArgs* t, const FsPhys* K, const EnsMap& m, const EnsSel& s, int ty,
                       hipStream_t st);
 void launch_ens_circ(const CircArgs* t, const FsPhys* K, const EnsMap& m, const EnsSel& s,
                      hipStream_t st);
@@ -311,7 +312,9 @@ void launch_ens_pcg_finish(const PcgArgs* t, const EnsMap& m, const EnsMap& one,
                            hipStream_t st);
 
 // a, K: device pointers (argument tables kept per handle)
-void launch_stage(const StageArgs* a, const FsPhys* K, int nblocks, hipStream_t s,
+// the stage kernel on nbx x ceil(by / ty) tiles of kTX x ty cells (ty = 4, 8, 16)
+int stage_blocks(int nbx, int by, int ty);
+void launch_stage(const StageArgs* a, const FsPhys* K, int nbx, int by, int ty, hipStream_t s,
                   int gated = 0);
 void launch_circ(const CircArgs& a, const FsPhys& K, hipStream_t s);
 void launch_circ_ctrl(const CircArgs& c, const FsPhys& K, const CtrlArgs& a, hipStream_t s);

@@ -35,6 +35,16 @@ static_assert(3 * (kSNT + kSNT / 4 + kSNT / 16 + kSNT / 64) <= 4 * kNF, "assembl
 constexpr int kNW = kNT / 64;
 constexpr int kSNW = kSNT / 64;
 
+// The stage kernel's tile: kTX x TY cells (TY = 4, 8 or 16, chosen per grid by
+// timing them), one thread per cell, a one-cell halo, its x and y faces.  Its
+// error sums are reduced per group of kStageRows rows (a divisor of every TY).
+template <int TY>
+struct StageTile {
+  static constexpr int kNT = kTX * TY, kNW = kNT / 64, kHY = TY + 2, kNH = kHX * kHY;
+  static constexpr int kNFX = (kTX + 1) * TY, kNFY = kTX * (TY + 1);
+  static constexpr int kNF = kNFX > kNFY ? kNFX : kNFY;
+};
+
 // Device pointers held in argument blocks in device memory are generic
 // pointers, and accesses through them compile to flat instructions, whose
 // completion every LDS wait also waits for.  These casts make them global
@@ -209,18 +219,19 @@ __device__ __forceinline__ Flux4 interior_face(const FsPhys& K, double E, double
 
 // ---------------------------------------------------------------------------
 // one instance per kernel: the LDS arrays declared here then belong to one kernel each
-template <int kInstance>
+template <int kInstance, int TY>
 __device__ __forceinline__ void stage_body(const StageArgs& A, const FsPhys& K, int bid, int nb) {
-  __shared__ double s_ni[kNH], s_ne[kNH], s_nt[kNH], s_nm[kNH], s_ph[kNH];
-  __shared__ double s_te[kNH], s_mue[kNH], s_de[kNH], s_xa[kNH];
-  __shared__ double s_f[4][kNF];
-  __shared__ double s_red[8 * kSNW];
+  using ST = StageTile<TY>;
+  __shared__ double s_ni[ST::kNH], s_ne[ST::kNH], s_nt[ST::kNH], s_nm[ST::kNH], s_ph[ST::kNH];
+  __shared__ double s_te[ST::kNH], s_mue[ST::kNH], s_de[ST::kNH], s_xa[ST::kNH];
+  __shared__ double s_f[4][ST::kNF];
+  __shared__ double s_red[8 * ST::kNW];
   __shared__ int s_bad;
 
   const Geo& G = A.G;
   const int bx = G.bx, by = G.by, P = G.P;
   const int t = xcd_tile(bid, nb);
-  const int i0 = (t % A.nbx) * kTX, j0 = (t / A.nbx) * kSTY;
+  const int i0 = (t % A.nbx) * kTX, j0 = (t / A.nbx) * TY;
   const double dt = A.sc->dt;
   if (threadIdx.x == 0) s_bad = 0;
   // The previous stage's circuit update (cc.stage > 0), k_circ's arithmetic in
@@ -241,7 +252,7 @@ __device__ __forceinline__ void stage_body(const StageArgs& A, const FsPhys& K, 
   __syncthreads();
 
   // ---- phase 1: stage densities + potential of tile and halo; cell properties
-  for (int h = threadIdx.x; h < kNH; h += kSNT) {
+  for (int h = threadIdx.x; h < ST::kNH; h += ST::kNT) {
     int hx = h % kHX, hy = h / kHX;
     int ci = i0 - 1 + hx, cj = j0 - 1 + hy;
     double vi = 0, ve = 0, vt = 0, vm = 0, vp = 0, te = 0, mue = 0, de = 0, xa = 0;
@@ -294,7 +305,7 @@ __device__ __forceinline__ void stage_body(const StageArgs& A, const FsPhys& K, 
   }
   if (cstage > 0) {
     __shared__ double s_vd;
-    block_sum<2, kSNW>(cv, s_red);
+    block_sum<2, ST::kNW>(cv, s_red);
     if (threadIdx.x == 0) {
       const CircArgs& C = A.cc;
       DevScalars* sc = A.sc;
@@ -320,10 +331,10 @@ __device__ __forceinline__ void stage_body(const StageArgs& A, const FsPhys& K, 
     }
     __syncthreads();
     const double vd = s_vd;
-    for (int q = threadIdx.x; q < 2 * kHY; q += kSNT) {  // circ_lib.F90:180-198
-      const int hy = q % kHY, cj = j0 - 1 + hy;
+    for (int q = threadIdx.x; q < 2 * ST::kHY; q += ST::kNT) {  // circ_lib.F90:180-198
+      const int hy = q % ST::kHY, cj = j0 - 1 + hy;
       if (cj < 0 || cj >= by) continue;
-      if (q < kHY) {
+      if (q < ST::kHY) {
         if (i0 == 0 && gld(A.cc.gl, cj)) s_ph[hy * kHX] = vd / 2.0;
       } else if (i0 + kTX >= bx && gld(A.cc.gr, cj)) {
         s_ph[hy * kHX + bx - i0 + 1] = vd / (-2.0);
@@ -333,7 +344,7 @@ __device__ __forceinline__ void stage_body(const StageArgs& A, const FsPhys& K, 
   __syncthreads();
 
   // ---- phase 2: x faces (left face of cells i0..i0+TX, rows j0..j0+TY-1)
-  for (int f = threadIdx.x; f < kNFX; f += kSNT) {
+  for (int f = threadIdx.x; f < ST::kNFX; f += ST::kNT) {
     int fx = f % (kTX + 1), fy = f / (kTX + 1);
     int fi = i0 + fx, fj = j0 + fy;
     Flux4 F = {0, 0, 0, 0};
@@ -394,7 +405,7 @@ __device__ __forceinline__ void stage_body(const StageArgs& A, const FsPhys& K, 
   __syncthreads();
 
   // ---- phase 2b: y faces (bottom face of rows j0..j0+TY, columns i0..i0+TX-1)
-  for (int f = threadIdx.x; f < kNFY; f += kSNT) {
+  for (int f = threadIdx.x; f < ST::kNFY; f += ST::kNT) {
     int fx = f % kTX, fy = f / kTX;
     int fci = i0 + fx, fj = j0 + fy;
     Flux4 F = {0, 0, 0, 0};
@@ -537,16 +548,36 @@ __device__ __forceinline__ void stage_body(const StageArgs& A, const FsPhys& K, 
       }
     }
   }
+  // The error sums and t_m of each group of kStageRows rows of kTX cells, by
+  // one reduction tree whatever the tile height (a shuffle tree within each
+  // row's wave, then the group's rows in order), so that the controller's
+  // inputs do not depend on the tile shape; at (row group, tile column).
   if (!A.rhs_only && stage == 5) {
-    block_sum<4, kSNW>(acc, s_red);
-    double tm = block_min<kSNW>(tmin, s_red);
-    if (threadIdx.x == 0) {
-      const size_t p = (size_t)bid * 8;
-      gst(A.part, p + 0, acc[0]);
-      gst(A.part, p + 1, acc[1]);
-      gst(A.part, p + 2, acc[2]);
-      gst(A.part, p + 3, acc[3]);
-      gst(A.part, p + 4, tm);
+#pragma unroll
+    for (int s = 0; s < 4; s++)
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) acc[s] += __shfl_down(acc[s], off, 64);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) tmin = fmin(tmin, __shfl_down(tmin, off, 64));
+    const int w = threadIdx.x >> 6;  // = ty: one wave per tile row
+    if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+      for (int s = 0; s < 4; s++) s_red[w * 5 + s] = acc[s];
+      s_red[w * 5 + 4] = tmin;
+    }
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0 && w % kStageRows == 0 && cj < by) {
+      double v[5];
+#pragma unroll
+      for (int k = 0; k < 5; k++) v[k] = s_red[w * 5 + k];
+      for (int r = 1; r < kStageRows; r++) {
+#pragma unroll
+        for (int s = 0; s < 4; s++) v[s] += s_red[(w + r) * 5 + s];
+        v[4] = fmin(v[4], s_red[(w + r) * 5 + 4]);
+      }
+      const size_t p = ((size_t)(cj / kStageRows) * A.nbx + (size_t)(i0 / kTX)) * 8;
+#pragma unroll
+      for (int k = 0; k < 5; k++) gst(A.part, p + k, v[k]);
     }
   }
   __syncthreads();
@@ -559,10 +590,11 @@ __device__ __forceinline__ void stage_body(const StageArgs& A, const FsPhys& K, 
 // argument block live in SGPRs and spills it to VGPR lanes).
 // gated: issued before the host has read the last pass's outcome; runs only
 // if k_accept_spec accepted that pass.
-__global__ void __launch_bounds__(kSNT, 4) k_stage(const StageArgs* __restrict__ A,
-                                               const FsPhys* __restrict__ K, int gated) {
+template <int TY>
+__global__ void __launch_bounds__(kTX * TY, 4) k_stage(const StageArgs* __restrict__ A,
+                                                   const FsPhys* __restrict__ K, int gated) {
   if (gated && A->sc->spec == 0) return;
-  stage_body<0>(*A, *K, blockIdx.x, gridDim.x);
+  stage_body<0, TY>(*A, *K, blockIdx.x, gridDim.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -2774,10 +2806,11 @@ __device__ __forceinline__ const T& ens_arg(const T* t, const EnsSel& s, int i) 
   return t[(size_t)(s.sel ? s.sel[i] : i) * s.stride + s.off];
 }
 
-__global__ void __launch_bounds__(kSNT, 4) kb_stage(const StageArgs* t, const FsPhys* K, EnsMap m,
-                                                EnsSel s) {
+template <int TY>
+__global__ void __launch_bounds__(kTX * TY, 4) kb_stage(const StageArgs* t, const FsPhys* K,
+                                                    EnsMap m, EnsSel s) {
   int i, bid, nb;
-  if (ens_slot(m, i, bid, nb)) stage_body<1>(ens_arg(t, s, i), K[i], bid, nb);
+  if (ens_slot(m, i, bid, nb)) stage_body<1, TY>(ens_arg(t, s, i), K[i], bid, nb);
 }
 __global__ void __launch_bounds__(kNT) kb_circ(const CircArgs* t, const FsPhys* K, EnsMap m,
                                                EnsSel s) {
@@ -2856,8 +2889,13 @@ static inline int nblocks_of(const Geo& G, int* nbx, int* nby) {
   return *nbx * *nby;
 }
 
-void launch_stage(const StageArgs* a, const FsPhys* K, int nblocks, hipStream_t s, int gated) {
-  hipLaunchKernelGGL(k_stage, dim3(nblocks), dim3(kSNT), 0, s, a, K, gated);
+int stage_blocks(int nbx, int by, int ty) { return nbx * ((by + ty - 1) / ty); }
+void launch_stage(const StageArgs* a, const FsPhys* K, int nbx, int by, int ty, hipStream_t s,
+                  int gated) {
+  const dim3 g(stage_blocks(nbx, by, ty));
+  if (ty == 4) hipLaunchKernelGGL(k_stage<4>, g, dim3(kTX * 4), 0, s, a, K, gated);
+  else if (ty == 16) hipLaunchKernelGGL(k_stage<16>, g, dim3(kTX * 16), 0, s, a, K, gated);
+  else hipLaunchKernelGGL(k_stage<8>, g, dim3(kTX * 8), 0, s, a, K, gated);
 }
 void launch_circ(const CircArgs& a, const FsPhys& K, hipStream_t s) {
   hipLaunchKernelGGL(k_circ, dim3(1), dim3(kNT), 0, s, a, K);
@@ -2939,9 +2977,13 @@ void launch_eval_flux(const FsPhys& K, const double* a, double* out, int n, int 
   hipLaunchKernelGGL(k_eval_flux, dim3((n + 255) / 256), dim3(256), 0, s, K, a, out, n, dfdph);
 }
 
-void launch_ens_stage(const StageArgs* t, const FsPhys* K, const EnsMap& m, const EnsSel& s,
+void launch_ens_stage(const StageArgs* t, const FsPhys* K, const EnsMap& m, const EnsSel& s, int ty,
                       hipStream_t st) {
-  if (m.nblocks) hipLaunchKernelGGL(kb_stage, dim3(m.nblocks), dim3(kSNT), 0, st, t, K, m, s);
+  if (!m.nblocks) return;
+  const dim3 g(m.nblocks);
+  if (ty == 4) hipLaunchKernelGGL(kb_stage<4>, g, dim3(kTX * 4), 0, st, t, K, m, s);
+  else if (ty == 16) hipLaunchKernelGGL(kb_stage<16>, g, dim3(kTX * 16), 0, st, t, K, m, s);
+  else hipLaunchKernelGGL(kb_stage<8>, g, dim3(kTX * 8), 0, st, t, K, m, s);
 }
 void launch_ens_circ(const CircArgs* t, const FsPhys* K, const EnsMap& m, const EnsSel& s,
                      hipStream_t st) {

@@ -166,6 +166,10 @@ class Discharge:
         r = np.ascontiguousarray(recs, dtype=np.float64).reshape(-1, _L.FS_REC_LEN)
         _L.check(_L.load().fs_set_replay(self.h, _dp(r), r.shape[0]))
 
+    def stage_tile(self) -> int:
+        """Tile height of the stage kernel in use (4, 8 or 16)."""
+        return _L.check(_L.load().fs_stage_tile(self.h))
+
     def log(self) -> np.ndarray:
         L = _L.load()
         n = L.fs_log_count(self.h)

@@ -38,6 +38,10 @@ class Ensemble:
     def __getitem__(self, i) -> Discharge:
         return self.members[i]
 
+    def stage_tile(self) -> int:
+        """Tile height of the batched stage launches (4, 8 or 16)."""
+        return _L.check(_L.load().fs_ensemble_stage_tile(self.h))
+
     def run(self, passes: int) -> None:
         _L.check(_L.load().fs_ensemble_run(self.h, int(passes)))
 

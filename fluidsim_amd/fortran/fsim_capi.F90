@@ -22,6 +22,7 @@ module fsim_capi
     integer(c_int32_t) :: rf, circuit
     real(c_double) :: t_rise, t_width, t_fall, dt0, lapl_rtol
     integer(c_int32_t) :: lapl_maxit, lapl_chunk, debug_keep_k, lapl_solver, use_graphs, lapl_warm
+    integer(c_int32_t) :: stage_tile
   end type
 
   type, bind(C) :: fs_step_info

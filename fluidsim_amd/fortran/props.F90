@@ -89,7 +89,7 @@ contains
     pr%t_rise = t_rise; pr%t_width = t_width; pr%t_fall = t_fall
     pr%dt0 = dt0; pr%lapl_rtol = lapl_rtol; pr%lapl_maxit = lapl_maxit
     pr%lapl_chunk = 32; pr%debug_keep_k = 0; pr%lapl_solver = lapl_solver
-    pr%use_graphs = 1; pr%lapl_warm = lapl_warm
+    pr%use_graphs = 1; pr%lapl_warm = lapl_warm; pr%stage_tile = 0
 
     rc = fs_create(gd, pr, g%h)
     call fs_check(rc, 'fs_create')

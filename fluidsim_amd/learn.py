@@ -137,3 +137,47 @@ def train_schedule(net: LearnedPreconditioner, phases=((1500, 16, 1e-3), (1500, 
     t0 = time.perf_counter()
     losses = [net.train(s, batch=b, lr=lr, seed=seed + k) for k, (s, b, lr) in enumerate(phases)]
     return np.concatenate(losses), time.perf_counter() - t0
+
+
+def kernels_workload(n: int = 512, seeds: int = 8, steps: int = 200, reps: int = 20) -> dict:
+    """A short workload of the learned preconditioner's own kernels that does
+    not advance any discharge: residuals harvested from the multigrid solves
+    of Gaussian initial states of `seeds` seeds, `steps` Adam steps of the
+    device training (forward, backward, weight gradients, Adam), then `reps`
+    network applications and learned flexible-CG iterations timed with HIP
+    events.  Returns the timings (ms) and the training summary."""
+    import time
+
+    from . import configs
+    from .discharge import Discharge
+
+    d = Discharge(configs.config1(n))
+    net = LearnedPreconditioner(max(64, 8 * seeds), 64, 64, seed=1)
+    t0 = time.perf_counter()
+    for s in range(seeds):
+        d.init_gaussian(seed=s)
+        net.harvest(d)
+    t1 = time.perf_counter()
+    losses = net.train(steps, batch=8, lr=1e-3, seed=2)
+    t2 = time.perf_counter()
+    net.attach(d)
+    d.set_solver(2)
+    out = {"n": n, "samples": net.samples, "harvest_s": t1 - t0, "train_s": t2 - t1,
+           "train_steps": steps, "loss_first": float(losses[0]), "loss_last": float(losses[-1]),
+           "network_apply_ms": d.time_kernel(15, reps), "learned_iteration_ms": d.time_kernel(16, reps)}
+    d.set_solver(1)
+    d.close()
+    net.close()
+    return out
+
+
+if __name__ == "__main__":
+    import argparse
+    import json
+
+    ap = argparse.ArgumentParser(description=kernels_workload.__doc__)
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--seeds", type=int, default=8)
+    ap.add_argument("--steps", type=int, default=200)
+    a = ap.parse_args()
+    print(json.dumps(kernels_workload(a.n, a.seeds, a.steps)))

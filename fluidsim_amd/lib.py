@@ -22,13 +22,15 @@ STATUS = {0: "ok", 1: "not converged", -1: "bad argument", -2: "negative density
 
 EXPORTS = [
     "fs_create", "fs_destroy", "fs_get_grid", "fs_set_field", "fs_get_field",
-    "fs_init_uniform", "fs_init_gaussian", "fs_circ_init", "fs_get_scalars", "fs_set_scalars",
+    "fs_init_uniform", "fs_init_gaussian", "fs_circ_init", "fs_stage_tile", "fs_get_scalars",
+    "fs_set_scalars",
     "fs_ptcl_step", "fs_lapl_solve", "fs_accept", "fs_reject", "fs_run", "fs_set_replay",
     "fs_log_count", "fs_get_log", "fs_clear_log", "fs_rhs_sweep", "fs_circ_step",
     "fs_lapl_assemble", "fs_constants", "fs_eval_fits", "fs_eval_flux", "fs_eval_dfdph",
     "fs_eval_flux_noalloc",
     "fs_get_timers", "fs_set_timing", "fs_set_solver", "fs_time_kernel", "fs_copy_bench",
     "fs_ensemble_create", "fs_ensemble_destroy", "fs_ensemble_member", "fs_ensemble_run",
+    "fs_ensemble_stage_tile",
     "fs_learn_create", "fs_learn_destroy", "fs_learn_nparams", "fs_learn_get_weights",
     "fs_learn_set_weights", "fs_learn_samples", "fs_learn_harvest", "fs_learn_train",
     "fs_learn_loss_grad", "fs_learn_attach", "fs_last_error",
@@ -48,7 +50,7 @@ class Params(C.Structure):
                 ("t_rise", C.c_double), ("t_width", C.c_double), ("t_fall", C.c_double),
                 ("dt0", C.c_double), ("lapl_rtol", C.c_double), ("lapl_maxit", C.c_int32),
                 ("lapl_chunk", C.c_int32), ("debug_keep_k", C.c_int32), ("lapl_solver", C.c_int32),
-                ("use_graphs", C.c_int32), ("lapl_warm", C.c_int32)]
+                ("use_graphs", C.c_int32), ("lapl_warm", C.c_int32), ("stage_tile", C.c_int32)]
 
 
 class StepInfo(C.Structure):
@@ -93,6 +95,7 @@ def load():
         "fs_init_uniform": [P],
         "fs_init_gaussian": [P, C.c_uint64, C.c_double, C.c_double, C.c_double],
         "fs_circ_init": [P, C.c_double],
+        "fs_stage_tile": [P],
         "fs_get_scalars": [P, D],
         "fs_set_scalars": [P, D],
         "fs_ptcl_step": [P, C.POINTER(StepInfo)],
@@ -121,6 +124,7 @@ def load():
         "fs_ensemble_destroy": [P],
         "fs_ensemble_member": [P, C.c_int32],
         "fs_ensemble_run": [P, C.c_int32],
+        "fs_ensemble_stage_tile": [P],
         "fs_learn_create": [C.c_int32, C.c_int32, C.c_int32, C.c_uint64, C.POINTER(P)],
         "fs_learn_destroy": [P],
         "fs_learn_nparams": [P],

@@ -85,6 +85,10 @@ typedef struct fs_params {
                              corrections extrapolated to the new step, instead of zero
                              (the reference's PETSc KSP starts from zero); fs_set_field
                              and fs_set_solver reset it */
+  int32_t stage_tile;     /* tile height of the stage kernel (tiles of 64 x stage_tile
+                             cells): 4, 8 or 16, or 0 to take the fastest of the three,
+                             timed on the grid at creation (cached per grid shape).
+                             Results do not depend on it. */
 } fs_params;
 
 /* Per-attempt controller record (replay and log), 10 doubles:
@@ -135,6 +139,9 @@ int fs_init_gaussian(fs_handle* h, uint64_t seed, double peak, double sigma, dou
 /* circ_init(Vset) (circ_lib.F90:213-223): the source amplitude Vmax and the
  * voltages Vsrc, Vd_pl, Vd_mi, Vd_or all take Vset (nondimensional). */
 int fs_circ_init(fs_handle* h, double vset);
+
+/* The stage kernel's tile height in use (4, 8 or 16; fs_params.stage_tile). */
+int fs_stage_tile(const fs_handle* h);
 
 /* Scalars: t, dt, err_prev, t_m, Vd_pl, Vd_mi, Vd_or, Ida, Idc, Vsrc, ts. */
 int fs_get_scalars(const fs_handle* h, double* out11);
@@ -250,6 +257,10 @@ int fs_ensemble_destroy(fs_ensemble* e);
 fs_handle* fs_ensemble_member(fs_ensemble* e, int32_t i);
 /* Every member advances `passes` main-loop passes (fs_run semantics). */
 int fs_ensemble_run(fs_ensemble* e, int32_t passes);
+/* The tile height of the ensemble's batched stage launches: its members'
+ * common fs_params.stage_tile, or (0) the fastest of 4, 8, 16 timed over all
+ * members at creation. */
+int fs_ensemble_stage_tile(const fs_ensemble* e);
 
 /* ---- learned Poisson preconditioner ------------------------------------
  * No counterpart in the reference (its Poisson solve is PETSc's KSP with the
