@@ -28,9 +28,11 @@ def gpu_run(args):
     from fluidsim_amd.lib import FsError
 
     if args.config == 2:
-        cfg = configs.config2(args.n, rtol=args.rtol, chunk=args.chunk, warm=int(not args.cold))
+        cfg = configs.config2(args.n, rtol=args.rtol, chunk=args.chunk, warm=int(not args.cold),
+                              graphs=int(not args.no_graphs))
     else:
-        cfg = configs.config1(args.n, rtol=args.rtol, chunk=args.chunk, warm=int(not args.cold))
+        cfg = configs.config1(args.n, rtol=args.rtol, chunk=args.chunk, warm=int(not args.cold),
+                              graphs=int(not args.no_graphs))
     d = Discharge(cfg)
     d.init_gaussian(seed=0)
     # warm-up: the two Poisson-only passes (main.F90:75) and the first steps,
@@ -89,7 +91,7 @@ def gpu_run(args):
     copy_gbs = copy_bandwidth(1 << 30, 20)
     return dict(cfg=cfg, elapsed=elapsed, accepted=accepted, attempts=attempts,
                 iters=iters, timers=timers, kernels=k, N=N, s1=s1, copy_gbs=copy_gbs,
-                poisson=poisson, snap=snap, same=same)
+                poisson=poisson, snap=snap, same=same, stage_tile=d.stage_tile())
 
 
 def snapshot(d):
@@ -202,6 +204,7 @@ def ensemble_run(args):
     cfgs = configs.config3(args.members)
     for c in cfgs:
         c["lapl_warm"] = int(not args.cold)
+        c["use_graphs"] = int(not args.no_graphs)
     t0 = time.perf_counter()
     E = Ensemble(cfgs)
     for m, c in zip(E.members, cfgs):
@@ -215,6 +218,7 @@ def ensemble_run(args):
     acc = sum(m.scalars()["ts"] - a for m, a in zip(E.members, ts0))
     attempts = sum(len(m.log()) for m in E.members)
     cells = sum(c["nx"] * c["ny"] for c in cfgs)
+    stage_tile = E.stage_tile()
     # solo reference: a sample of members, each run alone for the same passes
     k = min(args.solo_sample, len(cfgs))
     solo_t, solo_acc = 0.0, 0.0
@@ -229,7 +233,8 @@ def ensemble_run(args):
         solo_acc += d.scalars()["ts"] - a
     E.close()
     return dict(elapsed=el, accepted=acc, attempts=attempts, cells=cells, setup_s=t_setup,
-                solo_rate=solo_acc / solo_t if solo_t > 0 else None, solo_sample=k)
+                solo_rate=solo_acc / solo_t if solo_t > 0 else None, solo_sample=k,
+                stage_tile=stage_tile)
 
 
 def cpu_model() -> str:
@@ -316,10 +321,13 @@ def profiled_traffic(kernel: str, n: int):
     path = os.path.join(ROOT, "profiles", f"{kernel}_{n}x{n}_pulse_traffic.md")
     if not os.path.exists(path):
         return None
+    col = None
     for line in open(path):
         cells = [c.strip() for c in line.strip().strip("|").split("|")]
-        if cells and cells[0] == kernel and len(cells) >= 7:
-            return float(cells[6]) * 1e6
+        if "total MB" in cells:
+            col = cells.index("total MB")
+        elif col is not None and cells and cells[0] == kernel and len(cells) > col:
+            return float(cells[col]) * 1e6
     return None
 
 
@@ -336,6 +344,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--solve-reps", type=int, default=3)
     ap.add_argument("--no-learned", action="store_true", help="skip the learned preconditioner")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="launch every kernel from the host instead of replaying HIP graphs "
+                         "(for kernel tracers that fail on large graph launches)")
     ap.add_argument("--cold", action="store_true",
                     help="start every multigrid-PCG solve of the run from zero (no warm start)")
     ap.add_argument("--learn-samples", type=int, default=20000)
@@ -371,12 +382,14 @@ def main():
             "config": {"workload": f"{args.members} argon discharges, sides U{{64..256}}, "
                                    "150-400 V DC, 0.5-10 Torr log-uniform, gamma 0.05-0.15",
                        "members": args.members, "cells_total": e["cells"],
-                       "poisson_initial_guess": "zero" if args.cold else "extrapolated from the last two passes' corrections"},
+                       "poisson_initial_guess": "zero" if args.cold else "extrapolated from the last two passes' corrections",
+                   "hip_graphs": not args.no_graphs},
             "accepted_steps": e["accepted"], "ptcl_attempts": e["attempts"],
             "member_passes_per_s": args.members * args.steps / e["elapsed"],
             "solo_sequential_steps_per_s": e["solo_rate"], "solo_sample": e["solo_sample"],
             "batched_over_sequential": rate / e["solo_rate"] if e["solo_rate"] else None,
-            "setup_s": e["setup_s"], "cpu_baseline": None}))
+            "setup_s": e["setup_s"], "stage_tile": f"64x{e['stage_tile']}",
+            "cpu_baseline": None}))
         return
     r = gpu_run(args)
     steps_per_s = r["accepted"] / r["elapsed"]
@@ -420,10 +433,12 @@ def main():
                    "nx": args.n, "ny": args.n, "poisson_rtol": args.rtol,
                    "poisson_solver": "CG preconditioned by an aggregation multigrid V(2,2) "
                                      "cycle with red-black Gauss-Seidel smoothing",
-                   "poisson_initial_guess": "zero" if args.cold else "extrapolated from the last two passes' corrections"},
+                   "poisson_initial_guess": "zero" if args.cold else "extrapolated from the last two passes' corrections",
+                   "hip_graphs": not args.no_graphs},
         "accepted_steps": r["accepted"],
         "ptcl_attempts": r["attempts"],
         "pcg_iters_mean": sum(iters) / max(len(iters), 1),
+        "stage_tile": f"64x{r['stage_tile']}",
         "time_split_ms_per_step": {"stage_kernels": timers[0], "circuit_controller": timers[1],
                                    "assembly": timers[2], "poisson_solve": timers[3],
                                    "wall_with_events": timers[4], "steps": timers[5]},
