@@ -377,6 +377,7 @@ CircArgs circ_args(fs_handle* h, int stage) {
   c.t_fall = h->p.t_fall;
   c.rf = h->p.rf;
   c.circuit = h->p.circuit;
+  c.oned = h->G.oned;
   c.stage = stage;
   c.by = h->by;
   c.P = h->P;
@@ -449,6 +450,9 @@ int build_mg(fs_handle* h) {
   size_t total = 0;
   HIP_OK(hipMalloc(&h->d_mginv, kMgBottomMax * kMgBottomMax * sizeof(double)));
   HIP_OK(hipMalloc(&h->d_mgpk, kMgPackFloats * sizeof(float)));
+  // one dimension (ny = 1): the one-level hierarchy, a Jacobi-preconditioned
+  // solve of the tridiagonal system
+  if (h->by == 1) return 0;
   while ((size_t)nx * ny > (size_t)kMgBottomMax) {
     nx = (nx + 1) / 2;
     ny = (ny + 1) / 2;
@@ -941,7 +945,8 @@ int lapl_solve_impl(fs_handle* h, int* lerr, fs_solve_stats* stats, int mode = 0
     // flexible CG with the learned coarse solver (eager launches); without an
     // attached network the learned level gets a zero correction
     if (h->mg.size() < 2)
-      return fail(FS_E_ARG, "learned solver: grid of <= 64 cells has no coarse level%s");
+      return fail(FS_E_ARG,
+                  "learned solver: the grid has no coarse level (<= 64 cells, or 1-D)%s");
     for (size_t l = lapl_coarse_levels(h); l < learned_level(h); l++)
       launch_mg_coarsen(mg_args(h, l), h->st);
     MgPcgArgs ia = mgpcg_args(h, 1);
@@ -1773,7 +1778,7 @@ int fs_constants(double p, double gam, double* out) {
 
 int fs_create(const fs_grid_desc* g, const fs_params* p, fs_handle** out) {
   if (!g || !p || !out) return fail(FS_E_ARG, "null argument%s", "");
-  if (g->nx < 3 || g->ny < 2) return fail(FS_E_ARG, "need nx >= 3 and ny >= 2%s", "");
+  if (g->nx < 3 || g->ny < 1) return fail(FS_E_ARG, "need nx >= 3 and ny >= 1%s", "");
   if (p->rf < 0 || p->rf > 4) return fail(FS_E_ARG, "unknown waveform rf%s", "");
   if (p->p_torr <= 0) return fail(FS_E_ARG, "pressure must be positive%s", "");
   if (p->rf == 4 && (p->t_rise <= 0 || p->t_fall <= 0))
@@ -1781,6 +1786,12 @@ int fs_create(const fs_grid_desc* g, const fs_params* p, fs_handle** out) {
   fs_handle* h = new fs_handle();
   h->g = *g;
   h->p = *p;
+  const bool oned = g->ny == 1;
+  if (oned) {  // props.F90:64-65
+    h->g.cyl = 0;
+    h->g.rwall = 0;
+    g = &h->g;
+  }
   fs_phys_init(&h->K, p->p_torr, p->gam);
   h->bx = g->nx;
   h->by = g->ny;
@@ -1789,15 +1800,23 @@ int fs_create(const fs_grid_desc* g, const fs_params* p, fs_handle** out) {
   const int bx = h->bx, by = h->by;
   // grid: props.F90:96-218
   build_axis(g->grid_x, g->ratio_x, bx, g->l, 0, h->x);
-  build_axis(g->grid_y, g->ratio_y, by, g->w, 1, h->y);
   h->dx.resize(bx + 1);
   h->dlx.resize(bx);
   h->dy.resize(by + 1);
   h->dly.resize(by);
   for (int i = 0; i < bx + 1; i++) h->dx[i] = h->x[i + 1] - h->x[i];
   for (int i = 0; i < bx; i++) h->dlx[i] = 0.5 * (h->dx[i + 1] + h->dx[i]);
-  for (int j = 0; j < by + 1; j++) h->dy[j] = h->y[j + 1] - h->y[j];
-  for (int j = 0; j < by; j++) h->dly[j] = 0.5 * (h->dy[j + 1] + h->dy[j]);
+  if (!oned) {
+    build_axis(g->grid_y, g->ratio_y, by, g->w, 1, h->y);
+    for (int j = 0; j < by + 1; j++) h->dy[j] = h->y[j + 1] - h->y[j];
+    for (int j = 0; j < by; j++) h->dly[j] = 0.5 * (h->dy[j + 1] + h->dy[j]);
+  } else {
+    // props.F90:175-178: y = 1, dy = dx(1); the kernels skip every y term, the
+    // other y metrics only need to be finite (dly = dx(1), r = 1)
+    h->y.assign(3, 1.0);
+    h->dy.assign(2, h->dx[0]);
+    h->dly.assign(1, h->dx[0]);
+  }
   h->r = h->y;
   h->type_x.assign((size_t)bx * by, 0);
   h->type_y.assign((size_t)bx * by, 0);
@@ -1820,11 +1839,13 @@ int fs_create(const fs_grid_desc* g, const fs_params* p, fs_handle** out) {
     double rr = h->r[j + 1];
     double w = g->cyl ? h->dy[j] * h->r[j] * 2.0 * h->K.pi : h->dy[j] * g->w;
     if (!active && rr < g->ew) w = 0.0;
+    if (oned) w = g->w * g->w * h->K.pi;  // circ_lib.F90:98-99, 131-132
     dr[j] = el ? w : 0.0;
     dr[by + j] = el ? w : 0.0;
-    // ghost potential rows, circ_lib.F90:180-198
-    types[2 * by + j] = (rr <= g->ew && el) ? 1 : 0;
-    types[3 * by + j] = (rr <= g->ew && el) ? 1 : 0;
+    // ghost potential rows, circ_lib.F90:180-198 (one dimension: the
+    // electrode cells' rows, circ_lib.F90:199-202)
+    types[2 * by + j] = ((oned || rr <= g->ew) && el) ? 1 : 0;
+    types[3 * by + j] = ((oned || rr <= g->ew) && el) ? 1 : 0;
   }
   h->nblk = 0;
   h->nbx = (bx + kTX - 1) / kTX;
@@ -1904,6 +1925,7 @@ int fs_create(const fs_grid_desc* g, const fs_params* p, fs_handle** out) {
   h->G.P = h->P;
   h->G.cyl = g->cyl;
   h->G.rwall = g->rwall ? 1 : 0;
+  h->G.oned = oned ? 1 : 0;
   h->G.sig = h->d_sig;
   // zero state; PCG ghosts: r = p = x = 0 and diagonal 1 so that neighbour
   // recomputation at the boundary yields p = 0
@@ -2104,7 +2126,8 @@ int fs_init_gaussian(fs_handle* h, uint64_t seed, double peak, double sigma, dou
   for (auto& a : f) a.resize(h->PL);
   for (int j = 0; j < h->by + 2; j++)
     for (int i = 0; i < h->bx + 2; i++) {
-      double ddx = h->x[i] - xc, ddy = h->y[j] - yc;
+      // in one dimension the blob is a profile in x alone
+      double ddx = h->x[i] - xc, ddy = h->by > 1 ? h->y[j] - yc : 0.0;
       double base = peak * exp(-(ddx * ddx + ddy * ddy) / (2.0 * sigma * sigma));
       size_t m = (size_t)j * h->P + i;
       double ni = base * (1.0 + noise * unoise(seed, 0, i + 1, j + 1)) + K.n_zero;
@@ -2492,8 +2515,10 @@ int fs_ensemble_create(int32_t n, const fs_grid_desc* g, const fs_params* p, fs_
   if (n <= 0 || !g || !p || !out) return fail(FS_E_ARG, "bad argument%s");
   if (p[0].stage_tile != 0 && !valid_stage_tile(p[0].stage_tile))
     return fail(FS_E_ARG, "stage_tile must be 0, 4, 8 or 16%s");
-  for (int i = 0; i < n; i++)
+  for (int i = 0; i < n; i++) {
     if (g[i].rwall) return fail(FS_E_ARG, "ensemble members have no dielectric wall%s");
+    if (g[i].ny < 2) return fail(FS_E_ARG, "ensemble members are two-dimensional (ny >= 2)%s");
+  }
   fs_ensemble* E = new fs_ensemble();
   E->n = n;
   for (int i = 0; i < n; i++) {
@@ -2650,7 +2675,8 @@ int fs_learn_samples(const fs_learn* n) { return n ? n->net.count : -1; }
 
 int fs_learn_harvest(fs_learn* n, fs_handle* h, int32_t* added) {
   if (!n || !h) return fail(FS_E_ARG, "null argument%s");
-  if (h->mg.size() < 2) return fail(FS_E_ARG, "grid of <= 64 cells has no coarse level%s");
+  if (h->mg.size() < 2)
+    return fail(FS_E_ARG, "the grid has no coarse level (<= 64 cells, or 1-D)%s");
   const MgLevel& lc = h->mg[learned_level(h)];
   if (lc.nx > n->net.W || lc.ny > n->net.H)
     return fail(FS_E_ARG, "learned level larger than the training grid%s");

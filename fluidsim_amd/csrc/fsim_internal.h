@@ -65,7 +65,10 @@ struct Geo {
   // sig: 8 arrays of bx+2 (x-node index i+1 for cell i): sig_pl, sig_mi,
   // sig_or (sfc_lib.F90:7), then the surface-charge slopes of stages 1-5
   double* sig;
-  int32_t rwall, pad_g;
+  int32_t rwall;
+  // one dimension (ny = 1, by = 1): every y term is left out (ptcl_lib.F90:285,
+  // 455; lapl_lib.F90:255, 462, 577)
+  int32_t oned;
 };
 constexpr int kSigPl = 0, kSigMi = 1, kSigOr = 2, kSigK = 2;  // ks of stage s: kSigK + s
 
@@ -80,6 +83,8 @@ struct CircArgs {
   DevScalars* sc;
   double vset, res, t_rise, t_width, t_fall;
   int32_t rf, circuit, stage, by, P, bx;
+  int32_t oned;  // one dimension: the ghost potential is the source voltage (circ_lib.F90:199-202)
+  int32_t pad_c;
 };
 
 struct StageArgs {

@@ -319,7 +319,7 @@ __device__ __forceinline__ void stage_body(const StageArgs& A, const FsPhys& K, 
       const double vprev = k == 1 ? sc->Vd_mi : (pin ? Vsrc : circ_vd(k - 1, sc->Vd_or, dt, kv));
       kv[k] = -(cv[1] - (Vsrc - vprev) / C.res) / K.Cap;
       const double vd = pin ? Vsrc : circ_vd(k, sc->Vd_or, dt, kv);
-      s_vd = vd;
+      s_vd = C.oned ? Vsrc : vd;  // one dimension: circ_lib.F90:199-202
       if (bid == 0) {
         sc->kv[k] = kv[k];
         if (k > 1) sc->Vd_mi = vd;
@@ -452,18 +452,18 @@ __device__ __forceinline__ void stage_body(const StageArgs& A, const FsPhys& K, 
 #pragma unroll
       for (int s = 0; s < 4; s++) dfy[s] = (s_f[s][fT] - s_f[s][fB]) * rdly;
     }
-    int typy = cj == 0 ? -1 : (cj == by - 1 ? (G.rwall ? 3 : 1) : 0);
-    double eyB, eyT;
+    // (one dimension: no y branch at all, typy 2 is never assigned otherwise;
+    // Ey and the y fluxes stay zero, mue(2) is the x branch's)
+    const int typy = G.oned ? 2 : (cj == 0 ? -1 : (cj == by - 1 ? (G.rwall ? 3 : 1) : 0));
+    double eyB = 0.0, eyT = 0.0;
     if (typy == -1) {
-      eyB = 0.0;
       eyT = -(s_ph[hc + kHX] - s_ph[hc]) * gld(G.rdy, cj + 1);
     } else if (typy == 1) {
       eyB = -(s_ph[hc] - s_ph[hc - kHX]) * gld(G.rdy, cj);
-      eyT = 0.0;
     } else if (typy == 3) {  // ptcl_lib.F90:292-294
       eyB = -(s_ph[hc] - s_ph[hc - kHX]) * gld(G.rdy, cj);
       eyT = -gld(G.sig + (size_t)kSigMi * (bx + 2), ci + 1);
-    } else {
+    } else if (typy == 0) {
       eyB = -(s_ph[hc] - s_ph[hc - kHX]) * gld(G.rdy, cj);
       eyT = -(s_ph[hc + kHX] - s_ph[hc]) * gld(G.rdy, cj + 1);
     }
@@ -627,7 +627,8 @@ __device__ __forceinline__ void circ_body(const CircArgs& C, const FsPhys& K) {
     sc->Vsrc = Vsrc;
     sc->Ida = Ida;
     sc->Idc = Idc;
-    s_vd = stage == 5 ? sc->Vd_pl : sc->Vd_mi;
+    // (one dimension: the source voltage, circ_lib.F90:199-202)
+    s_vd = C.oned ? Vsrc : (stage == 5 ? sc->Vd_pl : sc->Vd_mi);
   }
   __syncthreads();
   const double vd = s_vd;
@@ -1043,67 +1044,71 @@ __device__ __forceinline__ void lapl_body(const LaplArgs& A, const FsPhys& K, in
     double dly = G.dly[cj], hB = G.dy[cj], hT = G.dy[cj + 1];
     double rj = G.r[cj + 1], rp = G.r[cj + 2], rm = G.r[cj];
     int typy = cj == 0 ? -1 : (cj == by - 1 ? (G.rwall ? 3 : 1) : 0);
-    double eyB, eyT;
-    if (typy == -1) {
-      eyB = 0.0;
-      eyT = -(s_ph[hc + kHX] - s_ph[hc]) / hT;
-    } else if (typy == 1) {
-      eyB = -(s_ph[hc] - s_ph[hc - kHX]) / hB;
-      eyT = 0.0;
-    } else if (typy == 3) {  // Ey(2) = -sig (lapl_lib.F90:263-265)
-      eyB = -(s_ph[hc] - s_ph[hc - kHX]) / hB;
-      eyT = -G.sig[(size_t)kSigPl * (bx + 2) + ci + 1];
-    } else {
-      eyB = -(s_ph[hc] - s_ph[hc - kHX]) / hB;
-      eyT = -(s_ph[hc + kHX] - s_ph[hc]) / hT;
-    }
-    double dfdy, dflxi_y, dflxe_y;
-    double fiB = s_f[0][fB], fiT = s_f[0][fT], feB = s_f[1][fB], feT = s_f[1][fT];
-    if (G.cyl) {
-      dfdy = -(eyT * (rp + rj) / 2.0 - eyB * (rj + rm) / 2.0) / dly / rj;
-      dflxi_y = (fiT * (rp + rj) / 2.0 - fiB * (rj + rm) / 2.0) / dly / rj;
-      dflxe_y = (feT * (rp + rj) / 2.0 - feB * (rj + rm) / 2.0) / dly / rj;
-    } else {
-      dfdy = -(eyT - eyB) / dly;
-      dflxi_y = (fiT - fiB) / dly;
-      dflxe_y = (feT - feB) / dly;
+    // one dimension: the y part is left out (lapl_lib.F90:255, 462, 577); dfdy,
+    // dflxi_y and dflxe_y are zero, there is no y column
+    double dfdy = 0.0, dflxi_y = 0.0, dflxe_y = 0.0;
+    double dfyi_mi = 0, dfye_mi = 0, dfyi_pl = 0, dfye_pl = 0, aS = 0, aN = 0, gT = 0;
+    double C = Cx;
+    if (!G.oned) {
+      double eyB, eyT;
+      if (typy == -1) {
+        eyB = 0.0;
+        eyT = -(s_ph[hc + kHX] - s_ph[hc]) / hT;
+      } else if (typy == 1) {
+        eyB = -(s_ph[hc] - s_ph[hc - kHX]) / hB;
+        eyT = 0.0;
+      } else if (typy == 3) {  // Ey(2) = -sig (lapl_lib.F90:263-265)
+        eyB = -(s_ph[hc] - s_ph[hc - kHX]) / hB;
+        eyT = -G.sig[(size_t)kSigPl * (bx + 2) + ci + 1];
+      } else {
+        eyB = -(s_ph[hc] - s_ph[hc - kHX]) / hB;
+        eyT = -(s_ph[hc + kHX] - s_ph[hc]) / hT;
+      }
+      double fiB = s_f[0][fB], fiT = s_f[0][fT], feB = s_f[1][fB], feT = s_f[1][fT];
+      if (G.cyl) {
+        dfdy = -(eyT * (rp + rj) / 2.0 - eyB * (rj + rm) / 2.0) / dly / rj;
+        dflxi_y = (fiT * (rp + rj) / 2.0 - fiB * (rj + rm) / 2.0) / dly / rj;
+        dflxe_y = (feT * (rp + rj) / 2.0 - feB * (rj + rm) / 2.0) / dly / rj;
+      } else {
+        dfdy = -(eyT - eyB) / dly;
+        dflxi_y = (fiT - fiB) / dly;
+        dflxe_y = (feT - feB) / dly;
+      }
+
+      // laplJ y part and diagonal, lapl_lib.F90:463-607
+      if (typy >= 0) {
+        dfyi_mi = s_f[2][fB];
+        dfye_mi = -s_f[3][fB];
+        if (G.cyl)
+          aS = 5e-1 * (rj + rm) / hB / dly / rj + dt * (dfyi_mi - dfye_mi) * (rj + rm) / 2.0 / dly / rj;
+        else
+          aS = 1.0 / hB / dly + dt * (dfyi_mi - dfye_mi) / dly;
+      }
+      if (typy <= 0) {
+        dfyi_pl = -s_f[2][fT];
+        dfye_pl = s_f[3][fT];
+        gT = 1.0 / hT - dt * (dfyi_pl - dfye_pl);
+        if (G.cyl)
+          aN = 5e-1 * (rp + rj) / hT / dly / rj - dt * (dfyi_pl - dfye_pl) * (rp + rj) / 2.0 / dly / rj;
+        else
+          aN = 1.0 / hT / dly - dt * (dfyi_pl - dfye_pl) / dly;
+      }
+      if (G.cyl) {
+        C = C + dt * ((dfyi_pl - dfye_pl) * (rp + rj) / 2.0 - (dfyi_mi - dfye_mi) * (rj + rm) / 2.0) /
+                    dly / rj;
+        if (typy == -1) C = C - 1.0 / hB / dly * (rp + rj) / 2.0 / rj;
+        else if (typy == 1) C = C - 1.0 / hB / dly * (rj + rm) / 2.0 / rj;
+        else C = C - 5e-1 * ((rp + rj) / hT + (rj + rm) / hB) / dly / rj;
+      } else {
+        C = C + dt * (dfyi_pl - dfyi_mi - dfye_pl + dfye_mi) / dly;
+        if (typy == -1 || typy == 1) C = C - 1.0 / hB / dly;
+        else C = C - 1.0 / dly * (1.0 / hT + 1.0 / hB);
+      }
     }
     // laplEqn residual, lapl_lib.F90:374-375
     size_t m = (size_t)(cj + 1) * P + (ci + 1);
     double ni = s_ni[hc], ne = s_ne[hc];
     double b = dfdx + dfdy + ni - ne - dt * (dflxi_x + dflxi_y - dflxe_x - dflxe_y);
-
-    // laplJ y part and diagonal, lapl_lib.F90:463-607
-    double dfyi_mi = 0, dfye_mi = 0, dfyi_pl = 0, dfye_pl = 0, aS = 0, aN = 0, gT = 0;
-    if (typy >= 0) {
-      dfyi_mi = s_f[2][fB];
-      dfye_mi = -s_f[3][fB];
-      if (G.cyl)
-        aS = 5e-1 * (rj + rm) / hB / dly / rj + dt * (dfyi_mi - dfye_mi) * (rj + rm) / 2.0 / dly / rj;
-      else
-        aS = 1.0 / hB / dly + dt * (dfyi_mi - dfye_mi) / dly;
-    }
-    if (typy <= 0) {
-      dfyi_pl = -s_f[2][fT];
-      dfye_pl = s_f[3][fT];
-      gT = 1.0 / hT - dt * (dfyi_pl - dfye_pl);
-      if (G.cyl)
-        aN = 5e-1 * (rp + rj) / hT / dly / rj - dt * (dfyi_pl - dfye_pl) * (rp + rj) / 2.0 / dly / rj;
-      else
-        aN = 1.0 / hT / dly - dt * (dfyi_pl - dfye_pl) / dly;
-    }
-    double C = Cx;
-    if (G.cyl) {
-      C = C + dt * ((dfyi_pl - dfye_pl) * (rp + rj) / 2.0 - (dfyi_mi - dfye_mi) * (rj + rm) / 2.0) /
-                  dly / rj;
-      if (typy == -1) C = C - 1.0 / hB / dly * (rp + rj) / 2.0 / rj;
-      else if (typy == 1) C = C - 1.0 / hB / dly * (rj + rm) / 2.0 / rj;
-      else C = C - 5e-1 * ((rp + rj) / hT + (rj + rm) / hB) / dly / rj;
-    } else {
-      C = C + dt * (dfyi_pl - dfyi_mi - dfye_pl + dfye_mi) / dly;
-      if (typy == -1 || typy == 1) C = C - 1.0 / hB / dly;
-      else C = C - 1.0 / dly * (1.0 / hT + 1.0 / hB);
-    }
     // symmetrised storage: row (i,j) times V = dlx*dly[*r]
     double dlx = G.dlx[ci];
     double rw = G.cyl ? rj : 1.0;
@@ -1112,7 +1117,7 @@ __device__ __forceinline__ void lapl_body(const LaplArgs& A, const FsPhys& K, in
     double dgv = -V * C;
     o_d = dgv;
     o_cx = typx <= 0 ? gR * dly * rw : 0.0;
-    o_cy = typy <= 0 ? gT * dlx * (G.cyl ? (rp + rj) / 2.0 : 1.0) : 0.0;
+    o_cy = !G.oned && typy <= 0 ? gT * dlx * (G.cyl ? (rp + rj) / 2.0 : 1.0) : 0.0;
     A.rhs0[m] = rhs;
     A.r[m] = rhs;
     A.dg[m] = o_d;
@@ -1779,6 +1784,28 @@ __device__ __forceinline__ void mg_down_body(const MgArgs& A, int bid, int nb) {
   __shared__ double s_res[kGsTY][kGsTX / 2];
   __shared__ double s_red[8 * kNW];
   DevScalars* sc = A.sc;
+  const MgLevel &F = A.F, &C = A.C;
+  const int ntx = gs_tiles_x(F);
+  const TileRange tr = tile_range(gs_ntiles(F), bid, nb);
+  const bool red = A.check || kCg;
+  // The first tile's loads and the partials'.  In the level-0 CG form they
+  // are issued with the solver state, before the test of `done`: every
+  // address is in bounds whatever the state (a block without tiles loads
+  // cells outside the level, which read the planes' first element), so that
+  // kernel waits for one memory round trip instead of two (measured: 0.4 us
+  // faster at 512^2; the coarser levels' kernels were measured 0.2-0.4 us
+  // slower so, and issue them after the test).  The LDS stores follow it.
+  CgCells cc;
+  CgRegs cgr;
+  GsRegs<kGsH, false> gsr;
+  Part3 pl;
+  auto issue_first = [&]() {
+    const int i0 = (tr.t0 % ntx) * kGsTX, j0 = (tr.t0 / ntx) * kGsTY;
+    if (kCg) cg_load(A, i0, j0, cc, cgr);
+    else gs_load<kGsH, false>(F, i0, j0, C, gsr);
+    if (red) part3_load(A.part_in, A.nb_in, pl);
+  };
+  if (kCg) issue_first();
   CheckState cs{};
   double gam_old = 1.0, alpha_old = 1.0;  // the previous CG iteration's gamma and alpha
   double wa = 1.0, wb = 0.0, wc = 0.0;  // warm start x = wa x + wb x2 + wc x3 (initial test)
@@ -1804,9 +1831,6 @@ __device__ __forceinline__ void mg_down_body(const MgArgs& A, int bid, int nb) {
     }
     if (done) return;
   }
-  const MgLevel &F = A.F, &C = A.C;
-  const int ntx = gs_tiles_x(F);
-  const TileRange tr = tile_range(gs_ntiles(F), bid, nb);
   if (A.xs) {
     // the initial test of MG-PCG.  A warm start is wa x + wb x2 (the init
     // kernel formed r = rhs - M (wa x + wb x2)); it is formed here, before the
@@ -1838,6 +1862,7 @@ __device__ __forceinline__ void mg_down_body(const MgArgs& A, int bid, int nb) {
       sc->x2_ok = wn;
     }
   }
+  if (!kCg) issue_first();
   double sacc[3] = {0.0, 0.0, 0.0};  // sum r_c^2 / d_c for the learned coarse solver
   double racc[3] = {0.0, 0.0, 0.0};  // sum |r/V|^2 after the CG update
   if (tr.t0 >= tr.t1) {  // block 0 always has tiles (it writes the scalars)
@@ -1845,24 +1870,8 @@ __device__ __forceinline__ void mg_down_body(const MgArgs& A, int bid, int nb) {
     if (kCg) write_partials3(racc, A.part_out, s_red, bid);
     return;
   }
-  const bool red = A.check || kCg;
-  // the first tile's loads, then the partials', then the LDS stores
-  CgCells cc;
-  Part3 pl;
-  {
-    const int i0 = (tr.t0 % ntx) * kGsTX, j0 = (tr.t0 / ntx) * kGsTY;
-    if (kCg) {
-      CgRegs g;
-      cg_load(A, i0, j0, cc, g);
-      if (red) part3_load(A.part_in, A.nb_in, pl);
-      cg_store(g, s, cc);
-    } else {
-      GsRegs<kGsH, false> g;
-      gs_load<kGsH, false>(F, i0, j0, C, g);
-      if (red) part3_load(A.part_in, A.nb_in, pl);
-      gs_store<kGsH, false>(g, s, 0.0);
-    }
-  }
+  if (kCg) cg_store(cgr, s, cc);
+  else gs_store<kGsH, false>(gsr, s, 0.0);
   double pv[3];
   if (red) part3_sum(pl, A.nb_in, pv);
   double alpha = 0.0, beta = 0.0;
@@ -1989,6 +1998,13 @@ __device__ __forceinline__ void mg_up_body(const MgArgs& A, int bid, int nb) {
   constexpr int W = GsTile<H>::W, R = GsTile<H>::R;
   __shared__ GsTile<H> s;
   __shared__ double s_red[8 * kNW];
+  const MgLevel &F = A.F, &C = A.C;
+  const int ntx = gs_tiles_x(F);
+  const TileRange tr = tile_range(gs_ntiles(F), bid, nb);
+  // the first tile's loads are issued before the test of `done` (in bounds
+  // whatever the state, as in the down kernel)
+  GsRegs<H, true> g0;
+  gs_load<H, true>(F, (tr.t0 % ntx) * kGsTX, (tr.t0 / ntx) * kGsTY, C, g0);
   {
     const int done = A.sc->done;
     if (kInstance == 0) {
@@ -1997,14 +2013,15 @@ __device__ __forceinline__ void mg_up_body(const MgArgs& A, int bid, int nb) {
     }
     if (done) return;
   }
-  const MgLevel &F = A.F, &C = A.C;
-  const int ntx = gs_tiles_x(F);
-  const TileRange tr = tile_range(gs_ntiles(F), bid, nb);
   double acc[3] = {0.0, 0.0, 0.0};
   for (int t = tr.t0; t < tr.t1; t++) {
     const int i0 = (t % ntx) * kGsTX, j0 = (t / ntx) * kGsTY;
-    if (t > tr.t0) __syncthreads();
-    gs_stage<H, true>(F, s, i0, j0, C, A.oc);
+    if (t > tr.t0) {
+      __syncthreads();
+      gs_stage<H, true>(F, s, i0, j0, C, A.oc);
+    } else {
+      gs_store<H, true>(g0, s, A.oc);
+    }
     __syncthreads();
     gs_half(s, i0, j0, 1, H - 1);
     __syncthreads();
@@ -2412,29 +2429,23 @@ __device__ __forceinline__ void mg_coarse_cycle_body(const MgCycleArgs& A) {
   constexpr int kB = kMgBottomMax;
   __shared__ double s_bp[kB][kB];
   __shared__ double s_w[16];
-  CheckState cs{};
-  {
-    const int done = A.sc->done;
-    if (A.check) cs = check_state(A.sc, A.it);
-    if (kInstance == 0) hold(cs);
-    static_assert(kMgLdsLevels == 5, "the hold below lists five levels");
-    if (kInstance == 0)  // one statement: one round of argument loads
-      asm volatile("" ::"s"(A.lv[0].r), "s"(A.lv[0].e), "s"(A.lv[0].nx), "s"(A.lv[0].ny),
-                   "s"(A.lv[1].nx), "s"(A.lv[1].ny), "s"(A.lv[2].nx), "s"(A.lv[2].ny),
-                   "s"(A.lv[3].nx), "s"(A.lv[3].ny), "s"(A.lv[4].nx), "s"(A.lv[4].ny),
-                   "s"(A.off[0]), "s"(A.off[1]), "s"(A.off[2]), "s"(A.off[3]), "s"(A.off[4]),
-                   "s"(A.nq), "s"(A.inv), "s"(A.pk), "s"(A.oc), "s"(A.part_in), "s"(A.nb_in),
-                   "s"(A.it), "s"(A.check));
-    if (done) return;
-  }
+  static_assert(kMgLdsLevels == 5, "the hold below lists five levels");
+  if (kInstance == 0)  // one statement: one round of argument loads
+    asm volatile("" ::"s"(A.lv[0].r), "s"(A.lv[0].e), "s"(A.lv[0].nx), "s"(A.lv[0].ny),
+                 "s"(A.lv[1].nx), "s"(A.lv[1].ny), "s"(A.lv[2].nx), "s"(A.lv[2].ny),
+                 "s"(A.lv[3].nx), "s"(A.lv[3].ny), "s"(A.lv[4].nx), "s"(A.lv[4].ny),
+                 "s"(A.off[0]), "s"(A.off[1]), "s"(A.off[2]), "s"(A.off[3]), "s"(A.off[4]),
+                 "s"(A.nq), "s"(A.inv), "s"(A.pk), "s"(A.oc), "s"(A.part_in), "s"(A.nb_in),
+                 "s"(A.it), "s"(A.check), "s"(A.sc));
   const int t = threadIdx.x;
   const int nq = A.nq;
   // The coarsest level's inverse: entry (brow, bcol) in thread brow + kB * bcol,
   // loaded with the partial (nb_in <= 1024: one per thread) and the operator
   // rows, whose loads overlap the convergence test.  Every load is issued
-  // unconditionally (rows of levels the hierarchy does not have are loaded
-  // from the table and never used; without the test the partial is read from
-  // the inverse), so that they all share one wait.
+  // unconditionally, and before the test of `done` (rows of levels the
+  // hierarchy does not have are loaded from the table and never used; without
+  // the test the partial is read from the inverse; every address is in bounds
+  // whatever the solver state), so that they all share one wait.
   const MgLevel& B = A.lv[nq - 1];  // coarsest
   const int nb = B.nx * B.ny;
   const int brow = t % kB, bcol = t / kB;
@@ -2449,6 +2460,13 @@ __device__ __forceinline__ void mg_coarse_cycle_body(const MgCycleArgs& A) {
 #pragma unroll
   for (int q = 1; q < kMgLdsLevels - 1; q++) lds_cell_rows(A.lv[q], R1[q - 1], A.pk, 4 + q - 1, t);
   lds_cell_pos(A.lv[nq - 1], Bc, t);
+  CheckState cs{};
+  {
+    const int done = A.sc->done;
+    if (A.check) cs = check_state(A.sc, A.it);
+    if (kInstance == 0) hold(cs);
+    if (done) return;
+  }
   if (A.check) {  // convergence test on the level-0 CG update's |r/V|^2 partials
     double v = t < A.nb_in ? part_v : 0.0;
 #pragma unroll

@@ -36,8 +36,8 @@ program main
   call cpu_time(time1)
   ts1 = 0
 
-  ! defaults (main.F90:28-44); the reference's 1-D default ny = 1 is 2-D here
-  nx = 40; ny = 8; px = 1; py = 1; dof = 1
+  ! defaults (main.F90:28-44), the one-dimensional ny = 1 included
+  nx = 40; ny = 1; px = 1; py = 1; dof = 1
   call load_constants()
   l = 1e-2 / x0
   w = 1d-2 / x0
@@ -243,8 +243,8 @@ contains
       write(*,*) 'Error: px * py must equal the number of processes (1). Stop.'
       stop 2
     end if
-    if (ny < 2) then
-      write(*,*) 'Error: this build is two-dimensional (ny >= 2). Stop.'
+    if (ny < 1 .or. nx < 3) then
+      write(*,*) 'Error: need nx >= 3 and ny >= 1. Stop.'
       stop 3
     end if
   end subroutine

@@ -71,6 +71,10 @@ contains
       stop 2
     end if
     call load_constants()
+    if (ny == 1) then  ! props.F90:64-65
+      cyl = .False.
+      rwall = .False.
+    end if
     g%nx = nx; g%ny = ny; g%bx = nx; g%by = ny; g%dof = dof
     g%l = l; g%w = w; g%ew = ew; g%gap = gap
 
@@ -112,7 +116,7 @@ contains
         open(newunit=u, file=path//trim(names(k))//'.dat', access='stream', &
              form='unformatted', status='replace')
         if (k == 12) write(u) g%x(2:g%bx+1) * x0
-        if (k == 13) write(u) g%y(2:g%by+1) * x0
+        if (k == 13 .and. g%ny > 1) write(u) g%y(2:g%by+1) * x0  ! props.F90:262-264
         close(u)
       end do
     end if

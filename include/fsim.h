@@ -52,7 +52,8 @@ extern "C" {
 /* Grid description: the arguments of g_init (props.F90:52) plus the build's
  * geometric spacing.  Lengths are nondimensional (units of x0 = 1e-4 m). */
 typedef struct fs_grid_desc {
-  int32_t nx, ny;          /* interior cells (nx >= 3, ny >= 2) */
+  int32_t nx, ny;          /* interior cells (nx >= 3, ny >= 1; ny = 1: the reference's
+                              one-dimensional mode, main.F90:29) */
   int32_t grid_x, grid_y;  /* 0 uniform (unif=T), 1 tanh (unif=F), 2 geometric */
   double ratio_x, ratio_y; /* geometric spacing ratio (grid kind 2) */
   double l, w, ew, gap;    /* as g_init */

@@ -376,9 +376,13 @@ static void build_axis(int kind, double ratio, int n, double len, int is_y, doub
 }
 
 osim* o_create(const ocfg* c) {
-  if (c->nx < 3 || c->ny < 2) return NULL;
+  if (c->nx < 3 || c->ny < 1) return NULL;
   osim* s = (osim*)calloc(1, sizeof(osim));
   s->c = *c;
+  if (c->ny == 1) { /* props.F90:64-65 */
+    s->c.cyl = 0;
+    s->c.rwall = 0;
+  }
   consts_init(&s->k, c->p_torr, c->gam);
   s->bx = c->nx;
   s->by = c->ny;
@@ -393,18 +397,28 @@ osim* o_create(const ocfg* c) {
   s->dly = calloc(by + 1, sizeof(double));
   s->r = calloc(by + 3, sizeof(double));
   build_axis(c->grid_x, c->ratio_x, bx, c->l, 0, s->x);
-  build_axis(c->grid_y, c->ratio_y, by, c->w, 1, s->y);
   for (int i = 1; i <= bx + 1; i++) s->dx[i] = s->x[i + 1] - s->x[i];        /* props.F90:118-120 */
   for (int i = 1; i <= bx; i++) s->dlx[i] = 0.5 * (s->dx[i + 1] + s->dx[i]);  /* props.F90:122-124 */
-  for (int j = 1; j <= by + 1; j++) s->dy[j] = s->y[j + 1] - s->y[j];        /* props.F90:157-159 */
-  for (int j = 1; j <= by; j++) s->dly[j] = 0.5 * (s->dy[j + 1] + s->dy[j]);  /* props.F90:161-163 */
-  for (int j = 1; j <= by + 2; j++) s->r[j] = s->y[j];                       /* props.F90:165-166 */
+  if (by > 1) {
+    build_axis(c->grid_y, c->ratio_y, by, c->w, 1, s->y);
+    for (int j = 1; j <= by + 1; j++) s->dy[j] = s->y[j + 1] - s->y[j];        /* props.F90:157-159 */
+    for (int j = 1; j <= by; j++) s->dly[j] = 0.5 * (s->dy[j + 1] + s->dy[j]);  /* props.F90:161-163 */
+    for (int j = 1; j <= by + 2; j++) s->r[j] = s->y[j];                       /* props.F90:165-166 */
+  } else {
+    /* one dimension (props.F90:175-178): y = 1 and dy = dx(1); the reference
+     * has no dly and no r there (every y term is skipped), they are set to
+     * dx(1) and 1 here so that the arrays hold finite values */
+    for (int j = 1; j <= 3; j++) s->y[j] = 1.0;
+    for (int j = 1; j <= 2; j++) s->dy[j] = s->dx[1];
+    s->dly[1] = s->dx[1];
+    for (int j = 1; j <= 3; j++) s->r[j] = 1.0;
+  }
   s->type_x = calloc((size_t)bx * by, sizeof(int));
   s->type_y = calloc((size_t)bx * by, sizeof(int));
   for (int j = 1; j <= by; j++) /* props.F90:186-218 */
     for (int i = 1; i <= bx; i++) {
       if (j == 1) TY(i, j) = -1;
-      if (j == by) TY(i, j) = c->rwall ? 3 : 1; /* props.F90:192-198 */
+      if (j == by) TY(i, j) = s->c.rwall ? 3 : 1; /* props.F90:192-198 */
       int el = (s->y[j] <= c->ew) || (s->y[j] >= c->gap);
       if (i == 1) TX(i, j) = el ? -2 : -1;
       if (i == bx) TX(i, j) = el ? 2 : 1;
@@ -538,7 +552,8 @@ static void gaussian_fill(const osim* s, uint64_t seed, double peak, double sigm
   double xc = s->c.l * 0.5, yc = s->c.w * 0.5;
   for (int j = 1; j <= s->by + 2; j++)
     for (int i = 1; i <= s->bx + 2; i++) {
-      double ddx = s->x[i] - xc, ddy = s->y[j] - yc;
+      /* in one dimension the blob is a profile in x alone */
+      double ddx = s->x[i] - xc, ddy = s->by > 1 ? s->y[j] - yc : 0.0;
       double base = peak * exp(-(ddx * ddx + ddy * ddy) / (2.0 * sigma * sigma));
       double v;
       if (sp == 3) {
@@ -682,8 +697,12 @@ static void ptcl_eqn(osim* s, int i, int j, int stage, double* tm_out) {
     }
   }
 
-  /* Y-dir: ptcl_lib.F90:285-443 (ny > 1 always here) */
-  if (ty == -1) {
+  /* Y-dir: ptcl_lib.F90:285-443, only for ny > 1; in one dimension Ey,
+   * the y fluxes and their divergence keep their initial zeros (the
+   * reference's saved locals, ptcl_lib.F90:103-107, are never assigned) */
+  const int twod = s->by > 1;
+  if (!twod) {
+  } else if (ty == -1) {
     Ey[1] = 0.0;
     Ey[2] = -(PH(i, j + 1) - PH(i, j)) / dy[j];
   } else if (ty == 1) {
@@ -696,7 +715,8 @@ static void ptcl_eqn(osim* s, int i, int j, int stage, double* tm_out) {
     Ey[1] = -(PH(i, j) - PH(i, j - 1)) / dy[j - 1];
     Ey[2] = -(PH(i, j + 1) - PH(i, j)) / dy[j];
   }
-  if (ty == 0) {
+  if (!twod) {
+  } else if (ty == 0) {
     Te[1] = get_Te(NT(i, j - 1), NE(i, j - 1));
     Te[2] = get_Te(NT(i, j), NE(i, j));
     Te[3] = get_Te(NT(i, j + 1), NE(i, j + 1));
@@ -759,7 +779,8 @@ static void ptcl_eqn(osim* s, int i, int j, int stage, double* tm_out) {
   dfluxm_dx = (fluxm_x[2] - fluxm_x[1]) / s->dlx[i - 1];
   fluxe_x[1] = (fluxe_x[2] + fluxe_x[1]) / 2.0;
   Ex[1] = (Ex[1] + Ex[2]) / 2.0;
-  if (s->c.cyl) {
+  if (!twod) {
+  } else if (s->c.cyl) {
     dfluxi_dy = (fluxi_y[2] * (r[j + 1] + r[j]) / 2.0 - fluxi_y[1] * (r[j] + r[j - 1]) / 2.0) /
                 s->dly[j - 1] / r[j];
     dfluxe_dy = (fluxe_y[2] * (r[j + 1] + r[j]) / 2.0 - fluxe_y[1] * (r[j] + r[j - 1]) / 2.0) /
@@ -774,8 +795,10 @@ static void ptcl_eqn(osim* s, int i, int j, int stage, double* tm_out) {
     dfluxt_dy = (fluxt_y[2] - fluxt_y[1]) / s->dly[j - 1];
     dfluxm_dy = (fluxm_y[2] - fluxm_y[1]) / s->dly[j - 1];
   }
-  fluxe_y[1] = (fluxe_y[2] + fluxe_y[1]) / 2.0;
-  Ey[1] = (Ey[1] + Ey[2]) / 2.0;
+  if (twod) {
+    fluxe_y[1] = (fluxe_y[2] + fluxe_y[1]) / 2.0;
+    Ey[1] = (Ey[1] + Ey[2]) / 2.0;
+  }
 
   /* rates: ptcl_lib.F90:483-488 */
   k_ir = get_k_ir(K, Te[2]);
@@ -868,8 +891,13 @@ void o_circ_step(osim* s, int stage) {
   s->Ida = 0;
   for (int j = 2; j <= by + 1; j++) {
     if (TX(i - 2, j - 1) == -2) {
-      double dr = s->c.cyl ? s->dy[j - 1] * s->r[j - 1] * 2.0 * K->pi : s->dy[j - 1] * s->c.w;
-      if (!active && s->r[j] < s->c.ew) dr = 0.0;
+      double dr;
+      if (by > 1) {
+        dr = s->c.cyl ? s->dy[j - 1] * s->r[j - 1] * 2.0 * K->pi : s->dy[j - 1] * s->c.w;
+        if (!active && s->r[j] < s->c.ew) dr = 0.0;
+      } else {
+        dr = s->c.w * s->c.w * K->pi; /* circ_lib.F90:98-99, 131-132 */
+      }
       double Ex = -(PH(i, j) - PH(i - 1, j)) / s->dx[i - 1];
       double T1 = get_Te(NT(i - 1, j), NE(i - 1, j)), T2 = get_Te(NT(i, j), NE(i, j));
       double mue = 5e-1 * (get_mue(K, T1) + get_mue(K, T2));
@@ -883,8 +911,13 @@ void o_circ_step(osim* s, int stage) {
   s->Idc = 0;
   for (int j = 2; j <= by + 1; j++) {
     if (TX(i, j - 1) == 2) {
-      double dr = s->c.cyl ? s->dy[j - 1] * s->r[j - 1] * 2.0 * K->pi : s->dy[j - 1] * s->c.w;
-      if (!active && s->r[j] < s->c.ew) dr = 0.0;
+      double dr;
+      if (by > 1) {
+        dr = s->c.cyl ? s->dy[j - 1] * s->r[j - 1] * 2.0 * K->pi : s->dy[j - 1] * s->c.w;
+        if (!active && s->r[j] < s->c.ew) dr = 0.0;
+      } else {
+        dr = s->c.w * s->c.w * K->pi; /* circ_lib.F90:98-99, 131-132 */
+      }
       double Ex = -(PH(i + 1, j) - PH(i, j)) / s->dx[i];
       double T1 = get_Te(NT(i, j), NE(i, j)), T2 = get_Te(NT(i + 1, j), NE(i + 1, j));
       double mue = 5e-1 * (get_mue(K, T1) + get_mue(K, T2));
@@ -906,11 +939,16 @@ void o_circ_step(osim* s, int stage) {
     s->Vd_mi = s->Vsrc;
   }
   i = 2;
-  for (int j = 2; j <= by + 1; j++) { /* circ_lib.F90:180-198 */
-    if (s->r[j] <= s->c.ew) {
-      if (TX(i - 1, j - 1) == -2) PH(i - 1, j) = (stage == 5 ? s->Vd_pl : s->Vd_mi) / 2.0;
-      if (TX(bx, j - 1) == 2) PH(bx + 2, j) = (stage == 5 ? s->Vd_pl : s->Vd_mi) / (-2.0);
+  if (by > 1) {
+    for (int j = 2; j <= by + 1; j++) { /* circ_lib.F90:180-198 */
+      if (s->r[j] <= s->c.ew) {
+        if (TX(i - 1, j - 1) == -2) PH(i - 1, j) = (stage == 5 ? s->Vd_pl : s->Vd_mi) / 2.0;
+        if (TX(bx, j - 1) == 2) PH(bx + 2, j) = (stage == 5 ? s->Vd_pl : s->Vd_mi) / (-2.0);
+      }
     }
+  } else { /* one dimension: the source voltage itself, circ_lib.F90:199-202 */
+    if (TX(1, 1) == -2) PH(1, 2) = s->Vsrc / 2.0;
+    if (TX(bx, 1) == 2) PH(bx + 2, 2) = s->Vsrc / (-2.0);
   }
 #undef NI
 #undef NE
@@ -1156,69 +1194,73 @@ static double lapl_eqn(osim* s, int i, int j, const double* ne, const double* ni
   dflxe_x = (flxe[2] - flxe[1]) / s->dlx[i - 1];
   dflxi_x = (flxi[2] - flxi[1]) / s->dlx[i - 1];
 
-  if (ty == -1) {
-    Ey[1] = 0.0;
-    Ey[2] = -(PH(i, j + 1) - PH(i, j)) / dy[j];
-  } else if (ty == 1) {
-    Ey[1] = -(PH(i, j) - PH(i, j - 1)) / dy[j - 1];
-    Ey[2] = 0.0;
-  } else if (ty == 3) { /* dielectric wall: lapl_lib.F90:263-265 */
-    Ey[1] = -(PH(i, j) - PH(i, j - 1)) / dy[j - 1];
-    Ey[2] = -sig;
-  } else {
-    Ey[1] = -(PH(i, j) - PH(i, j - 1)) / dy[j - 1];
-    Ey[2] = -(PH(i, j + 1) - PH(i, j)) / dy[j];
-  }
-  if (ty == 0) {
-    Te[1] = get_Te(NT(i, j - 1), NE(i, j - 1));
-    Te[2] = get_Te(NT(i, j), NE(i, j));
-    Te[3] = get_Te(NT(i, j + 1), NE(i, j + 1));
-    mue[1] = 5e-1 * (get_mue(K, Te[1]) + get_mue(K, Te[2]));
-    mue[2] = 5e-1 * (get_mue(K, Te[2]) + get_mue(K, Te[3]));
-    De[1] = 5e-1 * (get_De(K, Te[1]) + get_De(K, Te[2]));
-    De[2] = 5e-1 * (get_De(K, Te[2]) + get_De(K, Te[3]));
-    flxi[1] = get_flux(K, Ey[1], dy[j - 1], 1, mui, Di, NI(i, j - 1), NI(i, j));
-    flxe[1] = get_flux(K, Ey[1], dy[j - 1], -1, mue[1], De[1], NE(i, j - 1), NE(i, j));
-    flxi[2] = get_flux(K, Ey[2], dy[j], 1, mui, Di, NI(i, j), NI(i, j + 1));
-    flxe[2] = get_flux(K, Ey[2], dy[j], -1, mue[2], De[2], NE(i, j), NE(i, j + 1));
-  } else if (ty < 0) {
-    Te[2] = get_Te(NT(i, j), NE(i, j));
-    Te[3] = get_Te(NT(i, j + 1), NE(i, j + 1));
-    mue[2] = 5e-1 * (get_mue(K, Te[2]) + get_mue(K, Te[3]));
-    De[2] = 5e-1 * (get_De(K, Te[2]) + get_De(K, Te[3]));
-    flxi[2] = get_flux(K, Ey[2], dy[j], 1, mui, Di, NI(i, j), NI(i, j + 1));
-    flxe[2] = get_flux(K, Ey[2], dy[j], -1, mue[2], De[2], NE(i, j), NE(i, j + 1));
-    flxi[1] = 0.0;
-    flxe[1] = 0.0;
-  } else if (ty > 0) {
-    Te[1] = get_Te(NT(i, j - 1), NE(i, j - 1));
-    Te[2] = get_Te(NT(i, j), NE(i, j));
-    mue[1] = 5e-1 * (get_mue(K, Te[1]) + get_mue(K, Te[2]));
-    De[1] = 5e-1 * (get_De(K, Te[1]) + get_De(K, Te[2]));
-    flxi[1] = get_flux(K, Ey[1], dy[j - 1], 1, mui, Di, NI(i, j - 1), NI(i, j));
-    flxe[1] = get_flux(K, Ey[1], dy[j - 1], -1, mue[1], De[1], NE(i, j - 1), NE(i, j));
-    if (ty == 3) { /* lapl_lib.F90:324-335 */
-      Te[2] = get_Te(NT(i, j), NE(i, j));
-      mue[2] = get_mue(K, Te[2]);
-      De[2] = get_De(K, Te[2]);
-      flxi[2] = get_flux(K, Ey[2], dy[j], 1, mui, Di, NI(i, j), 0.0);
-      flxe[2] = get_flux(K, Ey[2], dy[j], -1, mue[2], De[2], NE(i, j), 0.0);
+  /* y part: lapl_lib.F90:255-372, only for ny > 1 (in one dimension dfdy,
+   * dflxi_y and dflxe_y keep their initial zeros, lapl_lib.F90:128-130) */
+  if (s->by > 1) {
+    if (ty == -1) {
+      Ey[1] = 0.0;
+      Ey[2] = -(PH(i, j + 1) - PH(i, j)) / dy[j];
+    } else if (ty == 1) {
+      Ey[1] = -(PH(i, j) - PH(i, j - 1)) / dy[j - 1];
+      Ey[2] = 0.0;
+    } else if (ty == 3) { /* dielectric wall: lapl_lib.F90:263-265 */
+      Ey[1] = -(PH(i, j) - PH(i, j - 1)) / dy[j - 1];
+      Ey[2] = -sig;
     } else {
-      flxi[2] = 0.0;
-      flxe[2] = 0.0;
+      Ey[1] = -(PH(i, j) - PH(i, j - 1)) / dy[j - 1];
+      Ey[2] = -(PH(i, j + 1) - PH(i, j)) / dy[j];
     }
-  }
-  if (s->c.cyl) {
-    dfdy = -(Ey[2] * (r[j + 1] + r[j]) / 2.0 - Ey[1] * (r[j] + r[j - 1]) / 2.0) / s->dly[j - 1] /
-           r[j];
-    dflxi_y = (flxi[2] * (r[j + 1] + r[j]) / 2.0 - flxi[1] * (r[j] + r[j - 1]) / 2.0) /
-              s->dly[j - 1] / r[j];
-    dflxe_y = (flxe[2] * (r[j + 1] + r[j]) / 2.0 - flxe[1] * (r[j] + r[j - 1]) / 2.0) /
-              s->dly[j - 1] / r[j];
-  } else {
-    dfdy = -(Ey[2] - Ey[1]) / s->dly[j - 1];
-    dflxi_y = (flxi[2] - flxi[1]) / s->dly[j - 1];
-    dflxe_y = (flxe[2] - flxe[1]) / s->dly[j - 1];
+    if (ty == 0) {
+      Te[1] = get_Te(NT(i, j - 1), NE(i, j - 1));
+      Te[2] = get_Te(NT(i, j), NE(i, j));
+      Te[3] = get_Te(NT(i, j + 1), NE(i, j + 1));
+      mue[1] = 5e-1 * (get_mue(K, Te[1]) + get_mue(K, Te[2]));
+      mue[2] = 5e-1 * (get_mue(K, Te[2]) + get_mue(K, Te[3]));
+      De[1] = 5e-1 * (get_De(K, Te[1]) + get_De(K, Te[2]));
+      De[2] = 5e-1 * (get_De(K, Te[2]) + get_De(K, Te[3]));
+      flxi[1] = get_flux(K, Ey[1], dy[j - 1], 1, mui, Di, NI(i, j - 1), NI(i, j));
+      flxe[1] = get_flux(K, Ey[1], dy[j - 1], -1, mue[1], De[1], NE(i, j - 1), NE(i, j));
+      flxi[2] = get_flux(K, Ey[2], dy[j], 1, mui, Di, NI(i, j), NI(i, j + 1));
+      flxe[2] = get_flux(K, Ey[2], dy[j], -1, mue[2], De[2], NE(i, j), NE(i, j + 1));
+    } else if (ty < 0) {
+      Te[2] = get_Te(NT(i, j), NE(i, j));
+      Te[3] = get_Te(NT(i, j + 1), NE(i, j + 1));
+      mue[2] = 5e-1 * (get_mue(K, Te[2]) + get_mue(K, Te[3]));
+      De[2] = 5e-1 * (get_De(K, Te[2]) + get_De(K, Te[3]));
+      flxi[2] = get_flux(K, Ey[2], dy[j], 1, mui, Di, NI(i, j), NI(i, j + 1));
+      flxe[2] = get_flux(K, Ey[2], dy[j], -1, mue[2], De[2], NE(i, j), NE(i, j + 1));
+      flxi[1] = 0.0;
+      flxe[1] = 0.0;
+    } else if (ty > 0) {
+      Te[1] = get_Te(NT(i, j - 1), NE(i, j - 1));
+      Te[2] = get_Te(NT(i, j), NE(i, j));
+      mue[1] = 5e-1 * (get_mue(K, Te[1]) + get_mue(K, Te[2]));
+      De[1] = 5e-1 * (get_De(K, Te[1]) + get_De(K, Te[2]));
+      flxi[1] = get_flux(K, Ey[1], dy[j - 1], 1, mui, Di, NI(i, j - 1), NI(i, j));
+      flxe[1] = get_flux(K, Ey[1], dy[j - 1], -1, mue[1], De[1], NE(i, j - 1), NE(i, j));
+      if (ty == 3) { /* lapl_lib.F90:324-335 */
+        Te[2] = get_Te(NT(i, j), NE(i, j));
+        mue[2] = get_mue(K, Te[2]);
+        De[2] = get_De(K, Te[2]);
+        flxi[2] = get_flux(K, Ey[2], dy[j], 1, mui, Di, NI(i, j), 0.0);
+        flxe[2] = get_flux(K, Ey[2], dy[j], -1, mue[2], De[2], NE(i, j), 0.0);
+      } else {
+        flxi[2] = 0.0;
+        flxe[2] = 0.0;
+      }
+    }
+    if (s->c.cyl) {
+      dfdy = -(Ey[2] * (r[j + 1] + r[j]) / 2.0 - Ey[1] * (r[j] + r[j - 1]) / 2.0) / s->dly[j - 1] /
+             r[j];
+      dflxi_y = (flxi[2] * (r[j + 1] + r[j]) / 2.0 - flxi[1] * (r[j] + r[j - 1]) / 2.0) /
+                s->dly[j - 1] / r[j];
+      dflxe_y = (flxe[2] * (r[j + 1] + r[j]) / 2.0 - flxe[1] * (r[j] + r[j - 1]) / 2.0) /
+                s->dly[j - 1] / r[j];
+    } else {
+      dfdy = -(Ey[2] - Ey[1]) / s->dly[j - 1];
+      dflxi_y = (flxi[2] - flxi[1]) / s->dly[j - 1];
+      dflxe_y = (flxe[2] - flxe[1]) / s->dly[j - 1];
+    }
   }
   double b = dfdx + dfdy + NI(i, j) - NE(i, j) - s->dt * (dflxi_x + dflxi_y - dflxe_x - dflxe_y);
 #undef NE
@@ -1291,8 +1333,11 @@ static void lapl_j(osim* s, int i, int j, const double* ne, const double* ni, co
     dfxi_pl = 0.0;
     dfxe_pl = 0.0;
   }
-  /* y (ny > 1): lapl_lib.F90:462-564 */
-  if (ty >= 0) {
+  /* y: lapl_lib.F90:462-564, only for ny > 1 (one dimension: no y columns) */
+  const int twod = s->by > 1;
+  if (!twod) {
+    dfyi_mi = dfye_mi = dfyi_pl = dfye_pl = 0.0;
+  } else if (ty >= 0) {
     Te1 = get_Te(NT(i, j - 1), NE(i, j - 1));
     Te2 = get_Te(NT(i, j), NE(i, j));
     mue = 5e-1 * (get_mue(K, Te1) + get_mue(K, Te2));
@@ -1312,7 +1357,8 @@ static void lapl_j(osim* s, int i, int j, const double* ne, const double* ni, co
     dfyi_mi = 0.0;
     dfye_mi = 0.0;
   }
-  if (ty <= 0) {
+  if (!twod) {
+  } else if (ty <= 0) {
     Te1 = get_Te(NT(i, j), NE(i, j));
     Te2 = get_Te(NT(i, j + 1), NE(i, j + 1));
     mue = 5e-1 * (get_mue(K, Te1) + get_mue(K, Te2));
@@ -1338,28 +1384,30 @@ static void lapl_j(osim* s, int i, int j, const double* ne, const double* ni, co
     C = C - 1.0 / dx[i - 1] / dlx[i - 1];
   else
     C = C - 1.0 / dlx[i - 1] * (1.0 / dx[i] + 1.0 / dx[i - 1]);
-  if (s->c.cyl)
-    C = C + dt * ((dfyi_pl - dfye_pl) * (r[j + 1] + r[j]) / 2.0 -
-                  (dfyi_mi - dfye_mi) * (r[j] + r[j - 1]) / 2.0) /
-                dly[j - 1] / r[j];
-  else
-    C = C + dt * (dfyi_pl - dfyi_mi - dfye_pl + dfye_mi) / dly[j - 1];
-  if (ty == -1) {
+  if (twod) { /* lapl_lib.F90:577-607 */
     if (s->c.cyl)
-      C = C - 1.0 / dy[j - 1] / dly[j - 1] * (r[j + 1] + r[j]) / 2.0 / r[j];
+      C = C + dt * ((dfyi_pl - dfye_pl) * (r[j + 1] + r[j]) / 2.0 -
+                    (dfyi_mi - dfye_mi) * (r[j] + r[j - 1]) / 2.0) /
+                  dly[j - 1] / r[j];
     else
-      C = C - 1.0 / dy[j - 1] / dly[j - 1];
-  } else if (ty == 1) {
-    if (s->c.cyl)
-      C = C - 1.0 / dy[j - 1] / dly[j - 1] * (r[j] + r[j - 1]) / 2.0 / r[j];
-    else
-      C = C - 1.0 / dy[j - 1] / dly[j - 1];
-  } else {
-    if (s->c.cyl)
-      C = C - 5e-1 * ((r[j + 1] + r[j]) / dy[j] + (r[j] + r[j - 1]) / dy[j - 1]) / dly[j - 1] /
-                  r[j];
-    else
-      C = C - 1.0 / dly[j - 1] * (1.0 / dy[j] + 1.0 / dy[j - 1]);
+      C = C + dt * (dfyi_pl - dfyi_mi - dfye_pl + dfye_mi) / dly[j - 1];
+    if (ty == -1) {
+      if (s->c.cyl)
+        C = C - 1.0 / dy[j - 1] / dly[j - 1] * (r[j + 1] + r[j]) / 2.0 / r[j];
+      else
+        C = C - 1.0 / dy[j - 1] / dly[j - 1];
+    } else if (ty == 1) {
+      if (s->c.cyl)
+        C = C - 1.0 / dy[j - 1] / dly[j - 1] * (r[j] + r[j - 1]) / 2.0 / r[j];
+      else
+        C = C - 1.0 / dy[j - 1] / dly[j - 1];
+    } else {
+      if (s->c.cyl)
+        C = C - 5e-1 * ((r[j + 1] + r[j]) / dy[j] + (r[j] + r[j - 1]) / dy[j - 1]) / dly[j - 1] /
+                    r[j];
+      else
+        C = C - 1.0 / dly[j - 1] * (1.0 / dy[j] + 1.0 / dy[j - 1]);
+    }
   }
   A[4] = C;
 #undef NE

@@ -30,14 +30,14 @@ def test_fortran_host_builds_and_links():
 @pytest.mark.skipif(not HAVE_FC, reason="amdflang not available")
 def test_fortran_driver_flag_errors():
     """read_in's checks stop before any device call: an odd argument count
-    prints the usage, px * py /= 1, ny = 1 (1-D) and unknown flags stop with
-    their codes."""
+    prints the usage, px * py /= 1, ny < 1 and unknown flags stop with their
+    codes."""
     from fluidsim_amd import build
 
     exe = build.build_fortran()
     r = subprocess.run([exe, "-nx"], capture_output=True, text=True, timeout=60)
     assert "Usage" in r.stdout and "-wall" in r.stdout and "-n0" in r.stdout
-    for args, code in ((["-px", "2"], 2), (["-ny", "1"], 3), (["-bogus", "1"], 2)):
+    for args, code in ((["-px", "2"], 2), (["-ny", "0"], 3), (["-bogus", "1"], 2)):
         r = subprocess.run([exe, *args], capture_output=True, text=True, timeout=60)
         assert r.returncode == code, (args, r.returncode, r.stdout)
 
@@ -167,3 +167,47 @@ def test_fortran_uniform_start_and_dielectric_wall(tmp_path):
     os.makedirs(res)
     _run(args + ["-resident", "T"], res)
     _same_outputs(out, res, nx * ny)
+
+
+@pytest.mark.gpu
+def test_fortran_one_dimensional_default(tmp_path):
+    """The reference's default mode, ny = 1 (main.F90:29): no -ny flag, the
+    uniform start, the DC source; the host's accept / reject block on the
+    module arrays against the library's loop bitwise, the device-resident
+    mode, and meshy.dat left empty (props.F90:262-264)."""
+    if not os.path.exists(EXE):
+        pytest.skip("Fortran host not built")
+    from fluidsim_amd import Discharge, configs
+    from fluidsim_amd.discharge import constants
+
+    nx, steps, n0 = 40, 40, 3e10
+    args = ["-nx", str(nx), "-p", "1.0", "-v", "200", "-r", "1e5", "-ic", "0", "-n0", str(n0),
+            "-steps", str(steps), "-rtol", "1e-13", "-l", "0.01", "-w", "0.01", "-ew", "0.01"]
+    out = str(tmp_path) + "/mirror/"
+    os.makedirs(out)
+    _run(args, out)
+    assert os.path.getsize(out + "meshy.dat") == 0
+
+    cfg = configs.base(nx, 1, p_torr=1.0, volts=200.0, res_ohm=1e5, rtol=1e-13, grid_x=0)
+    d = Discharge(cfg)
+    tab = constants(1.0, 0.1)
+    n = n0 * configs.X0 ** 3
+    Tg, kb, e, ph0 = tab[11], tab[4], tab[3], tab[7]
+    for lev in (1, 2, 3):
+        for name in ("ni", "ne", "nm"):
+            d.set(name, lev, np.full((3, nx + 2), n))
+        d.set("nt", lev, np.full((3, nx + 2), n) * Tg * kb / e / ph0)
+    while d.scalars()["ts"] < steps:
+        d.step()
+    x0 = configs.X0
+    f2 = _read_frames(out, "f2.dat", nx)[-1]
+    f1 = _read_frames(out, "f1.dat", nx)[-1]
+    assert np.array_equal(f2, d.get("ne", 1)[1, 1:-1] / x0 ** 3)
+    assert np.array_equal(f1, d.get("ph")[1, 1:-1] * configs.PH0)
+    assert np.fromfile(out + "time.dat")[-1] == d.scalars()["t"]
+    assert np.fromfile(out + "ida.dat")[-1] != 0.0
+
+    res = str(tmp_path) + "/resident/"
+    os.makedirs(res)
+    _run(args + ["-resident", "T"], res)
+    _same_outputs(out, res, nx)
