@@ -127,15 +127,23 @@ def same_method_gpu(d, snap, args, passes=5):
 
 
 def learned_setup(args):
-    """Harvest residuals from seeded runs of the 512^2 pulse workload, train the
-    network on the device (the library's own kernels) and return it."""
-    from fluidsim_amd.learn import LearnedPreconditioner, harvest_runs, train_schedule
+    """Harvest residuals from seeded runs of the 512^2 pulse workload (three
+    quarters of the set) and of the parameter-sweep ensemble (a quarter), train
+    the network on the device (the library's own kernels) and return it."""
+    from fluidsim_amd.learn import (LearnedPreconditioner, harvest_ensemble, harvest_runs,
+                                    train_schedule)
 
     net = LearnedPreconditioner(args.learn_samples, 64, 64, seed=1)
-    th = harvest_runs(net, 512, seeds=(0, 1, 2, 3), samples=args.learn_samples)
+    n_ens = args.learn_samples // 4
+    th = harvest_runs(net, 512, seeds=(0, 1, 2, 3), samples=args.learn_samples - n_ens)
+    n1 = net.samples
+    he = harvest_ensemble(net, 32, seeds=(0, 1), samples=n_ens)
     steps = args.learn_steps
     loss, tt = train_schedule(net, ((steps // 2, 16, 1e-3), (steps - steps // 2, 16, 2.5e-4)))
-    return net, {"samples": net.samples, "harvest_s": th, "steps": steps, "batch": 16,
+    return net, {"samples": net.samples, "samples_config1": n1,
+                 "samples_ensemble": net.samples - n1, "ensemble_members": he["members_used"],
+                 "ensemble_members_skipped": he["members_skipped"],
+                 "harvest_s": th + he["seconds"], "steps": steps, "batch": 16,
                  "train_s": tt, "loss_first": float(loss[:50].mean()),
                  "loss_last": float(loss[-50:].mean())}
 

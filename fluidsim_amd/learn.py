@@ -129,6 +129,49 @@ def harvest_runs(net: LearnedPreconditioner, n: int, seeds, samples: int, warm: 
     return time.perf_counter() - t0
 
 
+def harvest_ensemble(net: LearnedPreconditioner, members: int, seeds, samples: int,
+                     warm: int = 4, every: int = 2) -> dict:
+    """Add `samples` residuals from the multigrid solves of the parameter-sweep
+    ensemble (sides 64..256, DC 150..400 V, 0.5..10 Torr): for each seed an
+    ensemble of `members` discharges runs `warm` passes, then one solve of
+    every member is harvested every `every` passes.  Members whose learned
+    level does not fit the network's grid (a side above 2 x its width) are
+    skipped.  Returns seconds, members used and skipped."""
+    import time
+
+    from . import configs
+    from .ensemble import Ensemble
+
+    t0 = time.perf_counter()
+    target0 = net.samples
+    used, skipped = set(), set()
+    for k, seed in enumerate(seeds):
+        cfgs = configs.config3(members, base_seed=int(seed) * members)
+        E = Ensemble(cfgs)
+        for m, c in zip(E.members, cfgs):
+            m.init_gaussian(seed=c["seed"])
+        E.run(warm)
+        goal = target0 + samples * (k + 1) // len(seeds)
+        while net.samples < goal:
+            E.run(every)
+            got = 0
+            for i, m in enumerate(E.members):
+                if (k, i) in skipped:
+                    continue
+                try:
+                    got += net.harvest(m)
+                    used.add((k, i))
+                except _L.FsError:
+                    skipped.add((k, i))
+                if net.samples >= goal:
+                    break
+            if got == 0:
+                break
+        E.close()
+    return {"seconds": time.perf_counter() - t0, "members_used": len(used),
+            "members_skipped": len(skipped)}
+
+
 def train_schedule(net: LearnedPreconditioner, phases=((1500, 16, 1e-3), (1500, 16, 2.5e-4)),
                    seed: int = 0):
     """Adam phases of (steps, batch, learning rate); returns (losses, seconds)."""

@@ -122,6 +122,34 @@ def test_learned_flexible_cg_reaches_the_plain_solution():
     assert rel_l2(res["learned"][3][1:-1, 1:-1], ref) <= 1e-6
 
 
+def test_harvest_from_ensemble_members():
+    """Residuals harvested from the solves of parameter-sweep ensemble members
+    (the training set mixes them with the 512^2 pulse's): every member whose
+    learned level fits the network's grid contributes samples, a member of a
+    wider grid is skipped, and a network trained on the mixed set keeps the
+    learned solve's FP64 residual on a fresh discharge."""
+    from fluidsim_amd import Discharge, configs
+    from fluidsim_amd.learn import LearnedPreconditioner, harvest_ensemble
+
+    net = LearnedPreconditioner(3000, 64, 64, seed=1)
+    info = harvest_ensemble(net, 6, seeds=(0,), samples=400)
+    print("ensemble harvest", info, "samples", net.samples)
+    assert net.samples >= 400
+    # members 198x107, 254x139, 108x193, 205x192, 155x198 have learned levels
+    # of at most 64 x 64 (50x27 ... 52x48); 201x79's is 101x40
+    assert info["members_used"] == 5 and info["members_skipped"] == 1
+    loss = net.train(200, batch=8, lr=2e-3, seed=2)
+    assert np.all(np.isfinite(loss)) and loss[-20:].mean() < loss[:20].mean()
+    d = Discharge(configs.config0(rtol=1e-10))
+    d.init_gaussian(seed=9)
+    d.run(3)
+    net.attach(d)
+    d.set_solver(2)
+    lerr, st = d.lapl_solve()
+    assert lerr == 0 and st.resid <= 1e-10
+    d.set_solver(1)
+
+
 def test_learned_graph_replay_equals_eager_launches():
     """The learned iterations replayed from a captured HIP graph give bitwise
     the same correction as the same kernels launched one by one."""
