@@ -1594,18 +1594,36 @@ __device__ __forceinline__ void gs_stage(const MgLevel& F, GsTile<H>& s, int i0,
 }
 
 // One Gauss-Seidel half-sweep over the cells of colour c ((i + j) & 1 == c)
-// within h <= H - 1 cells of the tile at (i0, j0), entirely in LDS.
+// within h <= H - 1 cells of the tile at (i0, j0), entirely in LDS.  A thread
+// updates up to kGsHalfIt cells; cells of one colour do not read each other,
+// so every cell's operands are read before any new value is written, and the
+// LDS reads of all of a thread's cells are in flight together.
+template <int H>
+constexpr int kGsHalfIt =
+    ((kGsTY + 2 * (H - 1)) * ((kGsTX + 2 * (H - 1)) / 2) + kNT - 1) / kNT;
 template <int H>
 __device__ __forceinline__ void gs_half(GsTile<H>& s, int i0, int j0, int c, int h) {
   #pragma clang fp contract(fast)  // smoother arithmetic: FMAs allowed
   const int w2 = (kGsTX + 2 * h) / 2, rows = kGsTY + 2 * h, a0 = H - h;
-  for (int k = threadIdx.x; k < rows * w2; k += kNT) {
-    const int b = a0 + k / w2;
-    const int a = a0 + 2 * (k % w2) + ((c + i0 - H + a0 + j0 - H + b) & 1);
-    s.x[b][a] = (s.r[b][a] + s.cx[b][a - 1] * s.x[b][a - 1] + s.cx[b][a] * s.x[b][a + 1] +
-                 s.cy[b - 1][a] * s.x[b - 1][a] + s.cy[b][a] * s.x[b + 1][a]) *
-                s.dinv[b][a];
+  constexpr int NI = kGsHalfIt<H>;
+  double v[NI];
+  int at[NI];
+#pragma unroll
+  for (int q = 0; q < NI; q++) {
+    const int k = threadIdx.x + kNT * q;
+    at[q] = -1;
+    if (k < rows * w2) {
+      const int b = a0 + k / w2;
+      const int a = a0 + 2 * (k % w2) + ((c + i0 - H + a0 + j0 - H + b) & 1);
+      at[q] = b * GsTile<H>::W + a;
+      v[q] = (s.r[b][a] + s.cx[b][a - 1] * s.x[b][a - 1] + s.cx[b][a] * s.x[b][a + 1] +
+              s.cy[b - 1][a] * s.x[b - 1][a] + s.cy[b][a] * s.x[b + 1][a]) *
+             s.dinv[b][a];
+    }
   }
+#pragma unroll
+  for (int q = 0; q < NI; q++)
+    if (at[q] >= 0) (&s.x[0][0])[at[q]] = v[q];
 }
 
 template <int H>
@@ -1944,9 +1962,23 @@ __device__ __forceinline__ void mg_down_body(const MgArgs& A, int bid, int nb) {
     if (kCg) cg_update(A, s, i0, j0, cc, alpha, beta, racc[0]);
     __syncthreads();
     // first (red) half-sweep from zero is pointwise: the whole staged halo
-    for (int k = threadIdx.x; k < R * W; k += kNT) {
-      const int b = k / W, a = k % W;
-      s.x[b][a] = ((i0 + a + j0 + b) & 1) == 0 ? s.r[b][a] * s.dinv[b][a] : 0.0;
+    // (every read of a thread's cells before its writes)
+    {
+      constexpr int NP = (R * W + kNT - 1) / kNT;
+      double v[NP];
+#pragma unroll
+      for (int q = 0; q < NP; q++) {
+        const int k = threadIdx.x + kNT * q;
+        if (k < R * W) {
+          const int b = k / W, a = k % W;
+          v[q] = ((i0 + a + j0 + b) & 1) == 0 ? s.r[b][a] * s.dinv[b][a] : 0.0;
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < NP; q++) {
+        const int k = threadIdx.x + kNT * q;
+        if (k < R * W) (&s.x[0][0])[k] = v[q];
+      }
     }
     __syncthreads();
     gs_half(s, i0, j0, 1, 3);
