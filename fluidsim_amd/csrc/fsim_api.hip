@@ -757,6 +757,16 @@ int mg_prologue(fs_handle* h, bool warm) {
   return launch_vcycle(h, -1) || launch_vcycle(h, 0) ? FS_E_HIP : 0;
 }
 
+// The graph of 2^m MG-PCG iterations from an iteration of parity p.
+int capture_chain(fs_handle* h, int p, int m) {
+  const int base = p ? 1 : 2;
+  return capture(h, &h->mg_chain[p][m], [&]() {
+    for (int k = 0; k < (1 << m); k++)
+      if (launch_vcycle(h, base + k)) return FS_E_HIP;
+    return 0;
+  });
+}
+
 // MG-PCG iterations it .. it+n-1 (it >= 1): with graphs, as chains of 2^m
 // iterations (largest first), so a chunk costs a few graph launches.
 int launch_mg_chunk(fs_handle* h, int it, int n) {
@@ -769,13 +779,7 @@ int launch_mg_chunk(fs_handle* h, int it, int n) {
     int m = kChainLog - 1;
     while ((1 << m) > n) m--;
     hipGraphExec_t& g = h->mg_chain[it & 1][m];
-    const int base = (it & 1) ? 1 : 2;
-    if (!g && capture(h, &g, [&]() {
-          for (int k = 0; k < (1 << m); k++)
-            if (launch_vcycle(h, base + k)) return FS_E_HIP;
-          return 0;
-        }))
-      return FS_E_HIP;
+    if (!g && capture_chain(h, it & 1, m)) return FS_E_HIP;
     HIP_OK(hipGraphLaunch(g, h->st));
     it += 1 << m;
     n -= 1 << m;
@@ -914,6 +918,35 @@ int first_chunk(int last, int dflt) { return last > 0 ? last : dflt; }
 // with its finishing kernels are enqueued, the assembly skipping the whole
 // solve on the device unless the attempt is accepted; after the host has read
 // the scalars, mode 2 continues that solve.  Mode 0: a whole solve.
+// Every multigrid-PCG graph of the handle (set-up alone, set-up with the first
+// chunk and the finishing kernels, iteration chains; from zero and warm),
+// captured together at the handle's first graphed multigrid solve, so that no
+// later solve waits for a capture when its iteration count first takes a new
+// value.
+int ensure_mg_graphs(fs_handle* h) {
+  if (h->mg_pro[0]) return 0;
+  PcgArgs pa = pcg_args(h, 0);
+  pa.part_out = h->part(2);
+  pa.part_in = h->part(2);
+  for (int w = 0; w < 2; w++) {
+    const bool warm = w == 1;
+    if (capture(h, &h->mg_pro[w], [&]() { return mg_prologue(h, warm); })) return FS_E_HIP;
+    for (int n = 1; n <= kSolveGraphs; n++)
+      if (capture(h, &h->mg_solve[w][n - 1], [&]() {
+            if (mg_prologue(h, warm)) return FS_E_HIP;
+            for (int k = 0; k < n; k++)
+              if (launch_vcycle(h, 1 + k)) return FS_E_HIP;
+            launch_pcg_finish(pa, h->st);
+            return 0;
+          }))
+        return FS_E_HIP;
+  }
+  for (int p = 0; p < 2; p++)
+    for (int m = 0; m < kChainLog; m++)
+      if (capture_chain(h, p, m)) return FS_E_HIP;
+  return 0;
+}
+
 bool can_issue_ahead(const fs_handle* h) {
   return h->p.use_graphs && h->p.lapl_solver == 1 && h->mg.size() > 1;
 }
@@ -989,17 +1022,9 @@ int lapl_solve_impl(fs_handle* h, int* lerr, fs_solve_stats* stats, int mode = 0
       // boundary costs microseconds of idle device).
       mg_it = 1;
       mg_chunk = std::max(1, first_chunk(h->last_iters[1], chunk > 8 ? chunk / 4 : chunk) - 1);
+      if (h->p.use_graphs && ensure_mg_graphs(h)) return FS_E_HIP;
       if (h->p.use_graphs && mg_chunk <= kSolveGraphs) {
         hipGraphExec_t& g = h->mg_solve[warm ? 1 : 0][mg_chunk - 1];
-        const int n = mg_chunk;
-        if (!g && capture(h, &g, [&]() {
-              if (mg_prologue(h, warm)) return FS_E_HIP;
-              for (int k = 0; k < n; k++)
-                if (launch_vcycle(h, 1 + k)) return FS_E_HIP;
-              launch_pcg_finish(pa, h->st);
-              return 0;
-            }))
-          return FS_E_HIP;
         HIP_OK(hipGraphLaunch(g, h->st));
         time_mark(h, 6);
         if (mode == 0 && sync_scalars(h)) return FS_E_HIP;
@@ -1007,9 +1032,7 @@ int lapl_solve_impl(fs_handle* h, int* lerr, fs_solve_stats* stats, int mode = 0
         if (!h->p.use_graphs) {
           if (mg_prologue(h, warm)) return FS_E_HIP;
         } else {
-          hipGraphExec_t& g = h->mg_pro[warm ? 1 : 0];
-          if (!g && capture(h, &g, [&]() { return mg_prologue(h, warm); })) return FS_E_HIP;
-          HIP_OK(hipGraphLaunch(g, h->st));
+          HIP_OK(hipGraphLaunch(h->mg_pro[warm ? 1 : 0], h->st));
         }
         if (launch_mg_chunk(h, mg_it, mg_chunk)) return FS_E_HIP;
         if (int e = end_chunk(mode == 0)) return e;
