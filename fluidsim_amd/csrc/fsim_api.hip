@@ -200,6 +200,7 @@ struct fs_handle {
   bool solve_warm = false;            // the MG-PCG solve in flight started from the last correction
   int mg_it = 0, mg_chunk = 0;        // next MG-PCG iteration / chunk of a solve in progress
   hipGraphExec_t mg_chain[2][kChainLog] = {};  // 2^m MG-PCG iterations from parity p
+  bool mg_graphs = false;             // every mg_pro / mg_solve / mg_chain graph is captured
   hipGraphExec_t learn_graph[2] = {}; // one learned-FCG iteration, by parity
   // what the learned graphs were captured over: the network (or the handle:
   // none attached) and the allocation number of its inference activations,
@@ -792,6 +793,7 @@ void drop_graphs(fs_handle* h) {
     if (g) (void)hipGraphExecDestroy(g);
     g = nullptr;
   };
+  h->mg_graphs = false;
   drop(h->mg_pro[0]);
   drop(h->mg_pro[1]);
   for (int k = 0; k < 2; k++) {
@@ -924,7 +926,8 @@ int first_chunk(int last, int dflt) { return last > 0 ? last : dflt; }
 // later solve waits for a capture when its iteration count first takes a new
 // value.
 int ensure_mg_graphs(fs_handle* h) {
-  if (h->mg_pro[0]) return 0;
+  if (h->mg_graphs) return 0;
+  drop_graphs(h);
   PcgArgs pa = pcg_args(h, 0);
   pa.part_out = h->part(2);
   pa.part_in = h->part(2);
@@ -944,6 +947,7 @@ int ensure_mg_graphs(fs_handle* h) {
   for (int p = 0; p < 2; p++)
     for (int m = 0; m < kChainLog; m++)
       if (capture_chain(h, p, m)) return FS_E_HIP;
+  h->mg_graphs = true;
   return 0;
 }
 
