@@ -167,6 +167,9 @@ __global__ void __launch_bounds__(256) k_conv(ConvArgs A) {
     }
 }
 
+// Partial sums of a weight gradient: at most this many pixel ranges.
+constexpr int kWgradParts = 256;
+
 // Weight gradient dW[t][o][i] = sum_p x(p + tap t)[i] dpre(p)[o]: GEMM with
 // M = output channels, N = input channels, K = pixels.  Block (gx, gy): pixel
 // chunks of 64 in the range of gx, 16 (tap, o-tile, i-tile) combinations of gy
@@ -535,9 +538,10 @@ void wgrad(Net* net, const Acts& a, int l, hipStream_t st) {
   const int cin = layer_cin(l), len = 9 * w.cout * cin;
   const int ncombo = 9 * (w.cout / 16) * (cin / 16);
   const int total = a.N * (a.H >> kNet[l].level) * (a.W >> kNet[l].level);
-  int gx = ceil_div(total, 64 * 16);  // at least 16 chunks of 64 pixels per block
+  // pixel chunks of 64, at least 4 per block, at most kWgradParts blocks along x
+  int gx = ceil_div(total, 64 * 4);
   const int cap = (int)(net->wpart_len / (size_t)len);
-  gx = gx < 1 ? 1 : gx > 64 ? 64 : gx;
+  gx = gx < 1 ? 1 : gx > kWgradParts ? kWgradParts : gx;
   if (gx > cap) gx = cap;
   hipLaunchKernelGGL(k_wgrad, dim3(gx, ceil_div(ncombo, 16)), dim3(256), 0, st, w);
   hipLaunchKernelGGL(k_wgrad_reduce, dim3(ceil_div(len, 256)), dim3(256), 0, st, net->wpart, gx,
@@ -714,7 +718,7 @@ int net_init(Net* net, int cap, int H, int W, uint64_t seed, hipStream_t st) {
     const size_t len = (size_t)9 * kNet[l].cout * layer_cin(l);
     maxlen = len > maxlen ? len : maxlen;
   }
-  net->wpart_len = 64 * maxlen;
+  net->wpart_len = kWgradParts * maxlen;
   HIPC(hipMalloc(&net->w, o * 4));
   HIPC(hipMalloc(&net->g, o * 4));
   HIPC(hipMalloc(&net->m, o * 4));
