@@ -115,13 +115,18 @@ __device__ __forceinline__ float conv_in1(const ConvArgs& A, int p, int t, int c
   return bf2f(A.xb[(((size_t)n * A.H + hh) * A.W + ww) * A.cb + (c - A.ca)]);
 }
 
-// Forward (and input-gradient) 3x3 convolution with NT = cout / 16 output
-// tiles and CB = cin / 16 input-channel blocks, fully unrolled so that the
-// operand loads of all taps can be issued ahead of the MFMA chain.
+// Forward (and input-gradient) 3x3 convolution: each wave computes NT output
+// tiles of 16 channels (from tile NT * blockIdx.y on; cout = 16 * NT *
+// gridDim.y) for 16 pixels, over CB = cin / 16 input-channel blocks, fully
+// unrolled so that the operand loads of all taps are issued ahead of the MFMA
+// chain.  The network application (one sample) launches one tile per wave
+// and the output tiles along y, so that the deep layers' few pixels still
+// spread over many waves with short MFMA chains.
 template <int MODE, int NT, int CB>
 __global__ void __launch_bounds__(256) k_conv(ConvArgs A) {
   constexpr int cin = 16 * CB;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int co0 = (int)blockIdx.y * NT * 16;
   const int HW = A.H * A.W, total = A.N * HW;
   const int p0 = (blockIdx.x * 4 + wave) * 16;
   if (p0 >= total) return;  // whole wave; the kernel has no barriers
@@ -137,7 +142,7 @@ __global__ void __launch_bounds__(256) k_conv(ConvArgs A) {
   for (int t = 0; t < 9; t++) {
     const int hh = h + t / 3 - 1, ww = w + t % 3 - 1;
     const bool ok = pin && hh >= 0 && hh < A.H && ww >= 0 && ww < A.W;
-    const uint16_t* wt = A.w + ((size_t)t * (16 * NT) + col) * cin + 4 * kg;
+    const uint16_t* wt = A.w + ((size_t)t * A.cout + co0 + col) * cin + 4 * kg;
     bf4 a[CB], b[CB][NT];
 #pragma unroll
     for (int cb = 0; cb < CB; cb++) {
@@ -157,8 +162,8 @@ __global__ void __launch_bounds__(256) k_conv(ConvArgs A) {
     for (int i = 0; i < 4; i++) {
       const int p = p0 + 4 * kg + i;
       if (p >= total) continue;
-      const int co = nt * 16 + col;
-      const size_t o = (size_t)p * (16 * NT) + co;
+      const int co = co0 + nt * 16 + col;
+      const size_t o = (size_t)p * A.cout + co;
       float v = acc[nt][i];
       if (A.relu) v = fmaxf(v, 0.0f);
       if (A.mask && !(bf2f(A.mask[o]) > 0.0f)) v = 0.0f;
@@ -484,12 +489,16 @@ __global__ void k_nn_store(MgLevel L, int H, int W, float* out) {
   X(kPlain, 1, 1) X(kPlain, 2, 2) X(kPlain, 3, 3) X(kPlain, 4, 4) X(kPlain, 1, 2)      \
   X(kPlain, 2, 3) X(kPlain, 3, 4) X(kPlain, 7, 3) X(kPlain, 5, 2) X(kPlain, 3, 1)      \
   X(kPool, 2, 1) X(kPool, 3, 2) X(kPool, 4, 3) X(kUpCat, 3, 7) X(kUpCat, 2, 5)         \
-  X(kUpCat, 1, 3)
+  X(kUpCat, 1, 3) X(kPlain, 1, 3) X(kPlain, 1, 4) X(kPool, 1, 1) X(kPool, 1, 2)        \
+  X(kPool, 1, 3) X(kUpCat, 1, 5) X(kUpCat, 1, 7)
 
-int launch_conv(const ConvArgs& a, hipStream_t st) {
+// split: one output tile per wave, the tiles along the grid's y (the
+// network application); otherwise every tile in each wave (training batches).
+int launch_conv(const ConvArgs& a, hipStream_t st, bool split = false) {
   const int total = a.N * a.H * a.W;
-  const dim3 g((total + 63) / 64), b(256);
-  const int nt = a.cout / 16, cb = (a.mode == kUpCat ? a.ca + a.cb : a.ca) / 16;
+  const int ntile = a.cout / 16, cb = (a.mode == kUpCat ? a.ca + a.cb : a.ca) / 16;
+  const int nt = split ? 1 : ntile;
+  const dim3 g((total + 63) / 64, split ? ntile : 1), b(256);
 #define FSL_CONV_LAUNCH(M, NT, CB)                                        \
   if (a.mode == M && nt == NT && cb == CB) {                              \
     hipLaunchKernelGGL((k_conv<M, NT, CB>), g, b, 0, st, a);              \
@@ -520,9 +529,9 @@ ConvArgs fwd_args(const Net* net, const Acts& a, int l) {
   return c;
 }
 
-int forward(const Net* net, const Acts& a, hipStream_t st) {
+int forward(const Net* net, const Acts& a, hipStream_t st, bool split = false) {
   for (int l = 0; l < kLayers; l++)
-    if (launch_conv(fwd_args(net, a, l), st)) return -1;
+    if (launch_conv(fwd_args(net, a, l), st, split)) return -1;
   return 0;
 }
 
@@ -778,7 +787,7 @@ int net_apply(Net* net, const MgLevel& L, const double* part, int nb, DevScalars
   }
   hipLaunchKernelGGL(k_nn_input, dim3(ceil_div((size_t)H * W, 256)), dim3(256), 0, st, L, part,
                      nb, sc, H, W, a.in);
-  if (forward(net, a, st)) return -1;
+  if (forward(net, a, st, true)) return -1;
   hipLaunchKernelGGL(k_nn_output, dim3(ceil_div((size_t)L.nx * L.ny, 256)), dim3(256), 0, st,
                      a.y[13], net->w + net->off[kLayers], L, sc, W);
   return hipGetLastError() == hipSuccess ? 0 : -1;
@@ -849,7 +858,9 @@ int loss_grad(Net* net, int B, int H, int W, const float* in5, const float* rhat
   HIPC(hipMemcpyAsync(a.in, in.data(), in.size() * 2, hipMemcpyHostToDevice, st));
   HIPC(hipMemcpyAsync(a.rhat, rhat, n0 * 4, hipMemcpyHostToDevice, st));
   HIPC(hipMemcpyAsync(a.op, op, n0 * 3 * 4, hipMemcpyHostToDevice, st));
-  if (forward(net, a, st)) return -1;
+  // a single sample runs the forward in the network application's launch
+  // shape (one output tile per wave), a batch in the training shape
+  if (forward(net, a, st, B == 1)) return -1;
   hipLaunchKernelGGL(k_out_fwd, dim3(ceil_div(n0, 256)), dim3(256), 0, st, a.y[13],
                      net->w + net->off[kLayers], a.out, (int)n0);
   const int nbl = ceil_div(n0, 256);

@@ -55,6 +55,12 @@ def test_device_network_matches_cpu_reference():
     # order is the only difference left (both sides round to BF16 at the same
     # points); on the CPU network, leaving out every BF16 rounding moves the
     # output by 6e-3 and transposing one layer's 3x3 taps by 0.1 .. 0.9.
+    # Not every input stays this close: one 16x24 sample drawn from another
+    # seed differed by 1.2e-3 (in both launch shapes; 32x32 and 16x16 samples
+    # by 5e-9 and 1.4e-8).  The likely cause, not examined further: an FP32
+    # sum that lies within its rounding of a BF16 rounding boundary rounds to
+    # the neighbouring BF16 value on one side (one BF16 ulp, 0.4 %), and a deep
+    # layer spreads that element over the output.
     print("out rel L2", rel_l2(out, e), "loss", loss, L)
     assert rel_l2(out, e) <= 5e-6
     assert loss == pytest.approx(L, rel=5e-6, abs=1e-9)
@@ -65,6 +71,24 @@ def test_device_network_matches_cpu_reference():
         print(f"layer {l} grad rel L2 {err:.2e}")
         assert err <= 1.5e-6, (l, err)
     assert rel_l2(grad, g) <= 1e-6
+
+
+@pytest.mark.parametrize("H,W", [(16, 24), (64, 64)])
+def test_application_launch_shape_equals_training_shape(H, W):
+    """A single sample runs the forward in the network application's launch
+    shape (one output tile per wave, the tiles along the grid's y), a batch in
+    the training shape (every tile in each wave): each output is the same
+    MFMA chain in the same order, so the two agree bitwise."""
+    from fluidsim_amd.learn import LearnedPreconditioner
+
+    rng = np.random.default_rng(3)
+    net = LearnedPreconditioner(0, 8, 8, seed=7)
+    in5, rhat, op = _batch(rng, 1, H, W)
+    _, _, out1 = net.loss_grad(in5, rhat, op)
+    dup = lambda a: np.concatenate([a, a])
+    _, _, out2 = net.loss_grad(dup(in5), dup(rhat), dup(op))
+    assert np.array_equal(out1[0], out2[0]) and np.array_equal(out2[0], out2[1])
+    assert np.abs(out1).max() > 0
 
 
 def _harvested(nx=64, steps=24, seed=0):
