@@ -358,7 +358,7 @@ def main():
     ap.add_argument("--cold", action="store_true",
                     help="start every multigrid-PCG solve of the run from zero (no warm start)")
     ap.add_argument("--learn-samples", type=int, default=20000)
-    ap.add_argument("--learn-steps", type=int, default=3000)
+    ap.add_argument("--learn-steps", type=int, default=6000)
     ap.add_argument("--learned-n2", type=int, default=1024,
                     help="second grid side for the learned/plain comparison (0: none)")
     ap.add_argument("--config", type=int, default=1, choices=(1, 2, 3),
