@@ -788,7 +788,9 @@ int launch_mg_chunk(fs_handle* h, int it, int n) {
   return 0;
 }
 
-void drop_graphs(fs_handle* h) {
+// The handle's graphs: the multigrid-PCG ones and, with `learned`, the
+// learned-FCG iterations.
+void drop_graphs(fs_handle* h, bool learned = true) {
   auto drop = [](hipGraphExec_t& g) {
     if (g) (void)hipGraphExecDestroy(g);
     g = nullptr;
@@ -798,7 +800,7 @@ void drop_graphs(fs_handle* h) {
   drop(h->mg_pro[1]);
   for (int k = 0; k < 2; k++) {
     for (int c = 0; c < kSolveGraphs; c++) drop(h->mg_solve[k][c]);
-    drop(h->learn_graph[k]);
+    if (learned) drop(h->learn_graph[k]);
     for (int m = 0; m < kChainLog; m++) drop(h->mg_chain[k][m]);
   }
 }
@@ -915,11 +917,6 @@ void log_solve(fs_handle* h, int lerr) {
 // its kernels' launch latency).  Without a previous solve: `dflt`.
 int first_chunk(int last, int dflt) { return last > 0 ? last : dflt; }
 
-// Multigrid-PCG with graphs can be issued ahead of the host: after the
-// controller's kernel (mode 1) the assembly, the set-up and the first chunk
-// with its finishing kernels are enqueued, the assembly skipping the whole
-// solve on the device unless the attempt is accepted; after the host has read
-// the scalars, mode 2 continues that solve.  Mode 0: a whole solve.
 // Every multigrid-PCG graph of the handle (set-up alone, set-up with the first
 // chunk and the finishing kernels, iteration chains; from zero and warm),
 // captured together at the handle's first graphed multigrid solve, so that no
@@ -927,7 +924,7 @@ int first_chunk(int last, int dflt) { return last > 0 ? last : dflt; }
 // value.
 int ensure_mg_graphs(fs_handle* h) {
   if (h->mg_graphs) return 0;
-  drop_graphs(h);
+  drop_graphs(h, false);
   PcgArgs pa = pcg_args(h, 0);
   pa.part_out = h->part(2);
   pa.part_in = h->part(2);
@@ -951,6 +948,11 @@ int ensure_mg_graphs(fs_handle* h) {
   return 0;
 }
 
+// Multigrid-PCG with graphs can be issued ahead of the host: after the
+// controller's kernel (mode 1) the assembly, the set-up and the first chunk
+// with its finishing kernels are enqueued, the assembly skipping the whole
+// solve on the device unless the attempt is accepted; after the host has read
+// the scalars, mode 2 continues that solve.  Mode 0: a whole solve.
 bool can_issue_ahead(const fs_handle* h) {
   return h->p.use_graphs && h->p.lapl_solver == 1 && h->mg.size() > 1;
 }
