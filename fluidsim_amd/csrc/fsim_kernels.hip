@@ -1595,16 +1595,29 @@ __device__ __forceinline__ void gs_stage(const MgLevel& F, GsTile<H>& s, int i0,
 
 // One Gauss-Seidel half-sweep over the cells of colour c ((i + j) & 1 == c)
 // within h <= H - 1 cells of the tile at (i0, j0), entirely in LDS.  A thread
-// updates up to kGsHalfIt cells; cells of one colour do not read each other,
-// so every cell's operands are read before any new value is written, and the
-// LDS reads of all of a thread's cells are in flight together.
+// updates up to kGsHalfIt cells.  Batched (kBatch): cells of one colour do not
+// read each other, so every cell's operands are read before any new value is
+// written, and the LDS reads of all of a thread's cells are in flight
+// together; that holds their values in registers, which the ensemble's
+// kernels, whose grids fill the chip, cannot spare (three waves per SIMD need
+// at most 168 VGPRs), so they update one cell after the other.
 template <int H>
 constexpr int kGsHalfIt =
     ((kGsTY + 2 * (H - 1)) * ((kGsTX + 2 * (H - 1)) / 2) + kNT - 1) / kNT;
-template <int H>
+template <bool kBatch, int H>
 __device__ __forceinline__ void gs_half(GsTile<H>& s, int i0, int j0, int c, int h) {
   #pragma clang fp contract(fast)  // smoother arithmetic: FMAs allowed
   const int w2 = (kGsTX + 2 * h) / 2, rows = kGsTY + 2 * h, a0 = H - h;
+  if constexpr (!kBatch) {
+    for (int k = threadIdx.x; k < rows * w2; k += kNT) {
+      const int b = a0 + k / w2;
+      const int a = a0 + 2 * (k % w2) + ((c + i0 - H + a0 + j0 - H + b) & 1);
+      s.x[b][a] = (s.r[b][a] + s.cx[b][a - 1] * s.x[b][a - 1] + s.cx[b][a] * s.x[b][a + 1] +
+                   s.cy[b - 1][a] * s.x[b - 1][a] + s.cy[b][a] * s.x[b + 1][a]) *
+                  s.dinv[b][a];
+    }
+    return;
+  }
   constexpr int NI = kGsHalfIt<H>;
   double v[NI];
   int at[NI];
@@ -1806,13 +1819,18 @@ __device__ __forceinline__ void mg_down_body(const MgArgs& A, int bid, int nb) {
   const int ntx = gs_tiles_x(F);
   const TileRange tr = tile_range(gs_ntiles(F), bid, nb);
   const bool red = A.check || kCg;
-  // The first tile's loads and the partials'.  In the level-0 CG form they
-  // are issued with the solver state, before the test of `done`: every
-  // address is in bounds whatever the state (a block without tiles loads
-  // cells outside the level, which read the planes' first element), so that
-  // kernel waits for one memory round trip instead of two (measured: 0.4 us
-  // faster at 512^2; the coarser levels' kernels were measured 0.2-0.4 us
-  // slower so, and issue them after the test).  The LDS stores follow it.
+  // The first tile's loads and the partials'.  In the level-0 CG form of a
+  // single discharge they are issued with the solver state, before the test
+  // of `done`: every address is in bounds whatever the state (a block without
+  // tiles loads cells outside the level, which read the planes' first
+  // element), so that kernel waits for one memory round trip instead of two
+  // (measured: 0.4 us faster at 512^2; the coarser levels' kernels were
+  // measured 0.2-0.4 us slower so).  An ensemble's kernels issue them after
+  // the test and after the blocks without tiles have left: their launches
+  // fill the chip, where registers held across the warm-start block cost a
+  // wave per SIMD (256 members: 47.5 ms per pass against 35.5 ms), and they
+  // cover members whose solve has stopped.  The LDS stores follow the loads.
+  constexpr bool kEarly = kCg && kInstance == 0;
   CgCells cc;
   CgRegs cgr;
   GsRegs<kGsH, false> gsr;
@@ -1823,7 +1841,7 @@ __device__ __forceinline__ void mg_down_body(const MgArgs& A, int bid, int nb) {
     else gs_load<kGsH, false>(F, i0, j0, C, gsr);
     if (red) part3_load(A.part_in, A.nb_in, pl);
   };
-  if (kCg) issue_first();
+  if (kEarly) issue_first();
   CheckState cs{};
   double gam_old = 1.0, alpha_old = 1.0;  // the previous CG iteration's gamma and alpha
   double wa = 1.0, wb = 0.0, wc = 0.0;  // warm start x = wa x + wb x2 + wc x3 (initial test)
@@ -1880,7 +1898,6 @@ __device__ __forceinline__ void mg_down_body(const MgArgs& A, int bid, int nb) {
       sc->x2_ok = wn;
     }
   }
-  if (!kCg) issue_first();
   double sacc[3] = {0.0, 0.0, 0.0};  // sum r_c^2 / d_c for the learned coarse solver
   double racc[3] = {0.0, 0.0, 0.0};  // sum |r/V|^2 after the CG update
   if (tr.t0 >= tr.t1) {  // block 0 always has tiles (it writes the scalars)
@@ -1888,6 +1905,7 @@ __device__ __forceinline__ void mg_down_body(const MgArgs& A, int bid, int nb) {
     if (kCg) write_partials3(racc, A.part_out, s_red, bid);
     return;
   }
+  if (!kEarly) issue_first();
   if (kCg) cg_store(cgr, s, cc);
   else gs_store<kGsH, false>(gsr, s, 0.0);
   double pv[3];
@@ -1962,8 +1980,14 @@ __device__ __forceinline__ void mg_down_body(const MgArgs& A, int bid, int nb) {
     if (kCg) cg_update(A, s, i0, j0, cc, alpha, beta, racc[0]);
     __syncthreads();
     // first (red) half-sweep from zero is pointwise: the whole staged halo
-    // (every read of a thread's cells before its writes)
-    {
+    // (batched as the half-sweeps are: every read of a thread's cells before
+    // its writes)
+    if constexpr (kInstance != 0) {
+      for (int k = threadIdx.x; k < R * W; k += kNT) {
+        const int b = k / W, a = k % W;
+        s.x[b][a] = ((i0 + a + j0 + b) & 1) == 0 ? s.r[b][a] * s.dinv[b][a] : 0.0;
+      }
+    } else {
       constexpr int NP = (R * W + kNT - 1) / kNT;
       double v[NP];
 #pragma unroll
@@ -1981,11 +2005,11 @@ __device__ __forceinline__ void mg_down_body(const MgArgs& A, int bid, int nb) {
       }
     }
     __syncthreads();
-    gs_half(s, i0, j0, 1, 3);
+    gs_half<kInstance == 0>(s, i0, j0, 1, 3);
     __syncthreads();
-    gs_half(s, i0, j0, 0, 2);
+    gs_half<kInstance == 0>(s, i0, j0, 0, 2);
     __syncthreads();
-    gs_half(s, i0, j0, 1, 1);
+    gs_half<kInstance == 0>(s, i0, j0, 1, 1);
     __syncthreads();
     // residual of one horizontal cell pair (half an aggregate) per thread
     const int cI = threadIdx.x % (kGsTX / 2), fj = threadIdx.x / (kGsTX / 2);
@@ -2033,10 +2057,13 @@ __device__ __forceinline__ void mg_up_body(const MgArgs& A, int bid, int nb) {
   const MgLevel &F = A.F, &C = A.C;
   const int ntx = gs_tiles_x(F);
   const TileRange tr = tile_range(gs_ntiles(F), bid, nb);
-  // the first tile's loads are issued before the test of `done` (in bounds
-  // whatever the state, as in the down kernel)
+  // A single discharge issues the first tile's loads before the test of
+  // `done` (in bounds whatever the state, as in the down kernel).  An
+  // ensemble stages every tile in the loop: its launches cover members whose
+  // solve has stopped, and the first tile's registers, live across the loop,
+  // would cost it a wave per SIMD.
   GsRegs<H, true> g0;
-  gs_load<H, true>(F, (tr.t0 % ntx) * kGsTX, (tr.t0 / ntx) * kGsTY, C, g0);
+  if (kInstance == 0) gs_load<H, true>(F, (tr.t0 % ntx) * kGsTX, (tr.t0 / ntx) * kGsTY, C, g0);
   {
     const int done = A.sc->done;
     if (kInstance == 0) {
@@ -2048,20 +2075,20 @@ __device__ __forceinline__ void mg_up_body(const MgArgs& A, int bid, int nb) {
   double acc[3] = {0.0, 0.0, 0.0};
   for (int t = tr.t0; t < tr.t1; t++) {
     const int i0 = (t % ntx) * kGsTX, j0 = (t / ntx) * kGsTY;
-    if (t > tr.t0) {
-      __syncthreads();
+    if (kInstance != 0 || t > tr.t0) {
+      if (t > tr.t0) __syncthreads();
       gs_stage<H, true>(F, s, i0, j0, C, A.oc);
     } else {
       gs_store<H, true>(g0, s, A.oc);
     }
     __syncthreads();
-    gs_half(s, i0, j0, 1, H - 1);
+    gs_half<kInstance == 0>(s, i0, j0, 1, H - 1);
     __syncthreads();
-    gs_half(s, i0, j0, 0, H - 2);
+    gs_half<kInstance == 0>(s, i0, j0, 0, H - 2);
     __syncthreads();
-    gs_half(s, i0, j0, 1, H - 3);
+    gs_half<kInstance == 0>(s, i0, j0, 1, H - 3);
     __syncthreads();
-    gs_half(s, i0, j0, 0, H - 4);
+    gs_half<kInstance == 0>(s, i0, j0, 0, H - 4);
     __syncthreads();
     for (int k = threadIdx.x; k < kGsTX * kGsTY; k += kNT) {
       const int i = i0 + k % kGsTX, j = j0 + k / kGsTX;
@@ -2469,6 +2496,8 @@ __device__ __forceinline__ void mg_coarse_cycle_body(const MgCycleArgs& A) {
                  "s"(A.off[0]), "s"(A.off[1]), "s"(A.off[2]), "s"(A.off[3]), "s"(A.off[4]),
                  "s"(A.nq), "s"(A.inv), "s"(A.pk), "s"(A.oc), "s"(A.part_in), "s"(A.nb_in),
                  "s"(A.it), "s"(A.check), "s"(A.sc));
+  // an ensemble member whose solve has stopped leaves before any load
+  if (kInstance != 0 && A.sc->done) return;
   const int t = threadIdx.x;
   const int nq = A.nq;
   // The coarsest level's inverse: entry (brow, bcol) in thread brow + kB * bcol,
