@@ -1920,24 +1920,38 @@ __device__ __forceinline__ void mg_down_body(const MgArgs& A, int bid, int nb) {
       if (A.xs && wn && !(tot[0] < tot[1])) {
         zero_start = true;
         tot[0] = tot[1];
+        // the corrections before this solve did not predict it (e.g. the
+        // first transport step's, after the Poisson-only passes, is many
+        // times larger than the next ones): the next start extrapolates
+        // from this solve's correction alone, as after a solve from zero
+        if (bid == 0 && threadIdx.x == 0) {
+          sc->x2_ok = 0;
+          sc->x3_ok = 0;
+        }
       }
       if (mg_check(sc, cs, tot[0], tot[1], A.init, A.it, bid == 0 && threadIdx.x == 0)) return;
     } else {
       // gamma = r.u, delta = u.M u; p.s = delta - beta gamma / alpha_old
       // (0 < p.s <= delta in exact arithmetic for a definite operator).  Near
-      // convergence the difference can lose every digit; then the direction
-      // restarts from u (beta = 0, p.s = delta), as in the first iteration.
-      // The assembled operator is not always definite: dFdPh's two terms
-      // cancel at a near-zero field (lapl_lib.F90:625-631) and can leave a
-      // face coupling of the wrong sign.  A direction of negative curvature
-      // (p.s < 0 after the restart) is then stepped along (alpha < 0), as CG
-      // does on a symmetric indefinite system; only p.s = 0 stops the solve.
+      // convergence the difference can lose every digit; a p.s <= 0 of that
+      // size (|p.s| <= kPsCancel |delta|; rounding leaves ~1e-16 |delta|)
+      // restarts the direction from u (beta = 0, p.s = delta), as in the
+      // first iteration.  The assembled operator is not always definite:
+      // dFdPh's two terms cancel at a near-zero field (lapl_lib.F90:625-631)
+      // and can leave a face coupling of the wrong sign, and with it an
+      // eigenvalue below zero.  A larger negative p.s is then that negative
+      // curvature, stepped along (alpha < 0) as CG does on a symmetric
+      // indefinite system; restarting there instead throws the conjugacy
+      // away at every such step (measured on such a 512^2 system: the solve
+      // from zero diverged after 200+ restarts; kept, it converges in 16
+      // iterations).  Only p.s = 0 stops the solve.
+      constexpr double kPsCancel = 1e-8;
       const double gam = tot[0], del = tot[1];
       double ps = del;
       if (!A.cg.first) {
         beta = gam / gam_old;
         ps = del - beta * gam / alpha_old;
-        if (!(ps > 0.0) || !isfinite(beta)) {
+        if (!isfinite(beta) || !isfinite(ps) || (ps <= 0.0 && -ps <= kPsCancel * fabs(del))) {
           beta = 0.0;
           ps = del;
         }
