@@ -23,15 +23,32 @@ sys.path.insert(0, ROOT)
 PEAK_HBM_GBS = 8000.0
 
 
+def warm_mode(args):
+    """fs_params.lapl_warm of the run: 0 with --cold; else --warm, or by default
+    the projection onto the last corrections (2) on the 2048^2 workload, whose
+    extrapolated starts leave 18 iterations per solve (12 projected), and the
+    extrapolation (1) at 512^2, where it is already within 0.2 iterations of
+    the projection and the projection's two kernels cost more than that."""
+    if args.cold:
+        return 0
+    if args.warm:
+        return args.warm
+    return 2 if args.config == 2 else 1
+
+
+WARM_TEXT = {0: "zero", 1: "extrapolated from the last two passes' corrections",
+             2: "energy-norm projection onto the last three passes' corrections"}
+
+
 def gpu_run(args):
     from fluidsim_amd import Discharge, configs
     from fluidsim_amd.lib import FsError
 
     if args.config == 2:
-        cfg = configs.config2(args.n, rtol=args.rtol, chunk=args.chunk, warm=int(not args.cold),
+        cfg = configs.config2(args.n, rtol=args.rtol, chunk=args.chunk, warm=warm_mode(args),
                               graphs=int(not args.no_graphs))
     else:
-        cfg = configs.config1(args.n, rtol=args.rtol, chunk=args.chunk, warm=int(not args.cold),
+        cfg = configs.config1(args.n, rtol=args.rtol, chunk=args.chunk, warm=warm_mode(args),
                               graphs=int(not args.no_graphs))
     d = Discharge(cfg)
     d.init_gaussian(seed=0)
@@ -357,6 +374,9 @@ def main():
                          "(for kernel tracers that fail on large graph launches)")
     ap.add_argument("--cold", action="store_true",
                     help="start every multigrid-PCG solve of the run from zero (no warm start)")
+    ap.add_argument("--warm", type=int, choices=(1, 2), default=0,
+                    help="warm start: 1 extrapolation, 2 projection onto the last corrections "
+                         "(default: 2 for --config 2, 1 otherwise)")
     ap.add_argument("--learn-samples", type=int, default=20000)
     ap.add_argument("--learn-steps", type=int, default=6000)
     ap.add_argument("--learned-n2", type=int, default=1024,
@@ -441,7 +461,7 @@ def main():
                    "nx": args.n, "ny": args.n, "poisson_rtol": args.rtol,
                    "poisson_solver": "CG preconditioned by an aggregation multigrid V(2,2) "
                                      "cycle with red-black Gauss-Seidel smoothing",
-                   "poisson_initial_guess": "zero" if args.cold else "extrapolated from the last two passes' corrections",
+                   "poisson_initial_guess": WARM_TEXT[warm_mode(args)],
                    "hip_graphs": not args.no_graphs},
         "accepted_steps": r["accepted"],
         "ptcl_attempts": r["attempts"],

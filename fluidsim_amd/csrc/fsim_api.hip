@@ -160,6 +160,7 @@ struct fs_handle {
   double* d_dr = nullptr;     // dra, drc
   double* d_cur = nullptr;  // electrode face currents, 2*by per stage parity
   double* d_part = nullptr;
+  double* d_lspart = nullptr;  // least-squares warm start: nine sums per block
   DevScalars* d_sc = nullptr;
   DevScalars* h_sc = nullptr;  // pinned mirror
   DevScalars* h_pub = nullptr; // host-mapped copy written by k_publish (and its device address)
@@ -514,6 +515,7 @@ MgPcgArgs mgpcg_args(fs_handle* h, int it) {
   a.x2 = h->pois(PO_X2);
   a.x3 = h->pois(PO_X3);
   a.r = h->pois(PO_R0);
+  a.lspart = h->d_lspart;
   a.sc = h->d_sc;
   a.G = h->G;
   a.nbx = h->nbx;
@@ -754,6 +756,10 @@ int mg_prologue(fs_handle* h, bool warm) {
   MgPcgArgs ia = mgpcg_args(h, 1);
   ia.part_out = h->part(0);
   ia.warm = warm;
+  if (warm && h->p.lapl_warm == 2) {
+    ia.ls = 1;
+    launch_mgpcg_warm_ls(ia, h->st);
+  }
   launch_mgpcg_init(ia, h->st);
   return launch_vcycle(h, -1) || launch_vcycle(h, 0) ? FS_E_HIP : 0;
 }
@@ -1809,6 +1815,7 @@ int fs_create(const fs_grid_desc* g, const fs_params* p, fs_handle** out) {
   if (!g || !p || !out) return fail(FS_E_ARG, "null argument%s", "");
   if (g->nx < 3 || g->ny < 1) return fail(FS_E_ARG, "need nx >= 3 and ny >= 1%s", "");
   if (p->rf < 0 || p->rf > 4) return fail(FS_E_ARG, "unknown waveform rf%s", "");
+  if (p->lapl_warm < 0 || p->lapl_warm > 2) return fail(FS_E_ARG, "lapl_warm must be 0, 1 or 2%s", "");
   if (p->p_torr <= 0) return fail(FS_E_ARG, "pressure must be positive%s", "");
   if (p->rf == 4 && (p->t_rise <= 0 || p->t_fall <= 0))
     return fail(FS_E_ARG, "trapezoid needs t_rise, t_fall > 0%s", "");
@@ -1901,6 +1908,7 @@ int fs_create(const fs_grid_desc* g, const fs_params* p, fs_handle** out) {
       bad(hipMalloc(&h->d_dr, 2 * (size_t)by * sizeof(double))) ||
       bad(hipMalloc(&h->d_cur, 4 * (size_t)by * sizeof(double))) ||
       bad(hipMalloc(&h->d_part, 3 * h->npart * 8 * sizeof(double))) ||
+      bad(hipMalloc(&h->d_lspart, 9 * kMaxPartials * sizeof(double))) ||
       bad(hipMalloc(&h->d_sc, sizeof(DevScalars))) ||
       bad(hipHostMalloc(&h->h_sc, sizeof(DevScalars), hipHostMallocDefault)) ||
       bad(hipHostMalloc(&h->h_pub, sizeof(DevScalars),
@@ -2005,6 +2013,7 @@ int fs_destroy(fs_handle* h) {
   (void)hipFree(h->d_dr);
   (void)hipFree(h->d_cur);
   (void)hipFree(h->d_part);
+  (void)hipFree(h->d_lspart);
   (void)hipFree(h->d_sc);
   (void)hipHostFree(h->h_sc);
   if (h->h_pub) (void)hipHostFree(h->h_pub);
@@ -2547,6 +2556,8 @@ int fs_ensemble_create(int32_t n, const fs_grid_desc* g, const fs_params* p, fs_
   for (int i = 0; i < n; i++) {
     if (g[i].rwall) return fail(FS_E_ARG, "ensemble members have no dielectric wall%s");
     if (g[i].ny < 2) return fail(FS_E_ARG, "ensemble members are two-dimensional (ny >= 2)%s");
+    if (p[i].lapl_warm == 2)
+      return fail(FS_E_ARG, "ensemble members use lapl_warm 0 or 1%s");
   }
   fs_ensemble* E = new fs_ensemble();
   E->n = n;

@@ -254,9 +254,11 @@ struct MgPcgArgs {
   DevScalars* sc;
   Geo G;
   int32_t nbx, nby, first, flex;  // flex: Polak-Ribiere beta (flexible CG)
-  int32_t warm, pad_;             // init: start from the x in place (r = rhs - M x)
-  const double* x2;               // init: the correction before x
-  const double* x3;               // init: the one before x2
+  int32_t warm, ls;  // init: start from the x in place (r = rhs - M x); ls: with the
+                     // least-squares coefficients the warm-start kernels wrote
+  const double* x2;  // init: the correction before x
+  const double* x3;  // init: the one before x2
+  double* lspart;    // least-squares warm start: nine sums per block
 };
 
 constexpr int kTX = 64;  // tile width (one wave64 per tile row)
@@ -339,6 +341,7 @@ void launch_mg_up(const MgArgs& a, hipStream_t s);
 void launch_mg_coarse_setup(const MgCycleArgs& a, hipStream_t s);
 void launch_mg_coarse_cycle(const MgCycleArgs& a, hipStream_t s);
 void launch_mgpcg_init(const MgPcgArgs& a, hipStream_t s);
+void launch_mgpcg_warm_ls(const MgPcgArgs& a, hipStream_t s);
 void launch_mgpcg_pq(const MgPcgArgs& a, hipStream_t s);
 void launch_mgpcg_xr(const MgPcgArgs& a, hipStream_t s);
 void launch_fill(double* p, double v, size_t n, hipStream_t s);

@@ -2643,7 +2643,11 @@ __device__ __forceinline__ void mgpcg_init_body(const MgPcgArgs& A, int bid, int
   // floor and the quadratic weights amplify that noise more.
   constexpr double kQuadChange = 2e-3;
   double a = 1.0, b = 0.0, cq = 0.0;
-  if (A.warm) {
+  if (A.warm && A.ls) {  // the least-squares coefficients (k_mgpcg_warm_solve)
+    a = A.sc->warm_a;
+    b = A.sc->warm_b;
+    cq = A.sc->warm_c;
+  } else if (A.warm) {
     const DevScalars* sc = A.sc;
     const double dt = sc->dt_stage, dt1 = sc->dt_warm, dt2 = sc->dt_warm2;
     const double a1 = dt / dt1;
@@ -2667,9 +2671,11 @@ __device__ __forceinline__ void mgpcg_init_body(const MgPcgArgs& A, int bid, int
     }
   }
   if (bid == 0 && threadIdx.x == 0) {
-    A.sc->warm_a = a;
-    A.sc->warm_b = b;
-    A.sc->warm_c = cq;
+    if (!A.ls) {
+      A.sc->warm_a = a;
+      A.sc->warm_b = b;
+      A.sc->warm_c = cq;
+    }
     A.sc->warm_now = A.warm;
     A.sc->itmax = A.warm ? min(A.sc->maxit, kWarmIters) : A.sc->maxit;
   }
@@ -2711,6 +2717,113 @@ __device__ __forceinline__ void mgpcg_init_body(const MgPcgArgs& A, int bid, int
 }
 
 __global__ void __launch_bounds__(kNT) k_mgpcg_init(MgPcgArgs A) { mgpcg_init_body<0>(A, blockIdx.x, gridDim.x); }
+
+// Projected warm start (fs_params.lapl_warm = 2): the start x0 = a x + b x2 +
+// c x3 closest to the solution in the operator's energy norm |x0 - M^-1 rhs|_M
+// over the span of the last converged corrections (x; x2 when x2_ok; x3 when
+// x3_ok), a span that holds both the extrapolation and zero: the Galerkin
+// projection of the new system onto the past corrections, the error norm CG
+// itself minimises (a start that only minimises the residual norm leaves
+// smooth error that costs the multigrid-PCG more iterations, measured on the
+// 512^2 pulse).  Successive corrections are parallel in that norm to ~1e-11
+// (on the 512^2 pulse), so the span is taken in the difference basis u0 = x,
+// u1 = x - x2, u2 = x - 2 x2 + x3, each formed cell by cell before M is
+// applied: its Gram matrix G_kl = u_k . M u_l is well conditioned where that
+// of x, x2, x3 would hold the new directions below its rounding.  With
+// g_k = u_k . rhs the coefficients solve G c = g; x0 = (c0 + c1 + c2) x -
+// (c1 + 2 c2) x2 + c2 x3.  k_mgpcg_warm_ls writes each block's nine sums
+// (every product is formed, the invalid ones are not used);
+// k_mgpcg_warm_solve reduces them in a fixed order and solves by Cholesky,
+// keeping the leading basis vectors whose pivots stay above kLsPivot of their
+// diagonal (a direction already in the span of the earlier ones is dropped
+// with the later ones; on an indefinite operator a non-positive pivot drops
+// it too).  The init kernel then forms r = rhs - M x0 with these
+// coefficients, and the initial test discards a start no better than zero as
+// for the extrapolation.
+__global__ void __launch_bounds__(kNT) k_mgpcg_warm_ls(MgPcgArgs A) {
+  __shared__ double s_red[9 * kNW];
+  const Geo& G = A.G;
+  const int P = G.P;
+  const TileRange tr = tile_range(A.nbx * A.nby, blockIdx.x, gridDim.x);
+  // the basis vectors at padded index q, and M applied to one of them at m
+  auto u0 = [&](size_t q) { return A.x[q]; };
+  auto u1 = [&](size_t q) { return A.x[q] - A.x2[q]; };
+  auto u2 = [&](size_t q) { return (A.x[q] - A.x2[q]) - (A.x2[q] - A.x3[q]); };
+  auto mu = [&](auto u, size_t m) {
+    return A.dg[m] * u(m) - A.cx[m - 1] * u(m - 1) - A.cx[m] * u(m + 1) - A.cy[m - P] * u(m - P) -
+           A.cy[m] * u(m + P);
+  };
+  double acc[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int t = tr.t0; t < tr.t1; t++) {
+    for (int k = 0; k < kPcgRows; k++) {
+      const int ci = (t % A.nbx) * kTX + (int)(threadIdx.x % kTX);
+      const int cj = (t / A.nbx) * kPcgTY + (int)(threadIdx.x / kTX) + k * (kPcgTY / kPcgRows);
+      if (ci < G.bx && cj < G.by) {
+        const size_t m = (size_t)(cj + 1) * P + (ci + 1);
+        const double f = A.r[m], x0 = u0(m), x1 = u1(m), x2 = u2(m);
+        const double y0 = mu(u0, m), y1 = mu(u1, m), y2 = mu(u2, m);
+        acc[0] += x0 * y0;
+        acc[1] += 0.5 * (x0 * y1 + x1 * y0);
+        acc[2] += 0.5 * (x0 * y2 + x2 * y0);
+        acc[3] += x1 * y1;
+        acc[4] += 0.5 * (x1 * y2 + x2 * y1);
+        acc[5] += x2 * y2;
+        acc[6] += x0 * f;
+        acc[7] += x1 * f;
+        acc[8] += x2 * f;
+      }
+    }
+  }
+  block_sum<9>(acc, s_red);
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 9; k++) A.lspart[9 * (size_t)blockIdx.x + k] = acc[k];
+}
+
+__global__ void __launch_bounds__(kNT) k_mgpcg_warm_solve(MgPcgArgs A, int nb) {
+  __shared__ double s_red[9 * kNW];
+  constexpr double kLsPivot = 1e-10;
+  double acc[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int b = threadIdx.x; b < nb; b += kNT)
+    for (int k = 0; k < 9; k++) acc[k] += A.lspart[9 * (size_t)b + k];
+  block_sum<9>(acc, s_red);
+  if (threadIdx.x != 0) return;
+  DevScalars* sc = A.sc;
+  const double Gm[3][3] = {{acc[0], acc[1], acc[2]}, {acc[1], acc[3], acc[4]}, {acc[2], acc[4], acc[5]}};
+  const double g[3] = {acc[6], acc[7], acc[8]};
+  const int nv = sc->x2_ok ? (sc->x3_ok ? 3 : 2) : 1;
+  double c[3] = {0.0, 0.0, 0.0};
+  for (int n = nv; n >= 1; n--) {
+    double L[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
+    bool ok = true;
+    for (int i = 0; i < n && ok; i++)
+      for (int j = 0; j <= i && ok; j++) {
+        double v = Gm[i][j];
+        for (int k = 0; k < j; k++) v -= L[i][k] * L[j][k];
+        if (i == j) {
+          ok = v > kLsPivot * Gm[i][i] && isfinite(v);  // (false for NaN)
+          if (ok) L[i][i] = sqrt(v);
+        } else {
+          L[i][j] = v / L[j][j];
+        }
+      }
+    if (!ok) continue;
+    double z[3];
+    for (int i = 0; i < n; i++) {
+      double v = g[i];
+      for (int k = 0; k < i; k++) v -= L[i][k] * z[k];
+      z[i] = v / L[i][i];
+    }
+    for (int i = n - 1; i >= 0; i--) {
+      double v = z[i];
+      for (int k = i + 1; k < n; k++) v -= L[k][i] * c[k];
+      c[i] = v / L[i][i];
+    }
+    break;
+  }
+  sc->warm_a = c[0] + c[1] + c[2];
+  sc->warm_b = -c[1] - 2.0 * c[2];
+  sc->warm_c = c[2];
+}
 
 // Cells of one 64x8 PCG tile held by a thread (kPcgRows of them).
 struct PqCells {
@@ -3044,6 +3157,11 @@ void launch_mg_coarse_setup(const MgCycleArgs& a, hipStream_t s) {
 }
 void launch_mg_coarse_cycle(const MgCycleArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(k_mg_coarse_cycle, dim3(1), dim3(1024), 0, s, a);
+}
+void launch_mgpcg_warm_ls(const MgPcgArgs& a, hipStream_t s) {
+  const int nb = capped_grid(a.nbx * a.nby);
+  hipLaunchKernelGGL(k_mgpcg_warm_ls, dim3(nb), dim3(kNT), 0, s, a);
+  hipLaunchKernelGGL(k_mgpcg_warm_solve, dim3(1), dim3(kNT), 0, s, a, nb);
 }
 void launch_mgpcg_init(const MgPcgArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(k_mgpcg_init, dim3(capped_grid(a.nbx * a.nby)), dim3(kNT), 0, s, a);
