@@ -160,7 +160,7 @@ struct fs_handle {
   double* d_dr = nullptr;     // dra, drc
   double* d_cur = nullptr;  // electrode face currents, 2*by per stage parity
   double* d_part = nullptr;
-  double* d_lspart = nullptr;  // least-squares warm start: nine sums per block
+  double* d_lspart = nullptr;  // projected warm start: nine sums per block
   DevScalars* d_sc = nullptr;
   DevScalars* h_sc = nullptr;  // pinned mirror
   DevScalars* h_pub = nullptr; // host-mapped copy written by k_publish (and its device address)

@@ -255,10 +255,10 @@ struct MgPcgArgs {
   Geo G;
   int32_t nbx, nby, first, flex;  // flex: Polak-Ribiere beta (flexible CG)
   int32_t warm, ls;  // init: start from the x in place (r = rhs - M x); ls: with the
-                     // least-squares coefficients the warm-start kernels wrote
+                     // projection's coefficients the warm-start kernels wrote
   const double* x2;  // init: the correction before x
   const double* x3;  // init: the one before x2
-  double* lspart;    // least-squares warm start: nine sums per block
+  double* lspart;    // projected warm start: nine sums per block
 };
 
 constexpr int kTX = 64;  // tile width (one wave64 per tile row)

@@ -2643,7 +2643,7 @@ __device__ __forceinline__ void mgpcg_init_body(const MgPcgArgs& A, int bid, int
   // floor and the quadratic weights amplify that noise more.
   constexpr double kQuadChange = 2e-3;
   double a = 1.0, b = 0.0, cq = 0.0;
-  if (A.warm && A.ls) {  // the least-squares coefficients (k_mgpcg_warm_solve)
+  if (A.warm && A.ls) {  // the projection's coefficients (k_mgpcg_warm_solve)
     a = A.sc->warm_a;
     b = A.sc->warm_b;
     cq = A.sc->warm_c;
