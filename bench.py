@@ -25,15 +25,16 @@ PEAK_HBM_GBS = 8000.0
 
 def warm_mode(args):
     """fs_params.lapl_warm of the run: 0 with --cold; else --warm, or by default
-    the projection onto the last corrections (2) on the 2048^2 workload, whose
+    the projection onto the last corrections (2) for the 2048^2 workload, whose
     extrapolated starts leave 18 iterations per solve (12 projected), and the
+    256-member ensemble (9577 against 7379 aggregate steps/s), and the
     extrapolation (1) at 512^2, where it is already within 0.2 iterations of
     the projection and the projection's two kernels cost more than that."""
     if args.cold:
         return 0
     if args.warm:
         return args.warm
-    return 2 if args.config == 2 else 1
+    return 2 if args.config in (2, 3) else 1
 
 
 WARM_TEXT = {0: "zero", 1: "extrapolated from the last two passes' corrections",
@@ -228,7 +229,7 @@ def ensemble_run(args):
 
     cfgs = configs.config3(args.members)
     for c in cfgs:
-        c["lapl_warm"] = int(not args.cold)
+        c["lapl_warm"] = warm_mode(args)
         c["use_graphs"] = int(not args.no_graphs)
     t0 = time.perf_counter()
     E = Ensemble(cfgs)
@@ -376,7 +377,7 @@ def main():
                     help="start every multigrid-PCG solve of the run from zero (no warm start)")
     ap.add_argument("--warm", type=int, choices=(1, 2), default=0,
                     help="warm start: 1 extrapolation, 2 projection onto the last corrections "
-                         "(default: 2 for --config 2, 1 otherwise)")
+                         "(default: 2 for --config 2 and 3, 1 otherwise)")
     ap.add_argument("--learn-samples", type=int, default=20000)
     ap.add_argument("--learn-steps", type=int, default=6000)
     ap.add_argument("--learned-n2", type=int, default=1024,
@@ -410,7 +411,7 @@ def main():
             "config": {"workload": f"{args.members} argon discharges, sides U{{64..256}}, "
                                    "150-400 V DC, 0.5-10 Torr log-uniform, gamma 0.05-0.15",
                        "members": args.members, "cells_total": e["cells"],
-                       "poisson_initial_guess": "zero" if args.cold else "extrapolated from the last two passes' corrections",
+                       "poisson_initial_guess": WARM_TEXT[warm_mode(args)],
                    "hip_graphs": not args.no_graphs},
             "accepted_steps": e["accepted"], "ptcl_attempts": e["attempts"],
             "member_passes_per_s": args.members * args.steps / e["elapsed"],

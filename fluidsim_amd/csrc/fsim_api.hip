@@ -1328,6 +1328,7 @@ struct fs_ensemble {
   DevScalars* h_sc = nullptr;
   hipStream_t st = nullptr;
   int n = 0, maxL = 0, maxF = 0;
+  bool projected = false;  // some member takes the projected warm start (lapl_warm = 2)
   // argument tables
   FsPhys* t_phys = nullptr;
   StageArgs* t_stage = nullptr;   // [(i*6 + rot*2 + alias)*5 + stage-1]
@@ -1437,6 +1438,7 @@ int ens_tables(fs_ensemble* E) {
       in[i * 2 + w] = mgpcg_args(h, 1);
       in[i * 2 + w].part_out = h->part(0);
       in[i * 2 + w].warm = w;
+      in[i * 2 + w].ls = w && h->p.lapl_warm == 2;
     }
     static const int kVariantIt[4] = {-1, 1, 2, 0};
     for (int v = 0; v < 4; v++) {
@@ -1697,6 +1699,9 @@ int ens_round(fs_ensemble* E, std::vector<int>& left) {
                                   EnsSel{nullptr, kMgMaxLevels, l}, E->st);
           launch_ens_mg_coarse_setup(E->t_cc, with_mask(E->m_one, r.pois), EnsSel{nullptr, 4, 0},
                                      E->st);
+          if (E->projected)
+            launch_ens_mgpcg_warm_ls(E->t_init, with_mask(E->m_pcg, r.pois),
+                                     with_mask(E->m_one, r.pois), EnsSel{r.sel2, 1, 0}, E->st);
           launch_ens_mgpcg_init(E->t_init, with_mask(E->m_pcg, r.pois), EnsSel{r.sel2, 1, 0},
                                 E->st);
           ens_vcycle(E, 0, r.pois);
@@ -2556,11 +2561,11 @@ int fs_ensemble_create(int32_t n, const fs_grid_desc* g, const fs_params* p, fs_
   for (int i = 0; i < n; i++) {
     if (g[i].rwall) return fail(FS_E_ARG, "ensemble members have no dielectric wall%s");
     if (g[i].ny < 2) return fail(FS_E_ARG, "ensemble members are two-dimensional (ny >= 2)%s");
-    if (p[i].lapl_warm == 2)
-      return fail(FS_E_ARG, "ensemble members use lapl_warm 0 or 1%s");
+
   }
   fs_ensemble* E = new fs_ensemble();
   E->n = n;
+  for (int i = 0; i < n; i++) E->projected = E->projected || p[i].lapl_warm == 2;
   for (int i = 0; i < n; i++) {
     if (p[i].lapl_solver != 1) {
       ens_release(E);

@@ -313,6 +313,8 @@ void launch_ens_mg_up(const MgArgs* t, const EnsMap& m, const EnsSel& s, bool cg
 void launch_ens_mg_coarse_cycle(const MgCycleArgs* t, const EnsMap& m, const EnsSel& s,
                                 hipStream_t st);
 void launch_ens_mgpcg_init(const MgPcgArgs* t, const EnsMap& m, const EnsSel& s, hipStream_t st);
+void launch_ens_mgpcg_warm_ls(const MgPcgArgs* t, const EnsMap& m, const EnsMap& m_one,
+                              const EnsSel& s, hipStream_t st);
 // finish over map m (the members' capped PCG grids), then the true residual
 // with one block per member (map one, partial counts m.nb)
 void launch_ens_pcg_finish(const PcgArgs* t, const EnsMap& m, const EnsMap& one, const EnsSel& s,

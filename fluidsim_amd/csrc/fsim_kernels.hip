@@ -2740,11 +2740,11 @@ __global__ void __launch_bounds__(kNT) k_mgpcg_init(MgPcgArgs A) { mgpcg_init_bo
 // it too).  The init kernel then forms r = rhs - M x0 with these
 // coefficients, and the initial test discards a start no better than zero as
 // for the extrapolation.
-__global__ void __launch_bounds__(kNT) k_mgpcg_warm_ls(MgPcgArgs A) {
+__device__ __forceinline__ void mgpcg_warm_ls_body(const MgPcgArgs& A, int bid, int nb) {
   __shared__ double s_red[9 * kNW];
   const Geo& G = A.G;
   const int P = G.P;
-  const TileRange tr = tile_range(A.nbx * A.nby, blockIdx.x, gridDim.x);
+  const TileRange tr = tile_range(A.nbx * A.nby, bid, nb);
   // the basis vectors at padded index q, and M applied to one of them at m
   auto u0 = [&](size_t q) { return A.x[q]; };
   auto u1 = [&](size_t q) { return A.x[q] - A.x2[q]; };
@@ -2776,12 +2776,18 @@ __global__ void __launch_bounds__(kNT) k_mgpcg_warm_ls(MgPcgArgs A) {
   }
   block_sum<9>(acc, s_red);
   if (threadIdx.x == 0)
-    for (int k = 0; k < 9; k++) A.lspart[9 * (size_t)blockIdx.x + k] = acc[k];
+    for (int k = 0; k < 9; k++) A.lspart[9 * (size_t)bid + k] = acc[k];
+}
+__global__ void __launch_bounds__(kNT) k_mgpcg_warm_ls(MgPcgArgs A) {
+  mgpcg_warm_ls_body(A, blockIdx.x, gridDim.x);
 }
 
-__global__ void __launch_bounds__(kNT) k_mgpcg_warm_solve(MgPcgArgs A, int nb) {
+// (one workgroup of kNT threads; nb = the blocks of the sums' launch, that of
+// the start kernel)
+__device__ __forceinline__ void mgpcg_warm_solve_body(const MgPcgArgs& A) {
   __shared__ double s_red[9 * kNW];
   constexpr double kLsPivot = 1e-10;
+  const int nb = min(A.nbx * A.nby, kMaxPartials);
   double acc[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   for (int b = threadIdx.x; b < nb; b += kNT)
     for (int k = 0; k < 9; k++) acc[k] += A.lspart[9 * (size_t)b + k];
@@ -2824,6 +2830,7 @@ __global__ void __launch_bounds__(kNT) k_mgpcg_warm_solve(MgPcgArgs A, int nb) {
   sc->warm_b = -c[1] - 2.0 * c[2];
   sc->warm_c = c[2];
 }
+__global__ void __launch_bounds__(kNT) k_mgpcg_warm_solve(MgPcgArgs A) { mgpcg_warm_solve_body(A); }
 
 // Cells of one 64x8 PCG tile held by a thread (kPcgRows of them).
 struct PqCells {
@@ -3077,6 +3084,20 @@ __global__ void __launch_bounds__(kNT) kb_mgpcg_init(const MgPcgArgs* t, EnsMap 
   int i, bid, nb;
   if (ens_slot(m, i, bid, nb)) mgpcg_init_body<1>(ens_arg(t, s, i), bid, nb);
 }
+// the projected warm start of the members whose start takes it (the others
+// return at once): the start kernel's map, then one block per member
+__global__ void __launch_bounds__(kNT) kb_mgpcg_warm_ls(const MgPcgArgs* t, EnsMap m, EnsSel s) {
+  int i, bid, nb;
+  if (!ens_slot(m, i, bid, nb)) return;
+  const MgPcgArgs& A = ens_arg(t, s, i);
+  if (A.warm && A.ls) mgpcg_warm_ls_body(A, bid, nb);
+}
+__global__ void __launch_bounds__(kNT) kb_mgpcg_warm_solve(const MgPcgArgs* t, EnsMap m, EnsSel s) {
+  int i, bid, nb;
+  if (!ens_slot(m, i, bid, nb)) return;
+  const MgPcgArgs& A = ens_arg(t, s, i);
+  if (A.warm && A.ls) mgpcg_warm_solve_body(A);
+}
 __global__ void __launch_bounds__(kNT) kb_pcg_finish(const PcgArgs* t, EnsMap m, EnsSel s) {
   int i, bid, nb;
   if (ens_slot(m, i, bid, nb)) pcg_finish_body<1>(ens_arg(t, s, i), bid, nb);
@@ -3159,9 +3180,8 @@ void launch_mg_coarse_cycle(const MgCycleArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(k_mg_coarse_cycle, dim3(1), dim3(1024), 0, s, a);
 }
 void launch_mgpcg_warm_ls(const MgPcgArgs& a, hipStream_t s) {
-  const int nb = capped_grid(a.nbx * a.nby);
-  hipLaunchKernelGGL(k_mgpcg_warm_ls, dim3(nb), dim3(kNT), 0, s, a);
-  hipLaunchKernelGGL(k_mgpcg_warm_solve, dim3(1), dim3(kNT), 0, s, a, nb);
+  hipLaunchKernelGGL(k_mgpcg_warm_ls, dim3(capped_grid(a.nbx * a.nby)), dim3(kNT), 0, s, a);
+  hipLaunchKernelGGL(k_mgpcg_warm_solve, dim3(1), dim3(kNT), 0, s, a);
 }
 void launch_mgpcg_init(const MgPcgArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(k_mgpcg_init, dim3(capped_grid(a.nbx * a.nby)), dim3(kNT), 0, s, a);
@@ -3232,6 +3252,12 @@ void launch_ens_mg_coarse_cycle(const MgCycleArgs* t, const EnsMap& m, const Ens
                                 hipStream_t st) {
   if (m.nblocks)
     hipLaunchKernelGGL(kb_mg_coarse_cycle, dim3(m.nblocks), dim3(1024), 0, st, t, m, s);
+}
+void launch_ens_mgpcg_warm_ls(const MgPcgArgs* t, const EnsMap& m, const EnsMap& m_one,
+                              const EnsSel& s, hipStream_t st) {
+  if (m.nblocks) hipLaunchKernelGGL(kb_mgpcg_warm_ls, dim3(m.nblocks), dim3(kNT), 0, st, t, m, s);
+  if (m_one.nblocks)
+    hipLaunchKernelGGL(kb_mgpcg_warm_solve, dim3(m_one.nblocks), dim3(kNT), 0, st, t, m_one, s);
 }
 void launch_ens_mgpcg_init(const MgPcgArgs* t, const EnsMap& m, const EnsSel& s, hipStream_t st) {
   if (m.nblocks) hipLaunchKernelGGL(kb_mgpcg_init, dim3(m.nblocks), dim3(kNT), 0, st, t, m, s);

@@ -18,7 +18,8 @@ def _solo(c, passes):
 
 
 @pytest.mark.parametrize("passes,warm,n,lo,hi", [(1, 0, 5, 64, 96), (14, 0, 5, 64, 96),
-                                                 (14, 1, 5, 64, 96), (6, 1, 2, 256, 300)])
+                                                 (14, 1, 5, 64, 96), (6, 1, 2, 256, 300),
+                                                 (14, 2, 5, 64, 96), (6, 2, 2, 256, 300)])
 def test_members_equal_their_solo_runs_bitwise(passes, warm, n, lo, hi):
     """(256..300 cells a side: the assembly coarsens three multigrid levels
     itself in a solo run, while the ensemble's coarsening launches rewrite

@@ -106,9 +106,8 @@ def test_projected_warm_start_follows_the_cold_trajectory(cfg_name):
         assert sum(i2[5:]) <= sum(i1[5:]), (i1, i2)
 
 
-def test_projected_warm_start_argument_checks():
-    """lapl_warm takes 0, 1 or 2; ensembles take 0 or 1 (their members must
-    equal their solo runs, and the projection has no batched kernels)."""
+def test_warm_start_argument_checks():
+    """lapl_warm takes 0, 1 or 2, for single handles and ensemble members."""
     from fluidsim_amd import Discharge, configs
     from fluidsim_amd.ensemble import Ensemble
     from fluidsim_amd.lib import FsError
@@ -117,6 +116,6 @@ def test_projected_warm_start_argument_checks():
         Discharge(configs.config0(warm=3))
     cfgs = configs.config3(2, base_seed=1, lo=64, hi=72)
     for c in cfgs:
-        c["lapl_warm"] = 2
+        c["lapl_warm"] = 3
     with pytest.raises(FsError):
         Ensemble(cfgs)
