@@ -86,8 +86,7 @@ typedef struct fs_params {
                              corrections extrapolated to the new step, instead of zero
                              (the reference's PETSc KSP starts from zero); 2: from the
                              combination of the last three closest to the solution in
-                             the operator's energy norm (Galerkin projection; single
-                             handles, not ensembles);
+                             the operator's energy norm (Galerkin projection);
                              fs_set_field and fs_set_solver reset it */
   int32_t stage_tile;     /* tile height of the stage kernel (tiles of 64 x stage_tile
                              cells): 4, 8 or 16, or 0 to take the fastest of the three,
